@@ -1,0 +1,189 @@
+/*
+ * xbach.h — C ABI of the MI355X-native X-BACH PSF-matching + subtraction path.
+ *
+ * The reference (X-BACH, C++20 + OpenCL) reaches this path through its C++ stage
+ * API `init/sss/cmv/cd/ksc/conv/sub` (include/bach.h:66-87) over the shared
+ * `ClData` state (include/bach.h:7-64) and through the OpenCL plumbing in
+ * include/clUtil.h:9-41 (platform/device choice, program build).  This header is
+ * the drop-in replacement for that plumbing: plain C, opaque handles, pointers and
+ * sizes, integer status codes, no exceptions or exits across the boundary.
+ *
+ * Images cross the boundary as FP32 row-major (x fastest) arrays of w*h pixels,
+ * exactly what the reference reads from BITPIX -32 FITS files
+ * (src/fitsUtil.cpp:8-38).  Outputs are FP32 as the reference writes FLOAT_IMG
+ * (src/fitsUtil.cpp:40-64).
+ */
+#ifndef XBACH_H
+#define XBACH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference calls std::exit(1) or throws; see
+ *      src/bach.cpp:20-24,166-169, src/bachUtil.cpp:149-155) ---------------- */
+enum {
+  XB_OK = 0,
+  XB_ERR_ARG = 1,          /* bad argument / unsupported parameter            */
+  XB_ERR_SIZE_MISMATCH = 2,/* template and science sizes differ (bach.cpp:20)  */
+  XB_ERR_NO_SUBSTAMPS = 3, /* "No substamps found" (bach.cpp:166-169)          */
+  XB_ERR_STAMP_PIXELS = 4, /* "Not enough pixels in a stamp" (bachUtil.cpp:151)*/
+  XB_ERR_HIP = 5,          /* HIP runtime error                                */
+  XB_ERR_NO_DEVICE = 6,    /* no GPU / bad device index                        */
+  XB_ERR_NOT_BUILT = 7,    /* kernels for gfx950 missing                       */
+  XB_ERR_CAPACITY = 8      /* a size exceeds a documented maximum              */
+};
+
+/* ---- 16-bit pixel mask bits (include/datatypeUtil.h:172-188) ------------- */
+enum {
+  XB_MASK_BAD_PIX_VAL = 1 << 0,
+  XB_MASK_SAT_PIXEL = 1 << 1,
+  XB_MASK_LOW_PIXEL = 1 << 2,
+  XB_MASK_NAN_PIXEL = 1 << 3,
+  XB_MASK_BAD_CONV = 1 << 4,
+  XB_MASK_INPUT_MASK = 1 << 5,
+  XB_MASK_OK_CONV = 1 << 6,
+  XB_MASK_BAD_INPUT = 1 << 7,
+  XB_MASK_BAD_PIXEL_T = 1 << 8,
+  XB_MASK_SKIP_T = 1 << 9,
+  XB_MASK_BAD_PIXEL_S = 1 << 10,
+  XB_MASK_SKIP_S = 1 << 11,
+  XB_MASK_BAD_OUTPUT = 1 << 12,
+  XB_MASK_ALL = (1 << 13) - 1
+};
+
+/* ---- method parameters: the reference's `Arguments` (include/argsUtil.h:7-57)
+ *      minus the I/O names.  xb_params_default() fills the reference defaults. */
+typedef struct xb_params {
+  int32_t stampsx, stampsy;        /* stamp grid (-sx / -sy)                     */
+  double threshLow, threshHigh;    /* 0, 25000                                   */
+  double threshKernFit;            /* 20                                         */
+  double sigKernFit;               /* 2                                          */
+  double sigClipAlpha;             /* 3                                          */
+  double iqRange;                  /* 1.35                                       */
+  int32_t maxKSStamps;             /* 3 (6 candidates kept per stamp)            */
+  int32_t hSStampWidth, fSStampWidth; /* 15, 31                                  */
+  int32_t hKernelWidth, fKernelWidth; /* 10, 21                                  */
+  double inSpreadMaskFactor;       /* 1.0                                        */
+  int32_t normalizeTemplate;       /* 1                                          */
+  int32_t backgroundOrder;         /* 1                                          */
+  int32_t kernelOrder;             /* 2                                          */
+  int32_t ngauss;                  /* 3                                          */
+  int32_t dg[3];                   /* 6, 4, 2                                    */
+  float bg[3];                     /* 1/(2 sigma^2), sigma = 0.7, 1.5, 3.0 (float) */
+  int32_t verbose;
+} xb_params;
+
+/* ---- capacities of the trace record ------------------------------------- */
+#define XB_MAX_STAMPS 1024
+#define XB_MAX_SS 16
+#define XB_MAX_SOL 512
+
+/* Everything the path decides, recorded so two implementations can be compared
+ * field by field.  Image index 0 = template, 1 = science (before the cd swap). */
+typedef struct xb_trace {
+  int32_t w, h;
+  int32_t stampsx, stampsy, fStampWidth, nGrid;
+  int32_t retried;                              /* the <10 % refill retry ran   */
+  /* sss result (final pass), grid order */
+  double sky[2][XB_MAX_STAMPS];
+  double fwhm[2][XB_MAX_STAMPS];
+  int32_t ssCount[2][XB_MAX_STAMPS];            /* min(found, maxKSStamps)       */
+  int32_t ssX[2][XB_MAX_STAMPS * XB_MAX_SS];    /* 2*maxKSStamps slots / stamp   */
+  int32_t ssY[2][XB_MAX_STAMPS * XB_MAX_SS];    /* (INT_MAX padding)            */
+  double ssVal[2][XB_MAX_STAMPS * XB_MAX_SS];   /* (-INF padding)               */
+  int32_t stampCount[2];                        /* non-empty stamps             */
+  int32_t stampIndex[2][XB_MAX_STAMPS];         /* grid index of kept stamps    */
+  /* cd */
+  int32_t nTest[2];
+  double kernelSumCd[2];
+  double merit[2];
+  double cdSolution[2][XB_MAX_SOL];
+  int32_t convTemplate;
+  /* ksc */
+  int32_t kscIterations;
+  int32_t nSol;
+  double solution[XB_MAX_SOL];
+  int32_t finalCurrent[XB_MAX_STAMPS];         /* chosen direction, kept order */
+  /* conv */
+  double kernSum;
+} xb_trace;
+
+void xb_params_default(xb_params* p);
+
+/* ---- context: one per host thread, one primary HIP stream ---------------- */
+typedef struct xb_ctx xb_ctx;
+
+/* Replaces getDefaultPlatform/getDefaultDevice/loadBuildPrograms
+ * (include/clUtil.h:9-41, src/main.cpp:42-48).  `device` is honoured (the
+ * reference ignores -d, src/clUtil.cpp:45).  flags: reserved, pass 0. */
+int xb_ctx_create(int device, int flags, xb_ctx** out);
+int xb_ctx_destroy(xb_ctx* ctx);
+const char* xb_last_error(xb_ctx* ctx);
+int xb_device_name(xb_ctx* ctx, char* buf, size_t len);
+
+/* ---- stage API: the reference's stages over the context's resident state.
+ * Each mirrors one function of include/bach.h:66-87 and keeps its meaning. */
+
+/* init (src/bach.cpp:14-42): upload the pair, size check, maskInput+spreadMask. */
+int xb_init(xb_ctx* ctx, const xb_params* p, int w, int h, const float* tmpl,
+            const float* sci);
+/* Same as xb_init with device-resident FP32 inputs (no host copy). */
+int xb_init_device(xb_ctx* ctx, const xb_params* p, int w, int h,
+                   const void* d_tmpl, const void* d_sci);
+/* sss (src/bach.cpp:44-170): stamp stats + substamp selection, both images. */
+int xb_sss(xb_ctx* ctx);
+/* cmv (src/bach.cpp:172-268): basis + per-stamp W, Q, B, both directions. */
+int xb_cmv(xb_ctx* ctx);
+/* cd (src/bach.cpp:270-313): test fits, merits, direction.  *convTemplate out. */
+int xb_cd(xb_ctx* ctx, int* convTemplate);
+/* ksc (src/bach.cpp:315-323): global fit with the chi2 clip-refit loop. */
+int xb_ksc(xb_ctx* ctx);
+/* conv (src/bach.cpp:325-413): spatially varying convolution; *kernSum out. */
+int xb_conv(xb_ctx* ctx, double* kernSum);
+/* sub (src/bach.cpp:415-440): difference image. */
+int xb_sub(xb_ctx* ctx);
+
+/* Results of the last pipeline (host copies). */
+int xb_get_conv(xb_ctx* ctx, float* out);
+int xb_get_diff(xb_ctx* ctx, float* out);
+int xb_get_mask(xb_ctx* ctx, uint16_t* out);
+int xb_get_trace(xb_ctx* ctx, xb_trace* out);
+/* Device pointers of the results (valid until the next xb_init). */
+int xb_device_outputs(xb_ctx* ctx, const void** d_conv, const void** d_diff);
+
+/* Whole path init->sub in one call (host buffers in and out).  Any output
+ * pointer may be NULL.  stage_ms (9 doubles, may be NULL) receives the -vt
+ * breakdown Ini, SSS, CMV, CD, KSC, Conv, Sub, Fin(0), total. */
+int xb_pipeline_run(xb_ctx* ctx, const xb_params* p, int w, int h,
+                    const float* tmpl, const float* sci, float* conv, float* diff,
+                    double* stage_ms);
+/* Whole path on device-resident inputs; outputs stay resident
+ * (xb_device_outputs).  This is the form the throughput benchmark times. */
+int xb_pipeline_run_device(xb_ctx* ctx, const xb_params* p, int w, int h,
+                           const void* d_tmpl, const void* d_sci);
+
+/* ---- host-side helpers of the library (no GPU needed) -------------------- */
+/* Seeded synthetic pair: Gaussian stars (pixel-integrated), power-law fluxes,
+ * sky + linear gradient, Poisson noise.  FWHM of the science PSF ramps linearly
+ * from fwhmS0 at (0,0) by dfwhmX across x and dfwhmY across y.  xoshiro256**
+ * seeded by SplitMix64; Poisson by Knuth below lambda 30, PTRS above. */
+typedef struct xb_synth_cfg {
+  int32_t w, h;
+  int32_t nstars;
+  double fluxMin, fluxMax, fluxSlope;  /* dN/dF ~ F^fluxSlope on [min,max]  */
+  double fwhmT;                        /* template FWHM (px)                */
+  double fwhmS0, dfwhmX, dfwhmY;       /* science FWHM ramp                 */
+  double sky, skyGx, skyGy;            /* sky + gx*x + gy*y (both images)   */
+  double sciFluxScale;                 /* science/template flux ratio       */
+  uint64_t seed;
+} xb_synth_cfg;
+int xb_synth_pair(const xb_synth_cfg* cfg, float* tmpl, float* sci);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XBACH_H */

@@ -1,0 +1,444 @@
+"""Python host binding of libxbach, the MI355X-native X-BACH subtraction path.
+
+The product is the C-ABI library ``libxbach.so`` (``include/xbach.h``) and the
+C++ CLI ``xbach`` built next to it.  This module mirrors the reference's stage
+interface (include/bach.h:66-87: init, sss, cmv, cd, ksc, conv, sub) over that
+C ABI with ctypes, so Python callers (tests, the benchmark) drive exactly the
+code the CLI runs.  Loading fails loudly when the library has not been built:
+there is no CPU fallback.
+
+FITS helpers read/write single-HDU BITPIX -32/-64 images (the reference reads
+with CCfits, src/fitsUtil.cpp:8-38, and writes FLOAT_IMG, src/fitsUtil.cpp:40-64).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+PKG_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libxbach.so"
+
+MAX_STAMPS = 1024
+MAX_SS = 16
+MAX_SOL = 512
+
+# status codes (include/xbach.h)
+XB_OK = 0
+STATUS_NAMES = {
+    0: "XB_OK", 1: "XB_ERR_ARG", 2: "XB_ERR_SIZE_MISMATCH", 3: "XB_ERR_NO_SUBSTAMPS",
+    4: "XB_ERR_STAMP_PIXELS", 5: "XB_ERR_HIP", 6: "XB_ERR_NO_DEVICE",
+    7: "XB_ERR_NOT_BUILT", 8: "XB_ERR_CAPACITY",
+}
+
+# mask bits (include/datatypeUtil.h:172-188)
+MASK_BAD_PIX_VAL = 1 << 0
+MASK_SAT_PIXEL = 1 << 1
+MASK_LOW_PIXEL = 1 << 2
+MASK_NAN_PIXEL = 1 << 3
+MASK_BAD_CONV = 1 << 4
+MASK_OK_CONV = 1 << 6
+MASK_BAD_INPUT = 1 << 7
+MASK_BAD_PIXEL_T = 1 << 8
+MASK_SKIP_T = 1 << 9
+MASK_BAD_PIXEL_S = 1 << 10
+MASK_SKIP_S = 1 << 11
+MASK_BAD_OUTPUT = 1 << 12
+
+
+class Params(C.Structure):
+    """The reference's ``Arguments`` (include/argsUtil.h:7-57) minus I/O names."""
+
+    _fields_ = [
+        ("stampsx", C.c_int32), ("stampsy", C.c_int32),
+        ("threshLow", C.c_double), ("threshHigh", C.c_double),
+        ("threshKernFit", C.c_double), ("sigKernFit", C.c_double),
+        ("sigClipAlpha", C.c_double), ("iqRange", C.c_double),
+        ("maxKSStamps", C.c_int32),
+        ("hSStampWidth", C.c_int32), ("fSStampWidth", C.c_int32),
+        ("hKernelWidth", C.c_int32), ("fKernelWidth", C.c_int32),
+        ("inSpreadMaskFactor", C.c_double),
+        ("normalizeTemplate", C.c_int32),
+        ("backgroundOrder", C.c_int32), ("kernelOrder", C.c_int32),
+        ("ngauss", C.c_int32), ("dg", C.c_int32 * 3), ("bg", C.c_float * 3),
+        ("verbose", C.c_int32),
+    ]
+
+    @classmethod
+    def default(cls, **kw) -> "Params":
+        p = cls()
+        p.stampsx = p.stampsy = 10
+        p.threshLow, p.threshHigh = 0.0, 25000.0
+        p.threshKernFit, p.sigKernFit = 20.0, 2.0
+        p.sigClipAlpha, p.iqRange = 3.0, 1.35
+        p.maxKSStamps = 3
+        p.hSStampWidth, p.fSStampWidth = 15, 31
+        p.hKernelWidth, p.fKernelWidth = 10, 21
+        p.inSpreadMaskFactor = 1.0
+        p.normalizeTemplate = 1
+        p.backgroundOrder, p.kernelOrder = 1, 2
+        p.ngauss = 3
+        p.dg[:] = [6, 4, 2]
+        # 1/(2 sigma^2) for sigma = 0.7, 1.5, 3.0, stored as float (argsUtil.h:49-53)
+        p.bg[:] = [1.0 / (2.0 * 0.7 * 0.7), 1.0 / (2.0 * 1.5 * 1.5), 1.0 / (2.0 * 3.0 * 3.0)]
+        p.verbose = 0
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class Trace(C.Structure):
+    """Decision record of one run (include/xbach.h ``xb_trace``)."""
+
+    _fields_ = [
+        ("w", C.c_int32), ("h", C.c_int32),
+        ("stampsx", C.c_int32), ("stampsy", C.c_int32),
+        ("fStampWidth", C.c_int32), ("nGrid", C.c_int32),
+        ("retried", C.c_int32),
+        ("sky", (C.c_double * MAX_STAMPS) * 2),
+        ("fwhm", (C.c_double * MAX_STAMPS) * 2),
+        ("ssCount", (C.c_int32 * MAX_STAMPS) * 2),
+        ("ssX", (C.c_int32 * (MAX_STAMPS * MAX_SS)) * 2),
+        ("ssY", (C.c_int32 * (MAX_STAMPS * MAX_SS)) * 2),
+        ("ssVal", (C.c_double * (MAX_STAMPS * MAX_SS)) * 2),
+        ("stampCount", C.c_int32 * 2),
+        ("stampIndex", (C.c_int32 * MAX_STAMPS) * 2),
+        ("nTest", C.c_int32 * 2),
+        ("kernelSumCd", C.c_double * 2),
+        ("merit", C.c_double * 2),
+        ("cdSolution", (C.c_double * MAX_SOL) * 2),
+        ("convTemplate", C.c_int32),
+        ("kscIterations", C.c_int32),
+        ("nSol", C.c_int32),
+        ("solution", C.c_double * MAX_SOL),
+        ("finalCurrent", C.c_int32 * MAX_STAMPS),
+        ("kernSum", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        """Numpy views of the populated part of the record."""
+        n = self.nGrid
+        d = {
+            "w": self.w, "h": self.h, "stampsx": self.stampsx, "stampsy": self.stampsy,
+            "fStampWidth": self.fStampWidth, "nGrid": n, "retried": self.retried,
+            "convTemplate": self.convTemplate, "kscIterations": self.kscIterations,
+            "nSol": self.nSol, "kernSum": self.kernSum,
+            "stampCount": tuple(self.stampCount), "nTest": tuple(self.nTest),
+            "merit": tuple(self.merit), "kernelSumCd": tuple(self.kernelSumCd),
+        }
+        for im in range(2):
+            d[f"sky{im}"] = np.array(self.sky[im][:n])
+            d[f"fwhm{im}"] = np.array(self.fwhm[im][:n])
+            d[f"ssCount{im}"] = np.array(self.ssCount[im][:n])
+            d[f"ssX{im}"] = np.array(self.ssX[im][: n * MAX_SS]).reshape(n, MAX_SS)
+            d[f"ssY{im}"] = np.array(self.ssY[im][: n * MAX_SS]).reshape(n, MAX_SS)
+            d[f"ssVal{im}"] = np.array(self.ssVal[im][: n * MAX_SS]).reshape(n, MAX_SS)
+            d[f"stampIndex{im}"] = np.array(self.stampIndex[im][: self.stampCount[im]])
+            d[f"cdSolution{im}"] = np.array(self.cdSolution[im][: self.nSol])
+        d["solution"] = np.array(self.solution[: self.nSol])
+        d["finalCurrent"] = np.array(self.finalCurrent[: self.stampCount[1 - self.convTemplate]]
+                                     if self.convTemplate == 0 else
+                                     self.finalCurrent[: self.stampCount[0]])
+        return d
+
+
+class SynthCfg(C.Structure):
+    _fields_ = [
+        ("w", C.c_int32), ("h", C.c_int32), ("nstars", C.c_int32),
+        ("fluxMin", C.c_double), ("fluxMax", C.c_double), ("fluxSlope", C.c_double),
+        ("fwhmT", C.c_double), ("fwhmS0", C.c_double), ("dfwhmX", C.c_double),
+        ("dfwhmY", C.c_double), ("sky", C.c_double), ("skyGx", C.c_double),
+        ("skyGy", C.c_double), ("sciFluxScale", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+# Seeded synthetic workloads (names give what they are).
+SYNTH_CONFIGS = {
+    # 512x512, 300 stars, FWHM 2.0 (template) / 3.0 (science), sky 100
+    "pair512": dict(w=512, h=512, nstars=300, fluxMin=1e2, fluxMax=1e5, fluxSlope=-1.5,
+                    fwhmT=2.0, fwhmS0=3.0, dfwhmX=0.0, dfwhmY=0.0, sky=100.0, skyGx=0.0,
+                    skyGy=0.0, sciFluxScale=1.0, seed=512001),
+    # 2048x2048, 6000 stars, science FWHM 2.5 -> 3.5 ramp, template 1.8, sky 200 + gradient
+    "pair2048": dict(w=2048, h=2048, nstars=6000, fluxMin=1e2, fluxMax=1e5, fluxSlope=-1.5,
+                     fwhmT=1.8, fwhmS0=2.5, dfwhmX=0.5, dfwhmY=0.5, sky=200.0, skyGx=0.02,
+                     skyGy=0.01, sciFluxScale=1.0, seed=2048001),
+}
+
+
+def synth_cfg(name: str | None = None, **kw) -> SynthCfg:
+    base = dict(SYNTH_CONFIGS[name]) if name else {}
+    base.update(kw)
+    c = SynthCfg()
+    for k, v in base.items():
+        setattr(c, k, v)
+    return c
+
+
+# ----------------------------------------------------------------------------
+# FITS (single primary HDU, BITPIX -32 / -64)
+# ----------------------------------------------------------------------------
+def _fits_open(path):
+    path = str(path)
+    if path.endswith(".gz"):
+        import gzip
+        with gzip.open(path, "rb") as f:
+            return f.read()
+    with open(path, "rb") as f:
+        return f.read()
+
+
+def read_fits(path) -> tuple[dict, np.ndarray]:
+    """Header cards and FP64 data (h, w) of the primary HDU."""
+    raw = _fits_open(path)
+    hdr, off = {}, 0
+    while True:
+        blk = raw[off:off + 2880]
+        off += 2880
+        done = False
+        for i in range(36):
+            card = blk[i * 80:(i + 1) * 80].decode("ascii", "replace")
+            key = card[:8].strip()
+            if key == "END":
+                done = True
+                break
+            if card[8:10] == "= ":
+                val = card[10:].split("/")[0].strip().strip("'").strip()
+                hdr[key] = val
+        if done:
+            break
+    bitpix = int(hdr["BITPIX"])
+    if bitpix not in (-32, -64):
+        raise ValueError(f"fits image of type{bitpix} is not supported.")
+    n1, n2 = int(hdr["NAXIS1"]), int(hdr["NAXIS2"])
+    dt = ">f4" if bitpix == -32 else ">f8"
+    data = np.frombuffer(raw[off:off + n1 * n2 * abs(bitpix) // 8], dtype=dt)
+    data = data.reshape(n2, n1).astype(np.float64)
+    bscale = float(hdr.get("BSCALE", 1) or 1)
+    bzero = float(hdr.get("BZERO", 0) or 0)
+    if bscale != 1.0 or bzero != 0.0:
+        data = data * bscale + bzero
+    return hdr, data
+
+
+def write_fits(path, data: np.ndarray) -> None:
+    """FLOAT_IMG primary HDU, overwriting (the reference's "!" prefix)."""
+    data = np.asarray(data, dtype=">f4")
+    h, w = data.shape
+    cards = [
+        "SIMPLE  =                    T",
+        "BITPIX  =                  -32",
+        "NAXIS   =                    2",
+        f"NAXIS1  = {w:20d}",
+        f"NAXIS2  = {h:20d}",
+        "EXTEND  =                    T",
+        "END",
+    ]
+    hdr = "".join(c.ljust(80) for c in cards).encode("ascii")
+    hdr += b" " * ((2880 - len(hdr) % 2880) % 2880)
+    body = data.tobytes()
+    body += b"\0" * ((2880 - len(body) % 2880) % 2880)
+    with open(path, "wb") as f:
+        f.write(hdr + body)
+
+
+# ----------------------------------------------------------------------------
+# libxbach
+# ----------------------------------------------------------------------------
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libxbach.so.  Raises when it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR}` "
+            "(or __graft_entry__.build()); the GPU path has no fallback")
+    L = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    vp = C.c_void_p
+    L.xb_params_default.argtypes = [P(Params)]
+    L.xb_ctx_create.argtypes = [C.c_int, C.c_int, P(vp)]
+    L.xb_ctx_destroy.argtypes = [vp]
+    L.xb_last_error.argtypes = [vp]
+    L.xb_last_error.restype = C.c_char_p
+    L.xb_device_name.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.xb_init.argtypes = [vp, P(Params), C.c_int, C.c_int, vp, vp]
+    L.xb_init_device.argtypes = [vp, P(Params), C.c_int, C.c_int, vp, vp]
+    L.xb_sss.argtypes = [vp]
+    L.xb_cmv.argtypes = [vp]
+    L.xb_cd.argtypes = [vp, P(C.c_int)]
+    L.xb_ksc.argtypes = [vp]
+    L.xb_conv.argtypes = [vp, P(C.c_double)]
+    L.xb_sub.argtypes = [vp]
+    L.xb_get_conv.argtypes = [vp, vp]
+    L.xb_get_diff.argtypes = [vp, vp]
+    L.xb_get_mask.argtypes = [vp, vp]
+    L.xb_get_trace.argtypes = [vp, P(Trace)]
+    L.xb_device_outputs.argtypes = [vp, P(vp), P(vp)]
+    L.xb_pipeline_run.argtypes = [vp, P(Params), C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    L.xb_pipeline_run_device.argtypes = [vp, P(Params), C.c_int, C.c_int, vp, vp]
+    L.xb_synth_pair.argtypes = [P(SynthCfg), vp, vp]
+    _lib = L
+    return L
+
+
+def synth_pair(cfg: SynthCfg) -> tuple[np.ndarray, np.ndarray]:
+    """Seeded synthetic (template, science) FP32 pair of shape (h, w)."""
+    t = np.empty((cfg.h, cfg.w), np.float32)
+    s = np.empty((cfg.h, cfg.w), np.float32)
+    rc = lib().xb_synth_pair(C.byref(cfg), t.ctypes.data, s.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"xb_synth_pair failed: {STATUS_NAMES.get(rc, rc)}")
+    return t, s
+
+
+class XbachError(RuntimeError):
+    pass
+
+
+class Context:
+    """One device context (xb_ctx): stream, resident buffers, stage API."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.xb_ctx_create(device, 0, C.byref(h))
+        if rc != 0:
+            raise XbachError(f"xb_ctx_create failed: {STATUS_NAMES.get(rc, rc)}")
+        self._h = h
+        self.w = self.h = 0
+
+    def close(self):
+        if self._h:
+            self._L.xb_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.xb_last_error(self._h)
+            raise XbachError(f"{what}: {STATUS_NAMES.get(rc, rc)}: "
+                             f"{msg.decode() if msg else ''}")
+
+    @property
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        self._L.xb_device_name(self._h, buf, 256)
+        return buf.value.decode()
+
+    # ---- stages (include/bach.h:66-87) -------------------------------------
+    def init(self, params: Params, tmpl: np.ndarray, sci: np.ndarray):
+        t = np.ascontiguousarray(tmpl, np.float32)
+        s = np.ascontiguousarray(sci, np.float32)
+        self.h, self.w = t.shape
+        self._check(self._L.xb_init(self._h, C.byref(params), self.w, self.h,
+                                    t.ctypes.data, s.ctypes.data), "xb_init")
+
+    def init_device(self, params: Params, w: int, h: int, d_tmpl: int, d_sci: int):
+        self.w, self.h = w, h
+        self._check(self._L.xb_init_device(self._h, C.byref(params), w, h, d_tmpl, d_sci),
+                    "xb_init_device")
+
+    def sss(self):
+        self._check(self._L.xb_sss(self._h), "xb_sss")
+
+    def cmv(self):
+        self._check(self._L.xb_cmv(self._h), "xb_cmv")
+
+    def cd(self) -> bool:
+        ct = C.c_int()
+        self._check(self._L.xb_cd(self._h, C.byref(ct)), "xb_cd")
+        return bool(ct.value)
+
+    def ksc(self):
+        self._check(self._L.xb_ksc(self._h), "xb_ksc")
+
+    def conv(self) -> float:
+        ks = C.c_double()
+        self._check(self._L.xb_conv(self._h, C.byref(ks)), "xb_conv")
+        return ks.value
+
+    def sub(self):
+        self._check(self._L.xb_sub(self._h), "xb_sub")
+
+    # ---- results ------------------------------------------------------------
+    def conv_image(self) -> np.ndarray:
+        out = np.empty((self.h, self.w), np.float32)
+        self._check(self._L.xb_get_conv(self._h, out.ctypes.data), "xb_get_conv")
+        return out
+
+    def diff_image(self) -> np.ndarray:
+        out = np.empty((self.h, self.w), np.float32)
+        self._check(self._L.xb_get_diff(self._h, out.ctypes.data), "xb_get_diff")
+        return out
+
+    def mask(self) -> np.ndarray:
+        out = np.empty((self.h, self.w), np.uint16)
+        self._check(self._L.xb_get_mask(self._h, out.ctypes.data), "xb_get_mask")
+        return out
+
+    def trace(self) -> Trace:
+        tr = Trace()
+        self._check(self._L.xb_get_trace(self._h, C.byref(tr)), "xb_get_trace")
+        return tr
+
+    def run(self, params: Params, tmpl: np.ndarray, sci: np.ndarray):
+        """Whole path; returns (conv, diff, stage_ms[9])."""
+        t = np.ascontiguousarray(tmpl, np.float32)
+        s = np.ascontiguousarray(sci, np.float32)
+        self.h, self.w = t.shape
+        conv = np.empty_like(t)
+        diff = np.empty_like(t)
+        ms = np.zeros(9, np.float64)
+        self._check(self._L.xb_pipeline_run(self._h, C.byref(params), self.w, self.h,
+                                             t.ctypes.data, s.ctypes.data, conv.ctypes.data,
+                                             diff.ctypes.data, ms.ctypes.data),
+                    "xb_pipeline_run")
+        return conv, diff, ms
+
+    def run_device(self, params: Params, w: int, h: int, d_tmpl: int, d_sci: int):
+        self.w, self.h = w, h
+        self._check(self._L.xb_pipeline_run_device(self._h, C.byref(params), w, h,
+                                                   d_tmpl, d_sci), "xb_pipeline_run_device")
+
+    def device_outputs(self) -> tuple[int, int]:
+        c, d = C.c_void_p(), C.c_void_p()
+        self._check(self._L.xb_device_outputs(self._h, C.byref(c), C.byref(d)),
+                    "xb_device_outputs")
+        return c.value, d.value
+
+
+def diff_fits_stats(golden: np.ndarray, ours: np.ndarray) -> dict:
+    """The comparison of tools/run_test.py:34-93 (diff_fits): max/mean absolute
+    error, relative error only where the golden pixel is > 0, NaN mismatches."""
+    g = np.asarray(golden, np.float64)
+    b = np.asarray(ours, np.float64)
+    nan_g, nan_b = np.isnan(g), np.isnan(b)
+    wrong_nans = int(np.count_nonzero(nan_g != nan_b))
+    ok = ~(nan_g | nan_b)
+    err = np.abs(g - b)[ok]
+    pos = (g > 0) & ok
+    rel = np.abs(g - b)[pos] / g[pos]
+    return {
+        "max_abs": float(err.max()) if err.size else 0.0,
+        "mean_abs": float(err.mean()) if err.size else 0.0,
+        "max_rel": float(rel.max()) if rel.size else 0.0,
+        "mean_rel": float(rel.sum() / max(int(ok.sum()), 1)),
+        "wrong_nans": wrong_nans,
+    }
