@@ -1,0 +1,289 @@
+"""Pins of the CPU oracle (oracle/xbach_oracle.cpp) — CPU only.
+
+The oracle is the checker of the GPU path; these tests check the checker
+against what the reference itself holds and against what mathematics fixes:
+
+* test 1 of the reference (tools/run_test.py:13): HOTPANTS 5.1.11 goldens
+  tests/test01_{conv,sub}.fits within the reference's own tolerances, plus the
+  golden header's kernel sum (KSUM00) and convolution direction (CONVOL00);
+* the 293x293 global normal matrix dumped in tools/cond_num.py:3-297;
+* closed forms / textbook routines for the pieces those leave open.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import _oracle
+import xbach
+
+P = xbach.Params.default
+
+
+# --------------------------------------------------------------------------
+# reference-held pins
+# --------------------------------------------------------------------------
+def test_test01_matches_hotpants_goldens(test01_pair):
+    tmpl, sci, gconv, gsub = test01_pair
+    conv, diff, mask, tr, _ = _oracle.run(P(), tmpl, sci)
+    c = xbach.diff_fits_stats(gconv, conv)
+    s = xbach.diff_fits_stats(gsub, diff)
+    # tools/run_test.py:13 test 1 tolerances, :149-151 pass rule
+    assert c["max_abs"] < 2e-4 and c["max_rel"] < 5e-3 and c["wrong_nans"] == 0, c
+    assert s["max_abs"] < 5e-4 and s["max_rel"] < 4e-3 and s["wrong_nans"] == 0, s
+
+
+def test_test01_header_scalars(test01_pair):
+    tmpl, sci, _, _ = test01_pair
+    _, _, _, tr, _ = _oracle.run(P(), tmpl, sci)
+    # KSUM00 = '-0.1658' (kernel sum at the image centre), CONVOL00 = 'IMAGE'
+    # (HOTPANTS convolved the science image, i.e. convTemplate is false)
+    assert round(tr.kernSum, 4) == -0.1658
+    assert tr.convTemplate == 0
+    # NX2NRM00 = 96099 good pixels of the 330^2 interior (96,100 minus 1)
+    conv, diff, mask, _, _ = _oracle.run(P(), tmpl, sci)
+    interior = np.zeros_like(diff, bool)
+    interior[10:-10, 10:-10] = True
+    assert abs(int(np.count_nonzero(interior & (diff != np.float32(1e-30)))) - 96099) <= 200
+
+
+def test_test01_global_matrix_matches_cond_num_dump(test01_pair):
+    tmpl, sci, _, _ = test01_pair
+    ref = np.load(xbach.PKG_DIR.parent / "tests/golden/test01_mtx.npz")["mtx"]
+    O = _oracle.lib()
+    O.xo_capture(1)
+    try:
+        _oracle.run(P(), tmpl, sci)
+        mats = _oracle.captured_matrices()
+    finally:
+        O.xo_capture(0)
+    assert len(mats) >= 3
+    nz = ref != 0
+    best = min(float(np.max(np.abs(M[nz] - ref[nz]) / np.abs(ref[nz]))) for M, _ in mats)
+    # 6 significant digits in the dump -> <= 5e-6 relative rounding
+    assert best < 1e-5
+    matched = [M for M, _ in mats if np.max(np.abs(M[nz] - ref[nz]) / np.abs(ref[nz])) < 1e-5]
+    assert all(np.array_equal(M == 0, ref == 0) for M in matched)
+
+
+# --------------------------------------------------------------------------
+# sub / conv closed forms on the oracle's own outputs
+# --------------------------------------------------------------------------
+def test_sub_is_elementwise(test01_pair):
+    tmpl, sci, _, _ = test01_pair
+    conv, diff, mask, tr, _ = _oracle.run(P(), tmpl, sci)
+    ks = tr.kernSum
+    target = sci if tr.convTemplate == 1 else tmpl  # the image not convolved
+    scale = bool(tr.convTemplate)  # normalizeTemplate=1: scaleConv == convTemplate
+    interior = np.zeros_like(diff, bool)
+    interior[10:-10, 10:-10] = True
+    good = interior & ((mask & xbach.MASK_BAD_OUTPUT) == 0)
+    c = conv.astype(np.float64)
+    t = target.astype(np.float64)
+    cf, ff = (ks, -1.0 / ks) if scale else (1.0, 1.0)
+    expect = (c * cf - t) * ff
+    assert np.all(diff[~good] == np.float32(1e-30))
+    # conv and diff are stored as float32: allow both roundings (2^-23 each)
+    tol = 1.2e-7 * ((np.abs(c) * abs(cf) + np.abs(t)) * abs(ff) + np.abs(expect)) + 1e-12
+    err = np.abs(diff.astype(np.float64) - expect)
+    assert np.max(err[good] / tol[good]) < 1.0
+
+
+# --------------------------------------------------------------------------
+# ran1 (cl_kern/sss.cl:41-79): NR ran1, idum = -666
+# --------------------------------------------------------------------------
+def _ran1_python(n):
+    m1, ia1, ic1 = 259200, 7141, 54773
+    m2, ia2, ic2 = 134456, 8121, 28411
+    m3, ia3, ic3 = 243000, 4561, 51349
+    rm1, rm2 = 1.0 / m1, 1.0 / m2
+    idum = -666
+    ix1 = (ic1 - idum) % m1
+    ix1 = (ia1 * ix1 + ic1) % m1
+    ix2 = ix1 % m2
+    ix1 = (ia1 * ix1 + ic1) % m1
+    ix3 = ix1 % m3
+    r = [0.0] * 98
+    for j in range(1, 98):
+        ix1 = (ia1 * ix1 + ic1) % m1
+        ix2 = (ia2 * ix2 + ic2) % m2
+        r[j] = (ix1 + ix2 * rm2) * rm1
+    out = []
+    for _ in range(n):
+        ix1 = (ia1 * ix1 + ic1) % m1
+        ix2 = (ia2 * ix2 + ic2) % m2
+        ix3 = (ia3 * ix3 + ic3) % m3
+        j = 1 + (97 * ix3) // m3
+        out.append(r[j])
+        r[j] = (ix1 + ix2 * rm2) * rm1
+    return np.array(out)
+
+
+def test_ran1_sequence_and_uniformity():
+    got = _oracle.ran1(10000)
+    assert np.array_equal(got, _ran1_python(10000))
+    assert np.all((got > 0) & (got < 1))
+    # Kolmogorov-Smirnov against U(0,1)
+    x = np.sort(got)
+    n = len(x)
+    d = max(np.max(np.arange(1, n + 1) / n - x), np.max(x - np.arange(n) / n))
+    assert d < 1.63 / math.sqrt(n)  # 1 % level
+
+
+# --------------------------------------------------------------------------
+# sigma clip (src/bachUtil.cpp:34-128)
+# --------------------------------------------------------------------------
+def test_sigma_clip_closed_forms():
+    assert _oracle.sigma_clip([], 3) == (0.0, 1e10)
+    assert _oracle.sigma_clip([5.0], 3) == (0.0, 1e10)
+    d = np.array([1.0, 2.0, 3.0, 4.0, 100.0])
+    m, s = _oracle.sigma_clip(d, 10, 3.0)
+    assert m == 22.0 and s == pytest.approx(math.sqrt(1902.5), rel=1e-15)
+    # alpha 1.5 clips the outlier; the second pass gives the mean of 1..4
+    m, s = _oracle.sigma_clip(d, 10, 1.5)
+    assert m == 2.5 and s == pytest.approx(np.std([1, 2, 3, 4], ddof=1), rel=1e-15)
+    # maxIter 1 reports the first pass even though it clipped
+    m, s = _oracle.sigma_clip(d, 1, 1.5)
+    assert m == 22.0
+
+
+def test_sigma_clip_matches_numpy_iteration():
+    rng = np.random.default_rng(7)
+    d = np.concatenate([rng.normal(100, 5, 5000), rng.normal(400, 5, 50)])
+    m, s = _oracle.sigma_clip(d, 3, 3.0)
+    keep = np.ones(len(d), bool)
+    for _ in range(3):
+        mu, sd = d[keep].mean(), d[keep].std(ddof=1)
+        new = keep & (np.abs(d - mu) / sd <= 3.0)
+        if new.sum() == keep.sum():
+            break
+        keep = new
+    assert m == pytest.approx(mu, rel=1e-12) and s == pytest.approx(sd, rel=1e-10)
+
+
+# --------------------------------------------------------------------------
+# LU (NR ludcmp/lubksb, src/bachUtil.cpp:319-411)
+# --------------------------------------------------------------------------
+def test_lu_matches_numpy_solve():
+    rng = np.random.default_rng(3)
+    for n in (5, 50, 120):
+        a = rng.normal(size=(n, n)) + n * np.eye(n)
+        a[0, 0] = 0.0  # forces a pivot
+        b = rng.normal(size=n)
+        rc, x = _oracle.lu_solve(a, b)
+        assert rc == 0
+        assert np.allclose(x, np.linalg.solve(a, b), rtol=1e-10, atol=1e-12)
+
+
+def test_lu_zero_row_returns_status():
+    a = np.eye(4)
+    a[2, :] = 0.0
+    rc, _ = _oracle.lu_solve(a, np.ones(4))
+    assert rc == 1
+
+
+def test_stamp_lu_form_matches_numpy():
+    rng = np.random.default_rng(5)
+    m = 51
+    a = np.zeros((m, m))
+    q = rng.normal(size=(60, m - 1))
+    a[1:, 1:] = q.T @ q
+    b = np.zeros(m)
+    b[1:] = rng.normal(size=m - 1)
+    x = b.copy()
+    aa = a.copy()
+    _oracle.lib().xo_lu_solve_stamp(aa.ctypes.data, m, x.ctypes.data)
+    assert np.allclose(x[1:], np.linalg.solve(a[1:, 1:], b[1:]), rtol=1e-8)
+
+
+# --------------------------------------------------------------------------
+# bitonic sort (cl_kern/sss.cl:135-150)
+# --------------------------------------------------------------------------
+def test_bitonic_equals_sort():
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        v = np.concatenate([rng.normal(size=rng.integers(0, 100)), np.zeros(3)])
+        v = np.concatenate([v, np.full(128 - len(v), np.inf)]) if len(v) < 128 else v[:128]
+        assert np.array_equal(_oracle.bitonic(v), np.sort(v))
+
+
+# --------------------------------------------------------------------------
+# basis (cl_kern/cmv.cl:1-68, include/datatypeUtil.h:62-118)
+# --------------------------------------------------------------------------
+def test_basis_closed_form_and_sums():
+    p = P()
+    n, fx, fy, vec = _oracle.basis(p)
+    assert n == 49
+    x = np.arange(-10, 11, dtype=np.float64)
+    terms = [(g, i, j) for g, d in enumerate((6, 4, 2)) for i in range(d + 1) for j in range(d + 1 - i)]
+    for k, (g, i, j) in enumerate(terms):
+        b = np.float64(np.float32(p.bg[g]))
+        ex = np.exp(-x * x * b) * x ** i
+        ey = np.exp(-x * x * b) * x ** j
+        if i % 2 == 0 and j % 2 == 0:
+            ex, ey = ex / ex.sum(), ey / ey.sum()
+        assert np.allclose(fx[k], ex, rtol=1e-14, atol=1e-300)
+        assert np.allclose(fy[k], ey, rtol=1e-14, atol=1e-300)
+    sums = vec.reshape(49, -1).sum(axis=1)
+    assert sums[0] == pytest.approx(1.0, abs=1e-14)
+    assert np.all(np.abs(sums[1:]) < 1e-12)
+
+
+# --------------------------------------------------------------------------
+# fillStamps (cl_kern/cmv.cl:70-248): separable Y then X == direct 2D
+# convolution; Q = W^T W; B = W . target
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("mode", [_oracle.MODE_REF, _oracle.MODE_FP64])
+def test_fill_equals_direct_convolution(mode):
+    p = P()
+    rng = np.random.default_rng(11)
+    img = rng.poisson(200.0, size=(96, 96)).astype(np.float32).astype(np.float64)
+    img2 = rng.poisson(200.0, size=(96, 96)).astype(np.float32).astype(np.float64)
+    ssx, ssy = 50, 47
+    W, Q, B = _oracle.fill_one(p, img, img2, ssx, ssy, mode)
+    n, fx, fy, vec = _oracle.basis(p)
+    terms = [(i, j) for d in (6, 4, 2) for i in range(d + 1) for j in range(d + 1 - i)]
+    # direct true convolution of the 31x31 substamp with K[v][u] = fx[u] fy[v]
+    win = img[ssy - 25:ssy + 26, ssx - 25:ssx + 26]
+    raw0 = None
+    tol = 1e-6 if mode == _oracle.MODE_REF else 1e-12
+    for k in range(49):
+        K = np.outer(fy[k], fx[k])
+        out = np.zeros((31, 31))
+        for jy in range(31):
+            for ix in range(31):
+                patch = win[jy + 10 - 10 + 0: jy + 10 + 11, ix + 10 - 10: ix + 10 + 11]
+                out[jy, ix] = np.sum(patch[::-1, ::-1] * K)
+        if k == 0:
+            raw0 = out
+        expect = out - raw0 if (k > 0 and terms[k][0] % 2 == 0 and terms[k][1] % 2 == 0) else out
+        scale = np.abs(win).max() * np.abs(K).sum()
+        assert np.max(np.abs(W[k].reshape(31, 31) - expect)) <= tol * scale
+    # background rows: 1, yf, xf with FP32 xf, yf (cmv.cl:180-181)
+    yy, xx = np.mgrid[-15:16, -15:16]
+    xf = ((np.float32(ssx + xx) - np.float32(48.0)) / np.float32(48.0)).astype(np.float64)
+    yf = ((np.float32(ssy + yy) - np.float32(48.0)) / np.float32(48.0)).astype(np.float64)
+    assert np.array_equal(W[49], np.ones(961))
+    assert np.array_equal(W[50], yf.ravel())
+    assert np.array_equal(W[51], xf.ravel())
+    G = W[:50] @ W[:50].T
+    lower = np.tril(np.ones((50, 50), bool))
+    assert np.allclose(Q[1:, 1:][lower], G[lower], rtol=1e-12, atol=1e-9)
+    assert np.all(Q[0] == 0) and np.all(Q[:, 0] == 0) and np.all(Q[1:, 1:][~lower] == 0)
+    t = img2[ssy - 15:ssy + 16, ssx - 15:ssx + 16].ravel()
+    assert np.allclose(B[1:], W[:50] @ t, rtol=1e-12)
+    assert B[0] == 0.0
+
+
+# --------------------------------------------------------------------------
+# histogram sky / sky sigma (cl_kern/sss.cl:190-324)
+# --------------------------------------------------------------------------
+def test_histogram_recovers_gaussian_sky():
+    rng = np.random.default_rng(2)
+    mu, sig = 200.0, 14.0
+    pix = rng.normal(mu, sig, 30000)
+    samples = np.sort(rng.choice(pix, 100, replace=False))
+    padded = np.concatenate([samples, np.full(28, np.inf)])
+    sky, fw = _oracle.histogram(pix, padded, 100, mu, 1.0 / sig)
+    assert abs(sky - mu) < 0.1 * sig
+    assert abs(fw / sig - 1.0) < 0.1  # IQR / 1.35 of a normal is sigma

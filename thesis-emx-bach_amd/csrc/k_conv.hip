@@ -1,0 +1,285 @@
+// k_conv.hip — spatially varying convolution and the difference image.
+//
+// Tile kernels (src/bach.cpp:336-355 realises them on the host, 9,604 calls at
+// 2048^2): one workgroup per 21x21 tile forms the kernel at the tile centre in
+// FP64 with the reference's sums (src/bachUtil.cpp:502-541) and stores it as
+// FP32 for the convolution plus FP64 for the mask test, with sum |k| in the
+// convolution loop's order (cl_kern/conv.cl:409-426).
+//
+// Convolution (cl_kern/conv.cl:31-97): one wave per tile, lane = (column,
+// 7-row strip); the tile kernel is wave-uniform (scalar loads), image columns
+// stream through registers, FP32 FMA accumulation.  The output mask needs
+// sum |k| over unmasked pixels only when the window holds a flagged pixel; that
+// rare case re-walks the window in the reference order in FP64 so the
+// BAD_CONV decision is exact.  maskAfterConv (conv.cl:99-113) is fused.
+//
+// sub (cl_kern/sub.cl:15-32): elementwise, HBM-bound.
+#include <algorithm>
+
+#include "xb_dev.h"
+
+namespace xb {
+
+__device__ __forceinline__ double kernel_coeff_c(const double* sol, int i, double xf, double yf) {
+  if (i == 0) return sol[1];
+  int k = 2 + (i - 1) * NC2;
+  double c0 = 0.0, aX = 1.0;
+  for (int x = 0; x <= 2; x++) {
+    double aY = 1.0;
+    for (int y = 0; y <= 2 - x; y++) {
+      c0 += sol[k++] * aX * aY;
+      aY *= yf;
+    }
+    aX *= xf;
+  }
+  return c0;
+}
+
+__device__ __forceinline__ double background_c(const double* sol, int x, int y, int w, int h) {
+  double xf = (x - 0.5 * w) / (0.5 * w);
+  double yf = (y - 0.5 * h) / (0.5 * h);
+  double bg = 0.0, ax = 1.0;
+  int k = NCOMP + 2;
+  for (int i = 0; i <= 1; i++) {
+    double ay = 1.0;
+    for (int j = 0; j <= 1 - i; j++) {
+      bg += sol[k++] * ax * ay;
+      ay *= yf;
+    }
+    ax *= xf;
+  }
+  return bg;
+}
+
+// tile kernels; blockIdx.x = tile, or the extra block = kernel sum at the centre
+__global__ __launch_bounds__(512) void k_tile_kernels(const Basis* __restrict__ basis,
+                                                      const double* __restrict__ sol, int w,
+                                                      int h, int xSteps, int nTiles,
+                                                      float* tileK, double* tileK64,
+                                                      double* tileAks, double* kernSumOut) {
+  const int t = blockIdx.x;
+  const int tid = threadIdx.x;
+  int x, y;
+  if (t < nTiles) {
+    x = (t % xSteps) * KW + HKW + HKW;
+    y = (t / xSteps) * KW + HKW + HKW;
+  } else {
+    x = w / 2;
+    y = h / 2;
+  }
+  __shared__ double co[NPSF];
+  __shared__ double kern[KK];
+  const double hw = 0.5 * w, hh = 0.5 * h;
+  const double xf = (x - hw) / hw, yf = (y - hh) / hh;
+  if (tid < NPSF) co[tid] = kernel_coeff_c(sol, tid, xf, yf);
+  __syncthreads();
+  if (tid < KK) {
+    double k0 = 0.0;
+    for (int p = 0; p < NPSF; p++) k0 += co[p] * basis->vec[p * KK + tid];
+    kern[tid] = k0;
+    if (t < nTiles) {
+      tileK[(size_t)t * KK + tid] = (float)k0;
+      tileK64[(size_t)t * KK + tid] = k0;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    if (t < nTiles) {
+      double aks = 0.0;
+      for (int i = KK - 1; i >= 0; i--) aks += fabs(kern[i]);
+      tileAks[t] = aks;
+    } else {
+      double sum = 0.0;
+      for (int i = 0; i < KK; i++) sum += kern[i];
+      *kernSumOut = sum;
+    }
+  }
+}
+
+struct ConvArgs {
+  const float* T;      // image being convolved
+  const float* S;      // target image
+  uint16_t* mask;
+  float* out;
+  const float* tileK;
+  const double* tileK64;
+  const double* tileAks;
+  const double* sol;
+  const double* kernSum;
+  int w, h, xSteps, yTiles;
+  int scaleConv;
+  double hi, lo;
+};
+
+__device__ __forceinline__ uint16_t conv_flags(float t, double hi, double lo) {
+  double d = (double)t;
+  uint16_t m = 0;
+  if (d == 0.0) m |= XB_MASK_BAD_INPUT | XB_MASK_BAD_PIX_VAL;
+  if (d >= hi) m |= XB_MASK_BAD_INPUT | XB_MASK_SAT_PIXEL;
+  if (d <= lo) m |= XB_MASK_BAD_INPUT | XB_MASK_LOW_PIXEL;
+  return m;
+}
+
+__device__ __forceinline__ uint16_t after_flags(float t, double hi, double lo) {
+  double d = (double)t;
+  uint16_t m = 0;
+  if (d == 0.0) m |= XB_MASK_BAD_OUTPUT | XB_MASK_BAD_INPUT | XB_MASK_BAD_PIX_VAL;
+  if (d >= hi) m |= XB_MASK_BAD_OUTPUT | XB_MASK_BAD_INPUT | XB_MASK_SAT_PIXEL;
+  if (d <= lo) m |= XB_MASK_BAD_OUTPUT | XB_MASK_BAD_INPUT | XB_MASK_LOW_PIXEL;
+  return m;
+}
+
+constexpr int CR = 7;  // output rows per lane
+
+__global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
+  const int wave = threadIdx.x / WAVE;
+  const int lane = threadIdx.x % WAVE;
+  const int xs = blockIdx.x * 4 + wave;
+  const int ys = blockIdx.y;
+  if (xs >= A.xSteps) return;
+  const int x0 = HKW + xs * KW, y0t = HKW + ys * KW;
+  if (x0 >= A.w - HKW || y0t >= A.h - HKW) return;
+  const int tile = __builtin_amdgcn_readfirstlane(xs + ys * A.xSteps);
+  const int cx = lane % KW, rg = lane / KW;  // rg 0..2 active, 3 idle
+  const int x = x0 + cx;
+  const int y0 = y0t + rg * CR;
+  const bool laneOn = rg < 3;
+  const float* K = A.tileK + (size_t)tile * KK;
+  const int w = A.w;
+  float acc[CR];
+#pragma unroll
+  for (int r = 0; r < CR; r++) acc[r] = 0.0f;
+  unsigned rowBad = 0;  // bit q: a flagged pixel in window row q (27 rows)
+  const int xc = min(x, w - 1);
+  for (int ik = 0; ik < KW; ik++) {
+    const int xi = min(max(xc - ik + HKW, 0), w - 1);
+    float col[CR + 2 * HKW];
+#pragma unroll
+    for (int q = 0; q < CR + 2 * HKW; q++) {
+      int yy = min(y0 - HKW + q, A.h - 1);
+      float v = A.T[(size_t)xi + (size_t)yy * w];
+      col[q] = v;
+      if (conv_flags(v, A.hi, A.lo)) rowBad |= 1u << q;
+    }
+#pragma unroll
+    for (int jk = 0; jk < KW; jk++) {
+      const float kv = K[ik + jk * KW];
+#pragma unroll
+      for (int r = 0; r < CR; r++) acc[r] = __builtin_fmaf(kv, col[r - jk + 2 * HKW], acc[r]);
+    }
+  }
+  if (!laneOn || x >= w - HKW) return;
+  const double inv = A.scaleConv ? 1.0 / *A.kernSum : 1.0;
+  const double aks = A.tileAks[tile];
+  const double thr = (double)0.99f;
+  for (int r = 0; r < CR; r++) {
+    const int y = y0 + r;
+    if (y >= A.h - HKW) break;
+    const size_t id = (size_t)y * w + x;
+    double v = ((double)acc[r] + background_c(A.sol, x, y, A.w, A.h)) * inv;
+    A.out[id] = (float)v;
+    const uint16_t cm = conv_flags(A.T[id], A.hi, A.lo);
+    uint16_t nm = cm;
+    if (cm & XB_MASK_BAD_INPUT) nm |= XB_MASK_BAD_OUTPUT;
+    const bool anyBad = ((rowBad >> r) & ((1u << KW) - 1u)) != 0u;
+    if (anyBad) {
+      // exact sum |k| over unflagged pixels, reference loop order
+      const double* K64 = A.tileK64 + (size_t)tile * KK;
+      double uks = 0.0;
+      for (int j = y - HKW; j <= y + HKW; j++) {
+        int jk = y - j + HKW;
+        for (int i = x - HKW; i <= x + HKW; i++) {
+          int ik = x - i + HKW;
+          double kk = K64[ik + jk * KW];
+          if ((conv_flags(A.T[(size_t)i + (size_t)w * j], A.hi, A.lo) & XB_MASK_BAD_INPUT) == 0)
+            uks += fabs(kk);
+        }
+      }
+      if ((uks / aks) < thr) nm |= XB_MASK_BAD_OUTPUT | XB_MASK_BAD_CONV;
+      else nm |= XB_MASK_OK_CONV;
+    }
+    A.mask[id] = nm | after_flags(A.S[id], A.hi, A.lo);
+  }
+}
+
+// border pixels: conv = 1e-30, mask keeps its value plus maskAfterConv flags
+__global__ __launch_bounds__(256) void k_conv_border(const float* __restrict__ S,
+                                                     uint16_t* __restrict__ mask,
+                                                     float* __restrict__ out, int w, int h,
+                                                     double hi, double lo) {
+  const size_t n = (size_t)w * h;
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (size_t)gridDim.x * blockDim.x) {
+    int x = (int)(id % w), y = (int)(id / w);
+    bool interior = x >= HKW && x < w - HKW && y >= HKW && y < h - HKW;
+    if (interior) continue;
+    out[id] = 1e-30f;
+    mask[id] = mask[id] | after_flags(S[id], hi, lo);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sub(const float* __restrict__ S,
+                                             const float* __restrict__ I,
+                                             const uint16_t* __restrict__ mask,
+                                             float* __restrict__ D, int w, int h,
+                                             const double* kernSum, int scaleConv) {
+  const double ks = *kernSum;
+  const double convFactor = scaleConv ? ks : 1.0;
+  const double finalFactor = scaleConv ? -(1.0 / ks) : 1.0;
+  const size_t n = (size_t)w * h;
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (size_t)gridDim.x * blockDim.x) {
+    int x = (int)(id % w), y = (int)(id / w);
+    double d = 1e-30;
+    if (x >= HKW && x < w - HKW && y >= HKW && y < h - HKW)
+      if ((mask[id] & XB_MASK_BAD_OUTPUT) == 0)
+        d = ((double)I[id] * convFactor - (double)S[id]) * finalFactor;
+    D[id] = (float)d;
+  }
+}
+
+hipError_t launch_conv(Ctx& c) {
+  const int xSteps = (c.w + KW - 1) / KW, ySteps = (c.h + KW - 1) / KW;
+  const int nTiles = xSteps * ySteps;
+  const int ti = c.tIdx;
+  double* kernSum = &c.sc->kernSum;
+  hipLaunchKernelGGL(k_tile_kernels, dim3(nTiles + 1), dim3(512), 0, c.stream, c.basis, c.sol,
+                     c.w, c.h, xSteps, nTiles, c.tileK, c.tileK64, c.tileAks, kernSum);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  ConvArgs A;
+  A.T = c.img[ti];
+  A.S = c.img[1 - ti];
+  A.mask = c.mask;
+  A.out = c.conv;
+  A.tileK = c.tileK;
+  A.tileK64 = c.tileK64;
+  A.tileAks = c.tileAks;
+  A.sol = c.sol;
+  A.kernSum = kernSum;
+  A.w = c.w;
+  A.h = c.h;
+  A.xSteps = xSteps;
+  A.yTiles = ySteps;
+  A.scaleConv = (c.m.normalizeTemplate && c.convTemplate) || (!c.m.normalizeTemplate && !c.convTemplate);
+  A.hi = c.m.threshHigh;
+  A.lo = c.m.threshLow;
+  hipLaunchKernelGGL(k_conv, dim3((xSteps + 3) / 4, ySteps), dim3(256), 0, c.stream, A);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_conv_border, dim3(1024), dim3(256), 0, c.stream, c.img[1 - ti], c.mask,
+                     c.conv, c.w, c.h, c.m.threshHigh, c.m.threshLow);
+  return hipGetLastError();
+}
+
+hipError_t launch_sub(Ctx& c) {
+  const int ti = c.tIdx;
+  const int scaleConv = (c.m.normalizeTemplate && c.convTemplate) || (!c.m.normalizeTemplate && !c.convTemplate);
+  size_t n = (size_t)c.w * c.h;
+  unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_sub, dim3(blocks), dim3(256), 0, c.stream, c.img[1 - ti], c.conv, c.mask,
+                     c.diff, c.w, c.h, &c.sc->kernSum, scaleConv);
+  return hipGetLastError();
+}
+
+}  // namespace xb

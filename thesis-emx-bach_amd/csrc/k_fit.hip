@@ -1,0 +1,796 @@
+// k_fit.hip — kernel fit: per-stamp test solves, global normal equations,
+// FP64 LU, chi2 merits and the clip-refit checks (the reference's cd + ksc).
+//
+// LU note.  The reference factorises with Numerical Recipes' Crout ludcmp
+// (implicit scaling, partial pivoting; src/bachUtil.cpp:319-384 on the host,
+// cl_kern/bach.cl:68-154 per stamp).  Crout forms element (i,l) as
+// a_il - sum_k L_ik U_kl with k ascending; right-looking elimination applies
+// exactly the same updates to each element in the same ascending-k order and
+// the same pivot row swaps, so the factors are bit-identical while every step
+// is a parallel rank-1 update.  Pivot choice keeps the reference's `>=` rule
+// (the last row attaining the maximum wins; NaN never wins).
+#include "xb_dev.h"
+
+namespace xb {
+
+// ---------------------------------------------------------------------------
+// per-stamp 50x50 test system (cdkscUtil.cpp:42-72, bach.cl:68-194)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall,
+                                                 const double* __restrict__ Bcall,
+                                                 const int* nKept, double* ksum) {
+  const int s = blockIdx.x;
+  if (s >= *nKept) return;
+  const int tid = threadIdx.x;
+  __shared__ double A[QN][QN + 1];
+  __shared__ double b[QN], vv[QN];
+  __shared__ int index[QN];
+  __shared__ double rv[WAVE];
+  __shared__ int ri[WAVE];
+  const double* G = Gall + (size_t)s * WROWS * WROWS;
+  // createTestMat / createTestVec: A[i][j] = Q[max][min], Q[a][b] = G[a-1][b-1]
+  for (int e = tid; e < QN * QN; e += blockDim.x) {
+    int i = e / QN, j = e % QN;
+    double v = 0.0;
+    if (i > 0 && j > 0) {
+      int a = max(i, j), bb = min(i, j);
+      v = G[(a - 1) * WROWS + (bb - 1)];
+    }
+    A[i][j] = v;
+  }
+  for (int i = tid; i < QN; i += blockDim.x) b[i] = i > 0 ? Bcall[(size_t)s * WROWS + i - 1] : 0.0;
+  __syncthreads();
+  // ludcmpBig: vv[i] = 1/max_j |a_ij| (no zero-row exit on the device)
+  for (int i = tid; i < QN; i += blockDim.x) {
+    double v = 0.0;
+    if (i > 0) {
+      double big = 0.0;
+      for (int j = 1; j < QN; j++) {
+        double m = fabs(A[i][j]);
+        if (m > big) big = m;
+      }
+      v = 1.0 / big;
+    }
+    vv[i] = v;
+  }
+  __syncthreads();
+  const int n = QN - 1;
+  __shared__ int s_maxI;
+  if (tid == 0) s_maxI = 0;
+  __syncthreads();
+  for (int j = 1; j <= n; j++) {
+    // pivot: last i in [j, n] with vv[i]*|a_ij| equal to the running max
+    double bv = -1.0;
+    int bi = -1;
+    for (int i = j + tid; i <= n; i += blockDim.x) {
+      double dum = vv[i] * fabs(A[i][j]);
+      if (dum >= 0.0 && (dum > bv || dum == bv)) { bv = dum; bi = i; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      double ov = __shfl_xor(bv, o);
+      int oi = __shfl_xor(bi, o);
+      if (ov > bv || (ov == bv && oi > bi)) { bv = ov; bi = oi; }
+    }
+    if (tid == 0 && bi >= 0) s_maxI = bi;
+    __syncthreads();
+    const int maxI = s_maxI;
+    if (j != maxI) {
+      for (int k = 1 + tid; k <= n; k += blockDim.x) {
+        double t = A[maxI][k];
+        A[maxI][k] = A[j][k];
+        A[j][k] = t;
+      }
+      if (tid == 0) vv[maxI] = vv[j];
+    }
+    if (tid == 0) {
+      index[j] = maxI;
+      if (A[j][j] == 0.0) A[j][j] = 1.0e-20;
+    }
+    __syncthreads();
+    if (j != n) {
+      const double dum = 1.0 / A[j][j];
+      for (int i = j + 1 + tid; i <= n; i += blockDim.x) A[i][j] *= dum;
+      __syncthreads();
+      const int m = n - j;
+      for (int e = tid; e < m * m; e += blockDim.x) {
+        int i = j + 1 + e / m, l = j + 1 + e % m;
+        A[i][l] = A[i][l] - A[i][j] * A[j][l];
+      }
+    }
+    __syncthreads();
+  }
+  // lubksb (bach.cl:156-194), sequential on one lane (50 unknowns)
+  if (tid == 0) {
+    int ii = 0;
+    for (int i = 1; i <= n; i++) {
+      int ip = index[i];
+      double sum = b[ip];
+      b[ip] = b[i];
+      if (ii != 0) {
+        for (int jj = ii; jj <= i - 1; jj++) sum -= A[i][jj] * b[jj];
+      } else if (sum != 0.0) {
+        ii = i;
+      }
+      b[i] = sum;
+    }
+    for (int i = n; i >= 1; i--) {
+      double sum = b[i];
+      for (int jj = i + 1; jj <= n; jj++) sum -= A[i][jj] * b[jj];
+      b[i] = sum / A[i][i];
+    }
+    ksum[s] = b[1];
+  }
+  (void)rv;
+  (void)ri;
+}
+
+// ---------------------------------------------------------------------------
+// cd: kernel-sum clip and test-stamp selection (cdkscUtil.cpp:74-103)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_cd_select(const double* ksum, const int* nKeptP,
+                                                    int* list, DevScalars* sc, int im,
+                                                    double alpha, double sigKernFit) {
+  __shared__ double data[XB_MAX_STAMPS];
+  __shared__ uint8_t clip[XB_MAX_STAMPS];
+  __shared__ double part[64];
+  __shared__ double res[2];
+  __shared__ int isc[20];
+  __shared__ int s_n;
+  const int n = *nKeptP;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) data[i] = ksum[i];
+  __syncthreads();
+  sigma_clip_small(data, n, 10, alpha, clip, part, res, isc);
+  const double mean = res[0], sd = res[1];
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += blockDim.x) {
+    int s = base + threadIdx.x;
+    bool keep = false;
+    if (s < n) {
+      double diff = fabs((data[s] - mean) / sd);
+      keep = diff < sigKernFit;
+    }
+    int tot;
+    int pos = block_prefix_count(keep, isc, &tot);
+    int c0 = s_n;
+    if (keep) list[c0 + pos] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) s_n = c0 + tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sc->nTest[im] = s_n;
+    sc->nList = s_n;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// global normal matrix + right-hand side
+//   cd form:  cl_kern/cd.cl:152-352 over the test stamps
+//   ksc form: src/cdkscUtil.cpp:307-475 over all stamps with a substamp left
+// The per-element sums (over stamps, in order) are the same expressions in
+// both forms; only the background right-hand side order differs (row-major in
+// the cd form, column-major in the ksc form).
+// ---------------------------------------------------------------------------
+struct BuildArgs {
+  const double* G;
+  const double* Bc;
+  const double* Br;
+  const int* kept;
+  const int* cur;
+  const int* count;
+  const int* ssx;
+  const int* ssy;
+  const int* list;   // cd: test list; ksc: nullptr (all kept)
+  const int* nList;  // count of the list / kept
+  int w, h;
+  int cdForm;
+  double* M;
+  double* rhs;
+};
+
+__device__ __forceinline__ void stamp_weights(int w, int h, int x, int y, double* wt) {
+  // xf, yf in FP32 (cd.cl:170-171, cdkscUtil.cpp:317-332); monomials in the
+  // order of the kernelXy list (bach.cpp:278-282)
+  float hw = (float)(0.5 * w), hh = (float)(0.5 * h);
+  double fx = (double)(((float)x - hw) / hw);
+  double fy = (double)(((float)y - hh) / hh);
+  double fx2 = fx * fx, fy2 = fy * fy;
+  wt[0] = 1.0 * 1.0;
+  wt[1] = 1.0 * fy;
+  wt[2] = 1.0 * fy2;
+  wt[3] = fx * 1.0;
+  wt[4] = fx * fy;
+  wt[5] = fx2 * 1.0;
+}
+
+constexpr int BUILD_STAMPS_LDS = 256;
+
+__global__ __launch_bounds__(256) void k_global_build(BuildArgs A) {
+  const int n = *A.nList;
+  const int nLower = MATN * (MATN + 1) / 2;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double wts[BUILD_STAMPS_LDS][NC2];
+  __shared__ int sidx[BUILD_STAMPS_LDS];
+  __shared__ int svalid[BUILD_STAMPS_LDS];
+  // element coordinates (r >= c) of the lower triangle, or rhs entries after
+  int r = -1, c = -1, rr = -1;
+  if (e < nLower) {
+    // r(r+1)/2 <= e
+    int t = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while ((t + 1) * (t + 2) / 2 <= e) t++;
+    while (t * (t + 1) / 2 > e) t--;
+    r = t;
+    c = e - t * (t + 1) / 2;
+  } else if (e < nLower + NSOL) {
+    rr = e - nLower;
+  }
+  double acc = 0.0;
+  for (int s0 = 0; s0 < n; s0 += BUILD_STAMPS_LDS) {
+    const int cn = min(BUILD_STAMPS_LDS, n - s0);
+    __syncthreads();
+    for (int u = threadIdx.x; u < cn; u += blockDim.x) {
+      int s = A.list ? A.list[s0 + u] : (s0 + u);
+      int gk = A.kept[s];
+      int cu = A.cur[s];
+      bool valid = cu < A.count[gk];
+      sidx[u] = s;
+      svalid[u] = valid ? 1 : 0;
+      double wt[NC2];
+      if (valid) stamp_weights(A.w, A.h, A.ssx[gk * MAXSS + cu], A.ssy[gk * MAXSS + cu], wt);
+      else
+        for (int q = 0; q < NC2; q++) wt[q] = 0.0;
+      for (int q = 0; q < NC2; q++) wts[u][q] = wt[q];
+    }
+    __syncthreads();
+    for (int u = 0; u < cn; u++) {
+      if (!svalid[u]) continue;  // exhausted stamps add nothing (zeros)
+      const int s = sidx[u];
+      const double* G = A.G + (size_t)s * WROWS * WROWS;
+      const double* wt = wts[u];
+      if (r >= 0) {
+        if (r == 0 || c == 0) {
+        } else if (r == 1 && c == 1) {
+          acc += G[0];
+        } else if (r <= NCOMP + 1) {
+          // basis rows 2..289
+          int i = r - 2, i1 = i / NC2, i2 = i - i1 * NC2;
+          if (c == 1) {
+            acc += wt[i2] * G[(i1 + 1) * WROWS + 0];
+          } else {
+            int j = c - 2, j1 = j / NC2, j2 = j - j1 * NC2;
+            acc += wt[i2] * wt[j2] * G[(i1 + 1) * WROWS + (j1 + 1)];
+          }
+        } else {
+          // background rows 290..292: W row iv = 49 + iBG
+          int iv = NC1 + (r - (NCOMP + 2)) + 1;
+          if (c == 1) {
+            acc += G[iv * WROWS + 0];
+          } else if (c <= NCOMP + 1) {
+            int jj = c - 1;
+            int i1 = (jj - 1) / NC2 + 1, i2 = (jj - 1) % NC2;
+            acc += G[iv * WROWS + i1] * wt[i2];
+          } else {
+            int jv = NC1 + (c - (NCOMP + 2)) + 1;
+            acc += G[iv * WROWS + jv];
+          }
+        }
+      } else if (rr >= 0) {
+        const double* Bc = A.Bc + (size_t)s * WROWS;
+        if (rr == 1) {
+          acc += Bc[0];
+        } else if (rr >= 2 && rr <= NCOMP + 1) {
+          int i = (rr - 2) / NC2 + 1, j = (rr - 2) % NC2;
+          acc += Bc[i] * wt[j];
+        } else if (rr >= NCOMP + 2 && rr < NCOMP + 2 + NBG) {
+          int b = rr - (NCOMP + 2);
+          acc += A.cdForm ? A.Br[(size_t)s * NBG + b] : Bc[NPSF + b];
+        }
+      }
+    }
+  }
+  if (r >= 0) {
+    A.M[(size_t)r * MATN + c] = acc;
+    A.M[(size_t)c * MATN + r] = acc;
+  } else if (rr >= 0) {
+    A.rhs[rr] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Global FP64 LU (NR ludcmp semantics incl. the zero-row early return,
+// src/bachUtil.cpp:319-384) on the 292x292 system stored 293x293 row-major.
+// One 1024-thread workgroup; right-looking steps (see the note on top).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_global_lu(double* M, int* index, double* vvOut,
+                                                    int* status) {
+  const int n = MATSIZE, N = MATN;
+  const int tid = threadIdx.x;
+  __shared__ double vv[MATN];
+  __shared__ double colL[MATN], rowU[MATN];
+  __shared__ double wv[16];
+  __shared__ int wi[16];
+  __shared__ int s_zero, s_maxI;
+  if (tid == 0) { s_zero = 0; s_maxI = 0; }
+  for (int i = tid; i < N; i += blockDim.x) index[i] = 0;
+  __syncthreads();
+  for (int i = 1 + tid; i <= n; i += blockDim.x) {
+    double big = 0.0;
+    for (int j = 1; j <= n; j++) {
+      double t = fabs(M[(size_t)i * N + j]);
+      if (t > big) big = t;
+    }
+    if (big == 0.0) atomicOr(&s_zero, 1);
+    vv[i] = 1.0 / big;
+  }
+  __syncthreads();
+  if (s_zero) {
+    if (tid == 0) *status = 1;
+    return;  // ludcmp returns 1 before factorising
+  }
+  const int lane = lane_id(), wid = tid / WAVE, nw = blockDim.x / WAVE;
+  for (int j = 1; j <= n; j++) {
+    double bv = -1.0;
+    int bi = -1;
+    for (int i = j + tid; i <= n; i += blockDim.x) {
+      double dum = vv[i] * fabs(M[(size_t)i * N + j]);
+      if (dum >= 0.0 && dum >= bv) { bv = dum; bi = i; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      double ov = __shfl_xor(bv, o);
+      int oi = __shfl_xor(bi, o);
+      if (ov > bv || (ov == bv && oi > bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { wv[wid] = bv; wi[wid] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double b2 = wv[0];
+      int i2 = wi[0];
+      for (int q = 1; q < nw; q++)
+        if (wv[q] > b2 || (wv[q] == b2 && wi[q] > i2)) { b2 = wv[q]; i2 = wi[q]; }
+      if (i2 >= 0) s_maxI = i2;
+    }
+    __syncthreads();
+    const int maxI = s_maxI;
+    if (j != maxI) {
+      for (int k = 1 + tid; k <= n; k += blockDim.x) {
+        double t = M[(size_t)maxI * N + k];
+        M[(size_t)maxI * N + k] = M[(size_t)j * N + k];
+        M[(size_t)j * N + k] = t;
+      }
+      if (tid == 0) vv[maxI] = vv[j];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      index[j] = maxI;
+      if (M[(size_t)j * N + j] == 0.0) M[(size_t)j * N + j] = 1.0e-20;
+    }
+    __syncthreads();
+    if (j != n) {
+      const double dum = 1.0 / M[(size_t)j * N + j];
+      for (int i = j + 1 + tid; i <= n; i += blockDim.x) {
+        double v = M[(size_t)i * N + j] * dum;
+        M[(size_t)i * N + j] = v;
+        colL[i] = v;
+      }
+      for (int l = j + 1 + tid; l <= n; l += blockDim.x) rowU[l] = M[(size_t)j * N + l];
+      __syncthreads();
+      const int m = n - j;
+      for (int e = tid; e < m * m; e += blockDim.x) {
+        int i = j + 1 + e / m, l = j + 1 + e % m;
+        size_t a = (size_t)i * N + l;
+        M[a] = M[a] - colL[i] * rowU[l];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *status = 0;
+  (void)vvOut;
+}
+
+// lubksb (src/bachUtil.cpp:386-411) on one lane: the ascending-j sums of both
+// substitutions are sequential chains in the reference order.
+__global__ __launch_bounds__(64) void k_global_trsv(const double* M, const int* index,
+                                                    double* b, double* out) {
+  if (threadIdx.x != 0) return;
+  const int n = MATSIZE, N = MATN;
+  int ii = 0;
+  for (int i = 1; i <= n; i++) {
+    int ip = index[i];
+    double sum = b[ip];
+    b[ip] = b[i];
+    if (ii) {
+      const double* row = M + (size_t)i * N;
+      for (int j = ii; j <= i - 1; j++) sum -= row[j] * b[j];
+    } else if (sum != 0.0) {
+      ii = i;
+    }
+    b[i] = sum;
+  }
+  for (int i = n; i >= 1; i--) {
+    const double* row = M + (size_t)i * N;
+    double sum = b[i];
+    for (int j = i + 1; j <= n; j++) sum -= row[j] * b[j];
+    b[i] = sum / row[i];
+  }
+  if (out)
+    for (int i = 0; i < NSOL; i++) out[i] = b[i];
+}
+
+// ---------------------------------------------------------------------------
+// kernel coefficients at (xf, yf) (cd.cl:354-383, bachUtil.cpp:508-526)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double kernel_coeff(const double* sol, int i, double xf, double yf) {
+  if (i == 0) return sol[1];
+  int k = 2 + (i - 1) * NC2;
+  double c0 = 0.0, aX = 1.0;
+  for (int x = 0; x <= 2; x++) {
+    double aY = 1.0;
+    for (int y = 0; y <= 2 - x; y++) {
+      c0 += sol[k++] * aX * aY;
+      aY *= yf;
+    }
+    aX *= xf;
+  }
+  return c0;
+}
+
+// getBackground (cd.cl:444-467)
+__device__ __forceinline__ double background_at(const double* sol, int x, int y, int w, int h) {
+  double xf = (x - 0.5 * w) / (0.5 * w);
+  double yf = (y - 0.5 * h) / (0.5 * h);
+  double bg = 0.0, ax = 1.0;
+  int k = NCOMP + 2;
+  for (int i = 0; i <= 1; i++) {
+    double ay = 1.0;
+    for (int j = 0; j <= 1 - i; j++) {
+      bg += sol[k++] * ax * ay;
+      ay *= yf;
+    }
+    ax *= xf;
+  }
+  return bg;
+}
+
+// ---------------------------------------------------------------------------
+// calcSigs (cdkscUtil.cpp:477-583, cd.cl:469-655)
+// ---------------------------------------------------------------------------
+struct SigArgs {
+  const float* tImg;
+  const float* sImg;
+  uint16_t* mask;
+  int w, h;
+  const double* W;
+  const int* kept;
+  const int* cur;
+  const int* count;
+  const int* ssx;
+  const int* ssy;
+  const int* list;   // nullptr: all kept stamps
+  const int* nList;
+  const Basis* basis;
+  const double* sol;
+  double* sigOut;    // indexed by list position
+};
+
+__global__ __launch_bounds__(256) void k_calc_sigs(SigArgs A) {
+  const int u = blockIdx.x;
+  if (u >= *A.nList) return;
+  const int tid = threadIdx.x;
+  const int s = A.list ? A.list[u] : u;
+  const int gk = A.kept[s];
+  const int cu = A.cur[s];
+  const bool valid = cu < A.count[gk];
+  __shared__ double co[NPSF];
+  __shared__ double sv[32 * 31 + 32];
+  __shared__ int cv[32 * 31 + 32];
+  __shared__ double part[32];
+  __shared__ int pcnt[32];
+  __shared__ double s_bg;
+  int ssx = 0, ssy = 0;
+  if (valid) {
+    ssx = A.ssx[gk * MAXSS + cu];
+    ssy = A.ssy[gk * MAXSS + cu];
+  }
+  const double* sol = A.sol;
+  if (valid) {
+    double xf = (ssx - A.w * 0.5) / (A.w * 0.5);
+    double yf = (ssy - A.h * 0.5) / (A.h * 0.5);
+    if (tid < NPSF) co[tid] = kernel_coeff(sol, tid, xf, yf);
+    if (tid == 64) s_bg = background_at(sol, ssx, ssy, A.w, A.h);
+  } else if (tid == 64) {
+    s_bg = 0.0;
+  }
+  __syncthreads();
+  const double* W = A.W + (size_t)s * WROWS * NPIX;
+  const int* evenSub = A.basis->evenSub;
+  const int total = 32 * 31;  // reduceCount * localSize, cdkscUtil.cpp:484-541
+  for (int gi = tid; gi < total; gi += blockDim.x) {
+    const int gx = gi % SSW, gy = gi / SSW;
+    double s0 = 0.0;
+    int cnt = 0;
+    if (valid && gy < SSW) {
+      // makeModel (cd.cl:489-531): FP32 accumulator, FP64 terms
+      float m0 = (float)(sol[1] * W[gi]);
+      for (int i = 1; i < NPSF; i++) {
+        double wv = W[(size_t)i * NPIX + gi];
+        if (evenSub[i]) wv = wv - W[gi];
+        m0 = (float)((double)m0 + co[i] * wv);
+      }
+      const int absX = min(max(gx - SSW / 2 + ssx, 0), A.w - 1);
+      const int absY = min(max(gy - SSW / 2 + ssy, 0), A.h - 1);
+      const size_t ai = (size_t)absX + (size_t)absY * A.w;
+      const double tDat = (double)m0;
+      const double sDat = (double)A.sImg[ai];
+      const double diff = tDat - sDat + s_bg;
+      if ((A.mask[ai] & XB_MASK_BAD_INPUT) == 0 && fabs(sDat) > 1e-10) {
+        if (isnan(tDat) || isnan(sDat)) {
+          mask_or_atomic(A.mask, ai, XB_MASK_NAN_PIXEL);
+        } else {
+          cnt = 1;
+          s0 = diff * diff / (fabs((double)A.tImg[ai]) + fabs(sDat));
+        }
+      }
+    }
+    sv[gi] = s0;
+    cv[gi] = cnt;
+  }
+  __syncthreads();
+  if (tid < 31) {
+    double ls = 0.0;
+    int lc = 0;
+    for (int li = 0; li < 32; li++) {
+      ls += sv[tid * 32 + li];
+      lc += cv[tid * 32 + li];
+    }
+    part[tid] = ls;
+    pcnt[tid] = lc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sum = 0.0;
+    int lc = 0;
+    for (int g = 0; g < 31; g++) sum += part[g];
+    for (int g = 0; g < 31; g++) lc += pcnt[g];
+    if (lc == 0) {
+      sum = -1.0;
+    } else {
+      sum /= lc;
+      if (sum >= 1e10) sum = -1.0;
+    }
+    A.sigOut[u] = sum;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// cd: kernel sum at (0, 0) (cd.cl:385-442) and the merit (cdkscUtil.cpp:221-266)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_cd_finish(const Basis* basis, const double* sol,
+                                                    const double* merits, DevScalars* sc, int im,
+                                                    int w, int h, double alpha) {
+  __shared__ double co[NPSF];
+  __shared__ double kern[KK];
+  __shared__ double part[64];
+  __shared__ double data[XB_MAX_STAMPS];
+  __shared__ uint8_t clip[XB_MAX_STAMPS];
+  __shared__ double res[2];
+  __shared__ int isc[20];
+  __shared__ int s_n;
+  const int tid = threadIdx.x;
+  const int nTest = sc->nTest[im];
+  if (nTest == 0) {
+    if (tid == 0) { sc->merit[im] = 666; sc->kernelSumCd[im] = 0.0; }
+    return;
+  }
+  const double hw = 0.5 * w, hh = 0.5 * h;
+  const double xf = (0 - hw) / hw, yf = (0 - hh) / hh;
+  if (tid < NPSF) co[tid] = kernel_coeff(sol, tid, xf, yf);
+  __syncthreads();
+  if (tid < KK) {
+    double k0 = 0.0;
+    for (int p = 0; p < NPSF; p++) k0 += co[p] * basis->vec[p * KK + tid];
+    kern[tid] = k0;
+  }
+  __syncthreads();
+  const int nb = (KK + 31) / 32;
+  if (tid < nb) {
+    double sm = 0.0;
+    int lc = min(32, KK - tid * 32);
+    for (int i = 0; i < lc; i++) sm += kern[tid * 32 + i];
+    part[tid] = sm;
+  }
+  __syncthreads();
+  __shared__ double s_ksum;
+  if (tid == 0) {
+    double sm = 0.0;
+    for (int g = 0; g < nb; g++) sm += part[g];
+    s_ksum = sm;
+    sc->kernelSumCd[im] = sm;
+  }
+  // clean merits (removeBadSigs, cd.cl:657-695) in canonical order
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  for (int base = 0; base < nTest; base += blockDim.x) {
+    int i = base + tid;
+    bool keep = i < nTest && merits[i] >= 0.0;
+    int tot;
+    int pos = block_prefix_count(keep, isc, &tot);
+    int c0 = s_n;
+    if (keep) data[c0 + pos] = merits[i];
+    __syncthreads();
+    if (tid == 0) s_n = c0 + tot;
+    __syncthreads();
+  }
+  const int n = s_n;
+  if (n == 0) {
+    if (tid == 0) sc->merit[im] = 666;
+    return;
+  }
+  sigma_clip_small(data, n, 10, alpha, clip, part, res, isc);
+  if (tid == 0) sc->merit[im] = res[0] / s_ksum;
+}
+
+// ---------------------------------------------------------------------------
+// ksc: checkBadSubStamps + sigma clip + checkBadSubStampsFromSigmaClip
+// (ksc.cl:1-72, cdkscUtil.cpp:686-802).  Both rejection passes use the same
+// chi2 values; stamps rejected by either get their next substamp (refilled
+// once with the final index, which equals the reference's two refills).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int* kept,
+                                                    int* cur, const int* count, int* list,
+                                                    DevScalars* sc, int im, double alpha,
+                                                    double sigKernFit) {
+  __shared__ double data[XB_MAX_STAMPS];
+  __shared__ uint8_t clip[XB_MAX_STAMPS];
+  __shared__ uint8_t refill[XB_MAX_STAMPS];
+  __shared__ double part[64];
+  __shared__ double res[2];
+  __shared__ int isc[20];
+  __shared__ int s_n;
+  const int tid = threadIdx.x;
+  const int n = sc->nKept[im];
+  if (tid == 0) s_n = 0;
+  for (int s = tid; s < n; s += blockDim.x) refill[s] = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += blockDim.x) {
+    int s = base + tid;
+    bool isChi = false;
+    double v = 0.0;
+    if (s < n && cur[s] < count[kept[s]]) {
+      v = sig[s];
+      if (v == -1.0) {
+        cur[s] = cur[s] + 1;
+        refill[s] = 1;
+      } else {
+        isChi = true;
+      }
+    }
+    int tot;
+    int pos = block_prefix_count(isChi, isc, &tot);
+    int c0 = s_n;
+    if (isChi) data[c0 + pos] = v;
+    __syncthreads();
+    if (tid == 0) s_n = c0 + tot;
+    __syncthreads();
+  }
+  const int nChi = s_n;
+  sigma_clip_small(data, nChi, 10, alpha, clip, part, res, isc);
+  const double mean = res[0], sd = res[1];
+  for (int s = tid; s < n; s += blockDim.x) {
+    if (cur[s] < count[kept[s]]) {
+      if ((sig[s] - mean) > sigKernFit * sd) {
+        cur[s] = cur[s] + 1;
+        refill[s] = 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += blockDim.x) {
+    int s = base + tid;
+    bool r = s < n && refill[s];
+    int tot;
+    int pos = block_prefix_count(r, isc, &tot);
+    int c0 = s_n;
+    if (r) list[c0 + pos] = s;
+    __syncthreads();
+    if (tid == 0) s_n = c0 + tot;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    sc->nList = s_n;
+    sc->check = s_n > 0 ? 1 : 0;
+    sc->nChi2 = nChi;
+    sc->clipMean = mean;
+    sc->clipStd = sd;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_stamp_lu(Ctx& c, int im) {
+  if (c.nKept[im] <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stamp_lu, dim3(c.nKept[im]), dim3(64), 0, c.stream, c.st[im].G,
+                     c.st[im].Bc, &c.sc->nKept[im], c.st[im].ksum);
+  return hipGetLastError();
+}
+
+hipError_t launch_cd_select(Ctx& c, int im) {
+  hipLaunchKernelGGL(k_cd_select, dim3(1), dim3(1024), 0, c.stream, c.st[im].ksum,
+                     &c.sc->nKept[im], c.st[im].list, c.sc, im, c.m.sigClipAlpha,
+                     c.m.sigKernFit);
+  return hipGetLastError();
+}
+
+hipError_t launch_global_build(Ctx& c, int im, bool cdForm) {
+  BuildArgs A;
+  A.G = c.st[im].G;
+  A.Bc = c.st[im].Bc;
+  A.Br = c.st[im].Br;
+  A.kept = c.st[im].kept;
+  A.cur = c.st[im].cur;
+  A.count = c.st[im].count;
+  A.ssx = c.st[im].ssx;
+  A.ssy = c.st[im].ssy;
+  A.list = cdForm ? c.st[im].list : nullptr;
+  A.nList = cdForm ? &c.sc->nList : &c.sc->nKept[im];
+  A.w = c.w;
+  A.h = c.h;
+  A.cdForm = cdForm ? 1 : 0;
+  A.M = c.M;
+  A.rhs = c.rhs;
+  const int total = MATN * (MATN + 1) / 2 + NSOL;
+  hipLaunchKernelGGL(k_global_build, dim3((total + 255) / 256), dim3(256), 0, c.stream, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_global_solve(Ctx& c, double* d_out) {
+  hipLaunchKernelGGL(k_global_lu, dim3(1), dim3(1024), 0, c.stream, c.M, c.luIndex, c.luVV,
+                     &c.sc->status);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_global_trsv, dim3(1), dim3(64), 0, c.stream, c.M, c.luIndex, c.rhs, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol) {
+  SigArgs A;
+  const int src = im;
+  A.tImg = c.img[src];
+  A.sImg = c.img[1 - src];
+  A.mask = c.mask;
+  A.w = c.w;
+  A.h = c.h;
+  A.W = c.st[im].W;
+  A.kept = c.st[im].kept;
+  A.cur = c.st[im].cur;
+  A.count = c.st[im].count;
+  A.ssx = c.st[im].ssx;
+  A.ssy = c.st[im].ssy;
+  A.list = useList ? c.st[im].list : nullptr;
+  A.nList = useList ? &c.sc->nList : &c.sc->nKept[im];
+  A.basis = c.basis;
+  A.sol = d_sol;
+  A.sigOut = c.st[im].sig;
+  if (c.nKept[im] <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(256), 0, c.stream, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_cd_finish(Ctx& c, int im) {
+  hipLaunchKernelGGL(k_cd_finish, dim3(1), dim3(1024), 0, c.stream, c.basis,
+                     c.cdSol + (size_t)im * NSOL, c.st[im].sig, c.sc, im, c.w, c.h,
+                     c.m.sigClipAlpha);
+  return hipGetLastError();
+}
+
+hipError_t launch_ksc_check(Ctx& c, int im) {
+  hipLaunchKernelGGL(k_ksc_check, dim3(1), dim3(1024), 0, c.stream, c.st[im].sig,
+                     c.st[im].kept, c.st[im].cur, c.st[im].count, c.st[im].list, c.sc, im,
+                     c.m.sigClipAlpha, c.m.sigKernFit);
+  return hipGetLastError();
+}
+
+}  // namespace xb

@@ -1,0 +1,635 @@
+// k_sss.hip — stamp statistics and substamp selection (the reference's sss).
+//
+// One 256-thread workgroup per stamp.  The reference runs these steps as one
+// serial work-item per stamp (cl_kern/sss.cl:81-505) with ~1600 host round
+// trips per image for the per-stamp sigma clip (src/bachUtil.cpp:255-266).
+// Here each stamp's whole chain runs inside its workgroup:
+//   * ran1 draws are identical for every stamp (seed -666 reset per stamp,
+//     sss.cl:92-95), so they come from a table; candidate draws are tested in
+//     parallel and the first 100 valid ones are kept in draw order;
+//   * good pixels are gathered in raster order (the canonical order of the
+//     reference's atomic_inc list, sss.cl:184) with a block prefix count;
+//   * the sigma clip keeps the reference's 32-element block sums and their
+//     sequential sum, so mean/stddev are bit-identical;
+//   * histogram bins are integer counts (order-free, LDS atomics); the mode
+//     and quartile walk is the reference's serial code on one thread;
+//   * findSubStamps keeps the sequential scan: pixels are classified in
+//     256-pixel raster chunks and candidates are then processed one by one in
+//     raster order, each with a parallel window search and check.
+#include "xb_dev.h"
+
+namespace xb {
+
+struct StampGeom {
+  int ox, oy, sw, sh;
+};
+
+__device__ __forceinline__ StampGeom stamp_geom(int s, int nx, int ny, int w, int h, int fsw) {
+  StampGeom g;
+  int sx = s % nx, sy = s / nx;
+  g.ox = sx * w / nx;
+  g.oy = sy * h / ny;
+  g.sw = min(g.ox + fsw, w) - g.ox;
+  g.sh = min(g.oy + fsw, h) - g.oy;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// calcStats (src/bachUtil.cpp:130-280)
+// ---------------------------------------------------------------------------
+constexpr int MAX_CLIP_BLOCKS = 2560;  // ceil(fsw^2 / 32) for fsw <= 286
+constexpr int MAX_LEVELS = 32;         // findSubStamps threshold levels
+
+__global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img,
+                                                   uint16_t* __restrict__ mask, int w, int h,
+                                                   int nx, int ny, int fsw,
+                                                   const double* __restrict__ ran1,
+                                                   float* __restrict__ goodAll, double alpha,
+                                                   double iqRange, double* __restrict__ skyOut,
+                                                   double* __restrict__ fwhmOut) {
+  const int s = blockIdx.x;
+  const int tid = threadIdx.x;
+  const StampGeom g = stamp_geom(s, nx, ny, w, h, fsw);
+  __shared__ double smp[NSAMP_PAD];
+  __shared__ int isc[20];
+  __shared__ int s_cnt, s_nGood;
+  __shared__ double bs[MAX_CLIP_BLOCKS], bs2[MAX_CLIP_BLOCKS];
+  __shared__ double s_mean, s_sd, s_inv, s_sum, s_sum2;
+  __shared__ long long s_cur, s_prev;
+  __shared__ int bins[256];
+
+  // ---- sampleStamp (sss.cl:81-116) + pad (sss.cl:118-125)
+  for (int i = tid; i < NSAMP_PAD; i += blockDim.x) smp[i] = i < NSAMP ? 0.0 : INFINITY;
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
+  const int npix = g.sw * g.sh;
+  for (int base = 0; base < npix; base += blockDim.x) {
+    if (s_cnt >= NSAMP) break;
+    int t = base + tid;
+    bool valid = false;
+    double v = 0.0;
+    if (t < npix) {
+      int rx = (int)floor(ran1[2 * t] * (int)g.sw);
+      int ry = (int)floor(ran1[2 * t + 1] * (int)g.sh);
+      size_t idx = (size_t)(rx + g.ox) + (size_t)(ry + g.oy) * w;
+      v = (double)img[idx];
+      valid = !(mask[idx] > 0 || fabs(v) <= 1e-10);
+    }
+    int tot;
+    int pos = block_prefix_count(valid, isc, &tot);
+    int c0 = s_cnt;
+    if (valid && c0 + pos < NSAMP) smp[c0 + pos] = v;
+    __syncthreads();
+    if (tid == 0) s_cnt = min(NSAMP, c0 + tot);
+    __syncthreads();
+  }
+  const int sampleCount = s_cnt;
+  // ---- bitonic sort network (sss.cl:135-150, bachUtil.cpp:229-240)
+  for (int k = 2; k <= NSAMP_PAD; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (tid < NSAMP_PAD) {
+        int i = tid, ixj = i ^ j;
+        if (ixj > i) {
+          double a = smp[i], b = smp[ixj];
+          if ((i & k) == 0 && a > b) { smp[i] = b; smp[ixj] = a; }
+          if ((i & k) != 0 && a < b) { smp[i] = b; smp[ixj] = a; }
+        }
+      }
+      __syncthreads();
+    }
+
+  // ---- maskStamp (sss.cl:157-188): good pixels in raster order
+  float* good = goodAll + (size_t)s * fsw * fsw;
+  if (tid == 0) s_nGood = 0;
+  __syncthreads();
+  const int nAll = fsw * fsw;
+  for (int base = 0; base < nAll; base += blockDim.x) {
+    int k = base + tid;
+    bool ok = false;
+    float pv = 0.0f;
+    if (k < nAll) {
+      int x = k % fsw, y = k / fsw;
+      int xI = x + g.ox, yI = y + g.oy;
+      if (xI < w && yI < h) {
+        size_t idx = (size_t)xI + (size_t)yI * w;
+        pv = img[idx];
+        double pixel = (double)pv;
+        uint16_t m = mask[idx];
+        if (!(m > 0 || pixel <= 1e-10)) {
+          if (isnan(pixel)) mask[idx] = m | (XB_MASK_NAN_PIXEL | XB_MASK_BAD_INPUT);
+          else ok = true;
+        }
+      }
+    }
+    int tot;
+    int pos = block_prefix_count(ok, isc, &tot);
+    int c0 = s_nGood;
+    if (ok) good[c0 + pos] = pv;
+    __syncthreads();
+    if (tid == 0) s_nGood = c0 + tot;
+    __syncthreads();
+  }
+  const int n = s_nGood;
+
+  // ---- sigmaClip(maxIter 3) (bachUtil.cpp:34-128, bach.cl:196-251).
+  // Clipped entries are marked by negating the (positive) stored value.
+  if (tid == 0) {
+    s_cur = 0;
+    s_prev = n;
+    s_mean = 0.0;
+    s_sd = 1e10;
+  }
+  __syncthreads();
+  if (n > 0) {
+    const int nb = (n + 31) / 32;
+    for (int it = 0; it < 3; it++) {
+      if (s_cur == s_prev) break;
+      if (s_prev <= 1) {
+        if (tid == 0) { s_mean = 0.0; s_sd = 1e10; }
+        __syncthreads();
+        break;
+      }
+      for (int b = tid; b < nb; b += blockDim.x) {
+        double sm = 0.0, sm2 = 0.0;
+        int cnt = min(32, n - b * 32);
+        for (int i = 0; i < cnt; i++) {
+          float v = good[b * 32 + i];
+          double d = v > 0.0f ? (double)v : 0.0;
+          sm += d;
+          sm2 += d * d;
+        }
+        bs[b] = sm;
+        bs2[b] = sm2;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double sum = 0.0;
+        for (int b = 0; b < nb; b++) sum += bs[b];
+        s_sum = sum;
+      } else if (tid == 64) {
+        double sum2 = 0.0;
+        for (int b = 0; b < nb; b++) sum2 += bs2[b];
+        s_sum2 = sum2;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        s_cur = s_prev;
+        double np = (double)s_prev;
+        double m = s_sum / np;
+        double sd = sqrt((s_sum2 - np * m * m) / (double)(s_prev - 1));
+        s_mean = m;
+        s_sd = sd;
+        s_inv = 1.0 / sd;
+      }
+      __syncthreads();
+      const double m = s_mean, inv = s_inv;
+      int c = 0;
+      for (int k = tid; k < n; k += blockDim.x) {
+        float v = good[k];
+        if (v > 0.0f && fabs((double)v - m) * inv > alpha) {
+          good[k] = -v;
+          c++;
+        }
+      }
+      int tot = block_sum_int(c, isc);
+      if (tid == 0) s_prev = s_cur - tot;
+      __syncthreads();
+    }
+  }
+  const double mean = s_mean;
+  const double invStdDev = 1.0 / s_sd;
+
+  // ---- createHistogram (sss.cl:190-324).  The good list is exactly the
+  // histogram's pixel set (mask == 0 and value > 1e-10 over the stamp).
+  __shared__ double h_binSize, h_lowerBinVal, h_lower, h_upper, h_sky;
+  __shared__ int h_state;  // 0 continue, 1 done (setFwhm), 2 done (no fwhm)
+  if (tid == 0) {
+    double up = smp[(int)(0.9 * sampleCount)];
+    double mid = smp[(int)(0.5 * sampleCount)];
+    double binSize = (up - mid) / (double)NSAMP;
+    h_binSize = binSize;
+    h_lowerBinVal = mid - (128.0 * binSize);
+    h_lower = h_upper = h_sky = 0.0;
+    h_state = 0;
+  }
+  __syncthreads();
+  for (int attempts = 0;; attempts++) {
+    if (attempts >= 5) {
+      if (tid == 0) h_state = 2;
+      __syncthreads();
+      break;
+    }
+    for (int i = tid; i < 256; i += blockDim.x) bins[i] = 0;
+    __syncthreads();
+    const double lbv = h_lowerBinVal, bsz = h_binSize;
+    int ok = 0;
+    for (int k = tid; k < n; k += blockDim.x) {
+      double v = (double)fabsf(good[k]);
+      if (fabs(v - mean) * invStdDev > alpha) continue;
+      double q = floor((v - lbv) / bsz) + 1.0;
+      int idx = (q != q) ? 0 : (q <= 0.0 ? 0 : (q >= 255.0 ? 255 : (int)q));
+      atomicAdd(&bins[idx], 1);
+      ok++;
+    }
+    int okCount = block_sum_int(ok, isc);
+    if (tid == 0) {
+      double binSize = h_binSize, lowerBinVal = h_lowerBinVal;
+      if (okCount == 0 || binSize == 0.0) {
+        h_state = 2;
+      } else {
+        double sumBins = 0.0, maxDens = 0.0;
+        int lowerIndex = 1, upperIndex = 1, maxIndex = -1;
+        while (upperIndex < 255) {
+          while (sumBins < okCount / 10.0 && upperIndex < 255) sumBins += bins[upperIndex++];
+          if (sumBins / (upperIndex - lowerIndex) > maxDens) {
+            maxDens = sumBins / (upperIndex - lowerIndex);
+            maxIndex = lowerIndex;
+          }
+          sumBins -= bins[lowerIndex++];
+        }
+        if (maxIndex < 0 || maxIndex > 255) maxIndex = 0;
+        sumBins = 0.0;
+        double sumExpect = 0.0;
+        for (int i = maxIndex; sumBins < okCount / 10.0 && i < 255; i++) {
+          sumBins += bins[i];
+          sumExpect += i * bins[i];
+        }
+        double modeBin = sumExpect / sumBins + 0.5;
+        double skyEst = lowerBinVal + binSize * (modeBin - 1.0);
+        double lower = okCount * 0.25, upper = okCount * 0.75;
+        sumBins = 0.0;
+        int i = 0;
+        while (sumBins < lower) sumBins += bins[i++];
+        lower = i - (sumBins - lower) / bins[i - 1];
+        while (sumBins < upper) sumBins += bins[i++];
+        upper = i - (sumBins - upper) / bins[i - 1];
+        h_sky = skyEst;
+        h_lower = lower;
+        h_upper = upper;
+        if (lower < 1.0 || upper > 255.0) {
+          h_lowerBinVal = lowerBinVal - 128.0 * binSize;
+          h_binSize = binSize * 2;
+        } else if (upper - lower < 40.0) {
+          h_binSize = binSize / 3.0;
+          h_lowerBinVal = skyEst - 128.0 * h_binSize;
+        } else {
+          h_state = 1;
+        }
+      }
+    }
+    __syncthreads();
+    if (h_state != 0) break;
+  }
+  if (tid == 0) {
+    fwhmOut[s] = h_state == 1 ? h_binSize * (h_upper - h_lower) / iqRange : 0.0;
+    skyOut[s] = h_sky;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// findSubStamps (sss.cl:326-505, sssUtil.cpp:40-104)
+// ---------------------------------------------------------------------------
+struct FindArgs {
+  const float* img;
+  uint16_t* mask;
+  int w, h, nx, ny, fsw;
+  const double* sky;
+  const double* fwhm;
+  int* found;
+  int* count;
+  int* ssx;
+  int* ssy;
+  double* ssv;
+  double hi, tkf;
+  uint16_t badMask, badPixelMask, skipMask;
+};
+
+// Block-wide (value, index) argmax: larger value wins, ties -> smaller index.
+// NaN values never win.  Returns index (or INT_MAX when no candidate).
+__device__ int block_argmax_first(double v, int idx, double* dsc, int* isc2) {
+  const int lane = lane_id(), wid = threadIdx.x / WAVE;
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o);
+    int oi = __shfl_xor(idx, o);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  if (lane == 0) { dsc[wid] = v; isc2[wid] = idx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bv = dsc[0];
+    int bi = isc2[0];
+    for (int i = 1; i < (int)(blockDim.x / WAVE); i++)
+      if (dsc[i] > bv || (dsc[i] == bv && isc2[i] < bi)) { bv = dsc[i]; bi = isc2[i]; }
+    dsc[16] = bv;
+    isc2[16] = bi;
+  }
+  __syncthreads();
+  int r = isc2[16];
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_sum_double_tree(double v, double* dsc) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int wid = threadIdx.x / WAVE;
+  if (lane_id() == 0) dsc[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    for (int i = 0; i < (int)(blockDim.x / WAVE); i++) a += dsc[i];
+    dsc[16] = a;
+  }
+  __syncthreads();
+  double r = dsc[16];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ int float_exp_field(float f) {
+  int e = (int)((__float_as_uint(f) >> 23) & 0xff);
+  return e == 0 ? 1 : e;
+}
+
+__global__ __launch_bounds__(256) void k_sss_find(FindArgs A) {
+  const int s = blockIdx.x;
+  const int tid = threadIdx.x;
+  const StampGeom g = stamp_geom(s, A.nx, A.ny, A.w, A.h, A.fsw);
+  const double sky = A.sky[s], fwhm = A.fwhm[s];
+  const double floorV = sky + A.tkf * fwhm;
+  const int fsw = A.fsw;
+  const long imgW = A.w;
+  const float* img = A.img;
+  uint16_t* mask = A.mask;
+  const uint16_t badMask = A.badMask;
+  __shared__ int isc[20];
+  __shared__ double dsc[20];
+  __shared__ int isc2[20];
+  __shared__ int cand[256];
+  __shared__ int s_nc;
+  __shared__ int lx[MAXSS], ly[MAXSS];
+  __shared__ double lv[MAXSS];
+  __shared__ int s_counter;
+  if (tid == 0) s_counter = 0;
+  __syncthreads();
+  double dfrac = 0.9;
+  const int nAll = fsw * fsw;
+  // The reference loops until the threshold reaches the floor; with a
+  // non-finite sky or sigma that never happens, so the levels are capped
+  // (finite statistics end within 6 levels).
+  for (int level = 0; level < MAX_LEVELS && s_counter < MAXSS; level++) {
+    const double lim = std_max(floorV, sky + (A.hi - sky) * dfrac);
+    for (int base = 0; base < nAll && s_counter < MAXSS; base += blockDim.x) {
+      // classify this chunk against the current mask
+      int k = base + tid;
+      bool isCand = false;
+      if (k < nAll) {
+        int absx = k % fsw + g.ox, absy = k / fsw + g.oy;
+        long a = absx + absy * imgW;
+        uint16_t m = mask[a];
+        if ((m & badMask) == 0) {
+          double v = (double)img[a];
+          if (v > A.hi) {
+            mask[a] = m | A.badPixelMask;
+          } else if (!((v - sky) * (1.0 / fwhm) < A.tkf) && v > lim) {
+            isCand = true;
+          }
+        }
+      }
+      int tot;
+      int pos = block_prefix_count(isCand, isc, &tot);
+      if (isCand) cand[pos] = k;
+      if (tid == 0) s_nc = tot;
+      __syncthreads();
+      const int nc = s_nc;
+      for (int ci = 0; ci < nc; ci++) {
+        const int kk = cand[ci];
+        const int absx = kk % fsw + g.ox, absy = kk / fsw + g.oy;
+        const long a = absx + absy * imgW;
+        // the scan re-reads the mask at this pixel (earlier candidates may
+        // have marked it)
+        if ((mask[a] & badMask) != 0) {
+          __syncthreads();
+          continue;
+        }
+        // ---- window maximum (sss.cl:427-459)
+        const int startX = max(absx - HSS, g.ox), startY = max(absy - HSS, g.oy);
+        const int endX = min(absx + HSS, g.ox + fsw - 1), endY = min(absy + HSS, g.oy + fsw - 1);
+        const int ww = endX - startX + 1, wh = endY - startY + 1;
+        double bestV = 0.0;
+        int bestI = INT_MAX;
+        for (int p = tid; p < ww * wh; p += blockDim.x) {
+          int kx = startX + p % ww, ky = startY + p / ww;
+          long kc = kx + ky * imgW;
+          uint16_t km = mask[kc];
+          if ((km & badMask) > 0) continue;
+          double kv = (double)img[kc];
+          if (kv >= A.hi) {
+            mask[kc] = km | A.badPixelMask;
+            continue;
+          }
+          if ((kv - sky) * (1.0 / fwhm) < A.tkf) continue;
+          if (kv > bestV) { bestV = kv; bestI = p; }
+        }
+        if (bestI == INT_MAX) bestV = -INFINITY;
+        int win = block_argmax_first(bestV, bestI, dsc, isc2);
+        int mx = absx, my = absy;
+        if (win != INT_MAX) {
+          mx = startX + win % ww;
+          my = startY + win / ww;
+        }
+        // ---- checkSStamp (sss.cl:326-361)
+        const int cx0 = max(mx - HSS, g.ox), cy0 = max(my - HSS, g.oy);
+        const int cx1 = min(mx + HSS, g.ox + g.sw - 1), cy1 = min(my + HSS, g.oy + g.sh - 1);
+        const int cw = cx1 - cx0 + 1, ch = cy1 - cy0 + 1;
+        int firstBad = INT_MAX;
+        int badType = 0;
+        double part = 0.0;
+        int eMin = 1000, eMax = -1000;
+        for (int p = tid; p < cw * ch; p += blockDim.x) {
+          int x = cx0 + p % cw, y = cy0 + p / cw;
+          long ac = x + y * imgW;
+          if ((mask[ac] & badMask) > 0) {
+            if (p < firstBad) { firstBad = p; badType = 1; }
+            continue;
+          }
+          float fv = img[ac];
+          double v = (double)fv;
+          if (v > A.hi) {
+            if (p < firstBad) { firstBad = p; badType = 2; }
+            continue;
+          }
+          if ((v - sky) / fwhm > A.tkf) {
+            part += v;
+            if (fv != 0.0f) {
+              int e = float_exp_field(fv);
+              eMin = min(eMin, e);
+              eMax = max(eMax, e);
+            }
+          }
+        }
+        // first bad pixel in raster order
+        int fb = block_min_int(firstBad, isc);
+        if (fb != INT_MAX) {
+          if (firstBad == fb && badType == 2 && tid == (fb % blockDim.x)) {
+            int x = cx0 + fb % cw, y = cy0 + fb / cw;
+            long ac = x + y * imgW;
+            mask[ac] = mask[ac] | A.badPixelMask;
+          }
+          __syncthreads();
+          continue;  // checkSStamp returned 0
+        }
+        // sum of qualifying values in raster order: exact in any order when
+        // the exponent spread is small (all partial sums representable),
+        // otherwise summed sequentially by one thread.
+        int gMin = block_min_int(eMin, isc);
+        int gMax = block_max_int(eMax, isc);
+        double total;
+        if (gMax < gMin || gMax - gMin <= 19) {
+          total = block_sum_double_tree(part, dsc);
+        } else {
+          if (tid == 0) {
+            double r = 0.0;
+            for (int p = 0; p < cw * ch; p++) {
+              int x = cx0 + p % cw, y = cy0 + p / cw;
+              double v = (double)img[x + y * imgW];
+              if ((v - sky) / fwhm > A.tkf) r += v;
+            }
+            dsc[18] = r;
+          }
+          __syncthreads();
+          total = dsc[18];
+          __syncthreads();
+        }
+        if (total == 0.0) {
+          __syncthreads();
+          continue;
+        }
+        if (tid == 0) {
+          int c = s_counter;
+          lx[c] = mx;
+          ly[c] = my;
+          lv[c] = total;
+          s_counter = c + 1;
+        }
+        // skip marks (sss.cl:473-482), clamped with the stamp size
+        for (int p = tid; p < cw * ch; p += blockDim.x) {
+          int x = cx0 + p % cw, y = cy0 + p / cw;
+          long ac = x + y * imgW;
+          mask[ac] = mask[ac] | A.skipMask;
+        }
+        __syncthreads();
+        if (s_counter >= MAXSS) break;
+      }
+      __syncthreads();
+    }
+    if (lim == floorV) break;
+    dfrac -= 0.2;
+  }
+  if (tid == 0) {
+    int counter = s_counter;
+    for (int i = 1; i < counter; i++)
+      for (int j = i; j > 0 && lv[j - 1] < lv[j]; j--) {
+        int tx = lx[j - 1]; lx[j - 1] = lx[j]; lx[j] = tx;
+        int ty = ly[j - 1]; ly[j - 1] = ly[j]; ly[j] = ty;
+        double tv = lv[j - 1]; lv[j - 1] = lv[j]; lv[j] = tv;
+      }
+    A.found[s] = counter;
+    A.count[s] = min(counter, KEEPSS);
+    for (int i = 0; i < MAXSS; i++) {
+      size_t o = (size_t)s * MAXSS + i;
+      A.ssx[o] = i < counter ? lx[i] : INT_MAX;
+      A.ssy[o] = i < counter ? ly[i] : INT_MAX;
+      A.ssv[o] = i < counter ? lv[i] : -INFINITY;
+    }
+  }
+}
+
+// removeEmptyStamps (sss.cl:507-571): ordered compaction of non-empty stamps,
+// both images; retry flag when either keeps less than 10 % (bach.cpp:132-142).
+__global__ __launch_bounds__(1024) void k_sss_compact(const int* __restrict__ count0,
+                                                      const int* __restrict__ count1,
+                                                      int* kept0, int* kept1, int* cur0,
+                                                      int* cur1, DevScalars* sc, int nGrid) {
+  __shared__ int isc[20];
+  __shared__ int s_n;
+  for (int im = 0; im < 2; im++) {
+    const int* cnt = im == 0 ? count0 : count1;
+    int* kept = im == 0 ? kept0 : kept1;
+    int* cur = im == 0 ? cur0 : cur1;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int base = 0; base < nGrid; base += blockDim.x) {
+      int s = base + threadIdx.x;
+      bool keep = s < nGrid && cnt[s] > 0;
+      int tot;
+      int pos = block_prefix_count(keep, isc, &tot);
+      int c0 = s_n;
+      if (keep) {
+        kept[c0 + pos] = s;
+        cur[c0 + pos] = 0;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) s_n = c0 + tot;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) sc->nKept[im] = s_n;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double f0 = (double)sc->nKept[0] / nGrid, f1 = (double)sc->nKept[1] / nGrid;
+    sc->retry = (f0 < 0.1 || f1 < 0.1) ? 1 : 0;
+  }
+}
+
+__global__ void k_reset_skip(uint16_t* mask, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) mask[i] &= (uint16_t) ~(XB_MASK_SKIP_S | XB_MASK_SKIP_T);
+}
+
+hipError_t launch_sss_stats(Ctx& c, int im) {
+  if ((c.fsw * c.fsw + 31) / 32 > MAX_CLIP_BLOCKS) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sss_stats, dim3(c.nGrid), dim3(256), 0, c.stream, c.img[im], c.mask, c.w,
+                     c.h, c.stampsx, c.stampsy, c.fsw, c.ran1, c.goodPix, c.m.sigClipAlpha,
+                     c.m.iqRange, c.st[im].sky, c.st[im].fwhm);
+  return hipGetLastError();
+}
+
+hipError_t launch_sss_find(Ctx& c, int im) {
+  FindArgs A;
+  A.img = c.img[im];
+  A.mask = c.mask;
+  A.w = c.w; A.h = c.h; A.nx = c.stampsx; A.ny = c.stampsy; A.fsw = c.fsw;
+  A.sky = c.st[im].sky; A.fwhm = c.st[im].fwhm;
+  A.found = c.st[im].found; A.count = c.st[im].count;
+  A.ssx = c.st[im].ssx; A.ssy = c.st[im].ssy; A.ssv = c.st[im].ssv;
+  A.hi = c.m.threshHigh; A.tkf = c.m.threshKernFit;
+  uint16_t badMask = XB_MASK_ALL & ~XB_MASK_OK_CONV;
+  if (im == 0) {
+    badMask &= ~(XB_MASK_BAD_PIXEL_S | XB_MASK_SKIP_S);
+    A.badPixelMask = XB_MASK_BAD_PIXEL_T;
+    A.skipMask = XB_MASK_SKIP_T;
+  } else {
+    badMask &= ~(XB_MASK_BAD_PIXEL_T | XB_MASK_SKIP_T);
+    A.badPixelMask = XB_MASK_BAD_PIXEL_S;
+    A.skipMask = XB_MASK_SKIP_S;
+  }
+  A.badMask = badMask;
+  hipLaunchKernelGGL(k_sss_find, dim3(c.nGrid), dim3(256), 0, c.stream, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_sss_compact(Ctx& c) {
+  hipLaunchKernelGGL(k_sss_compact, dim3(1), dim3(1024), 0, c.stream, c.st[0].count,
+                     c.st[1].count, c.st[0].kept, c.st[1].kept, c.st[0].cur, c.st[1].cur, c.sc,
+                     c.nGrid);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_skip(Ctx& c) {
+  size_t n = (size_t)c.w * c.h;
+  hipLaunchKernelGGL(k_reset_skip, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
+                     c.mask, n);
+  return hipGetLastError();
+}
+
+}  // namespace xb

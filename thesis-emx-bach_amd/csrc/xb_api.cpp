@@ -1,0 +1,713 @@
+// xb_api.cpp — host side of libxbach: context, resident buffers, the stage
+// functions of the reference pipeline (src/bach.cpp:14-440) and the C ABI.
+//
+// Control flow stays on the host as in the reference (stage order, the <10 %
+// retry, the cd direction choice, the ksc do-while), but every stage is a few
+// stream-ordered launches and the host reads back only the handful of scalars
+// that steer it (kept-stamp counts, merits, the ksc `check` flag).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "xb_internal.h"
+
+using namespace xb;
+
+struct xb_ctx {
+  Ctx c;
+};
+
+namespace {
+
+#define XB_HIP(expr)                                                              \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      c.err = std::string(#expr) + ": " + hipGetErrorString(_e);                  \
+      return XB_ERR_HIP;                                                          \
+    }                                                                             \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+  if (*p) {
+    hipError_t e = hipFree(*p);
+    if (e != hipSuccess) return e;
+    *p = nullptr;
+  }
+  if (n == 0) n = 1;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+
+template <class T>
+void dfree(T** p) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+}
+
+// ---- basis tables (cl_kern/cmv.cl:1-68, include/datatypeUtil.h:37-118) ----
+void buildBasis(const xb_params& a, Basis& B) {
+  int n = 0;
+  for (int g = 0; g < 3; g++)
+    for (int x = 0; x <= a.dg[g]; x++)
+      for (int y = 0; y <= a.dg[g] - x; y++) {
+        const double b = (double)a.bg[g];
+        double sumX = 0.0, sumY = 0.0;
+        for (int i = 0; i < KW; i++) {
+          double xv = (double)(i - KW / 2);
+          double qe = std::exp(-xv * xv * b);
+          double px = 1.0, py = 1.0;
+          for (int e = 0; e < x; e++) px *= xv;
+          for (int e = 0; e < y; e++) py *= xv;
+          B.fx[n * KW + i] = qe * px;
+          B.fy[n * KW + i] = qe * py;
+          sumX += B.fx[n * KW + i];
+          sumY += B.fy[n * KW + i];
+        }
+        const bool even = (x % 2 == 0) && (y % 2 == 0);
+        if (even) {
+          double ix = 1.0 / sumX, iy = 1.0 / sumY;
+          for (int i = 0; i < KW; i++) {
+            B.fx[n * KW + i] *= ix;
+            B.fy[n * KW + i] *= iy;
+          }
+        }
+        B.kx[n] = x;
+        B.ky[n] = y;
+        B.evenSub[n] = (even && n > 0) ? 1 : 0;
+        n++;
+      }
+  for (int k = 0; k < NPSF; k++)
+    for (int v = 0; v < KW; v++)
+      for (int u = 0; u < KW; u++) {
+        double vv = B.fx[k * KW + u] * B.fy[k * KW + v];
+        if (B.evenSub[k]) vv -= B.fx[u] * B.fy[v];
+        B.vec[k * KK + v * KW + u] = vv;
+      }
+}
+
+// ---- NR ran1 draws, seed -666 (cl_kern/sss.cl:41-79) ----------------------
+void ran1Table(int n, std::vector<double>& out) {
+  const long M1 = 259200, IA1 = 7141, IC1 = 54773;
+  const long M2 = 134456, IA2 = 8121, IC2 = 28411;
+  const long M3 = 243000, IA3 = 4561, IC3 = 51349;
+  const double RM1 = 1.0 / M1, RM2 = 1.0 / M2;
+  long ix1, ix2, ix3;
+  double r[98];
+  int idum = -666;
+  ix1 = (IC1 - idum) % M1;
+  ix1 = (IA1 * ix1 + IC1) % M1;
+  ix2 = ix1 % M2;
+  ix1 = (IA1 * ix1 + IC1) % M1;
+  ix3 = ix1 % M3;
+  for (int j = 1; j <= 97; j++) {
+    ix1 = (IA1 * ix1 + IC1) % M1;
+    ix2 = (IA2 * ix2 + IC2) % M2;
+    r[j] = (ix1 + ix2 * RM2) * RM1;
+  }
+  out.resize((size_t)n);
+  for (int k = 0; k < n; k++) {
+    ix1 = (IA1 * ix1 + IC1) % M1;
+    ix2 = (IA2 * ix2 + IC2) % M2;
+    ix3 = (IA3 * ix3 + IC3) % M3;
+    int j = (int)(1 + ((97 * ix3) / M3));
+    out[k] = r[j];
+    r[j] = (ix1 + ix2 * RM2) * RM1;
+  }
+}
+
+int checkParams(const xb_params& p, std::string& err) {
+  bool ok = p.fKernelWidth == KW && p.hKernelWidth == HKW && p.fSStampWidth == SSW &&
+            p.hSStampWidth == HSS && p.maxKSStamps == KEEPSS && p.kernelOrder == 2 &&
+            p.backgroundOrder == 1 && p.ngauss == 3 && p.dg[0] == 6 && p.dg[1] == 4 &&
+            p.dg[2] == 2;
+  if (!ok) {
+    err = "libxbach is built for the reference's fixed method constants "
+          "(21x21 kernel, 31x31 substamps, dg 6/4/2, kernel order 2, background order 1, "
+          "maxKSStamps 3)";
+    return XB_ERR_ARG;
+  }
+  if (p.stampsx <= 0 || p.stampsy <= 0) {
+    err = "stamp grid must be positive";
+    return XB_ERR_ARG;
+  }
+  return XB_OK;
+}
+
+int ensureImage(Ctx& c, int w, int h) {
+  size_t n = (size_t)w * h;
+  if (n <= c.capPix) return XB_OK;
+  XB_HIP(dalloc(&c.img[0], n));
+  XB_HIP(dalloc(&c.img[1], n));
+  XB_HIP(dalloc(&c.mask, n));
+  XB_HIP(dalloc(&c.conv, n));
+  XB_HIP(dalloc(&c.diff, n));
+  const int xSteps = (w + KW - 1) / KW, ySteps = (h + KW - 1) / KW;
+  const int tiles = xSteps * ySteps;
+  XB_HIP(dalloc(&c.tileK, (size_t)tiles * KK));
+  XB_HIP(dalloc(&c.tileK64, (size_t)tiles * KK));
+  XB_HIP(dalloc(&c.tileAks, (size_t)tiles));
+  c.capTiles = tiles;
+  c.capPix = n;
+  return XB_OK;
+}
+
+int ensureStamps(Ctx& c, int N, int fsw) {
+  if (N > c.capStamps) {
+    for (int im = 0; im < 2; im++) {
+      DevStamps& s = c.st[im];
+      XB_HIP(dalloc(&s.sky, N));
+      XB_HIP(dalloc(&s.fwhm, N));
+      XB_HIP(dalloc(&s.found, N));
+      XB_HIP(dalloc(&s.count, N));
+      XB_HIP(dalloc(&s.ssx, (size_t)N * MAXSS));
+      XB_HIP(dalloc(&s.ssy, (size_t)N * MAXSS));
+      XB_HIP(dalloc(&s.ssv, (size_t)N * MAXSS));
+      XB_HIP(dalloc(&s.kept, N));
+      XB_HIP(dalloc(&s.cur, N));
+      XB_HIP(dalloc(&s.W, (size_t)N * WROWS * NPIX));
+      XB_HIP(dalloc(&s.G, (size_t)N * WROWS * WROWS));
+      XB_HIP(dalloc(&s.Bc, (size_t)N * WROWS));
+      XB_HIP(dalloc(&s.Br, (size_t)N * NBG));
+      XB_HIP(dalloc(&s.sig, N));
+      XB_HIP(dalloc(&s.ksum, N));
+      XB_HIP(dalloc(&s.list, N));
+    }
+    c.capStamps = N;
+    c.capFsw = 0;  // good-pixel scratch depends on N too
+  }
+  if (fsw > c.capFsw) {
+    XB_HIP(dalloc(&c.goodPix, (size_t)c.capStamps * fsw * fsw));
+    c.capFsw = fsw;
+  }
+  const int draws = 2 * fsw * fsw;
+  if (draws > c.ran1Count) {
+    std::vector<double> t;
+    ran1Table(draws, t);
+    XB_HIP(dalloc(&c.ran1, (size_t)draws));
+    XB_HIP(hipMemcpyAsync(c.ran1, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice,
+                          c.stream));
+    XB_HIP(hipStreamSynchronize(c.stream));
+    c.ran1Count = draws;
+  }
+  return XB_OK;
+}
+
+int readScalars(Ctx& c, DevScalars& out) {
+  XB_HIP(hipMemcpyAsync(&out, c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
+  XB_HIP(hipStreamSynchronize(c.stream));
+  return XB_OK;
+}
+
+double nowMs() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// ---- stages -----------------------------------------------------------------
+int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
+  if (!p || w <= 0 || h <= 0) {
+    c.err = "bad arguments";
+    return XB_ERR_ARG;
+  }
+  int rc = checkParams(*p, c.err);
+  if (rc) return rc;
+  c.p = *p;
+  c.m.threshLow = p->threshLow;
+  c.m.threshHigh = p->threshHigh;
+  c.m.threshKernFit = p->threshKernFit;
+  c.m.sigKernFit = p->sigKernFit;
+  c.m.sigClipAlpha = p->sigClipAlpha;
+  c.m.iqRange = p->iqRange;
+  c.m.normalizeTemplate = p->normalizeTemplate;
+  c.w = w;
+  c.h = h;
+  c.haveResult = false;
+  rc = ensureImage(c, w, h);
+  if (rc) return rc;
+  if (!c.basis) XB_HIP(hipMalloc((void**)&c.basis, sizeof(Basis)));
+  if (!c.sc) XB_HIP(hipMalloc((void**)&c.sc, sizeof(DevScalars)));
+  if (!c.M) {
+    XB_HIP(hipMalloc((void**)&c.M, sizeof(double) * MATN * MATN));
+    XB_HIP(hipMalloc((void**)&c.rhs, sizeof(double) * NSOL));
+    XB_HIP(hipMalloc((void**)&c.sol, sizeof(double) * NSOL));
+    XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
+    XB_HIP(hipMalloc((void**)&c.luIndex, sizeof(int) * MATN));
+    XB_HIP(hipMalloc((void**)&c.luVV, sizeof(double) * MATN));
+  }
+  static_assert(sizeof(Basis) < (1 << 20), "basis size");
+  Basis hb;
+  std::memset(&hb, 0, sizeof(hb));
+  buildBasis(c.p, hb);
+  XB_HIP(hipMemcpyAsync(c.basis, &hb, sizeof(Basis), hipMemcpyHostToDevice, c.stream));
+  XB_HIP(hipMemsetAsync(c.sc, 0, sizeof(DevScalars), c.stream));
+  XB_HIP(hipStreamSynchronize(c.stream));
+  return XB_OK;
+}
+
+int stageMask(Ctx& c) {
+  XB_HIP(launch_mask_input(c));
+  return XB_OK;
+}
+
+// sss (src/bach.cpp:44-170)
+int stageSss(Ctx& c) {
+  const xb_params& a = c.p;
+  int fsw = std::min(c.w / a.stampsx, c.h / a.stampsy);
+  int nx = a.stampsx, ny = a.stampsy;
+  fsw -= a.fKernelWidth;
+  fsw -= fsw % 2 == 0 ? 1 : 0;
+  if (fsw < a.fSStampWidth) {
+    fsw = a.fSStampWidth + a.fKernelWidth;
+    fsw -= fsw % 2 == 0 ? 1 : 0;
+    nx = c.w / fsw;
+    ny = c.h / fsw;
+  }
+  if (nx <= 0 || ny <= 0 || nx * ny > XB_MAX_STAMPS) {
+    c.err = "stamp grid outside 1..1024 stamps";
+    return XB_ERR_CAPACITY;
+  }
+  if ((fsw * fsw + 31) / 32 > 2560) {
+    c.err = "stamps wider than 286 px are not supported";
+    return XB_ERR_CAPACITY;
+  }
+  c.fsw = fsw;
+  c.stampsx = nx;
+  c.stampsy = ny;
+  c.nGrid = nx * ny;
+  // the substamp scan covers fsw x fsw from each origin (sss.cl:406-410)
+  if ((nx - 1) * c.w / nx + fsw > c.w || (ny - 1) * c.h / ny + fsw > c.h) {
+    c.err = "stamp grid exceeds the image";
+    return XB_ERR_ARG;
+  }
+  for (int s = 0; s < c.nGrid; s++) {  // bachUtil.cpp:144-156
+    int sx = s % nx, sy = s / nx;
+    int ox = sx * c.w / nx, oy = sy * c.h / ny;
+    int sw = std::min(ox + fsw, c.w) - ox, sh = std::min(oy + fsw, c.h) - oy;
+    if (sw * sh < NSAMP) {
+      c.err = "Not enough pixels in a stamp";
+      return XB_ERR_STAMP_PIXELS;
+    }
+  }
+  int rc = ensureStamps(c, c.nGrid, fsw);
+  if (rc) return rc;
+  auto identify = [&]() -> int {
+    XB_HIP(launch_sss_stats(c, 0));
+    XB_HIP(launch_sss_stats(c, 1));
+    XB_HIP(launch_sss_find(c, 0));
+    XB_HIP(launch_sss_find(c, 1));
+    XB_HIP(launch_sss_compact(c));
+    return XB_OK;
+  };
+  rc = identify();
+  if (rc) return rc;
+  DevScalars sc;
+  rc = readScalars(c, sc);
+  if (rc) return rc;
+  c.retried = 0;
+  if (sc.retry) {
+    c.retried = 1;
+    XB_HIP(launch_reset_skip(c));
+    rc = identify();
+    if (rc) return rc;
+    rc = readScalars(c, sc);
+    if (rc) return rc;
+  }
+  c.nKept[0] = sc.nKept[0];
+  c.nKept[1] = sc.nKept[1];
+  if (c.nKept[0] == 0 && c.nKept[1] == 0) {
+    c.err = "No substamps found";
+    return XB_ERR_NO_SUBSTAMPS;
+  }
+  return XB_OK;
+}
+
+// cmv (src/bach.cpp:172-268)
+int stageCmv(Ctx& c) {
+  for (int im = 0; im < 2; im++) XB_HIP(launch_fill(c, im, nullptr, c.nKept[im]));
+  return XB_OK;
+}
+
+// cd (src/bach.cpp:270-313, testFit src/cdkscUtil.cpp:4-266)
+int stageCd(Ctx& c, int* convTemplate) {
+  for (int im = 0; im < 2; im++) {
+    if (c.nKept[im] == 0) continue;
+    double* cdSol = c.cdSol + (size_t)im * NSOL;
+    XB_HIP(launch_stamp_lu(c, im));
+    XB_HIP(launch_cd_select(c, im));
+    XB_HIP(launch_global_build(c, im, true));
+    XB_HIP(launch_global_solve(c, cdSol));
+    XB_HIP(launch_calc_sigs(c, im, true, cdSol));
+    XB_HIP(launch_cd_finish(c, im));
+  }
+  DevScalars sc;
+  int rc = readScalars(c, sc);
+  if (rc) return rc;
+  double merit[2];
+  for (int im = 0; im < 2; im++) merit[im] = c.nKept[im] == 0 ? 666 : sc.merit[im];
+  c.convTemplate = merit[1] > merit[0] ? 1 : 0;
+  c.tIdx = c.convTemplate ? 0 : 1;
+  if (convTemplate) *convTemplate = c.convTemplate;
+  return XB_OK;
+}
+
+// ksc (src/cdkscUtil.cpp:585-684)
+int stageKsc(Ctx& c) {
+  const int im = c.tIdx;
+  int it = 0;
+  for (;;) {
+    XB_HIP(launch_global_build(c, im, false));
+    XB_HIP(launch_global_solve(c, c.sol));
+    XB_HIP(launch_calc_sigs(c, im, false, c.sol));
+    XB_HIP(launch_ksc_check(c, im));
+    XB_HIP(launch_fill(c, im, c.st[im].list, c.nKept[im]));
+    it++;
+    int check = 0;
+    XB_HIP(hipMemcpyAsync(&check, &c.sc->check, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    XB_HIP(hipStreamSynchronize(c.stream));
+    if (!check) break;
+    if (it > 3 * XB_MAX_STAMPS + 2) {
+      c.err = "ksc loop did not terminate";
+      return XB_ERR_HIP;
+    }
+  }
+  c.kscIterations = it;
+  return XB_OK;
+}
+
+int stageConv(Ctx& c, double* kernSum, bool readSum) {
+  XB_HIP(launch_conv(c));
+  if (readSum) {
+    XB_HIP(hipMemcpyAsync(&c.kernSum, &c.sc->kernSum, sizeof(double), hipMemcpyDeviceToHost,
+                          c.stream));
+    XB_HIP(hipStreamSynchronize(c.stream));
+    if (kernSum) *kernSum = c.kernSum;
+  }
+  return XB_OK;
+}
+
+int stageSub(Ctx& c) {
+  XB_HIP(launch_sub(c));
+  c.haveResult = true;
+  return XB_OK;
+}
+
+void freeAll(Ctx& c) {
+  dfree(&c.img[0]);
+  dfree(&c.img[1]);
+  dfree(&c.mask);
+  dfree(&c.conv);
+  dfree(&c.diff);
+  for (int im = 0; im < 2; im++) {
+    DevStamps& s = c.st[im];
+    dfree(&s.sky); dfree(&s.fwhm); dfree(&s.found); dfree(&s.count);
+    dfree(&s.ssx); dfree(&s.ssy); dfree(&s.ssv); dfree(&s.kept); dfree(&s.cur);
+    dfree(&s.W); dfree(&s.G); dfree(&s.Bc); dfree(&s.Br); dfree(&s.sig);
+    dfree(&s.ksum); dfree(&s.list);
+  }
+  dfree(&c.sc);
+  dfree(&c.basis);
+  dfree(&c.ran1);
+  dfree(&c.goodPix);
+  dfree(&c.M); dfree(&c.rhs); dfree(&c.sol); dfree(&c.cdSol);
+  dfree(&c.luIndex); dfree(&c.luVV);
+  dfree(&c.tileK); dfree(&c.tileK64); dfree(&c.tileAks);
+}
+
+int setDev(Ctx& c) {
+  XB_HIP(hipSetDevice(c.device));
+  return XB_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+void xb_params_default(xb_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->stampsx = p->stampsy = 10;
+  p->threshLow = 0.0;
+  p->threshHigh = 25000.0;
+  p->threshKernFit = 20.0;
+  p->sigKernFit = 2.0;
+  p->sigClipAlpha = 3.0;
+  p->iqRange = 1.35;
+  p->maxKSStamps = 3;
+  p->hSStampWidth = 15;
+  p->fSStampWidth = 31;
+  p->hKernelWidth = 10;
+  p->fKernelWidth = 21;
+  p->inSpreadMaskFactor = 1.0;
+  p->normalizeTemplate = 1;
+  p->backgroundOrder = 1;
+  p->kernelOrder = 2;
+  p->ngauss = 3;
+  p->dg[0] = 6;
+  p->dg[1] = 4;
+  p->dg[2] = 2;
+  p->bg[0] = (float)(1.0 / (2.0 * 0.7 * 0.7));
+  p->bg[1] = (float)(1.0 / (2.0 * 1.5 * 1.5));
+  p->bg[2] = (float)(1.0 / (2.0 * 3.0 * 3.0));
+  p->verbose = 0;
+}
+
+int xb_ctx_create(int device, int flags, xb_ctx** out) {
+  (void)flags;
+  if (!out) return XB_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return XB_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return XB_ERR_NO_DEVICE;
+  xb_ctx* x = new xb_ctx();
+  Ctx& c = x->c;
+  c.device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete x;
+    return XB_ERR_NO_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    std::snprintf(c.name, sizeof(c.name), "%s (%s)", prop.name, prop.gcnArchName);
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+      delete x;
+      return XB_ERR_NOT_BUILT;
+    }
+  }
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete x;
+    return XB_ERR_HIP;
+  }
+  *out = x;
+  return XB_OK;
+}
+
+int xb_ctx_destroy(xb_ctx* x) {
+  if (!x) return XB_OK;
+  Ctx& c = x->c;
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  freeAll(c);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete x;
+  return XB_OK;
+}
+
+const char* xb_last_error(xb_ctx* x) { return x ? x->c.err.c_str() : "no context"; }
+
+int xb_device_name(xb_ctx* x, char* buf, size_t len) {
+  if (!x || !buf || len == 0) return XB_ERR_ARG;
+  std::snprintf(buf, len, "%s", x->c.name);
+  return XB_OK;
+}
+
+int xb_init(xb_ctx* x, const xb_params* p, int w, int h, const float* tmpl, const float* sci) {
+  if (!x || !tmpl || !sci) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  int rc = setDev(c);
+  if (rc) return rc;
+  rc = stageInitCommon(c, p, w, h);
+  if (rc) return rc;
+  size_t n = (size_t)w * h;
+  XB_HIP(hipMemcpyAsync(c.img[0], tmpl, n * sizeof(float), hipMemcpyHostToDevice, c.stream));
+  XB_HIP(hipMemcpyAsync(c.img[1], sci, n * sizeof(float), hipMemcpyHostToDevice, c.stream));
+  return stageMask(c);
+}
+
+int xb_init_device(xb_ctx* x, const xb_params* p, int w, int h, const void* d_tmpl,
+                   const void* d_sci) {
+  if (!x || !d_tmpl || !d_sci) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  int rc = setDev(c);
+  if (rc) return rc;
+  rc = stageInitCommon(c, p, w, h);
+  if (rc) return rc;
+  size_t n = (size_t)w * h;
+  XB_HIP(hipMemcpyAsync(c.img[0], d_tmpl, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+  XB_HIP(hipMemcpyAsync(c.img[1], d_sci, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+  return stageMask(c);
+}
+
+int xb_sss(xb_ctx* x) { return x ? stageSss(x->c) : XB_ERR_ARG; }
+int xb_cmv(xb_ctx* x) { return x ? stageCmv(x->c) : XB_ERR_ARG; }
+int xb_cd(xb_ctx* x, int* convTemplate) { return x ? stageCd(x->c, convTemplate) : XB_ERR_ARG; }
+int xb_ksc(xb_ctx* x) { return x ? stageKsc(x->c) : XB_ERR_ARG; }
+int xb_conv(xb_ctx* x, double* kernSum) { return x ? stageConv(x->c, kernSum, true) : XB_ERR_ARG; }
+int xb_sub(xb_ctx* x) {
+  if (!x) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  int rc = stageSub(c);
+  if (rc) return rc;
+  XB_HIP(hipStreamSynchronize(c.stream));
+  return XB_OK;
+}
+
+static int getImage(xb_ctx* x, const void* d, void* out, size_t elem) {
+  if (!x || !out) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  if (!c.haveResult) {
+    c.err = "no result yet";
+    return XB_ERR_ARG;
+  }
+  XB_HIP(hipMemcpyAsync(out, d, (size_t)c.w * c.h * elem, hipMemcpyDeviceToHost, c.stream));
+  XB_HIP(hipStreamSynchronize(c.stream));
+  return XB_OK;
+}
+
+int xb_get_conv(xb_ctx* x, float* out) { return x ? getImage(x, x->c.conv, out, 4) : XB_ERR_ARG; }
+int xb_get_diff(xb_ctx* x, float* out) { return x ? getImage(x, x->c.diff, out, 4) : XB_ERR_ARG; }
+int xb_get_mask(xb_ctx* x, uint16_t* out) { return x ? getImage(x, x->c.mask, out, 2) : XB_ERR_ARG; }
+
+int xb_device_outputs(xb_ctx* x, const void** d_conv, const void** d_diff) {
+  if (!x) return XB_ERR_ARG;
+  if (d_conv) *d_conv = x->c.conv;
+  if (d_diff) *d_diff = x->c.diff;
+  return XB_OK;
+}
+
+int xb_get_trace(xb_ctx* x, xb_trace* tr) {
+  if (!x || !tr) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  std::memset(tr, 0, sizeof(*tr));
+  tr->w = c.w;
+  tr->h = c.h;
+  tr->stampsx = c.stampsx;
+  tr->stampsy = c.stampsy;
+  tr->fStampWidth = c.fsw;
+  tr->nGrid = c.nGrid;
+  tr->retried = c.retried;
+  const int N = c.nGrid;
+  std::vector<int> ix((size_t)N * MAXSS), iy((size_t)N * MAXSS), cnt(N), kept(N), cur(N);
+  std::vector<double> sv((size_t)N * MAXSS), sky(N), fw(N);
+  for (int im = 0; im < 2; im++) {
+    DevStamps& s = c.st[im];
+    if (N == 0) break;
+    XB_HIP(hipMemcpy(sky.data(), s.sky, N * sizeof(double), hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(fw.data(), s.fwhm, N * sizeof(double), hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(cnt.data(), s.count, N * sizeof(int), hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(ix.data(), s.ssx, ix.size() * sizeof(int), hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(iy.data(), s.ssy, iy.size() * sizeof(int), hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(sv.data(), s.ssv, sv.size() * sizeof(double), hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(kept.data(), s.kept, N * sizeof(int), hipMemcpyDeviceToHost));
+    for (int g = 0; g < N; g++) {
+      tr->sky[im][g] = sky[g];
+      tr->fwhm[im][g] = fw[g];
+      tr->ssCount[im][g] = cnt[g];
+      for (int i = 0; i < MAXSS; i++) {
+        tr->ssX[im][g * XB_MAX_SS + i] = ix[(size_t)g * MAXSS + i];
+        tr->ssY[im][g * XB_MAX_SS + i] = iy[(size_t)g * MAXSS + i];
+        tr->ssVal[im][g * XB_MAX_SS + i] = sv[(size_t)g * MAXSS + i];
+      }
+    }
+    tr->stampCount[im] = c.nKept[im];
+    for (int k = 0; k < c.nKept[im]; k++) tr->stampIndex[im][k] = kept[k];
+  }
+  DevScalars sc;
+  int rc = readScalars(c, sc);
+  if (rc) return rc;
+  for (int im = 0; im < 2; im++) {
+    tr->nTest[im] = c.nKept[im] ? sc.nTest[im] : 0;
+    tr->merit[im] = c.nKept[im] ? sc.merit[im] : 666;
+    tr->kernelSumCd[im] = c.nKept[im] ? sc.kernelSumCd[im] : 0.0;
+    XB_HIP(hipMemcpy(tr->cdSolution[im], c.cdSol + (size_t)im * NSOL, NSOL * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  }
+  tr->convTemplate = c.convTemplate;
+  tr->kscIterations = c.kscIterations;
+  tr->nSol = NSOL;
+  XB_HIP(hipMemcpy(tr->solution, c.sol, NSOL * sizeof(double), hipMemcpyDeviceToHost));
+  if (N > 0) {
+    XB_HIP(hipMemcpy(cur.data(), c.st[c.tIdx].cur, N * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < c.nKept[c.tIdx]; k++) tr->finalCurrent[k] = cur[k];
+  }
+  tr->kernSum = sc.kernSum;
+  return XB_OK;
+}
+
+static int runStages(Ctx& c, double* ms, bool timed) {
+  double t0 = nowMs(), t = t0;
+  auto mark = [&](int i) -> int {
+    if (!timed) return XB_OK;
+    XB_HIP(hipStreamSynchronize(c.stream));
+    double n = nowMs();
+    ms[i] = n - t;
+    t = n;
+    return XB_OK;
+  };
+  int rc = mark(0);
+  if (rc) return rc;
+  rc = stageSss(c);
+  if (rc) return rc;
+  if ((rc = mark(1))) return rc;
+  rc = stageCmv(c);
+  if (rc) return rc;
+  if ((rc = mark(2))) return rc;
+  rc = stageCd(c, nullptr);
+  if (rc) return rc;
+  if ((rc = mark(3))) return rc;
+  rc = stageKsc(c);
+  if (rc) return rc;
+  if ((rc = mark(4))) return rc;
+  rc = stageConv(c, nullptr, false);
+  if (rc) return rc;
+  if ((rc = mark(5))) return rc;
+  rc = stageSub(c);
+  if (rc) return rc;
+  if ((rc = mark(6))) return rc;
+  return XB_OK;
+}
+
+int xb_pipeline_run(xb_ctx* x, const xb_params* p, int w, int h, const float* tmpl,
+                    const float* sci, float* conv, float* diff, double* stage_ms) {
+  if (!x) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  double ms[9] = {0};
+  double t0 = nowMs();
+  int rc = xb_init(x, p, w, h, tmpl, sci);
+  if (rc) return rc;
+  XB_HIP(hipStreamSynchronize(c.stream));
+  ms[0] = nowMs() - t0;
+  double ms2[9] = {0};
+  rc = runStages(c, ms2, true);
+  if (rc) return rc;
+  for (int i = 1; i <= 6; i++) ms[i] = ms2[i];
+  double tf = nowMs();
+  size_t n = (size_t)w * h;
+  if (conv)
+    XB_HIP(hipMemcpyAsync(conv, c.conv, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+  if (diff)
+    XB_HIP(hipMemcpyAsync(diff, c.diff, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+  XB_HIP(hipStreamSynchronize(c.stream));
+  XB_HIP(hipMemcpy(&c.kernSum, &c.sc->kernSum, sizeof(double), hipMemcpyDeviceToHost));
+  ms[7] = nowMs() - tf;
+  ms[8] = nowMs() - t0;
+  if (stage_ms)
+    for (int i = 0; i < 9; i++) stage_ms[i] = ms[i];
+  return XB_OK;
+}
+
+int xb_pipeline_run_device(xb_ctx* x, const xb_params* p, int w, int h, const void* d_tmpl,
+                           const void* d_sci) {
+  if (!x) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  int rc = xb_init_device(x, p, w, h, d_tmpl, d_sci);
+  if (rc) return rc;
+  double ms[9];
+  rc = runStages(c, ms, false);
+  if (rc) return rc;
+  XB_HIP(hipStreamSynchronize(c.stream));
+  return XB_OK;
+}
+
+}  // extern "C"

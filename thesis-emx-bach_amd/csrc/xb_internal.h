@@ -1,0 +1,158 @@
+// xb_internal.h — shared definitions of libxbach (MI355X / gfx950).
+//
+// The device path is specialised for the reference's fixed method constants
+// (include/argsUtil.h:29-56): 21x21 kernel, 31x31 substamps, three Gaussians of
+// degrees 6/4/2 (49 basis terms), kernel order 2 (6 spatial terms), background
+// order 1 (3 terms), 3 substamps kept of 6 candidates.  xb_init rejects other
+// values with XB_ERR_ARG.  Everything the reference decides from data (stamp
+// grid, thresholds, sigma-clip constants) stays a runtime parameter.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/xbach.h"
+
+namespace xb {
+
+constexpr int KW = 21;             // fKernelWidth
+constexpr int HKW = 10;            // hKernelWidth
+constexpr int SSW = 31;            // fSStampWidth
+constexpr int HSS = 15;            // hSStampWidth
+constexpr int NPIX = SSW * SSW;    // 961 substamp pixels
+constexpr int NPSF = 49;           // basis terms
+constexpr int NBG = 3;             // background terms
+constexpr int WROWS = NPSF + NBG;  // 52 rows of W
+constexpr int NC1 = NPSF - 1;      // 48
+constexpr int NC2 = 6;             // spatial terms of kernel order 2
+constexpr int NCOMP = NC1 * NC2;   // 288
+constexpr int MATSIZE = NCOMP + NBG + 1;  // 292 unknowns (1-based)
+constexpr int MATN = MATSIZE + 1;         // 293 rows/cols stored
+constexpr int NSOL = NPSF * NC2 + NBG + 1;  // 298 solution slots
+constexpr int QN = NPSF + 2;              // 51: per-stamp test system (1-based)
+constexpr int MAXSS = 6;                  // 2 * maxKSStamps
+constexpr int KEEPSS = 3;                 // maxKSStamps
+constexpr int TW = 2 * (HSS + HKW) + 1;   // 51: Y-pass window width
+constexpr int TH = SSW;                   // 31: Y-pass rows
+constexpr int KK = KW * KW;               // 441
+constexpr int NSAMP = 100, NSAMP_PAD = 128;
+
+// Method parameters the kernels read (derived from xb_params).
+struct Method {
+  double threshLow, threshHigh, threshKernFit, sigKernFit, sigClipAlpha, iqRange;
+  int normalizeTemplate;
+};
+
+// Basis tables built once on the host in FP64 (glibc exp; bg rounded to float
+// as include/argsUtil.h:49-53) and uploaded, so the fit (cl_kern/cmv.cl:1-68)
+// and the convolution (include/datatypeUtil.h:62-118) use identical values.
+struct Basis {
+  double fx[NPSF * KW];
+  double fy[NPSF * KW];
+  double vec[NPSF * KK];
+  int kx[NPSF], ky[NPSF];
+  int evenSub[NPSF];  // 1 when W[n] -= W[0] (even-even, n > 0; cmv.cl:137-154)
+};
+
+// Per-image stamp arrays on the device.  Grid order for the sss results,
+// "kept" (compacted, ascending grid index) order for the fit data.
+struct DevStamps {
+  double* sky = nullptr;   // [N]
+  double* fwhm = nullptr;  // [N]
+  int* found = nullptr;    // [N]
+  int* count = nullptr;    // [N] min(found, 3)
+  int* ssx = nullptr;      // [N*MAXSS]
+  int* ssy = nullptr;
+  double* ssv = nullptr;
+  int* kept = nullptr;     // [N] grid index of kept stamp k
+  int* cur = nullptr;      // [N] current substamp of kept stamp k
+  double* W = nullptr;     // [N][WROWS][NPIX] (raw rows; even rows minus row 0 on use)
+  double* G = nullptr;     // [N][WROWS][WROWS] Gram of final W rows (lower part)
+  double* Bc = nullptr;    // [N][WROWS] W_r . target, column-major pixel order
+  double* Br = nullptr;    // [N][NBG]   bg rows . target, row-major pixel order
+  double* sig = nullptr;   // [N] chi2 per kept stamp
+  double* ksum = nullptr;  // [N] per-stamp test-fit kernel sums
+  int* list = nullptr;     // [N] stamp list scratch (test stamps / refill)
+};
+
+// Small device-side scalars of one pass (one struct, one allocation).
+struct DevScalars {
+  int nKept[2];
+  int nList;            // entries of the current stamp list
+  int check;            // ksc: a substamp was replaced
+  int nTest[2];
+  int retry;            // sss: refill ratio below 10 %
+  int status;           // device-detected error
+  double merit[2];
+  double kernelSumCd[2];
+  double clipMean, clipStd;
+  double kernSum;       // kernel sum at the image centre (conv)
+  int nChi2;
+  int pad;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  char name[256] = {0};
+  // problem
+  xb_params p{};
+  Method m{};
+  int w = 0, h = 0;
+  int stampsx = 0, stampsy = 0, fsw = 0, nGrid = 0;
+  size_t capPix = 0;
+  int capStamps = 0, capFsw = 0;
+  // device memory
+  float* img[2] = {nullptr, nullptr};  // 0 template, 1 science (FP32)
+  uint16_t* mask = nullptr;
+  float* conv = nullptr;
+  float* diff = nullptr;
+  DevStamps st[2];
+  DevScalars* sc = nullptr;
+  Basis* basis = nullptr;
+  double* ran1 = nullptr;       // ran1 draws (identical per stamp: seed -666)
+  int ran1Count = 0;
+  float* goodPix = nullptr;     // [N][fsw*fsw] good-pixel scratch of sss
+  double* M = nullptr;          // [MATN*MATN] global matrix
+  double* rhs = nullptr;        // [NSOL] right-hand side, then the solution
+  double* sol = nullptr;        // [NSOL] current kernel solution
+  double* cdSol = nullptr;      // [2][NSOL]
+  int* luIndex = nullptr;       // [MATN]
+  double* luVV = nullptr;       // [MATN]
+  float* tileK = nullptr;       // [tiles][KK] FP32 tile kernels
+  double* tileK64 = nullptr;    // [tiles][KK] FP64 tile kernels (mask test)
+  double* tileAks = nullptr;    // [tiles] sum |k| in the conv loop order
+  int capTiles = 0;
+  // host mirrors
+  int nKept[2] = {0, 0};
+  int convTemplate = 1;
+  int tIdx = 0;                 // image being convolved
+  double kernSum = 0.0;
+  int kscIterations = 0;
+  int retried = 0;
+  bool haveResult = false;
+  double stageMs[9] = {0};
+};
+
+// ---- launchers (defined in the .hip files) --------------------------------
+hipError_t launch_mask_input(Ctx& c);
+hipError_t launch_sss_stats(Ctx& c, int im);
+hipError_t launch_sss_find(Ctx& c, int im);
+hipError_t launch_sss_compact(Ctx& c);
+hipError_t launch_reset_skip(Ctx& c);
+hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax);
+hipError_t launch_stamp_lu(Ctx& c, int im);
+hipError_t launch_cd_select(Ctx& c, int im);
+hipError_t launch_global_build(Ctx& c, int im, bool cdForm);
+hipError_t launch_global_solve(Ctx& c, double* d_out);
+hipError_t launch_cd_finish(Ctx& c, int im);
+hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol);
+hipError_t launch_ksc_check(Ctx& c, int im);
+hipError_t launch_conv(Ctx& c);
+hipError_t launch_sub(Ctx& c);
+
+}  // namespace xb
