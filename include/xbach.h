@@ -166,6 +166,21 @@ int xb_pipeline_run(xb_ctx* ctx, const xb_params* p, int w, int h,
 int xb_pipeline_run_device(xb_ctx* ctx, const xb_params* p, int w, int h,
                            const void* d_tmpl, const void* d_sci);
 
+/* ---- device buffers (replaces the cl::Buffer handling of the reference,
+ *      src/bach.cpp:29-39; owned by the caller, freed with xb_buf_free) ---- */
+int xb_buf_alloc(xb_ctx* ctx, size_t bytes, void** d_out);
+int xb_buf_free(xb_ctx* ctx, void* d);
+int xb_buf_upload(xb_ctx* ctx, void* d, const void* host, size_t bytes);
+int xb_buf_download(xb_ctx* ctx, void* host, const void* d, size_t bytes);
+
+/* ---- per-kernel timing with HIP events on the context's stream ---------- */
+enum { XB_PROF_CONV = 0, XB_PROF_SUB = 1, XB_PROF_SOLVE = 2, XB_PROF_FILL = 3,
+       XB_PROF_SSS = 4, XB_PROF_MASK = 5, XB_PROF_N = 6 };
+/* enable != 0 starts recording (and clears the totals). */
+int xb_profile_enable(xb_ctx* ctx, int enable);
+/* Total milliseconds and launch counts per XB_PROF_* slot since enabling. */
+int xb_profile_get(xb_ctx* ctx, double* ms, int* counts, int n);
+
 /* ---- host-side helpers of the library (no GPU needed) -------------------- */
 /* Seeded synthetic pair: Gaussian stars (pixel-integrated), power-law fluxes,
  * sky + linear gradient, Poisson noise.  FWHM of the science PSF ramps linearly

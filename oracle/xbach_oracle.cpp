@@ -532,7 +532,10 @@ void findSubStamps(const Cfg& c, Pair& P, int which, int fsw, Stamps& st) {
     int lx[XB_MAX_SS], ly[XB_MAX_SS];
     double lv[XB_MAX_SS];
     int counter = 0;
-    while (counter < maxSS) {
+    /* The reference loops until the threshold reaches the floor; with a
+     * non-finite sky or sigma it never does, so the levels are capped at 32
+     * (finite statistics finish within 6 levels). */
+    for (int level = 0; level < 32 && counter < maxSS; level++) {
       double lim = std::max(floorV, sky + (hi - sky) * dfrac);
       for (int y = 0; y < fsw; y++) {
         int absy = y + oy;

@@ -223,10 +223,12 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax) {
   A.G = c.st[im].G;
   A.Bc = c.st[im].Bc;
   A.Br = c.st[im].Br;
+  prof_begin(c, XB_PROF_FILL);
   hipLaunchKernelGGL(k_fill_conv, dim3(nMax, WROWS), dim3(256), 0, c.stream, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fill_gram, dim3(nMax, 11), dim3(256), 0, c.stream, A);
+  prof_end(c, XB_PROF_FILL);
   return hipGetLastError();
 }
 

@@ -264,8 +264,10 @@ hipError_t launch_conv(Ctx& c) {
   A.scaleConv = (c.m.normalizeTemplate && c.convTemplate) || (!c.m.normalizeTemplate && !c.convTemplate);
   A.hi = c.m.threshHigh;
   A.lo = c.m.threshLow;
+  prof_begin(c, XB_PROF_CONV);
   hipLaunchKernelGGL(k_conv, dim3((xSteps + 3) / 4, ySteps), dim3(256), 0, c.stream, A);
   e = hipGetLastError();
+  prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_conv_border, dim3(1024), dim3(256), 0, c.stream, c.img[1 - ti], c.mask,
                      c.conv, c.w, c.h, c.m.threshHigh, c.m.threshLow);
@@ -277,8 +279,10 @@ hipError_t launch_sub(Ctx& c) {
   const int scaleConv = (c.m.normalizeTemplate && c.convTemplate) || (!c.m.normalizeTemplate && !c.convTemplate);
   size_t n = (size_t)c.w * c.h;
   unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  prof_begin(c, XB_PROF_SUB);
   hipLaunchKernelGGL(k_sub, dim3(blocks), dim3(256), 0, c.stream, c.img[1 - ti], c.conv, c.mask,
                      c.diff, c.w, c.h, &c.sc->kernSum, scaleConv);
+  prof_end(c, XB_PROF_SUB);
   return hipGetLastError();
 }
 

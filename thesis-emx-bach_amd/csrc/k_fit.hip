@@ -1,6 +1,8 @@
 // k_fit.hip — kernel fit: per-stamp test solves, global normal equations,
 // FP64 LU, chi2 merits and the clip-refit checks (the reference's cd + ksc).
 //
+// The global solve lives in k_lu.hip.
+//
 // LU note.  The reference factorises with Numerical Recipes' Crout ludcmp
 // (implicit scaling, partial pivoting; src/bachUtil.cpp:319-384 on the host,
 // cl_kern/bach.cl:68-154 per stamp).  Crout forms element (i,l) as
@@ -295,126 +297,6 @@ __global__ __launch_bounds__(256) void k_global_build(BuildArgs A) {
   } else if (rr >= 0) {
     A.rhs[rr] = acc;
   }
-}
-
-// ---------------------------------------------------------------------------
-// Global FP64 LU (NR ludcmp semantics incl. the zero-row early return,
-// src/bachUtil.cpp:319-384) on the 292x292 system stored 293x293 row-major.
-// One 1024-thread workgroup; right-looking steps (see the note on top).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_global_lu(double* M, int* index, double* vvOut,
-                                                    int* status) {
-  const int n = MATSIZE, N = MATN;
-  const int tid = threadIdx.x;
-  __shared__ double vv[MATN];
-  __shared__ double colL[MATN], rowU[MATN];
-  __shared__ double wv[16];
-  __shared__ int wi[16];
-  __shared__ int s_zero, s_maxI;
-  if (tid == 0) { s_zero = 0; s_maxI = 0; }
-  for (int i = tid; i < N; i += blockDim.x) index[i] = 0;
-  __syncthreads();
-  for (int i = 1 + tid; i <= n; i += blockDim.x) {
-    double big = 0.0;
-    for (int j = 1; j <= n; j++) {
-      double t = fabs(M[(size_t)i * N + j]);
-      if (t > big) big = t;
-    }
-    if (big == 0.0) atomicOr(&s_zero, 1);
-    vv[i] = 1.0 / big;
-  }
-  __syncthreads();
-  if (s_zero) {
-    if (tid == 0) *status = 1;
-    return;  // ludcmp returns 1 before factorising
-  }
-  const int lane = lane_id(), wid = tid / WAVE, nw = blockDim.x / WAVE;
-  for (int j = 1; j <= n; j++) {
-    double bv = -1.0;
-    int bi = -1;
-    for (int i = j + tid; i <= n; i += blockDim.x) {
-      double dum = vv[i] * fabs(M[(size_t)i * N + j]);
-      if (dum >= 0.0 && dum >= bv) { bv = dum; bi = i; }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      double ov = __shfl_xor(bv, o);
-      int oi = __shfl_xor(bi, o);
-      if (ov > bv || (ov == bv && oi > bi)) { bv = ov; bi = oi; }
-    }
-    if (lane == 0) { wv[wid] = bv; wi[wid] = bi; }
-    __syncthreads();
-    if (tid == 0) {
-      double b2 = wv[0];
-      int i2 = wi[0];
-      for (int q = 1; q < nw; q++)
-        if (wv[q] > b2 || (wv[q] == b2 && wi[q] > i2)) { b2 = wv[q]; i2 = wi[q]; }
-      if (i2 >= 0) s_maxI = i2;
-    }
-    __syncthreads();
-    const int maxI = s_maxI;
-    if (j != maxI) {
-      for (int k = 1 + tid; k <= n; k += blockDim.x) {
-        double t = M[(size_t)maxI * N + k];
-        M[(size_t)maxI * N + k] = M[(size_t)j * N + k];
-        M[(size_t)j * N + k] = t;
-      }
-      if (tid == 0) vv[maxI] = vv[j];
-    }
-    __syncthreads();
-    if (tid == 0) {
-      index[j] = maxI;
-      if (M[(size_t)j * N + j] == 0.0) M[(size_t)j * N + j] = 1.0e-20;
-    }
-    __syncthreads();
-    if (j != n) {
-      const double dum = 1.0 / M[(size_t)j * N + j];
-      for (int i = j + 1 + tid; i <= n; i += blockDim.x) {
-        double v = M[(size_t)i * N + j] * dum;
-        M[(size_t)i * N + j] = v;
-        colL[i] = v;
-      }
-      for (int l = j + 1 + tid; l <= n; l += blockDim.x) rowU[l] = M[(size_t)j * N + l];
-      __syncthreads();
-      const int m = n - j;
-      for (int e = tid; e < m * m; e += blockDim.x) {
-        int i = j + 1 + e / m, l = j + 1 + e % m;
-        size_t a = (size_t)i * N + l;
-        M[a] = M[a] - colL[i] * rowU[l];
-      }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) *status = 0;
-  (void)vvOut;
-}
-
-// lubksb (src/bachUtil.cpp:386-411) on one lane: the ascending-j sums of both
-// substitutions are sequential chains in the reference order.
-__global__ __launch_bounds__(64) void k_global_trsv(const double* M, const int* index,
-                                                    double* b, double* out) {
-  if (threadIdx.x != 0) return;
-  const int n = MATSIZE, N = MATN;
-  int ii = 0;
-  for (int i = 1; i <= n; i++) {
-    int ip = index[i];
-    double sum = b[ip];
-    b[ip] = b[i];
-    if (ii) {
-      const double* row = M + (size_t)i * N;
-      for (int j = ii; j <= i - 1; j++) sum -= row[j] * b[j];
-    } else if (sum != 0.0) {
-      ii = i;
-    }
-    b[i] = sum;
-  }
-  for (int i = n; i >= 1; i--) {
-    const double* row = M + (size_t)i * N;
-    double sum = b[i];
-    for (int j = i + 1; j <= n; j++) sum -= row[j] * b[j];
-    b[i] = sum / row[i];
-  }
-  if (out)
-    for (int i = 0; i < NSOL; i++) out[i] = b[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -743,15 +625,6 @@ hipError_t launch_global_build(Ctx& c, int im, bool cdForm) {
   A.rhs = c.rhs;
   const int total = MATN * (MATN + 1) / 2 + NSOL;
   hipLaunchKernelGGL(k_global_build, dim3((total + 255) / 256), dim3(256), 0, c.stream, A);
-  return hipGetLastError();
-}
-
-hipError_t launch_global_solve(Ctx& c, double* d_out) {
-  hipLaunchKernelGGL(k_global_lu, dim3(1), dim3(1024), 0, c.stream, c.M, c.luIndex, c.luVV,
-                     &c.sc->status);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_global_trsv, dim3(1), dim3(64), 0, c.stream, c.M, c.luIndex, c.rhs, d_out);
   return hipGetLastError();
 }
 

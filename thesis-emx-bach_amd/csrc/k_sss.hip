@@ -588,9 +588,11 @@ __global__ void k_reset_skip(uint16_t* mask, size_t n) {
 
 hipError_t launch_sss_stats(Ctx& c, int im) {
   if ((c.fsw * c.fsw + 31) / 32 > MAX_CLIP_BLOCKS) return hipErrorInvalidValue;
+  prof_begin(c, XB_PROF_SSS);
   hipLaunchKernelGGL(k_sss_stats, dim3(c.nGrid), dim3(256), 0, c.stream, c.img[im], c.mask, c.w,
                      c.h, c.stampsx, c.stampsy, c.fsw, c.ran1, c.goodPix, c.m.sigClipAlpha,
                      c.m.iqRange, c.st[im].sky, c.st[im].fwhm);
+  prof_end(c, XB_PROF_SSS);
   return hipGetLastError();
 }
 
@@ -614,7 +616,9 @@ hipError_t launch_sss_find(Ctx& c, int im) {
     A.skipMask = XB_MASK_SKIP_S;
   }
   A.badMask = badMask;
+  prof_begin(c, XB_PROF_SSS);
   hipLaunchKernelGGL(k_sss_find, dim3(c.nGrid), dim3(256), 0, c.stream, A);
+  prof_end(c, XB_PROF_SSS);
   return hipGetLastError();
 }
 

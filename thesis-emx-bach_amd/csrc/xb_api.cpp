@@ -242,6 +242,7 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
     XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
     XB_HIP(hipMalloc((void**)&c.luIndex, sizeof(int) * MATN));
     XB_HIP(hipMalloc((void**)&c.luVV, sizeof(double) * MATN));
+    XB_HIP(hipMalloc((void**)&c.luState, 64));
   }
   static_assert(sizeof(Basis) < (1 << 20), "basis size");
   Basis hb;
@@ -418,7 +419,7 @@ void freeAll(Ctx& c) {
   dfree(&c.ran1);
   dfree(&c.goodPix);
   dfree(&c.M); dfree(&c.rhs); dfree(&c.sol); dfree(&c.cdSol);
-  dfree(&c.luIndex); dfree(&c.luVV);
+  dfree(&c.luIndex); dfree(&c.luVV); dfree(&c.luState);
   dfree(&c.tileK); dfree(&c.tileK64); dfree(&c.tileAks);
 }
 
@@ -427,7 +428,59 @@ int setDev(Ctx& c) {
   return XB_OK;
 }
 
+hipEvent_t takeEvent(Ctx& c) {
+  if (!c.evPool.empty()) {
+    hipEvent_t e = c.evPool.back();
+    c.evPool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Resolve recorded event pairs into the per-slot totals.
+int drainProfile(Ctx& c) {
+  if (c.evPending.empty()) return XB_OK;
+  XB_HIP(hipStreamSynchronize(c.stream));
+  for (auto& p : c.evPending) {
+    float ms = 0.0f;
+    XB_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    c.profMs[p.slot] += ms;
+    c.profCnt[p.slot] += 1;
+    c.evPool.push_back(p.a);
+    c.evPool.push_back(p.b);
+  }
+  c.evPending.clear();
+  return XB_OK;
+}
+
 }  // namespace
+
+namespace xb {
+void prof_begin(Ctx& c, int slot) {
+  if (!c.profOn || slot < 0 || slot >= XB_PROF_N) return;
+  hipEvent_t e = takeEvent(c);
+  if (!e) return;
+  if (hipEventRecord(e, c.stream) != hipSuccess) {
+    c.evPool.push_back(e);
+    return;
+  }
+  c.evOpen[slot] = e;
+}
+
+void prof_end(Ctx& c, int slot) {
+  if (!c.profOn || slot < 0 || slot >= XB_PROF_N || !c.evOpen[slot]) return;
+  hipEvent_t e = takeEvent(c);
+  if (!e) return;
+  if (hipEventRecord(e, c.stream) != hipSuccess) {
+    c.evPool.push_back(e);
+    return;
+  }
+  c.evPending.push_back({slot, c.evOpen[slot], e});
+  c.evOpen[slot] = nullptr;
+}
+}  // namespace xb
 
 // ============================================================================
 // C ABI
@@ -498,6 +551,13 @@ int xb_ctx_destroy(xb_ctx* x) {
   Ctx& c = x->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
+  for (auto& p : c.evPending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c.evOpen)
+    if (e) (void)hipEventDestroy(e);
+  for (auto e : c.evPool) (void)hipEventDestroy(e);
   freeAll(c);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete x;
@@ -568,6 +628,66 @@ static int getImage(xb_ctx* x, const void* d, void* out, size_t elem) {
 int xb_get_conv(xb_ctx* x, float* out) { return x ? getImage(x, x->c.conv, out, 4) : XB_ERR_ARG; }
 int xb_get_diff(xb_ctx* x, float* out) { return x ? getImage(x, x->c.diff, out, 4) : XB_ERR_ARG; }
 int xb_get_mask(xb_ctx* x, uint16_t* out) { return x ? getImage(x, x->c.mask, out, 2) : XB_ERR_ARG; }
+
+int xb_buf_alloc(xb_ctx* x, size_t bytes, void** d_out) {
+  if (!x || !d_out) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  *d_out = nullptr;
+  int rc = setDev(c);
+  if (rc) return rc;
+  XB_HIP(hipMalloc(d_out, bytes ? bytes : 1));
+  return XB_OK;
+}
+
+int xb_buf_free(xb_ctx* x, void* d) {
+  if (!x) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  if (!d) return XB_OK;
+  XB_HIP(hipStreamSynchronize(c.stream));
+  XB_HIP(hipFree(d));
+  return XB_OK;
+}
+
+int xb_buf_upload(xb_ctx* x, void* d, const void* host, size_t bytes) {
+  if (!x || !d || !host) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  XB_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, c.stream));
+  XB_HIP(hipStreamSynchronize(c.stream));
+  return XB_OK;
+}
+
+int xb_buf_download(xb_ctx* x, void* host, const void* d, size_t bytes) {
+  if (!x || !d || !host) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  XB_HIP(hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, c.stream));
+  XB_HIP(hipStreamSynchronize(c.stream));
+  return XB_OK;
+}
+
+int xb_profile_enable(xb_ctx* x, int enable) {
+  if (!x) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  int rc = drainProfile(c);
+  if (rc) return rc;
+  c.profOn = enable != 0;
+  for (int i = 0; i < XB_PROF_N; i++) {
+    c.profMs[i] = 0.0;
+    c.profCnt[i] = 0;
+  }
+  return XB_OK;
+}
+
+int xb_profile_get(xb_ctx* x, double* ms, int* counts, int n) {
+  if (!x) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  int rc = drainProfile(c);
+  if (rc) return rc;
+  for (int i = 0; i < n && i < XB_PROF_N; i++) {
+    if (ms) ms[i] = c.profMs[i];
+    if (counts) counts[i] = c.profCnt[i];
+  }
+  return XB_OK;
+}
 
 int xb_device_outputs(xb_ctx* x, const void** d_conv, const void** d_diff) {
   if (!x) return XB_ERR_ARG;

@@ -121,7 +121,8 @@ struct Ctx {
   double* rhs = nullptr;        // [NSOL] right-hand side, then the solution
   double* sol = nullptr;        // [NSOL] current kernel solution
   double* cdSol = nullptr;      // [2][NSOL]
-  int* luIndex = nullptr;       // [MATN]
+  int* luIndex = nullptr;       // [MATN] pivot rows
+  void* luState = nullptr;      // LU bookkeeping (zero row, last pivot, ii)
   double* luVV = nullptr;       // [MATN]
   float* tileK = nullptr;       // [tiles][KK] FP32 tile kernels
   double* tileK64 = nullptr;    // [tiles][KK] FP64 tile kernels (mask test)
@@ -136,7 +137,23 @@ struct Ctx {
   int retried = 0;
   bool haveResult = false;
   double stageMs[9] = {0};
+  // per-kernel HIP-event timing (xb_profile_*)
+  bool profOn = false;
+  struct PendingEv {
+    int slot;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> evPool;
+  std::vector<PendingEv> evPending;
+  hipEvent_t evOpen[XB_PROF_N] = {};
+  double profMs[XB_PROF_N] = {0};
+  int profCnt[XB_PROF_N] = {0};
 };
+
+// Bracket a launch group with HIP events on the context's stream (no-ops
+// unless profiling is enabled).
+void prof_begin(Ctx& c, int slot);
+void prof_end(Ctx& c, int slot);
 
 // ---- launchers (defined in the .hip files) --------------------------------
 hipError_t launch_mask_input(Ctx& c);

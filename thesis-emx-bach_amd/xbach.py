@@ -283,8 +283,18 @@ def lib() -> C.CDLL:
     L.xb_pipeline_run.argtypes = [vp, P(Params), C.c_int, C.c_int, vp, vp, vp, vp, vp]
     L.xb_pipeline_run_device.argtypes = [vp, P(Params), C.c_int, C.c_int, vp, vp]
     L.xb_synth_pair.argtypes = [P(SynthCfg), vp, vp]
+    L.xb_buf_alloc.argtypes = [vp, C.c_size_t, P(vp)]
+    L.xb_buf_free.argtypes = [vp, vp]
+    L.xb_buf_upload.argtypes = [vp, vp, vp, C.c_size_t]
+    L.xb_buf_download.argtypes = [vp, vp, vp, C.c_size_t]
+    L.xb_profile_enable.argtypes = [vp, C.c_int]
+    L.xb_profile_get.argtypes = [vp, vp, vp, C.c_int]
     _lib = L
     return L
+
+
+# per-kernel timing slots (include/xbach.h XB_PROF_*)
+PROF_SLOTS = ["conv", "sub", "solve", "fill", "sss", "mask"]
 
 
 def synth_pair(cfg: SynthCfg) -> tuple[np.ndarray, np.ndarray]:
@@ -416,6 +426,33 @@ class Context:
         self.w, self.h = w, h
         self._check(self._L.xb_pipeline_run_device(self._h, C.byref(params), w, h,
                                                    d_tmpl, d_sci), "xb_pipeline_run_device")
+
+    # ---- device buffers / profiling -----------------------------------------
+    def buf_alloc(self, nbytes: int) -> int:
+        d = C.c_void_p()
+        self._check(self._L.xb_buf_alloc(self._h, nbytes, C.byref(d)), "xb_buf_alloc")
+        return d.value
+
+    def buf_free(self, d: int):
+        self._check(self._L.xb_buf_free(self._h, d), "xb_buf_free")
+
+    def buf_upload(self, d: int, arr: np.ndarray):
+        a = np.ascontiguousarray(arr)
+        self._check(self._L.xb_buf_upload(self._h, d, a.ctypes.data, a.nbytes), "xb_buf_upload")
+
+    def buf_download(self, arr: np.ndarray, d: int):
+        self._check(self._L.xb_buf_download(self._h, arr.ctypes.data, d, arr.nbytes),
+                    "xb_buf_download")
+
+    def profile(self, enable: bool = True):
+        self._check(self._L.xb_profile_enable(self._h, 1 if enable else 0), "xb_profile_enable")
+
+    def profile_get(self) -> dict:
+        ms = np.zeros(len(PROF_SLOTS))
+        cnt = np.zeros(len(PROF_SLOTS), np.int32)
+        self._check(self._L.xb_profile_get(self._h, ms.ctypes.data, cnt.ctypes.data,
+                                           len(PROF_SLOTS)), "xb_profile_get")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_SLOTS)}
 
     def device_outputs(self) -> tuple[int, int]:
         c, d = C.c_void_p(), C.c_void_p()
