@@ -166,6 +166,13 @@ int xb_pipeline_run(xb_ctx* ctx, const xb_params* p, int w, int h,
 int xb_pipeline_run_device(xb_ctx* ctx, const xb_params* p, int w, int h,
                            const void* d_tmpl, const void* d_sci);
 
+/* ---- the global FP64 solve on its own: ludcmp + lubksb for the 292-unknown
+ * system (include/bachUtil.h:33-36, src/bachUtil.cpp:319-411), 1-based as
+ * the reference: a is 293x293 row-major (row/column 0 unused), b has 298
+ * entries (1..292 used), x receives 298 entries.  Bit-identical to the
+ * reference's arithmetic, including its zero-row early return. */
+int xb_lu_solve(xb_ctx* ctx, const double* a, const double* b, double* x);
+
 /* ---- device buffers (replaces the cl::Buffer handling of the reference,
  *      src/bach.cpp:29-39; owned by the caller, freed with xb_buf_free) ---- */
 int xb_buf_alloc(xb_ctx* ctx, size_t bytes, void** d_out);

@@ -1,0 +1,107 @@
+"""The GPU global solve (xb_lu_solve) against the oracle's Numerical Recipes
+ludcmp + lubksb (src/bachUtil.cpp:319-411) on edge cases: bitwise equal
+results (NaN positions equal, NaN payloads free)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import _oracle
+import xbach
+
+pytestmark = pytest.mark.gpu
+N = 292
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = xbach.Context(0)
+    yield c
+    c.close()
+
+
+def oracle_solve(a, b):
+    A = np.ascontiguousarray(a, np.float64).copy()
+    B = np.zeros(N + 1)
+    B[:] = b[: N + 1]
+    idx = np.zeros(N + 1, np.int32)
+    _oracle.lib().xo_lu_solve(A.ctypes.data, N, B.ctypes.data, idx.ctypes.data)
+    return B
+
+
+def same(x, y):
+    x = np.asarray(x[1:N + 1])
+    y = np.asarray(y[1:N + 1])
+    nx, ny = np.isnan(x), np.isnan(y)
+    return bool(np.array_equal(nx, ny) and np.array_equal(x[~nx].view(np.uint64),
+                                                          y[~ny].view(np.uint64)))
+
+
+def padded(m):
+    a = np.zeros((N + 1, N + 1))
+    a[1:, 1:] = m
+    return a
+
+
+def rhs(rng):
+    b = np.zeros(298)
+    b[1:N + 1] = rng.normal(size=N)
+    return b
+
+
+def test_normal_matrix_like(ctx):
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(400, N)) * np.logspace(0, 5, N)
+    a = padded(x.T @ x)
+    b = rhs(rng)
+    assert same(ctx.lu_solve(a, b), oracle_solve(a, b))
+
+
+def test_general_matrix_with_pivoting(ctx):
+    rng = np.random.default_rng(2)
+    m = rng.normal(size=(N, N))
+    m[np.arange(0, N, 3), np.arange(0, N, 3)] = 0.0
+    a = padded(m)
+    b = rhs(rng)
+    assert same(ctx.lu_solve(a, b), oracle_solve(a, b))
+
+
+def test_pivot_ties_pick_last_row(ctx):
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(N)
+    m = np.zeros((N, N))
+    m[np.arange(N), perm] = 2.0 ** rng.integers(-4, 4, N)
+    m[:, 7] += 1.0  # ties between rows in several columns
+    a = padded(m)
+    b = rhs(rng)
+    assert same(ctx.lu_solve(a, b), oracle_solve(a, b))
+
+
+def test_zero_row_returns_before_factorising(ctx):
+    rng = np.random.default_rng(4)
+    m = rng.normal(size=(N, N))
+    m[100, :] = 0.0
+    a = padded(m)
+    b = rhs(rng)
+    assert same(ctx.lu_solve(a, b), oracle_solve(a, b))
+
+
+@pytest.mark.parametrize("col", [1, 2, 40, 200, 292])
+def test_nan_column_replays_reference_pivoting(ctx, col):
+    rng = np.random.default_rng(col)
+    m = rng.normal(size=(N, N)) + 3 * np.eye(N)
+    m[:, col - 1] = np.nan
+    a = padded(m)
+    b = rhs(rng)
+    assert same(ctx.lu_solve(a, b), oracle_solve(a, b))
+
+
+def test_rhs_leading_zeros(ctx):
+    # lubksb's ii rule: substitution starts at the first nonzero entry
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=(400, N))
+    a = padded(x.T @ x)
+    b = rhs(rng)
+    b[1:60] = 0.0
+    b[5] = -0.0
+    assert same(ctx.lu_solve(a, b), oracle_solve(a, b))

@@ -7,8 +7,12 @@
 // convolution loop's order (cl_kern/conv.cl:409-426).
 //
 // Convolution (cl_kern/conv.cl:31-97): one wave per tile, lane = (column,
-// 7-row strip); the tile kernel is wave-uniform (scalar loads), image columns
-// stream through registers, FP32 FMA accumulation.  The output mask needs
+// 7-row strip); the tile kernel is wave-uniform (scalar loads), image rows
+// stream through registers, FP64 accumulation in the reference's term order
+// (multiply and add unfused), so the convolved and difference images equal
+// the CPU restatement bit for bit.  (With packed FP32 disabled an FP64 VALU
+// op issues at the FP32 rate, so the FP64 form costs the same instruction
+// count as an FP32 one; its price is the unfused multiply-add.)  The output mask needs
 // sum |k| over unmasked pixels only when the window holds a flagged pixel; that
 // rare case re-walks the window in the reference order in FP64 so the
 // BAD_CONV decision is exact.  maskAfterConv (conv.cl:99-113) is fused.
@@ -101,6 +105,7 @@ struct ConvArgs {
   const float* S;      // target image
   uint16_t* mask;
   float* out;
+  double* outD;        // FP64 convolved image (what sub reads, as the reference)
   const float* tileK;
   const double* tileK64;
   const double* tileAks;
@@ -144,28 +149,35 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   const int x = x0 + cx;
   const int y0 = y0t + rg * CR;
   const bool laneOn = rg < 3;
-  const float* K = A.tileK + (size_t)tile * KK;
+  // All lanes share the tile, and the kernel index of every term depends
+  // only on the loop counters, so K is wave-uniform (scalar loads).
+  const double* K = A.tileK64 + (size_t)tile * KK;
   const int w = A.w;
-  float acc[CR];
+  double acc[CR];
 #pragma unroll
-  for (int r = 0; r < CR; r++) acc[r] = 0.0f;
-  unsigned rowBad = 0;  // bit q: a flagged pixel in window row q (27 rows)
+  for (int r = 0; r < CR; r++) acc[r] = 0.0;
+  unsigned rowBad = 0;  // bit jr: a flagged pixel in window row jr (27 rows)
   const int xc = min(x, w - 1);
-  for (int ik = 0; ik < KW; ik++) {
-    const int xi = min(max(xc - ik + HKW, 0), w - 1);
-    float col[CR + 2 * HKW];
+  // Reference order (conv.cl:409-426): image rows ascending, then columns
+  // ascending; acc = acc + k*t without fusion.  Output row r (y = y0 + r)
+  // takes window rows jr = r .. r+20 with kernel row jk = r - jr + 20.
+  for (int jr = 0; jr < CR + 2 * HKW; jr++) {
+    const int yy = min(y0 - HKW + jr, A.h - 1);
+    const float* Trow = A.T + (size_t)yy * w;
+    double row[KW];
 #pragma unroll
-    for (int q = 0; q < CR + 2 * HKW; q++) {
-      int yy = min(y0 - HKW + q, A.h - 1);
-      float v = A.T[(size_t)xi + (size_t)yy * w];
-      col[q] = v;
-      if (conv_flags(v, A.hi, A.lo)) rowBad |= 1u << q;
+    for (int ii = 0; ii < KW; ii++) {
+      const int xi = min(max(xc - HKW + ii, 0), w - 1);
+      const float v = Trow[xi];
+      row[ii] = (double)v;
+      if (conv_flags(v, A.hi, A.lo)) rowBad |= 1u << jr;
     }
 #pragma unroll
-    for (int jk = 0; jk < KW; jk++) {
-      const float kv = K[ik + jk * KW];
+    for (int r = 0; r < CR; r++) {
+      if (jr < r || jr > r + 2 * HKW) continue;  // uniform
+      const double* Kr = K + (r - jr + 2 * HKW) * KW;
 #pragma unroll
-      for (int r = 0; r < CR; r++) acc[r] = __builtin_fmaf(kv, col[r - jk + 2 * HKW], acc[r]);
+      for (int ii = 0; ii < KW; ii++) acc[r] = acc[r] + Kr[2 * HKW - ii] * row[ii];
     }
   }
   if (!laneOn || x >= w - HKW) return;
@@ -176,7 +188,10 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
     const int y = y0 + r;
     if (y >= A.h - HKW) break;
     const size_t id = (size_t)y * w + x;
-    double v = ((double)acc[r] + background_c(A.sol, x, y, A.w, A.h)) * inv;
+    double v = acc[r];
+    v = v + background_c(A.sol, x, y, A.w, A.h);
+    v = v * inv;
+    A.outD[id] = v;
     A.out[id] = (float)v;
     const uint16_t cm = conv_flags(A.T[id], A.hi, A.lo);
     uint16_t nm = cm;
@@ -205,7 +220,8 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
 // border pixels: conv = 1e-30, mask keeps its value plus maskAfterConv flags
 __global__ __launch_bounds__(256) void k_conv_border(const float* __restrict__ S,
                                                      uint16_t* __restrict__ mask,
-                                                     float* __restrict__ out, int w, int h,
+                                                     float* __restrict__ out,
+                                                     double* __restrict__ outD, int w, int h,
                                                      double hi, double lo) {
   const size_t n = (size_t)w * h;
   for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
@@ -214,12 +230,13 @@ __global__ __launch_bounds__(256) void k_conv_border(const float* __restrict__ S
     bool interior = x >= HKW && x < w - HKW && y >= HKW && y < h - HKW;
     if (interior) continue;
     out[id] = 1e-30f;
+    outD[id] = 1e-30;
     mask[id] = mask[id] | after_flags(S[id], hi, lo);
   }
 }
 
 __global__ __launch_bounds__(256) void k_sub(const float* __restrict__ S,
-                                             const float* __restrict__ I,
+                                             const double* __restrict__ I,
                                              const uint16_t* __restrict__ mask,
                                              float* __restrict__ D, int w, int h,
                                              const double* kernSum, int scaleConv) {
@@ -233,7 +250,7 @@ __global__ __launch_bounds__(256) void k_sub(const float* __restrict__ S,
     double d = 1e-30;
     if (x >= HKW && x < w - HKW && y >= HKW && y < h - HKW)
       if ((mask[id] & XB_MASK_BAD_OUTPUT) == 0)
-        d = ((double)I[id] * convFactor - (double)S[id]) * finalFactor;
+        d = (I[id] * convFactor - (double)S[id]) * finalFactor;
     D[id] = (float)d;
   }
 }
@@ -252,6 +269,7 @@ hipError_t launch_conv(Ctx& c) {
   A.S = c.img[1 - ti];
   A.mask = c.mask;
   A.out = c.conv;
+  A.outD = c.convD;
   A.tileK = c.tileK;
   A.tileK64 = c.tileK64;
   A.tileAks = c.tileAks;
@@ -270,7 +288,7 @@ hipError_t launch_conv(Ctx& c) {
   prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_conv_border, dim3(1024), dim3(256), 0, c.stream, c.img[1 - ti], c.mask,
-                     c.conv, c.w, c.h, c.m.threshHigh, c.m.threshLow);
+                     c.conv, c.convD, c.w, c.h, c.m.threshHigh, c.m.threshLow);
   return hipGetLastError();
 }
 
@@ -280,7 +298,7 @@ hipError_t launch_sub(Ctx& c) {
   size_t n = (size_t)c.w * c.h;
   unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
   prof_begin(c, XB_PROF_SUB);
-  hipLaunchKernelGGL(k_sub, dim3(blocks), dim3(256), 0, c.stream, c.img[1 - ti], c.conv, c.mask,
+  hipLaunchKernelGGL(k_sub, dim3(blocks), dim3(256), 0, c.stream, c.img[1 - ti], c.convD, c.mask,
                      c.diff, c.w, c.h, &c.sc->kernSum, scaleConv);
   prof_end(c, XB_PROF_SUB);
   return hipGetLastError();

@@ -189,6 +189,8 @@ struct BuildArgs {
   int cdForm;
   double* M;
   double* rhs;
+  double* M0;    // untouched copies for the LU's replay paths
+  double* rhs0;
 };
 
 __device__ __forceinline__ void stamp_weights(int w, int h, int x, int y, double* wt) {
@@ -294,8 +296,11 @@ __global__ __launch_bounds__(256) void k_global_build(BuildArgs A) {
   if (r >= 0) {
     A.M[(size_t)r * MATN + c] = acc;
     A.M[(size_t)c * MATN + r] = acc;
+    A.M0[(size_t)r * MATN + c] = acc;
+    A.M0[(size_t)c * MATN + r] = acc;
   } else if (rr >= 0) {
     A.rhs[rr] = acc;
+    A.rhs0[rr] = acc;
   }
 }
 
@@ -623,6 +628,8 @@ hipError_t launch_global_build(Ctx& c, int im, bool cdForm) {
   A.cdForm = cdForm ? 1 : 0;
   A.M = c.M;
   A.rhs = c.rhs;
+  A.M0 = c.M0;
+  A.rhs0 = c.rhs0;
   const int total = MATN * (MATN + 1) / 2 + NSOL;
   hipLaunchKernelGGL(k_global_build, dim3((total + 255) / 256), dim3(256), 0, c.stream, A);
   return hipGetLastError();

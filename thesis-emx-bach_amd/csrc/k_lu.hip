@@ -32,10 +32,12 @@ constexpr int NPANEL = (MATSIZE + LB - 1) / LB;
 constexpr int LU_THREADS = 256;
 
 struct LuState {
-  int zero;
-  int lastMaxI;
-  int ii;
-  int pad;
+  int zero;      // a zero row: ludcmp returned 1 without factorising
+  int lastMaxI;  // pivot row of the previous column (kept when none qualifies)
+  int ii;        // lubksb's first nonzero index
+  int fallback;  // a column without a finite pivot kept an earlier row: replay
+  int skip;      // empty stamp list: the reference never solves
+  int pad[3];
 };
 
 // Factorise the panel held in T (rows r0..n of the matrix at T rows
@@ -74,6 +76,14 @@ __device__ void panel_factor(double (*T)[LB + 1], int off, int c0, int cB, doubl
     }
     __syncthreads();
     const int maxI = s_maxI;
+    if (maxI < k) {
+      // No row qualified and the kept pivot row lies above this column: the
+      // reference swaps with an already-factorised row, which only a literal
+      // replay reproduces (k_lu_fallback).
+      if (tid == 0) st->fallback = 1;
+      __syncthreads();
+      return;
+    }
     if (maxI != k) {
       if (tid < cB) {
         double t = T[off + maxI - c0][tid];
@@ -108,7 +118,7 @@ __device__ void panel_factor(double (*T)[LB + 1], int off, int c0, int cB, doubl
 }
 
 __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, int* piv,
-                                                         LuState* st) {
+                                                         LuState* st, const int* skipIfZero) {
   const int n = MATSIZE, N = MATN;
   const int tid = threadIdx.x;
   __shared__ int s_zero;
@@ -116,6 +126,13 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, 
   __shared__ double vvL[MATSIZE];
   if (tid == 0) s_zero = 0;
   for (int i = tid; i < N; i += blockDim.x) piv[i] = 0;
+  const int skip = (skipIfZero && *skipIfZero == 0) ? 1 : 0;
+  if (tid == 0) {
+    st->skip = skip;
+    st->fallback = 0;
+    st->zero = 0;
+  }
+  if (skip) return;  // testFit returns 666 before solving (cdkscUtil.cpp:101-103)
   __syncthreads();
   for (int i = 1 + tid; i <= n; i += blockDim.x) {
     double big = 0.0;
@@ -152,7 +169,7 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, 
 __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, int* piv,
                                                         double* b, LuState* st, int p,
                                                         int nColBlocks) {
-  if (st->zero) return;
+  if (st->zero || st->skip || st->fallback) return;
   const int n = MATSIZE, N = MATN;
   const int tid = threadIdx.x;
   const int c0 = 1 + p * LB;
@@ -251,14 +268,77 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
 
 // Back substitution (and, after a zero row, the reference's lubksb on the
 // unfactorised matrix with an all-zero index).
-__global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const int* piv, double* b,
+// Literal replay of ludcmp (src/bachUtil.cpp:319-384) on the saved matrix,
+// one thread, only after a column had no finite pivot candidate.
+__global__ __launch_bounds__(64) void k_lu_fallback(double* M, const double* M0, int* piv,
+                                                    LuState* st) {
+  if (!st->fallback || st->skip || st->zero) return;
+  const int n = MATSIZE, N = MATN;
+  for (int i = threadIdx.x; i < N * N; i += blockDim.x) M[i] = M0[i];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double vv[MATN];
+  for (int i = 0; i < N; i++) piv[i] = 0;
+  vv[0] = 0.0;
+  for (int i = 1; i <= n; i++) {
+    double big = 0.0;
+    for (int j = 1; j <= n; j++) {
+      double t = fabs(M[(size_t)i * N + j]);
+      if (t > big) big = t;
+    }
+    vv[i] = 1.0 / big;
+  }
+  int maxI = 0;
+  for (int j = 1; j <= n; j++) {
+    for (int i = 1; i < j; i++) {
+      double sum = M[(size_t)i * N + j];
+      for (int k = 1; k < i; k++) sum -= M[(size_t)i * N + k] * M[(size_t)k * N + j];
+      M[(size_t)i * N + j] = sum;
+    }
+    double big = 0.0;
+    for (int i = j; i <= n; i++) {
+      double sum = M[(size_t)i * N + j];
+      for (int k = 1; k < j; k++) sum -= M[(size_t)i * N + k] * M[(size_t)k * N + j];
+      M[(size_t)i * N + j] = sum;
+      double dum = vv[i] * fabs(sum);
+      if (dum >= big) {
+        big = dum;
+        maxI = i;
+      }
+    }
+    if (j != maxI) {
+      for (int k = 1; k <= n; k++) {
+        double d = M[(size_t)maxI * N + k];
+        M[(size_t)maxI * N + k] = M[(size_t)j * N + k];
+        M[(size_t)j * N + k] = d;
+      }
+      vv[maxI] = vv[j];
+    }
+    piv[j] = maxI;
+    if (M[(size_t)j * N + j] == 0.0) M[(size_t)j * N + j] = 1.0e-20;
+    if (j != n) {
+      double dum = 1.0 / M[(size_t)j * N + j];
+      for (int i = j + 1; i <= n; i++) M[(size_t)i * N + j] *= dum;
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const double* M0,
+                                                   const int* piv, double* b, const double* b0,
                                                    const LuState* st, double* out) {
   const int n = MATSIZE, N = MATN;
   __shared__ double x[NSOL];
-  for (int i = threadIdx.x; i < NSOL; i += blockDim.x) x[i] = b[i];
+  if (st->skip) {
+    for (int i = threadIdx.x; i < NSOL; i += blockDim.x)
+      if (out) out[i] = 0.0;
+    return;
+  }
+  const bool full = st->zero || st->fallback;  // forward pass not fused
+  if (st->zero) M = M0;                         // ludcmp returned before factorising
+  for (int i = threadIdx.x; i < NSOL; i += blockDim.x) x[i] = full ? b0[i] : b[i];
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (st->zero) {
+    if (full) {
       int ii = 0;
       for (int i = 1; i <= n; i++) {
         int ip = piv[i];
@@ -295,11 +375,11 @@ __global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const int* p
   }
 }
 
-hipError_t launch_global_solve(Ctx& c, double* d_out) {
+hipError_t launch_global_solve(Ctx& c, double* d_out, const int* d_skipIfZero) {
   LuState* st = reinterpret_cast<LuState*>(c.luState);
   prof_begin(c, XB_PROF_SOLVE);
   hipLaunchKernelGGL(k_lu_begin, dim3(1), dim3(LU_THREADS), 0, c.stream, c.M, c.luVV, c.luIndex,
-                     st);
+                     st, d_skipIfZero);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   for (int p = 0; p < NPANEL; p++) {
@@ -312,8 +392,11 @@ hipError_t launch_global_solve(Ctx& c, double* d_out) {
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(64), 0, c.stream, c.M, c.luIndex, c.rhs, st,
-                     d_out);
+  hipLaunchKernelGGL(k_lu_fallback, dim3(1), dim3(64), 0, c.stream, c.M, c.M0, c.luIndex, st);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(64), 0, c.stream, c.M, c.M0, c.luIndex, c.rhs,
+                     c.rhs0, st, d_out);
   prof_end(c, XB_PROF_SOLVE);
   return hipGetLastError();
 }

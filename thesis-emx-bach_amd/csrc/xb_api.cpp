@@ -148,6 +148,7 @@ int ensureImage(Ctx& c, int w, int h) {
   XB_HIP(dalloc(&c.img[1], n));
   XB_HIP(dalloc(&c.mask, n));
   XB_HIP(dalloc(&c.conv, n));
+  XB_HIP(dalloc(&c.convD, n));
   XB_HIP(dalloc(&c.diff, n));
   const int xSteps = (w + KW - 1) / KW, ySteps = (h + KW - 1) / KW;
   const int tiles = xSteps * ySteps;
@@ -238,6 +239,8 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
   if (!c.M) {
     XB_HIP(hipMalloc((void**)&c.M, sizeof(double) * MATN * MATN));
     XB_HIP(hipMalloc((void**)&c.rhs, sizeof(double) * NSOL));
+    XB_HIP(hipMalloc((void**)&c.M0, sizeof(double) * MATN * MATN));
+    XB_HIP(hipMalloc((void**)&c.rhs0, sizeof(double) * NSOL));
     XB_HIP(hipMalloc((void**)&c.sol, sizeof(double) * NSOL));
     XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
     XB_HIP(hipMalloc((void**)&c.luIndex, sizeof(int) * MATN));
@@ -345,7 +348,7 @@ int stageCd(Ctx& c, int* convTemplate) {
     XB_HIP(launch_stamp_lu(c, im));
     XB_HIP(launch_cd_select(c, im));
     XB_HIP(launch_global_build(c, im, true));
-    XB_HIP(launch_global_solve(c, cdSol));
+    XB_HIP(launch_global_solve(c, cdSol, &c.sc->nTest[im]));
     XB_HIP(launch_calc_sigs(c, im, true, cdSol));
     XB_HIP(launch_cd_finish(c, im));
   }
@@ -366,7 +369,7 @@ int stageKsc(Ctx& c) {
   int it = 0;
   for (;;) {
     XB_HIP(launch_global_build(c, im, false));
-    XB_HIP(launch_global_solve(c, c.sol));
+    XB_HIP(launch_global_solve(c, c.sol, nullptr));
     XB_HIP(launch_calc_sigs(c, im, false, c.sol));
     XB_HIP(launch_ksc_check(c, im));
     XB_HIP(launch_fill(c, im, c.st[im].list, c.nKept[im]));
@@ -406,6 +409,7 @@ void freeAll(Ctx& c) {
   dfree(&c.img[1]);
   dfree(&c.mask);
   dfree(&c.conv);
+  dfree(&c.convD);
   dfree(&c.diff);
   for (int im = 0; im < 2; im++) {
     DevStamps& s = c.st[im];
@@ -419,6 +423,7 @@ void freeAll(Ctx& c) {
   dfree(&c.ran1);
   dfree(&c.goodPix);
   dfree(&c.M); dfree(&c.rhs); dfree(&c.sol); dfree(&c.cdSol);
+  dfree(&c.M0); dfree(&c.rhs0);
   dfree(&c.luIndex); dfree(&c.luVV); dfree(&c.luState);
   dfree(&c.tileK); dfree(&c.tileK64); dfree(&c.tileAks);
 }
@@ -628,6 +633,33 @@ static int getImage(xb_ctx* x, const void* d, void* out, size_t elem) {
 int xb_get_conv(xb_ctx* x, float* out) { return x ? getImage(x, x->c.conv, out, 4) : XB_ERR_ARG; }
 int xb_get_diff(xb_ctx* x, float* out) { return x ? getImage(x, x->c.diff, out, 4) : XB_ERR_ARG; }
 int xb_get_mask(xb_ctx* x, uint16_t* out) { return x ? getImage(x, x->c.mask, out, 2) : XB_ERR_ARG; }
+
+int xb_lu_solve(xb_ctx* x, const double* a, const double* b, double* out) {
+  if (!x || !a || !b || !out) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  int rc = setDev(c);
+  if (rc) return rc;
+  if (!c.M) {
+    XB_HIP(hipMalloc((void**)&c.M, sizeof(double) * MATN * MATN));
+    XB_HIP(hipMalloc((void**)&c.rhs, sizeof(double) * NSOL));
+    XB_HIP(hipMalloc((void**)&c.M0, sizeof(double) * MATN * MATN));
+    XB_HIP(hipMalloc((void**)&c.rhs0, sizeof(double) * NSOL));
+    XB_HIP(hipMalloc((void**)&c.sol, sizeof(double) * NSOL));
+    XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
+    XB_HIP(hipMalloc((void**)&c.luIndex, sizeof(int) * MATN));
+    XB_HIP(hipMalloc((void**)&c.luVV, sizeof(double) * MATN));
+    XB_HIP(hipMalloc((void**)&c.luState, 64));
+  }
+  const size_t mb = sizeof(double) * MATN * MATN, vb = sizeof(double) * NSOL;
+  XB_HIP(hipMemcpyAsync(c.M, a, mb, hipMemcpyHostToDevice, c.stream));
+  XB_HIP(hipMemcpyAsync(c.M0, a, mb, hipMemcpyHostToDevice, c.stream));
+  XB_HIP(hipMemcpyAsync(c.rhs, b, vb, hipMemcpyHostToDevice, c.stream));
+  XB_HIP(hipMemcpyAsync(c.rhs0, b, vb, hipMemcpyHostToDevice, c.stream));
+  XB_HIP(launch_global_solve(c, c.sol, nullptr));
+  XB_HIP(hipMemcpyAsync(out, c.sol, vb, hipMemcpyDeviceToHost, c.stream));
+  XB_HIP(hipStreamSynchronize(c.stream));
+  return XB_OK;
+}
 
 int xb_buf_alloc(xb_ctx* x, size_t bytes, void** d_out) {
   if (!x || !d_out) return XB_ERR_ARG;

@@ -110,6 +110,7 @@ struct Ctx {
   float* img[2] = {nullptr, nullptr};  // 0 template, 1 science (FP32)
   uint16_t* mask = nullptr;
   float* conv = nullptr;
+  double* convD = nullptr;      // FP64 convolved image (sub reads it)
   float* diff = nullptr;
   DevStamps st[2];
   DevScalars* sc = nullptr;
@@ -119,6 +120,8 @@ struct Ctx {
   float* goodPix = nullptr;     // [N][fsw*fsw] good-pixel scratch of sss
   double* M = nullptr;          // [MATN*MATN] global matrix
   double* rhs = nullptr;        // [NSOL] right-hand side, then the solution
+  double* M0 = nullptr;         // copies of the system as built (LU replay paths)
+  double* rhs0 = nullptr;
   double* sol = nullptr;        // [NSOL] current kernel solution
   double* cdSol = nullptr;      // [2][NSOL]
   int* luIndex = nullptr;       // [MATN] pivot rows
@@ -165,7 +168,9 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax);
 hipError_t launch_stamp_lu(Ctx& c, int im);
 hipError_t launch_cd_select(Ctx& c, int im);
 hipError_t launch_global_build(Ctx& c, int im, bool cdForm);
-hipError_t launch_global_solve(Ctx& c, double* d_out);
+// d_skipIfZero: when it points at a zero count the solve is skipped and
+// d_out is zeroed (the reference returns before solving an empty fit).
+hipError_t launch_global_solve(Ctx& c, double* d_out, const int* d_skipIfZero);
 hipError_t launch_cd_finish(Ctx& c, int im);
 hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol);
 hipError_t launch_ksc_check(Ctx& c, int im);

@@ -283,6 +283,7 @@ def lib() -> C.CDLL:
     L.xb_pipeline_run.argtypes = [vp, P(Params), C.c_int, C.c_int, vp, vp, vp, vp, vp]
     L.xb_pipeline_run_device.argtypes = [vp, P(Params), C.c_int, C.c_int, vp, vp]
     L.xb_synth_pair.argtypes = [P(SynthCfg), vp, vp]
+    L.xb_lu_solve.argtypes = [vp, vp, vp, vp]
     L.xb_buf_alloc.argtypes = [vp, C.c_size_t, P(vp)]
     L.xb_buf_free.argtypes = [vp, vp]
     L.xb_buf_upload.argtypes = [vp, vp, vp, C.c_size_t]
@@ -426,6 +427,18 @@ class Context:
         self.w, self.h = w, h
         self._check(self._L.xb_pipeline_run_device(self._h, C.byref(params), w, h,
                                                    d_tmpl, d_sci), "xb_pipeline_run_device")
+
+    def lu_solve(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        """Global solve (ludcmp + lubksb) of a 293x293 1-based system."""
+        A = np.ascontiguousarray(a, np.float64)
+        B = np.zeros(298)
+        B[: len(b)] = b
+        if A.shape != (293, 293):
+            raise ValueError("expects the 293x293 (1-based) global system")
+        x = np.empty(298)
+        self._check(self._L.xb_lu_solve(self._h, A.ctypes.data, B.ctypes.data, x.ctypes.data),
+                    "xb_lu_solve")
+        return x
 
     # ---- device buffers / profiling -----------------------------------------
     def buf_alloc(self, nbytes: int) -> int:
