@@ -141,26 +141,48 @@ __global__ __launch_bounds__(256) void k_fill_gram(FillArgs A) {
         tgt[i] = A.img2[(size_t)xr + (size_t)yr * A.w];
       }
     __syncthreads();
-    if (tid < WROWS) {
-      const int r = tid;
-      double acc = 0.0;
-      if (valid) {
-        const bool sub = r < NPSF && evenSub[r];
-        for (int x = 0; x < SSW; x++)
-          for (int y = 0; y < SSW; y++) {
-            int k = x + SSW * y;
-            double wv = W[(size_t)r * NPIX + k];
-            if (sub) wv = wv - W[k];
-            acc += wv * (double)tgt[k];
-          }
+    // Bc: column-major pixel order (x outer, y inner): chunks of 64 such
+    // positions staged through LDS, one thread per row summing in order.
+    __shared__ double wc[WROWS][GK + 1];
+    double acc = 0.0;
+    const bool sub = tid < NPSF && evenSub[tid];
+    for (int c0 = 0; c0 < NPIX && valid; c0 += GK) {
+      const int kn = min(GK, NPIX - c0);
+      for (int i = tid; i < WROWS * GK; i += blockDim.x) {
+        int r = i / GK, cc = i % GK;
+        double v = 0.0;
+        if (cc < kn) {
+          int c = c0 + cc, k = (c / SSW) + SSW * (c % SSW);
+          v = W[(size_t)r * NPIX + k];
+          if (r < NPSF && evenSub[r]) v = v - W[k];
+        }
+        wc[r][cc] = v;
       }
-      A.Bc[(size_t)s * WROWS + r] = acc;
-    } else if (tid >= 64 && tid < 64 + NBG) {
-      const int b = tid - 64, r = NPSF + b;
-      double acc = 0.0;
+      __syncthreads();
+      if (tid < WROWS)
+        for (int cc = 0; cc < kn; cc++) {
+          int c = c0 + cc, k = (c / SSW) + SSW * (c % SSW);
+          acc += wc[tid][cc] * (double)tgt[k];
+        }
+      __syncthreads();
+    }
+    (void)sub;
+    if (tid < WROWS) A.Bc[(size_t)s * WROWS + tid] = valid ? acc : 0.0;
+    // Br: row-major order over the background rows, whose values are the
+    // monomials 1, yf, xf written by k_fill_conv (recomputed identically).
+    if (tid >= 64 && tid < 64 + NBG) {
+      const int b = tid - 64;
+      const float hw = (float)A.w * 0.5f, hh = (float)A.h * 0.5f;
+      double a2 = 0.0;
       if (valid)
-        for (int k = 0; k < NPIX; k++) acc += W[(size_t)r * NPIX + k] * (double)tgt[k];
-      A.Br[(size_t)s * NBG + b] = acc;
+        for (int k = 0; k < NPIX; k++) {
+          int x = ssx - HSS + k % SSW, y = ssy - HSS + k / SSW;
+          double xf = (double)(((float)x - hw) / hw);
+          double yf = (double)(((float)y - hh) / hh);
+          double bgv = (b == 2 ? xf : 1.0) * (b == 1 ? yf : 1.0);
+          a2 += bgv * (double)tgt[k];
+        }
+      A.Br[(size_t)s * NBG + b] = a2;
     }
     return;
   }

@@ -30,6 +30,7 @@ namespace xb {
 constexpr int LB = 16;  // panel width = trailing columns per workgroup
 constexpr int NPANEL = (MATSIZE + LB - 1) / LB;
 constexpr int LU_THREADS = 256;
+constexpr int STEP_THREADS = 64;  // one wave: the panel's barriers stay wave-local
 
 struct LuState {
   int zero;      // a zero row: ludcmp returned 1 without factorising
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, 
   for (int i = tid; i < n; i += blockDim.x) vv[1 + i] = vvL[i];
 }
 
-__global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, int* piv,
+__global__ __launch_bounds__(STEP_THREADS) void k_lu_step(double* M, double* vv, int* piv,
                                                         double* b, LuState* st, int p,
                                                         int nColBlocks) {
   if (st->zero || st->skip || st->fallback) return;
@@ -352,21 +353,51 @@ __global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const double
         x[i] = sum;
       }
     }
-    for (int i = n; i >= 1; i--) {
-      const double* row = M + (size_t)i * N;
-      double sum = x[i];
-      int j = i + 1;
-      for (; j + 3 <= n; j += 4) {
-        double p0 = row[j] * x[j], p1 = row[j + 1] * x[j + 1];
-        double p2 = row[j + 2] * x[j + 2], p3 = row[j + 3] * x[j + 3];
-        sum = sum - p0;
-        sum = sum - p1;
-        sum = sum - p2;
-        sum = sum - p3;
-      }
-      for (; j <= n; j++) sum -= row[j] * x[j];
-      x[i] = sum / row[i];
+  }
+  __syncthreads();
+  // Back substitution x_i = (y_i - sum_{j>i} U_ij x_j) / U_ii with the sum
+  // in ascending j (lubksb).  The products need only final x values, so all
+  // lanes form them (row i-1 is prefetched into registers meanwhile); one lane
+  // runs the ordered subtraction chain, the only sequential part.
+  __shared__ double P[MATN];
+  constexpr int PER = (MATSIZE + 63) / 64;  // row entries per lane
+  const int lane = threadIdx.x;
+  double cur[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int j = n + 1 + lane + 64 * q;  // row n has no entries right of the diagonal
+    cur[q] = 0.0;
+    (void)j;
+  }
+  double diag = M[(size_t)n * N + n];
+  for (int i = n; i >= 1; i--) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int j = i + 1 + lane + 64 * q;
+      if (j <= n) P[j] = cur[q] * x[j];
     }
+    // prefetch row i-1 (columns i..n) and its diagonal
+    double nxt[PER];
+    double nd = 0.0;
+    if (i > 1) {
+      const double* r1 = M + (size_t)(i - 1) * N;
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        int j = i + lane + 64 * q;
+        nxt[q] = j <= n ? r1[j] : 0.0;
+      }
+      nd = r1[i - 1];
+    }
+    __syncthreads();
+    if (lane == 0) {
+      double sum = x[i];
+      for (int j = i + 1; j <= n; j++) sum = sum - P[j];
+      x[i] = sum / diag;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) cur[q] = nxt[q];
+    diag = nd;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < NSOL; i += blockDim.x) {
@@ -387,7 +418,7 @@ hipError_t launch_global_solve(Ctx& c, double* d_out, const int* d_skipIfZero) {
     const int cB = std::min(LB, MATSIZE - c0 + 1);
     const int trailing = MATSIZE - (c0 + cB) + 1;
     const int nColBlocks = (trailing + LB - 1) / LB;
-    hipLaunchKernelGGL(k_lu_step, dim3(nColBlocks + 1), dim3(LU_THREADS), 0, c.stream, c.M,
+    hipLaunchKernelGGL(k_lu_step, dim3(nColBlocks + 1), dim3(STEP_THREADS), 0, c.stream, c.M,
                        c.luVV, c.luIndex, c.rhs, st, p, nColBlocks);
     e = hipGetLastError();
     if (e != hipSuccess) return e;

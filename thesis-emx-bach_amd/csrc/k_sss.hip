@@ -302,6 +302,7 @@ struct FindArgs {
   double* ssv;
   double hi, tkf;
   uint16_t badMask, badPixelMask, skipMask;
+  int* candAll;  // [N][fsw*fsw] candidate-list scratch
 };
 
 // Block-wide (value, index) argmax: larger value wins, ties -> smaller index.
@@ -364,8 +365,6 @@ __global__ __launch_bounds__(256) void k_sss_find(FindArgs A) {
   __shared__ int isc[20];
   __shared__ double dsc[20];
   __shared__ int isc2[20];
-  __shared__ int cand[256];
-  __shared__ int s_nc;
   __shared__ int lx[MAXSS], ly[MAXSS];
   __shared__ double lv[MAXSS];
   __shared__ int s_counter;
@@ -373,44 +372,65 @@ __global__ __launch_bounds__(256) void k_sss_find(FindArgs A) {
   __syncthreads();
   double dfrac = 0.9;
   const int nAll = fsw * fsw;
+  // ordered candidate list of the current level (scratch of this stamp)
+  int* cand = A.candAll + (size_t)s * nAll;
+  const int per = (nAll + blockDim.x - 1) / blockDim.x;  // raster range per thread
+  const int p0 = tid * per, p1 = min(nAll, p0 + per);
   // The reference loops until the threshold reaches the floor; with a
   // non-finite sky or sigma that never happens, so the levels are capped
   // (finite statistics end within 6 levels).
   for (int level = 0; level < MAX_LEVELS && s_counter < MAXSS; level++) {
     const double lim = std_max(floorV, sky + (A.hi - sky) * dfrac);
-    for (int base = 0; base < nAll && s_counter < MAXSS; base += blockDim.x) {
-      // classify this chunk against the current mask
-      int k = base + tid;
-      bool isCand = false;
-      if (k < nAll) {
-        int absx = k % fsw + g.ox, absy = k / fsw + g.oy;
-        long a = absx + absy * imgW;
-        uint16_t m = mask[a];
-        if ((m & badMask) == 0) {
-          double v = (double)img[a];
-          if (v > A.hi) {
-            mask[a] = m | A.badPixelMask;
-          } else if (!((v - sky) * (1.0 / fwhm) < A.tkf) && v > lim) {
-            isCand = true;
-          }
+    // Classify every pixel against the mask as it is now.  Mask bits are
+    // only ever added, so a pixel rejected here stays rejected; candidates
+    // are re-checked in raster order below (earlier ones may mark them).
+    // Flagging pixels above threshHigh commutes with everything else.
+    int myCount = 0;
+    for (int k = p0; k < p1; k++) {
+      int absx = k % fsw + g.ox, absy = k / fsw + g.oy;
+      long a = absx + absy * imgW;
+      uint16_t m = mask[a];
+      if ((m & badMask) == 0) {
+        double v = (double)img[a];
+        if (v > A.hi) {
+          mask[a] = m | A.badPixelMask;
+        } else if (!((v - sky) * (1.0 / fwhm) < A.tkf) && v > lim) {
+          myCount++;
         }
       }
-      int tot;
-      int pos = block_prefix_count(isCand, isc, &tot);
-      if (isCand) cand[pos] = k;
-      if (tid == 0) s_nc = tot;
-      __syncthreads();
-      const int nc = s_nc;
-      for (int ci = 0; ci < nc; ci++) {
-        const int kk = cand[ci];
-        const int absx = kk % fsw + g.ox, absy = kk / fsw + g.oy;
-        const long a = absx + absy * imgW;
-        // the scan re-reads the mask at this pixel (earlier candidates may
-        // have marked it)
-        if ((mask[a] & badMask) != 0) {
-          __syncthreads();
+    }
+    int nc;
+    int off = block_prefix_sum(myCount, isc, &nc);
+    for (int k = p0; k < p1 && myCount > 0; k++) {
+      int absx = k % fsw + g.ox, absy = k / fsw + g.oy;
+      long a = absx + absy * imgW;
+      if ((mask[a] & badMask) == 0) {
+        double v = (double)img[a];
+        if (!(v > A.hi) && !((v - sky) * (1.0 / fwhm) < A.tkf) && v > lim) cand[off++] = k;
+      }
+    }
+    __syncthreads();
+    for (int pos = 0; pos < nc && s_counter < MAXSS;) {
+      // first candidate from pos on whose pixel is still unmasked
+      {
+        int q = pos + tid;
+        int ok = INT_MAX;
+        if (q < nc) {
+          int kk = cand[q];
+          long a = (kk % fsw + g.ox) + (kk / fsw + g.oy) * imgW;
+          if ((mask[a] & badMask) == 0) ok = q;
+        }
+        int first = block_min_int(ok, isc);
+        if (first == INT_MAX) {
+          pos += blockDim.x;
           continue;
         }
+        pos = first;
+      }
+      const int kk = cand[pos];
+      pos++;
+      {
+        const int absx = kk % fsw + g.ox, absy = kk / fsw + g.oy;
         // ---- window maximum (sss.cl:427-459)
         const int startX = max(absx - HSS, g.ox), startY = max(absy - HSS, g.oy);
         const int endX = min(absx + HSS, g.ox + fsw - 1), endY = min(absy + HSS, g.oy + fsw - 1);
@@ -616,6 +636,8 @@ hipError_t launch_sss_find(Ctx& c, int im) {
     A.skipMask = XB_MASK_SKIP_S;
   }
   A.badMask = badMask;
+  // the good-pixel scratch of k_sss_stats is free again (same size)
+  A.candAll = reinterpret_cast<int*>(c.goodPix);
   prof_begin(c, XB_PROF_SSS);
   hipLaunchKernelGGL(k_sss_find, dim3(c.nGrid), dim3(256), 0, c.stream, A);
   prof_end(c, XB_PROF_SSS);

@@ -48,6 +48,36 @@ __device__ __forceinline__ int block_prefix_count(bool flag, int* scratch, int* 
   return res;
 }
 
+// Exclusive prefix sum of a non-negative int over the block (thread order).
+// *total receives the block total.  scratch >= 17 ints.
+__device__ __forceinline__ int block_prefix_sum(int v, int* scratch, int* total) {
+  const int tid = threadIdx.x;
+  const int wid = tid / WAVE;
+  const int nw = blockDim.x / WAVE;
+  const int lane = lane_id();
+  int incl = v;
+  for (int o = 1; o < WAVE; o <<= 1) {
+    int t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == WAVE - 1) scratch[wid] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int i = 0; i < nw; i++) {
+      int t = scratch[i];
+      scratch[i] = acc;
+      acc += t;
+    }
+    scratch[16] = acc;
+  }
+  __syncthreads();
+  int res = scratch[wid] + incl - v;
+  *total = scratch[16];
+  __syncthreads();
+  return res;
+}
+
 // Block-wide sum of an int (order-independent).  scratch >= 17 ints.
 __device__ __forceinline__ int block_sum_int(int v, int* scratch) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
