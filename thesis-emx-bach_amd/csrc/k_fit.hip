@@ -210,6 +210,47 @@ __device__ __forceinline__ void stamp_weights(int w, int h, int x, int y, double
 
 constexpr int BUILD_STAMPS_LDS = 256;
 
+// One stamp's contribution to global element (r, c) (r >= c) or to rhs entry
+// rr (r < 0), in the reference's expression forms.
+__device__ __forceinline__ double build_term(const BuildArgs& A, int s, const double* wt, int r,
+                                             int c, int rr) {
+  const double* G = A.G + (size_t)s * WROWS * WROWS;
+  if (r >= 0) {
+    if (r == 0 || c == 0) return 0.0;
+    if (r == 1 && c == 1) return G[0];
+    if (r <= NCOMP + 1) {
+      // basis rows 2..289
+      const int i = r - 2, i1 = i / NC2, i2 = i - i1 * NC2;
+      if (c == 1) return wt[i2] * G[(i1 + 1) * WROWS + 0];
+      const int j = c - 2, j1 = j / NC2, j2 = j - j1 * NC2;
+      return wt[i2] * wt[j2] * G[(i1 + 1) * WROWS + (j1 + 1)];
+    }
+    // background rows 290..292: W row iv = 49 + iBG
+    const int iv = NC1 + (r - (NCOMP + 2)) + 1;
+    if (c == 1) return G[iv * WROWS + 0];
+    if (c <= NCOMP + 1) {
+      const int jj = c - 1;
+      const int i1 = (jj - 1) / NC2 + 1, i2 = (jj - 1) % NC2;
+      return G[iv * WROWS + i1] * wt[i2];
+    }
+    const int jv = NC1 + (c - (NCOMP + 2)) + 1;
+    return G[iv * WROWS + jv];
+  }
+  if (rr >= 0) {
+    const double* Bc = A.Bc + (size_t)s * WROWS;
+    if (rr == 1) return Bc[0];
+    if (rr >= 2 && rr <= NCOMP + 1) {
+      const int i = (rr - 2) / NC2 + 1, j = (rr - 2) % NC2;
+      return Bc[i] * wt[j];
+    }
+    if (rr >= NCOMP + 2 && rr < NCOMP + 2 + NBG) {
+      const int b = rr - (NCOMP + 2);
+      return A.cdForm ? A.Br[(size_t)s * NBG + b] : Bc[NPSF + b];
+    }
+  }
+  return 0.0;
+}
+
 __global__ __launch_bounds__(256) void k_global_build(BuildArgs A) {
   const int n = *A.nList;
   const int nLower = MATN * (MATN + 1) / 2;
@@ -247,50 +288,17 @@ __global__ __launch_bounds__(256) void k_global_build(BuildArgs A) {
       for (int q = 0; q < NC2; q++) wts[u][q] = wt[q];
     }
     __syncthreads();
-    for (int u = 0; u < cn; u++) {
-      if (!svalid[u]) continue;  // exhausted stamps add nothing (zeros)
-      const int s = sidx[u];
-      const double* G = A.G + (size_t)s * WROWS * WROWS;
-      const double* wt = wts[u];
-      if (r >= 0) {
-        if (r == 0 || c == 0) {
-        } else if (r == 1 && c == 1) {
-          acc += G[0];
-        } else if (r <= NCOMP + 1) {
-          // basis rows 2..289
-          int i = r - 2, i1 = i / NC2, i2 = i - i1 * NC2;
-          if (c == 1) {
-            acc += wt[i2] * G[(i1 + 1) * WROWS + 0];
-          } else {
-            int j = c - 2, j1 = j / NC2, j2 = j - j1 * NC2;
-            acc += wt[i2] * wt[j2] * G[(i1 + 1) * WROWS + (j1 + 1)];
-          }
-        } else {
-          // background rows 290..292: W row iv = 49 + iBG
-          int iv = NC1 + (r - (NCOMP + 2)) + 1;
-          if (c == 1) {
-            acc += G[iv * WROWS + 0];
-          } else if (c <= NCOMP + 1) {
-            int jj = c - 1;
-            int i1 = (jj - 1) / NC2 + 1, i2 = (jj - 1) % NC2;
-            acc += G[iv * WROWS + i1] * wt[i2];
-          } else {
-            int jv = NC1 + (c - (NCOMP + 2)) + 1;
-            acc += G[iv * WROWS + jv];
-          }
-        }
-      } else if (rr >= 0) {
-        const double* Bc = A.Bc + (size_t)s * WROWS;
-        if (rr == 1) {
-          acc += Bc[0];
-        } else if (rr >= 2 && rr <= NCOMP + 1) {
-          int i = (rr - 2) / NC2 + 1, j = (rr - 2) % NC2;
-          acc += Bc[i] * wt[j];
-        } else if (rr >= NCOMP + 2 && rr < NCOMP + 2 + NBG) {
-          int b = rr - (NCOMP + 2);
-          acc += A.cdForm ? A.Br[(size_t)s * NBG + b] : Bc[NPSF + b];
-        }
+    // terms of 8 stamps loaded together, then added in stamp order
+    for (int u0 = 0; u0 < cn; u0 += 8) {
+      double t[8];
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        const int u = u0 + v;
+        t[v] = (u < cn && svalid[u]) ? build_term(A, sidx[u], wts[u], r, c, rr) : 0.0;
       }
+#pragma unroll
+      for (int v = 0; v < 8; v++)
+        if (u0 + v < cn && svalid[u0 + v]) acc += t[v];  // exhausted stamps add nothing
     }
   }
   if (r >= 0) {

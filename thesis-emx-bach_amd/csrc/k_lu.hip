@@ -3,34 +3,45 @@
 //
 // Right-looking blocked elimination with the reference's per-element order:
 // element (i,l) receives -L_ik U_kl for k ascending, exactly as Crout forms it,
-// and rows are swapped by the same pivot sequence (implicit scaling vv, `>=`
-// rule: the last row attaining the maximum wins; NaN never wins; no candidate
-// keeps the previous row).  Applying a panel's row swaps to a trailing column
-// before that panel's updates is equivalent to interleaving them, because the
-// swaps only exchange rows that the panel's updates treat identically.
+// and rows are interchanged by the same pivot sequence (implicit scaling vv,
+// `>=` rule: the last row attaining the maximum wins; NaN never wins; no
+// candidate keeps the previous row).  Applying a panel's row swaps to a
+// trailing column before that panel's updates is equivalent to interleaving
+// them, because the swaps only exchange rows that the panel's updates treat
+// identically.
 //
-//   k_lu_begin   vv, zero-row test (ludcmp returns 1 before factorising), and
-//                the first 16-column panel factorised in LDS;
+//   k_lu_scale   vv (ludcmp's implicit scaling) and zero-row flags, a wave
+//                per row;
+//   k_lu_begin   the zero-row test (ludcmp returns 1 before factorising) and
+//                the first 16-column panel;
 //   k_lu_step(p) one workgroup per 16 trailing columns: apply the panel's
 //                swaps, the in-panel triangular update, the rank-16 update of
-//                the rows below (each element summed in k order); workgroup 0
-//                owns the next panel's columns and factorises that panel
-//                right after updating it (one launch per panel).  A last
-//                workgroup carries the right-hand side as one more column:
-//                lubksb's forward substitution, including its `ii` rule
-//                (updates start at the first nonzero entry), fused here;
+//                the rows below (each element summed in k order), with the
+//                factorised panel staged once in LDS; workgroup 0 owns the
+//                next panel's columns and factorises that panel right after
+//                updating it (one launch per panel).  A last workgroup carries
+//                the right-hand side as one more column: lubksb's forward
+//                substitution, including its `ii` rule (updates start at the
+//                first nonzero entry), fused here;
 //   k_lu_backsub back substitution; its ascending-j sums are one sequential
 //                chain per row in the reference order.
+//
+// Panels are factorised by a single wave holding the rows in registers (lane
+// L: rows c1 + L + 64 s).  A row interchange relabels the two rows (logical
+// position numbers) instead of moving them; vv stays attached to its row, as
+// ludcmp's vv[imax] = vv[j] keeps it for every position still searched.  The
+// factorised rows are stored at their final positions.
 #include <algorithm>
 
 #include "xb_dev.h"
 
 namespace xb {
 
-constexpr int LB = 16;  // panel width = trailing columns per workgroup
+constexpr int LB = 16;          // panel width = trailing columns per workgroup
+constexpr int LDP = LB + 1;     // LDS pitch of a 16-column slab
 constexpr int NPANEL = (MATSIZE + LB - 1) / LB;
 constexpr int LU_THREADS = 256;
-constexpr int STEP_THREADS = 64;  // one wave: the panel's barriers stay wave-local
+constexpr int NSLOT = (MATSIZE + WAVE - 1) / WAVE;  // panel rows per lane
 
 struct LuState {
   int zero;      // a zero row: ludcmp returned 1 without factorising
@@ -41,90 +52,216 @@ struct LuState {
   int pad[3];
 };
 
-// Factorise the panel held in T (rows r0..n of the matrix at T rows
-// off..off+R-1, columns 0..cB-1).  c0 = matrix column of T column 0.
-__device__ void panel_factor(double (*T)[LB + 1], int off, int c0, int cB, double* vvL,
-                             int* piv, LuState* st) {
-  const int n = MATSIZE;
-  const int tid = threadIdx.x;
-  const int lane = lane_id(), wid = tid / WAVE, nw = blockDim.x / WAVE;
-  __shared__ double wv[16];
-  __shared__ int wi[16];
-  __shared__ int s_maxI;
-  if (tid == 0) s_maxI = st->lastMaxI;
-  __syncthreads();
-  for (int q = 0; q < cB; q++) {
-    const int k = c0 + q;
-    double bv = -1.0;
-    int bi = -1;
-    for (int i = k + tid; i <= n; i += blockDim.x) {
-      double dum = vvL[i - c0] * fabs(T[off + i - c0][q]);
-      if (dum >= 0.0 && dum >= bv) { bv = dum; bi = i; }
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// One step of a (value, index) maximum across lanes (ties to the larger
+// index), the partner chosen by a DPP control within each row of 16 lanes.
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp(double& bv, int& bi) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(bv), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(bv), CTRL, 0xF, 0xF, false);
+  const int oi = __builtin_amdgcn_update_dpp(0, bi, CTRL, 0xF, 0xF, false);
+  const double ov = __hiloint2double(hi, lo);
+  if (ov > bv || (ov == bv && oi > bi)) {
+    bv = ov;
+    bi = oi;
+  }
+}
+
+// Wave-wide (value, index) maximum: rotations by 1, 2, 4, 8 within each row
+// of 16 lanes (DPP row_ror), then the two cross-row exchanges.  Every lane
+// ends with the result (a maximum under a total order: any combining order).
+__device__ __forceinline__ int wave_argmax(double bv, int bi) {
+  argmax_dpp<0x121>(bv, bi);
+  argmax_dpp<0x122>(bv, bi);
+  argmax_dpp<0x124>(bv, bi);
+  argmax_dpp<0x128>(bv, bi);
+#pragma unroll
+  for (int o = 16; o <= 32; o <<= 1) {
+    const double ov = __shfl_xor(bv, o);
+    const int oi = __shfl_xor(bi, o);
+    if (ov > bv || (ov == bv && oi > bi)) {
+      bv = ov;
+      bi = oi;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      double ov = __shfl_xor(bv, o);
-      int oi = __shfl_xor(bi, o);
-      if (ov > bv || (ov == bv && oi > bi)) { bv = ov; bi = oi; }
-    }
-    if (lane == 0) { wv[wid] = bv; wi[wid] = bi; }
-    __syncthreads();
-    if (tid == 0) {
-      double b2 = wv[0];
-      int i2 = wi[0];
-      for (int w = 1; w < nw; w++)
-        if (wv[w] > b2 || (wv[w] == b2 && wi[w] > i2)) { b2 = wv[w]; i2 = wi[w]; }
-      if (i2 >= 0) s_maxI = i2;
-    }
-    __syncthreads();
-    const int maxI = s_maxI;
-    if (maxI < k) {
-      // No row qualified and the kept pivot row lies above this column: the
-      // reference swaps with an already-factorised row, which only a literal
-      // replay reproduces (k_lu_fallback).
-      if (tid == 0) st->fallback = 1;
-      __syncthreads();
-      return;
-    }
-    if (maxI != k) {
-      if (tid < cB) {
-        double t = T[off + maxI - c0][tid];
-        T[off + maxI - c0][tid] = T[off + k - c0][tid];
-        T[off + k - c0][tid] = t;
+  }
+  return __builtin_amdgcn_readfirstlane(bi);
+}
+
+// Factorise columns c1..c1+cB-1 over rows c1..n with the calling wave.
+// S[r - c1][q] holds matrix element (r, c1 + q) (updated by all earlier
+// columns); vin[r - c1] the row scale of position r; lastMaxI the pivot row of
+// column c1 - 1.  Results go to M, vout (by position), piv and st.
+__device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1, int cB,
+                                                  const double* vin, int lastMaxI, double* M,
+                                                  double* vout, int* piv, LuState* st) {
+  const int n = MATSIZE, N = MATN;
+  const int lane = lane_id();
+  double a[NSLOT][LB];
+  double vr[NSLOT];
+  int lidx[NSLOT];
+#pragma unroll
+  for (int s = 0; s < NSLOT; s++) {
+    const int r = c1 + lane + WAVE * s;
+    const bool ok = r <= n;
+    lidx[s] = ok ? r : -1;
+    vr[s] = ok ? vin[r - c1] : 0.0;
+#pragma unroll
+    for (int q = 0; q < LB; q++) a[s][q] = (ok && q < cB) ? S[r - c1][q] : 0.0;
+  }
+  int maxI = lastMaxI;
+  bool fb = false;
+#pragma unroll
+  for (int q = 0; q < LB; q++) {
+    // fully unrolled (register-resident a[][]): columns past cB, or after a
+    // replay was flagged, are skipped by a wave-uniform guard
+    const int k = c1 + q;
+    if (q < cB && !fb) {
+      // pivot search: max vv*|a| over positions >= k, ties to the larger position
+      double bv = -1.0;
+      int bi = -1;
+#pragma unroll
+      for (int s = 0; s < NSLOT; s++) {
+        const double dum = vr[s] * fabs(a[s][q]);
+        if (lidx[s] >= k && dum >= 0.0 && (dum > bv || (dum == bv && lidx[s] > bi))) {
+          bv = dum;
+          bi = lidx[s];
+        }
       }
-      if (tid == 0) vvL[maxI - c0] = vvL[k - c0];
-    }
-    __syncthreads();
-    if (tid == 0) {
-      piv[k] = maxI;
-      if (T[off + k - c0][q] == 0.0) T[off + k - c0][q] = 1.0e-20;
-    }
-    __syncthreads();
-    if (k != n) {
-      const double dum = 1.0 / T[off + k - c0][q];
-      for (int i = k + 1 + tid; i <= n; i += blockDim.x) T[off + i - c0][q] *= dum;
-      __syncthreads();
-      const int m = cB - q - 1;  // remaining panel columns
-      if (m > 0) {
-        const int rows = n - k;
-        for (int e = tid; e < rows * m; e += blockDim.x) {
-          int i = k + 1 + e / m, l = q + 1 + e % m;
-          T[off + i - c0][l] = T[off + i - c0][l] - T[off + i - c0][q] * T[off + k - c0][l];
+      bi = wave_argmax(bv, bi);
+      if (bi >= 0) maxI = bi;
+      if (maxI < k) {
+        // No row qualified and the kept pivot row lies above this column:
+        // the reference swaps with an already-factorised row, which only a
+        // literal replay reproduces (k_lu_fallback).
+        fb = true;
+      } else {
+        if (maxI != k) {
+#pragma unroll
+          for (int s = 0; s < NSLOT; s++)
+            lidx[s] = lidx[s] == k ? maxI : (lidx[s] == maxI ? k : lidx[s]);
+        }
+        if (lane == 0) piv[k] = maxI;
+#pragma unroll
+        for (int s = 0; s < NSLOT; s++)
+          if (lidx[s] == k && a[s][q] == 0.0) a[s][q] = 1.0e-20;
+        if (k != n) {
+          // the pivot row (position k): lane pl, slot ps — both wave-uniform
+          int pl = 0, ps = 0;
+#pragma unroll
+          for (int s = 0; s < NSLOT; s++) {
+            const unsigned long long m = __ballot(lidx[s] == k);
+            if (m) {
+              pl = __ffsll((unsigned long long)m) - 1;
+              ps = s;
+            }
+          }
+          double u[LB];
+#pragma unroll
+          for (int l = 0; l < LB; l++) u[l] = 0.0;
+#pragma unroll
+          for (int s = 0; s < NSLOT; s++)
+            if (s == ps) {
+#pragma unroll
+              for (int l = q; l < LB; l++) u[l] = readlane_f64(a[s][l], pl);
+            }
+          const double dinv = 1.0 / u[q];
+#pragma unroll
+          for (int s = 0; s < NSLOT; s++)
+            if (lidx[s] > k) a[s][q] *= dinv;
+#pragma unroll
+          for (int l = q + 1; l < LB; l++) {
+#pragma unroll
+            for (int s = 0; s < NSLOT; s++)
+              if (lidx[s] > k) a[s][l] = a[s][l] - a[s][q] * u[l];
+          }
         }
       }
     }
-    __syncthreads();
   }
-  if (tid == 0) st->lastMaxI = s_maxI;
-  __syncthreads();
+  if (fb) {
+    if (lane == 0) st->fallback = 1;
+    return;
+  }
+#pragma unroll
+  for (int s = 0; s < NSLOT; s++) {
+    if (lidx[s] > 0) {
+      double* row = M + (size_t)lidx[s] * N + c1;
+#pragma unroll
+      for (int q = 0; q < LB; q++)
+        if (q < cB) row[q] = a[s][q];
+      vout[lidx[s]] = vr[s];
+    }
+  }
+  if (lane == 0) st->lastMaxI = maxI;
+}
+
+__device__ __forceinline__ int* lu_zero_flags(LuState* st) {
+  return reinterpret_cast<int*>(reinterpret_cast<char*>(st) + 64);
+}
+
+// vv (ludcmp's implicit row scaling) and the zero-row flags: one wave per row,
+// lanes across the columns.  The maximum of |a| with NaN never taken is the
+// same in any order.
+__global__ __launch_bounds__(LU_THREADS) void k_lu_scale(const double* M, double* vv,
+                                                         LuState* st, const int* skipIfZero) {
+  if (skipIfZero && *skipIfZero == 0) return;
+  const int n = MATSIZE, N = MATN;
+  const int lane = lane_id();
+  const int i = 1 + (int)blockIdx.x * (LU_THREADS / WAVE) + (int)threadIdx.x / WAVE;
+  if (i > n) return;
+  const double* row = M + (size_t)i * N;
+  double v[NSLOT];
+#pragma unroll
+  for (int s = 0; s < NSLOT; s++) {
+    const int j = 1 + lane + WAVE * s;
+    v[s] = j <= n ? fabs(row[j]) : 0.0;
+  }
+  double big = 0.0;
+#pragma unroll
+  for (int s = 0; s < NSLOT; s++)
+    if (v[s] > big) big = v[s];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(big, o);
+    if (t > big) big = t;
+  }
+  if (lane == 0) {
+    vv[i] = 1.0 / big;
+    lu_zero_flags(st)[i] = big == 0.0;
+  }
+}
+
+// Staging of rows r0..r0+nr-1, columns c..c+LB-1 (zero past cB) of M into an
+// LDS slab, with every thread's loads in flight together.
+__device__ __forceinline__ void stage_slab(double (*D)[LDP], const double* M, int r0, int nr,
+                                           int c, int cB) {
+  constexpr int IT = (MATSIZE * LB + LU_THREADS - 1) / LU_THREADS;
+  double v[IT];
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int e = (int)threadIdx.x + it * LU_THREADS;
+    const int r = e / LB, q = e % LB;
+    v[it] = (r < nr && q < cB) ? M[(size_t)(r0 + r) * MATN + c + q] : 0.0;
+  }
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int e = (int)threadIdx.x + it * LU_THREADS;
+    const int r = e / LB, q = e % LB;
+    if (r < nr) D[r][q] = v[it];
+  }
 }
 
 __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, int* piv,
                                                          LuState* st, const int* skipIfZero) {
   const int n = MATSIZE, N = MATN;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wid = tid / WAVE;
   __shared__ int s_zero;
-  __shared__ double T[MATSIZE][LB + 1];
-  __shared__ double vvL[MATSIZE];
+  __shared__ double S[MATSIZE][LDP];
   if (tid == 0) s_zero = 0;
   for (int i = tid; i < N; i += blockDim.x) piv[i] = 0;
   const int skip = (skipIfZero && *skipIfZero == 0) ? 1 : 0;
@@ -135,39 +272,21 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, 
   }
   if (skip) return;  // testFit returns 666 before solving (cdkscUtil.cpp:101-103)
   __syncthreads();
-  for (int i = 1 + tid; i <= n; i += blockDim.x) {
-    double big = 0.0;
-    for (int j = 1; j <= n; j++) {
-      double t = fabs(M[(size_t)i * N + j]);
-      if (t > big) big = t;
-    }
-    if (big == 0.0) atomicOr(&s_zero, 1);
-    double v = 1.0 / big;
-    vv[i] = v;
-    vvL[i - 1] = v;
-  }
+  int z = 0;
+  for (int i = 1 + tid; i <= n; i += blockDim.x) z |= lu_zero_flags(st)[i];
+  if (z) atomicOr(&s_zero, 1);
+  stage_slab(S, M, 1, n, 1, LB);
   __syncthreads();
   if (tid == 0) {
     st->zero = s_zero;
     st->lastMaxI = 0;
     st->ii = 0;
   }
-  if (s_zero) return;
-  const int cB = min(LB, n);
-  for (int e = tid; e < n * cB; e += blockDim.x) {
-    int r = e / cB, q = e % cB;
-    T[r][q] = M[(size_t)(1 + r) * N + 1 + q];
-  }
-  __syncthreads();
-  panel_factor(T, 0, 1, cB, vvL, piv, st);
-  for (int e = tid; e < n * cB; e += blockDim.x) {
-    int r = e / cB, q = e % cB;
-    M[(size_t)(1 + r) * N + 1 + q] = T[r][q];
-  }
-  for (int i = tid; i < n; i += blockDim.x) vv[1 + i] = vvL[i];
+  if (s_zero) return;  // ludcmp returns before factorising
+  if (wid == 0) wave_panel_factor(S, 1, LB, vv + 1, 0, M, vv, piv, st);
 }
 
-__global__ __launch_bounds__(STEP_THREADS) void k_lu_step(double* M, double* vv, int* piv,
+__global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, int* piv,
                                                         double* b, LuState* st, int p,
                                                         int nColBlocks) {
   if (st->zero || st->skip || st->fallback) return;
@@ -176,99 +295,135 @@ __global__ __launch_bounds__(STEP_THREADS) void k_lu_step(double* M, double* vv,
   const int c0 = 1 + p * LB;
   const int cB = min(LB, n - c0 + 1);
   const int R = n - c0 + 1;  // rows c0..n
-  __shared__ double T[MATSIZE][LB + 1];
-  __shared__ double vvL[MATSIZE];
-  __shared__ int s_ii;
-  if ((int)blockIdx.x == nColBlocks) {
+  const bool rhsBlock = (int)blockIdx.x == nColBlocks;
+  const int l0 = c0 + cB + (int)blockIdx.x * LB;  // first column of this block
+  const int nc = rhsBlock ? 0 : min(LB, n - l0 + 1);
+  __shared__ double Lp[MATSIZE][LDP];  // the factorised panel, rows c0..n
+  __shared__ double T[MATSIZE][LDP];   // this block's columns, rows c0..n
+  __shared__ double vS[MATSIZE];       // block 0: row scales; rhs block: b (rows c0..n)
+  __shared__ int pv[LB];
+  __shared__ int s_int;                // block 0: last pivot row; rhs block: ii
+  {
+    // every load of the step in flight together: one memory round trip
+    constexpr int IT = (MATSIZE * LB + LU_THREADS - 1) / LU_THREADS;
+    constexpr int IV = (MATSIZE + LU_THREADS - 1) / LU_THREADS;
+    double va[IT], vt[IT], vr[IV];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = tid + it * LU_THREADS;
+      const int r = e / LB, q = e % LB;
+      const bool in = r < R;
+      va[it] = (in && q < cB) ? M[(size_t)(c0 + r) * N + c0 + q] : 0.0;
+      vt[it] = (in && q < nc) ? M[(size_t)(c0 + r) * N + l0 + q] : 0.0;
+    }
+    const bool wantV = rhsBlock || blockIdx.x == 0;
+#pragma unroll
+    for (int it = 0; it < IV; it++) {
+      const int r = tid + it * LU_THREADS;
+      vr[it] = (wantV && r < R) ? (rhsBlock ? b[c0 + r] : vv[c0 + r]) : 0.0;
+    }
+    const int pvl = tid < cB ? piv[c0 + tid] : 0;
+    const int si = tid == 0 ? (rhsBlock ? st->ii : st->lastMaxI) : 0;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = tid + it * LU_THREADS;
+      const int r = e / LB, q = e % LB;
+      if (r < R) {
+        Lp[r][q] = va[it];
+        T[r][q] = vt[it];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IV; it++) {
+      const int r = tid + it * LU_THREADS;
+      if (r < R) vS[r] = vr[it];
+    }
+    if (tid < LB) pv[tid] = pvl;
+    if (tid == 0) s_int = si;
+  }
+  __syncthreads();
+  if (rhsBlock) {
     // right-hand side column: permutation + forward substitution (lubksb)
-    double* y = &T[0][0];  // R doubles
-    for (int r = tid; r < R; r += blockDim.x) y[r] = b[c0 + r];
-    if (tid == 0) s_ii = st->ii;
-    __syncthreads();
+    double* y = vS;
     if (tid == 0)
       for (int q = 0; q < cB; q++) {
-        int k = c0 + q, m = piv[k];
-        if (m != k) {
-          double t = y[m - c0];
-          y[m - c0] = y[k - c0];
-          y[k - c0] = t;
+        const int m = pv[q] - c0;
+        if (m != q) {
+          const double t = y[m];
+          y[m] = y[q];
+          y[q] = t;
         }
       }
     __syncthreads();
     for (int q = 0; q < cB; q++) {
-      const int k = c0 + q;
-      if (tid == 0 && s_ii == 0 && !(y[k - c0] == 0.0)) s_ii = k;
+      if (tid == 0 && s_int == 0 && !(y[q] == 0.0)) s_int = c0 + q;
       __syncthreads();
-      if (s_ii != 0) {
-        const double yk = y[k - c0];
-        for (int r = k + 1 + tid; r <= n; r += blockDim.x)
-          y[r - c0] = y[r - c0] - M[(size_t)r * N + k] * yk;
+      if (s_int != 0) {
+        const double yk = y[q];
+        for (int r = q + 1 + tid; r < R; r += LU_THREADS) y[r] = y[r] - Lp[r][q] * yk;
       }
       __syncthreads();
     }
-    for (int r = tid; r < R; r += blockDim.x) b[c0 + r] = y[r];
-    if (tid == 0) st->ii = s_ii;
+    for (int r = tid; r < R; r += LU_THREADS) b[c0 + r] = y[r];
+    if (tid == 0) st->ii = s_int;
     return;
   }
-  // trailing column block
-  const int l0 = c0 + cB + blockIdx.x * LB;
-  const int nc = min(LB, n - l0 + 1);
-  if (nc <= 0) return;
-  for (int e = tid; e < R * nc; e += blockDim.x) {
-    int r = e / nc, t = e % nc;
-    T[r][t] = M[(size_t)(c0 + r) * N + l0 + t];
-  }
-  __syncthreads();
-  if (tid < nc)
+  // trailing column block (panels with trailing columns are full: cB == LB)
+  if (tid < LB) {
+    // the panel's interchanges, in order, then U12 = L11^-1 A12 (k ascending
+    // per element), one thread per column with the 16 rows in registers
     for (int q = 0; q < cB; q++) {
-      int k = c0 + q, m = piv[k];
-      if (m != k) {
-        double t = T[m - c0][tid];
-        T[m - c0][tid] = T[k - c0][tid];
-        T[k - c0][tid] = t;
+      const int m = pv[q] - c0;
+      if (m != q) {
+        const double t = T[m][tid];
+        T[m][tid] = T[q][tid];
+        T[q][tid] = t;
       }
     }
-  __syncthreads();
-  // in-panel rows: U12 = L11^-1 A12 with k ascending per element
-  for (int q = 0; q + 1 < cB; q++) {
-    const int k = c0 + q;
-    const int rows = cB - q - 1;
-    for (int e = tid; e < rows * nc; e += blockDim.x) {
-      int r = k + 1 + e / nc, t = e % nc;
-      T[r - c0][t] = T[r - c0][t] - M[(size_t)r * N + k] * T[q][t];
+    double u[LB];
+#pragma unroll
+    for (int r = 0; r < LB; r++) u[r] = T[r][tid];
+#pragma unroll
+    for (int r = 1; r < LB; r++) {
+      double s = u[r];
+#pragma unroll
+      for (int q = 0; q < r; q++) s = s - Lp[r][q] * u[q];
+      u[r] = s;
     }
-    __syncthreads();
-  }
-  // rows below the panel: rank-cB update, k ascending per element
-  for (int e = tid; e < (R - cB) * nc; e += blockDim.x) {
-    int r = c0 + cB + e / nc, t = e % nc;
-    const double* Lr = M + (size_t)r * N + c0;
-    double s = T[r - c0][t];
-    for (int q = 0; q < cB; q++) s = s - Lr[q] * T[q][t];
-    T[r - c0][t] = s;
+#pragma unroll
+    for (int r = 0; r < LB; r++) T[r][tid] = u[r];
   }
   __syncthreads();
-  for (int e = tid; e < R * nc; e += blockDim.x) {
-    int r = e / nc, t = e % nc;
-    M[(size_t)(c0 + r) * N + l0 + t] = T[r][t];
-  }
-  if (blockIdx.x == 0 && nc > 0) {
-    // lookahead: this block holds the next panel (columns l0..l0+nc-1) for
-    // rows c0..n; factorise its rows l0..n now.
-    const int c1 = l0;
-    for (int i = tid; i <= n - c1; i += blockDim.x) vvL[i] = vv[c1 + i];
-    __syncthreads();
-    panel_factor(T, c1 - c0, c1, nc, vvL, piv, st);
-    for (int e = tid; e < (n - c1 + 1) * nc; e += blockDim.x) {
-      int r = e / nc, t = e % nc;
-      M[(size_t)(c1 + r) * N + l0 + t] = T[c1 - c0 + r][t];
+  {
+    // rows below the panel: rank-16 update, k ascending per element
+    const int t = tid % LB, rg = tid / LB;
+    double U[LB];
+#pragma unroll
+    for (int q = 0; q < LB; q++) U[q] = T[q][t];
+    for (int r = LB + rg; r < R; r += LU_THREADS / LB) {
+      double s = T[r][t];
+#pragma unroll
+      for (int q = 0; q < LB; q++) s = s - Lp[r][q] * U[q];
+      T[r][t] = s;
     }
-    for (int i = tid; i <= n - c1; i += blockDim.x) vv[c1 + i] = vvL[i];
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    // lookahead: these columns are the next panel; store its U rows, then
+    // factorise rows l0..n with one wave
+    for (int e = tid; e < LB * LB; e += LU_THREADS) {
+      const int r = e / LB, t = e % LB;
+      if (t < nc) M[(size_t)(c0 + r) * N + l0 + t] = T[r][t];
+    }
+    if (tid < WAVE) wave_panel_factor(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, st);
+    return;
+  }
+  for (int e = tid; e < R * LB; e += LU_THREADS) {
+    const int r = e / LB, t = e % LB;
+    if (t < nc) M[(size_t)(c0 + r) * N + l0 + t] = T[r][t];
   }
 }
 
-// Back substitution (and, after a zero row, the reference's lubksb on the
-// unfactorised matrix with an all-zero index).
 // Literal replay of ludcmp (src/bachUtil.cpp:319-384) on the saved matrix,
 // one thread, only after a column had no finite pivot candidate.
 __global__ __launch_bounds__(64) void k_lu_fallback(double* M, const double* M0, int* piv,
@@ -324,6 +479,8 @@ __global__ __launch_bounds__(64) void k_lu_fallback(double* M, const double* M0,
   }
 }
 
+// Back substitution (and, after a zero row or a replay, the reference's whole
+// lubksb: its forward pass was not fused into the elimination then).
 __global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const double* M0,
                                                    const int* piv, double* b, const double* b0,
                                                    const LuState* st, double* out) {
@@ -355,49 +512,84 @@ __global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const double
     }
   }
   __syncthreads();
-  // Back substitution x_i = (y_i - sum_{j>i} U_ij x_j) / U_ii with the sum
-  // in ascending j (lubksb).  The products need only final x values, so all
-  // lanes form them (row i-1 is prefetched into registers meanwhile); one lane
-  // runs the ordered subtraction chain, the only sequential part.
-  __shared__ double P[MATN];
-  constexpr int PER = (MATSIZE + 63) / 64;  // row entries per lane
+  // x_i = (y_i - sum_{j>i} U_ij x_j) / U_ii with the sum in ascending j.  The
+  // products need only final x values, so all lanes form them (row i-1 is
+  // prefetched into registers meanwhile); lane 0 runs the ordered subtraction
+  // chain, the only sequential part, with its operands loaded one 8-block
+  // ahead of their use.
+  constexpr int CH = 8;  // chain operands loaded per block
+  // products P[j] = U_ij x_j; entries past n stay +0.0, and s - (+0.0) is s
+  // bit for bit (also for -0, inf and NaN), so the chain always consumes
+  // whole blocks of CH without a scalar tail
+  __shared__ double P[MATN + CH];
+  constexpr int PER = (MATSIZE + WAVE - 1) / WAVE;  // row entries per lane
+  constexpr int D = 8;                               // rows prefetched ahead
   const int lane = threadIdx.x;
-  double cur[PER];
+  for (int j = lane; j < MATN + CH; j += WAVE) P[j] = 0.0;
+  // ring of the next D rows of U (entries right of the diagonal, and the
+  // diagonal), slot d holding rows i with (n - i) % D == d
+  double ring[D][PER];
+  double rdg[D];
 #pragma unroll
-  for (int q = 0; q < PER; q++) {
-    int j = n + 1 + lane + 64 * q;  // row n has no entries right of the diagonal
-    cur[q] = 0.0;
-    (void)j;
-  }
-  double diag = M[(size_t)n * N + n];
-  for (int i = n; i >= 1; i--) {
+  for (int d = 0; d < D; d++) {
+    const int i = n - d;
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      int j = i + 1 + lane + 64 * q;
-      if (j <= n) P[j] = cur[q] * x[j];
+      const int j = i + 1 + lane + WAVE * q;
+      ring[d][q] = j <= n ? M[(size_t)i * N + j] : 0.0;
     }
-    // prefetch row i-1 (columns i..n) and its diagonal
-    double nxt[PER];
-    double nd = 0.0;
-    if (i > 1) {
-      const double* r1 = M + (size_t)(i - 1) * N;
+    rdg[d] = M[(size_t)i * N + i];
+  }
+  for (int base = n; base >= 1; base -= D) {
 #pragma unroll
-      for (int q = 0; q < PER; q++) {
-        int j = i + lane + 64 * q;
-        nxt[q] = j <= n ? r1[j] : 0.0;
+    for (int d = 0; d < D; d++) {
+      const int i = base - d;
+      if (i >= 1) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+          const int j = i + 1 + lane + WAVE * q;
+          if (j <= n) P[j] = ring[d][q] * x[j];
+        }
+        const double dg = rdg[d];
+        const int i2 = i - D;
+        if (i2 >= 1) {
+#pragma unroll
+          for (int q = 0; q < PER; q++) {
+            const int j = i2 + 1 + lane + WAVE * q;
+            ring[d][q] = j <= n ? M[(size_t)i2 * N + j] : 0.0;
+          }
+          rdg[d] = M[(size_t)i2 * N + i2];
+        }
+        // one wave: its LDS accesses complete in issue order, so only the
+        // compiler needs fencing here (a workgroup barrier would also wait
+        // for the prefetches in flight)
+        __builtin_amdgcn_wave_barrier();
+        __asm__ __volatile__("" ::: "memory");
+        if (lane == 0) {
+          double s = x[i];
+          if (i < n) {
+            // blocks j0 = i+1, i+1+CH, ... up to the one holding n
+            double c[CH];
+#pragma unroll
+            for (int t = 0; t < CH; t++) c[t] = P[i + 1 + t];
+            for (int j = i + 1 + CH; j <= n; j += CH) {
+              double e[CH];
+#pragma unroll
+              for (int t = 0; t < CH; t++) e[t] = P[j + t];
+#pragma unroll
+              for (int t = 0; t < CH; t++) s = s - c[t];
+#pragma unroll
+              for (int t = 0; t < CH; t++) c[t] = e[t];
+            }
+#pragma unroll
+            for (int t = 0; t < CH; t++) s = s - c[t];
+          }
+          x[i] = s / dg;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __asm__ __volatile__("" ::: "memory");
       }
-      nd = r1[i - 1];
     }
-    __syncthreads();
-    if (lane == 0) {
-      double sum = x[i];
-      for (int j = i + 1; j <= n; j++) sum = sum - P[j];
-      x[i] = sum / diag;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; q++) cur[q] = nxt[q];
-    diag = nd;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < NSOL; i += blockDim.x) {
@@ -409,6 +601,8 @@ __global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const double
 hipError_t launch_global_solve(Ctx& c, double* d_out, const int* d_skipIfZero) {
   LuState* st = reinterpret_cast<LuState*>(c.luState);
   prof_begin(c, XB_PROF_SOLVE);
+  hipLaunchKernelGGL(k_lu_scale, dim3((MATSIZE + LU_THREADS / WAVE - 1) / (LU_THREADS / WAVE)),
+                     dim3(LU_THREADS), 0, c.stream, c.M, c.luVV, st, d_skipIfZero);
   hipLaunchKernelGGL(k_lu_begin, dim3(1), dim3(LU_THREADS), 0, c.stream, c.M, c.luVV, c.luIndex,
                      st, d_skipIfZero);
   hipError_t e = hipGetLastError();
@@ -418,7 +612,7 @@ hipError_t launch_global_solve(Ctx& c, double* d_out, const int* d_skipIfZero) {
     const int cB = std::min(LB, MATSIZE - c0 + 1);
     const int trailing = MATSIZE - (c0 + cB) + 1;
     const int nColBlocks = (trailing + LB - 1) / LB;
-    hipLaunchKernelGGL(k_lu_step, dim3(nColBlocks + 1), dim3(STEP_THREADS), 0, c.stream, c.M,
+    hipLaunchKernelGGL(k_lu_step, dim3(nColBlocks + 1), dim3(LU_THREADS), 0, c.stream, c.M,
                        c.luVV, c.luIndex, c.rhs, st, p, nColBlocks);
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -13,9 +13,10 @@
 //     sequential sum, so mean/stddev are bit-identical;
 //   * histogram bins are integer counts (order-free, LDS atomics); the mode
 //     and quartile walk is the reference's serial code on one thread;
-//   * findSubStamps keeps the sequential scan: pixels are classified in
-//     256-pixel raster chunks and candidates are then processed one by one in
-//     raster order, each with a parallel window search and check.
+//   * findSubStamps keeps the sequential scan: each threshold level's pixels
+//     are classified by all waves (coalesced, raster-ordered candidate lists)
+//     and one wave then processes the candidates in raster order, each with a
+//     wave-parallel window search and check.
 #include "xb_dev.h"
 
 namespace xb {
@@ -305,45 +306,36 @@ struct FindArgs {
   int* candAll;  // [N][fsw*fsw] candidate-list scratch
 };
 
-// Block-wide (value, index) argmax: larger value wins, ties -> smaller index.
-// NaN values never win.  Returns index (or INT_MAX when no candidate).
-__device__ int block_argmax_first(double v, int idx, double* dsc, int* isc2) {
-  const int lane = lane_id(), wid = threadIdx.x / WAVE;
-  for (int o = 32; o > 0; o >>= 1) {
-    double ov = __shfl_xor(v, o);
-    int oi = __shfl_xor(idx, o);
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-  }
-  if (lane == 0) { dsc[wid] = v; isc2[wid] = idx; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double bv = dsc[0];
-    int bi = isc2[0];
-    for (int i = 1; i < (int)(blockDim.x / WAVE); i++)
-      if (dsc[i] > bv || (dsc[i] == bv && isc2[i] < bi)) { bv = dsc[i]; bi = isc2[i]; }
-    dsc[16] = bv;
-    isc2[16] = bi;
-  }
-  __syncthreads();
-  int r = isc2[16];
-  __syncthreads();
-  return r;
+// Wave-wide reductions (every lane gets the result).
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
 }
-
-__device__ double block_sum_double_tree(double v, double* dsc) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  const int wid = threadIdx.x / WAVE;
-  if (lane_id() == 0) dsc[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0.0;
-    for (int i = 0; i < (int)(blockDim.x / WAVE); i++) a += dsc[i];
-    dsc[16] = a;
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+// (value, index) maximum: larger value wins, ties -> smaller index; NaN never
+// wins (callers pass -inf for lanes without a candidate).
+__device__ __forceinline__ int wave_argmax_first(double v, int idx) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(v, o);
+    const int oi = __shfl_xor(idx, o);
+    if (ov > v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
   }
-  __syncthreads();
-  double r = dsc[16];
-  __syncthreads();
-  return r;
+  return idx;
+}
+// Tree sum; used only where every partial sum is exact (any order agrees).
+__device__ __forceinline__ double wave_sum_exact(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
 }
 
 __device__ __forceinline__ int float_exp_field(float f) {
@@ -351,9 +343,21 @@ __device__ __forceinline__ int float_exp_field(float f) {
   return e == 0 ? 1 : e;
 }
 
-__global__ __launch_bounds__(256) void k_sss_find(FindArgs A) {
+constexpr int FIND_THREADS = 1024;
+constexpr int FIND_WAVES = FIND_THREADS / WAVE;
+constexpr int WIN_PER_LANE = (SSW * SSW + WAVE - 1) / WAVE;  // 31x31 window / 64 lanes
+
+// One 1024-thread workgroup per stamp.  Per threshold level, all 16 waves
+// classify the stamp's pixels against the mask as it is now: wave w scans the
+// raster segment [w*seg, (w+1)*seg) with coalesced loads and appends its
+// candidates, in raster order, to its own slice of the scratch list (ballot +
+// popcount).  Mask bits are only ever added, so a pixel rejected here stays
+// rejected; flagging pixels above threshHigh commutes with everything else.
+// Wave 0 then processes the candidates one by one in raster order, re-checking
+// each against the current mask, with wave-level reductions only.
+__global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
   const int s = blockIdx.x;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
   const StampGeom g = stamp_geom(s, A.nx, A.ny, A.w, A.h, A.fsw);
   const double sky = A.sky[s], fwhm = A.fwhm[s];
   const double floorV = sky + A.tkf * fwhm;
@@ -362,9 +366,7 @@ __global__ __launch_bounds__(256) void k_sss_find(FindArgs A) {
   const float* img = A.img;
   uint16_t* mask = A.mask;
   const uint16_t badMask = A.badMask;
-  __shared__ int isc[20];
-  __shared__ double dsc[20];
-  __shared__ int isc2[20];
+  __shared__ int pre[FIND_WAVES + 1];
   __shared__ int lx[MAXSS], ly[MAXSS];
   __shared__ double lv[MAXSS];
   __shared__ int s_counter;
@@ -372,64 +374,79 @@ __global__ __launch_bounds__(256) void k_sss_find(FindArgs A) {
   __syncthreads();
   double dfrac = 0.9;
   const int nAll = fsw * fsw;
-  // ordered candidate list of the current level (scratch of this stamp)
+  const int seg = ((nAll + FIND_WAVES - 1) / FIND_WAVES + WAVE - 1) / WAVE * WAVE;
   int* cand = A.candAll + (size_t)s * nAll;
-  const int per = (nAll + blockDim.x - 1) / blockDim.x;  // raster range per thread
-  const int p0 = tid * per, p1 = min(nAll, p0 + per);
   // The reference loops until the threshold reaches the floor; with a
   // non-finite sky or sigma that never happens, so the levels are capped
   // (finite statistics end within 6 levels).
   for (int level = 0; level < MAX_LEVELS && s_counter < MAXSS; level++) {
     const double lim = std_max(floorV, sky + (A.hi - sky) * dfrac);
-    // Classify every pixel against the mask as it is now.  Mask bits are
-    // only ever added, so a pixel rejected here stays rejected; candidates
-    // are re-checked in raster order below (earlier ones may mark them).
-    // Flagging pixels above threshHigh commutes with everything else.
-    int myCount = 0;
-    for (int k = p0; k < p1; k++) {
-      int absx = k % fsw + g.ox, absy = k / fsw + g.oy;
-      long a = absx + absy * imgW;
-      uint16_t m = mask[a];
-      if ((m & badMask) == 0) {
-        double v = (double)img[a];
-        if (v > A.hi) {
-          mask[a] = m | A.badPixelMask;
-        } else if (!((v - sky) * (1.0 / fwhm) < A.tkf) && v > lim) {
-          myCount++;
+    {
+      const int k0 = wid * seg, k1 = min(nAll, k0 + seg);
+      int cnt = 0;
+      for (int base = k0; base < k1; base += 4 * WAVE) {
+        uint16_t m[4];
+        float v[4];
+        long a[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int k = base + u * WAVE + lane;
+          a[u] = (k % fsw + g.ox) + (long)(k / fsw + g.oy) * imgW;
+          m[u] = k < k1 ? mask[a[u]] : (uint16_t)0xffff;
+          v[u] = k < k1 ? img[a[u]] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int k = base + u * WAVE + lane;
+          bool c = false;
+          if (k < k1 && (m[u] & badMask) == 0) {
+            const double dv = (double)v[u];
+            if (dv > A.hi) {
+              mask[a[u]] = m[u] | A.badPixelMask;
+            } else if (!((dv - sky) * (1.0 / fwhm) < A.tkf) && dv > lim) {
+              c = true;
+            }
+          }
+          const unsigned long long bal = __ballot(c);
+          if (c) cand[k0 + cnt + __popcll(bal & ((1ull << lane) - 1ull))] = k;
+          cnt += __popcll(bal);
         }
       }
-    }
-    int nc;
-    int off = block_prefix_sum(myCount, isc, &nc);
-    for (int k = p0; k < p1 && myCount > 0; k++) {
-      int absx = k % fsw + g.ox, absy = k / fsw + g.oy;
-      long a = absx + absy * imgW;
-      if ((mask[a] & badMask) == 0) {
-        double v = (double)img[a];
-        if (!(v > A.hi) && !((v - sky) * (1.0 / fwhm) < A.tkf) && v > lim) cand[off++] = k;
-      }
+      if (lane == 0) pre[wid + 1] = cnt;
+      if (tid == 0) pre[0] = 0;
     }
     __syncthreads();
-    for (int pos = 0; pos < nc && s_counter < MAXSS;) {
-      // first candidate from pos on whose pixel is still unmasked
-      {
-        int q = pos + tid;
-        int ok = INT_MAX;
-        if (q < nc) {
-          int kk = cand[q];
-          long a = (kk % fsw + g.ox) + (kk / fsw + g.oy) * imgW;
-          if ((mask[a] & badMask) == 0) ok = q;
+    if (wid == 0) {
+      if (lane == 0)
+        for (int w = 0; w < FIND_WAVES; w++) pre[w + 1] += pre[w];
+      __threadfence_block();
+      const int nc = pre[FIND_WAVES];
+      int counter = s_counter;
+      // logical list position -> raster index
+      auto candAt = [&](int q) {
+        int w = 0;
+        while (w + 1 < FIND_WAVES && pre[w + 1] <= q) w++;
+        return cand[w * seg + (q - pre[w])];
+      };
+      for (int pos = 0; pos < nc && counter < MAXSS;) {
+        // first candidate from pos on whose pixel is still unmasked
+        {
+          const int q = pos + lane;
+          bool ok = false;
+          if (q < nc) {
+            const int kk = candAt(q);
+            const long a = (kk % fsw + g.ox) + (long)(kk / fsw + g.oy) * imgW;
+            ok = (mask[a] & badMask) == 0;
+          }
+          const unsigned long long bal = __ballot(ok);
+          if (bal == 0ull) {
+            pos += WAVE;
+            continue;
+          }
+          pos += __ffsll((unsigned long long)bal) - 1;
         }
-        int first = block_min_int(ok, isc);
-        if (first == INT_MAX) {
-          pos += blockDim.x;
-          continue;
-        }
-        pos = first;
-      }
-      const int kk = cand[pos];
-      pos++;
-      {
+        const int kk = candAt(pos);
+        pos++;
         const int absx = kk % fsw + g.ox, absy = kk / fsw + g.oy;
         // ---- window maximum (sss.cl:427-459)
         const int startX = max(absx - HSS, g.ox), startY = max(absy - HSS, g.oy);
@@ -437,21 +454,37 @@ __global__ __launch_bounds__(256) void k_sss_find(FindArgs A) {
         const int ww = endX - startX + 1, wh = endY - startY + 1;
         double bestV = 0.0;
         int bestI = INT_MAX;
-        for (int p = tid; p < ww * wh; p += blockDim.x) {
-          int kx = startX + p % ww, ky = startY + p / ww;
-          long kc = kx + ky * imgW;
-          uint16_t km = mask[kc];
-          if ((km & badMask) > 0) continue;
-          double kv = (double)img[kc];
-          if (kv >= A.hi) {
-            mask[kc] = km | A.badPixelMask;
-            continue;
+        {
+          uint16_t km[WIN_PER_LANE];
+          float kvf[WIN_PER_LANE];
+          long kc[WIN_PER_LANE];
+#pragma unroll
+          for (int u = 0; u < WIN_PER_LANE; u++) {
+            const int p = lane + u * WAVE;
+            const bool in = p < ww * wh;
+            kc[u] = in ? (startX + p % ww) + (long)(startY + p / ww) * imgW : 0;
+            km[u] = in ? mask[kc[u]] : (uint16_t)0xffff;
+            kvf[u] = in ? img[kc[u]] : 0.0f;
           }
-          if ((kv - sky) * (1.0 / fwhm) < A.tkf) continue;
-          if (kv > bestV) { bestV = kv; bestI = p; }
+#pragma unroll
+          for (int u = 0; u < WIN_PER_LANE; u++) {
+            const int p = lane + u * WAVE;
+            if (p >= ww * wh || (km[u] & badMask) > 0) continue;
+            const double kv = (double)kvf[u];
+            if (kv >= A.hi) {
+              mask[kc[u]] = km[u] | A.badPixelMask;
+              continue;
+            }
+            if ((kv - sky) * (1.0 / fwhm) < A.tkf) continue;
+            if (kv > bestV) {
+              bestV = kv;
+              bestI = p;
+            }
+          }
         }
         if (bestI == INT_MAX) bestV = -INFINITY;
-        int win = block_argmax_first(bestV, bestI, dsc, isc2);
+        const int win = wave_argmax_first(bestV, bestI);
+        __threadfence_block();  // the window's marks before the check reads
         int mx = absx, my = absy;
         if (win != INT_MAX) {
           mx = startX + win % ww;
@@ -465,83 +498,85 @@ __global__ __launch_bounds__(256) void k_sss_find(FindArgs A) {
         int badType = 0;
         double part = 0.0;
         int eMin = 1000, eMax = -1000;
-        for (int p = tid; p < cw * ch; p += blockDim.x) {
-          int x = cx0 + p % cw, y = cy0 + p / cw;
-          long ac = x + y * imgW;
-          if ((mask[ac] & badMask) > 0) {
-            if (p < firstBad) { firstBad = p; badType = 1; }
-            continue;
+        {
+          uint16_t cm[WIN_PER_LANE];
+          float cvf[WIN_PER_LANE];
+#pragma unroll
+          for (int u = 0; u < WIN_PER_LANE; u++) {
+            const int p = lane + u * WAVE;
+            const bool in = p < cw * ch;
+            const long ac = in ? (cx0 + p % cw) + (long)(cy0 + p / cw) * imgW : 0;
+            cm[u] = in ? mask[ac] : (uint16_t)0;
+            cvf[u] = in ? img[ac] : 0.0f;
           }
-          float fv = img[ac];
-          double v = (double)fv;
-          if (v > A.hi) {
-            if (p < firstBad) { firstBad = p; badType = 2; }
-            continue;
-          }
-          if ((v - sky) / fwhm > A.tkf) {
-            part += v;
-            if (fv != 0.0f) {
-              int e = float_exp_field(fv);
-              eMin = min(eMin, e);
-              eMax = max(eMax, e);
+#pragma unroll
+          for (int u = 0; u < WIN_PER_LANE; u++) {
+            const int p = lane + u * WAVE;
+            if (p >= cw * ch) continue;
+            if ((cm[u] & badMask) > 0) {
+              if (p < firstBad) { firstBad = p; badType = 1; }
+              continue;
+            }
+            const float fv = cvf[u];
+            const double v = (double)fv;
+            if (v > A.hi) {
+              if (p < firstBad) { firstBad = p; badType = 2; }
+              continue;
+            }
+            if ((v - sky) / fwhm > A.tkf) {
+              part += v;
+              if (fv != 0.0f) {
+                const int e = float_exp_field(fv);
+                eMin = min(eMin, e);
+                eMax = max(eMax, e);
+              }
             }
           }
         }
         // first bad pixel in raster order
-        int fb = block_min_int(firstBad, isc);
+        const int fb = wave_min_int(firstBad);
         if (fb != INT_MAX) {
-          if (firstBad == fb && badType == 2 && tid == (fb % blockDim.x)) {
-            int x = cx0 + fb % cw, y = cy0 + fb / cw;
-            long ac = x + y * imgW;
+          if (firstBad == fb && badType == 2) {
+            const long ac = (cx0 + fb % cw) + (long)(cy0 + fb / cw) * imgW;
             mask[ac] = mask[ac] | A.badPixelMask;
           }
-          __syncthreads();
+          __threadfence_block();
           continue;  // checkSStamp returned 0
         }
         // sum of qualifying values in raster order: exact in any order when
         // the exponent spread is small (all partial sums representable),
-        // otherwise summed sequentially by one thread.
-        int gMin = block_min_int(eMin, isc);
-        int gMax = block_max_int(eMax, isc);
+        // otherwise summed sequentially by one lane.
+        const int gMin = wave_min_int(eMin);
+        const int gMax = wave_max_int(eMax);
         double total;
         if (gMax < gMin || gMax - gMin <= 19) {
-          total = block_sum_double_tree(part, dsc);
+          total = wave_sum_exact(part);
         } else {
-          if (tid == 0) {
-            double r = 0.0;
+          double r = 0.0;
+          if (lane == 0)
             for (int p = 0; p < cw * ch; p++) {
-              int x = cx0 + p % cw, y = cy0 + p / cw;
-              double v = (double)img[x + y * imgW];
+              const double v = (double)img[(cx0 + p % cw) + (long)(cy0 + p / cw) * imgW];
               if ((v - sky) / fwhm > A.tkf) r += v;
             }
-            dsc[18] = r;
-          }
-          __syncthreads();
-          total = dsc[18];
-          __syncthreads();
+          total = __shfl(r, 0);
         }
-        if (total == 0.0) {
-          __syncthreads();
-          continue;
+        if (total == 0.0) continue;
+        if (lane == 0) {
+          lx[counter] = mx;
+          ly[counter] = my;
+          lv[counter] = total;
         }
-        if (tid == 0) {
-          int c = s_counter;
-          lx[c] = mx;
-          ly[c] = my;
-          lv[c] = total;
-          s_counter = c + 1;
-        }
+        counter++;
         // skip marks (sss.cl:473-482), clamped with the stamp size
-        for (int p = tid; p < cw * ch; p += blockDim.x) {
-          int x = cx0 + p % cw, y = cy0 + p / cw;
-          long ac = x + y * imgW;
+        for (int p = lane; p < cw * ch; p += WAVE) {
+          const long ac = (cx0 + p % cw) + (long)(cy0 + p / cw) * imgW;
           mask[ac] = mask[ac] | A.skipMask;
         }
-        __syncthreads();
-        if (s_counter >= MAXSS) break;
+        __threadfence_block();
       }
-      __syncthreads();
+      if (lane == 0) s_counter = counter;
     }
+    __syncthreads();
     if (lim == floorV) break;
     dfrac -= 0.2;
   }
@@ -639,7 +674,7 @@ hipError_t launch_sss_find(Ctx& c, int im) {
   // the good-pixel scratch of k_sss_stats is free again (same size)
   A.candAll = reinterpret_cast<int*>(c.goodPix);
   prof_begin(c, XB_PROF_SSS);
-  hipLaunchKernelGGL(k_sss_find, dim3(c.nGrid), dim3(256), 0, c.stream, A);
+  hipLaunchKernelGGL(k_sss_find, dim3(c.nGrid), dim3(FIND_THREADS), 0, c.stream, A);
   prof_end(c, XB_PROF_SSS);
   return hipGetLastError();
 }

@@ -245,7 +245,7 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
     XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
     XB_HIP(hipMalloc((void**)&c.luIndex, sizeof(int) * MATN));
     XB_HIP(hipMalloc((void**)&c.luVV, sizeof(double) * MATN));
-    XB_HIP(hipMalloc((void**)&c.luState, 64));
+    XB_HIP(hipMalloc((void**)&c.luState, LU_STATE_BYTES));
   }
   static_assert(sizeof(Basis) < (1 << 20), "basis size");
   Basis hb;
@@ -648,7 +648,7 @@ int xb_lu_solve(xb_ctx* x, const double* a, const double* b, double* out) {
     XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
     XB_HIP(hipMalloc((void**)&c.luIndex, sizeof(int) * MATN));
     XB_HIP(hipMalloc((void**)&c.luVV, sizeof(double) * MATN));
-    XB_HIP(hipMalloc((void**)&c.luState, 64));
+    XB_HIP(hipMalloc((void**)&c.luState, LU_STATE_BYTES));
   }
   const size_t mb = sizeof(double) * MATN * MATN, vb = sizeof(double) * NSOL;
   XB_HIP(hipMemcpyAsync(c.M, a, mb, hipMemcpyHostToDevice, c.stream));
