@@ -29,15 +29,20 @@ sys.path.insert(0, str(ROOT / "thesis-emx-bach_amd"))
 import xbach  # noqa: E402
 
 METRIC = "2048² diffs/sec: plain FP32 vs mixed+batched+screened, at equal residual RMS"
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix alike)
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per k_conv launch on the 2048x2048 pair: rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE of k_conv, each scaled by what the same counter reads for a
+# device copy of known size (bench.py --copy-calibration); the summary with
+# both commands is profiles/k_conv_2048_traffic.md.  None until measured.
+CONV_TRAFFIC_BYTES = None
 
 WORKLOADS = {
     "pair2048": ("2048x2048 FP32 pair, 6000 stars (dN/dF ~ F^-1.5, 1e2-1e5), template FWHM 1.8 px, "
                  "science FWHM 2.5->3.5 px ramp in x and y, sky 200 + 0.02x + 0.01y, Poisson noise; "
                  "49-term basis (sigma 0.7/1.5/3.0, degrees 6/4/2), 21x21 kernel, 10x10 stamps, "
-                 "31x31 substamps, kernel order 2, background order 1; plain form: FP64 fit "
-                 "bit-identical to the reference's arithmetic, FP32 convolution"),
+                 "31x31 substamps, kernel order 2, background order 1; plain form: FP64 fit and "
+                 "FP64 convolution bit-identical to the reference's arithmetic, FP32 images"),
     "pair512": ("512x512 FP32 pair, 300 stars, FWHM 2.0 / 3.0 px, sky 100, Poisson noise; "
                 "plain form"),
 }
@@ -80,7 +85,20 @@ def main() -> int:
     ap.add_argument("--workload", default="pair2048", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device-to-device copy of 256 MiB (counter scale for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 256 << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU split is a design note, not a built path)", file=sys.stderr)
@@ -118,7 +136,9 @@ def main() -> int:
     good = diff != np.float32(1e-30)
     rms = float(np.sqrt(np.mean(diff[good].astype(np.float64) ** 2)))
 
-    # roofline of the convolution kernel (the data-parallel hot kernel)
+    # roofline of the convolution kernel (the data-parallel hot kernel): the
+    # FP64 441-tap stencil at 882 flop per output pixel is compute-bound
+    # (about 70 flop per HBM byte, far above the FP64 ridge of ~10)
     conv_ms, conv_n = prof["conv"]
     conv_avg_s = conv_ms / max(conv_n, 1) * 1e-3
     conv_flops = 2.0 * 441 * (w - 20) * (h - 20)
@@ -142,15 +162,16 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 fit + f32 conv",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.workload], "name": args.workload,
                    "form": "plain", "w": w, "h": h, "seed": int(cfg.seed)},
-        # compute-bound FP32 kernel; 157.3 TFLOP/s is MI355X's dense FP32 peak
-        # (MFMA and packed VALU alike) -- k_conv issues scalar FP32 FMAs
-        "roofline": {"bound": "mfma", "kernel": "k_conv (fp32 VALU FMA)",
-                     "achieved": conv_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": conv_tflops / PEAK_FP32_TFLOPS, "traffic": None,
+        # k_conv issues FP64 VALU multiply-adds (unfused, in the reference's
+        # term order); MI355X's dense FP64 peak is 78.6 TFLOP/s
+        "roofline": {"bound": "mfma", "kernel": "k_conv (fp64 VALU, unfused mul+add)",
+                     "achieved": conv_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": conv_tflops / PEAK_FP64_TFLOPS,
+                     "traffic": CONV_TRAFFIC_BYTES if (w, h) == (2048, 2048) else None,
                      "algorithmic": f"2*441*(W-20)*(H-20) = {conv_flops:.4g} flop per launch"},
         "kernels": kernels,
         "residual_rms": rms,

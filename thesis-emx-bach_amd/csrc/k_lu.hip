@@ -517,15 +517,15 @@ __global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const double
   // prefetched into registers meanwhile); lane 0 runs the ordered subtraction
   // chain, the only sequential part, with its operands loaded one 8-block
   // ahead of their use.
-  constexpr int CH = 8;  // chain operands loaded per block
+  constexpr int CH = 16;  // chain operands loaded per block
   // products P[j] = U_ij x_j; entries past n stay +0.0, and s - (+0.0) is s
   // bit for bit (also for -0, inf and NaN), so the chain always consumes
-  // whole blocks of CH without a scalar tail
-  __shared__ double P[MATN + CH];
+  // whole blocks of CH without a scalar tail (and may load one block past it)
+  __shared__ double P[MATN + 2 * CH];
   constexpr int PER = (MATSIZE + WAVE - 1) / WAVE;  // row entries per lane
   constexpr int D = 8;                               // rows prefetched ahead
   const int lane = threadIdx.x;
-  for (int j = lane; j < MATN + CH; j += WAVE) P[j] = 0.0;
+  for (int j = lane; j < MATN + 2 * CH; j += WAVE) P[j] = 0.0;
   // ring of the next D rows of U (entries right of the diagonal, and the
   // diagonal), slot d holding rows i with (n - i) % D == d
   double ring[D][PER];
@@ -568,21 +568,26 @@ __global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const double
         if (lane == 0) {
           double s = x[i];
           if (i < n) {
-            // blocks j0 = i+1, i+1+CH, ... up to the one holding n
-            double c[CH];
+            // blocks of CH from j = i+1 up to the one holding n, two register
+            // sets in turn: each block is loaded one block ahead of its use
+            // and no register copies sit on the chain's issue slots
+            int j = i + 1;
+            double a[CH], c[CH];
 #pragma unroll
-            for (int t = 0; t < CH; t++) c[t] = P[i + 1 + t];
-            for (int j = i + 1 + CH; j <= n; j += CH) {
-              double e[CH];
+            for (int t = 0; t < CH; t++) a[t] = P[j + t];
+            for (;;) {
 #pragma unroll
-              for (int t = 0; t < CH; t++) e[t] = P[j + t];
+              for (int t = 0; t < CH; t++) c[t] = P[j + CH + t];
+#pragma unroll
+              for (int t = 0; t < CH; t++) s = s - a[t];
+              if (j + CH > n) break;
+#pragma unroll
+              for (int t = 0; t < CH; t++) a[t] = P[j + 2 * CH + t];
 #pragma unroll
               for (int t = 0; t < CH; t++) s = s - c[t];
-#pragma unroll
-              for (int t = 0; t < CH; t++) c[t] = e[t];
+              if (j + 2 * CH > n) break;
+              j += 2 * CH;
             }
-#pragma unroll
-            for (int t = 0; t < CH; t++) s = s - c[t];
           }
           x[i] = s / dg;
         }
