@@ -93,10 +93,11 @@ __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall
       const double dum = 1.0 / A[j][j];
       for (int i = j + 1 + tid; i <= n; i += blockDim.x) A[i][j] *= dum;
       __syncthreads();
-      const int m = n - j;
-      for (int e = tid; e < m * m; e += blockDim.x) {
-        int i = j + 1 + e / m, l = j + 1 + e % m;
-        A[i][l] = A[i][l] - A[i][j] * A[j][l];
+      // threads across the columns (n <= 64), rows in turn: no index division
+      const int l = j + 1 + tid;
+      if (l <= n) {
+        const double ujl = A[j][l];
+        for (int i = j + 1; i <= n; i++) A[i][l] = A[i][l] - A[i][j] * ujl;
       }
     }
     __syncthreads();

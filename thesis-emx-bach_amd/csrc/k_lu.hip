@@ -26,8 +26,8 @@
 //   k_lu_backsub back substitution; its ascending-j sums are one sequential
 //                chain per row in the reference order.
 //
-// Panels are factorised by a single wave holding the rows in registers (lane
-// L: rows c1 + L + 64 s).  A row interchange relabels the two rows (logical
+// Panels are factorised by the whole workgroup holding the rows in registers
+// (block_panel_factor).  A row interchange relabels the two rows (logical
 // position numbers) instead of moving them; vv stays attached to its row, as
 // ludcmp's vv[imax] = vv[j] keeps it for every position still searched.  The
 // factorised rows are stored at their final positions.
@@ -41,7 +41,7 @@ constexpr int LB = 16;          // panel width = trailing columns per workgroup
 constexpr int LDP = LB + 1;     // LDS pitch of a 16-column slab
 constexpr int NPANEL = (MATSIZE + LB - 1) / LB;
 constexpr int LU_THREADS = 256;
-constexpr int NSLOT = (MATSIZE + WAVE - 1) / WAVE;  // panel rows per lane
+constexpr int NSLOT = (MATSIZE + WAVE - 1) / WAVE;  // row entries per lane
 
 struct LuState {
   int zero;      // a zero row: ludcmp returned 1 without factorising
@@ -51,12 +51,6 @@ struct LuState {
   int skip;      // empty stamp list: the reference never solves
   int pad[3];
 };
-
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-  return __hiloint2double(hi, lo);
-}
 
 // One step of a (value, index) maximum across lanes (ties to the larger
 // index), the partner chosen by a DPP control within each row of 16 lanes.
@@ -75,7 +69,7 @@ __device__ __forceinline__ void argmax_dpp(double& bv, int& bi) {
 // Wave-wide (value, index) maximum: rotations by 1, 2, 4, 8 within each row
 // of 16 lanes (DPP row_ror), then the two cross-row exchanges.  Every lane
 // ends with the result (a maximum under a total order: any combining order).
-__device__ __forceinline__ int wave_argmax(double bv, int bi) {
+__device__ __forceinline__ void wave_argmax(double& bv, int& bi) {
   argmax_dpp<0x121>(bv, bi);
   argmax_dpp<0x122>(bv, bi);
   argmax_dpp<0x124>(bv, bi);
@@ -89,24 +83,46 @@ __device__ __forceinline__ int wave_argmax(double bv, int bi) {
       bi = oi;
     }
   }
-  return __builtin_amdgcn_readfirstlane(bi);
 }
 
-// Factorise columns c1..c1+cB-1 over rows c1..n with the calling wave.
-// S[r - c1][q] holds matrix element (r, c1 + q) (updated by all earlier
-// columns); vin[r - c1] the row scale of position r; lastMaxI the pivot row of
-// column c1 - 1.  Results go to M, vout (by position), piv and st.
-__device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1, int cB,
-                                                  const double* vin, int lastMaxI, double* M,
-                                                  double* vout, int* piv, LuState* st) {
+constexpr int PSLOT = (MATSIZE + LU_THREADS - 1) / LU_THREADS;  // panel rows per thread
+
+// Factorise columns c1..c1+cB-1 over rows c1..n with the whole workgroup
+// (every thread calls it).  S[r - c1][q] holds matrix element (r, c1 + q)
+// (updated by all earlier columns); vin[r - c1] the row scale of position r;
+// lastMaxI the pivot row of column c1 - 1.  Thread t holds rows
+// c1 + t + 256 s in registers.  A row interchange relabels the two rows
+// (logical positions) instead of moving them.  Each wave's best candidate
+// row travels to LDS with the vote, so every thread reads the pivot row right
+// after the column's single barrier.  When a row becomes the pivot of column
+// k its owner also copies it to LDS (its L part and U part are final then; it
+// is stored at position k after the last column); from then on its
+// registers are never read again (the pivot search only looks at positions
+// >= k and the final store only writes rows that were never pivots), so the
+// column updates run unconditionally on every register row.
+__device__ __forceinline__ void block_panel_factor(const double (*S)[LDP], int c1, int cB,
+                                                   const double* vin, int lastMaxI, double* M,
+                                                   double* vout, int* piv, LuState* st) {
   const int n = MATSIZE, N = MATN;
-  const int lane = lane_id();
-  double a[NSLOT][LB];
-  double vr[NSLOT];
-  int lidx[NSLOT];
+  constexpr int NW = LU_THREADS / WAVE;
+  const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
+  // per column (double-buffered by parity): each wave's best candidate
+  // (value, position) and that candidate's row, so the vote and the pivot
+  // row arrive behind one barrier
+  __shared__ double s_bv[2][NW];
+  __shared__ int s_bi[2][NW];
+  __shared__ double cand[2][NW][LB];
+  // pivot rows and pivot positions of this panel: kept in LDS and stored to
+  // global memory once at the end (a barrier waits for outstanding global
+  // stores, so none are issued inside the column loop)
+  __shared__ double prow[LB][LB];
+  __shared__ int ppiv[LB];
+  double a[PSLOT][LB];
+  double vr[PSLOT];
+  int lidx[PSLOT];
 #pragma unroll
-  for (int s = 0; s < NSLOT; s++) {
-    const int r = c1 + lane + WAVE * s;
+  for (int s = 0; s < PSLOT; s++) {
+    const int r = c1 + tid + LU_THREADS * s;
     const bool ok = r <= n;
     lidx[s] = ok ? r : -1;
     vr[s] = ok ? vin[r - c1] : 0.0;
@@ -118,21 +134,46 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
 #pragma unroll
   for (int q = 0; q < LB; q++) {
     // fully unrolled (register-resident a[][]): columns past cB, or after a
-    // replay was flagged, are skipped by a wave-uniform guard
+    // replay was flagged, are skipped by a workgroup-uniform guard
     const int k = c1 + q;
+    const int par = q & 1;
     if (q < cB && !fb) {
       // pivot search: max vv*|a| over positions >= k, ties to the larger position
       double bv = -1.0;
       int bi = -1;
 #pragma unroll
-      for (int s = 0; s < NSLOT; s++) {
+      for (int s = 0; s < PSLOT; s++) {
         const double dum = vr[s] * fabs(a[s][q]);
         if (lidx[s] >= k && dum >= 0.0 && (dum > bv || (dum == bv && lidx[s] > bi))) {
           bv = dum;
           bi = lidx[s];
         }
       }
-      bi = wave_argmax(bv, bi);
+      wave_argmax(bv, bi);
+      if (lane == 0) {
+        s_bv[par][wid] = bv;
+        s_bi[par][wid] = bi;
+      }
+#pragma unroll
+      for (int s = 0; s < PSLOT; s++)
+        if (bi >= 0 && lidx[s] == bi) {
+#pragma unroll
+          for (int l = 0; l < LB; l++) cand[par][wid][l] = a[s][l];
+        }
+      __syncthreads();
+      bv = s_bv[par][0];
+      bi = s_bi[par][0];
+      int bw = 0;
+#pragma unroll
+      for (int w = 1; w < NW; w++) {
+        const double ov = s_bv[par][w];
+        const int oi = s_bi[par][w];
+        if (ov > bv || (ov == bv && oi > bi)) {
+          bv = ov;
+          bi = oi;
+          bw = w;
+        }
+      }
       if (bi >= 0) maxI = bi;
       if (maxI < k) {
         // No row qualified and the kept pivot row lies above this column:
@@ -140,64 +181,71 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
         // literal replay reproduces (k_lu_fallback).
         fb = true;
       } else {
+        if (bi < 0) {
+          // no row qualified: the kept pivot row maxI (>= k) is used, and its
+          // owner publishes it first
+          __syncthreads();
+#pragma unroll
+          for (int s = 0; s < PSLOT; s++)
+            if (lidx[s] == maxI) {
+#pragma unroll
+              for (int l = 0; l < LB; l++) cand[par][0][l] = a[s][l];
+            }
+          __syncthreads();
+          bw = 0;
+        }
         if (maxI != k) {
 #pragma unroll
-          for (int s = 0; s < NSLOT; s++)
+          for (int s = 0; s < PSLOT; s++)
             lidx[s] = lidx[s] == k ? maxI : (lidx[s] == maxI ? k : lidx[s]);
         }
-        if (lane == 0) piv[k] = maxI;
+        if (tid == 0) ppiv[q] = maxI;
 #pragma unroll
-        for (int s = 0; s < NSLOT; s++)
-          if (lidx[s] == k && a[s][q] == 0.0) a[s][q] = 1.0e-20;
-        if (k != n) {
-          // the pivot row (position k): lane pl, slot ps — both wave-uniform
-          int pl = 0, ps = 0;
+        for (int s = 0; s < PSLOT; s++)
+          if (lidx[s] == k) {
+            if (a[s][q] == 0.0) a[s][q] = 1.0e-20;
 #pragma unroll
-          for (int s = 0; s < NSLOT; s++) {
-            const unsigned long long m = __ballot(lidx[s] == k);
-            if (m) {
-              pl = __ffsll((unsigned long long)m) - 1;
-              ps = s;
-            }
+            for (int l = 0; l < LB; l++) prow[q][l] = a[s][l];
           }
+        if (k != n) {
           double u[LB];
 #pragma unroll
-          for (int l = 0; l < LB; l++) u[l] = 0.0;
-#pragma unroll
-          for (int s = 0; s < NSLOT; s++)
-            if (s == ps) {
-#pragma unroll
-              for (int l = q; l < LB; l++) u[l] = readlane_f64(a[s][l], pl);
-            }
+          for (int l = q; l < LB; l++) u[l] = cand[par][bw][l];
+          if (u[q] == 0.0) u[q] = 1.0e-20;
           const double dinv = 1.0 / u[q];
 #pragma unroll
-          for (int s = 0; s < NSLOT; s++)
-            if (lidx[s] > k) a[s][q] *= dinv;
+          for (int s = 0; s < PSLOT; s++) a[s][q] *= dinv;
 #pragma unroll
           for (int l = q + 1; l < LB; l++) {
 #pragma unroll
-            for (int s = 0; s < NSLOT; s++)
-              if (lidx[s] > k) a[s][l] = a[s][l] - a[s][q] * u[l];
+            for (int s = 0; s < PSLOT; s++) a[s][l] = a[s][l] - a[s][q] * u[l];
           }
         }
       }
     }
   }
+  __syncthreads();
   if (fb) {
-    if (lane == 0) st->fallback = 1;
+    if (tid == 0) st->fallback = 1;
     return;
   }
+  // the panel's pivot rows (positions c1..c1+cB-1) and pivot indices
+  for (int e = tid; e < LB * LB; e += LU_THREADS) {
+    const int q = e / LB, l = e % LB;
+    if (q < cB && l < cB) M[(size_t)(c1 + q) * N + c1 + l] = prow[q][l];
+  }
+  if (tid < cB) piv[c1 + tid] = ppiv[tid];
 #pragma unroll
-  for (int s = 0; s < NSLOT; s++) {
-    if (lidx[s] > 0) {
+  for (int s = 0; s < PSLOT; s++) {
+    if (lidx[s] >= c1 + cB) {
       double* row = M + (size_t)lidx[s] * N + c1;
 #pragma unroll
       for (int q = 0; q < LB; q++)
         if (q < cB) row[q] = a[s][q];
-      vout[lidx[s]] = vr[s];
     }
+    if (lidx[s] > 0) vout[lidx[s]] = vr[s];
   }
-  if (lane == 0) st->lastMaxI = maxI;
+  if (tid == 0) st->lastMaxI = maxI;
 }
 
 __device__ __forceinline__ int* lu_zero_flags(LuState* st) {
@@ -283,7 +331,7 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, 
     st->ii = 0;
   }
   if (s_zero) return;  // ludcmp returns before factorising
-  if (wid == 0) wave_panel_factor(S, 1, LB, vv + 1, 0, M, vv, piv, st);
+  block_panel_factor(S, 1, LB, vv + 1, 0, M, vv, piv, st);
 }
 
 __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, int* piv,
@@ -410,12 +458,12 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
   __syncthreads();
   if (blockIdx.x == 0) {
     // lookahead: these columns are the next panel; store its U rows, then
-    // factorise rows l0..n with one wave
+    // factorise rows l0..n
     for (int e = tid; e < LB * LB; e += LU_THREADS) {
       const int r = e / LB, t = e % LB;
       if (t < nc) M[(size_t)(c0 + r) * N + l0 + t] = T[r][t];
     }
-    if (tid < WAVE) wave_panel_factor(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, st);
+    block_panel_factor(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, st);
     return;
   }
   for (int e = tid; e < R * LB; e += LU_THREADS) {
