@@ -314,7 +314,7 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax) {
   A.w = c.w;
   A.h = c.h;
   A.list = d_list;
-  A.nList = d_list ? &c.sc->nList : &c.sc->nKept[im];
+  A.nList = d_list ? &c.sc->nList[im] : &c.sc->nKept[im];
   A.kept = c.st[im].kept;
   A.cur = c.st[im].cur;
   A.count = c.st[im].count;
@@ -326,14 +326,14 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax) {
   A.Bc = c.st[im].Bc;
   A.Br = c.st[im].Br;
   prof_begin(c, XB_PROF_FILL);
-  hipLaunchKernelGGL(k_fill_conv, dim3(nMax, WROWS), dim3(256), 0, c.stream, A);
+  hipLaunchKernelGGL(k_fill_conv, dim3(nMax, WROWS), dim3(256), 0, c.cur, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill_gram, dim3(8 * ((nMax * NGT + 7) / 8)), dim3(WAVE), 0, c.stream, A,
+  hipLaunchKernelGGL(k_fill_gram, dim3(8 * ((nMax * NGT + 7) / 8)), dim3(WAVE), 0, c.cur, A,
                      nMax);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill_target, dim3(nMax), dim3(256), 0, c.stream, A);
+  hipLaunchKernelGGL(k_fill_target, dim3(nMax), dim3(256), 0, c.cur, A);
   prof_end(c, XB_PROF_FILL);
   return hipGetLastError();
 }

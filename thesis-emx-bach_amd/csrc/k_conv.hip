@@ -59,8 +59,8 @@ __device__ __forceinline__ double background_c(const double* sol, int x, int y, 
 __global__ __launch_bounds__(512) void k_tile_kernels(const Basis* __restrict__ basis,
                                                       const double* __restrict__ sol, int w,
                                                       int h, int xSteps, int nTiles,
-                                                      float* tileK, double* tileK64,
-                                                      double* tileAks, double* kernSumOut) {
+                                                      double* tileK64, double* tileAks,
+                                                      double* kernSumOut) {
   const int t = blockIdx.x;
   const int tid = threadIdx.x;
   int x, y;
@@ -79,12 +79,11 @@ __global__ __launch_bounds__(512) void k_tile_kernels(const Basis* __restrict__ 
   __syncthreads();
   if (tid < KK) {
     double k0 = 0.0;
+    // 7 terms' loads in flight together; the sum stays in term order
+#pragma unroll 7
     for (int p = 0; p < NPSF; p++) k0 += co[p] * basis->vec[p * KK + tid];
     kern[tid] = k0;
-    if (t < nTiles) {
-      tileK[(size_t)t * KK + tid] = (float)k0;
-      tileK64[(size_t)t * KK + tid] = k0;
-    }
+    if (t < nTiles) tileK64[(size_t)t * KK + tid] = k0;
   }
   __syncthreads();
   if (tid == 0) {
@@ -106,7 +105,6 @@ struct ConvArgs {
   uint16_t* mask;
   float* out;
   double* outD;        // FP64 convolved image (what sub reads, as the reference)
-  const float* tileK;
   const double* tileK64;
   const double* tileAks;
   const double* sol;
@@ -260,8 +258,8 @@ hipError_t launch_conv(Ctx& c) {
   const int nTiles = xSteps * ySteps;
   const int ti = c.tIdx;
   double* kernSum = &c.sc->kernSum;
-  hipLaunchKernelGGL(k_tile_kernels, dim3(nTiles + 1), dim3(512), 0, c.stream, c.basis, c.sol,
-                     c.w, c.h, xSteps, nTiles, c.tileK, c.tileK64, c.tileAks, kernSum);
+  hipLaunchKernelGGL(k_tile_kernels, dim3(nTiles + 1), dim3(512), 0, c.cur, c.basis, c.sol,
+                     c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks, kernSum);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   ConvArgs A;
@@ -270,7 +268,6 @@ hipError_t launch_conv(Ctx& c) {
   A.mask = c.mask;
   A.out = c.conv;
   A.outD = c.convD;
-  A.tileK = c.tileK;
   A.tileK64 = c.tileK64;
   A.tileAks = c.tileAks;
   A.sol = c.sol;
@@ -283,11 +280,11 @@ hipError_t launch_conv(Ctx& c) {
   A.hi = c.m.threshHigh;
   A.lo = c.m.threshLow;
   prof_begin(c, XB_PROF_CONV);
-  hipLaunchKernelGGL(k_conv, dim3((xSteps + 3) / 4, ySteps), dim3(256), 0, c.stream, A);
+  hipLaunchKernelGGL(k_conv, dim3((xSteps + 3) / 4, ySteps), dim3(256), 0, c.cur, A);
   e = hipGetLastError();
   prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_conv_border, dim3(1024), dim3(256), 0, c.stream, c.img[1 - ti], c.mask,
+  hipLaunchKernelGGL(k_conv_border, dim3(1024), dim3(256), 0, c.cur, c.img[1 - ti], c.mask,
                      c.conv, c.convD, c.w, c.h, c.m.threshHigh, c.m.threshLow);
   return hipGetLastError();
 }
@@ -298,7 +295,7 @@ hipError_t launch_sub(Ctx& c) {
   size_t n = (size_t)c.w * c.h;
   unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
   prof_begin(c, XB_PROF_SUB);
-  hipLaunchKernelGGL(k_sub, dim3(blocks), dim3(256), 0, c.stream, c.img[1 - ti], c.convD, c.mask,
+  hipLaunchKernelGGL(k_sub, dim3(blocks), dim3(256), 0, c.cur, c.img[1 - ti], c.convD, c.mask,
                      c.diff, c.w, c.h, &c.sc->kernSum, scaleConv);
   prof_end(c, XB_PROF_SUB);
   return hipGetLastError();

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(1024) void k_cd_select(const double* ksum, const in
   }
   if (threadIdx.x == 0) {
     sc->nTest[im] = s_n;
-    sc->nList = s_n;
+    sc->nList[im] = s_n;
   }
 }
 
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
     __syncthreads();
   }
   if (tid == 0) {
-    sc->nList = s_n;
+    sc->nList[im] = s_n;
     sc->check = s_n > 0 ? 1 : 0;
     sc->nChi2 = nChi;
     sc->clipMean = mean;
@@ -608,13 +608,13 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
 // ---------------------------------------------------------------------------
 hipError_t launch_stamp_lu(Ctx& c, int im) {
   if (c.nKept[im] <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_stamp_lu, dim3(c.nKept[im]), dim3(64), 0, c.stream, c.st[im].G,
+  hipLaunchKernelGGL(k_stamp_lu, dim3(c.nKept[im]), dim3(64), 0, c.cur, c.st[im].G,
                      c.st[im].Bc, &c.sc->nKept[im], c.st[im].ksum);
   return hipGetLastError();
 }
 
 hipError_t launch_cd_select(Ctx& c, int im) {
-  hipLaunchKernelGGL(k_cd_select, dim3(1), dim3(1024), 0, c.stream, c.st[im].ksum,
+  hipLaunchKernelGGL(k_cd_select, dim3(1), dim3(1024), 0, c.cur, c.st[im].ksum,
                      &c.sc->nKept[im], c.st[im].list, c.sc, im, c.m.sigClipAlpha,
                      c.m.sigKernFit);
   return hipGetLastError();
@@ -631,16 +631,16 @@ hipError_t launch_global_build(Ctx& c, int im, bool cdForm) {
   A.ssx = c.st[im].ssx;
   A.ssy = c.st[im].ssy;
   A.list = cdForm ? c.st[im].list : nullptr;
-  A.nList = cdForm ? &c.sc->nList : &c.sc->nKept[im];
+  A.nList = cdForm ? &c.sc->nList[im] : &c.sc->nKept[im];
   A.w = c.w;
   A.h = c.h;
   A.cdForm = cdForm ? 1 : 0;
-  A.M = c.M;
-  A.rhs = c.rhs;
-  A.M0 = c.M0;
-  A.rhs0 = c.rhs0;
+  A.M = c.lu[im].M;
+  A.rhs = c.lu[im].rhs;
+  A.M0 = c.lu[im].M0;
+  A.rhs0 = c.lu[im].rhs0;
   const int total = MATN * (MATN + 1) / 2 + NSOL;
-  hipLaunchKernelGGL(k_global_build, dim3((total + 255) / 256), dim3(256), 0, c.stream, A);
+  hipLaunchKernelGGL(k_global_build, dim3((total + 255) / 256), dim3(256), 0, c.cur, A);
   return hipGetLastError();
 }
 
@@ -659,24 +659,24 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol) {
   A.ssx = c.st[im].ssx;
   A.ssy = c.st[im].ssy;
   A.list = useList ? c.st[im].list : nullptr;
-  A.nList = useList ? &c.sc->nList : &c.sc->nKept[im];
+  A.nList = useList ? &c.sc->nList[im] : &c.sc->nKept[im];
   A.basis = c.basis;
   A.sol = d_sol;
   A.sigOut = c.st[im].sig;
   if (c.nKept[im] <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(256), 0, c.stream, A);
+  hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(256), 0, c.cur, A);
   return hipGetLastError();
 }
 
 hipError_t launch_cd_finish(Ctx& c, int im) {
-  hipLaunchKernelGGL(k_cd_finish, dim3(1), dim3(1024), 0, c.stream, c.basis,
+  hipLaunchKernelGGL(k_cd_finish, dim3(1), dim3(1024), 0, c.cur, c.basis,
                      c.cdSol + (size_t)im * NSOL, c.st[im].sig, c.sc, im, c.w, c.h,
                      c.m.sigClipAlpha);
   return hipGetLastError();
 }
 
 hipError_t launch_ksc_check(Ctx& c, int im) {
-  hipLaunchKernelGGL(k_ksc_check, dim3(1), dim3(1024), 0, c.stream, c.st[im].sig,
+  hipLaunchKernelGGL(k_ksc_check, dim3(1), dim3(1024), 0, c.cur, c.st[im].sig,
                      c.st[im].kept, c.st[im].cur, c.st[im].count, c.st[im].list, c.sc, im,
                      c.m.sigClipAlpha, c.m.sigKernFit);
   return hipGetLastError();

@@ -65,7 +65,7 @@ hipError_t launch_mask_input(Ctx& c) {
   if (w2 < 0 || w2 > MAX_W2) return hipErrorInvalidValue;
   dim3 grid((c.w + MT_X - 1) / MT_X, (c.h + MT_Y - 1) / MT_Y);
   prof_begin(c, XB_PROF_MASK);
-  hipLaunchKernelGGL(k_mask_input, grid, dim3(256), 0, c.stream, c.img[0], c.img[1], c.mask,
+  hipLaunchKernelGGL(k_mask_input, grid, dim3(256), 0, c.cur, c.img[0], c.img[1], c.mask,
                      c.w, c.h, border, c.m.threshHigh, c.m.threshLow, w2);
   prof_end(c, XB_PROF_MASK);
   return hipGetLastError();

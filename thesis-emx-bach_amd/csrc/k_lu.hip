@@ -307,7 +307,7 @@ __device__ __forceinline__ void stage_slab(double (*D)[LDP], const double* M, in
 __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, int* piv,
                                                          LuState* st, const int* skipIfZero) {
   const int n = MATSIZE, N = MATN;
-  const int tid = threadIdx.x, wid = tid / WAVE;
+  const int tid = threadIdx.x;
   __shared__ int s_zero;
   __shared__ double S[MATSIZE][LDP];
   if (tid == 0) s_zero = 0;
@@ -651,13 +651,13 @@ __global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const double
   }
 }
 
-hipError_t launch_global_solve(Ctx& c, double* d_out, const int* d_skipIfZero) {
-  LuState* st = reinterpret_cast<LuState*>(c.luState);
+hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int* d_skipIfZero) {
+  LuState* st = reinterpret_cast<LuState*>(L.luState);
   prof_begin(c, XB_PROF_SOLVE);
   hipLaunchKernelGGL(k_lu_scale, dim3((MATSIZE + LU_THREADS / WAVE - 1) / (LU_THREADS / WAVE)),
-                     dim3(LU_THREADS), 0, c.stream, c.M, c.luVV, st, d_skipIfZero);
-  hipLaunchKernelGGL(k_lu_begin, dim3(1), dim3(LU_THREADS), 0, c.stream, c.M, c.luVV, c.luIndex,
-                     st, d_skipIfZero);
+                     dim3(LU_THREADS), 0, c.cur, L.M, L.luVV, st, d_skipIfZero);
+  hipLaunchKernelGGL(k_lu_begin, dim3(1), dim3(LU_THREADS), 0, c.cur, L.M, L.luVV, L.luIndex, st,
+                     d_skipIfZero);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   for (int p = 0; p < NPANEL; p++) {
@@ -665,16 +665,16 @@ hipError_t launch_global_solve(Ctx& c, double* d_out, const int* d_skipIfZero) {
     const int cB = std::min(LB, MATSIZE - c0 + 1);
     const int trailing = MATSIZE - (c0 + cB) + 1;
     const int nColBlocks = (trailing + LB - 1) / LB;
-    hipLaunchKernelGGL(k_lu_step, dim3(nColBlocks + 1), dim3(LU_THREADS), 0, c.stream, c.M,
-                       c.luVV, c.luIndex, c.rhs, st, p, nColBlocks);
+    hipLaunchKernelGGL(k_lu_step, dim3(nColBlocks + 1), dim3(LU_THREADS), 0, c.cur, L.M, L.luVV,
+                       L.luIndex, L.rhs, st, p, nColBlocks);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_lu_fallback, dim3(1), dim3(64), 0, c.stream, c.M, c.M0, c.luIndex, st);
+  hipLaunchKernelGGL(k_lu_fallback, dim3(1), dim3(64), 0, c.cur, L.M, L.M0, L.luIndex, st);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(64), 0, c.stream, c.M, c.M0, c.luIndex, c.rhs,
-                     c.rhs0, st, d_out);
+  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(64), 0, c.cur, L.M, L.M0, L.luIndex, L.rhs,
+                     L.rhs0, st, d_out);
   prof_end(c, XB_PROF_SOLVE);
   return hipGetLastError();
 }

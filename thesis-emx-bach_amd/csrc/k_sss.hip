@@ -354,7 +354,9 @@ constexpr int WIN_PER_LANE = (SSW * SSW + WAVE - 1) / WAVE;  // 31x31 window / 6
 // popcount).  Mask bits are only ever added, so a pixel rejected here stays
 // rejected; flagging pixels above threshHigh commutes with everything else.
 // Wave 0 then processes the candidates one by one in raster order, re-checking
-// each against the current mask, with wave-level reductions only.
+// each against the current mask, with wave-level reductions only.  Mask bits
+// are set with 32-bit atomic ORs: the template and science searches run at the
+// same time, each reading only its own bits (badMask) but sharing mask words.
 __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
   const int s = blockIdx.x;
   const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
@@ -402,7 +404,7 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
           if (k < k1 && (m[u] & badMask) == 0) {
             const double dv = (double)v[u];
             if (dv > A.hi) {
-              mask[a[u]] = m[u] | A.badPixelMask;
+              mask_or_atomic(mask, (size_t)a[u], A.badPixelMask);
             } else if (!((dv - sky) * (1.0 / fwhm) < A.tkf) && dv > lim) {
               c = true;
             }
@@ -472,7 +474,7 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
             if (p >= ww * wh || (km[u] & badMask) > 0) continue;
             const double kv = (double)kvf[u];
             if (kv >= A.hi) {
-              mask[kc[u]] = km[u] | A.badPixelMask;
+              mask_or_atomic(mask, (size_t)kc[u], A.badPixelMask);
               continue;
             }
             if ((kv - sky) * (1.0 / fwhm) < A.tkf) continue;
@@ -538,7 +540,7 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
         if (fb != INT_MAX) {
           if (firstBad == fb && badType == 2) {
             const long ac = (cx0 + fb % cw) + (long)(cy0 + fb / cw) * imgW;
-            mask[ac] = mask[ac] | A.badPixelMask;
+            mask_or_atomic(mask, (size_t)ac, A.badPixelMask);
           }
           __threadfence_block();
           continue;  // checkSStamp returned 0
@@ -570,7 +572,7 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
         // skip marks (sss.cl:473-482), clamped with the stamp size
         for (int p = lane; p < cw * ch; p += WAVE) {
           const long ac = (cx0 + p % cw) + (long)(cy0 + p / cw) * imgW;
-          mask[ac] = mask[ac] | A.skipMask;
+          mask_or_atomic(mask, (size_t)ac, A.skipMask);
         }
         __threadfence_block();
       }
@@ -644,7 +646,7 @@ __global__ void k_reset_skip(uint16_t* mask, size_t n) {
 hipError_t launch_sss_stats(Ctx& c, int im) {
   if ((c.fsw * c.fsw + 31) / 32 > MAX_CLIP_BLOCKS) return hipErrorInvalidValue;
   prof_begin(c, XB_PROF_SSS);
-  hipLaunchKernelGGL(k_sss_stats, dim3(c.nGrid), dim3(256), 0, c.stream, c.img[im], c.mask, c.w,
+  hipLaunchKernelGGL(k_sss_stats, dim3(c.nGrid), dim3(256), 0, c.cur, c.img[im], c.mask, c.w,
                      c.h, c.stampsx, c.stampsy, c.fsw, c.ran1, c.goodPix, c.m.sigClipAlpha,
                      c.m.iqRange, c.st[im].sky, c.st[im].fwhm);
   prof_end(c, XB_PROF_SSS);
@@ -671,16 +673,17 @@ hipError_t launch_sss_find(Ctx& c, int im) {
     A.skipMask = XB_MASK_SKIP_S;
   }
   A.badMask = badMask;
-  // the good-pixel scratch of k_sss_stats is free again (same size)
-  A.candAll = reinterpret_cast<int*>(c.goodPix);
+  // the good-pixel scratch of k_sss_stats is free again; each image's search
+  // uses its own half (the two searches run concurrently)
+  A.candAll = reinterpret_cast<int*>(c.goodPix) + (size_t)im * c.capStamps * c.capFsw * c.capFsw;
   prof_begin(c, XB_PROF_SSS);
-  hipLaunchKernelGGL(k_sss_find, dim3(c.nGrid), dim3(FIND_THREADS), 0, c.stream, A);
+  hipLaunchKernelGGL(k_sss_find, dim3(c.nGrid), dim3(FIND_THREADS), 0, c.cur, A);
   prof_end(c, XB_PROF_SSS);
   return hipGetLastError();
 }
 
 hipError_t launch_sss_compact(Ctx& c) {
-  hipLaunchKernelGGL(k_sss_compact, dim3(1), dim3(1024), 0, c.stream, c.st[0].count,
+  hipLaunchKernelGGL(k_sss_compact, dim3(1), dim3(1024), 0, c.cur, c.st[0].count,
                      c.st[1].count, c.st[0].kept, c.st[1].kept, c.st[0].cur, c.st[1].cur, c.sc,
                      c.nGrid);
   return hipGetLastError();
@@ -688,7 +691,7 @@ hipError_t launch_sss_compact(Ctx& c) {
 
 hipError_t launch_reset_skip(Ctx& c) {
   size_t n = (size_t)c.w * c.h;
-  hipLaunchKernelGGL(k_reset_skip, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
+  hipLaunchKernelGGL(k_reset_skip, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.cur,
                      c.mask, n);
   return hipGetLastError();
 }

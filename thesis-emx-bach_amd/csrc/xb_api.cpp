@@ -152,7 +152,6 @@ int ensureImage(Ctx& c, int w, int h) {
   XB_HIP(dalloc(&c.diff, n));
   const int xSteps = (w + KW - 1) / KW, ySteps = (h + KW - 1) / KW;
   const int tiles = xSteps * ySteps;
-  XB_HIP(dalloc(&c.tileK, (size_t)tiles * KK));
   XB_HIP(dalloc(&c.tileK64, (size_t)tiles * KK));
   XB_HIP(dalloc(&c.tileAks, (size_t)tiles));
   c.capTiles = tiles;
@@ -185,7 +184,9 @@ int ensureStamps(Ctx& c, int N, int fsw) {
     c.capFsw = 0;  // good-pixel scratch depends on N too
   }
   if (fsw > c.capFsw) {
-    XB_HIP(dalloc(&c.goodPix, (size_t)c.capStamps * fsw * fsw));
+    // two halves: the substamp searches of the two images run at the same
+    // time and each keeps its candidate lists in its own half
+    XB_HIP(dalloc(&c.goodPix, (size_t)2 * c.capStamps * fsw * fsw));
     c.capFsw = fsw;
   }
   const int draws = 2 * fsw * fsw;
@@ -214,6 +215,43 @@ double nowMs() {
 }
 
 // ---- stages -----------------------------------------------------------------
+// The two global systems (one per direction) and the solution buffers.
+int ensureSolver(Ctx& c) {
+  if (c.sol) return XB_OK;
+  for (int i = 0; i < 2; i++) {
+    LuSet& L = c.lu[i];
+    XB_HIP(hipMalloc((void**)&L.M, sizeof(double) * MATN * MATN));
+    XB_HIP(hipMalloc((void**)&L.rhs, sizeof(double) * NSOL));
+    XB_HIP(hipMalloc((void**)&L.M0, sizeof(double) * MATN * MATN));
+    XB_HIP(hipMalloc((void**)&L.rhs0, sizeof(double) * NSOL));
+    XB_HIP(hipMalloc((void**)&L.luIndex, sizeof(int) * MATN));
+    XB_HIP(hipMalloc((void**)&L.luVV, sizeof(double) * MATN));
+    XB_HIP(hipMalloc((void**)&L.luState, LU_STATE_BYTES));
+  }
+  XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
+  XB_HIP(hipMalloc((void**)&c.sol, sizeof(double) * NSOL));
+  return XB_OK;
+}
+
+// Runs f(0) on the primary stream and f(1) on the side stream, both after the
+// work already queued on the primary stream; the primary stream then waits for
+// both.  Used where the two images' work is independent (the reference runs it
+// one after the other; the results do not depend on the order).
+template <class F>
+int forkBoth(Ctx& c, F f) {
+  XB_HIP(hipEventRecord(c.evFork, c.stream));
+  XB_HIP(hipStreamWaitEvent(c.side, c.evFork, 0));
+  int rc = XB_OK;
+  for (int im = 0; im < 2 && rc == XB_OK; im++) {
+    c.cur = im == 0 ? c.stream : c.side;
+    rc = f(im);
+  }
+  c.cur = c.stream;
+  XB_HIP(hipEventRecord(c.evJoin, c.side));
+  XB_HIP(hipStreamWaitEvent(c.stream, c.evJoin, 0));
+  return rc;
+}
+
 int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
   if (!p || w <= 0 || h <= 0) {
     c.err = "bad arguments";
@@ -236,16 +274,9 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
   if (rc) return rc;
   if (!c.basis) XB_HIP(hipMalloc((void**)&c.basis, sizeof(Basis)));
   if (!c.sc) XB_HIP(hipMalloc((void**)&c.sc, sizeof(DevScalars)));
-  if (!c.M) {
-    XB_HIP(hipMalloc((void**)&c.M, sizeof(double) * MATN * MATN));
-    XB_HIP(hipMalloc((void**)&c.rhs, sizeof(double) * NSOL));
-    XB_HIP(hipMalloc((void**)&c.M0, sizeof(double) * MATN * MATN));
-    XB_HIP(hipMalloc((void**)&c.rhs0, sizeof(double) * NSOL));
-    XB_HIP(hipMalloc((void**)&c.sol, sizeof(double) * NSOL));
-    XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
-    XB_HIP(hipMalloc((void**)&c.luIndex, sizeof(int) * MATN));
-    XB_HIP(hipMalloc((void**)&c.luVV, sizeof(double) * MATN));
-    XB_HIP(hipMalloc((void**)&c.luState, LU_STATE_BYTES));
+  {
+    int rc2 = ensureSolver(c);
+    if (rc2) return rc2;
   }
   static_assert(sizeof(Basis) < (1 << 20), "basis size");
   Basis hb;
@@ -304,10 +335,15 @@ int stageSss(Ctx& c) {
   int rc = ensureStamps(c, c.nGrid, fsw);
   if (rc) return rc;
   auto identify = [&]() -> int {
+    // calcStats T, calcStats S, then both searches (src/sssUtil.cpp:11-14);
+    // the searches read only their own image's mask bits, so they overlap
     XB_HIP(launch_sss_stats(c, 0));
     XB_HIP(launch_sss_stats(c, 1));
-    XB_HIP(launch_sss_find(c, 0));
-    XB_HIP(launch_sss_find(c, 1));
+    int rc2 = forkBoth(c, [&](int im) -> int {
+      XB_HIP(launch_sss_find(c, im));
+      return XB_OK;
+    });
+    if (rc2) return rc2;
     XB_HIP(launch_sss_compact(c));
     return XB_OK;
   };
@@ -336,24 +372,29 @@ int stageSss(Ctx& c) {
 
 // cmv (src/bach.cpp:172-268)
 int stageCmv(Ctx& c) {
-  for (int im = 0; im < 2; im++) XB_HIP(launch_fill(c, im, nullptr, c.nKept[im]));
-  return XB_OK;
+  return forkBoth(c, [&](int im) -> int {
+    XB_HIP(launch_fill(c, im, nullptr, c.nKept[im]));
+    return XB_OK;
+  });
 }
 
 // cd (src/bach.cpp:270-313, testFit src/cdkscUtil.cpp:4-266)
 int stageCd(Ctx& c, int* convTemplate) {
-  for (int im = 0; im < 2; im++) {
-    if (c.nKept[im] == 0) continue;
+  // the two trial fits are independent: each direction has its own system
+  int rc = forkBoth(c, [&](int im) -> int {
+    if (c.nKept[im] == 0) return XB_OK;
     double* cdSol = c.cdSol + (size_t)im * NSOL;
     XB_HIP(launch_stamp_lu(c, im));
     XB_HIP(launch_cd_select(c, im));
     XB_HIP(launch_global_build(c, im, true));
-    XB_HIP(launch_global_solve(c, cdSol, &c.sc->nTest[im]));
+    XB_HIP(launch_global_solve(c, c.lu[im], cdSol, &c.sc->nTest[im]));
     XB_HIP(launch_calc_sigs(c, im, true, cdSol));
     XB_HIP(launch_cd_finish(c, im));
-  }
+    return XB_OK;
+  });
+  if (rc) return rc;
   DevScalars sc;
-  int rc = readScalars(c, sc);
+  rc = readScalars(c, sc);
   if (rc) return rc;
   double merit[2];
   for (int im = 0; im < 2; im++) merit[im] = c.nKept[im] == 0 ? 666 : sc.merit[im];
@@ -369,7 +410,7 @@ int stageKsc(Ctx& c) {
   int it = 0;
   for (;;) {
     XB_HIP(launch_global_build(c, im, false));
-    XB_HIP(launch_global_solve(c, c.sol, nullptr));
+    XB_HIP(launch_global_solve(c, c.lu[im], c.sol, nullptr));
     XB_HIP(launch_calc_sigs(c, im, false, c.sol));
     XB_HIP(launch_ksc_check(c, im));
     XB_HIP(launch_fill(c, im, c.st[im].list, c.nKept[im]));
@@ -422,10 +463,13 @@ void freeAll(Ctx& c) {
   dfree(&c.basis);
   dfree(&c.ran1);
   dfree(&c.goodPix);
-  dfree(&c.M); dfree(&c.rhs); dfree(&c.sol); dfree(&c.cdSol);
-  dfree(&c.M0); dfree(&c.rhs0);
-  dfree(&c.luIndex); dfree(&c.luVV); dfree(&c.luState);
-  dfree(&c.tileK); dfree(&c.tileK64); dfree(&c.tileAks);
+  for (int i = 0; i < 2; i++) {
+    LuSet& L = c.lu[i];
+    dfree(&L.M); dfree(&L.rhs); dfree(&L.M0); dfree(&L.rhs0);
+    dfree(&L.luIndex); dfree(&L.luVV); dfree(&L.luState);
+  }
+  dfree(&c.sol); dfree(&c.cdSol);
+  dfree(&c.tileK64); dfree(&c.tileAks);
 }
 
 int setDev(Ctx& c) {
@@ -467,7 +511,7 @@ void prof_begin(Ctx& c, int slot) {
   if (!c.profOn || slot < 0 || slot >= XB_PROF_N) return;
   hipEvent_t e = takeEvent(c);
   if (!e) return;
-  if (hipEventRecord(e, c.stream) != hipSuccess) {
+  if (hipEventRecord(e, c.cur) != hipSuccess) {
     c.evPool.push_back(e);
     return;
   }
@@ -478,7 +522,7 @@ void prof_end(Ctx& c, int slot) {
   if (!c.profOn || slot < 0 || slot >= XB_PROF_N || !c.evOpen[slot]) return;
   hipEvent_t e = takeEvent(c);
   if (!e) return;
-  if (hipEventRecord(e, c.stream) != hipSuccess) {
+  if (hipEventRecord(e, c.cur) != hipSuccess) {
     c.evPool.push_back(e);
     return;
   }
@@ -543,10 +587,19 @@ int xb_ctx_create(int device, int flags, xb_ctx** out) {
       return XB_ERR_NOT_BUILT;
     }
   }
-  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+  // two streams of the default priority, and the events that fork and join them
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c.evFork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.evJoin, hipEventDisableTiming) != hipSuccess) {
+    if (c.evFork) (void)hipEventDestroy(c.evFork);
+    if (c.evJoin) (void)hipEventDestroy(c.evJoin);
+    if (c.side) (void)hipStreamDestroy(c.side);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
     delete x;
     return XB_ERR_HIP;
   }
+  c.cur = c.stream;
   *out = x;
   return XB_OK;
 }
@@ -556,6 +609,7 @@ int xb_ctx_destroy(xb_ctx* x) {
   Ctx& c = x->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
+  if (c.side) (void)hipStreamSynchronize(c.side);
   for (auto& p : c.evPending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -564,6 +618,9 @@ int xb_ctx_destroy(xb_ctx* x) {
     if (e) (void)hipEventDestroy(e);
   for (auto e : c.evPool) (void)hipEventDestroy(e);
   freeAll(c);
+  if (c.evFork) (void)hipEventDestroy(c.evFork);
+  if (c.evJoin) (void)hipEventDestroy(c.evJoin);
+  if (c.side) (void)hipStreamDestroy(c.side);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete x;
   return XB_OK;
@@ -639,23 +696,17 @@ int xb_lu_solve(xb_ctx* x, const double* a, const double* b, double* out) {
   Ctx& c = x->c;
   int rc = setDev(c);
   if (rc) return rc;
-  if (!c.M) {
-    XB_HIP(hipMalloc((void**)&c.M, sizeof(double) * MATN * MATN));
-    XB_HIP(hipMalloc((void**)&c.rhs, sizeof(double) * NSOL));
-    XB_HIP(hipMalloc((void**)&c.M0, sizeof(double) * MATN * MATN));
-    XB_HIP(hipMalloc((void**)&c.rhs0, sizeof(double) * NSOL));
-    XB_HIP(hipMalloc((void**)&c.sol, sizeof(double) * NSOL));
-    XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
-    XB_HIP(hipMalloc((void**)&c.luIndex, sizeof(int) * MATN));
-    XB_HIP(hipMalloc((void**)&c.luVV, sizeof(double) * MATN));
-    XB_HIP(hipMalloc((void**)&c.luState, LU_STATE_BYTES));
+  {
+    int rc2 = ensureSolver(c);
+    if (rc2) return rc2;
   }
   const size_t mb = sizeof(double) * MATN * MATN, vb = sizeof(double) * NSOL;
-  XB_HIP(hipMemcpyAsync(c.M, a, mb, hipMemcpyHostToDevice, c.stream));
-  XB_HIP(hipMemcpyAsync(c.M0, a, mb, hipMemcpyHostToDevice, c.stream));
-  XB_HIP(hipMemcpyAsync(c.rhs, b, vb, hipMemcpyHostToDevice, c.stream));
-  XB_HIP(hipMemcpyAsync(c.rhs0, b, vb, hipMemcpyHostToDevice, c.stream));
-  XB_HIP(launch_global_solve(c, c.sol, nullptr));
+  LuSet& L = c.lu[0];
+  XB_HIP(hipMemcpyAsync(L.M, a, mb, hipMemcpyHostToDevice, c.stream));
+  XB_HIP(hipMemcpyAsync(L.M0, a, mb, hipMemcpyHostToDevice, c.stream));
+  XB_HIP(hipMemcpyAsync(L.rhs, b, vb, hipMemcpyHostToDevice, c.stream));
+  XB_HIP(hipMemcpyAsync(L.rhs0, b, vb, hipMemcpyHostToDevice, c.stream));
+  XB_HIP(launch_global_solve(c, L, c.sol, nullptr));
   XB_HIP(hipMemcpyAsync(out, c.sol, vb, hipMemcpyDeviceToHost, c.stream));
   XB_HIP(hipStreamSynchronize(c.stream));
   return XB_OK;

@@ -83,7 +83,7 @@ struct DevStamps {
 // Small device-side scalars of one pass (one struct, one allocation).
 struct DevScalars {
   int nKept[2];
-  int nList;            // entries of the current stamp list
+  int nList[2];         // entries of the current stamp list, per image
   int check;            // ksc: a substamp was replaced
   int nTest[2];
   int retry;            // sss: refill ratio below 10 %
@@ -96,9 +96,23 @@ struct DevScalars {
   int pad;
 };
 
+// The global FP64 system of one fit and its LU work buffers.
+struct LuSet {
+  double* M = nullptr;          // [MATN*MATN] global matrix
+  double* rhs = nullptr;        // [NSOL] right-hand side, then the solution
+  double* M0 = nullptr;         // copies of the system as built (LU replay paths)
+  double* rhs0 = nullptr;
+  int* luIndex = nullptr;       // [MATN] pivot rows
+  void* luState = nullptr;      // LU bookkeeping (zero row, last pivot, ii, zero flags)
+  double* luVV = nullptr;       // [MATN]
+};
+
 struct Ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // primary stream: every host read-back syncs it
+  hipStream_t side = nullptr;    // second stream for the other image's independent work
+  hipStream_t cur = nullptr;     // stream the launchers use (stream or side)
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
   std::string err;
   char name[256] = {0};
   // problem
@@ -119,18 +133,11 @@ struct Ctx {
   Basis* basis = nullptr;
   double* ran1 = nullptr;       // ran1 draws (identical per stamp: seed -666)
   int ran1Count = 0;
-  float* goodPix = nullptr;     // [N][fsw*fsw] good-pixel scratch of sss
-  double* M = nullptr;          // [MATN*MATN] global matrix
-  double* rhs = nullptr;        // [NSOL] right-hand side, then the solution
-  double* M0 = nullptr;         // copies of the system as built (LU replay paths)
-  double* rhs0 = nullptr;
+  float* goodPix = nullptr;     // [2][N][fsw*fsw] good-pixel / candidate scratch of sss
+  LuSet lu[2];                  // one global system per direction (cd runs both at once)
   double* sol = nullptr;        // [NSOL] current kernel solution
   double* cdSol = nullptr;      // [2][NSOL]
-  int* luIndex = nullptr;       // [MATN] pivot rows
-  void* luState = nullptr;      // LU bookkeeping (zero row, last pivot, ii)
-  double* luVV = nullptr;       // [MATN]
-  float* tileK = nullptr;       // [tiles][KK] FP32 tile kernels
-  double* tileK64 = nullptr;    // [tiles][KK] FP64 tile kernels (mask test)
+  double* tileK64 = nullptr;    // [tiles][KK] FP64 tile kernels
   double* tileAks = nullptr;    // [tiles] sum |k| in the conv loop order
   int capTiles = 0;
   // host mirrors
@@ -170,9 +177,10 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax);
 hipError_t launch_stamp_lu(Ctx& c, int im);
 hipError_t launch_cd_select(Ctx& c, int im);
 hipError_t launch_global_build(Ctx& c, int im, bool cdForm);
-// d_skipIfZero: when it points at a zero count the solve is skipped and
-// d_out is zeroed (the reference returns before solving an empty fit).
-hipError_t launch_global_solve(Ctx& c, double* d_out, const int* d_skipIfZero);
+// Solves system set L.  d_skipIfZero: when it points at a zero count the solve
+// is skipped and d_out is zeroed (the reference returns before solving an
+// empty fit).
+hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int* d_skipIfZero);
 hipError_t launch_cd_finish(Ctx& c, int im);
 hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol);
 hipError_t launch_ksc_check(Ctx& c, int im);
