@@ -34,8 +34,8 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_conv launch on the 2048x2048 pair: rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE of k_conv, each scaled by what the same counter reads for a
 # device copy of known size (bench.py --copy-calibration); the summary with
-# both commands is profiles/k_conv_2048_traffic.md.  None until measured.
-CONV_TRAFFIC_BYTES = None
+# both commands is profiles/k_conv_2048_traffic.md.
+CONV_TRAFFIC_BYTES = 206_582_626
 
 WORKLOADS = {
     "pair2048": ("2048x2048 FP32 pair, 6000 stars (dN/dF ~ F^-1.5, 1e2-1e5), template FWHM 1.8 px, "
