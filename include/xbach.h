@@ -34,7 +34,8 @@ enum {
   XB_ERR_HIP = 5,          /* HIP runtime error                                */
   XB_ERR_NO_DEVICE = 6,    /* no GPU / bad device index                        */
   XB_ERR_NOT_BUILT = 7,    /* kernels for gfx950 missing                       */
-  XB_ERR_CAPACITY = 8      /* a size exceeds a documented maximum              */
+  XB_ERR_CAPACITY = 8,     /* a size exceeds a documented maximum              */
+  XB_ERR_IO = 9            /* FITS file cannot be opened, read or written       */
 };
 
 /* ---- 16-bit pixel mask bits (include/datatypeUtil.h:172-188) ------------- */
@@ -204,6 +205,24 @@ typedef struct xb_synth_cfg {
   uint64_t seed;
 } xb_synth_cfg;
 int xb_synth_pair(const xb_synth_cfg* cfg, float* tmpl, float* sci);
+
+/* ---- FITS I/O (replaces readImage / writeImage, src/fitsUtil.cpp:8-64) ----
+ * xb_fits_read: primary HDU of BITPIX -32 or -64 (other types are refused
+ * with the reference's message), BSCALE/BZERO applied, pixels narrowed to
+ * float as Image::operator[] does (include/datatypeUtil.h:158); plain or
+ * gzip-compressed files.  out == NULL queries the size only; cap = number of
+ * floats out can hold.  msg (optional) receives the error text.
+ * xb_fits_write: FLOAT_IMG primary HDU, overwriting like the reference's "!"
+ * prefix (include/datatypeUtil.h:169).  Neither needs a GPU. */
+int xb_fits_read(const char* path, int* w, int* h, float* out, size_t cap, char* msg,
+                 size_t msglen);
+int xb_fits_write(const char* path, int w, int h, const float* data, char* msg, size_t msglen);
+
+/* Waits until the work queued on the context has finished.  The reference's
+ * stage functions return only when their device work is done (blocking reads
+ * after every launch, e.g. src/bach.cpp:29-39); a caller that times stages
+ * the way src/main.cpp:56-142 does calls this after each stage. */
+int xb_sync(xb_ctx* ctx);
 
 #ifdef __cplusplus
 }

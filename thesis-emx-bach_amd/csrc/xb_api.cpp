@@ -663,6 +663,16 @@ int xb_init_device(xb_ctx* x, const xb_params* p, int w, int h, const void* d_tm
 
 int xb_sss(xb_ctx* x) { return x ? stageSss(x->c) : XB_ERR_ARG; }
 int xb_cmv(xb_ctx* x) { return x ? stageCmv(x->c) : XB_ERR_ARG; }
+
+int xb_sync(xb_ctx* x) {
+  if (!x) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  int rc = setDev(c);
+  if (rc) return rc;
+  XB_HIP(hipStreamSynchronize(c.stream));
+  XB_HIP(hipStreamSynchronize(c.side));
+  return XB_OK;
+}
 int xb_cd(xb_ctx* x, int* convTemplate) { return x ? stageCd(x->c, convTemplate) : XB_ERR_ARG; }
 int xb_ksc(xb_ctx* x) { return x ? stageKsc(x->c) : XB_ERR_ARG; }
 int xb_conv(xb_ctx* x, double* kernSum) { return x ? stageConv(x->c, kernSum, true) : XB_ERR_ARG; }

@@ -30,7 +30,7 @@ XB_OK = 0
 STATUS_NAMES = {
     0: "XB_OK", 1: "XB_ERR_ARG", 2: "XB_ERR_SIZE_MISMATCH", 3: "XB_ERR_NO_SUBSTAMPS",
     4: "XB_ERR_STAMP_PIXELS", 5: "XB_ERR_HIP", 6: "XB_ERR_NO_DEVICE",
-    7: "XB_ERR_NOT_BUILT", 8: "XB_ERR_CAPACITY",
+    7: "XB_ERR_NOT_BUILT", 8: "XB_ERR_CAPACITY", 9: "XB_ERR_IO",
 }
 
 # mask bits (include/datatypeUtil.h:172-188)
@@ -249,6 +249,10 @@ def write_fits(path, data: np.ndarray) -> None:
 _lib = None
 
 
+class XbachError(RuntimeError):
+    pass
+
+
 def lib() -> C.CDLL:
     """Load libxbach.so.  Raises when it has not been built (no fallback)."""
     global _lib
@@ -290,8 +294,37 @@ def lib() -> C.CDLL:
     L.xb_buf_download.argtypes = [vp, vp, vp, C.c_size_t]
     L.xb_profile_enable.argtypes = [vp, C.c_int]
     L.xb_profile_get.argtypes = [vp, vp, vp, C.c_int]
+    L.xb_fits_read.argtypes = [C.c_char_p, P(C.c_int), P(C.c_int), vp, C.c_size_t, C.c_char_p,
+                               C.c_size_t]
+    L.xb_fits_write.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, C.c_char_p, C.c_size_t]
+    L.xb_sync.argtypes = [vp]
     _lib = L
     return L
+
+
+def fits_read_native(path) -> np.ndarray:
+    """FP32 image (h, w) read by the library's FITS reader (xb_fits_read)."""
+    L = lib()
+    w, h = C.c_int(), C.c_int()
+    msg = C.create_string_buffer(512)
+    p = str(path).encode()
+    rc = L.xb_fits_read(p, C.byref(w), C.byref(h), None, 0, msg, 512)
+    if rc != 0:
+        raise XbachError(msg.value.decode())
+    out = np.empty((h.value, w.value), np.float32)
+    rc = L.xb_fits_read(p, C.byref(w), C.byref(h), out.ctypes.data, out.size, msg, 512)
+    if rc != 0:
+        raise XbachError(msg.value.decode())
+    return out
+
+
+def fits_write_native(path, data: np.ndarray) -> None:
+    """FLOAT_IMG primary HDU written by the library (xb_fits_write)."""
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    msg = C.create_string_buffer(512)
+    rc = lib().xb_fits_write(str(path).encode(), d.shape[1], d.shape[0], d.ctypes.data, msg, 512)
+    if rc != 0:
+        raise XbachError(msg.value.decode())
 
 
 # per-kernel timing slots (include/xbach.h XB_PROF_*)
@@ -306,10 +339,6 @@ def synth_pair(cfg: SynthCfg) -> tuple[np.ndarray, np.ndarray]:
     if rc != 0:
         raise RuntimeError(f"xb_synth_pair failed: {STATUS_NAMES.get(rc, rc)}")
     return t, s
-
-
-class XbachError(RuntimeError):
-    pass
 
 
 class Context:
