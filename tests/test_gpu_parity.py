@@ -78,6 +78,14 @@ def test_stamp_grid_20x20_ragged(ctx):
     check_case(ctx, t, s, xbach.Params.default(stampsx=20, stampsy=20))
 
 
+def test_wide_stamps_unstaged_search(ctx):
+    # 6x6 stamps on 1600x1600: 245-px stamps do not fit in LDS, so the
+    # substamp search reads the image and mask from global memory
+    cfg = xbach.synth_cfg("pair2048", w=1600, h=1600, nstars=2500, seed=1601)
+    t, s = xbach.synth_pair(cfg)
+    check_case(ctx, t, s, xbach.Params.default(stampsx=6, stampsy=6))
+
+
 def test_science_convolved_direction(ctx):
     # science sharper than the template: the science image is convolved
     cfg = xbach.synth_cfg("pair512", fwhmT=3.2, fwhmS0=1.9, seed=5)

@@ -307,16 +307,6 @@ struct FindArgs {
 };
 
 // Wave-wide reductions (every lane gets the result).
-__device__ __forceinline__ int wave_min_int(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-  return v;
-}
-__device__ __forceinline__ int wave_max_int(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-  return v;
-}
 // (value, index) maximum: larger value wins, ties -> smaller index; NaN never
 // wins (callers pass -inf for lanes without a candidate).
 __device__ __forceinline__ int wave_argmax_first(double v, int idx) {
@@ -331,12 +321,6 @@ __device__ __forceinline__ int wave_argmax_first(double v, int idx) {
   }
   return idx;
 }
-// Tree sum; used only where every partial sum is exact (any order agrees).
-__device__ __forceinline__ double wave_sum_exact(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
 
 __device__ __forceinline__ int float_exp_field(float f) {
   int e = (int)((__float_as_uint(f) >> 23) & 0xff);
@@ -344,38 +328,125 @@ __device__ __forceinline__ int float_exp_field(float f) {
 }
 
 constexpr int FIND_THREADS = 1024;
+// dynamic LDS of the staged search (gfx950: 160 KiB per workgroup), leaving
+// room for the kernel's static arrays
+constexpr size_t FIND_LDS_MAX = 160 * 1024 - 2048;
 constexpr int FIND_WAVES = FIND_THREADS / WAVE;
-constexpr int WIN_PER_LANE = (SSW * SSW + WAVE - 1) / WAVE;  // 31x31 window / 64 lanes
+// Windows (at most 31x31) are walked in a 32-wide layout, slot pp -> column
+// pp & 31, row pp >> 5, so no lane divides by the window width; the raster
+// index row * width + column keeps the reference's scan order for ties.
+constexpr int WIN_PER_LANE = 32 * 32 / WAVE;
+
+// n / d for n < 2^20 and 2 <= d <= 1024 by a reciprocal multiply (exact over
+// that range: every pair was checked against integer division).
+__device__ __forceinline__ int fdiv(int n, unsigned m) { return (int)__umulhi((unsigned)n, m); }
+__host__ __device__ __forceinline__ unsigned fdiv_magic(int d) { return 0xFFFFFFFFu / (unsigned)d + 1u; }
+
+// Pixel access of one stamp's search.  With STAGED, the stamp's fsw x fsw
+// region lives in LDS: the pixel values (float) and one bit per pixel telling
+// whether the mask is bad for this search ((mask & badMask) != 0).  Every
+// access of the search stays inside that region, and only this search sets
+// bits of its own badMask (the other image's search sets the other image's
+// bits), so the bitmap is an exact view of the mask; mask bits are still ORed
+// into the global mask, which nothing reads back here.  Without STAGED the
+// global image and mask are read directly and each mask write is fenced
+// before later reads.
+template <bool STAGED>
+struct StampPix {
+  const float* img;
+  uint16_t* mask;
+  long imgW;
+  int ox, oy, fsw;
+  uint16_t badMask;
+  float* simg;     // STAGED: [fsw*fsw]
+  uint32_t* sbad;  // STAGED: [(fsw*fsw + 63) / 64 * 2]
+  __device__ __forceinline__ long addr(int x, int y) const { return x + (long)y * imgW; }
+  __device__ __forceinline__ int local(int x, int y) const { return (y - oy) * fsw + (x - ox); }
+  __device__ __forceinline__ float value(int x, int y) const {
+    return STAGED ? simg[local(x, y)] : img[addr(x, y)];
+  }
+  __device__ __forceinline__ bool bad(int x, int y) const {
+    if (STAGED) {
+      const int k = local(x, y);
+      return (sbad[k >> 5] >> (k & 31)) & 1u;
+    }
+    return (mask[addr(x, y)] & badMask) != 0;
+  }
+  // bits always belong to badMask (bad pixel or skip bit of this image)
+  __device__ __forceinline__ void mark(int x, int y, uint16_t bits) const {
+    mask_or_atomic(mask, (size_t)addr(x, y), bits);
+    if (STAGED) {
+      const int k = local(x, y);
+      atomicOr(&sbad[k >> 5], 1u << (k & 31));
+    }
+  }
+  // make this wave's marks visible to its later reads
+  __device__ __forceinline__ void settle() const {
+    if (STAGED) {
+      __builtin_amdgcn_wave_barrier();  // LDS accesses of a wave complete in order
+      __asm__ __volatile__("" ::: "memory");
+    } else {
+      __threadfence_block();
+    }
+  }
+};
 
 // One 1024-thread workgroup per stamp.  Per threshold level, all 16 waves
 // classify the stamp's pixels against the mask as it is now: wave w scans the
-// raster segment [w*seg, (w+1)*seg) with coalesced loads and appends its
-// candidates, in raster order, to its own slice of the scratch list (ballot +
-// popcount).  Mask bits are only ever added, so a pixel rejected here stays
-// rejected; flagging pixels above threshHigh commutes with everything else.
-// Wave 0 then processes the candidates one by one in raster order, re-checking
-// each against the current mask, with wave-level reductions only.  Mask bits
-// are set with 32-bit atomic ORs: the template and science searches run at the
+// raster segment [w*seg, (w+1)*seg) and appends its candidates, in raster
+// order, to its own slice of the scratch list (ballot + popcount).  Mask bits
+// are only ever added, so a pixel rejected here stays rejected; flagging
+// pixels above threshHigh commutes with everything else.  Wave 0 then
+// processes the candidates one by one in raster order, re-checking each
+// against the current mask, with wave-level reductions only.  Mask bits are
+// set with 32-bit atomic ORs: the template and science searches run at the
 // same time, each reading only its own bits (badMask) but sharing mask words.
+template <bool STAGED>
 __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
+  extern __shared__ uint32_t find_lds[];
   const int s = blockIdx.x;
   const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
   const StampGeom g = stamp_geom(s, A.nx, A.ny, A.w, A.h, A.fsw);
   const double sky = A.sky[s], fwhm = A.fwhm[s];
   const double floorV = sky + A.tkf * fwhm;
   const int fsw = A.fsw;
-  const long imgW = A.w;
-  const float* img = A.img;
-  uint16_t* mask = A.mask;
-  const uint16_t badMask = A.badMask;
+  const int nAll = fsw * fsw;
+  const unsigned mf = fdiv_magic(fsw);  // stamp-relative raster index -> row
+  StampPix<STAGED> P;
+  P.img = A.img;
+  P.mask = A.mask;
+  P.imgW = A.w;
+  P.ox = g.ox;
+  P.oy = g.oy;
+  P.fsw = fsw;
+  P.badMask = A.badMask;
+  P.simg = reinterpret_cast<float*>(find_lds);
+  P.sbad = find_lds + (nAll + 63) / 64 * 64;
   __shared__ int pre[FIND_WAVES + 1];
   __shared__ int lx[MAXSS], ly[MAXSS];
   __shared__ double lv[MAXSS];
   __shared__ int s_counter;
   if (tid == 0) s_counter = 0;
+  if (STAGED) {
+    // stage the stamp: values, and the bad bits 64 pixels per ballot
+    for (int base = wid * WAVE; base < nAll; base += FIND_THREADS) {
+      const int k = base + lane;
+      bool b = false;
+      if (k < nAll) {
+        const int ky = fdiv(k, mf), kx = k - ky * fsw;
+        const long a = (kx + g.ox) + (long)(ky + g.oy) * A.w;
+        P.simg[k] = A.img[a];
+        b = (A.mask[a] & A.badMask) != 0;
+      }
+      const unsigned long long bal = __ballot(b);
+      if (lane == 0) {
+        P.sbad[base >> 5] = (uint32_t)bal;
+        P.sbad[(base >> 5) + 1] = (uint32_t)(bal >> 32);
+      }
+    }
+  }
   __syncthreads();
   double dfrac = 0.9;
-  const int nAll = fsw * fsw;
   const int seg = ((nAll + FIND_WAVES - 1) / FIND_WAVES + WAVE - 1) / WAVE * WAVE;
   int* cand = A.candAll + (size_t)s * nAll;
   // The reference loops until the threshold reaches the floor; with a
@@ -387,24 +458,28 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
       const int k0 = wid * seg, k1 = min(nAll, k0 + seg);
       int cnt = 0;
       for (int base = k0; base < k1; base += 4 * WAVE) {
-        uint16_t m[4];
+        bool bd[4];
         float v[4];
-        long a[4];
+        int xs[4], ys[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int k = base + u * WAVE + lane;
-          a[u] = (k % fsw + g.ox) + (long)(k / fsw + g.oy) * imgW;
-          m[u] = k < k1 ? mask[a[u]] : (uint16_t)0xffff;
-          v[u] = k < k1 ? img[a[u]] : 0.0f;
+          const int kc = min(k, k1 - 1);  // in bounds; past the segment counts as bad
+          const int ky = fdiv(kc, mf);
+          xs[u] = kc - ky * fsw + g.ox;
+          ys[u] = ky + g.oy;
+          const bool b = P.bad(xs[u], ys[u]);
+          v[u] = P.value(xs[u], ys[u]);
+          bd[u] = k >= k1 || b;
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int k = base + u * WAVE + lane;
           bool c = false;
-          if (k < k1 && (m[u] & badMask) == 0) {
+          if (k < k1 && !bd[u]) {
             const double dv = (double)v[u];
             if (dv > A.hi) {
-              mask_or_atomic(mask, (size_t)a[u], A.badPixelMask);
+              P.mark(xs[u], ys[u], A.badPixelMask);
             } else if (!((dv - sky) * (1.0 / fwhm) < A.tkf) && dv > lim) {
               c = true;
             }
@@ -430,151 +505,165 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
         while (w + 1 < FIND_WAVES && pre[w + 1] <= q) w++;
         return cand[w * seg + (q - pre[w])];
       };
-      for (int pos = 0; pos < nc && counter < MAXSS;) {
-        // first candidate from pos on whose pixel is still unmasked
-        {
-          const int q = pos + lane;
-          bool ok = false;
-          if (q < nc) {
-            const int kk = candAt(q);
-            const long a = (kk % fsw + g.ox) + (long)(kk / fsw + g.oy) * imgW;
-            ok = (mask[a] & badMask) == 0;
-          }
-          const unsigned long long bal = __ballot(ok);
-          if (bal == 0ull) {
-            pos += WAVE;
-            continue;
-          }
-          pos += __ffsll((unsigned long long)bal) - 1;
-        }
-        const int kk = candAt(pos);
-        pos++;
-        const int absx = kk % fsw + g.ox, absy = kk / fsw + g.oy;
-        // ---- window maximum (sss.cl:427-459)
-        const int startX = max(absx - HSS, g.ox), startY = max(absy - HSS, g.oy);
-        const int endX = min(absx + HSS, g.ox + fsw - 1), endY = min(absy + HSS, g.oy + fsw - 1);
-        const int ww = endX - startX + 1, wh = endY - startY + 1;
-        double bestV = 0.0;
-        int bestI = INT_MAX;
-        {
-          uint16_t km[WIN_PER_LANE];
-          float kvf[WIN_PER_LANE];
-          long kc[WIN_PER_LANE];
+      // candidates in batches of 64 (one per lane, raster order); within a
+      // batch the next candidate is the first pending one whose pixel is
+      // unmasked now -- the same one a sequential walk reaches next
+      for (int b0 = 0; b0 < nc && counter < MAXSS; b0 += WAVE) {
+        const int q = b0 + lane;
+        const int kq = q < nc ? candAt(q) : 0;
+        const int qy = fdiv(kq, mf) + g.oy, qx = kq - (qy - g.oy) * fsw + g.ox;
+        unsigned long long pending = __ballot(q < nc);
+        while (pending != 0ull && counter < MAXSS) {
+          const bool okq = ((pending >> lane) & 1ull) && !P.bad(qx, qy);
+          const unsigned long long bal = __ballot(okq);
+          if (bal == 0ull) break;
+          const int first = __ffsll((unsigned long long)bal) - 1;
+          pending &= ~((2ull << first) - 1ull);  // drop this lane and those before it
+          const int absx = __shfl(qx, first), absy = __shfl(qy, first);
+          // ---- window maximum (sss.cl:427-459)
+          const int startX = max(absx - HSS, g.ox), startY = max(absy - HSS, g.oy);
+          const int endX = min(absx + HSS, g.ox + fsw - 1), endY = min(absy + HSS, g.oy + fsw - 1);
+          const int ww = endX - startX + 1, wh = endY - startY + 1;
+          double bestV = 0.0;
+          int bestI = INT_MAX;
+          {
+            bool kb[WIN_PER_LANE];
+            float kvf[WIN_PER_LANE];
+            // loads at coordinates clamped into the window (always in
+            // bounds, no branch per slot); slots outside count as bad
 #pragma unroll
-          for (int u = 0; u < WIN_PER_LANE; u++) {
-            const int p = lane + u * WAVE;
-            const bool in = p < ww * wh;
-            kc[u] = in ? (startX + p % ww) + (long)(startY + p / ww) * imgW : 0;
-            km[u] = in ? mask[kc[u]] : (uint16_t)0xffff;
-            kvf[u] = in ? img[kc[u]] : 0.0f;
-          }
+            for (int u = 0; u < WIN_PER_LANE; u++) {
+              const int pp = lane + u * WAVE, px = pp & 31, py = pp >> 5;
+              const bool in = px < ww && py < wh;
+              const int x = startX + min(px, ww - 1), y = startY + min(py, wh - 1);
+              const bool b = P.bad(x, y);
+              kvf[u] = P.value(x, y);
+              kb[u] = !in || b;
+            }
 #pragma unroll
-          for (int u = 0; u < WIN_PER_LANE; u++) {
-            const int p = lane + u * WAVE;
-            if (p >= ww * wh || (km[u] & badMask) > 0) continue;
-            const double kv = (double)kvf[u];
-            if (kv >= A.hi) {
-              mask_or_atomic(mask, (size_t)kc[u], A.badPixelMask);
-              continue;
-            }
-            if ((kv - sky) * (1.0 / fwhm) < A.tkf) continue;
-            if (kv > bestV) {
-              bestV = kv;
-              bestI = p;
-            }
-          }
-        }
-        if (bestI == INT_MAX) bestV = -INFINITY;
-        const int win = wave_argmax_first(bestV, bestI);
-        __threadfence_block();  // the window's marks before the check reads
-        int mx = absx, my = absy;
-        if (win != INT_MAX) {
-          mx = startX + win % ww;
-          my = startY + win / ww;
-        }
-        // ---- checkSStamp (sss.cl:326-361)
-        const int cx0 = max(mx - HSS, g.ox), cy0 = max(my - HSS, g.oy);
-        const int cx1 = min(mx + HSS, g.ox + g.sw - 1), cy1 = min(my + HSS, g.oy + g.sh - 1);
-        const int cw = cx1 - cx0 + 1, ch = cy1 - cy0 + 1;
-        int firstBad = INT_MAX;
-        int badType = 0;
-        double part = 0.0;
-        int eMin = 1000, eMax = -1000;
-        {
-          uint16_t cm[WIN_PER_LANE];
-          float cvf[WIN_PER_LANE];
-#pragma unroll
-          for (int u = 0; u < WIN_PER_LANE; u++) {
-            const int p = lane + u * WAVE;
-            const bool in = p < cw * ch;
-            const long ac = in ? (cx0 + p % cw) + (long)(cy0 + p / cw) * imgW : 0;
-            cm[u] = in ? mask[ac] : (uint16_t)0;
-            cvf[u] = in ? img[ac] : 0.0f;
-          }
-#pragma unroll
-          for (int u = 0; u < WIN_PER_LANE; u++) {
-            const int p = lane + u * WAVE;
-            if (p >= cw * ch) continue;
-            if ((cm[u] & badMask) > 0) {
-              if (p < firstBad) { firstBad = p; badType = 1; }
-              continue;
-            }
-            const float fv = cvf[u];
-            const double v = (double)fv;
-            if (v > A.hi) {
-              if (p < firstBad) { firstBad = p; badType = 2; }
-              continue;
-            }
-            if ((v - sky) / fwhm > A.tkf) {
-              part += v;
-              if (fv != 0.0f) {
-                const int e = float_exp_field(fv);
-                eMin = min(eMin, e);
-                eMax = max(eMax, e);
+            for (int u = 0; u < WIN_PER_LANE; u++) {
+              const int pp = lane + u * WAVE, px = pp & 31, py = pp >> 5;
+              if (kb[u]) continue;  // also every slot outside the window
+              const int p = py * ww + px;
+              const double kv = (double)kvf[u];
+              if (kv >= A.hi) {
+                P.mark(startX + px, startY + py, A.badPixelMask);
+                continue;
+              }
+              if ((kv - sky) * (1.0 / fwhm) < A.tkf) continue;
+              if (kv > bestV) {
+                bestV = kv;
+                bestI = p;
               }
             }
           }
-        }
-        // first bad pixel in raster order
-        const int fb = wave_min_int(firstBad);
-        if (fb != INT_MAX) {
-          if (firstBad == fb && badType == 2) {
-            const long ac = (cx0 + fb % cw) + (long)(cy0 + fb / cw) * imgW;
-            mask_or_atomic(mask, (size_t)ac, A.badPixelMask);
+          if (bestI == INT_MAX) bestV = -INFINITY;
+          const int win = wave_argmax_first(bestV, bestI);
+          P.settle();  // the window's marks before the check reads
+          int mx = absx, my = absy;
+          if (win != INT_MAX) {
+            mx = startX + win % ww;
+            my = startY + win / ww;
           }
-          __threadfence_block();
-          continue;  // checkSStamp returned 0
-        }
-        // sum of qualifying values in raster order: exact in any order when
-        // the exponent spread is small (all partial sums representable),
-        // otherwise summed sequentially by one lane.
-        const int gMin = wave_min_int(eMin);
-        const int gMax = wave_max_int(eMax);
-        double total;
-        if (gMax < gMin || gMax - gMin <= 19) {
-          total = wave_sum_exact(part);
-        } else {
-          double r = 0.0;
-          if (lane == 0)
-            for (int p = 0; p < cw * ch; p++) {
-              const double v = (double)img[(cx0 + p % cw) + (long)(cy0 + p / cw) * imgW];
-              if ((v - sky) / fwhm > A.tkf) r += v;
+          // ---- checkSStamp (sss.cl:326-361)
+          const int cx0 = max(mx - HSS, g.ox), cy0 = max(my - HSS, g.oy);
+          const int cx1 = min(mx + HSS, g.ox + g.sw - 1), cy1 = min(my + HSS, g.oy + g.sh - 1);
+          const int cw = cx1 - cx0 + 1, ch = cy1 - cy0 + 1;
+          int firstBad = INT_MAX;
+          int badType = 0;
+          double part = 0.0;
+          int eMin = 1000, eMax = -1000;
+          {
+            bool cb[WIN_PER_LANE];
+            float cvf[WIN_PER_LANE];
+            bool cin[WIN_PER_LANE];
+#pragma unroll
+            for (int u = 0; u < WIN_PER_LANE; u++) {
+              const int pp = lane + u * WAVE, px = pp & 31, py = pp >> 5;
+              cin[u] = px < cw && py < ch;
+              const int x = cx0 + min(px, cw - 1), y = cy0 + min(py, ch - 1);
+              cb[u] = P.bad(x, y);
+              cvf[u] = P.value(x, y);
             }
-          total = __shfl(r, 0);
+#pragma unroll
+            for (int u = 0; u < WIN_PER_LANE; u++) {
+              const int pp = lane + u * WAVE, px = pp & 31, py = pp >> 5;
+              if (!cin[u]) continue;
+              const int p = py * cw + px;
+              if (cb[u]) {
+                if (p < firstBad) { firstBad = p; badType = 1; }
+                continue;
+              }
+              const float fv = cvf[u];
+              const double v = (double)fv;
+              if (v > A.hi) {
+                if (p < firstBad) { firstBad = p; badType = 2; }
+                continue;
+              }
+              if ((v - sky) / fwhm > A.tkf) {
+                part += v;
+                if (fv != 0.0f) {
+                  const int e = float_exp_field(fv);
+                  eMin = min(eMin, e);
+                  eMax = max(eMax, e);
+                }
+              }
+            }
+          }
+          // one pass of reductions: first bad pixel (with its type), exponent
+          // range and the partial sums, interleaved
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) {
+            const int ofb = __shfl_xor(firstBad, o), obt = __shfl_xor(badType, o);
+            const int omn = __shfl_xor(eMin, o), omx = __shfl_xor(eMax, o);
+            const double op = __shfl_xor(part, o);
+            if (ofb < firstBad) {
+              firstBad = ofb;
+              badType = obt;
+            }
+            eMin = min(eMin, omn);
+            eMax = max(eMax, omx);
+            part += op;
+          }
+          // first bad pixel in raster order
+          const int fb = firstBad;
+          if (fb != INT_MAX) {
+            if (lane == 0 && badType == 2) P.mark(cx0 + fb % cw, cy0 + fb / cw, A.badPixelMask);
+            P.settle();
+            continue;  // checkSStamp returned 0
+          }
+          // sum of qualifying values in raster order: exact in any order when
+          // the exponent spread is small (all partial sums representable; the
+          // butterfly gives every lane the same value), otherwise summed
+          // sequentially by one lane.
+          const int gMin = eMin;
+          const int gMax = eMax;
+          double total;
+          if (gMax < gMin || gMax - gMin <= 19) {
+            total = part;
+          } else {
+            double r = 0.0;
+            if (lane == 0)
+              for (int p = 0; p < cw * ch; p++) {
+                const double v = (double)P.value(cx0 + p % cw, cy0 + p / cw);
+                if ((v - sky) / fwhm > A.tkf) r += v;
+              }
+            total = __shfl(r, 0);
+          }
+          if (total == 0.0) continue;
+          if (lane == 0) {
+            lx[counter] = mx;
+            ly[counter] = my;
+            lv[counter] = total;
+          }
+          counter++;
+          // skip marks (sss.cl:473-482), clamped with the stamp size
+          for (int pp = lane; pp < 32 * 32; pp += WAVE) {
+            const int px = pp & 31, py = pp >> 5;
+            if (px < cw && py < ch) P.mark(cx0 + px, cy0 + py, A.skipMask);
+          }
+          P.settle();
         }
-        if (total == 0.0) continue;
-        if (lane == 0) {
-          lx[counter] = mx;
-          ly[counter] = my;
-          lv[counter] = total;
-        }
-        counter++;
-        // skip marks (sss.cl:473-482), clamped with the stamp size
-        for (int p = lane; p < cw * ch; p += WAVE) {
-          const long ac = (cx0 + p % cw) + (long)(cy0 + p / cw) * imgW;
-          mask_or_atomic(mask, (size_t)ac, A.skipMask);
-        }
-        __threadfence_block();
       }
       if (lane == 0) s_counter = counter;
     }
@@ -676,8 +765,23 @@ hipError_t launch_sss_find(Ctx& c, int im) {
   // the good-pixel scratch of k_sss_stats is free again; each image's search
   // uses its own half (the two searches run concurrently)
   A.candAll = reinterpret_cast<int*>(c.goodPix) + (size_t)im * c.capStamps * c.capFsw * c.capFsw;
+  // the stamp region in LDS when it fits (float values + one bad bit per pixel)
+  const size_t nAll = (size_t)c.fsw * c.fsw;
+  const size_t lds = ((nAll + 63) / 64 * 64) * sizeof(float) + (nAll + 63) / 64 * 8;
   prof_begin(c, XB_PROF_SSS);
-  hipLaunchKernelGGL(k_sss_find, dim3(c.nGrid), dim3(FIND_THREADS), 0, c.cur, A);
+  if (lds <= FIND_LDS_MAX) {
+    static bool attrSet = false;  // per process: the attribute belongs to the kernel
+    if (!attrSet) {
+      hipError_t e = hipFuncSetAttribute((const void*)k_sss_find<true>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)FIND_LDS_MAX);
+      if (e != hipSuccess) return e;
+      attrSet = true;
+    }
+    hipLaunchKernelGGL(k_sss_find<true>, dim3(c.nGrid), dim3(FIND_THREADS), lds, c.cur, A);
+  } else {
+    hipLaunchKernelGGL(k_sss_find<false>, dim3(c.nGrid), dim3(FIND_THREADS), 0, c.cur, A);
+  }
   prof_end(c, XB_PROF_SSS);
   return hipGetLastError();
 }
