@@ -38,8 +38,92 @@ __device__ __forceinline__ StampGeom stamp_geom(int s, int nx, int ny, int w, in
 // ---------------------------------------------------------------------------
 // calcStats (src/bachUtil.cpp:130-280)
 // ---------------------------------------------------------------------------
+// n / d for n < 2^20 and 2 <= d <= 1024 by a reciprocal multiply (exact over
+// that range: every pair was checked against integer division).
+__device__ __forceinline__ int fdiv(int n, unsigned m) { return (int)__umulhi((unsigned)n, m); }
+__host__ __device__ __forceinline__ unsigned fdiv_magic(int d) { return 0xFFFFFFFFu / (unsigned)d + 1u; }
+
 constexpr int MAX_CLIP_BLOCKS = 2560;  // ceil(fsw^2 / 32) for fsw <= 286
 constexpr int MAX_LEVELS = 32;         // findSubStamps threshold levels
+
+
+// Block-wide helpers of k_sss_stats (256 threads).
+// Inclusive prefix maximum over thread order.  scratch: >= 17 ints.
+__device__ __forceinline__ int block_prefix_max_incl(int v, int* scratch) {
+  const int lane = lane_id(), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const int t = __shfl_up(v, o);
+    if (lane >= o) v = max(v, t);
+  }
+  if (lane == WAVE - 1) scratch[wid] = v;
+  __syncthreads();
+  int before = INT_MIN;
+  for (int i = 0; i < wid && i < nw; i++) before = max(before, scratch[i]);
+  __syncthreads();
+  return max(v, before);
+}
+// (value, index) maximum, ties to the smaller index; every thread gets the
+// index.  dsc: >= 17 doubles, isc: >= 17 ints.
+__device__ __forceinline__ int block_argmax_first(double v, int idx, double* dsc, int* isc) {
+  const int lane = lane_id(), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(v, o);
+    const int oi = __shfl_xor(idx, o);
+    if (ov > v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+  if (lane == 0) {
+    dsc[wid] = v;
+    isc[wid] = idx;
+  }
+  __syncthreads();
+  double bv = dsc[0];
+  int bi = isc[0];
+  for (int i = 1; i < nw; i++)
+    if (dsc[i] > bv || (dsc[i] == bv && isc[i] < bi)) {
+      bv = dsc[i];
+      bi = isc[i];
+    }
+  __syncthreads();
+  return bi;
+}
+// Sum of 64-bit integers (exact in any order).
+__device__ __forceinline__ long long block_sum_ll(long long v, long long* scratch) {
+  const int lane = lane_id(), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (lane == 0) scratch[wid] = v;
+  __syncthreads();
+  long long t = 0;
+  for (int i = 0; i < nw; i++) t += scratch[i];
+  __syncthreads();
+  return t;
+}
+// Sequential sum of a[0..n-1] in index order (one thread), its operands loaded
+// one 8-block ahead; a[n .. n+15] must hold +0.0 (adding +0.0 to a partial sum
+// that is never -0 changes nothing).
+__device__ __forceinline__ double ordered_sum(const double* a, int n) {
+  double s = 0.0;
+  double c[8], d[8];
+#pragma unroll
+  for (int t = 0; t < 8; t++) c[t] = a[t];
+  for (int b = 0; b < n; b += 16) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) d[t] = a[b + 8 + t];
+#pragma unroll
+    for (int t = 0; t < 8; t++) s += c[t];
+    if (b + 8 >= n) break;
+#pragma unroll
+    for (int t = 0; t < 8; t++) c[t] = a[b + 16 + t];
+#pragma unroll
+    for (int t = 0; t < 8; t++) s += d[t];
+  }
+  return s;
+}
 
 __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img,
                                                    uint16_t* __restrict__ mask, int w, int h,
@@ -54,7 +138,7 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
   __shared__ double smp[NSAMP_PAD];
   __shared__ int isc[20];
   __shared__ int s_cnt, s_nGood;
-  __shared__ double bs[MAX_CLIP_BLOCKS], bs2[MAX_CLIP_BLOCKS];
+  __shared__ double bs[MAX_CLIP_BLOCKS + 16], bs2[MAX_CLIP_BLOCKS + 16];
   __shared__ double s_mean, s_sd, s_inv, s_sum, s_sum2;
   __shared__ long long s_cur, s_prev;
   __shared__ int bins[256];
@@ -99,35 +183,66 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
       __syncthreads();
     }
 
-  // ---- maskStamp (sss.cl:157-188): good pixels in raster order
+  // ---- maskStamp (sss.cl:157-188): good pixels in raster order.  Each wave
+  // owns a raster segment of the stamp: it counts its good pixels (flagging
+  // NaN pixels in the mask), the segment offsets follow from the counts, and
+  // a second scan writes the values at the wave's offset.  The second scan
+  // sees a NaN pixel either flagged or NaN: rejected either way.
   float* good = goodAll + (size_t)s * fsw * fsw;
-  if (tid == 0) s_nGood = 0;
-  __syncthreads();
   const int nAll = fsw * fsw;
-  for (int base = 0; base < nAll; base += blockDim.x) {
-    int k = base + tid;
-    bool ok = false;
-    float pv = 0.0f;
-    if (k < nAll) {
-      int x = k % fsw, y = k / fsw;
-      int xI = x + g.ox, yI = y + g.oy;
-      if (xI < w && yI < h) {
-        size_t idx = (size_t)xI + (size_t)yI * w;
-        pv = img[idx];
-        double pixel = (double)pv;
-        uint16_t m = mask[idx];
-        if (!(m > 0 || pixel <= 1e-10)) {
-          if (isnan(pixel)) mask[idx] = m | (XB_MASK_NAN_PIXEL | XB_MASK_BAD_INPUT);
-          else ok = true;
+  const unsigned mf = fdiv_magic(fsw);
+  const int lane = lane_id(), wid = tid / WAVE;
+  constexpr int NWS = 256 / WAVE;
+  const int seg = ((nAll + NWS - 1) / NWS + WAVE - 1) / WAVE * WAVE;
+  const int k0 = wid * seg, k1 = min(nAll, k0 + seg);
+  __shared__ int wcnt[NWS];
+  auto scan = [&](bool write, int off) -> int {
+    int cnt = 0;
+    for (int base = k0; base < k1; base += 8 * WAVE) {
+      float pv[8];
+      uint16_t m[8];
+      size_t ix[8];
+      bool in[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int k = base + u * WAVE + lane;
+        const int kc = min(k, k1 - 1);
+        const int y = fdiv(kc, mf), x = kc - y * fsw;
+        const int xI = x + g.ox, yI = y + g.oy;
+        in[u] = k < k1 && xI < w && yI < h;
+        ix[u] = (size_t)min(xI, w - 1) + (size_t)min(yI, h - 1) * w;
+        pv[u] = img[ix[u]];
+        m[u] = mask[ix[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const double pixel = (double)pv[u];
+        bool ok = false;
+        if (in[u] && !(m[u] > 0 || pixel <= 1e-10)) {
+          if (isnan(pixel)) {
+            if (!write) mask[ix[u]] = m[u] | (XB_MASK_NAN_PIXEL | XB_MASK_BAD_INPUT);
+          } else {
+            ok = true;
+          }
         }
+        const unsigned long long bal = __ballot(ok);
+        if (write && ok) good[off + cnt + __popcll(bal & ((1ull << lane) - 1ull))] = pv[u];
+        cnt += __popcll(bal);
       }
     }
-    int tot;
-    int pos = block_prefix_count(ok, isc, &tot);
-    int c0 = s_nGood;
-    if (ok) good[c0 + pos] = pv;
+    return cnt;
+  };
+  {
+    const int cnt = scan(false, 0);
+    if (lane == 0) wcnt[wid] = cnt;
     __syncthreads();
-    if (tid == 0) s_nGood = c0 + tot;
+    int off = 0, tot = 0;
+    for (int i = 0; i < NWS; i++) {
+      if (i < wid) off += wcnt[i];
+      tot += wcnt[i];
+    }
+    scan(true, off);
+    if (tid == 0) s_nGood = tot;
     __syncthreads();
   }
   const int n = s_nGood;
@@ -151,27 +266,32 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
         break;
       }
       for (int b = tid; b < nb; b += blockDim.x) {
+        // the block's 32 loads in flight together, summed in list order
+        const int cnt = min(32, n - b * 32);
+        float vb[32];
+#pragma unroll
+        for (int i = 0; i < 32; i++) vb[i] = good[b * 32 + min(i, cnt - 1)];
         double sm = 0.0, sm2 = 0.0;
-        int cnt = min(32, n - b * 32);
-        for (int i = 0; i < cnt; i++) {
-          float v = good[b * 32 + i];
-          double d = v > 0.0f ? (double)v : 0.0;
-          sm += d;
-          sm2 += d * d;
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+          if (i < cnt) {
+            const float v = vb[i];
+            const double d = v > 0.0f ? (double)v : 0.0;
+            sm += d;
+            sm2 += d * d;
+          }
         }
         bs[b] = sm;
         bs2[b] = sm2;
       }
-      __syncthreads();
-      if (tid == 0) {
-        double sum = 0.0;
-        for (int b = 0; b < nb; b++) sum += bs[b];
-        s_sum = sum;
-      } else if (tid == 64) {
-        double sum2 = 0.0;
-        for (int b = 0; b < nb; b++) sum2 += bs2[b];
-        s_sum2 = sum2;
+      if (tid < 16) {
+        bs[nb + tid] = 0.0;
+        bs2[nb + tid] = 0.0;
       }
+      __syncthreads();
+      // the block sums added in block order (the reference's std::accumulate)
+      if (tid == 0) s_sum = ordered_sum(bs, nb);
+      else if (tid == 64) s_sum2 = ordered_sum(bs2, nb);
       __syncthreads();
       if (tid == 0) {
         s_cur = s_prev;
@@ -185,11 +305,18 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
       __syncthreads();
       const double m = s_mean, inv = s_inv;
       int c = 0;
-      for (int k = tid; k < n; k += blockDim.x) {
-        float v = good[k];
-        if (v > 0.0f && fabs((double)v - m) * inv > alpha) {
-          good[k] = -v;
-          c++;
+      for (int kb = tid; kb < n; kb += 8 * (int)blockDim.x) {
+        float vb[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) vb[u] = good[min(kb + u * (int)blockDim.x, n - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int k = kb + u * (int)blockDim.x;
+          const float v = vb[u];
+          if (k < n && v > 0.0f && fabs((double)v - m) * inv > alpha) {
+            good[k] = -v;
+            c++;
+          }
         }
       }
       int tot = block_sum_int(c, isc);
@@ -204,6 +331,9 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
   // histogram's pixel set (mask == 0 and value > 1e-10 over the stamp).
   __shared__ double h_binSize, h_lowerBinVal, h_lower, h_upper, h_sky;
   __shared__ int h_state;  // 0 continue, 1 done (setFwhm), 2 done (no fwhm)
+  __shared__ int hC[257], hU[256];  // bin prefix sums, window upper ends
+  __shared__ double hdv[20];
+  __shared__ long long hll[20];
   if (tid == 0) {
     double up = smp[(int)(0.9 * sampleCount)];
     double mid = smp[(int)(0.5 * sampleCount)];
@@ -224,58 +354,117 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
     __syncthreads();
     const double lbv = h_lowerBinVal, bsz = h_binSize;
     int ok = 0;
-    for (int k = tid; k < n; k += blockDim.x) {
-      double v = (double)fabsf(good[k]);
-      if (fabs(v - mean) * invStdDev > alpha) continue;
-      double q = floor((v - lbv) / bsz) + 1.0;
-      int idx = (q != q) ? 0 : (q <= 0.0 ? 0 : (q >= 255.0 ? 255 : (int)q));
-      atomicAdd(&bins[idx], 1);
-      ok++;
+    for (int kb = tid; kb < n; kb += 8 * (int)blockDim.x) {
+      float vb[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) vb[u] = good[min(kb + u * (int)blockDim.x, n - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (kb + u * (int)blockDim.x >= n) continue;
+        const double v = (double)fabsf(vb[u]);
+        if (fabs(v - mean) * invStdDev > alpha) continue;
+        const double q = floor((v - lbv) / bsz) + 1.0;
+        const int idx = (q != q) ? 0 : (q <= 0.0 ? 0 : (q >= 255.0 ? 255 : (int)q));
+        atomicAdd(&bins[idx], 1);
+        ok++;
+      }
     }
     int okCount = block_sum_int(ok, isc);
+    // The reference walks the bins serially, accumulating integer counts in
+    // doubles: every partial sum is an exact integer, so each walk is a
+    // difference of exact prefix sums C and each stopping point a first
+    // crossing of C (binary search).  The sliding window's upper end is the
+    // running maximum of the per-start first crossings.
+    const double binSize = h_binSize, lowerBinVal = h_lowerBinVal;
+    if (okCount == 0 || binSize == 0.0) {
+      if (tid == 0) h_state = 2;
+      __syncthreads();
+      break;
+    }
+    {
+      int tot;
+      const int pre = block_prefix_sum(bins[tid], isc, &tot);
+      hC[tid] = pre;
+      if (tid == 0) hC[256] = tot;
+      __syncthreads();
+    }
+    const double thr = okCount / 10.0;
+    // window start L = tid (1..254): first V in [L+1, 255] with C[V]-C[L] >= thr
+    int V = 0;
+    if (tid >= 1 && tid <= 254) {
+      const int cl = hC[tid];
+      int lo = tid + 1, hi = 255;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((double)(hC[mid] - cl) >= thr) hi = mid;
+        else lo = mid + 1;
+      }
+      V = lo;
+    }
+    const int Uincl = max(1, block_prefix_max_incl(V, isc));
+    hU[tid] = Uincl;
+    __syncthreads();
+    double dens = -INFINITY;
+    if (tid >= 1 && tid <= 254) {
+      const int prevU = tid == 1 ? 1 : hU[tid - 1];
+      if (prevU < 255) {  // the walk reaches start tid
+        const double d = (double)(hC[Uincl] - hC[tid]) / (double)(Uincl - tid);
+        if (d > 0.0) dens = d;
+      }
+    }
+    int maxIndex = block_argmax_first(dens, tid, hdv, isc);
+    {
+      // every thread took part, so hdv holds the per-wave maxima: no start
+      // had a positive density when all of them are -inf
+      bool any = false;
+      for (int i = 0; i < 4; i++) any = any || hdv[i] > -INFINITY;
+      if (!any) maxIndex = -1;
+    }
+    if (maxIndex < 0 || maxIndex > 255) maxIndex = 0;
+    // mode window [maxIndex, iEnd)
+    int iEnd;
+    {
+      const int cm = hC[maxIndex];
+      int lo = maxIndex, hi = 255;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((double)(hC[mid] - cm) >= thr) hi = mid;
+        else lo = mid + 1;
+      }
+      iEnd = lo;
+    }
+    const long long se = block_sum_ll(tid >= maxIndex && tid < iEnd ? (long long)tid * bins[tid] : 0ll,
+                                      hll);
     if (tid == 0) {
-      double binSize = h_binSize, lowerBinVal = h_lowerBinVal;
-      if (okCount == 0 || binSize == 0.0) {
-        h_state = 2;
+      const double sumBins = (double)(hC[iEnd] - hC[maxIndex]);
+      const double sumExpect = (double)se;
+      double modeBin = sumExpect / sumBins + 0.5;
+      double skyEst = lowerBinVal + binSize * (modeBin - 1.0);
+      double lower = okCount * 0.25, upper = okCount * 0.75;
+      auto firstAtLeast = [&](int from, double v) {
+        int lo = from, hi = 256;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if ((double)hC[mid] >= v) hi = mid;
+          else lo = mid + 1;
+        }
+        return lo;
+      };
+      int i = firstAtLeast(0, lower);
+      lower = i - ((double)hC[i] - lower) / bins[i - 1];
+      i = firstAtLeast(i, upper);
+      upper = i - ((double)hC[i] - upper) / bins[i - 1];
+      h_sky = skyEst;
+      h_lower = lower;
+      h_upper = upper;
+      if (lower < 1.0 || upper > 255.0) {
+        h_lowerBinVal = lowerBinVal - 128.0 * binSize;
+        h_binSize = binSize * 2;
+      } else if (upper - lower < 40.0) {
+        h_binSize = binSize / 3.0;
+        h_lowerBinVal = skyEst - 128.0 * h_binSize;
       } else {
-        double sumBins = 0.0, maxDens = 0.0;
-        int lowerIndex = 1, upperIndex = 1, maxIndex = -1;
-        while (upperIndex < 255) {
-          while (sumBins < okCount / 10.0 && upperIndex < 255) sumBins += bins[upperIndex++];
-          if (sumBins / (upperIndex - lowerIndex) > maxDens) {
-            maxDens = sumBins / (upperIndex - lowerIndex);
-            maxIndex = lowerIndex;
-          }
-          sumBins -= bins[lowerIndex++];
-        }
-        if (maxIndex < 0 || maxIndex > 255) maxIndex = 0;
-        sumBins = 0.0;
-        double sumExpect = 0.0;
-        for (int i = maxIndex; sumBins < okCount / 10.0 && i < 255; i++) {
-          sumBins += bins[i];
-          sumExpect += i * bins[i];
-        }
-        double modeBin = sumExpect / sumBins + 0.5;
-        double skyEst = lowerBinVal + binSize * (modeBin - 1.0);
-        double lower = okCount * 0.25, upper = okCount * 0.75;
-        sumBins = 0.0;
-        int i = 0;
-        while (sumBins < lower) sumBins += bins[i++];
-        lower = i - (sumBins - lower) / bins[i - 1];
-        while (sumBins < upper) sumBins += bins[i++];
-        upper = i - (sumBins - upper) / bins[i - 1];
-        h_sky = skyEst;
-        h_lower = lower;
-        h_upper = upper;
-        if (lower < 1.0 || upper > 255.0) {
-          h_lowerBinVal = lowerBinVal - 128.0 * binSize;
-          h_binSize = binSize * 2;
-        } else if (upper - lower < 40.0) {
-          h_binSize = binSize / 3.0;
-          h_lowerBinVal = skyEst - 128.0 * h_binSize;
-        } else {
-          h_state = 1;
-        }
+        h_state = 1;
       }
     }
     __syncthreads();
@@ -336,11 +525,6 @@ constexpr int FIND_WAVES = FIND_THREADS / WAVE;
 // pp & 31, row pp >> 5, so no lane divides by the window width; the raster
 // index row * width + column keeps the reference's scan order for ties.
 constexpr int WIN_PER_LANE = 32 * 32 / WAVE;
-
-// n / d for n < 2^20 and 2 <= d <= 1024 by a reciprocal multiply (exact over
-// that range: every pair was checked against integer division).
-__device__ __forceinline__ int fdiv(int n, unsigned m) { return (int)__umulhi((unsigned)n, m); }
-__host__ __device__ __forceinline__ unsigned fdiv_magic(int d) { return 0xFFFFFFFFu / (unsigned)d + 1u; }
 
 // Pixel access of one stamp's search.  With STAGED, the stamp's fsw x fsw
 // region lives in LDS: the pixel values (float) and one bit per pixel telling
