@@ -1,13 +1,23 @@
 #!/usr/bin/env python3
 """Throughput of the X-BACH PSF-matching + subtraction path on one MI355X.
 
-One step = one full init -> sub pipeline (masks, stamp statistics and substamp
-selection, basis convolution + normal equations, the direction test fits, the
-global fit with its clip-refit loop, the spatially varying convolution and the
-difference image) over one seeded synthetic 2048x2048 FP32 pair that is
-already resident in HBM; outputs stay in HBM.  Prints one JSON line.
+Two forms of the same path are measured on seeded synthetic 2048x2048 FP32
+pairs that are already resident in HBM (outputs stay in HBM):
 
-    python bench.py [--steps K] [--warmup W] [--workload pair2048]
+* plain: one pair at a time, one step = one full init -> sub pipeline (masks,
+  stamp statistics and substamp selection, basis convolution + normal
+  equations, the direction test fits, the global fit with its clip-refit loop,
+  the spatially varying convolution and the difference image);
+* batched: one step = one batch of --batch distinct pairs run by the batch
+  runtime (--workers pairs in flight, each through the same bit-identical
+  pipeline); pair 0 of the batch is the plain pair, and its outputs are
+  checked to equal the plain run's bit for bit.
+
+`value` is the batched form's diffs/s (the plain form's figures are under
+"forms"); both forms produce identical difference images, so the residual
+RMS is equal by construction.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--batch B] [--workers N] [--form both]
 
 cpu_baseline: the CPU restatement of the same path (oracle/, FP64 with the
 reference's FP32 steps) timed on this host's cores on a bounded number of
@@ -21,6 +31,12 @@ import os
 import pathlib
 import sys
 import time
+
+# The batch runtime keeps one HIP stream per pair in flight; HIP maps streams
+# onto GPU_MAX_HW_QUEUES hardware queues (4 by default), and streams that share
+# a queue run one after the other.  Read before the HIP runtime starts.
+HW_QUEUES = 16
+os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
 
 import numpy as np
 
@@ -77,11 +93,78 @@ def fp64_residual(params, tmpl, sci) -> float:
     return _compare.residual_rms(d)
 
 
+def synth_batch(base, n: int) -> list:
+    """n distinct pairs of the named workload: seeds base.seed + i (pair 0 is
+    the plain pair)."""
+    cfgs = []
+    for i in range(n):
+        c = xbach.SynthCfg.from_buffer_copy(base)
+        c.seed = base.seed + i
+        cfgs.append(c)
+    return xbach.synth_pairs(cfgs, threads=min(16, os.cpu_count() or 1))
+
+
+def run_plain(ctx, params, w, h, d_t, d_s, steps, warmup):
+    for _ in range(warmup):
+        ctx.run_device(params, w, h, d_t, d_s)
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.run_device(params, w, h, d_t, d_s)  # returns after the stream drained
+    t1 = time.perf_counter()
+    prof = ctx.profile_get()
+    ctx.profile(False)
+    return (t1 - t0) * 1e3 / steps, prof
+
+
+def run_batched(ctx, params, pairs, workers, steps, warmup):
+    """Time `steps` batches of all pairs (device-resident in and out)."""
+    bufs = []
+    for t, s in pairs:
+        q = [ctx.buf_alloc(t.nbytes) for _ in range(4)]
+        ctx.buf_upload(q[0], t)
+        ctx.buf_upload(q[1], s)
+        bufs.append(q)
+    ws = [t.shape[1] for t, _ in pairs]
+    hs = [t.shape[0] for t, _ in pairs]
+    cols = [[q[k] for q in bufs] for k in range(4)]
+    with xbach.Batch(0, workers=workers) as b:
+        for _ in range(warmup):
+            b.run_device(params, ws, hs, *cols)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            res = b.run_device(params, ws, hs, *cols)
+        t1 = time.perf_counter()
+    conv0 = np.empty_like(pairs[0][0])
+    diff0 = np.empty_like(pairs[0][0])
+    ctx.buf_download(conv0, bufs[0][2])
+    ctx.buf_download(diff0, bufs[0][3])
+    for q in bufs:
+        for d in q:
+            ctx.buf_free(d)
+    return (t1 - t0) * 1e3 / steps, res, conv0, diff0
+
+
+def residual(diff) -> float:
+    good = diff != np.float32(1e-30)
+    return float(np.sqrt(np.mean(diff[good].astype(np.float64) ** 2)))
+
+
+def same_bits(a, b) -> bool:
+    nan = np.isnan(b)
+    return bool(np.array_equal(np.isnan(a), nan)
+                and np.array_equal(a[~nan].view(np.uint32), b[~nan].view(np.uint32)))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=40, help="plain pipelines timed")
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=48, help="pairs per batched step")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--workers", type=int, default=8, help="pairs in flight")
+    ap.add_argument("--form", default="both", choices=["both", "plain"])
     ap.add_argument("--workload", default="pair2048", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -115,30 +198,19 @@ def main() -> int:
     ctx.buf_upload(d_t, tmpl)
     ctx.buf_upload(d_s, sci)
 
-    for _ in range(args.warmup):
-        ctx.run_device(params, w, h, d_t, d_s)
-
-    ctx.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.run_device(params, w, h, d_t, d_s)  # returns after the stream drained
-    t1 = time.perf_counter()
-    prof = ctx.profile_get()
-    ctx.profile(False)
-
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = 1e3 / ms_per_step
-
-    # result of the timed run
+    plain_ms, prof = run_plain(ctx, params, w, h, d_t, d_s, args.steps, args.warmup)
+    plain_value = 1e3 / plain_ms
     d_conv, d_diff = ctx.device_outputs()
+    conv = np.empty_like(tmpl)
     diff = np.empty_like(tmpl)
+    ctx.buf_download(conv, d_conv)
     ctx.buf_download(diff, d_diff)
-    good = diff != np.float32(1e-30)
-    rms = float(np.sqrt(np.mean(diff[good].astype(np.float64) ** 2)))
+    rms = residual(diff)
 
-    # roofline of the convolution kernel (the data-parallel hot kernel): the
-    # FP64 441-tap stencil at 882 flop per output pixel is compute-bound
-    # (about 70 flop per HBM byte, far above the FP64 ridge of ~10)
+    # roofline of the convolution kernel (the data-parallel hot kernel), from
+    # the plain run's HIP events on the stream k_conv runs on: the FP64 441-tap
+    # stencil at 882 flop per output pixel is compute-bound (about 70 flop per
+    # HBM byte, far above the FP64 ridge of ~10)
     conv_ms, conv_n = prof["conv"]
     conv_avg_s = conv_ms / max(conv_n, 1) * 1e-3
     conv_flops = 2.0 * 441 * (w - 20) * (h - 20)
@@ -151,13 +223,36 @@ def main() -> int:
     kernels["sub"]["achieved_GBs"] = sub_bytes / sub_avg_s / 1e9
     kernels["conv"]["achieved_TFLOPs"] = conv_tflops
 
+    forms = {"plain": {"value": plain_value, "ms_per_diff": plain_ms, "steps": args.steps,
+                       "warmup": args.warmup, "residual_rms": rms}}
+    value, ms_per_step, steps, warmup, form = plain_value, plain_ms, args.steps, args.warmup, "plain"
+    if args.form == "both":
+        pairs = synth_batch(cfg, args.batch)
+        assert np.array_equal(pairs[0][0], tmpl) and np.array_equal(pairs[0][1], sci)
+        b_ms, res, conv0, diff0 = run_batched(ctx, params, pairs, args.workers,
+                                              args.batch_steps, 1)
+        del pairs
+        ok = all(r.status == 0 for r in res)
+        b_value = args.batch * 1e3 / b_ms
+        forms["batched"] = {
+            "value": b_value, "ms_per_batch": b_ms, "pairs_per_batch": args.batch,
+            "pairs_in_flight": args.workers, "hw_queues": HW_QUEUES,
+            "steps": args.batch_steps, "warmup": 1, "all_pairs_ok": ok,
+            "pair0_bit_identical_to_plain": same_bits(conv0, conv) and same_bits(diff0, diff),
+            "residual_rms_pair0": residual(diff0),
+            "pair_ms_mean": float(np.mean([r.ms for r in res])),
+            "speedup_vs_plain": b_value / plain_value}
+        if ok and forms["batched"]["pair0_bit_identical_to_plain"]:
+            value, ms_per_step, steps, warmup, form = (b_value, b_ms, args.batch_steps, 1,
+                                                       "batched")
+
     line = {
         "metric": METRIC,
         "value": value,
         "unit": "diffs/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -165,7 +260,8 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.workload], "name": args.workload,
-                   "form": "plain", "w": w, "h": h, "seed": int(cfg.seed)},
+                   "form": form, "w": w, "h": h, "seed": int(cfg.seed),
+                   "pairs_per_step": args.batch if form == "batched" else 1},
         # k_conv issues FP64 VALU multiply-adds (unfused, in the reference's
         # term order); MI355X's dense FP64 peak is 78.6 TFLOP/s
         "roofline": {"bound": "mfma", "kernel": "k_conv (fp64 VALU, unfused mul+add)",
@@ -173,6 +269,7 @@ def main() -> int:
                      "frac": conv_tflops / PEAK_FP64_TFLOPS,
                      "traffic": CONV_TRAFFIC_BYTES if (w, h) == (2048, 2048) else None,
                      "algorithmic": f"2*441*(W-20)*(H-20) = {conv_flops:.4g} flop per launch"},
+        "forms": forms,
         "kernels": kernels,
         "residual_rms": rms,
     }
@@ -182,6 +279,7 @@ def main() -> int:
         line["residual_rms_fp64_oracle"] = ref
         line["residual_rms_rel_diff"] = abs(rms - ref) / ref
         line["gpu_vs_cpu"] = value / line["cpu_baseline"]["value"]
+        line["plain_gpu_vs_cpu"] = plain_value / line["cpu_baseline"]["value"]
     ctx.buf_free(d_t)
     ctx.buf_free(d_s)
     ctx.close()

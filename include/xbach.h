@@ -224,6 +224,51 @@ int xb_fits_write(const char* path, int w, int h, const float* data, char* msg, 
  * the way src/main.cpp:56-142 does calls this after each stage. */
 int xb_sync(xb_ctx* ctx);
 
+/* xb_ctx_create flag: run both images' independent work on the one primary
+ * stream instead of forking a second stream (used by the batch runtime, where
+ * the concurrency comes from several pairs in flight). */
+#define XB_CTX_ONE_STREAM 1
+
+/* ---- batches of independent pairs ----------------------------------------
+ * The reference processes one pair per process (src/main.cpp:16-159).  A batch
+ * runs many pairs, of any sizes, through the same pipeline at once: `workers`
+ * host threads each own a context and pull pairs (largest first) from a shared
+ * queue, so the latency-bound stages of one pair (the FP64 LU, the clip-refit
+ * read-backs) overlap other pairs' work on the GPU.  Every pair's outputs are
+ * bit-identical to xb_pipeline_run on that pair alone.  One batch handle per
+ * host thread; runs on one handle do not overlap. */
+typedef struct xb_batch xb_batch;
+
+typedef struct xb_pair_result {
+  int32_t status;          /* XB_OK or the pair's error code                  */
+  int32_t convTemplate;    /* direction chosen by cd                          */
+  int32_t kscIterations;   /* clip-refit passes                               */
+  int32_t worker;          /* worker that ran the pair                        */
+  double kernSum;          /* kernel sum at the image centre                  */
+  double ms;               /* wall time of the pair on its worker             */
+} xb_pair_result;
+
+/* workers >= 1 (contexts created up front, flags as xb_ctx_create plus
+ * XB_CTX_ONE_STREAM implied). */
+int xb_batch_create(int device, int workers, int flags, xb_batch** out);
+int xb_batch_destroy(xb_batch* b);
+const char* xb_batch_last_error(xb_batch* b);
+/* Pairs in host memory: tmpl[i], sci[i] of w[i] x h[i] FP32 pixels; conv[i],
+ * diff[i] receive the outputs (either array, or any entry, may be NULL).
+ * results (n entries, may be NULL).  Returns XB_OK when every pair succeeded,
+ * else the first failing pair's code (all pairs are still attempted). */
+int xb_batch_run(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
+                 const float* const* tmpl, const float* const* sci, float* const* conv,
+                 float* const* diff, xb_pair_result* results);
+/* Same with device-resident inputs and outputs (device pointers from
+ * xb_buf_alloc or any HIP allocation on the batch's device). */
+int xb_batch_run_device(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
+                        const void* const* d_tmpl, const void* const* d_sci,
+                        void* const* d_conv, void* const* d_diff, xb_pair_result* results);
+/* Number of pairs in flight used by the next run (1..workers); 0 restores
+ * "all workers".  The autotuner sets this per size bucket. */
+int xb_batch_set_inflight(xb_batch* b, int inflight);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,116 @@
+"""Batches of independent pairs (xb_batch_*): many pairs of uneven sizes in
+flight at once, each bit-identical to the single-pair pipeline.
+
+Bar: every pair's convolved and difference images, kernel sum, direction and
+clip-refit count equal a lone xb_pipeline_run of the same pair bit for bit
+(the single-pair path is itself bit-identical to the oracle, and one pair per
+batch is checked against the oracle here too); a pair that fails reports its
+own status without disturbing the others; host- and device-resident batches
+agree; results do not depend on the number of pairs in flight.
+"""
+import numpy as np
+import pytest
+
+import _compare
+import _oracle
+import xbach
+
+pytestmark = pytest.mark.gpu
+
+NO_SUBSTAMPS = 3
+
+
+def _same(a, b):
+    nan = np.isnan(b)
+    return np.array_equal(np.isnan(a), nan) and np.array_equal(a[~nan].view(np.uint32),
+                                                               b[~nan].view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def pairs():
+    # uneven square sides (the batching law, sides cut to 512..1280 to keep the
+    # test short) plus a ragged rectangle and a pair with no usable stamps
+    cfgs = xbach.batch_cfgs(9, seed=77, side=(512, 1280))
+    ps = [xbach.synth_pair(c) for c in cfgs]
+    ps.append(xbach.synth_pair(xbach.synth_cfg("pair512", w=900, h=700, nstars=1500, seed=31)))
+    z = np.zeros((300, 300), np.float32)
+    ps.append((z, z))
+    return ps
+
+
+@pytest.fixture(scope="module")
+def singles(pairs):
+    p = xbach.Params.default()
+    out = []
+    with xbach.Context(0) as c:
+        for t, s in pairs[:-1]:
+            conv, diff, _ = c.run(p, t, s)
+            tr = c.trace()
+            out.append((conv, diff, tr.kernSum, tr.convTemplate, tr.kscIterations))
+    return out
+
+
+def test_batch_host_equals_single_runs(pairs, singles):
+    p = xbach.Params.default()
+    with xbach.Batch(0, workers=4) as b:
+        outs, res = b.run(p, pairs, check=False)
+    for i, (conv, diff, ks, ct, it) in enumerate(singles):
+        r = res[i]
+        assert r.status == 0, (i, r.status)
+        assert _same(outs[i][0], conv) and _same(outs[i][1], diff), i
+        assert (r.kernSum, r.convTemplate, r.kscIterations) == (ks, ct, it), i
+    assert res[-1].status == NO_SUBSTAMPS
+    assert all(0 <= r.worker < 4 for r in res)
+
+
+def test_batch_pair_against_oracle(pairs):
+    # the smallest pair of the batch, straight against the CPU restatement
+    p = xbach.Params.default()
+    k = min(range(len(pairs) - 1), key=lambda i: pairs[i][0].size)
+    with xbach.Batch(0, workers=2) as b:
+        outs, res = b.run(p, [pairs[k], pairs[0]])
+    cr, dr, _, trr, _ = _oracle.run(p, *pairs[k])
+    assert _same(outs[0][0], cr) and _same(outs[0][1], dr)
+    assert res[0].kernSum == trr.kernSum
+
+
+@pytest.mark.parametrize("inflight", [1, 3])
+def test_batch_device_resident(pairs, singles, inflight):
+    p = xbach.Params.default()
+    use = pairs[:-1]
+    with xbach.Context(0) as c, xbach.Batch(0, workers=3) as b:
+        bufs = []
+        try:
+            for t, s in use:
+                dt, ds, dc, dd = (c.buf_alloc(t.nbytes) for _ in range(4))
+                c.buf_upload(dt, t)
+                c.buf_upload(ds, s)
+                bufs.append((dt, ds, dc, dd))
+            b.set_inflight(inflight)
+            res = b.run_device(p, [t.shape[1] for t, _ in use], [t.shape[0] for t, _ in use],
+                               [q[0] for q in bufs], [q[1] for q in bufs],
+                               [q[2] for q in bufs], [q[3] for q in bufs])
+            assert {r.worker for r in res} <= set(range(inflight))
+            for i, (t, _) in enumerate(use):
+                conv = np.empty_like(t)
+                diff = np.empty_like(t)
+                c.buf_download(conv, bufs[i][2])
+                c.buf_download(diff, bufs[i][3])
+                assert _same(conv, singles[i][0]) and _same(diff, singles[i][1]), i
+                assert res[i].kernSum == singles[i][2]
+        finally:
+            for q in bufs:
+                for d in q:
+                    c.buf_free(d)
+
+
+def test_batch_argument_errors():
+    p = xbach.Params.default()
+    with xbach.Batch(0, workers=2) as b:
+        with pytest.raises(xbach.XbachError, match="ERR_ARG"):
+            b.set_inflight(3)
+        t = np.ones((64, 64), np.float32)
+        with pytest.raises(xbach.XbachError, match="ERR_ARG"):
+            b.run(xbach.Params.default(fKernelWidth=19), [(t, t)])
+        outs, res = b.run(p, [])
+        assert outs == [] and res == []

@@ -49,6 +49,26 @@ def test_no_gpu_is_an_error_not_a_crash():
         xbach.lib().xb_ctx_destroy(h)
     else:
         assert rc == 6
+    b = C.c_void_p()
+    assert xbach.lib().xb_batch_create(0, 0, 0, C.byref(b)) == 1  # no workers: XB_ERR_ARG
+    rc = xbach.lib().xb_batch_create(0, 2, 0, C.byref(b))
+    if rc == 0:
+        xbach.lib().xb_batch_destroy(b)
+    else:
+        assert rc == 6 and not b.value
+
+
+def test_batch_cfgs_follow_the_batching_law():
+    cfgs = xbach.batch_cfgs(64)
+    assert [c.seed for c in cfgs] == [64064 * 1000 + i for i in range(64)]
+    sides = np.array([c.w for c in cfgs])
+    assert np.array_equal(sides, [c.h for c in cfgs])
+    assert sides.min() >= 512 and sides.max() <= 3072 and np.ptp(sides) > 1500
+    dens = np.array([c.nstars / c.w ** 2 for c in cfgs])
+    assert dens.min() > 0.99e-3 and dens.max() < 5.01e-3
+    f = np.array([[c.fwhmT, c.fwhmS0] for c in cfgs])
+    assert f.min() >= 1.5 and f.max() <= 4.0
+    assert [c.seed for c in xbach.batch_cfgs(3)] == [c.seed for c in cfgs[:3]]
 
 
 def test_synth_is_seeded_and_poisson():

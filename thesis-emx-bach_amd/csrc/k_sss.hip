@@ -954,14 +954,12 @@ hipError_t launch_sss_find(Ctx& c, int im) {
   const size_t lds = ((nAll + 63) / 64 * 64) * sizeof(float) + (nAll + 63) / 64 * 8;
   prof_begin(c, XB_PROF_SSS);
   if (lds <= FIND_LDS_MAX) {
-    static bool attrSet = false;  // per process: the attribute belongs to the kernel
-    if (!attrSet) {
-      hipError_t e = hipFuncSetAttribute((const void*)k_sss_find<true>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)FIND_LDS_MAX);
-      if (e != hipSuccess) return e;
-      attrSet = true;
-    }
+    // once per process (the attribute belongs to the kernel); the static's
+    // initialisation is thread-safe, so batch workers may race here
+    static const hipError_t attrRc = hipFuncSetAttribute(
+        (const void*)k_sss_find<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (int)FIND_LDS_MAX);
+    if (attrRc != hipSuccess) return attrRc;
     hipLaunchKernelGGL(k_sss_find<true>, dim3(c.nGrid), dim3(FIND_THREADS), lds, c.cur, A);
   } else {
     hipLaunchKernelGGL(k_sss_find<false>, dim3(c.nGrid), dim3(FIND_THREADS), 0, c.cur, A);

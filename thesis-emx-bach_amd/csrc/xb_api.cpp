@@ -18,10 +18,6 @@
 
 using namespace xb;
 
-struct xb_ctx {
-  Ctx c;
-};
-
 namespace {
 
 #define XB_HIP(expr)                                                              \
@@ -239,6 +235,11 @@ int ensureSolver(Ctx& c) {
 // one after the other; the results do not depend on the order).
 template <class F>
 int forkBoth(Ctx& c, F f) {
+  if (c.side == c.stream) {  // one-stream context: the two parts run in order
+    int rc = XB_OK;
+    for (int im = 0; im < 2 && rc == XB_OK; im++) rc = f(im);
+    return rc;
+  }
   XB_HIP(hipEventRecord(c.evFork, c.stream));
   XB_HIP(hipStreamWaitEvent(c.side, c.evFork, 0));
   int rc = XB_OK;
@@ -566,7 +567,6 @@ void xb_params_default(xb_params* p) {
 }
 
 int xb_ctx_create(int device, int flags, xb_ctx** out) {
-  (void)flags;
   if (!out) return XB_ERR_ARG;
   *out = nullptr;
   int n = 0;
@@ -588,13 +588,16 @@ int xb_ctx_create(int device, int flags, xb_ctx** out) {
     }
   }
   // two streams of the default priority, and the events that fork and join them
+  // (one stream serves both roles under XB_CTX_ONE_STREAM)
+  const bool one = (flags & XB_CTX_ONE_STREAM) != 0;
   if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess ||
+      (one ? (c.side = c.stream, false)
+           : hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess) ||
       hipEventCreateWithFlags(&c.evFork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c.evJoin, hipEventDisableTiming) != hipSuccess) {
     if (c.evFork) (void)hipEventDestroy(c.evFork);
     if (c.evJoin) (void)hipEventDestroy(c.evJoin);
-    if (c.side) (void)hipStreamDestroy(c.side);
+    if (c.side && c.side != c.stream) (void)hipStreamDestroy(c.side);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete x;
     return XB_ERR_HIP;
@@ -609,7 +612,7 @@ int xb_ctx_destroy(xb_ctx* x) {
   Ctx& c = x->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
-  if (c.side) (void)hipStreamSynchronize(c.side);
+  if (c.side && c.side != c.stream) (void)hipStreamSynchronize(c.side);
   for (auto& p : c.evPending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -620,7 +623,7 @@ int xb_ctx_destroy(xb_ctx* x) {
   freeAll(c);
   if (c.evFork) (void)hipEventDestroy(c.evFork);
   if (c.evJoin) (void)hipEventDestroy(c.evJoin);
-  if (c.side) (void)hipStreamDestroy(c.side);
+  if (c.side && c.side != c.stream) (void)hipStreamDestroy(c.side);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete x;
   return XB_OK;
@@ -670,7 +673,7 @@ int xb_sync(xb_ctx* x) {
   int rc = setDev(c);
   if (rc) return rc;
   XB_HIP(hipStreamSynchronize(c.stream));
-  XB_HIP(hipStreamSynchronize(c.side));
+  if (c.side != c.stream) XB_HIP(hipStreamSynchronize(c.side));
   return XB_OK;
 }
 int xb_cd(xb_ctx* x, int* convTemplate) { return x ? stageCd(x->c, convTemplate) : XB_ERR_ARG; }

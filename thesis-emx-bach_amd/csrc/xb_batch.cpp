@@ -1,0 +1,232 @@
+// xb_batch.cpp — many independent pairs through the pipeline at once.
+//
+// The reference runs one pair per process (src/main.cpp:16-159), and a single
+// pair cannot fill an MI355X: the global FP64 LU (src/bachUtil.cpp:319-411) is
+// a chain of ~300 dependent pivot steps on a handful of CUs, and the clip-refit
+// loop (src/cdkscUtil.cpp:585-684) stops at a host read-back every pass.  The
+// batch runtime keeps several pairs in flight instead: each worker is a host
+// thread that owns one context (one HIP stream, its own resident buffers) and
+// pulls pairs from a shared queue, largest first, so one pair's latency-bound
+// stages overlap the other pairs' work.  A pair's arithmetic is exactly the
+// single-pair pipeline's, so every output is bit-identical to xb_pipeline_run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "xb_internal.h"
+
+struct xb_batch {
+  int device = 0;
+  int inflight = 0;  // 0: all workers
+  std::vector<xb_ctx*> ctx;
+  std::string err;
+};
+
+namespace {
+
+double nowMs() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Job {
+  const xb_params* p;
+  int n;
+  const int* w;
+  const int* h;
+  bool device;
+  const void* const* tin;
+  const void* const* sin;
+  void* const* cout;
+  void* const* dout;
+  xb_pair_result* res;
+  std::vector<int> order;   // pair indices, largest first
+  std::vector<int> status;  // per pair
+  std::atomic<int> next{0};
+  std::mutex mu;
+  int firstBad = -1;        // lowest failing pair index
+  std::string firstMsg;
+};
+
+int hipFail(xb_ctx* x, hipError_t e, const char* what) {
+  x->c.err = std::string(what) + ": " + hipGetErrorString(e);
+  return XB_ERR_HIP;
+}
+
+// One pair on one worker's context: the single-pair pipeline, then the outputs
+// copied to the caller's buffers on the context's stream.
+int runPair(xb_ctx* x, const Job& J, int i, xb_pair_result* r) {
+  xb::Ctx& c = x->c;
+  const size_t bytes = (size_t)J.w[i] * J.h[i] * sizeof(float);
+  if (J.device) {
+    int rc = xb_pipeline_run_device(x, J.p, J.w[i], J.h[i], J.tin[i], J.sin[i]);
+    if (rc) return rc;
+    hipError_t e;
+    if (J.cout && J.cout[i] &&
+        (e = hipMemcpyAsync(J.cout[i], c.conv, bytes, hipMemcpyDeviceToDevice, c.stream)))
+      return hipFail(x, e, "copy conv");
+    if (J.dout && J.dout[i] &&
+        (e = hipMemcpyAsync(J.dout[i], c.diff, bytes, hipMemcpyDeviceToDevice, c.stream)))
+      return hipFail(x, e, "copy diff");
+    if ((e = hipMemcpyAsync(&c.kernSum, &c.sc->kernSum, sizeof(double), hipMemcpyDeviceToHost,
+                            c.stream)))
+      return hipFail(x, e, "read kernel sum");
+    if ((e = hipStreamSynchronize(c.stream))) return hipFail(x, e, "sync");
+  } else {
+    int rc = xb_pipeline_run(x, J.p, J.w[i], J.h[i], static_cast<const float*>(J.tin[i]),
+                             static_cast<const float*>(J.sin[i]),
+                             J.cout ? static_cast<float*>(J.cout[i]) : nullptr,
+                             J.dout ? static_cast<float*>(J.dout[i]) : nullptr, nullptr);
+    if (rc) return rc;
+  }
+  if (r) {
+    r->convTemplate = c.convTemplate;
+    r->kscIterations = c.kscIterations;
+    r->kernSum = c.kernSum;
+  }
+  return XB_OK;
+}
+
+void worker(xb_batch* b, int wi, Job* J) {
+  xb_ctx* x = b->ctx[wi];
+  if (hipSetDevice(b->device) != hipSuccess) return;  // pairs stay XB_ERR_HIP
+  for (;;) {
+    const int k = J->next.fetch_add(1);
+    if (k >= J->n) break;
+    const int i = J->order[k];
+    xb_pair_result local{};
+    xb_pair_result* r = J->res ? &J->res[i] : &local;
+    const double t0 = nowMs();
+    const int rc = runPair(x, *J, i, r);
+    r->status = rc;
+    r->worker = wi;
+    r->ms = nowMs() - t0;
+    J->status[i] = rc;
+    if (rc) {
+      std::lock_guard<std::mutex> g(J->mu);
+      if (J->firstBad < 0 || i < J->firstBad) {
+        J->firstBad = i;
+        J->firstMsg = "pair " + std::to_string(i) + ": " + x->c.err;
+      }
+      // a failed pair may leave work queued; start the next from a clean stream
+      (void)hipStreamSynchronize(x->c.stream);
+    }
+  }
+}
+
+int runBatch(xb_batch* b, Job& J) {
+  if (!b) return XB_ERR_ARG;
+  if (!J.p || J.n < 0 || (J.n > 0 && (!J.w || !J.h || !J.tin || !J.sin))) {
+    b->err = "bad arguments";
+    return XB_ERR_ARG;
+  }
+  for (int i = 0; i < J.n; i++) {
+    if (J.w[i] <= 0 || J.h[i] <= 0 || !J.tin[i] || !J.sin[i]) {
+      b->err = "pair " + std::to_string(i) + ": bad size or input pointer";
+      return XB_ERR_ARG;
+    }
+  }
+  if (J.n == 0) return XB_OK;
+  J.order.resize(J.n);
+  std::iota(J.order.begin(), J.order.end(), 0);
+  // largest pairs first (longest-processing-time order balances the workers)
+  std::stable_sort(J.order.begin(), J.order.end(), [&](int a, int c) {
+    return (long long)J.w[a] * J.h[a] > (long long)J.w[c] * J.h[c];
+  });
+  J.status.assign(J.n, XB_ERR_HIP);
+  const int nw = std::min<int>(J.n, b->inflight > 0 ? std::min<int>(b->inflight, (int)b->ctx.size())
+                                                    : (int)b->ctx.size());
+  std::vector<std::thread> th;
+  th.reserve(nw);
+  for (int wi = 0; wi < nw; wi++) th.emplace_back(worker, b, wi, &J);
+  for (auto& t : th) t.join();
+  if (J.firstBad >= 0) {
+    b->err = J.firstMsg;
+    return J.status[J.firstBad];
+  }
+  b->err.clear();
+  return XB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xb_batch_create(int device, int workers, int flags, xb_batch** out) {
+  if (!out || workers < 1 || workers > 64) return XB_ERR_ARG;
+  *out = nullptr;
+  xb_batch* b = new xb_batch();
+  b->device = device;
+  for (int i = 0; i < workers; i++) {
+    xb_ctx* x = nullptr;
+    const int rc = xb_ctx_create(device, flags | XB_CTX_ONE_STREAM, &x);
+    if (rc) {
+      for (auto* y : b->ctx) xb_ctx_destroy(y);
+      delete b;
+      return rc;
+    }
+    b->ctx.push_back(x);
+  }
+  *out = b;
+  return XB_OK;
+}
+
+int xb_batch_destroy(xb_batch* b) {
+  if (!b) return XB_OK;
+  for (auto* x : b->ctx) xb_ctx_destroy(x);
+  delete b;
+  return XB_OK;
+}
+
+const char* xb_batch_last_error(xb_batch* b) { return b ? b->err.c_str() : "no batch"; }
+
+int xb_batch_set_inflight(xb_batch* b, int inflight) {
+  if (!b || inflight < 0 || inflight > (int)b->ctx.size()) return XB_ERR_ARG;
+  b->inflight = inflight;
+  return XB_OK;
+}
+
+int xb_batch_run(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
+                 const float* const* tmpl, const float* const* sci, float* const* conv,
+                 float* const* diff, xb_pair_result* results) {
+  Job J;
+  J.p = p;
+  J.n = n;
+  J.w = w;
+  J.h = h;
+  J.device = false;
+  J.tin = reinterpret_cast<const void* const*>(tmpl);
+  J.sin = reinterpret_cast<const void* const*>(sci);
+  J.cout = reinterpret_cast<void* const*>(conv);
+  J.dout = reinterpret_cast<void* const*>(diff);
+  J.res = results;
+  return runBatch(b, J);
+}
+
+int xb_batch_run_device(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
+                        const void* const* d_tmpl, const void* const* d_sci,
+                        void* const* d_conv, void* const* d_diff, xb_pair_result* results) {
+  Job J;
+  J.p = p;
+  J.n = n;
+  J.w = w;
+  J.h = h;
+  J.device = true;
+  J.tin = d_tmpl;
+  J.sin = d_sci;
+  J.cout = d_conv;
+  J.dout = d_diff;
+  J.res = results;
+  return runBatch(b, J);
+}
+
+}  // extern "C"

@@ -188,3 +188,8 @@ hipError_t launch_conv(Ctx& c);
 hipError_t launch_sub(Ctx& c);
 
 }  // namespace xb
+
+// the opaque handle of the C ABI
+struct xb_ctx {
+  xb::Ctx c;
+};

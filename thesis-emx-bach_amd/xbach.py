@@ -167,6 +167,22 @@ SYNTH_CONFIGS = {
 }
 
 
+def batch_cfgs(n: int, seed: int = 64064, side=(512, 3072)) -> list[SynthCfg]:
+    """Seeded batch of independent pairs: square side ~ U[side], star density
+    ~ U[1e-3, 5e-3] per px, template and science FWHM each ~ U[1.5, 4.0] px,
+    fluxes and sky as pair512.  The draws come from numpy's PCG64 seeded with
+    `seed`; pair i's pixels are seeded with seed * 1000 + i."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        s = int(rng.integers(side[0], side[1] + 1))
+        dens = float(rng.uniform(1e-3, 5e-3))
+        ft, fs = (float(v) for v in rng.uniform(1.5, 4.0, 2))
+        out.append(synth_cfg("pair512", w=s, h=s, nstars=int(round(dens * s * s)), fwhmT=ft,
+                             fwhmS0=fs, seed=seed * 1000 + i))
+    return out
+
+
 def synth_cfg(name: str | None = None, **kw) -> SynthCfg:
     base = dict(SYNTH_CONFIGS[name]) if name else {}
     base.update(kw)
@@ -298,6 +314,13 @@ def lib() -> C.CDLL:
                                C.c_size_t]
     L.xb_fits_write.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, C.c_char_p, C.c_size_t]
     L.xb_sync.argtypes = [vp]
+    L.xb_batch_create.argtypes = [C.c_int, C.c_int, C.c_int, P(vp)]
+    L.xb_batch_destroy.argtypes = [vp]
+    L.xb_batch_last_error.argtypes = [vp]
+    L.xb_batch_last_error.restype = C.c_char_p
+    L.xb_batch_set_inflight.argtypes = [vp, C.c_int]
+    L.xb_batch_run.argtypes = [vp, P(Params), C.c_int, vp, vp, vp, vp, vp, vp, vp]
+    L.xb_batch_run_device.argtypes = [vp, P(Params), C.c_int, vp, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -339,6 +362,14 @@ def synth_pair(cfg: SynthCfg) -> tuple[np.ndarray, np.ndarray]:
     if rc != 0:
         raise RuntimeError(f"xb_synth_pair failed: {STATUS_NAMES.get(rc, rc)}")
     return t, s
+
+
+def synth_pairs(cfgs, threads: int = 8) -> list[tuple[np.ndarray, np.ndarray]]:
+    """synth_pair over many configurations on host threads (same results)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(synth_pair, cfgs))
 
 
 class Context:
@@ -501,6 +532,91 @@ class Context:
         self._check(self._L.xb_device_outputs(self._h, C.byref(c), C.byref(d)),
                     "xb_device_outputs")
         return c.value, d.value
+
+
+class PairResult(C.Structure):
+    """xb_pair_result: per-pair outcome of a batch run."""
+    _fields_ = [("status", C.c_int32), ("convTemplate", C.c_int32),
+                ("kscIterations", C.c_int32), ("worker", C.c_int32),
+                ("kernSum", C.c_double), ("ms", C.c_double)]
+
+
+def _ptr_array(ptrs) -> C.Array:
+    return (C.c_void_p * max(len(ptrs), 1))(*[p if p else None for p in ptrs])
+
+
+class Batch:
+    """Many independent pairs through the pipeline at once (xb_batch): worker
+    threads, each with its own context, pull pairs largest first."""
+
+    def __init__(self, device: int = 0, workers: int = 4):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.xb_batch_create(device, workers, 0, C.byref(h))
+        if rc != 0:
+            raise XbachError(f"xb_batch_create failed: {STATUS_NAMES.get(rc, rc)}")
+        self._h = h
+        self.workers = workers
+
+    def close(self):
+        if self._h:
+            self._L.xb_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.xb_batch_last_error(self._h)
+            raise XbachError(f"{what}: {STATUS_NAMES.get(rc, rc)}: "
+                             f"{msg.decode() if msg else ''}")
+
+    def set_inflight(self, n: int):
+        self._check(self._L.xb_batch_set_inflight(self._h, n), "xb_batch_set_inflight")
+
+    def run(self, params: Params, pairs, check: bool = True):
+        """pairs: list of (tmpl, sci) host arrays.  Returns ([(conv, diff)], results)."""
+        ts = [np.ascontiguousarray(t, np.float32) for t, _ in pairs]
+        ss = [np.ascontiguousarray(s, np.float32) for _, s in pairs]
+        n = len(ts)
+        w = np.array([t.shape[1] for t in ts], np.int32)
+        h = np.array([t.shape[0] for t in ts], np.int32)
+        convs = [np.empty_like(t) for t in ts]
+        diffs = [np.empty_like(t) for t in ts]
+        res = (PairResult * max(n, 1))()
+        rc = self._L.xb_batch_run(self._h, C.byref(params), n, w.ctypes.data, h.ctypes.data,
+                                  _ptr_array([t.ctypes.data for t in ts]),
+                                  _ptr_array([s.ctypes.data for s in ss]),
+                                  _ptr_array([c.ctypes.data for c in convs]),
+                                  _ptr_array([d.ctypes.data for d in diffs]), res)
+        if check:
+            self._check(rc, "xb_batch_run")
+        return list(zip(convs, diffs)), [res[i] for i in range(n)]
+
+    def run_device(self, params: Params, w, h, d_tmpl, d_sci, d_conv=None, d_diff=None,
+                   check: bool = True):
+        """Device-resident pairs (lists of device pointers); returns the results."""
+        n = len(w)
+        wa = np.ascontiguousarray(w, np.int32)
+        ha = np.ascontiguousarray(h, np.int32)
+        res = (PairResult * max(n, 1))()
+        rc = self._L.xb_batch_run_device(
+            self._h, C.byref(params), n, wa.ctypes.data, ha.ctypes.data, _ptr_array(d_tmpl),
+            _ptr_array(d_sci), _ptr_array(d_conv) if d_conv is not None else None,
+            _ptr_array(d_diff) if d_diff is not None else None, res)
+        if check:
+            self._check(rc, "xb_batch_run_device")
+        return [res[i] for i in range(n)]
 
 
 def diff_fits_stats(golden: np.ndarray, ours: np.ndarray) -> dict:
