@@ -153,6 +153,17 @@ int xb_get_conv(xb_ctx* ctx, float* out);
 int xb_get_diff(xb_ctx* ctx, float* out);
 int xb_get_mask(xb_ctx* ctx, uint16_t* out);
 int xb_get_trace(xb_ctx* ctx, xb_trace* out);
+/* Fit data of kept stamp k (kept order) of image im (0 template, 1 science)
+ * as the last cmv or ksc refill left it: the W rows of fillStamps
+ * (cl_kern/cmv.cl:70-193; 52 x 961 row-major, raw: the even-even rows before
+ * the W[n] - W[0] of cmv.cl:137-154), G (52 x 52, lower triangle: the Gram
+ * matrix of the rows after that subtraction; createQ, cmv.cl:195-215, is G
+ * shifted to 1-based), Bc (52: the rows' products with the target, createB
+ * cmv.cl:217-248), Br (3: the background rows' products in row-major order,
+ * cl_kern/cd.cl:318-344), and the substamp centre ssxy[2] (-1 when the stamp
+ * has run out of substamps).  Any output may be NULL. */
+int xb_get_stamp_fit(xb_ctx* ctx, int im, int k, double* W, double* G, double* Bc, double* Br,
+                     int* ssxy);
 /* Device pointers of the results (valid until the next xb_init). */
 int xb_device_outputs(xb_ctx* ctx, const void** d_conv, const void** d_diff);
 
@@ -224,6 +235,17 @@ int xb_fits_write(const char* path, int w, int h, const float* data, char* msg, 
  * the way src/main.cpp:56-142 does calls this after each stage. */
 int xb_sync(xb_ctx* ctx);
 
+/* ---- forms of the path ----------------------------------------------------
+ * XB_FORM_PLAIN (0): the reference's arithmetic, bit-identical to it.
+ * XB_FORM_MIXED: cmv's basis convolution and normal-matrix products run on
+ * the matrix cores in BF16 split precision (hi + lo) with FP32 accumulation;
+ * the stamp and global LU solves, chi2, convolution and difference stay FP64.
+ * Results differ from the plain form at the 1e-5 level of W and Q; the
+ * selected substamps (sss) are the same.  Applies from the next xb_init. */
+#define XB_FORM_PLAIN 0
+#define XB_FORM_MIXED 1
+int xb_set_form(xb_ctx* ctx, int form);
+
 /* xb_ctx_create flag: run both images' independent work on the one primary
  * stream instead of forking a second stream (used by the batch runtime, where
  * the concurrency comes from several pairs in flight). */
@@ -268,6 +290,8 @@ int xb_batch_run_device(xb_batch* b, const xb_params* p, int n, const int* w, co
 /* Number of pairs in flight used by the next run (1..workers); 0 restores
  * "all workers".  The autotuner sets this per size bucket. */
 int xb_batch_set_inflight(xb_batch* b, int inflight);
+/* Form (XB_FORM_*) of every pair of the next run. */
+int xb_batch_set_form(xb_batch* b, int form);
 
 #ifdef __cplusplus
 }

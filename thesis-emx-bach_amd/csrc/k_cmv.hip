@@ -20,41 +20,6 @@
 
 namespace xb {
 
-struct FillArgs {
-  const float* img;   // image being convolved
-  const float* img2;  // target image
-  int w, h;
-  const int* list;    // stamp list (kept indices) or nullptr for identity
-  const int* nList;   // device count of the list
-  const int* kept;
-  const int* cur;
-  const int* count;
-  const int* ssx;
-  const int* ssy;
-  const Basis* basis;
-  double* W;
-  double* G;
-  double* Bc;
-  double* Br;
-};
-
-__device__ __forceinline__ bool fill_stamp(const FillArgs& A, int k, int* s, int* x, int* y) {
-  int n = *A.nList;
-  if (k >= n) return false;
-  int st = A.list ? A.list[k] : k;
-  *s = st;
-  int g = A.kept[st];
-  int c = A.cur[st];
-  if (c < A.count[g]) {
-    *x = A.ssx[g * MAXSS + c];
-    *y = A.ssy[g * MAXSS + c];
-    return true;
-  }
-  *x = 0;
-  *y = 0;
-  return true;
-}
-
 __global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
   int s, ssx, ssy;
   if (!fill_stamp(A, blockIdx.x, &s, &ssx, &ssy)) return;
@@ -326,6 +291,11 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax) {
   A.Bc = c.st[im].Bc;
   A.Br = c.st[im].Br;
   prof_begin(c, XB_PROF_FILL);
+  if (c.form & XB_FORM_MIXED) {
+    const hipError_t em = launch_fill_mixed(c, A, nMax);
+    prof_end(c, XB_PROF_FILL);
+    return em;
+  }
   hipLaunchKernelGGL(k_fill_conv, dim3(nMax, WROWS), dim3(256), 0, c.cur, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

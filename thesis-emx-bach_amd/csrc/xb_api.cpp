@@ -90,6 +90,43 @@ void buildBasis(const xb_params& a, Basis& B) {
 }
 
 // ---- NR ran1 draws, seed -666 (cl_kern/sss.cl:41-79) ----------------------
+// BF16 bits of x, round to nearest even (NaN stays a quiet NaN).
+uint16_t bf16Bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// The mixed form's 2-D basis as MFMA B operands (k_cmv_mixed.hip): for MFMA
+// step ks, basis tile nt, part (BF16 hi, then lo) and lane l, 8 taps
+// k = 16 ks + 8 (l / 32) + t of basis n = 32 nt + l % 32, tap k = (u, v) =
+// (k / 24, k % 24); value fy_n[20-u] * fx_n[20-v] for u, v < 21 and n < 49,
+// else 0.  x = hi + lo with hi = bf16(x), lo = bf16(x - hi).
+void mixedTable(const Basis& B, std::vector<uint16_t>& out) {
+  constexpr int KSTEPS = 32, TAPW = 24;
+  out.assign((size_t)KSTEPS * 2 * 2 * 64 * 8, 0);
+  for (int ks = 0; ks < KSTEPS; ks++)
+    for (int nt = 0; nt < 2; nt++)
+      for (int l = 0; l < 64; l++)
+        for (int t = 0; t < 8; t++) {
+          const int k = 16 * ks + 8 * (l / 32) + t, u = k / TAPW, v = k % TAPW;
+          const int n = 32 * nt + l % 32;
+          double x = 0.0;
+          if (u < KW && v < KW && n < NPSF) x = B.fy[n * KW + 20 - u] * B.fx[n * KW + 20 - v];
+          const float xf = (float)x;
+          const uint16_t hi = bf16Bits(xf);
+          uint32_t hb = (uint32_t)hi << 16;
+          float hf;
+          std::memcpy(&hf, &hb, 4);
+          const uint16_t lo = bf16Bits(xf - hf);
+          const size_t at = ((((size_t)ks * 2 + nt) * 2) * 64 + l) * 8 + t;
+          out[at] = hi;
+          out[at + 64 * 8] = lo;
+        }
+}
+
 void ran1Table(int n, std::vector<double>& out) {
   const long M1 = 259200, IA1 = 7141, IC1 = 54773;
   const long M2 = 134456, IA2 = 8121, IC2 = 28411;
@@ -284,6 +321,15 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
   std::memset(&hb, 0, sizeof(hb));
   buildBasis(c.p, hb);
   XB_HIP(hipMemcpyAsync(c.basis, &hb, sizeof(Basis), hipMemcpyHostToDevice, c.stream));
+  std::vector<uint16_t> tab;
+  if ((c.form & XB_FORM_MIXED) &&
+      (!c.mixTab || std::memcmp(c.mixBg, c.p.bg, sizeof(c.mixBg)) != 0)) {
+    mixedTable(hb, tab);
+    if (!c.mixTab) XB_HIP(hipMalloc(&c.mixTab, tab.size() * sizeof(uint16_t)));
+    XB_HIP(hipMemcpyAsync(c.mixTab, tab.data(), tab.size() * sizeof(uint16_t),
+                          hipMemcpyHostToDevice, c.stream));
+    std::memcpy(c.mixBg, c.p.bg, sizeof(c.mixBg));
+  }
   XB_HIP(hipMemsetAsync(c.sc, 0, sizeof(DevScalars), c.stream));
   XB_HIP(hipStreamSynchronize(c.stream));
   return XB_OK;
@@ -471,6 +517,8 @@ void freeAll(Ctx& c) {
   }
   dfree(&c.sol); dfree(&c.cdSol);
   dfree(&c.tileK64); dfree(&c.tileAks);
+  if (c.mixTab) (void)hipFree(c.mixTab);
+  c.mixTab = nullptr;
 }
 
 int setDev(Ctx& c) {
@@ -664,6 +712,12 @@ int xb_init_device(xb_ctx* x, const xb_params* p, int w, int h, const void* d_tm
   return stageMask(c);
 }
 
+int xb_set_form(xb_ctx* x, int form) {
+  if (!x || (form & ~XB_FORM_MIXED)) return XB_ERR_ARG;
+  x->c.form = form;
+  return XB_OK;
+}
+
 int xb_sss(xb_ctx* x) { return x ? stageSss(x->c) : XB_ERR_ARG; }
 int xb_cmv(xb_ctx* x) { return x ? stageCmv(x->c) : XB_ERR_ARG; }
 
@@ -781,6 +835,40 @@ int xb_profile_get(xb_ctx* x, double* ms, int* counts, int n) {
   for (int i = 0; i < n && i < XB_PROF_N; i++) {
     if (ms) ms[i] = c.profMs[i];
     if (counts) counts[i] = c.profCnt[i];
+  }
+  return XB_OK;
+}
+
+int xb_get_stamp_fit(xb_ctx* x, int im, int k, double* W, double* G, double* Bc, double* Br,
+                     int* ssxy) {
+  if (!x || im < 0 || im > 1) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  if (k < 0 || k >= c.nKept[im]) {
+    c.err = "no such kept stamp";
+    return XB_ERR_ARG;
+  }
+  XB_HIP(hipStreamSynchronize(c.stream));
+  const DevStamps& S = c.st[im];
+  if (W) XB_HIP(hipMemcpy(W, S.W + (size_t)k * WROWS * NPIX, sizeof(double) * WROWS * NPIX,
+                          hipMemcpyDeviceToHost));
+  if (G) XB_HIP(hipMemcpy(G, S.G + (size_t)k * WROWS * WROWS, sizeof(double) * WROWS * WROWS,
+                          hipMemcpyDeviceToHost));
+  if (Bc) XB_HIP(hipMemcpy(Bc, S.Bc + (size_t)k * WROWS, sizeof(double) * WROWS,
+                           hipMemcpyDeviceToHost));
+  if (Br) XB_HIP(hipMemcpy(Br, S.Br + (size_t)k * NBG, sizeof(double) * NBG,
+                           hipMemcpyDeviceToHost));
+  if (ssxy) {
+    int g = 0, cur = 0, cnt = 0;
+    XB_HIP(hipMemcpy(&g, S.kept + k, sizeof(int), hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(&cur, S.cur + k, sizeof(int), hipMemcpyDeviceToHost));
+    XB_HIP(hipMemcpy(&cnt, S.count + g, sizeof(int), hipMemcpyDeviceToHost));
+    ssxy[0] = ssxy[1] = -1;
+    if (cur < cnt) {
+      XB_HIP(hipMemcpy(&ssxy[0], S.ssx + (size_t)g * MAXSS + cur, sizeof(int),
+                       hipMemcpyDeviceToHost));
+      XB_HIP(hipMemcpy(&ssxy[1], S.ssy + (size_t)g * MAXSS + cur, sizeof(int),
+                       hipMemcpyDeviceToHost));
+    }
   }
   return XB_OK;
 }

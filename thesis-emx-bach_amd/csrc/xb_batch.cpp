@@ -195,6 +195,15 @@ int xb_batch_set_inflight(xb_batch* b, int inflight) {
   return XB_OK;
 }
 
+int xb_batch_set_form(xb_batch* b, int form) {
+  if (!b) return XB_ERR_ARG;
+  for (auto* x : b->ctx) {
+    const int rc = xb_set_form(x, form);
+    if (rc) return rc;
+  }
+  return XB_OK;
+}
+
 int xb_batch_run(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
                  const float* const* tmpl, const float* const* sci, float* const* conv,
                  float* const* diff, xb_pair_result* results) {

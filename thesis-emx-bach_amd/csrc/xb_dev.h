@@ -193,4 +193,43 @@ __device__ __forceinline__ void mask_or_atomic(uint16_t* mask, size_t idx, uint1
   atomicOr(w, ((unsigned)bits) << sh);
 }
 
+// ---- cmv: the stamps one fill launch covers (plain and mixed forms) ----
+struct FillArgs {
+  const float* img;   // image being convolved
+  const float* img2;  // target image
+  int w, h;
+  const int* list;    // stamp list (kept indices) or nullptr for identity
+  const int* nList;   // device count of the list
+  const int* kept;
+  const int* cur;
+  const int* count;
+  const int* ssx;
+  const int* ssy;
+  const Basis* basis;
+  double* W;
+  double* G;
+  double* Bc;
+  double* Br;
+};
+
+__device__ __forceinline__ bool fill_stamp(const FillArgs& A, int k, int* s, int* x, int* y) {
+  int n = *A.nList;
+  if (k >= n) return false;
+  int st = A.list ? A.list[k] : k;
+  *s = st;
+  int g = A.kept[st];
+  int c = A.cur[st];
+  if (c < A.count[g]) {
+    *x = A.ssx[g * MAXSS + c];
+    *y = A.ssy[g * MAXSS + c];
+    return true;
+  }
+  *x = 0;
+  *y = 0;
+  return true;
+}
+
+// the mixed-precision form of the fill (k_cmv_mixed.hip)
+hipError_t launch_fill_mixed(Ctx& c, const FillArgs& A, int nMax);
+
 }  // namespace xb

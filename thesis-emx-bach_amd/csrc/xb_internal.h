@@ -137,6 +137,9 @@ struct Ctx {
   LuSet lu[2];                  // one global system per direction (cd runs both at once)
   double* sol = nullptr;        // [NSOL] current kernel solution
   double* cdSol = nullptr;      // [2][NSOL]
+  int form = 0;                 // XB_FORM_* of the next run
+  void* mixTab = nullptr;       // mixed form: split BF16 2-D basis, MFMA fragment order
+  float mixBg[3] = {0, 0, 0};   // basis widths the table was built for
   double* tileK64 = nullptr;    // [tiles][KK] FP64 tile kernels
   double* tileAks = nullptr;    // [tiles] sum |k| in the conv loop order
   int capTiles = 0;

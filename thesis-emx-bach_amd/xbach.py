@@ -314,6 +314,9 @@ def lib() -> C.CDLL:
                                C.c_size_t]
     L.xb_fits_write.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, C.c_char_p, C.c_size_t]
     L.xb_sync.argtypes = [vp]
+    L.xb_set_form.argtypes = [vp, C.c_int]
+    L.xb_batch_set_form.argtypes = [vp, C.c_int]
+    L.xb_get_stamp_fit.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
     L.xb_batch_create.argtypes = [C.c_int, C.c_int, C.c_int, P(vp)]
     L.xb_batch_destroy.argtypes = [vp]
     L.xb_batch_last_error.argtypes = [vp]
@@ -349,6 +352,9 @@ def fits_write_native(path, data: np.ndarray) -> None:
     if rc != 0:
         raise XbachError(msg.value.decode())
 
+
+# forms of the path (include/xbach.h XB_FORM_*)
+FORM_PLAIN, FORM_MIXED = 0, 1
 
 # per-kernel timing slots (include/xbach.h XB_PROF_*)
 PROF_SLOTS = ["conv", "sub", "solve", "fill", "sss", "mask"]
@@ -527,6 +533,24 @@ class Context:
                                            len(PROF_SLOTS)), "xb_profile_get")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_SLOTS)}
 
+    def set_form(self, form: int):
+        """FORM_PLAIN (the reference's arithmetic) or FORM_MIXED (cmv on MFMA in
+        BF16 split precision); applies from the next init."""
+        self._check(self._L.xb_set_form(self._h, form), "xb_set_form")
+
+    def stamp_fit(self, im: int, k: int) -> dict:
+        """W (52x961 raw rows), G (52x52 lower), Bc, Br and the substamp centre
+        of kept stamp k of image im, as the last fill left them."""
+        W = np.empty((52, 961))
+        G = np.empty((52, 52))
+        Bc = np.empty(52)
+        Br = np.empty(3)
+        xy = np.empty(2, np.int32)
+        self._check(self._L.xb_get_stamp_fit(self._h, im, k, W.ctypes.data, G.ctypes.data,
+                                             Bc.ctypes.data, Br.ctypes.data, xy.ctypes.data),
+                    "xb_get_stamp_fit")
+        return {"W": W, "G": G, "Bc": Bc, "Br": Br, "ssx": int(xy[0]), "ssy": int(xy[1])}
+
     def device_outputs(self) -> tuple[int, int]:
         c, d = C.c_void_p(), C.c_void_p()
         self._check(self._L.xb_device_outputs(self._h, C.byref(c), C.byref(d)),
@@ -580,6 +604,9 @@ class Batch:
             msg = self._L.xb_batch_last_error(self._h)
             raise XbachError(f"{what}: {STATUS_NAMES.get(rc, rc)}: "
                              f"{msg.decode() if msg else ''}")
+
+    def set_form(self, form: int):
+        self._check(self._L.xb_batch_set_form(self._h, form), "xb_batch_set_form")
 
     def set_inflight(self, n: int):
         self._check(self._L.xb_batch_set_inflight(self._h, n), "xb_batch_set_inflight")
