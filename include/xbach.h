@@ -244,7 +244,48 @@ int xb_sync(xb_ctx* ctx);
  * selected substamps (sss) are the same.  Applies from the next xb_init. */
 #define XB_FORM_PLAIN 0
 #define XB_FORM_MIXED 1
+/* XB_FORM_SCREEN: after each of the first 16 passes of the clip-refit loop
+ * (ksc), a small network (xb_screen_load) predicts from 32 features per stamp
+ * which of the substamps the pass kept a later pass would replace, and
+ * replaces them at once; the loop runs as usual and judges every substamp by
+ * its chi2.  XB_FORM_RECORD: keep the 32 features of every substamp each of
+ * those passes kept and whether the loop finally replaced it
+ * (xb_screen_samples): the training data of the network, labelled by the
+ * method itself.  Forms combine (bitwise or). */
+#define XB_FORM_SCREEN 2
+#define XB_FORM_RECORD 4
 int xb_set_form(xb_ctx* ctx, int form);
+
+/* ---- the learned substamp screen ------------------------------------------
+ * Model: XB_MLP_NPARAM floats = feature mean[32], feature sd[32],
+ * W1[64][32] (row = hidden unit), b1[64], w2[64], b2, threshold.  The network
+ * is 32 -> 64 (ReLU) -> 1 (logistic) on (feature - mean) / sd; W1 and w2 are
+ * used rounded to FP16.  A substamp is screened out when p > threshold. */
+#define XB_MLP_NFEAT 32
+#define XB_MLP_NPARAM 2242
+#define XB_MLP_NGRAD 2177 /* W1, b1, w2, b2 */
+int xb_screen_load(xb_ctx* ctx, const float* model);
+/* Samples of the last run with XB_FORM_RECORD or XB_FORM_SCREEN: for each
+ * recorded pass (first pass first) the kept stamps of the convolved image in
+ * kept order, feat[n][32], label[n] (2 replaced by that pass itself, 1 kept by
+ * that pass but replaced later — by the loop or by the screen, 0 used by the
+ * final fit, -1 no substamp), prob[n] (screen only, may be NULL).
+ * cap = capacity in samples; *n receives the count, *screened the number of
+ * substamps the screen replaced.  Any output may be NULL. */
+int xb_screen_samples(xb_ctx* ctx, float* feat, int* label, float* prob, int cap, int* n,
+                      int* screened);
+/* The network on its own (host arrays; used to train and to check it):
+ * xb_mlp_forward: prob[n] for features X[n][32].
+ * xb_mlp_grad: gradient of L = sum_k wt[k] * BCE(p_k, y_k) / n with respect
+ * to W1, b1, w2, b2 (grad[XB_MLP_NGRAD], in that order) and L (grad[NGRAD]).
+ * xb_mlp_train: Adam (lr, 0.9, 0.999) over `epochs` passes in batches of
+ * `batch` samples, in a seeded order; model is updated in place, loss[epochs]
+ * (may be NULL) receives the mean loss per epoch. */
+int xb_mlp_forward(xb_ctx* ctx, const float* model, const float* X, int n, float* prob);
+int xb_mlp_grad(xb_ctx* ctx, const float* model, const float* X, const float* y,
+                const float* wt, int n, float* grad);
+int xb_mlp_train(xb_ctx* ctx, float* model, const float* X, const float* y, const float* wt,
+                 int n, int epochs, int batch, float lr, uint64_t seed, float* loss);
 
 /* xb_ctx_create flag: run both images' independent work on the one primary
  * stream instead of forking a second stream (used by the batch runtime, where
@@ -292,6 +333,8 @@ int xb_batch_run_device(xb_batch* b, const xb_params* p, int n, const int* w, co
 int xb_batch_set_inflight(xb_batch* b, int inflight);
 /* Form (XB_FORM_*) of every pair of the next run. */
 int xb_batch_set_form(xb_batch* b, int form);
+/* Screen model of every worker (as xb_screen_load). */
+int xb_batch_screen_load(xb_batch* b, const float* model);
 
 #ifdef __cplusplus
 }

@@ -451,15 +451,52 @@ int stageCd(Ctx& c, int* convTemplate) {
   return XB_OK;
 }
 
+int ensureScreen(Ctx& c) {
+  ScreenBufs& S = c.scr;
+  if (S.cap >= c.capStamps) return XB_OK;
+  dfree(&S.feat);
+  dfree(&S.passCur);
+  dfree(&S.label);
+  dfree(&S.prob);
+  dfree(&S.flag);
+  dfree(&S.zPrev);
+  const size_t np = (size_t)SCR_PASSES * c.capStamps;
+  XB_HIP(dalloc(&S.feat, np * XB_MLP_NFEAT));
+  XB_HIP(dalloc(&S.passCur, np));
+  XB_HIP(dalloc(&S.label, np));
+  XB_HIP(dalloc(&S.prob, np));
+  XB_HIP(dalloc(&S.flag, (size_t)c.capStamps));
+  XB_HIP(dalloc(&S.zPrev, (size_t)c.capStamps));
+  S.cap = c.capStamps;
+  return XB_OK;
+}
+
 // ksc (src/cdkscUtil.cpp:585-684)
 int stageKsc(Ctx& c) {
   const int im = c.tIdx;
+  const bool screen = (c.form & XB_FORM_SCREEN) != 0;
+  const bool keep = screen || (c.form & XB_FORM_RECORD) != 0;
+  c.scrSamples = 0;
+  c.nScreened = 0;
+  if (screen && !c.scr.haveModel) {
+    c.err = "XB_FORM_SCREEN needs a model (xb_screen_load)";
+    return XB_ERR_ARG;
+  }
+  if (keep) {
+    int rc = ensureScreen(c);
+    if (rc) return rc;
+  }
   int it = 0;
   for (;;) {
     XB_HIP(launch_global_build(c, im, false));
     XB_HIP(launch_global_solve(c, c.lu[im], c.sol, nullptr));
     XB_HIP(launch_calc_sigs(c, im, false, c.sol));
+    const bool rec = keep && it < SCR_PASSES;
+    if (rec)  // the substamps of this pass
+      XB_HIP(hipMemcpyAsync(c.scr.passCur + (size_t)it * c.nKept[im], c.st[im].cur,
+                            sizeof(int) * c.nKept[im], hipMemcpyDeviceToDevice, c.cur));
     XB_HIP(launch_ksc_check(c, im));
+    if (rec) XB_HIP(launch_screen(c, im, it, screen));
     XB_HIP(launch_fill(c, im, c.st[im].list, c.nKept[im]));
     it++;
     int check = 0;
@@ -472,6 +509,15 @@ int stageKsc(Ctx& c) {
     }
   }
   c.kscIterations = it;
+  if (keep) {
+    const int nRec = std::min(it, SCR_PASSES);
+    XB_HIP(launch_screen_labels(c, im, nRec));
+    if (screen)
+      XB_HIP(hipMemcpyAsync(&c.nScreened, &c.sc->nScreened, sizeof(int), hipMemcpyDeviceToHost,
+                            c.stream));
+    XB_HIP(hipStreamSynchronize(c.stream));
+    c.scrSamples = nRec * c.nKept[im];
+  }
   return XB_OK;
 }
 
@@ -517,6 +563,10 @@ void freeAll(Ctx& c) {
   }
   dfree(&c.sol); dfree(&c.cdSol);
   dfree(&c.tileK64); dfree(&c.tileAks);
+  dfree(&c.scr.feat); dfree(&c.scr.passCur); dfree(&c.scr.label); dfree(&c.scr.prob);
+  dfree(&c.scr.flag); dfree(&c.scr.zPrev); dfree(&c.scr.model);
+  c.scr.cap = 0;
+  c.scr.haveModel = false;
   if (c.mixTab) (void)hipFree(c.mixTab);
   c.mixTab = nullptr;
 }
@@ -713,8 +763,49 @@ int xb_init_device(xb_ctx* x, const xb_params* p, int w, int h, const void* d_tm
 }
 
 int xb_set_form(xb_ctx* x, int form) {
-  if (!x || (form & ~XB_FORM_MIXED)) return XB_ERR_ARG;
+  if (!x || (form & ~(XB_FORM_MIXED | XB_FORM_SCREEN | XB_FORM_RECORD))) return XB_ERR_ARG;
   x->c.form = form;
+  return XB_OK;
+}
+
+int xb_screen_load(xb_ctx* x, const float* model) {
+  if (!x || !model) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  for (int i = 0; i < XB_MLP_NFEAT; i++)
+    if (!(model[XB_MLP_NFEAT + i] > 0.0f)) {
+      c.err = "screen model: feature sd must be positive";
+      return XB_ERR_ARG;
+    }
+  XB_HIP(hipSetDevice(c.device));
+  if (!c.scr.model) XB_HIP(hipMalloc((void**)&c.scr.model, sizeof(float) * XB_MLP_NPARAM));
+  XB_HIP(hipMemcpy(c.scr.model, model, sizeof(float) * XB_MLP_NPARAM, hipMemcpyHostToDevice));
+  c.scr.haveModel = true;
+  return XB_OK;
+}
+
+int xb_screen_samples(xb_ctx* x, float* feat, int* label, float* prob, int cap, int* n,
+                      int* screened) {
+  if (!x || cap < 0) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  const int N = c.scrSamples;
+  if (n) *n = N;
+  if (screened) *screened = c.nScreened;
+  if (N == 0 || (!feat && !label && !prob)) return XB_OK;  // size query
+  if (cap < N) {
+    c.err = "sample buffer too small";
+    return XB_ERR_CAPACITY;
+  }
+  XB_HIP(hipStreamSynchronize(c.stream));
+  if (feat) XB_HIP(hipMemcpy(feat, c.scr.feat, sizeof(float) * N * XB_MLP_NFEAT,
+                             hipMemcpyDeviceToHost));
+  if (label) XB_HIP(hipMemcpy(label, c.scr.label, sizeof(int) * N, hipMemcpyDeviceToHost));
+  if (prob) {
+    if (c.form & XB_FORM_SCREEN) {
+      XB_HIP(hipMemcpy(prob, c.scr.prob, sizeof(float) * N, hipMemcpyDeviceToHost));
+    } else {
+      for (int i = 0; i < N; i++) prob[i] = -1.0f;
+    }
+  }
   return XB_OK;
 }
 

@@ -204,6 +204,15 @@ int xb_batch_set_form(xb_batch* b, int form) {
   return XB_OK;
 }
 
+int xb_batch_screen_load(xb_batch* b, const float* model) {
+  if (!b) return XB_ERR_ARG;
+  for (auto* x : b->ctx) {
+    const int rc = xb_screen_load(x, model);
+    if (rc) return rc;
+  }
+  return XB_OK;
+}
+
 int xb_batch_run(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
                  const float* const* tmpl, const float* const* sci, float* const* conv,
                  float* const* diff, xb_pair_result* results) {

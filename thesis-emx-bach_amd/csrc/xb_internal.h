@@ -93,7 +93,7 @@ struct DevScalars {
   double clipMean, clipStd;
   double kernSum;       // kernel sum at the image centre (conv)
   int nChi2;
-  int pad;
+  int nScreened;        // substamps the learned screen replaced
 };
 
 // The global FP64 system of one fit and its LU work buffers.
@@ -105,6 +105,21 @@ struct LuSet {
   int* luIndex = nullptr;       // [MATN] pivot rows
   void* luState = nullptr;      // LU bookkeeping (zero row, last pivot, ii, zero flags)
   double* luVV = nullptr;       // [MATN]
+};
+
+// The learned substamp screen's per-pass buffers and model (the first
+// SCR_PASSES passes of the clip-refit loop are screened and recorded).
+constexpr int SCR_PASSES = 16;
+struct ScreenBufs {
+  float* feat = nullptr;    // [pass][N][32] features of each pass's substamps
+  int* passCur = nullptr;   // [pass][N] substamp of each stamp in that pass
+  int* label = nullptr;     // [pass][N] 2 replaced by the pass, 1 later, 0 kept, -1 none
+  float* prob = nullptr;    // [pass][N] p(reject)
+  uint8_t* flag = nullptr;  // [N] screened out in the current pass
+  float* zPrev = nullptr;   // [N] chi2 z-score of the previous pass
+  float* model = nullptr;   // [XB_MLP_NPARAM]
+  int cap = 0;
+  bool haveModel = false;
 };
 
 struct Ctx {
@@ -138,6 +153,9 @@ struct Ctx {
   double* sol = nullptr;        // [NSOL] current kernel solution
   double* cdSol = nullptr;      // [2][NSOL]
   int form = 0;                 // XB_FORM_* of the next run
+  ScreenBufs scr;
+  int nScreened = 0;
+  int scrSamples = 0;           // samples the last ksc recorded
   void* mixTab = nullptr;       // mixed form: split BF16 2-D basis, MFMA fragment order
   float mixBg[3] = {0, 0, 0};   // basis widths the table was built for
   double* tileK64 = nullptr;    // [tiles][KK] FP64 tile kernels
@@ -189,6 +207,10 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol);
 hipError_t launch_ksc_check(Ctx& c, int im);
 hipError_t launch_conv(Ctx& c);
 hipError_t launch_sub(Ctx& c);
+// learned screen (k_screen.hip): features of a pass's substamps, and with
+// apply, the network's decision merged into the pass's refill list
+hipError_t launch_screen(Ctx& c, int im, int pass, bool apply);
+hipError_t launch_screen_labels(Ctx& c, int im, int nRec);
 
 }  // namespace xb
 

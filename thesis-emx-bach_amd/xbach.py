@@ -315,6 +315,13 @@ def lib() -> C.CDLL:
     L.xb_fits_write.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, C.c_char_p, C.c_size_t]
     L.xb_sync.argtypes = [vp]
     L.xb_set_form.argtypes = [vp, C.c_int]
+    L.xb_screen_load.argtypes = [vp, vp]
+    L.xb_batch_screen_load.argtypes = [vp, vp]
+    L.xb_screen_samples.argtypes = [vp, vp, vp, vp, C.c_int, P(C.c_int), P(C.c_int)]
+    L.xb_mlp_forward.argtypes = [vp, vp, vp, C.c_int, vp]
+    L.xb_mlp_grad.argtypes = [vp, vp, vp, vp, vp, C.c_int, vp]
+    L.xb_mlp_train.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                               C.c_uint64, vp]
     L.xb_batch_set_form.argtypes = [vp, C.c_int]
     L.xb_get_stamp_fit.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
     L.xb_batch_create.argtypes = [C.c_int, C.c_int, C.c_int, P(vp)]
@@ -354,7 +361,9 @@ def fits_write_native(path, data: np.ndarray) -> None:
 
 
 # forms of the path (include/xbach.h XB_FORM_*)
-FORM_PLAIN, FORM_MIXED = 0, 1
+FORM_PLAIN, FORM_MIXED, FORM_SCREEN, FORM_RECORD = 0, 1, 2, 4
+# the screen's network (include/xbach.h XB_MLP_*)
+MLP_NFEAT, MLP_NPARAM, MLP_NGRAD = 32, 2242, 2177
 
 # per-kernel timing slots (include/xbach.h XB_PROF_*)
 PROF_SLOTS = ["conv", "sub", "solve", "fill", "sss", "mask"]
@@ -533,10 +542,63 @@ class Context:
                                            len(PROF_SLOTS)), "xb_profile_get")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_SLOTS)}
 
+    def sync(self):
+        """Wait for the work queued on the context (xb_sync)."""
+        self._check(self._L.xb_sync(self._h), "xb_sync")
+
     def set_form(self, form: int):
         """FORM_PLAIN (the reference's arithmetic) or FORM_MIXED (cmv on MFMA in
         BF16 split precision); applies from the next init."""
         self._check(self._L.xb_set_form(self._h, form), "xb_set_form")
+
+    def screen_load(self, model: np.ndarray):
+        m = np.ascontiguousarray(model, np.float32)
+        if m.size != MLP_NPARAM:
+            raise ValueError(f"screen model has {MLP_NPARAM} floats")
+        self._check(self._L.xb_screen_load(self._h, m.ctypes.data), "xb_screen_load")
+
+    def screen_samples(self):
+        """(features[n][32], labels[n], prob[n], n_screened) of the last run."""
+        n, scr = C.c_int(), C.c_int()
+        self._check(self._L.xb_screen_samples(self._h, None, None, None, 0, C.byref(n),
+                                              C.byref(scr)), "xb_screen_samples")
+        f = np.empty((n.value, MLP_NFEAT), np.float32)
+        lab = np.empty(n.value, np.int32)
+        pr = np.empty(n.value, np.float32)
+        if n.value:
+            self._check(self._L.xb_screen_samples(self._h, f.ctypes.data, lab.ctypes.data,
+                                                  pr.ctypes.data, n.value, C.byref(n),
+                                                  C.byref(scr)), "xb_screen_samples")
+        return f, lab, pr, scr.value
+
+    def mlp_forward(self, model, X) -> np.ndarray:
+        m = np.ascontiguousarray(model, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        p = np.empty(len(X), np.float32)
+        self._check(self._L.xb_mlp_forward(self._h, m.ctypes.data, X.ctypes.data, len(X),
+                                           p.ctypes.data), "xb_mlp_forward")
+        return p
+
+    def mlp_grad(self, model, X, y, wt) -> tuple[np.ndarray, float]:
+        m = np.ascontiguousarray(model, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        wt = np.ascontiguousarray(wt, np.float32)
+        g = np.empty(MLP_NGRAD + 1, np.float32)
+        self._check(self._L.xb_mlp_grad(self._h, m.ctypes.data, X.ctypes.data, y.ctypes.data,
+                                        wt.ctypes.data, len(X), g.ctypes.data), "xb_mlp_grad")
+        return g[:MLP_NGRAD], float(g[MLP_NGRAD])
+
+    def mlp_train(self, model, X, y, wt, epochs=40, batch=1024, lr=3e-3, seed=1):
+        m = np.array(model, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        wt = np.ascontiguousarray(wt, np.float32)
+        loss = np.empty(epochs, np.float32)
+        self._check(self._L.xb_mlp_train(self._h, m.ctypes.data, X.ctypes.data, y.ctypes.data,
+                                         wt.ctypes.data, len(X), epochs, batch, lr, seed,
+                                         loss.ctypes.data), "xb_mlp_train")
+        return m, loss
 
     def stamp_fit(self, im: int, k: int) -> dict:
         """W (52x961 raw rows), G (52x52 lower), Bc, Br and the substamp centre
@@ -607,6 +669,11 @@ class Batch:
 
     def set_form(self, form: int):
         self._check(self._L.xb_batch_set_form(self._h, form), "xb_batch_set_form")
+
+    def screen_load(self, model: np.ndarray):
+        m = np.ascontiguousarray(model, np.float32)
+        self._check(self._L.xb_batch_screen_load(self._h, m.ctypes.data),
+                    "xb_batch_screen_load")
 
     def set_inflight(self, n: int):
         self._check(self._L.xb_batch_set_inflight(self._h, n), "xb_batch_set_inflight")
