@@ -14,7 +14,7 @@ of the selected substamps contaminated by cosmic-ray hits (1-3 px, 5e3-5e4
 counts), stars saturated and clipped at 6e4, and variable sources (science
 flux x U[0.5, 2]).
 
-    python screen.py --pairs 160 --out screen_model.npy     (needs the GPU)
+    python screen.py --pairs 120 --clean 120 --out screen_model.npy   (needs the GPU)
 """
 from __future__ import annotations
 
@@ -111,11 +111,15 @@ def pair_cfgs(n: int, seed0: int, side=(1536, 3072), density=(2e-3, 3.5e-3)):
 
 def contaminated_pairs(ctx, params, cfgs, frac: float = 0.2, chunk: int = 16):
     """Yields (tmpl, sci) per configuration, contaminated at `frac` of the
-    substamps sss selects on the clean pair (synthesis on host threads)."""
+    substamps sss selects on the clean pair (synthesis on host threads);
+    frac = 0 yields the clean pairs."""
     for c0 in range(0, len(cfgs), chunk):
         part = cfgs[c0:c0 + chunk]
         clean = xbach.synth_pairs(part, threads=16)
         for cfg, (t, s) in zip(part, clean):
+            if frac <= 0.0:
+                yield t, s
+                continue
             ctx.set_form(xbach.FORM_PLAIN)
             try:
                 ctx.init(params, t, s)
@@ -125,6 +129,21 @@ def contaminated_pairs(ctx, params, cfgs, frac: float = 0.2, chunk: int = 16):
             centres = substamp_centres(ctx.trace().as_dict())
             t2, s2, _ = contaminate(t, s, centres, frac, int(cfg.seed) + 7)
             yield t2, s2
+
+
+def training_sets(ctx, n_crowded: int, n_clean: int, seed0: int = 900000):
+    """(params, pair source) of the training mixture: contaminated crowded
+    fields on a 20x20 stamp grid, and clean fields of the batching law and of
+    the 2048-pair law (ramping science PSF) on the default 10x10 grid — so the
+    network also learns where the plain loop needs no help."""
+    p20 = xbach.Params.default(stampsx=20, stampsy=20)
+    p10 = xbach.Params.default()
+    half = n_clean // 2
+    pair2048 = [xbach.synth_cfg("pair2048", seed=seed0 + 500000 + i) for i in range(n_clean - half)]
+    return [(p20, contaminated_pairs(ctx, p20, pair_cfgs(n_crowded, seed0))),
+            (p10, contaminated_pairs(ctx, p10, xbach.batch_cfgs(half, seed=seed0 // 1000 + 1),
+                                     frac=0.0)),
+            (p10, contaminated_pairs(ctx, p10, pair2048, frac=0.0))]
 
 
 def collect(ctx, params, pairs) -> tuple[np.ndarray, np.ndarray, list[int]]:
@@ -194,16 +213,24 @@ def train(ctx, X, y, seed=1, epochs=60, batch=1024, lr=3e-3):
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--pairs", type=int, default=160)
+    ap.add_argument("--pairs", type=int, default=120, help="contaminated crowded pairs")
+    ap.add_argument("--clean", type=int, default=120, help="clean pairs")
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--out", default=str(MODEL_PATH))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--min-precision", type=float, default=0.9)
     args = ap.parse_args()
-    params = xbach.Params.default(stampsx=20, stampsy=20)
     ctx = xbach.Context(0)
     t0 = time.time()
-    X, y, its = collect(ctx, params, contaminated_pairs(ctx, params, pair_cfgs(args.pairs, 900000)))
+    Xs, ys, its = [], [], []
+    for params, src in training_sets(ctx, args.pairs, args.clean):
+        X1, y1, it1 = collect(ctx, params, src)
+        Xs.append(X1)
+        ys.append(y1)
+        its += it1
+        print(f"set: {len(it1)} pairs, {len(y1)} samples, replaced {y1.mean():.3f}, "
+              f"passes mean {np.mean(it1):.2f}", flush=True)
+    X, y = np.concatenate(Xs), np.concatenate(ys)
     print(f"pairs: {len(its)}, samples: {len(y)}, replaced {int(y.sum())} ({y.mean():.3f}); "
           f"clip-refit passes mean {np.mean(its):.2f} ({time.time() - t0:.1f} s)", flush=True)
     rng = np.random.default_rng(args.seed)
