@@ -1,23 +1,25 @@
 """The mixed-precision form (XB_FORM_MIXED): cmv's basis convolution and
-normal-matrix products on the matrix cores in BF16 split precision (hi + lo)
-with FP32 accumulation; everything else FP64.
+normal-matrix products on the matrix cores in BF16 split precision (three
+BF16 parts, six part products) with FP32 accumulation; everything else FP64.
 
 Bars:
   * the stamp fit data read back per stamp (xb_get_stamp_fit) of the plain
     form equal the oracle's fillStamps/createQ/createB (cl_kern/cmv.cl:70-248)
     bit for bit — this pins how the read-back maps onto the reference's W, Q, B;
   * the mixed form's W, Q, B against the oracle in full FP64: per W row,
-    max |dW| <= 5e-5 max |W|; |dQ_ij| <= 5e-5 sqrt(Q_ii Q_jj);
-    |dB_i| <= 5e-5 sqrt(Q_ii * sum(target^2)) (a two-term BF16 split carries
-    16 significand bits, 2^-16 = 1.5e-5; measured worst 2.6e-5 on pair512);
+    max |dW| <= 5e-6 max |W|; |dQ_ij| <= 5e-6 sqrt(Q_ii Q_jj);
+    |dB_i| <= 5e-6 sqrt(Q_ii * sum(target^2)) (the split is exact in FP32,
+    the dropped part products are below 2^-24, and FP32 accumulation over
+    504 taps or 961 pixels leaves ~1e-6; measured worst 1.7e-6 on pair512);
   * end to end on the 2048x2048 pair: the substamps selected (sss) are the
     plain form's, bit for bit, and the difference image's residual RMS over
-    good interior pixels is within 1 % of the FP64 oracle's.  The clip-refit
-    loop may reject other substamps than the FP64 fit does (its chi2 values
-    move at the 1e-5 level), so the kernel coefficients are not bounded
-    pointwise; their difference, and the per-pixel difference of the
-    difference image in units of sigma_pix = sqrt(|T| + |S|), are reported
-    (measured: residual RMS +0.11 %, clip-refit passes 3 vs 4).
+    good interior pixels is within 1 % of the FP64 oracle's (measured
+    -0.016 %), and the pixels within 0.5 sigma_pix = sqrt(|T| + |S|) of the
+    FP64 oracle's (measured: median 1.2e-3, max 0.16).  The clip-refit loop
+    could in principle reject other substamps than the FP64 fit (its chi2
+    values move at the 1e-6 level), so the kernel coefficients — which the
+    ill-conditioned system leaves loosely determined along some directions —
+    are reported, not bounded pointwise.
 """
 import numpy as np
 import pytest
@@ -111,7 +113,7 @@ def test_mixed_stamp_fit_against_fp64(ctx, pair512):
             worst["Q"] = max(worst["Q"], float(dq.max()))
             worst["B"] = max(worst["B"], float(db.max()))
     print("mixed fill errors vs FP64:", worst)
-    assert worst["W"] <= 5e-5 and worst["Q"] <= 5e-5 and worst["B"] <= 5e-5, worst
+    assert worst["W"] <= 5e-6 and worst["Q"] <= 5e-6 and worst["B"] <= 5e-6, worst
 
 
 def test_mixed_pipeline_2048_against_fp64_oracle(ctx):
@@ -145,3 +147,4 @@ def test_mixed_pipeline_2048_against_fp64_oracle(ctx):
           f"{dsol:.3e} of the largest; ksc iterations fp64 {tr64['kscIterations']} "
           f"plain {trp['kscIterations']} mixed {trm['kscIterations']}")
     assert abs(rm - r64) <= 0.01 * r64
+    assert rel.max() <= 0.5

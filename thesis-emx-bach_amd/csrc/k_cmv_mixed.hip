@@ -3,8 +3,13 @@
 // The plain form (k_cmv.hip) follows the reference's arithmetic: separable
 // Y/X passes (cl_kern/cmv.cl:70-135) and the normal-matrix sums in FP64
 // (cmv.cl:195-248).  This form computes the same quantities as two dense
-// products on the matrix cores, in BF16 split precision (x = hi + lo, both
-// BF16; a*b ~ hi*hi + hi*lo + lo*hi) with FP32 accumulation:
+// products on the matrix cores, in BF16 split precision with FP32
+// accumulation: x = h + m + l (three BF16 parts, 24 significand bits — every
+// FP32 value exactly) and a*b ~ h*h + h*m + m*h + h*l + l*h + m*m (the
+// dropped terms are below 2^-24 of the product).  A two-part split (16 bits)
+// is not enough: the normal equations are ill-conditioned, and its 2e-5
+// errors in W and Q grew into chi2 changes that sent the clip-refit loop
+// through many more passes than the plain form's.  The products are:
 //
 //   W  (961 x 49)  = P (961 x 504) . V (504 x 49)
 //       P: the 51x51 window around the substamp, unrolled per pixel (tap (u,v)
@@ -47,7 +52,7 @@ constexpr int MX_SROWS = WROWS + 1;    // staged X columns: W rows 0..48, backgr
 static_assert(MX_WAVES * MX_PASSES * MX_TPP >= MX_MT, "tiles cover the substamp");
 
 // dynamic LDS carve (bytes)
-constexpr size_t MX_WIN_B = sizeof(uint32_t) * TW * MX_WP;                   // 11,424
+constexpr size_t MX_WIN_B = sizeof(uint2) * TW * MX_WP;                      // 22,848
 constexpr size_t MX_STAGE_B = sizeof(float) * MX_WAVES * MX_SROWS * MX_SP;   // 110,240
 constexpr size_t MX_TGT_B = sizeof(float) * 976;                             // 3,904
 constexpr size_t MX_MONO_B = sizeof(float) * 2 * 32;                         // 256
@@ -55,39 +60,53 @@ constexpr size_t MX_LDS = MX_WIN_B + MX_STAGE_B + MX_TGT_B + MX_MONO_B;
 static_assert(MX_LDS <= 160 * 1024, "mixed fill LDS");
 static_assert(MX_WAVES * 3 * 16 * WAVE * sizeof(float) <= MX_STAGE_B, "partial G' fits the stage");
 
-// hi | lo << 16 with hi = bf16(x), lo = bf16(x - hi), both rounded to
-// nearest even (gfx950 converts FP32 to BF16 in one instruction)
-__device__ __forceinline__ uint32_t split_word(float x) {
-  const __bf16 hi = (__bf16)x;
-  const __bf16 lo = (__bf16)(x - (float)hi);
-  return (uint32_t)__builtin_bit_cast(uint16_t, hi) |
-         ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+// x = h + m + l, each part BF16 rounded to nearest even (gfx950 converts
+// FP32 to BF16 in one instruction; the two differences are exact in FP32).
+// Packed as .x = h | m << 16, .y = l.
+__device__ __forceinline__ uint2 split3(float x) {
+  const __bf16 h = (__bf16)x;
+  const float r1 = x - (float)h;
+  const __bf16 m = (__bf16)r1;
+  const __bf16 l = (__bf16)(r1 - (float)m);
+  return make_uint2((uint32_t)__builtin_bit_cast(uint16_t, h) |
+                        ((uint32_t)__builtin_bit_cast(uint16_t, m) << 16),
+                    (uint32_t)__builtin_bit_cast(uint16_t, l));
 }
 
-// 8 split words -> (hi x8, lo x8) MFMA operands
-__device__ __forceinline__ void unpack8(const uint32_t w[8], bf16x8& hi, bf16x8& lo) {
-  uint4 h, l;
-  h.x = __builtin_amdgcn_perm(w[1], w[0], 0x05040100u);
-  h.y = __builtin_amdgcn_perm(w[3], w[2], 0x05040100u);
-  h.z = __builtin_amdgcn_perm(w[5], w[4], 0x05040100u);
-  h.w = __builtin_amdgcn_perm(w[7], w[6], 0x05040100u);
-  l.x = __builtin_amdgcn_perm(w[1], w[0], 0x07060302u);
-  l.y = __builtin_amdgcn_perm(w[3], w[2], 0x07060302u);
-  l.z = __builtin_amdgcn_perm(w[5], w[4], 0x07060302u);
-  l.w = __builtin_amdgcn_perm(w[7], w[6], 0x07060302u);
-  hi = __builtin_bit_cast(bf16x8, h);
-  lo = __builtin_bit_cast(bf16x8, l);
+struct Split8 {  // 8 values as the three MFMA operands
+  bf16x8 h, m, l;
+};
+
+__device__ __forceinline__ Split8 unpack8(const uint2 w[8]) {
+  uint4 h, m, l;
+  h.x = __builtin_amdgcn_perm(w[1].x, w[0].x, 0x05040100u);
+  h.y = __builtin_amdgcn_perm(w[3].x, w[2].x, 0x05040100u);
+  h.z = __builtin_amdgcn_perm(w[5].x, w[4].x, 0x05040100u);
+  h.w = __builtin_amdgcn_perm(w[7].x, w[6].x, 0x05040100u);
+  m.x = __builtin_amdgcn_perm(w[1].x, w[0].x, 0x07060302u);
+  m.y = __builtin_amdgcn_perm(w[3].x, w[2].x, 0x07060302u);
+  m.z = __builtin_amdgcn_perm(w[5].x, w[4].x, 0x07060302u);
+  m.w = __builtin_amdgcn_perm(w[7].x, w[6].x, 0x07060302u);
+  l.x = __builtin_amdgcn_perm(w[1].y, w[0].y, 0x05040100u);
+  l.y = __builtin_amdgcn_perm(w[3].y, w[2].y, 0x05040100u);
+  l.z = __builtin_amdgcn_perm(w[5].y, w[4].y, 0x05040100u);
+  l.w = __builtin_amdgcn_perm(w[7].y, w[6].y, 0x05040100u);
+  return {__builtin_bit_cast(bf16x8, h), __builtin_bit_cast(bf16x8, m),
+          __builtin_bit_cast(bf16x8, l)};
 }
 
-__device__ __forceinline__ f32x16 mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8& bh,
-                                        const bf16x8& bl, f32x16 c) {
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c, 0, 0, 0);
+// c += a * b over the six leading part products, smallest first
+__device__ __forceinline__ f32x16 mfma6(const Split8& a, const Split8& b, f32x16 c) {
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.m, b.m, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.l, b.h, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.h, b.l, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.m, b.h, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.h, b.m, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.h, b.h, c, 0, 0, 0);
   return c;
 }
 
-// vtab: [MX_KSTEPS][2 n-tiles][hi, lo][64 lanes] x 8 BF16 (uint4)
+// vtab: [MX_KSTEPS][2 n-tiles][h, m, l][64 lanes] x 8 BF16 (uint4)
 __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
                                                            const uint4* __restrict__ vtab) {
   int s, ssx, ssy;
@@ -107,7 +126,7 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
     return;
   }
   extern __shared__ __align__(16) unsigned char mx_lds[];
-  uint32_t* win = reinterpret_cast<uint32_t*>(mx_lds);
+  uint2* win = reinterpret_cast<uint2*>(mx_lds);
   float* stage = reinterpret_cast<float*>(mx_lds + MX_WIN_B);
   float* tgt = reinterpret_cast<float*>(mx_lds + MX_WIN_B + MX_STAGE_B);
   float* mono = reinterpret_cast<float*>(mx_lds + MX_WIN_B + MX_STAGE_B + MX_TGT_B);
@@ -120,7 +139,7 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
       const int xr = min(max(x0 + c, 0), A.w - 1), yr = min(max(y0 + r, 0), A.h - 1);
       v = A.img[(size_t)xr + (size_t)yr * A.w];
     }
-    win[i] = split_word(v);
+    win[i] = split3(v);
   }
   for (int i = tid; i < 976; i += MX_THREADS) {
     float v = 0.0f;
@@ -170,37 +189,36 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
       const int p = min((mt0 + gi) * 32 + col, NPIX - 1);
       base[gi] = (p / SSW) * MX_WP + p % SSW;
     }
+    // (the other wave on the SIMD covers the B loads' latency)
     const uint4* vt = vtab + lane;
-    uint4 nb0 = vt[0], nb1 = vt[WAVE], nb2 = vt[2 * WAVE], nb3 = vt[3 * WAVE];
     for (int ks = 0; ks < MX_KSTEPS; ks++) {
-      const bf16x8 bh0 = __builtin_bit_cast(bf16x8, nb0);
-      const bf16x8 bl0 = __builtin_bit_cast(bf16x8, nb1);
-      const bf16x8 bh1 = __builtin_bit_cast(bf16x8, nb2);
-      const bf16x8 bl1 = __builtin_bit_cast(bf16x8, nb3);
-      if (ks + 1 < MX_KSTEPS) {  // B one step ahead
-        const uint4* vn = vt + (size_t)(ks + 1) * 4 * WAVE;
-        nb0 = vn[0];
-        nb1 = vn[WAVE];
-        nb2 = vn[2 * WAVE];
-        nb3 = vn[3 * WAVE];
-      }
+      const uint4* vn = vt + (size_t)ks * 6 * WAVE;
+      const Split8 b0 = {__builtin_bit_cast(bf16x8, vn[0]), __builtin_bit_cast(bf16x8, vn[WAVE]),
+                         __builtin_bit_cast(bf16x8, vn[2 * WAVE])};
+      const Split8 b1 = {__builtin_bit_cast(bf16x8, vn[3 * WAVE]),
+                         __builtin_bit_cast(bf16x8, vn[4 * WAVE]),
+                         __builtin_bit_cast(bf16x8, vn[5 * WAVE])};
       int rowOff;
       uint32_t m45, m67, m03;
       tapRow(ks, rowOff, m45, m67, m03);
-      bf16x8 ah[MX_TPP], al[MX_TPP];
+      Split8 a[MX_TPP];
 #pragma unroll
       for (int gi = 0; gi < MX_TPP; gi++) {
-        uint32_t w[8];
+        uint2 w[8];
 #pragma unroll
         for (int t = 0; t < 8; t++) w[t] = win[base[gi] + rowOff + t];  // inside the window
-        w[0] &= m03; w[1] &= m03; w[2] &= m03; w[3] &= m03;
-        w[4] &= m03; w[5] &= m45; w[6] &= m67; w[7] &= m67;
-        unpack8(w, ah[gi], al[gi]);
+        const uint32_t mk[8] = {m03, m03, m03, m03, m03, m45, m67, m67};
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+          w[t].x &= mk[t];
+          w[t].y &= mk[t];
+        }
+        a[gi] = unpack8(w);
       }
 #pragma unroll
       for (int gi = 0; gi < MX_TPP; gi++) {
-        acc[gi][0] = mfma3(ah[gi], al[gi], bh0, bl0, acc[gi][0]);
-        acc[gi][1] = mfma3(ah[gi], al[gi], bh1, bl1, acc[gi][1]);
+        acc[gi][0] = mfma6(a[gi], b0, acc[gi][0]);
+        acc[gi][1] = mfma6(a[gi], b1, acc[gi][1]);
       }
     }
     // ---- stage the slab (rows 0..48) and X columns 49..52
@@ -238,7 +256,7 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
     const float* s1 = st + min(a1, WROWS) * MX_SP;
 #pragma unroll
     for (int kk = 0; kk < MX_SLAB / 16; kk++) {
-      uint32_t w0[8], w1[8];
+      uint2 w0[8], w1[8];
 #pragma unroll
       for (int t = 0; t < 8; t++) {
         const int j = kk * 16 + 8 * half + t;
@@ -247,15 +265,13 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
         float f0 = s0[j], f1 = s1[j];
         f0 = sub0 ? f0 - r0 : f0;
         f1 = sub1 ? f1 - r0 : f1;
-        w0[t] = in ? split_word(f0) : 0u;
-        w1[t] = in && use1 ? split_word(f1) : 0u;
+        w0[t] = in ? split3(f0) : make_uint2(0u, 0u);
+        w1[t] = in && use1 ? split3(f1) : make_uint2(0u, 0u);
       }
-      bf16x8 h0, l0, h1, l1;
-      unpack8(w0, h0, l0);
-      unpack8(w1, h1, l1);
-      g00 = mfma3(h0, l0, h0, l0, g00);
-      g10 = mfma3(h1, l1, h0, l0, g10);
-      g11 = mfma3(h1, l1, h1, l1, g11);
+      const Split8 x0 = unpack8(w0), x1 = unpack8(w1);
+      g00 = mfma6(x0, x0, g00);
+      g10 = mfma6(x1, x0, g10);
+      g11 = mfma6(x1, x1, g11);
     }
     __syncthreads();  // the stage is rewritten by the next pass
   }

@@ -100,13 +100,21 @@ uint16_t bf16Bits(float x) {
 }
 
 // The mixed form's 2-D basis as MFMA B operands (k_cmv_mixed.hip): for MFMA
-// step ks, basis tile nt, part (BF16 hi, then lo) and lane l, 8 taps
+// step ks, basis tile nt, part (BF16 h, m, l) and lane l, 8 taps
 // k = 16 ks + 8 (l / 32) + t of basis n = 32 nt + l % 32, tap k = (u, v) =
 // (k / 24, k % 24); value fy_n[20-u] * fx_n[20-v] for u, v < 21 and n < 49,
-// else 0.  x = hi + lo with hi = bf16(x), lo = bf16(x - hi).
+// else 0, rounded to FP32 and split x = h + m + l (each part BF16 of what the
+// previous parts leave).
+float bf16Float(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
 void mixedTable(const Basis& B, std::vector<uint16_t>& out) {
-  constexpr int KSTEPS = 32, TAPW = 24;
-  out.assign((size_t)KSTEPS * 2 * 2 * 64 * 8, 0);
+  constexpr int KSTEPS = 32, TAPW = 24, PARTS = 3;
+  out.assign((size_t)KSTEPS * 2 * PARTS * 64 * 8, 0);
   for (int ks = 0; ks < KSTEPS; ks++)
     for (int nt = 0; nt < 2; nt++)
       for (int l = 0; l < 64; l++)
@@ -116,14 +124,14 @@ void mixedTable(const Basis& B, std::vector<uint16_t>& out) {
           double x = 0.0;
           if (u < KW && v < KW && n < NPSF) x = B.fy[n * KW + 20 - u] * B.fx[n * KW + 20 - v];
           const float xf = (float)x;
-          const uint16_t hi = bf16Bits(xf);
-          uint32_t hb = (uint32_t)hi << 16;
-          float hf;
-          std::memcpy(&hf, &hb, 4);
-          const uint16_t lo = bf16Bits(xf - hf);
-          const size_t at = ((((size_t)ks * 2 + nt) * 2) * 64 + l) * 8 + t;
-          out[at] = hi;
-          out[at + 64 * 8] = lo;
+          const uint16_t h = bf16Bits(xf);
+          const float r1 = xf - bf16Float(h);
+          const uint16_t m = bf16Bits(r1);
+          const uint16_t lo = bf16Bits(r1 - bf16Float(m));
+          const size_t at = ((((size_t)ks * 2 + nt) * PARTS) * 64 + l) * 8 + t;
+          out[at] = h;
+          out[at + 64 * 8] = m;
+          out[at + 2 * 64 * 8] = lo;
         }
 }
 
