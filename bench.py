@@ -159,8 +159,8 @@ def cmv_mfma_rate(ctx, params, w, h, d_t, d_s, reps=10) -> dict:
     """The mixed form's cmv stage (both images' W, Q and B on the matrix cores)
     timed on its own (host clock around the stage, both streams drained): algorithmic flops per the separable
     method (basis 2*(1581+961)*21*49, Q 2*(50*51/2)*961, B 2*50*961 per stamp)
-    and the MFMA flops actually issued (three BF16 products of the 504-tap
-    unrolled convolution and the 64x64 Gram tiles)."""
+    and the MFMA flops actually issued (six BF16 part products of the 512-tap
+    unrolled convolution and of the 64x64 Gram tiles)."""
     ctx.set_form(xbach.FORM_MIXED)
     ms = []
     for _ in range(reps):
@@ -176,8 +176,8 @@ def cmv_mfma_rate(ctx, params, w, h, d_t, d_s, reps=10) -> dict:
     n = int(tr.stampCount[0]) + int(tr.stampCount[1])
     algo = n * (2.0 * (1581 + 961) * 21 * 49 + 2.0 * (50 * 51 / 2) * 961 + 2.0 * 50 * 961)
     # per stamp: 32 pixel tiles x 2 basis tiles x 32 steps (W) and 64 steps x
-    # 3 tiles (G'), three 32x32x16 products each
-    issued = n * 3 * (32 * 2 * 32 + 64 * 3) * (32 * 32 * 16 * 2)
+    # 3 tiles (G'), six 32x32x16 part products each
+    issued = n * 6 * (32 * 2 * 32 + 64 * 3) * (32 * 32 * 16 * 2)
     t = float(np.median(ms)) * 1e-3
     return {"stamps": n, "ms": t * 1e3, "algorithmic_TFLOPs": algo / t / 1e12,
             "issued_TFLOPs": issued / t / 1e12,

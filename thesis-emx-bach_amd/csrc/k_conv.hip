@@ -1,10 +1,10 @@
 // k_conv.hip — spatially varying convolution and the difference image.
 //
 // Tile kernels (src/bach.cpp:336-355 realises them on the host, 9,604 calls at
-// 2048^2): one workgroup per 21x21 tile forms the kernel at the tile centre in
-// FP64 with the reference's sums (src/bachUtil.cpp:502-541) and stores it as
-// FP32 for the convolution plus FP64 for the mask test, with sum |k| in the
-// convolution loop's order (cl_kern/conv.cl:409-426).
+// 2048^2): each workgroup forms the kernels at 16 tile centres in FP64 with the
+// reference's sums (src/bachUtil.cpp:502-541), stores them (FP64, read by the
+// convolution and the mask test), and sum |k| in the convolution loop's order
+// (cl_kern/conv.cl:409-426).
 //
 // Convolution (cl_kern/conv.cl:31-97): one wave per tile, lane = (column,
 // 7-row strip); the tile kernel is wave-uniform (scalar loads), image rows
@@ -55,45 +55,64 @@ __device__ __forceinline__ double background_c(const double* sol, int x, int y, 
   return bg;
 }
 
-// tile kernels; blockIdx.x = tile, or the extra block = kernel sum at the centre
+// Tile kernels: one workgroup per TK_TILES consecutive tiles (the last tile
+// index, nTiles, is the image centre whose kernel sum the sub stage needs).
+// Each basis value vec[p][k] is read once per workgroup and used for all its
+// tiles; every kernel element is still the sum over p = 0..48 in order
+// (unfused multiply, then add), as src/bachUtil.cpp:502-541.
+constexpr int TK_TILES = 16;
+
 __global__ __launch_bounds__(512) void k_tile_kernels(const Basis* __restrict__ basis,
                                                       const double* __restrict__ sol, int w,
                                                       int h, int xSteps, int nTiles,
                                                       double* tileK64, double* tileAks,
                                                       double* kernSumOut) {
-  const int t = blockIdx.x;
+  const int t0 = blockIdx.x * TK_TILES;
+  const int nt = min(TK_TILES, nTiles + 1 - t0);
   const int tid = threadIdx.x;
-  int x, y;
-  if (t < nTiles) {
-    x = (t % xSteps) * KW + HKW + HKW;
-    y = (t / xSteps) * KW + HKW + HKW;
-  } else {
-    x = w / 2;
-    y = h / 2;
-  }
-  __shared__ double co[NPSF];
-  __shared__ double kern[KK];
+  __shared__ double co[TK_TILES][NPSF];
+  __shared__ double kern[TK_TILES][KK];
   const double hw = 0.5 * w, hh = 0.5 * h;
-  const double xf = (x - hw) / hw, yf = (y - hh) / hh;
-  if (tid < NPSF) co[tid] = kernel_coeff_c(sol, tid, xf, yf);
+  for (int e = tid; e < nt * NPSF; e += blockDim.x) {
+    const int q = e / NPSF, n = e % NPSF, t = t0 + q;
+    int x, y;
+    if (t < nTiles) {
+      x = (t % xSteps) * KW + HKW + HKW;
+      y = (t / xSteps) * KW + HKW + HKW;
+    } else {
+      x = w / 2;
+      y = h / 2;
+    }
+    co[q][n] = kernel_coeff_c(sol, n, (x - hw) / hw, (y - hh) / hh);
+  }
   __syncthreads();
   if (tid < KK) {
-    double k0 = 0.0;
-    // 7 terms' loads in flight together; the sum stays in term order
-#pragma unroll 7
-    for (int p = 0; p < NPSF; p++) k0 += co[p] * basis->vec[p * KK + tid];
-    kern[tid] = k0;
-    if (t < nTiles) tileK64[(size_t)t * KK + tid] = k0;
+    double acc[TK_TILES];
+#pragma unroll
+    for (int q = 0; q < TK_TILES; q++) acc[q] = 0.0;
+    for (int p = 0; p < NPSF; p++) {
+      const double v = basis->vec[p * KK + tid];
+#pragma unroll
+      for (int q = 0; q < TK_TILES; q++) acc[q] += co[q][p] * v;
+    }
+#pragma unroll
+    for (int q = 0; q < TK_TILES; q++) {
+      if (q < nt) {
+        kern[q][tid] = acc[q];
+        if (t0 + q < nTiles) tileK64[(size_t)(t0 + q) * KK + tid] = acc[q];
+      }
+    }
   }
   __syncthreads();
-  if (tid == 0) {
+  if (tid < nt) {  // one thread per tile: the sums in the reference's orders
+    const int t = t0 + tid;
     if (t < nTiles) {
       double aks = 0.0;
-      for (int i = KK - 1; i >= 0; i--) aks += fabs(kern[i]);
+      for (int i = KK - 1; i >= 0; i--) aks += fabs(kern[tid][i]);
       tileAks[t] = aks;
     } else {
       double sum = 0.0;
-      for (int i = 0; i < KK; i++) sum += kern[i];
+      for (int i = 0; i < KK; i++) sum += kern[tid][i];
       *kernSumOut = sum;
     }
   }
@@ -258,8 +277,9 @@ hipError_t launch_conv(Ctx& c) {
   const int nTiles = xSteps * ySteps;
   const int ti = c.tIdx;
   double* kernSum = &c.sc->kernSum;
-  hipLaunchKernelGGL(k_tile_kernels, dim3(nTiles + 1), dim3(512), 0, c.cur, c.basis, c.sol,
-                     c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks, kernSum);
+  hipLaunchKernelGGL(k_tile_kernels, dim3((nTiles + 1 + TK_TILES - 1) / TK_TILES), dim3(512), 0,
+                     c.cur, c.basis, c.sol, c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks,
+                     kernSum);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   ConvArgs A;
