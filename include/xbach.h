@@ -164,6 +164,28 @@ int xb_get_trace(xb_ctx* ctx, xb_trace* out);
  * has run out of substamps).  Any output may be NULL. */
 int xb_get_stamp_fit(xb_ctx* ctx, int im, int k, double* W, double* G, double* Bc, double* Br,
                      int* ssxy);
+/* Quality figures of the last result, the ones HOTPANTS writes to the
+ * difference image's header (tests/test01_sub.fits: KSUM00, SSSIG00,
+ * SSSCAT00, FSIG00, FSCAT00, X2NRM00, NX2NRM00, DMEAN00, DSIGE00, DSIG00 and
+ * the *O00 "OK pixel" variants).  Good pixels: interior (10 px border) and not
+ * BAD_OUTPUT; OK pixels: good and OK_CONV.  noise = sqrt(|conv| + |S|), the
+ * per-pixel noise of the reference's chi2 (cl_kern/cd.cl:578).  The stamp
+ * figures of merit are the clipped mean and sd of the stamps' chi2 in the first
+ * and in the last pass of the clip-refit loop.  A standard deviation over fewer
+ * than two pixels reads 1e10, as HOTPANTS writes it. */
+typedef struct xb_diag {
+  double kernSum;                 /* KSUM00                                  */
+  double sssig, ssscat;           /* SSSIG00, SSSCAT00 (first pass)          */
+  double fsig, fscat;             /* FSIG00, FSCAT00 (final pass)            */
+  double x2nrm;                   /* X2NRM00: mean (d / noise)^2, good px    */
+  int64_t nx2nrm;                 /* NX2NRM00: good pixels                   */
+  double dmean, dsige, dsig;      /* DMEAN00, DSIGE00, DSIG00 (good pixels)  */
+  int64_t nok;                    /* OK pixels                               */
+  double dmeano, dsigeo, dsigo;   /* DMEANO00, DSIGEO00, DSIGO00             */
+  int32_t convTemplate;           /* CONVOL00: 1 template, 0 science         */
+  int32_t normalizeTemplate;      /* PHOTNORM: 1 template, 0 science         */
+} xb_diag;
+int xb_get_diagnostics(xb_ctx* ctx, xb_diag* out);
 /* Device pointers of the results (valid until the next xb_init). */
 int xb_device_outputs(xb_ctx* ctx, const void** d_conv, const void** d_diff);
 
@@ -228,6 +250,10 @@ int xb_synth_pair(const xb_synth_cfg* cfg, float* tmpl, float* sci);
 int xb_fits_read(const char* path, int* w, int* h, float* out, size_t cap, char* msg,
                  size_t msglen);
 int xb_fits_write(const char* path, int w, int h, const float* data, char* msg, size_t msglen);
+/* As xb_fits_write with ncards extra 80-character header cards (cards holds
+ * ncards * 80 characters, no terminators needed) before END. */
+int xb_fits_write_cards(const char* path, int w, int h, const float* data, const char* cards,
+                        int ncards, char* msg, size_t msglen);
 
 /* Waits until the work queued on the context has finished.  The reference's
  * stage functions return only when their device work is done (blocking reads

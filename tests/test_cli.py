@@ -8,7 +8,11 @@ and the CLI's argument errors (src/argsUtil.cpp:23-81).
 GPU: the CLI on test 1 end to end, as tools/run_test.py drives the reference:
 outputs named <op><o> and <op>sub.fits, nine -vt timing lines in the order
 tools/run_measurements.py parses, images bit-identical to the library's own
-run and within the HOTPANTS-golden tolerances of tools/run_test.py:13.
+run and within the HOTPANTS-golden tolerances of tools/run_test.py:13.  The
+difference image carries HOTPANTS's quality cards: KSUM00 equals the golden
+header's '-0.1658', CONVOL00 names the convolved image as the golden does, and
+NX2NRM00, DMEAN00, DSIGE00, DSIG00 and X2NRM00 equal the same figures computed
+here with numpy from the output images and mask (to the 4 written decimals).
 """
 import gzip
 import pathlib
@@ -127,3 +131,28 @@ def test_cli_test01_end_to_end(tmp_path, test01_pair):
     s = xbach.diff_fits_stats(gsub, diff)
     assert c["max_abs"] < 2e-4 and c["max_rel"] < 5e-3 and c["wrong_nans"] == 0, c
     assert s["max_abs"] < 5e-4 and s["max_rel"] < 4e-3 and s["wrong_nans"] == 0, s
+    # HOTPANTS quality cards
+    hdr, _ = xbach.read_fits(out / "sub.fits")
+    ghdr, _ = xbach.read_fits(GOLDEN / "test01_sub_hotpants.fits.gz")
+    assert hdr["KSUM00"] == ghdr["KSUM00"] == "-0.1658"
+    assert hdr["CONVOL00"] == ghdr["CONVOL00"] == "IMAGE"
+    assert hdr["PHOTNORM"] == ghdr["PHOTNORM"] == "t"
+    ctx = xbach.Context(0)
+    try:
+        ctx.run(xbach.Params.default(), tmpl, sci)
+        mask = ctx.mask()
+        diag = ctx.diagnostics()
+    finally:
+        ctx.close()
+    h, w = diff.shape
+    yy, xx = np.mgrid[0:h, 0:w]
+    good = (xx >= 10) & (xx < w - 10) & (yy >= 10) & (yy < h - 10) & ((mask & (1 << 12)) == 0)
+    d = diff[good].astype(np.float64)
+    noise = np.sqrt(np.abs(conv[good].astype(np.float64)) + np.abs(tmpl[good].astype(np.float64)))
+    assert int(hdr["NX2NRM00"]) == diag["nx2nrm"] == int(good.sum())
+    for key, val in (("DMEAN00", d.mean()), ("DSIGE00", d.std(ddof=1)), ("DSIG00", noise.mean()),
+                     ("X2NRM00", np.mean((d / noise) ** 2))):
+        assert abs(float(hdr[key]) - val) <= 1e-4 * max(1.0, abs(val)) + 5e-5, (key, hdr[key], val)
+    print("quality cards:", {k: hdr[k] for k in ("KSUM00", "SSSIG00", "FSIG00", "X2NRM00",
+                                                 "NX2NRM00", "DMEAN00", "DSIGE00", "DSIG00")},
+          "HOTPANTS:", {k: ghdr[k] for k in ("SSSIG00", "FSIG00", "X2NRM00", "NX2NRM00")})

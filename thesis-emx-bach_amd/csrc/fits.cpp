@@ -147,11 +147,17 @@ int xb_fits_read(const char* path, int* w, int* h, float* out, size_t cap, char*
 }
 
 int xb_fits_write(const char* path, int w, int h, const float* data, char* msg, size_t msglen) {
+  return xb_fits_write_cards(path, w, h, data, nullptr, 0, msg, msglen);
+}
+
+int xb_fits_write_cards(const char* path, int w, int h, const float* data, const char* cards,
+                        int ncards, char* msg, size_t msglen) {
   auto fail = [&](int rc, const std::string& m) {
     if (msg && msglen) std::snprintf(msg, msglen, "%s", m.c_str());
     return rc;
   };
-  if (!path || !data || w <= 0 || h <= 0) return fail(XB_ERR_ARG, "bad arguments");
+  if (!path || !data || w <= 0 || h <= 0 || ncards < 0 || (ncards > 0 && !cards))
+    return fail(XB_ERR_ARG, "bad arguments");
   std::string hdr;
   char buf[96];
   put_card(hdr, "SIMPLE  =                    T / file does conform to FITS standard");
@@ -162,6 +168,7 @@ int xb_fits_write(const char* path, int w, int h, const float* data, char* msg, 
   std::snprintf(buf, sizeof(buf), "NAXIS2  = %20d / length of data axis 2", h);
   put_card(hdr, buf);
   put_card(hdr, "EXTEND  =                    T / FITS dataset may contain extensions");
+  for (int i = 0; i < ncards; i++) hdr += std::string(cards + (size_t)i * kCard, kCard);
   put_card(hdr, "END");
   hdr.resize((hdr.size() + kBlock - 1) / kBlock * kBlock, ' ');
   FILE* f = std::fopen(path, "wb");  // overwrites, as "!" does for cfitsio

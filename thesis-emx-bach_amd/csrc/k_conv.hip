@@ -272,6 +272,65 @@ __global__ __launch_bounds__(256) void k_sub(const float* __restrict__ S,
   }
 }
 
+// Diagnostics of the difference image (the quality figures HOTPANTS writes to
+// its header, tests/test01_sub.fits): over good pixels (interior, no
+// BAD_OUTPUT) and over OK pixels (good and OK_CONV), the count, sum and sum of
+// squares of d, the sum of the noise sqrt(|conv| + |S|) — the per-pixel noise
+// of the reference's chi2 (cl_kern/cd.cl:578) — and the sum of (d / noise)^2.
+// Block partials in FP64, added in block order by k_diag_final.
+__global__ __launch_bounds__(256) void k_diag(const float* __restrict__ D,
+                                              const double* __restrict__ conv,
+                                              const float* __restrict__ S,
+                                              const uint16_t* __restrict__ mask, int w, int h,
+                                              double* __restrict__ part) {
+  double q[10];
+#pragma unroll
+  for (int k = 0; k < 10; k++) q[k] = 0.0;
+  const size_t n = (size_t)w * h;
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(id % w), y = (int)(id / w);
+    if (x < HKW || x >= w - HKW || y < HKW || y >= h - HKW) continue;
+    const uint16_t m = mask[id];
+    if (m & XB_MASK_BAD_OUTPUT) continue;
+    const double d = (double)D[id];
+    const double nz = sqrt(fabs(conv[id]) + fabs((double)S[id]));
+    const double r = nz > 0.0 ? d / nz : 0.0;
+    q[0] += 1.0; q[1] += d; q[2] += d * d; q[3] += nz; q[4] += r * r;
+    if (m & XB_MASK_OK_CONV) {
+      q[5] += 1.0; q[6] += d; q[7] += d * d; q[8] += nz; q[9] += r * r;
+    }
+  }
+  __shared__ double red[10][256];
+#pragma unroll
+  for (int k = 0; k < 10; k++) red[k][threadIdx.x] = q[k];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+#pragma unroll
+      for (int k = 0; k < 10; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 10) part[(size_t)blockIdx.x * 10 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_diag_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= 10) return;
+  double a = 0.0;
+  for (int b = 0; b < nb; b++) a += part[(size_t)b * 10 + k];
+  out[k] = a;
+}
+
+hipError_t launch_diag(Ctx& c, double* d_part, double* d_out) {
+  hipLaunchKernelGGL(k_diag, dim3(DIAG_BLOCKS), dim3(256), 0, c.stream, c.diff, c.convD,
+                     c.img[1 - c.tIdx], c.mask, c.w, c.h, d_part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, c.stream, d_part, DIAG_BLOCKS, d_out);
+  return hipGetLastError();
+}
+
 hipError_t launch_conv(Ctx& c) {
   const int xSteps = (c.w + KW - 1) / KW, ySteps = (c.h + KW - 1) / KW;
   const int nTiles = xSteps * ySteps;

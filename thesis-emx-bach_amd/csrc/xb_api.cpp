@@ -504,6 +504,9 @@ int stageKsc(Ctx& c) {
       XB_HIP(hipMemcpyAsync(c.scr.passCur + (size_t)it * c.nKept[im], c.st[im].cur,
                             sizeof(int) * c.nKept[im], hipMemcpyDeviceToDevice, c.cur));
     XB_HIP(launch_ksc_check(c, im));
+    if (it == 0)  // the first pass's clip statistics (the diagnostics report them)
+      XB_HIP(hipMemcpyAsync(c.sc->firstClip, &c.sc->clipMean, 2 * sizeof(double),
+                            hipMemcpyDeviceToDevice, c.cur));
     if (rec) XB_HIP(launch_screen(c, im, it, screen));
     XB_HIP(launch_fill(c, im, c.st[im].list, c.nKept[im]));
     it++;
@@ -969,6 +972,48 @@ int xb_get_stamp_fit(xb_ctx* x, int im, int k, double* W, double* G, double* Bc,
                        hipMemcpyDeviceToHost));
     }
   }
+  return XB_OK;
+}
+
+int xb_get_diagnostics(xb_ctx* x, xb_diag* out) {
+  if (!x || !out) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  if (!c.haveResult) {
+    c.err = "no result yet";
+    return XB_ERR_ARG;
+  }
+  const int rc = setDev(c);
+  if (rc) return rc;
+  double* d = nullptr;
+  XB_HIP(hipMalloc((void**)&d, sizeof(double) * (DIAG_BLOCKS * 10 + 10)));
+  hipError_t e = launch_diag(c, d, d + DIAG_BLOCKS * 10);
+  double q[10] = {0};
+  DevScalars sc;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(q, d + DIAG_BLOCKS * 10, sizeof(q), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&sc, c.sc, sizeof(sc), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  (void)hipFree(d);
+  XB_HIP(e);
+  auto fill = [](const double* s, int64_t& n, double& mean, double& sige, double& sig,
+                 double* x2) {
+    n = (int64_t)s[0];
+    mean = n > 0 ? s[1] / s[0] : 0.0;
+    sige = n > 1 ? std::sqrt(std::max(0.0, (s[2] - s[1] * s[1] / s[0]) / (s[0] - 1.0))) : 1e10;
+    sig = n > 0 ? s[3] / s[0] : 0.0;
+    if (x2) *x2 = n > 0 ? s[4] / s[0] : 0.0;
+  };
+  std::memset(out, 0, sizeof(*out));
+  out->kernSum = sc.kernSum;
+  out->convTemplate = c.convTemplate;
+  out->normalizeTemplate = c.m.normalizeTemplate;
+  out->sssig = sc.firstClip[0];
+  out->ssscat = sc.firstClip[1];
+  out->fsig = sc.clipMean;
+  out->fscat = sc.clipStd;
+  fill(q, out->nx2nrm, out->dmean, out->dsige, out->dsig, &out->x2nrm);
+  fill(q + 5, out->nok, out->dmeano, out->dsigeo, out->dsigo, nullptr);
   return XB_OK;
 }
 

@@ -91,6 +91,7 @@ struct DevScalars {
   double merit[2];
   double kernelSumCd[2];
   double clipMean, clipStd;
+  double firstClip[2];  // clip mean and sd of the first ksc pass
   double kernSum;       // kernel sum at the image centre (conv)
   int nChi2;
   int nScreened;        // substamps the learned screen replaced
@@ -207,6 +208,10 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol);
 hipError_t launch_ksc_check(Ctx& c, int im);
 hipError_t launch_conv(Ctx& c);
 hipError_t launch_sub(Ctx& c);
+// difference-image diagnostics: out[10] = good pixels (n, sum d, sum d^2,
+// sum noise, sum (d/noise)^2), then the same over OK_CONV pixels
+hipError_t launch_diag(Ctx& c, double* d_part, double* d_out);
+constexpr int DIAG_BLOCKS = 512;
 // learned screen (k_screen.hip): features of a pass's substamps, and with
 // apply, the network's decision merged into the pass's refill list
 hipError_t launch_screen(Ctx& c, int im, int pass, bool apply);

@@ -10,6 +10,8 @@
 // reference does (src/main.cpp:35, include/datatypeUtil.h:160-169).  With
 // -vt the nine "<Stage> took N ms" lines are printed in the reference's order
 // (Ini SSS CMV CD KSC Conv Sub Fin BACH), each stage timed to its completion.
+// The difference image carries HOTPANTS's quality cards (KSUM00, X2NRM00 ...;
+// tests/test01_sub.fits) computed by xb_get_diagnostics.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -92,6 +94,57 @@ void getArguments(int argc, const char* argv[], Arguments& a) {
 using Clock = std::chrono::steady_clock;
 long long ms(Clock::time_point b, Clock::time_point a) {
   return std::chrono::duration_cast<std::chrono::milliseconds>(b - a).count();
+}
+
+// A string-valued header card laid out as HOTPANTS writes them
+// (tests/test01_sub.fits): KEY     = 'value   '           / comment
+std::string card(const char* key, const std::string& value, const char* comment) {
+  std::string v = value;
+  if (v.size() < 8) v.resize(8, ' ');
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "%-8s= %-20s / %s", key, ("'" + v + "'").c_str(), comment);
+  std::string c = buf;
+  c.resize(80, ' ');
+  return c;
+}
+
+std::string num(double v) {
+  char b[64];
+  std::snprintf(b, sizeof(b), "%.4f", v);
+  return b;
+}
+
+// HOTPANTS's quality cards for the difference image (xb_get_diagnostics).
+std::string diagCards(const xb_diag& d, int w, int h) {
+  std::string c;
+  c += card("SOFTNAME", "X-BACH MI355X", "The software that differenced this image");
+  c += card("REGION00", "[1:" + std::to_string(w) + ",1:" + std::to_string(h) + "]",
+            "Region of the image");
+  c += card("CONVOL00", d.convTemplate ? "TEMPLATE" : "IMAGE", "Image that was convolved");
+  c += card("KSUM00", num(d.kernSum), "Kernel Sum");
+  c += card("SSSIG00", num(d.sssig), "Average Figure of Merit across Stamps");
+  c += card("SSSCAT00", num(d.ssscat), "Stdev in Figure of Merit");
+  c += card("FSIG00", num(d.fsig), "Final SSSIG");
+  c += card("FSCAT00", num(d.fscat), "Final SSSCAT");
+  c += card("X2NRM00", num(d.x2nrm), "1/N * SUM (diff/noise)^2");
+  c += card("NX2NRM00", std::to_string(d.nx2nrm), "Number of pixels in X2NRM");
+  c += card("DMEAN00", num(d.dmean), "Mean of diff image; good pixels");
+  c += card("DSIGE00", num(d.dsige), "Stdev of diff image; good pixels");
+  c += card("DSIG00", num(d.dsig), "Mean of noise image; good pixels");
+  c += card("DMEANO00", num(d.dmeano), "Mean of diff image; OK pixels");
+  c += card("DSIGEO00", num(d.dsigeo), "Stdev of diff image; OK pixels");
+  c += card("DSIGO00", num(d.dsigo), "Mean of noise image; OK pixels");
+  c += card("PHOTNORM", d.normalizeTemplate ? "t" : "i", "Direction of photometric normalization");
+  char b[96];
+  std::snprintf(b, sizeof(b), "%-8s= %20d / %s", "NREGION", 1, "Number of independent regions");
+  std::string r = b;
+  r.resize(80, ' ');
+  c += r;
+  std::snprintf(b, sizeof(b), "%-8s= %20s / %s", "MASKVAL", "1.0E-30", "Value of Masked Pixels");
+  r = b;
+  r.resize(80, ' ');
+  c += r;
+  return c;
 }
 
 int fail(xb_ctx* ctx, int rc, const char* what) {
@@ -217,8 +270,12 @@ int main(int argc, const char* argv[]) {
   if ((rc = xb_get_conv(ctx, conv.data())) != XB_OK || (rc = xb_get_diff(ctx, diff.data())))
     return fail(ctx, rc, "fin");
   const std::string convOut = args.outPath + args.outName, diffOut = args.outPath + "sub.fits";
+  xb_diag diag;
+  if ((rc = xb_get_diagnostics(ctx, &diag)) != XB_OK) return fail(ctx, rc, "fin");
+  const std::string cards = diagCards(diag, w, h);
   if (xb_fits_write(convOut.c_str(), w, h, conv.data(), msg, sizeof(msg)) != XB_OK ||
-      xb_fits_write(diffOut.c_str(), w, h, diff.data(), msg, sizeof(msg)) != XB_OK) {
+      xb_fits_write_cards(diffOut.c_str(), w, h, diff.data(), cards.data(),
+                          (int)(cards.size() / 80), msg, sizeof(msg)) != XB_OK) {
     std::cout << msg << std::endl;
     xb_ctx_destroy(ctx);
     return 1;

@@ -144,6 +144,19 @@ class Trace(C.Structure):
         return d
 
 
+class Diag(C.Structure):
+    """xb_diag: HOTPANTS-style quality figures of a result."""
+    _fields_ = [("kernSum", C.c_double), ("sssig", C.c_double), ("ssscat", C.c_double),
+                ("fsig", C.c_double), ("fscat", C.c_double), ("x2nrm", C.c_double),
+                ("nx2nrm", C.c_int64), ("dmean", C.c_double), ("dsige", C.c_double),
+                ("dsig", C.c_double), ("nok", C.c_int64), ("dmeano", C.c_double),
+                ("dsigeo", C.c_double), ("dsigo", C.c_double), ("convTemplate", C.c_int32),
+                ("normalizeTemplate", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SynthCfg(C.Structure):
     _fields_ = [
         ("w", C.c_int32), ("h", C.c_int32), ("nstars", C.c_int32),
@@ -315,6 +328,7 @@ def lib() -> C.CDLL:
     L.xb_fits_write.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, C.c_char_p, C.c_size_t]
     L.xb_sync.argtypes = [vp]
     L.xb_set_form.argtypes = [vp, C.c_int]
+    L.xb_get_diagnostics.argtypes = [vp, P(Diag)]
     L.xb_screen_load.argtypes = [vp, vp]
     L.xb_batch_screen_load.argtypes = [vp, vp]
     L.xb_screen_samples.argtypes = [vp, vp, vp, vp, C.c_int, P(C.c_int), P(C.c_int)]
@@ -599,6 +613,11 @@ class Context:
                                          wt.ctypes.data, len(X), epochs, batch, lr, seed,
                                          loss.ctypes.data), "xb_mlp_train")
         return m, loss
+
+    def diagnostics(self) -> dict:
+        d = Diag()
+        self._check(self._L.xb_get_diagnostics(self._h, C.byref(d)), "xb_get_diagnostics")
+        return d.as_dict()
 
     def stamp_fit(self, im: int, k: int) -> dict:
         """W (52x961 raw rows), G (52x52 lower), Bc, Br and the substamp centre
