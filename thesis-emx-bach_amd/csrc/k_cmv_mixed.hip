@@ -1,35 +1,42 @@
-// k_cmv_mixed.hip — the mixed-precision form of cmv on MFMA.
+// k_cmv_mixed.hip — the mixed-precision form of cmv on the matrix cores.
 //
 // The plain form (k_cmv.hip) follows the reference's arithmetic: separable
 // Y/X passes (cl_kern/cmv.cl:70-135) and the normal-matrix sums in FP64
 // (cmv.cl:195-248).  This form computes the same quantities as two dense
-// products on the matrix cores, in BF16 split precision with FP32
-// accumulation: x = h + m + l (three BF16 parts, 24 significand bits — every
-// FP32 value exactly) and a*b ~ h*h + h*m + m*h + h*l + l*h + m*m (the
-// dropped terms are below 2^-24 of the product).  A two-part split (16 bits)
-// is not enough: the normal equations are ill-conditioned, and its 2e-5
-// errors in W and Q grew into chi2 changes that sent the clip-refit loop
-// through many more passes than the plain form's.  The products are:
+// products on the matrix cores:
 //
-//   W  (961 x 49)  = P (961 x 504) . V (504 x 49)
-//       P: the 51x51 window around the substamp, unrolled per pixel (tap (u,v)
-//       of pixel p reads window[p/31 + u][p%31 + v]; taps padded 21 -> 24 per
-//       row so an MFMA's 8 taps lie in one window row); V: the 2-D basis
-//       fy_n[20-u] * fx_n[20-v] (the separable pair of cmv.cl:70-135 as one
-//       21x21 filter), split once on the host.
-//   G' (53 x 53)   = X^T X,  X = [W'_0 .. W'_51 | target]  (961 x 53)
-//       W'_n = W_n - W_0 for the even-even rows (cmv.cl:137-154), rows 49-51
-//       the background monomials (cmv.cl:156-193); G = G'[0:52, 0:52] is the
-//       Gram matrix of createQ and G'[52, :] the target products of createB
-//       (cmv.cl:195-248) and of the cd background right-hand side
-//       (cl_kern/cd.cl:318-344).
+//   W  (961 x 49) = P (961 x 504) . V (504 x 49), in BF16 split precision:
+//       P is the 51x51 window around the substamp unrolled per pixel (tap
+//       (u,v) of pixel p reads window[p/31 + u][p%31 + v]; taps padded
+//       21 -> 24 per row so an MFMA's 8 taps lie in one window row), V the
+//       2-D basis fy_n[20-u] * fx_n[20-v] (the separable pair of
+//       cmv.cl:70-135 as one 21x21 filter), split once on the host.  Every
+//       operand is x = h + m + l, three BF16 parts (24 significand bits: every
+//       FP32 value exactly), and a*b ~ h*h + h*m + m*h + h*l + l*h + m*m (the
+//       dropped terms are below 2^-24 of the product), accumulated in FP32
+//       (v_mfma_f32_32x32x16_bf16).  A two-part split (16 bits) left W 2e-5
+//       from FP64, which moved chi2 enough to send the clip-refit loop through
+//       many more passes than the plain form's.
+//   G' (53 x 53) = X^T X, X = [W'_0 .. W'_51 | target], on the FP64 matrix
+//       cores (v_mfma_f64_16x16x4f64): W'_n = W_n - W_0 for the even-even rows
+//       (cmv.cl:137-154), rows 49-51 the background monomials (cmv.cl:156-193);
+//       G = G'[0:52, 0:52] is the Gram matrix of createQ, G'[52, :] the target
+//       products of createB (cmv.cl:195-248) and of the cd background
+//       right-hand side (cl_kern/cd.cl:318-344).  FP32-accumulated products
+//       are not enough here: near a bright contaminant (a cosmic ray, a star
+//       clipped at saturation) the FP32 rounding of partial sums makes Q more
+//       than a small perturbation of an exact Gram matrix, and the
+//       ill-conditioned fit lost a quarter of its stamps on a crowded field.
+//       The W values are FP32, so their products are exact in FP64.
 //
-// One workgroup (8 waves, two per SIMD) per stamp.  Each wave owns 4 of the
-// 31 pixel tiles (two passes of 2), keeps 2x2 32x32 accumulators, and after
-// each pass stages its 64-pixel slab of W in LDS: the slab is written to HBM
-// in FP64 (the downstream fit reads W as the plain form does) and summed into
-// the wave's partial G' straight from LDS.  The eight partial G' are added in
-// wave order, so the result is deterministic.  Everything after cmv (stamp LUs, global
+// k_fill_mixed: one workgroup (8 waves, two per SIMD) per stamp.  Each wave
+// owns 4 of the 31 pixel tiles (two passes of 2) and keeps 2x2 32x32
+// accumulators; after each pass it stages its 64-pixel slab in LDS and writes
+// it to HBM in FP64 (the downstream fit reads W as in the plain form).
+// k_fill_gram64: one workgroup (16 waves) per stamp stages X in 64-pixel
+// chunks in LDS; wave (J, Q) accumulates the lower tiles (I >= J, 16x16) of
+// column block J over pixel quarter Q, and the quarters are added in order,
+// so the result is deterministic.  Everything after cmv (stamp LUs, global
 // system, solve, chi2, convolution) is the plain FP64 code.
 #include "xb_dev.h"
 
@@ -48,17 +55,14 @@ constexpr int MX_PASSES = 2;           // passes per wave (8 waves x 2 x 2 = 32 
 constexpr int MX_SLAB = MX_TPP * 32;   // pixels per pass
 constexpr int MX_WP = 56;              // window row pitch (words; columns >= 51 are 0)
 constexpr int MX_SP = MX_SLAB + 1;     // stage row pitch (floats)
-constexpr int MX_SROWS = WROWS + 1;    // staged X columns: W rows 0..48, background rows, target
 static_assert(MX_WAVES * MX_PASSES * MX_TPP >= MX_MT, "tiles cover the substamp");
 
 // dynamic LDS carve (bytes)
 constexpr size_t MX_WIN_B = sizeof(uint2) * TW * MX_WP;                      // 22,848
-constexpr size_t MX_STAGE_B = sizeof(float) * MX_WAVES * MX_SROWS * MX_SP;   // 110,240
-constexpr size_t MX_TGT_B = sizeof(float) * 976;                             // 3,904
+constexpr size_t MX_STAGE_B = sizeof(float) * MX_WAVES * NPSF * MX_SP;      // 101,920
 constexpr size_t MX_MONO_B = sizeof(float) * 2 * 32;                         // 256
-constexpr size_t MX_LDS = MX_WIN_B + MX_STAGE_B + MX_TGT_B + MX_MONO_B;
+constexpr size_t MX_LDS = MX_WIN_B + MX_STAGE_B + MX_MONO_B;
 static_assert(MX_LDS <= 160 * 1024, "mixed fill LDS");
-static_assert(MX_WAVES * 3 * 16 * WAVE * sizeof(float) <= MX_STAGE_B, "partial G' fits the stage");
 
 // x = h + m + l, each part BF16 rounded to nearest even (gfx950 converts
 // FP32 to BF16 in one instruction; the two differences are exact in FP32).
@@ -115,21 +119,14 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
   const bool valid = A.cur[s] < A.count[g];
   const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
   double* Ws = A.W + (size_t)s * WROWS * NPIX;
-  if (!valid) {  // as the plain form: zero rows, Gram and targets
+  if (!valid) {  // as the plain form: zero rows
     for (int i = tid; i < WROWS * NPIX; i += MX_THREADS) Ws[i] = 0.0;
-    for (int i = tid; i < WROWS * WROWS; i += MX_THREADS) {
-      const int r = i / WROWS, q = i % WROWS;
-      if (q <= r) A.G[(size_t)s * WROWS * WROWS + i] = 0.0;
-    }
-    if (tid < WROWS) A.Bc[(size_t)s * WROWS + tid] = 0.0;
-    if (tid < NBG) A.Br[(size_t)s * NBG + tid] = 0.0;
     return;
   }
   extern __shared__ __align__(16) unsigned char mx_lds[];
   uint2* win = reinterpret_cast<uint2*>(mx_lds);
   float* stage = reinterpret_cast<float*>(mx_lds + MX_WIN_B);
-  float* tgt = reinterpret_cast<float*>(mx_lds + MX_WIN_B + MX_STAGE_B);
-  float* mono = reinterpret_cast<float*>(mx_lds + MX_WIN_B + MX_STAGE_B + MX_TGT_B);
+  float* mono = reinterpret_cast<float*>(mx_lds + MX_WIN_B + MX_STAGE_B);
 
   const int x0 = ssx - (HSS + HKW), y0 = ssy - (HSS + HKW);
   for (int i = tid; i < TW * MX_WP; i += MX_THREADS) {
@@ -141,25 +138,13 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
     }
     win[i] = split3(v);
   }
-  for (int i = tid; i < 976; i += MX_THREADS) {
-    float v = 0.0f;
-    if (i < NPIX) {
-      const int xr = min(max(ssx - HSS + i % SSW, 0), A.w - 1);
-      const int yr = min(max(ssy - HSS + i / SSW, 0), A.h - 1);
-      v = A.img2[(size_t)xr + (size_t)yr * A.w];
-    }
-    tgt[i] = v;
-  }
   if (tid < 2 * SSW) {  // background monomials (FP32, as cmv.cl:156-193)
     const int b = tid / SSW, i = tid % SSW;
     const float hw = (float)A.w * 0.5f, hh = (float)A.h * 0.5f;
     mono[b * 32 + i] = b == 0 ? ((float)(ssx - HSS + i) - hw) / hw
                               : ((float)(ssy - HSS + i) - hh) / hh;
   }
-  // the even-even flags of this lane's two X columns, loaded before any store
   const int half = lane / 32, col = lane % 32;
-  const bool sub0 = A.basis->evenSub[col] != 0;
-  const bool sub1 = 32 + col < NPSF && A.basis->evenSub[32 + col] != 0;
   __syncthreads();
 
   // k step ks: this lane's 8 taps start at tap row u, column v0; taps past
@@ -174,8 +159,7 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
     m45 = nreal == 8 ? 0xffffffffu : nreal == 5 ? 0x0000ffffu : 0u;
     m67 = nreal == 8 ? 0xffffffffu : 0u;
   };
-  float* st = stage + (size_t)wv * MX_SROWS * MX_SP;
-  f32x16 g00 = {0}, g10 = {0}, g11 = {0};
+  float* st = stage + (size_t)wv * NPSF * MX_SP;
   for (int ps = 0; ps < MX_PASSES; ps++) {
     const int mt0 = MX_PASSES * MX_TPP * wv + MX_TPP * ps;
     // ---- W slab: MX_TPP pixel tiles x 2 basis tiles (tile 31 is past the
@@ -221,7 +205,7 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
         acc[gi][1] = mfma6(a[gi], b1, acc[gi][1]);
       }
     }
-    // ---- stage the slab (rows 0..48) and X columns 49..52
+    // ---- stage the slab (rows 0..48)
 #pragma unroll
     for (int gi = 0; gi < MX_TPP; gi++)
 #pragma unroll
@@ -233,13 +217,6 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
             st[n * MX_SP + gi * 32 + 8 * (v / 4) + 4 * half + (v % 4)] = acc[gi][nt][v];
       }
     const int pBase = mt0 * 32;
-    // X columns 49..52 over the slab: the background monomials 1, yf, xf
-    // (W rows 49..51) and the target
-    for (int i = lane; i < 4 * MX_SLAB; i += WAVE) {
-      const int r = i / MX_SLAB, j = i % MX_SLAB, p = min(pBase + j, NPIX - 1);
-      const float v = r == 0 ? 1.0f : r == 1 ? mono[32 + p / SSW] : r == 2 ? mono[p % SSW] : tgt[p];
-      st[(NPSF + r) * MX_SP + j] = v;
-    }
     __syncthreads();
     // ---- W rows 0..48 of the slab in FP64 (one pixel per lane and row)
     const bool inW = pBase + lane < NPIX;
@@ -248,59 +225,120 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
       const float v = st[n * MX_SP + lane];
       if (inW) Ws[(size_t)n * NPIX + pBase + lane] = (double)v;
     }
-    // ---- partial G' over the slab: X columns col (F0) and 32 + col (F1);
-    // columns past 52 are zero (they read row 52 and drop it)
-    const int a0 = col, a1 = 32 + col;
-    const bool use1 = a1 <= WROWS;
-    const float* s0 = st + a0 * MX_SP;
-    const float* s1 = st + min(a1, WROWS) * MX_SP;
-#pragma unroll
-    for (int kk = 0; kk < MX_SLAB / 16; kk++) {
-      uint2 w0[8], w1[8];
-#pragma unroll
-      for (int t = 0; t < 8; t++) {
-        const int j = kk * 16 + 8 * half + t;
-        const bool in = pBase + j < NPIX;
-        const float r0 = st[j];
-        float f0 = s0[j], f1 = s1[j];
-        f0 = sub0 ? f0 - r0 : f0;
-        f1 = sub1 ? f1 - r0 : f1;
-        w0[t] = in ? split3(f0) : make_uint2(0u, 0u);
-        w1[t] = in && use1 ? split3(f1) : make_uint2(0u, 0u);
-      }
-      const Split8 x0 = unpack8(w0), x1 = unpack8(w1);
-      g00 = mfma6(x0, x0, g00);
-      g10 = mfma6(x1, x0, g10);
-      g11 = mfma6(x1, x1, g11);
-    }
     __syncthreads();  // the stage is rewritten by the next pass
   }
-  // ---- add the partial G' in wave order
-  float* red = stage;  // [wave][tile][v][lane]
-#pragma unroll
-  for (int v = 0; v < 16; v++) {
-    red[((wv * 3 + 0) * 16 + v) * WAVE + lane] = g00[v];
-    red[((wv * 3 + 1) * 16 + v) * WAVE + lane] = g10[v];
-    red[((wv * 3 + 2) * 16 + v) * WAVE + lane] = g11[v];
-  }
-  __syncthreads();
   for (int i = tid; i < NBG * NPIX; i += MX_THREADS) {  // W rows 49..51 (the plain form's values)
     const int b = i / NPIX, p = i % NPIX;
     const double xf = (double)mono[p % SSW], yf = (double)mono[32 + p / SSW];
     Ws[(size_t)(NPSF + b) * NPIX + p] = (b == 2 ? xf : 1.0) * (b == 1 ? yf : 1.0);
   }
+}
+
+// ---------------------------------------------------------------------------
+// G' = X^T X on the FP64 matrix cores
+// ---------------------------------------------------------------------------
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int G64_WAVES = 16;
+constexpr int G64_THREADS = G64_WAVES * WAVE;
+constexpr int G64_CH = 64;                        // pixels per LDS chunk
+constexpr int G64_NCH = (NPIX + G64_CH - 1) / G64_CH;  // 16 chunks (1024 px, zero past 961)
+constexpr int G64_XP = 64 + 1;                    // LDS pitch of a staged pixel (64 columns)
+constexpr int G64_TILES = 10;                     // lower 16x16 tiles of the 64x64 G'
+
+__device__ __forceinline__ int g64_tile(int I, int J) { return I * (I + 1) / 2 + J; }
+
+__global__ __launch_bounds__(G64_THREADS) void k_fill_gram64(FillArgs A) {
+  int s, ssx, ssy;
+  if (!fill_stamp(A, blockIdx.x, &s, &ssx, &ssy)) return;
+  const int g = A.kept[s];
+  const bool valid = A.cur[s] < A.count[g];
+  const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
   double* G = A.G + (size_t)s * WROWS * WROWS;
-  for (int e = tid; e < 3 * 16 * WAVE; e += MX_THREADS) {
-    const int tile = e / (16 * WAVE), v = (e / WAVE) % 16, l = e % WAVE;
-    float sum = red[((0 * 3 + tile) * 16 + v) * WAVE + l];
-    for (int q = 1; q < MX_WAVES; q++) sum += red[((q * 3 + tile) * 16 + v) * WAVE + l];
-    const int row = (tile == 0 ? 0 : 32) + 8 * (v / 4) + 4 * (l / 32) + (v % 4);
-    const int cc = (tile == 2 ? 32 : 0) + l % 32;
+  if (!valid) {  // as the plain form: zero Gram and targets
+    for (int i = tid; i < WROWS * WROWS; i += G64_THREADS) {
+      const int r = i / WROWS, q = i % WROWS;
+      if (q <= r) G[i] = 0.0;
+    }
+    if (tid < WROWS) A.Bc[(size_t)s * WROWS + tid] = 0.0;
+    if (tid < NBG) A.Br[(size_t)s * NBG + tid] = 0.0;
+    return;
+  }
+  const double* Ws = A.W + (size_t)s * WROWS * NPIX;
+  const int* evenSub = A.basis->evenSub;
+  __shared__ double X[G64_CH][G64_XP];
+  __shared__ double red[G64_TILES][4][WAVE];
+  const int J = wv % 4, Q = wv / 4;  // column block, pixel quarter of each chunk
+  f64x4 acc[4];
+#pragma unroll
+  for (int I = 0; I < 4; I++) acc[I] = f64x4{0.0, 0.0, 0.0, 0.0};
+  for (int ch = 0; ch < G64_NCH; ch++) {
+    // stage X for pixels ch*64 .. ch*64+63: column c = W'_c (c < 52), the
+    // target (c = 52), zero (c > 52 or past the last pixel)
+    for (int e = tid; e < 64 * G64_CH; e += G64_THREADS) {
+      const int c = e / G64_CH, pp = e % G64_CH, p = ch * G64_CH + pp;
+      double v = 0.0;
+      if (p < NPIX) {
+        if (c < WROWS) {
+          v = Ws[(size_t)c * NPIX + p];
+          if (c < NPSF && evenSub[c]) v = v - Ws[p];
+        } else if (c == WROWS) {
+          const int xr = min(max(ssx - HSS + p % SSW, 0), A.w - 1);
+          const int yr = min(max(ssy - HSS + p / SSW, 0), A.h - 1);
+          v = (double)A.img2[(size_t)xr + (size_t)yr * A.w];
+        }
+      }
+      X[pp][c] = v;
+    }
+    __syncthreads();
+    // wave (J, Q): pixels 16 Q .. 16 Q + 15 of the chunk, 4 per MFMA;
+    // lane: pixel k = l / 16 of the step, column l % 16 of the block
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int pp = 16 * Q + 4 * kk + lane / 16;
+      const double* xr = &X[pp][lane % 16];
+      const double xb = xr[16 * J];
+      // tiles I >= J (J is wave-uniform); the products of the other tiles are
+      // formed with a zero operand and never read
+      const double x0 = J <= 0 ? xr[0] : 0.0, x1 = J <= 1 ? xr[16] : 0.0;
+      const double x2 = J <= 2 ? xr[32] : 0.0, x3 = xr[48];
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, xb, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, xb, acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(x2, xb, acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(x3, xb, acc[3], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // add the four pixel quarters in order
+  for (int q = 0; q < 4; q++) {
+    if (Q == q) {
+#pragma unroll
+      for (int I = 0; I < 4; I++) {
+        if (I >= J) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            double* r = &red[g64_tile(I, J)][v][lane];
+            *r = q == 0 ? acc[I][v] : *r + acc[I][v];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // tile (I, J), lane l, value v: G'[16 I + l / 16 + 4 v][16 J + l % 16] (the
+  // accumulator layout of v_mfma_f64_16x16x4f64)
+  for (int e = tid; e < G64_TILES * 4 * WAVE; e += G64_THREADS) {
+    const int t = e / (4 * WAVE), v = (e / WAVE) % 4, l = e % WAVE;
+    int I = 0;
+    while (g64_tile(I + 1, 0) <= t) I++;
+    const int Jt = t - g64_tile(I, 0);
+    const int row = 16 * I + l / 16 + 4 * v, cc = 16 * Jt + l % 16;
+    const double val = red[t][v][l];
     if (row < WROWS && cc < WROWS && cc <= row) {
-      G[row * WROWS + cc] = (double)sum;
+      G[row * WROWS + cc] = val;
     } else if (row == WROWS && cc < WROWS) {
-      A.Bc[(size_t)s * WROWS + cc] = (double)sum;
-      if (cc >= NPSF) A.Br[(size_t)s * NBG + cc - NPSF] = (double)sum;
+      A.Bc[(size_t)s * WROWS + cc] = val;
+      if (cc >= NPSF) A.Br[(size_t)s * NBG + cc - NPSF] = val;
     }
   }
 }
@@ -311,6 +349,9 @@ hipError_t launch_fill_mixed(Ctx& c, const FillArgs& A, int nMax) {
   if (attrRc != hipSuccess) return attrRc;
   hipLaunchKernelGGL(k_fill_mixed, dim3(nMax), dim3(MX_THREADS), MX_LDS, c.cur, A,
                      reinterpret_cast<const uint4*>(c.mixTab));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fill_gram64, dim3(nMax), dim3(G64_THREADS), 0, c.cur, A);
   return hipGetLastError();
 }
 

@@ -177,6 +177,14 @@ SYNTH_CONFIGS = {
     "pair2048": dict(w=2048, h=2048, nstars=6000, fluxMin=1e2, fluxMax=1e5, fluxSlope=-1.5,
                      fwhmT=1.8, fwhmS0=2.5, dfwhmX=0.5, dfwhmY=0.5, sky=200.0, skyGx=0.02,
                      skyGy=0.01, sciFluxScale=1.0, seed=2048001),
+    # 4096x4096, 20000 stars, the 2048 pair's PSFs and sky (fit on a 20x20 stamp grid)
+    "pair4096": dict(w=4096, h=4096, nstars=20000, fluxMin=1e2, fluxMax=1e5, fluxSlope=-1.5,
+                     fwhmT=1.8, fwhmS0=2.5, dfwhmX=0.5, dfwhmY=0.5, sky=200.0, skyGx=0.02,
+                     skyGy=0.01, sciFluxScale=1.0, seed=4096001),
+    # 3072x3072 crowded field, 30000 stars, FWHM 2.0 / 3.0, sky 100
+    "crowd3072": dict(w=3072, h=3072, nstars=30000, fluxMin=1e2, fluxMax=1e5, fluxSlope=-1.5,
+                      fwhmT=2.0, fwhmS0=3.0, dfwhmX=0.0, dfwhmY=0.0, sky=100.0, skyGx=0.0,
+                      skyGy=0.0, sciFluxScale=1.0, seed=3072001),
 }
 
 
