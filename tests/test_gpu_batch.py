@@ -114,3 +114,40 @@ def test_batch_argument_errors():
             b.run(xbach.Params.default(fKernelWidth=19), [(t, t)])
         outs, res = b.run(p, [])
         assert outs == [] and res == []
+
+
+def test_batch_autotune(pairs, singles):
+    # the autotuner picks a count per size bucket; runs afterwards go bucket by
+    # bucket and stay bit-identical to lone runs
+    p = xbach.Params.default()
+    use = pairs[:-1]
+    w = [t.shape[1] for t, _ in use]
+    h = [t.shape[0] for t, _ in use]
+    with xbach.Context(0) as c, xbach.Batch(0, workers=4) as b:
+        bufs = []
+        try:
+            for t, s in use:
+                dt, ds, dc, dd = (c.buf_alloc(t.nbytes) for _ in range(4))
+                c.buf_upload(dt, t)
+                c.buf_upload(ds, s)
+                bufs.append((dt, ds, dc, dd))
+            args = ([q[0] for q in bufs], [q[1] for q in bufs], [q[2] for q in bufs],
+                    [q[3] for q in bufs])
+            table = b.autotune(p, w, h, *args, reps=1)
+            present = {0 if max(a, c_) <= 1024 else 1 for a, c_ in zip(w, h)}
+            for k in range(4):
+                assert (table[k] > 0) == (k in present), table
+                assert table[k] <= 4
+            res = b.run_device(p, w, h, *args)
+            for i, (t, _) in enumerate(use):
+                conv = np.empty_like(t)
+                diff = np.empty_like(t)
+                c.buf_download(conv, bufs[i][2])
+                c.buf_download(diff, bufs[i][3])
+                assert res[i].status == 0
+                assert _same(conv, singles[i][0]) and _same(diff, singles[i][1]), i
+                assert res[i].kernSum == singles[i][2]
+        finally:
+            for q in bufs:
+                for d in q:
+                    c.buf_free(d)

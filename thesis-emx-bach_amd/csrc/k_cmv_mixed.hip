@@ -272,25 +272,39 @@ __global__ __launch_bounds__(G64_THREADS) void k_fill_gram64(FillArgs A) {
   f64x4 acc[4];
 #pragma unroll
   for (int I = 0; I < 4; I++) acc[I] = f64x4{0.0, 0.0, 0.0, 0.0};
-  for (int ch = 0; ch < G64_NCH; ch++) {
-    // stage X for pixels ch*64 .. ch*64+63: column c = W'_c (c < 52), the
-    // target (c = 52), zero (c > 52 or past the last pixel)
-    for (int e = tid; e < 64 * G64_CH; e += G64_THREADS) {
-      const int c = e / G64_CH, pp = e % G64_CH, p = ch * G64_CH + pp;
-      double v = 0.0;
-      if (p < NPIX) {
-        if (c < WROWS) {
-          v = Ws[(size_t)c * NPIX + p];
-          if (c < NPSF && evenSub[c]) v = v - Ws[p];
-        } else if (c == WROWS) {
-          const int xr = min(max(ssx - HSS + p % SSW, 0), A.w - 1);
-          const int yr = min(max(ssy - HSS + p / SSW, 0), A.h - 1);
-          v = (double)A.img2[(size_t)xr + (size_t)yr * A.w];
-        }
+  // X for pixels ch*64 .. ch*64+63: column c = W'_c (c < 52), the target
+  // (c = 52), zero (c > 52 or past the last pixel).  Thread t holds the
+  // entries e = t + 1024 r (r < 4) of a chunk; the next chunk's entries are
+  // loaded into registers while the current chunk's products run.
+  constexpr int PER_T = 64 * G64_CH / G64_THREADS;  // 4
+  auto xval = [&](int e, int ch) -> double {
+    const int c = e / G64_CH, pp = e % G64_CH, p = ch * G64_CH + pp;
+    double v = 0.0;
+    if (p < NPIX) {
+      if (c < WROWS) {
+        v = Ws[(size_t)c * NPIX + p];
+        if (c < NPSF && evenSub[c]) v = v - Ws[p];
+      } else if (c == WROWS) {
+        const int xr = min(max(ssx - HSS + p % SSW, 0), A.w - 1);
+        const int yr = min(max(ssy - HSS + p / SSW, 0), A.h - 1);
+        v = (double)A.img2[(size_t)xr + (size_t)yr * A.w];
       }
-      X[pp][c] = v;
+    }
+    return v;
+  };
+  double nx[PER_T];
+#pragma unroll
+  for (int r = 0; r < PER_T; r++) nx[r] = xval(tid + G64_THREADS * r, 0);
+  for (int ch = 0; ch < G64_NCH; ch++) {
+#pragma unroll
+    for (int r = 0; r < PER_T; r++) {
+      const int e = tid + G64_THREADS * r;
+      X[e % G64_CH][e / G64_CH] = nx[r];
     }
     __syncthreads();
+    if (ch + 1 < G64_NCH)
+#pragma unroll
+      for (int r = 0; r < PER_T; r++) nx[r] = xval(tid + G64_THREADS * r, ch + 1);
     // wave (J, Q): pixels 16 Q .. 16 Q + 15 of the chunk, 4 per MFMA;
     // lane: pixel k = l / 16 of the step, column l % 16 of the block
 #pragma unroll

@@ -23,14 +23,22 @@
 
 #include "xb_internal.h"
 
+constexpr int XB_NBUCKET = 4;  // pair sizes: max(w, h) <= 1024, 2048, 3072, larger
+
 struct xb_batch {
   int device = 0;
-  int inflight = 0;  // 0: all workers
+  int inflight = 0;                    // 0: all workers
+  int tuned[XB_NBUCKET] = {0, 0, 0, 0};  // pairs in flight per size bucket (0: untuned)
   std::vector<xb_ctx*> ctx;
   std::string err;
 };
 
 namespace {
+
+int sizeBucket(int w, int h) {
+  const int m = std::max(w, h);
+  return m <= 1024 ? 0 : m <= 2048 ? 1 : m <= 3072 ? 2 : 3;
+}
 
 double nowMs() {
   return std::chrono::duration<double, std::milli>(
@@ -52,6 +60,7 @@ struct Job {
   std::vector<int> order;   // pair indices, largest first
   std::vector<int> status;  // per pair
   std::atomic<int> next{0};
+  int end = 0;              // pairs J.order[next .. end) are handed out
   std::mutex mu;
   int firstBad = -1;        // lowest failing pair index
   std::string firstMsg;
@@ -101,7 +110,7 @@ void worker(xb_batch* b, int wi, Job* J) {
   if (hipSetDevice(b->device) != hipSuccess) return;  // pairs stay XB_ERR_HIP
   for (;;) {
     const int k = J->next.fetch_add(1);
-    if (k >= J->n) break;
+    if (k >= J->end) break;
     const int i = J->order[k];
     xb_pair_result local{};
     xb_pair_result* r = J->res ? &J->res[i] : &local;
@@ -121,6 +130,16 @@ void worker(xb_batch* b, int wi, Job* J) {
       (void)hipStreamSynchronize(x->c.stream);
     }
   }
+}
+
+// Runs the pairs J.order[k0 .. k1) with nw workers.
+void runSpan(xb_batch* b, Job& J, int k0, int k1, int nw) {
+  J.next.store(k0);
+  J.end = k1;
+  std::vector<std::thread> th;
+  th.reserve(nw);
+  for (int wi = 0; wi < nw; wi++) th.emplace_back(worker, b, wi, &J);
+  for (auto& t : th) t.join();
 }
 
 int runBatch(xb_batch* b, Job& J) {
@@ -143,12 +162,24 @@ int runBatch(xb_batch* b, Job& J) {
     return (long long)J.w[a] * J.h[a] > (long long)J.w[c] * J.h[c];
   });
   J.status.assign(J.n, XB_ERR_HIP);
-  const int nw = std::min<int>(J.n, b->inflight > 0 ? std::min<int>(b->inflight, (int)b->ctx.size())
-                                                    : (int)b->ctx.size());
-  std::vector<std::thread> th;
-  th.reserve(nw);
-  for (int wi = 0; wi < nw; wi++) th.emplace_back(worker, b, wi, &J);
-  for (auto& t : th) t.join();
+  const int all = b->inflight > 0 ? std::min<int>(b->inflight, (int)b->ctx.size())
+                                  : (int)b->ctx.size();
+  const bool tuned = std::any_of(b->tuned, b->tuned + XB_NBUCKET, [](int v) { return v > 0; });
+  if (!tuned) {
+    runSpan(b, J, 0, J.n, std::min(J.n, all));
+  } else {
+    // largest bucket first (the order is sorted by size), each with the
+    // number of pairs in flight the autotuner chose for it
+    int k0 = 0;
+    while (k0 < J.n) {
+      const int bk = sizeBucket(J.w[J.order[k0]], J.h[J.order[k0]]);
+      int k1 = k0;
+      while (k1 < J.n && sizeBucket(J.w[J.order[k1]], J.h[J.order[k1]]) == bk) k1++;
+      const int nw = b->tuned[bk] > 0 ? std::min<int>(b->tuned[bk], (int)b->ctx.size()) : all;
+      runSpan(b, J, k0, k1, std::min(k1 - k0, nw));
+      k0 = k1;
+    }
+  }
   if (J.firstBad >= 0) {
     b->err = J.firstMsg;
     return J.status[J.firstBad];
@@ -192,6 +223,60 @@ const char* xb_batch_last_error(xb_batch* b) { return b ? b->err.c_str() : "no b
 int xb_batch_set_inflight(xb_batch* b, int inflight) {
   if (!b || inflight < 0 || inflight > (int)b->ctx.size()) return XB_ERR_ARG;
   b->inflight = inflight;
+  return XB_OK;
+}
+
+int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
+                      const void* const* d_tmpl, const void* const* d_sci, void* const* d_conv,
+                      void* const* d_diff, int reps, int* table) {
+  if (!b || reps < 1) return XB_ERR_ARG;
+  std::fill(b->tuned, b->tuned + XB_NBUCKET, 0);
+  const int workers = (int)b->ctx.size();
+  const int cands[] = {1, 2, 4, 6, 8, 12, 16, 24, 32};
+  for (int bk = 0; bk < XB_NBUCKET; bk++) {
+    std::vector<int> idx;
+    for (int i = 0; i < n; i++)
+      if (sizeBucket(w[i], h[i]) == bk) idx.push_back(i);
+    if (idx.empty()) continue;
+    std::vector<int> bw, bh;
+    std::vector<const void*> bt, bs;
+    std::vector<void*> bc, bd;
+    for (int i : idx) {
+      bw.push_back(w[i]);
+      bh.push_back(h[i]);
+      bt.push_back(d_tmpl[i]);
+      bs.push_back(d_sci[i]);
+      bc.push_back(d_conv ? d_conv[i] : nullptr);
+      bd.push_back(d_diff ? d_diff[i] : nullptr);
+    }
+    double best = 1e300;
+    int bestC = 1;
+    const int saved = b->inflight;
+    for (int c : cands) {
+      if (c > workers) break;
+      b->inflight = c;
+      double t = 1e300;
+      for (int r = 0; r < reps + 1; r++) {  // first run warms the contexts up
+        const double t0 = nowMs();
+        const int rc = xb_batch_run_device(b, p, (int)idx.size(), bw.data(), bh.data(), bt.data(),
+                                           bs.data(), bc.data(), bd.data(), nullptr);
+        if (rc) {
+          b->inflight = saved;
+          return rc;
+        }
+        if (r > 0) t = std::min(t, nowMs() - t0);
+      }
+      if (t < best * 0.97) {  // a larger count must win by 3 %
+        best = t;
+        bestC = c;
+      }
+      if (c >= (int)idx.size()) break;
+    }
+    b->inflight = saved;
+    b->tuned[bk] = bestC;
+  }
+  if (table)
+    for (int k = 0; k < XB_NBUCKET; k++) table[k] = b->tuned[k];
   return XB_OK;
 }
 

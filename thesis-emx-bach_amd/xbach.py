@@ -351,6 +351,7 @@ def lib() -> C.CDLL:
     L.xb_batch_last_error.argtypes = [vp]
     L.xb_batch_last_error.restype = C.c_char_p
     L.xb_batch_set_inflight.argtypes = [vp, C.c_int]
+    L.xb_batch_autotune.argtypes = [vp, P(Params), C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, vp]
     L.xb_batch_run.argtypes = [vp, P(Params), C.c_int, vp, vp, vp, vp, vp, vp, vp]
     L.xb_batch_run_device.argtypes = [vp, P(Params), C.c_int, vp, vp, vp, vp, vp, vp, vp]
     _lib = L
@@ -701,6 +702,20 @@ class Batch:
         m = np.ascontiguousarray(model, np.float32)
         self._check(self._L.xb_batch_screen_load(self._h, m.ctypes.data),
                     "xb_batch_screen_load")
+
+    def autotune(self, params: Params, w, h, d_tmpl, d_sci, d_conv=None, d_diff=None,
+                 reps: int = 2) -> list[int]:
+        """Pairs in flight per size bucket (<=1024, <=2048, <=3072, larger)."""
+        n = len(w)
+        wa = np.ascontiguousarray(w, np.int32)
+        ha = np.ascontiguousarray(h, np.int32)
+        table = np.zeros(4, np.int32)
+        self._check(self._L.xb_batch_autotune(
+            self._h, C.byref(params), n, wa.ctypes.data, ha.ctypes.data, _ptr_array(d_tmpl),
+            _ptr_array(d_sci), _ptr_array(d_conv) if d_conv is not None else None,
+            _ptr_array(d_diff) if d_diff is not None else None, reps, table.ctypes.data),
+            "xb_batch_autotune")
+        return [int(v) for v in table]
 
     def set_inflight(self, n: int):
         self._check(self._L.xb_batch_set_inflight(self._h, n), "xb_batch_set_inflight")
