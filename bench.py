@@ -9,9 +9,10 @@ pairs that are already resident in HBM (outputs stay in HBM):
   equations, the direction test fits, the global fit with its clip-refit loop,
   the spatially varying convolution and the difference image);
 * batched: one step = one batch of --batch distinct pairs run by the batch
-  runtime (--workers pairs in flight, each through the same bit-identical
-  pipeline); pair 0 of the batch is the plain pair, and its outputs are
-  checked to equal the plain run's bit for bit.
+  runtime (up to --workers pairs in flight, the count per size bucket chosen
+  by the batch autotuner before the timed batches, each pair through the same
+  bit-identical pipeline); pair 0 of the batch is the plain pair, and its
+  outputs are checked to equal the plain run's bit for bit.
 
 `value` is the batched form's diffs/s (the plain form's figures are under
 "forms"); both forms produce identical difference images, so the residual
@@ -131,16 +132,21 @@ def upload_pairs(ctx, pairs):
 
 
 def run_batched(ctx, params, pairs, bufs, workers, steps, warmup, form=0, model=None,
-                keep=4):
+                keep=4, tune=True):
     """Time `steps` batches of all pairs (device-resident in and out); returns
-    ms per batch, the per-pair results and the diffs of the first `keep` pairs."""
+    ms per batch, the per-pair results, the diffs of the first `keep` pairs and
+    the autotuner's pairs in flight per size bucket (set up before the timed
+    batches)."""
     ws = [t.shape[1] for t, _ in pairs]
     hs = [t.shape[0] for t, _ in pairs]
     cols = [[q[k] for q in bufs] for k in range(4)]
+    table = None
     with xbach.Batch(0, workers=workers) as b:
         if model is not None:
             b.screen_load(model)
         b.set_form(form)
+        if tune:
+            table = b.autotune(params, ws, hs, *cols, reps=1)
         for _ in range(warmup):
             b.run_device(params, ws, hs, *cols)
         t0 = time.perf_counter()
@@ -152,15 +158,16 @@ def run_batched(ctx, params, pairs, bufs, workers, steps, warmup, form=0, model=
         d = np.empty_like(pairs[i][0])
         ctx.buf_download(d, bufs[i][3])
         diffs.append(d)
-    return (t1 - t0) * 1e3 / steps, res, diffs
+    return (t1 - t0) * 1e3 / steps, res, diffs, table
 
 
 def cmv_mfma_rate(ctx, params, w, h, d_t, d_s, reps=10) -> dict:
     """The mixed form's cmv stage (both images' W, Q and B on the matrix cores)
     timed on its own (host clock around the stage, both streams drained): algorithmic flops per the separable
     method (basis 2*(1581+961)*21*49, Q 2*(50*51/2)*961, B 2*50*961 per stamp)
-    and the MFMA flops actually issued (six BF16 part products of the 512-tap
-    unrolled convolution and of the 64x64 Gram tiles)."""
+    and the matrix-core flops actually issued: six BF16 part products of the
+    512-tap unrolled convolution (W), and the FP64 MFMA products of the 64x64
+    normal matrix over 16 chunks of 64 pixels (G')."""
     ctx.set_form(xbach.FORM_MIXED)
     ms = []
     for _ in range(reps):
@@ -175,14 +182,19 @@ def cmv_mfma_rate(ctx, params, w, h, d_t, d_s, reps=10) -> dict:
     ctx.set_form(xbach.FORM_PLAIN)
     n = int(tr.stampCount[0]) + int(tr.stampCount[1])
     algo = n * (2.0 * (1581 + 961) * 21 * 49 + 2.0 * (50 * 51 / 2) * 961 + 2.0 * 50 * 961)
-    # per stamp: 32 pixel tiles x 2 basis tiles x 32 steps (W) and 64 steps x
-    # 3 tiles (G'), six 32x32x16 part products each
-    issued = n * 6 * (32 * 2 * 32 + 64 * 3) * (32 * 32 * 16 * 2)
+    # per stamp: W = 32 pixel tiles x 2 basis tiles x 32 steps of six
+    # 32x32x16 BF16 part products; G' = 16 chunks x 16 waves x 4 steps x 4
+    # tiles of 16x16x4 FP64 products
+    issued_bf16 = n * 6 * (32 * 2 * 32) * (32 * 32 * 16 * 2)
+    issued_fp64 = n * (16 * 16 * 4 * 4) * (16 * 16 * 4 * 2)
     t = float(np.median(ms)) * 1e-3
     return {"stamps": n, "ms": t * 1e3, "algorithmic_TFLOPs": algo / t / 1e12,
-            "issued_TFLOPs": issued / t / 1e12,
-            "mfma_util_issued": issued / t / 1e12 / PEAK_BF16_TFLOPS,
-            "peak_bf16_TFLOPs": PEAK_BF16_TFLOPS}
+            "issued_bf16_TFLOPs": issued_bf16 / t / 1e12,
+            "issued_fp64_TFLOPs": issued_fp64 / t / 1e12,
+            # share of the stage's time the matrix cores would need at peak
+            "mfma_busy_at_peak": (issued_bf16 / PEAK_BF16_TFLOPS
+                                  + issued_fp64 / PEAK_FP64_TFLOPS) / t / 1e12,
+            "peak_bf16_TFLOPs": PEAK_BF16_TFLOPS, "peak_fp64_TFLOPs": PEAK_FP64_TFLOPS}
 
 
 def residual(diff) -> float:
@@ -203,7 +215,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=48, help="pairs per batched step")
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--workers", type=int, default=8, help="pairs in flight")
+    ap.add_argument("--workers", type=int, default=16,
+                    help="worker contexts (the most pairs in flight)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="run all workers' pairs at once instead of the autotuned counts")
     ap.add_argument("--form", default="both", choices=["both", "plain"])
     ap.add_argument("--workload", default="pair2048", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -274,13 +289,15 @@ def main() -> int:
         assert np.array_equal(pairs[0][0], tmpl) and np.array_equal(pairs[0][1], sci)
         bufs = upload_pairs(ctx, pairs)
         keep = 8
-        b_ms, res, dplain = run_batched(ctx, params, pairs, bufs, args.workers,
-                                        args.batch_steps, 1, keep=keep)
+        b_ms, res, dplain, btab = run_batched(ctx, params, pairs, bufs, args.workers,
+                                              args.batch_steps, 1, keep=keep,
+                                              tune=not args.no_tune)
         ok = all(r.status == 0 for r in res)
         b_value = args.batch * 1e3 / b_ms
         forms["batched"] = {
             "value": b_value, "ms_per_batch": b_ms, "pairs_per_batch": args.batch,
-            "pairs_in_flight": args.workers, "hw_queues": HW_QUEUES,
+            "workers": args.workers, "pairs_in_flight_per_bucket": btab,
+            "hw_queues": HW_QUEUES,
             "steps": args.batch_steps, "warmup": 1, "all_pairs_ok": ok,
             "pair0_bit_identical_to_plain": same_bits(dplain[0], diff),
             "pair_ms_mean": float(np.mean([r.ms for r in res])),
@@ -291,8 +308,9 @@ def main() -> int:
         # mixed precision + batched + learned screen
         model = np.load(screen.MODEL_PATH) if screen.MODEL_PATH.exists() else None
         fform = xbach.FORM_MIXED | (xbach.FORM_SCREEN if model is not None else 0)
-        f_ms, fres, dfull = run_batched(ctx, params, pairs, bufs, args.workers,
-                                        args.batch_steps, 1, form=fform, model=model, keep=keep)
+        f_ms, fres, dfull, ftab = run_batched(ctx, params, pairs, bufs, args.workers,
+                                              args.batch_steps, 1, form=fform, model=model,
+                                              keep=keep, tune=not args.no_tune)
         for q in bufs:
             for d in q:
                 ctx.buf_free(d)
@@ -302,7 +320,7 @@ def main() -> int:
         rel = [abs(residual(a) - residual(b)) / residual(b) for a, b in zip(dfull, dplain)]
         full = {
             "value": f_value, "ms_per_batch": f_ms, "pairs_per_batch": args.batch,
-            "pairs_in_flight": args.workers, "form": "mixed" + ("+screened" if model is not None
+            "workers": args.workers, "pairs_in_flight_per_bucket": ftab, "form": "mixed" + ("+screened" if model is not None
                                                               else ""),
             "steps": args.batch_steps, "warmup": 1, "all_pairs_ok": fok,
             "residual_rms_rel_vs_plain_max": float(max(rel)), "pairs_checked": len(rel),

@@ -355,7 +355,7 @@ int xb_batch_run_device(xb_batch* b, const xb_params* p, int n, const int* w, co
                         const void* const* d_tmpl, const void* const* d_sci,
                         void* const* d_conv, void* const* d_diff, xb_pair_result* results);
 /* Number of pairs in flight used by the next run (1..workers); 0 restores
- * "all workers".  The autotuner sets this per size bucket. */
+ * "all workers".  Clears the autotuner's per-bucket counts. */
 int xb_batch_set_inflight(xb_batch* b, int inflight);
 /* Form (XB_FORM_*) of every pair of the next run. */
 int xb_batch_set_form(xb_batch* b, int form);
@@ -363,8 +363,9 @@ int xb_batch_set_form(xb_batch* b, int form);
  * pairs (max(w, h) <= 1024, <= 2048, <= 3072, larger), times the bucket's
  * pairs at 1, 2, 4, 6, 8, 12, 16, 24 and 32 pairs in flight (up to the
  * workers; best of `reps` runs after a warm-up, a larger count must win by
- * 3 %) and keeps the fastest.  Later runs process the buckets largest first,
- * each with its own count.  table[4] (may be NULL) receives the counts (0 for
+ * 3 %) and keeps the fastest.  In later runs one pool of workers takes the
+ * pairs largest first and starts a pair only while fewer pairs run than its
+ * bucket's count.  table[4] (may be NULL) receives the counts (0 for
  * an absent bucket).  The timing runs are ordinary runs: outputs are written. */
 int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
                       const void* const* d_tmpl, const void* const* d_sci, void* const* d_conv,

@@ -12,9 +12,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <condition_variable>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -59,8 +59,11 @@ struct Job {
   xb_pair_result* res;
   std::vector<int> order;   // pair indices, largest first
   std::vector<int> status;  // per pair
-  std::atomic<int> next{0};
-  int end = 0;              // pairs J.order[next .. end) are handed out
+  std::vector<int> cap;     // per order position: pairs that may run at once
+  int next = 0;             // next order position to hand out (under gate)
+  int running = 0;          // pairs running now (under gate)
+  std::mutex gate;
+  std::condition_variable cv;
   std::mutex mu;
   int firstBad = -1;        // lowest failing pair index
   std::string firstMsg;
@@ -109,8 +112,15 @@ void worker(xb_batch* b, int wi, Job* J) {
   xb_ctx* x = b->ctx[wi];
   if (hipSetDevice(b->device) != hipSuccess) return;  // pairs stay XB_ERR_HIP
   for (;;) {
-    const int k = J->next.fetch_add(1);
-    if (k >= J->end) break;
+    // the next pair starts once fewer pairs run than its size bucket allows
+    int k;
+    {
+      std::unique_lock<std::mutex> g(J->gate);
+      J->cv.wait(g, [J] { return J->next >= J->n || J->running < J->cap[J->next]; });
+      if (J->next >= J->n) break;
+      k = J->next++;
+      J->running++;
+    }
     const int i = J->order[k];
     xb_pair_result local{};
     xb_pair_result* r = J->res ? &J->res[i] : &local;
@@ -129,17 +139,12 @@ void worker(xb_batch* b, int wi, Job* J) {
       // a failed pair may leave work queued; start the next from a clean stream
       (void)hipStreamSynchronize(x->c.stream);
     }
+    {
+      std::lock_guard<std::mutex> g(J->gate);
+      J->running--;
+    }
+    J->cv.notify_all();
   }
-}
-
-// Runs the pairs J.order[k0 .. k1) with nw workers.
-void runSpan(xb_batch* b, Job& J, int k0, int k1, int nw) {
-  J.next.store(k0);
-  J.end = k1;
-  std::vector<std::thread> th;
-  th.reserve(nw);
-  for (int wi = 0; wi < nw; wi++) th.emplace_back(worker, b, wi, &J);
-  for (auto& t : th) t.join();
 }
 
 int runBatch(xb_batch* b, Job& J) {
@@ -164,22 +169,21 @@ int runBatch(xb_batch* b, Job& J) {
   J.status.assign(J.n, XB_ERR_HIP);
   const int all = b->inflight > 0 ? std::min<int>(b->inflight, (int)b->ctx.size())
                                   : (int)b->ctx.size();
-  const bool tuned = std::any_of(b->tuned, b->tuned + XB_NBUCKET, [](int v) { return v > 0; });
-  if (!tuned) {
-    runSpan(b, J, 0, J.n, std::min(J.n, all));
-  } else {
-    // largest bucket first (the order is sorted by size), each with the
-    // number of pairs in flight the autotuner chose for it
-    int k0 = 0;
-    while (k0 < J.n) {
-      const int bk = sizeBucket(J.w[J.order[k0]], J.h[J.order[k0]]);
-      int k1 = k0;
-      while (k1 < J.n && sizeBucket(J.w[J.order[k1]], J.h[J.order[k1]]) == bk) k1++;
-      const int nw = b->tuned[bk] > 0 ? std::min<int>(b->tuned[bk], (int)b->ctx.size()) : all;
-      runSpan(b, J, k0, k1, std::min(k1 - k0, nw));
-      k0 = k1;
-    }
+  // pairs in flight: the autotuner's count for the pair's size bucket, else
+  // the batch-wide setting; one pool of workers serves every bucket, so the
+  // count changes as the queue moves from large pairs to small ones
+  J.cap.resize(J.n);
+  int nw = 1;
+  for (int k = 0; k < J.n; k++) {
+    const int t = b->tuned[sizeBucket(J.w[J.order[k]], J.h[J.order[k]])];
+    J.cap[k] = t > 0 ? std::min<int>(t, (int)b->ctx.size()) : all;
+    nw = std::max(nw, J.cap[k]);
   }
+  nw = std::min(nw, J.n);
+  std::vector<std::thread> th;
+  th.reserve(nw);
+  for (int wi = 0; wi < nw; wi++) th.emplace_back(worker, b, wi, &J);
+  for (auto& t : th) t.join();
   if (J.firstBad >= 0) {
     b->err = J.firstMsg;
     return J.status[J.firstBad];
@@ -223,6 +227,7 @@ const char* xb_batch_last_error(xb_batch* b) { return b ? b->err.c_str() : "no b
 int xb_batch_set_inflight(xb_batch* b, int inflight) {
   if (!b || inflight < 0 || inflight > (int)b->ctx.size()) return XB_ERR_ARG;
   b->inflight = inflight;
+  std::fill(b->tuned, b->tuned + XB_NBUCKET, 0);
   return XB_OK;
 }
 
