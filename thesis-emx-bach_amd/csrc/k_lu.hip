@@ -14,8 +14,9 @@
 //                per row;
 //   k_lu_begin   the zero-row test (ludcmp returns 1 before factorising) and
 //                the first 16-column panel;
-//   k_lu_step(p) one workgroup per 16 trailing columns: apply the panel's
-//                swaps, the in-panel triangular update, the rank-16 update of
+//   k_lu_step(p) one workgroup per 16 trailing columns: load them with the
+//                panel's interchanges applied (each row straight to its final
+//                position), the in-panel triangular update, the rank-16 update of
 //                the rows below (each element summed in k order), with the
 //                factorised panel staged once in LDS; workgroup 0 owns the
 //                next panel's columns and factorises that panel right after
@@ -26,11 +27,15 @@
 //   k_lu_backsub back substitution; its ascending-j sums are one sequential
 //                chain per row in the reference order.
 //
-// Panels are factorised by the whole workgroup holding the rows in registers
-// (block_panel_factor).  A row interchange relabels the two rows (logical
-// position numbers) instead of moving them; vv stays attached to its row, as
-// ludcmp's vv[imax] = vv[j] keeps it for every position still searched.  The
-// factorised rows are stored at their final positions.
+// Panels are factorised by one wave holding the panel's rows in registers
+// (wave_panel_factor): the pivot vote is a DPP reduction and the pivot row a
+// v_readlane broadcast, so a column costs no barrier and no LDS round trip.
+// A row interchange relabels the two rows (logical position numbers) instead
+// of moving them; vv stays attached to its row, as ludcmp's vv[imax] = vv[j]
+// keeps it for every position still searched.  The factorised rows are
+// stored at their final positions, and the map from each row's position
+// before the panel to its final one goes to the next step, which applies the
+// interchanges to the trailing columns and b as it loads them.
 #include <algorithm>
 
 #include "xb_dev.h"
@@ -85,44 +90,79 @@ __device__ __forceinline__ void wave_argmax(double& bv, int& bi) {
   }
 }
 
-constexpr int PSLOT = (MATSIZE + LU_THREADS - 1) / LU_THREADS;  // panel rows per thread
+constexpr int WSLOT = (MATSIZE + WAVE - 1) / WAVE;  // panel rows per lane (one wave)
 
-// Factorise columns c1..c1+cB-1 over rows c1..n with the whole workgroup
-// (every thread calls it).  S[r - c1][q] holds matrix element (r, c1 + q)
-// (updated by all earlier columns); vin[r - c1] the row scale of position r;
-// lastMaxI the pivot row of column c1 - 1.  Thread t holds rows
-// c1 + t + 256 s in registers.  A row interchange relabels the two rows
-// (logical positions) instead of moving them.  Each wave's best candidate
-// row travels to LDS with the vote, so every thread reads the pivot row right
-// after the column's single barrier.  When a row becomes the pivot of column
-// k its owner also copies it to LDS (its L part and U part are final then; it
-// is stored at position k after the last column); from then on its
-// registers are never read again (the pivot search only looks at positions
-// >= k and the final store only writes rows that were never pivots), so the
-// column updates run unconditionally on every register row.
-__device__ __forceinline__ void block_panel_factor(const double (*S)[LDP], int c1, int cB,
-                                                   const double* vin, int lastMaxI, double* M,
-                                                   double* vout, int* piv, LuState* st) {
+// DPP move of a double (both halves), lanes outside RMASK's rows keep v.
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, RMASK,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, RMASK,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Wave-wide maxima, returned wave-uniform: rotations by 1, 2, 4, 8 within
+// each row of 16 lanes give every lane its row's maximum; the row broadcasts
+// of lanes 15 and 31 carry them into the last row, whose lane 63 holds the
+// maximum of all 64 lanes.  A maximum is exact in any combining order.
+__device__ __forceinline__ double wave_max_f64(double v) {
+  v = fmax(v, dpp_f64<0x121, 0xF>(v));
+  v = fmax(v, dpp_f64<0x122, 0xF>(v));
+  v = fmax(v, dpp_f64<0x124, 0xF>(v));
+  v = fmax(v, dpp_f64<0x128, 0xF>(v));
+  v = fmax(v, dpp_f64<0x142, 0xA>(v));
+  v = fmax(v, dpp_f64<0x143, 0xC>(v));
+  return readlane_f64(v, 63);
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x121, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x122, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Factorise columns c1..c1+cB-1 over rows c1..n with ONE wave (lanes of wave
+// 0 only; no workgroup barrier inside).  S[r - c1][q] holds matrix element
+// (r, c1 + q) (updated by all earlier columns); vin[r - c1] the row scale of
+// position r; lastMaxI the pivot row of column c1 - 1.  Lane l holds rows
+// c1 + l + 64 s (s < 5) in registers.  Per column: each lane's best
+// candidate (selects, no branches), the wave maximum of vv*|a| (DPP), then
+// the largest position attaining it (ludcmp's `>=` keeps the last row with
+// the maximum; NaN never qualifies): when a single lane holds the maximum,
+// that lane's row, else a second DPP maximum over the positions.  The pivot
+// row is read from its owner lane (v_readlane) into wave-uniform registers,
+// so the scaling and the rank-1 update of the later panel columns need no
+// memory traffic.  A row interchange relabels the two rows (logical position
+// numbers) instead of moving them; vv stays attached to its row.  When a row
+// becomes the pivot of column k it is copied to LDS (its L and U parts are
+// final then); its registers are never read again.  Outputs: the pivot rows
+// at positions c1.., the other rows' panel columns at their final positions,
+// vv by final position, piv, and dst[physical row] = final position, with
+// which the next step loads the trailing columns.
+__device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1, int cB,
+                                                  const double* vin, int lastMaxI, double* M,
+                                                  double* vout, int* piv, int* dst,
+                                                  LuState* st) {
   const int n = MATSIZE, N = MATN;
-  constexpr int NW = LU_THREADS / WAVE;
-  const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
-  // per column (double-buffered by parity): each wave's best candidate
-  // (value, position) and that candidate's row, so the vote and the pivot
-  // row arrive behind one barrier
-  __shared__ double s_bv[2][NW];
-  __shared__ int s_bi[2][NW];
-  __shared__ double cand[2][NW][LB];
-  // pivot rows and pivot positions of this panel: kept in LDS and stored to
-  // global memory once at the end (a barrier waits for outstanding global
-  // stores, so none are issued inside the column loop)
+  const int lane = lane_id();
   __shared__ double prow[LB][LB];
   __shared__ int ppiv[LB];
-  double a[PSLOT][LB];
-  double vr[PSLOT];
-  int lidx[PSLOT];
+  double a[WSLOT][LB];
+  double vr[WSLOT];
+  int lidx[WSLOT];
 #pragma unroll
-  for (int s = 0; s < PSLOT; s++) {
-    const int r = c1 + tid + LU_THREADS * s;
+  for (int s = 0; s < WSLOT; s++) {
+    const int r = c1 + lane + WAVE * s;
     const bool ok = r <= n;
     lidx[s] = ok ? r : -1;
     vr[s] = ok ? vin[r - c1] : 0.0;
@@ -133,123 +173,108 @@ __device__ __forceinline__ void block_panel_factor(const double (*S)[LDP], int c
   bool fb = false;
 #pragma unroll
   for (int q = 0; q < LB; q++) {
-    // fully unrolled (register-resident a[][]): columns past cB, or after a
-    // replay was flagged, are skipped by a workgroup-uniform guard
     const int k = c1 + q;
-    const int par = q & 1;
     if (q < cB && !fb) {
       // pivot search: max vv*|a| over positions >= k, ties to the larger position
       double bv = -1.0;
       int bi = -1;
 #pragma unroll
-      for (int s = 0; s < PSLOT; s++) {
+      for (int s = 0; s < WSLOT; s++) {
         const double dum = vr[s] * fabs(a[s][q]);
-        if (lidx[s] >= k && dum >= 0.0 && (dum > bv || (dum == bv && lidx[s] > bi))) {
-          bv = dum;
-          bi = lidx[s];
-        }
+        const bool take =
+            lidx[s] >= k && dum >= 0.0 && (dum > bv || (dum == bv && lidx[s] > bi));
+        bv = take ? dum : bv;
+        bi = take ? lidx[s] : bi;
       }
-      wave_argmax(bv, bi);
-      if (lane == 0) {
-        s_bv[par][wid] = bv;
-        s_bi[par][wid] = bi;
-      }
-#pragma unroll
-      for (int s = 0; s < PSLOT; s++)
-        if (bi >= 0 && lidx[s] == bi) {
-#pragma unroll
-          for (int l = 0; l < LB; l++) cand[par][wid][l] = a[s][l];
-        }
-      __syncthreads();
-      bv = s_bv[par][0];
-      bi = s_bi[par][0];
-      int bw = 0;
-#pragma unroll
-      for (int w = 1; w < NW; w++) {
-        const double ov = s_bv[par][w];
-        const int oi = s_bi[par][w];
-        if (ov > bv || (ov == bv && oi > bi)) {
-          bv = ov;
-          bi = oi;
-          bw = w;
-        }
-      }
-      if (bi >= 0) maxI = bi;
+      const double big = wave_max_f64(bv);
+      const bool tie = bi >= 0 && bv == big;
+      const unsigned long long cand = __ballot(tie);
+      int best = -1;
+      if (cand != 0ull)
+        best = (cand & (cand - 1ull)) == 0ull
+                   ? __builtin_amdgcn_readlane(bi, __builtin_ctzll(cand))
+                   : wave_max_i32(tie ? bi : -1);
+      if (best >= 0) maxI = best;
       if (maxI < k) {
         // No row qualified and the kept pivot row lies above this column:
         // the reference swaps with an already-factorised row, which only a
         // literal replay reproduces (k_lu_fallback).
         fb = true;
       } else {
-        if (bi < 0) {
-          // no row qualified: the kept pivot row maxI (>= k) is used, and its
-          // owner publishes it first
-          __syncthreads();
+        double u[LB];
+        bool found = false;
 #pragma unroll
-          for (int s = 0; s < PSLOT; s++)
-            if (lidx[s] == maxI) {
+        for (int s = 0; s < WSLOT; s++) {
+          const unsigned long long own = __ballot(lidx[s] == maxI);
+          if (!found && own != 0ull) {
+            const int src = __builtin_ctzll(own);
 #pragma unroll
-              for (int l = 0; l < LB; l++) cand[par][0][l] = a[s][l];
-            }
-          __syncthreads();
-          bw = 0;
+            for (int l = 0; l < LB; l++) u[l] = readlane_f64(a[s][l], src);
+            found = true;
+          }
+        }
+        if (u[q] == 0.0) u[q] = 1.0e-20;
+        if (lane == 0) {
+#pragma unroll
+          for (int l = 0; l < LB; l++) prow[q][l] = u[l];
+          ppiv[q] = maxI;
         }
         if (maxI != k) {
 #pragma unroll
-          for (int s = 0; s < PSLOT; s++)
+          for (int s = 0; s < WSLOT; s++)
             lidx[s] = lidx[s] == k ? maxI : (lidx[s] == maxI ? k : lidx[s]);
         }
-        if (tid == 0) ppiv[q] = maxI;
-#pragma unroll
-        for (int s = 0; s < PSLOT; s++)
-          if (lidx[s] == k) {
-            if (a[s][q] == 0.0) a[s][q] = 1.0e-20;
-#pragma unroll
-            for (int l = 0; l < LB; l++) prow[q][l] = a[s][l];
-          }
         if (k != n) {
-          double u[LB];
-#pragma unroll
-          for (int l = q; l < LB; l++) u[l] = cand[par][bw][l];
-          if (u[q] == 0.0) u[q] = 1.0e-20;
           const double dinv = 1.0 / u[q];
 #pragma unroll
-          for (int s = 0; s < PSLOT; s++) a[s][q] *= dinv;
+          for (int s = 0; s < WSLOT; s++) a[s][q] *= dinv;
 #pragma unroll
           for (int l = q + 1; l < LB; l++) {
 #pragma unroll
-            for (int s = 0; s < PSLOT; s++) a[s][l] = a[s][l] - a[s][q] * u[l];
+            for (int s = 0; s < WSLOT; s++) a[s][l] = a[s][l] - a[s][q] * u[l];
           }
         }
       }
     }
   }
-  __syncthreads();
+  // lane 0's LDS writes precede these reads in the wave's LDS order; only
+  // the compiler needs fencing
+  __builtin_amdgcn_wave_barrier();
+  __asm__ __volatile__("" ::: "memory");
   if (fb) {
-    if (tid == 0) st->fallback = 1;
+    if (lane == 0) st->fallback = 1;
     return;
   }
-  // the panel's pivot rows (positions c1..c1+cB-1) and pivot indices
-  for (int e = tid; e < LB * LB; e += LU_THREADS) {
+  for (int e = lane; e < LB * LB; e += WAVE) {
     const int q = e / LB, l = e % LB;
     if (q < cB && l < cB) M[(size_t)(c1 + q) * N + c1 + l] = prow[q][l];
   }
-  if (tid < cB) piv[c1 + tid] = ppiv[tid];
+  if (lane < cB) piv[c1 + lane] = ppiv[lane];
 #pragma unroll
-  for (int s = 0; s < PSLOT; s++) {
+  for (int s = 0; s < WSLOT; s++) {
     if (lidx[s] >= c1 + cB) {
       double* row = M + (size_t)lidx[s] * N + c1;
 #pragma unroll
       for (int q = 0; q < LB; q++)
         if (q < cB) row[q] = a[s][q];
     }
-    if (lidx[s] > 0) vout[lidx[s]] = vr[s];
+    if (lidx[s] > 0) {
+      vout[lidx[s]] = vr[s];
+      dst[c1 + lane + WAVE * s] = lidx[s];
+    }
   }
-  if (tid == 0) st->lastMaxI = maxI;
+  if (lane == 0) st->lastMaxI = maxI;
 }
 
 __device__ __forceinline__ int* lu_zero_flags(LuState* st) {
   return reinterpret_cast<int*>(reinterpret_cast<char*>(st) + 64);
+}
+
+// final position of each physical row after a panel's interchanges, two
+// buffers by panel parity (a step reads its panel's while its workgroup 0
+// writes the next panel's)
+__device__ __forceinline__ int* lu_dst(LuState* st, int parity) {
+  return lu_zero_flags(st) + MATN * (1 + parity);
 }
 
 // vv (ludcmp's implicit row scaling) and the zero-row flags: one wave per row,
@@ -331,7 +356,7 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, 
     st->ii = 0;
   }
   if (s_zero) return;  // ludcmp returns before factorising
-  block_panel_factor(S, 1, LB, vv + 1, 0, M, vv, piv, st);
+  if (tid < WAVE) wave_panel_factor(S, 1, LB, vv + 1, 0, M, vv, piv, lu_dst(st, 0), st);
 }
 
 __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, int* piv,
@@ -349,13 +374,18 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
   __shared__ double Lp[MATSIZE][LDP];  // the factorised panel, rows c0..n
   __shared__ double T[MATSIZE][LDP];   // this block's columns, rows c0..n
   __shared__ double vS[MATSIZE];       // block 0: row scales; rhs block: b (rows c0..n)
-  __shared__ int pv[LB];
   __shared__ int s_int;                // block 0: last pivot row; rhs block: ii
   {
-    // every load of the step in flight together: one memory round trip
+    // every load of the step in flight together: one memory round trip.  The
+    // panel's row interchanges are applied here: the trailing columns and b
+    // are still in the order before them, and each of their rows goes to its
+    // final position dst[row] (written when the panel was factorised); the
+    // panel's own columns were stored at their final positions then.
     constexpr int IT = (MATSIZE * LB + LU_THREADS - 1) / LU_THREADS;
     constexpr int IV = (MATSIZE + LU_THREADS - 1) / LU_THREADS;
+    const int* dmap = lu_dst(st, p & 1);
     double va[IT], vt[IT], vr[IV];
+    int dt[IT], dv[IV];
 #pragma unroll
     for (int it = 0; it < IT; it++) {
       const int e = tid + it * LU_THREADS;
@@ -363,14 +393,15 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
       const bool in = r < R;
       va[it] = (in && q < cB) ? M[(size_t)(c0 + r) * N + c0 + q] : 0.0;
       vt[it] = (in && q < nc) ? M[(size_t)(c0 + r) * N + l0 + q] : 0.0;
+      dt[it] = (in && nc > 0) ? dmap[c0 + r] : c0 + r;
     }
     const bool wantV = rhsBlock || blockIdx.x == 0;
 #pragma unroll
     for (int it = 0; it < IV; it++) {
       const int r = tid + it * LU_THREADS;
       vr[it] = (wantV && r < R) ? (rhsBlock ? b[c0 + r] : vv[c0 + r]) : 0.0;
+      dv[it] = (rhsBlock && r < R) ? dmap[c0 + r] : c0 + r;
     }
-    const int pvl = tid < cB ? piv[c0 + tid] : 0;
     const int si = tid == 0 ? (rhsBlock ? st->ii : st->lastMaxI) : 0;
 #pragma unroll
     for (int it = 0; it < IT; it++) {
@@ -378,31 +409,20 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
       const int r = e / LB, q = e % LB;
       if (r < R) {
         Lp[r][q] = va[it];
-        T[r][q] = vt[it];
+        T[min(max(dt[it] - c0, 0), R - 1)][q] = vt[it];
       }
     }
 #pragma unroll
     for (int it = 0; it < IV; it++) {
       const int r = tid + it * LU_THREADS;
-      if (r < R) vS[r] = vr[it];
+      if (r < R) vS[min(max(dv[it] - c0, 0), R - 1)] = vr[it];  // vv is stored by final position
     }
-    if (tid < LB) pv[tid] = pvl;
     if (tid == 0) s_int = si;
   }
   __syncthreads();
   if (rhsBlock) {
-    // right-hand side column: permutation + forward substitution (lubksb)
+    // right-hand side column (already interchanged): forward substitution (lubksb)
     double* y = vS;
-    if (tid == 0)
-      for (int q = 0; q < cB; q++) {
-        const int m = pv[q] - c0;
-        if (m != q) {
-          const double t = y[m];
-          y[m] = y[q];
-          y[q] = t;
-        }
-      }
-    __syncthreads();
     for (int q = 0; q < cB; q++) {
       if (tid == 0 && s_int == 0 && !(y[q] == 0.0)) s_int = c0 + q;
       __syncthreads();
@@ -418,16 +438,8 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
   }
   // trailing column block (panels with trailing columns are full: cB == LB)
   if (tid < LB) {
-    // the panel's interchanges, in order, then U12 = L11^-1 A12 (k ascending
-    // per element), one thread per column with the 16 rows in registers
-    for (int q = 0; q < cB; q++) {
-      const int m = pv[q] - c0;
-      if (m != q) {
-        const double t = T[m][tid];
-        T[m][tid] = T[q][tid];
-        T[q][tid] = t;
-      }
-    }
+    // U12 = L11^-1 A12 (k ascending per element), one thread per column with
+    // the 16 rows in registers
     double u[LB];
 #pragma unroll
     for (int r = 0; r < LB; r++) u[r] = T[r][tid];
@@ -463,7 +475,8 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
       const int r = e / LB, t = e % LB;
       if (t < nc) M[(size_t)(c0 + r) * N + l0 + t] = T[r][t];
     }
-    block_panel_factor(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, st);
+    if (tid < WAVE)
+      wave_panel_factor(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, lu_dst(st, (p + 1) & 1), st);
     return;
   }
   for (int e = tid; e < R * LB; e += LU_THREADS) {

@@ -33,7 +33,7 @@ constexpr int MATSIZE = NCOMP + NBG + 1;  // 292 unknowns (1-based)
 constexpr int MATN = MATSIZE + 1;         // 293 rows/cols stored
 constexpr int NSOL = NPSF * NC2 + NBG + 1;  // 298 solution slots
 // LU bookkeeping block: a 64-byte state record, then one zero-row flag per row
-constexpr size_t LU_STATE_BYTES = 64 + sizeof(int) * MATN;
+constexpr size_t LU_STATE_BYTES = 64 + sizeof(int) * MATN * 3;  // flags, zero rows, 2 row maps
 constexpr int QN = NPSF + 2;              // 51: per-stamp test system (1-based)
 constexpr int MAXSS = 6;                  // 2 * maxKSStamps
 constexpr int KEEPSS = 3;                 // maxKSStamps
