@@ -542,127 +542,174 @@ __global__ __launch_bounds__(64) void k_lu_fallback(double* M, const double* M0,
 
 // Back substitution (and, after a zero row or a replay, the reference's whole
 // lubksb: its forward pass was not fused into the elimination then).
-__global__ __launch_bounds__(64) void k_lu_backsub(const double* M, const double* M0,
-                                                   const int* piv, double* b, const double* b0,
-                                                   const LuState* st, double* out) {
+//
+// x_t = (y_t - sum_{j>t} U_tj x_j) / U_tt with the sum as one chain in
+// ascending j, the reference's order.  Wave 0 runs the chains; waves 1-3
+// prepare the next row one phase ahead, so a phase (one row) costs the chain
+// plus one division and one barrier:
+//   * the products P_j = U_tj x_j (j >= t+2; +0.0 elsewhere) go to LDS, and
+//     wave 0 loads them into registers laid out by 16-column block (lane l of
+//     every row of 16 lanes holds column 16 m + l of block m), so each chain
+//     step is one v_fmac_f64 whose operand arrives by DPP row broadcast:
+//     fma(P_j, -1, s) is s - P_j rounded once, the reference's subtraction;
+//     the +0.0 entries before t+2 let every chain start at a block boundary
+//     (s - (+0.0) is s bit for bit);
+//   * the first term U_t,t+1 x_t+1 needs the x computed in the phase before,
+//     so wave 0 forms it itself;
+//   * U rows stream from global memory into an 8-row LDS ring, loaded four
+//     phases ahead into four register sets by row mod 4.
+// The phase barrier waits only for LDS traffic (the ring's global loads stay
+// in flight across it).
+constexpr int BS_THREADS = 256;
+constexpr int BS_RING = 8;                             // staged U rows
+constexpr int BS_COLS = 320;                           // padded row length
+constexpr int BS_BLK = (MATSIZE + 2 + 15) / 16;        // 19 blocks of 16 columns
+constexpr int BS_PL = BS_THREADS - WAVE;               // 192 producer lanes
+constexpr int BS_PER = (BS_COLS + BS_PL - 1) / BS_PL;  // 2 columns per producer lane
+static_assert(16 * BS_BLK <= BS_COLS && MATSIZE % 4 == 0, "back substitution layout");
+
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ __volatile__("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+#define XB_CHAIN(K)                                                                   \
+  __asm__ __volatile__("v_fmac_f64_dpp %0, %1, %2 row_newbcast:" #K                   \
+                       " row_mask:0xf bank_mask:0xf"                                  \
+                       : "+v"(s)                                                      \
+                       : "v"(p), "v"(m1));
+
+__global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(const double* M, const double* M0,
+                                                           const int* piv, double* b,
+                                                           const double* b0, const LuState* st,
+                                                           double* out) {
   const int n = MATSIZE, N = MATN;
-  __shared__ double x[NSOL];
+  const int tid = threadIdx.x, lane = lane_id();
+  __shared__ double xs[NSOL];                // y, replaced by x as rows finish
+  __shared__ double Pb[2][BS_COLS];          // products of a row, by row parity
+  __shared__ double ring[BS_RING][BS_COLS];  // U rows, row r in slot r % 8
+  __shared__ double scb[2][3];               // y_t, U_tt, U_t,t+1 by row parity
   if (st->skip) {
-    for (int i = threadIdx.x; i < NSOL; i += blockDim.x)
+    for (int i = tid; i < NSOL; i += BS_THREADS)
       if (out) out[i] = 0.0;
     return;
   }
   const bool full = st->zero || st->fallback;  // forward pass not fused
   if (st->zero) M = M0;                         // ludcmp returned before factorising
-  for (int i = threadIdx.x; i < NSOL; i += blockDim.x) x[i] = full ? b0[i] : b[i];
+  for (int i = tid; i < NSOL; i += BS_THREADS) xs[i] = full ? b0[i] : b[i];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (full) {
-      int ii = 0;
-      for (int i = 1; i <= n; i++) {
-        int ip = piv[i];
-        double sum = x[ip];
-        x[ip] = x[i];
-        if (ii) {
-          for (int j = ii; j <= i - 1; j++) sum -= M[(size_t)i * N + j] * x[j];
-        } else if (sum != 0.0) {
-          ii = i;
+  if (full && tid == 0) {
+    int ii = 0;
+    for (int i = 1; i <= n; i++) {
+      int ip = piv[i];
+      double sum = xs[ip];
+      xs[ip] = xs[i];
+      if (ii) {
+        for (int j = ii; j <= i - 1; j++) sum -= M[(size_t)i * N + j] * xs[j];
+      } else if (sum != 0.0) {
+        ii = i;
+      }
+      xs[i] = sum;
+    }
+  }
+  __syncthreads();
+  if (tid < WAVE) {
+    lds_barrier();  // ring filled
+    lds_barrier();  // row n prepared
+    const double m1 = -1.0;
+    double xnext = 0.0;
+#pragma unroll
+    for (int mb = BS_BLK - 1; mb >= 0; mb--) {
+      // rows t whose products start in block mb: (t + 2) / 16 == mb
+      const int tHi = min(n, 16 * mb + 13), tLo = max(1, 16 * mb - 2);
+      for (int t = tHi; t >= tLo; t--) {
+        const int par = t & 1;
+        double P[BS_BLK];
+#pragma unroll
+        for (int m = mb; m < BS_BLK; m++) P[m] = Pb[par][16 * m + (lane & 15)];
+        const double ud = scb[par][1], u1 = scb[par][2];
+        double s = scb[par][0];
+        if (t < n) s = s - u1 * xnext;
+#pragma unroll
+        for (int m = mb; m < BS_BLK; m++) {
+          const double p = P[m];
+          XB_CHAIN(0) XB_CHAIN(1) XB_CHAIN(2) XB_CHAIN(3) XB_CHAIN(4) XB_CHAIN(5)
+          XB_CHAIN(6) XB_CHAIN(7) XB_CHAIN(8) XB_CHAIN(9) XB_CHAIN(10) XB_CHAIN(11)
+          XB_CHAIN(12) XB_CHAIN(13) XB_CHAIN(14) XB_CHAIN(15)
         }
-        x[i] = sum;
+        const double xt = s / ud;
+        xnext = xt;
+        if (lane == 0) xs[t] = xt;
+        lds_barrier();
+      }
+    }
+  } else {
+    const int pl = tid - WAVE;
+    // every phase issues exactly BS_PER loads per lane (clamped addresses
+    // inside the matrix, masked when stored), so the wait for a set loaded
+    // four phases earlier is vmcnt(3 * BS_PER)
+    auto loadRow = [&](int r, double (&v)[BS_PER]) {
+      const int rr = min(max(r, 1), n);
+#pragma unroll
+      for (int h = 0; h < BS_PER; h++) {
+        const int j = min(pl + BS_PL * h, n);
+        v[h] = M[(size_t)rr * N + j];
+      }
+    };
+    auto storeRow = [&](int r, const double (&v)[BS_PER]) {
+#pragma unroll
+      for (int h = 0; h < BS_PER; h++) {
+        const int j = pl + BS_PL * h;
+        if (r >= 1 && j < BS_COLS) ring[r % BS_RING][j] = (j >= r && j <= n) ? v[h] : 0.0;
+      }
+    };
+    // products and scalars of row r (r >= 1), from its staged U row
+    auto prep = [&](int r) {
+      const double* U = ring[r % BS_RING];
+#pragma unroll
+      for (int h = 0; h < BS_PER; h++) {
+        const int j = pl + BS_PL * h;
+        if (j < BS_COLS) Pb[r & 1][j] = (j >= r + 2 && j <= n) ? U[j] * xs[j] : 0.0;
+      }
+      if (pl == 0) {
+        scb[r & 1][0] = xs[r];
+        scb[r & 1][1] = U[r];
+        scb[r & 1][2] = r < n ? U[r + 1] : 0.0;
+      }
+    };
+    double ld[4][BS_PER];  // row r in flight in set r % 4
+    {
+      double v[BS_RING][BS_PER];
+#pragma unroll
+      for (int q = 0; q < BS_RING; q++) loadRow(n - q, v[q]);
+#pragma unroll
+      for (int r = n - BS_RING; r > n - BS_RING - 4; r--) loadRow(r, ld[r & 3]);
+#pragma unroll
+      for (int q = 0; q < BS_RING; q++) storeRow(n - q, v[q]);
+    }
+    lds_barrier();
+    prep(n);
+    lds_barrier();
+    for (int c = 0; c < n / 4; c++) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int t = n - 4 * c - u;  // wave 0's row in this phase
+        const int S = (n - u) & 3;  // == t % 4, a constant after unrolling
+        if (t - 1 >= 1) prep(t - 1);
+        // vmcnt(6): the set loaded four phases ago (expcnt, lgkmcnt not waited)
+        static_assert(3 * BS_PER == 6, "vmcnt below");
+        __builtin_amdgcn_s_waitcnt(0x0F76);
+        storeRow(t - BS_RING, ld[S]);
+        loadRow(t - BS_RING - 4, ld[S]);
+        lds_barrier();
       }
     }
   }
-  __syncthreads();
-  // x_i = (y_i - sum_{j>i} U_ij x_j) / U_ii with the sum in ascending j.  The
-  // products need only final x values, so all lanes form them (row i-1 is
-  // prefetched into registers meanwhile); lane 0 runs the ordered subtraction
-  // chain, the only sequential part, with its operands loaded one 8-block
-  // ahead of their use.
-  constexpr int CH = 16;  // chain operands loaded per block
-  // products P[j] = U_ij x_j; entries past n stay +0.0, and s - (+0.0) is s
-  // bit for bit (also for -0, inf and NaN), so the chain always consumes
-  // whole blocks of CH without a scalar tail (and may load one block past it)
-  __shared__ double P[MATN + 2 * CH];
-  constexpr int PER = (MATSIZE + WAVE - 1) / WAVE;  // row entries per lane
-  constexpr int D = 8;                               // rows prefetched ahead
-  const int lane = threadIdx.x;
-  for (int j = lane; j < MATN + 2 * CH; j += WAVE) P[j] = 0.0;
-  // ring of the next D rows of U (entries right of the diagonal, and the
-  // diagonal), slot d holding rows i with (n - i) % D == d
-  double ring[D][PER];
-  double rdg[D];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int i = n - d;
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      const int j = i + 1 + lane + WAVE * q;
-      ring[d][q] = j <= n ? M[(size_t)i * N + j] : 0.0;
-    }
-    rdg[d] = M[(size_t)i * N + i];
-  }
-  for (int base = n; base >= 1; base -= D) {
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      const int i = base - d;
-      if (i >= 1) {
-#pragma unroll
-        for (int q = 0; q < PER; q++) {
-          const int j = i + 1 + lane + WAVE * q;
-          if (j <= n) P[j] = ring[d][q] * x[j];
-        }
-        const double dg = rdg[d];
-        const int i2 = i - D;
-        if (i2 >= 1) {
-#pragma unroll
-          for (int q = 0; q < PER; q++) {
-            const int j = i2 + 1 + lane + WAVE * q;
-            ring[d][q] = j <= n ? M[(size_t)i2 * N + j] : 0.0;
-          }
-          rdg[d] = M[(size_t)i2 * N + i2];
-        }
-        // one wave: its LDS accesses complete in issue order, so only the
-        // compiler needs fencing here (a workgroup barrier would also wait
-        // for the prefetches in flight)
-        __builtin_amdgcn_wave_barrier();
-        __asm__ __volatile__("" ::: "memory");
-        if (lane == 0) {
-          double s = x[i];
-          if (i < n) {
-            // blocks of CH from j = i+1 up to the one holding n, two register
-            // sets in turn: each block is loaded one block ahead of its use
-            // and no register copies sit on the chain's issue slots
-            int j = i + 1;
-            double a[CH], c[CH];
-#pragma unroll
-            for (int t = 0; t < CH; t++) a[t] = P[j + t];
-            for (;;) {
-#pragma unroll
-              for (int t = 0; t < CH; t++) c[t] = P[j + CH + t];
-#pragma unroll
-              for (int t = 0; t < CH; t++) s = s - a[t];
-              if (j + CH > n) break;
-#pragma unroll
-              for (int t = 0; t < CH; t++) a[t] = P[j + 2 * CH + t];
-#pragma unroll
-              for (int t = 0; t < CH; t++) s = s - c[t];
-              if (j + 2 * CH > n) break;
-              j += 2 * CH;
-            }
-          }
-          x[i] = s / dg;
-        }
-        __builtin_amdgcn_wave_barrier();
-        __asm__ __volatile__("" ::: "memory");
-      }
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < NSOL; i += blockDim.x) {
-    b[i] = x[i];
-    if (out) out[i] = x[i];
+  for (int i = tid; i < NSOL; i += BS_THREADS) {
+    b[i] = xs[i];
+    if (out) out[i] = xs[i];
   }
 }
+#undef XB_CHAIN
 
 hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int* d_skipIfZero) {
   LuState* st = reinterpret_cast<LuState*>(L.luState);
@@ -686,7 +733,7 @@ hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int*
   hipLaunchKernelGGL(k_lu_fallback, dim3(1), dim3(64), 0, c.cur, L.M, L.M0, L.luIndex, st);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(64), 0, c.cur, L.M, L.M0, L.luIndex, L.rhs,
+  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(BS_THREADS), 0, c.cur, L.M, L.M0, L.luIndex, L.rhs,
                      L.rhs0, st, d_out);
   prof_end(c, XB_PROF_SOLVE);
   return hipGetLastError();
