@@ -317,6 +317,12 @@ int xb_mlp_train(xb_ctx* ctx, float* model, const float* X, const float* y, cons
  * stream instead of forking a second stream (used by the batch runtime, where
  * the concurrency comes from several pairs in flight). */
 #define XB_CTX_ONE_STREAM 1
+/* xb_ctx_create flag: launch the global solve's kernels one by one.  By
+ * default each distinct solve of a context (a direction trial, the clip-refit
+ * system, xb_lu_solve) is captured once into a hipGraph (its 23 kernels:
+ * scaling, first panel, 19 panel steps, replay guard, substitution) and
+ * replayed on later calls; the results are identical either way. */
+#define XB_CTX_EAGER 2
 
 /* ---- batches of independent pairs ----------------------------------------
  * The reference processes one pair per process (src/main.cpp:16-159).  A batch

@@ -1,0 +1,62 @@
+"""The global solve as a replayed hipGraph (the default) against the same
+kernels launched one by one (XB_CTX_EAGER).
+
+Each context captures a solve once per (system, output, skip flag) and replays
+it on every later call: the two direction trials once per pipeline, the
+clip-refit system once per pass.  Bar: bit-identical outputs, kernel sums,
+directions and pass counts, over several pipelines in a row on one context
+(eager host work and synchronisations between the replays), on both a
+two-stream and a one-stream context, and against the oracle for one pair.
+"""
+import numpy as np
+import pytest
+
+import _oracle
+import xbach
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b):
+    nan = np.isnan(b)
+    return np.array_equal(np.isnan(a), nan) and np.array_equal(a[~nan].view(np.uint32),
+                                                               b[~nan].view(np.uint32))
+
+
+def _runs(flags, pairs, reps):
+    p = xbach.Params.default()
+    out = []
+    with xbach.Context(0, flags=flags) as c:
+        for _ in range(reps):
+            for t, s in pairs:
+                conv, diff, _ = c.run(p, t, s)
+                tr = c.trace()
+                out.append((conv, diff, tr.kernSum, tr.convTemplate, tr.kscIterations))
+    return out
+
+
+@pytest.fixture(scope="module")
+def pairs():
+    cfgs = xbach.batch_cfgs(3, seed=505, side=(512, 1024))
+    return [xbach.synth_pair(c) for c in cfgs]
+
+
+@pytest.mark.parametrize("one_stream", [False, True])
+def test_graph_replays_equal_eager(pairs, one_stream):
+    extra = xbach.CTX_ONE_STREAM if one_stream else 0
+    eager = _runs(xbach.CTX_EAGER | extra, pairs, 1)
+    graph = _runs(extra, pairs, 3)  # the second and third rounds replay captured graphs
+    for k, g in enumerate(graph):
+        e = eager[k % len(pairs)]
+        assert _same(g[0], e[0]) and _same(g[1], e[1]), k
+        assert g[2:] == e[2:], k
+
+
+def test_graph_pair_against_oracle(pairs):
+    p = xbach.Params.default()
+    t, s = pairs[0]
+    with xbach.Context(0) as c:
+        for _ in range(2):
+            conv, diff, _ = c.run(p, t, s)
+    cr, dr, _, trr, _ = _oracle.run(p, t, s)
+    assert _same(conv, cr) and _same(diff, dr)

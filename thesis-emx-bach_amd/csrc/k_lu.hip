@@ -711,12 +711,13 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(const double* M, cons
 }
 #undef XB_CHAIN
 
-hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int* d_skipIfZero) {
+// The solve's kernels in order on stream s.
+static hipError_t enqueue_solve(hipStream_t s, const LuSet& L, double* d_out,
+                                const int* d_skipIfZero) {
   LuState* st = reinterpret_cast<LuState*>(L.luState);
-  prof_begin(c, XB_PROF_SOLVE);
   hipLaunchKernelGGL(k_lu_scale, dim3((MATSIZE + LU_THREADS / WAVE - 1) / (LU_THREADS / WAVE)),
-                     dim3(LU_THREADS), 0, c.cur, L.M, L.luVV, st, d_skipIfZero);
-  hipLaunchKernelGGL(k_lu_begin, dim3(1), dim3(LU_THREADS), 0, c.cur, L.M, L.luVV, L.luIndex, st,
+                     dim3(LU_THREADS), 0, s, L.M, L.luVV, st, d_skipIfZero);
+  hipLaunchKernelGGL(k_lu_begin, dim3(1), dim3(LU_THREADS), 0, s, L.M, L.luVV, L.luIndex, st,
                      d_skipIfZero);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -725,18 +726,50 @@ hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int*
     const int cB = std::min(LB, MATSIZE - c0 + 1);
     const int trailing = MATSIZE - (c0 + cB) + 1;
     const int nColBlocks = (trailing + LB - 1) / LB;
-    hipLaunchKernelGGL(k_lu_step, dim3(nColBlocks + 1), dim3(LU_THREADS), 0, c.cur, L.M, L.luVV,
+    hipLaunchKernelGGL(k_lu_step, dim3(nColBlocks + 1), dim3(LU_THREADS), 0, s, L.M, L.luVV,
                        L.luIndex, L.rhs, st, p, nColBlocks);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_lu_fallback, dim3(1), dim3(64), 0, c.cur, L.M, L.M0, L.luIndex, st);
+  hipLaunchKernelGGL(k_lu_fallback, dim3(1), dim3(64), 0, s, L.M, L.M0, L.luIndex, st);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(BS_THREADS), 0, c.cur, L.M, L.M0, L.luIndex, L.rhs,
+  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(BS_THREADS), 0, s, L.M, L.M0, L.luIndex, L.rhs,
                      L.rhs0, st, d_out);
-  prof_end(c, XB_PROF_SOLVE);
   return hipGetLastError();
+}
+
+// The global solve on the context's current stream: replayed from a hipGraph
+// captured at the first call with these buffers (kernels only, no memset
+// nodes), or launched kernel by kernel under XB_CTX_EAGER.
+hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int* d_skipIfZero) {
+  prof_begin(c, XB_PROF_SOLVE);
+  hipError_t e = hipSuccess;
+  if (c.eagerSolve) {
+    e = enqueue_solve(c.cur, L, d_out, d_skipIfZero);
+  } else {
+    hipGraphExec_t g = nullptr;
+    for (const auto& sg : c.solveGraphs)
+      if (sg.M == L.M && sg.out == d_out && sg.skip == d_skipIfZero) {
+        g = sg.exec;
+        break;
+      }
+    if (!g) {
+      e = hipStreamBeginCapture(c.cur, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        const hipError_t e1 = enqueue_solve(c.cur, L, d_out, d_skipIfZero);
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(c.cur, &graph);
+        e = e1 != hipSuccess ? e1 : e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e == hipSuccess) c.solveGraphs.push_back({L.M, d_out, d_skipIfZero, g});
+      }
+    }
+    if (e == hipSuccess) e = hipGraphLaunch(g, c.cur);
+  }
+  prof_end(c, XB_PROF_SOLVE);
+  return e;
 }
 
 }  // namespace xb

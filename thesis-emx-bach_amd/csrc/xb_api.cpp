@@ -550,6 +550,8 @@ int stageSub(Ctx& c) {
 }
 
 void freeAll(Ctx& c) {
+  for (auto& sg : c.solveGraphs) (void)hipGraphExecDestroy(sg.exec);
+  c.solveGraphs.clear();
   dfree(&c.img[0]);
   dfree(&c.img[1]);
   dfree(&c.mask);
@@ -712,6 +714,7 @@ int xb_ctx_create(int device, int flags, xb_ctx** out) {
     return XB_ERR_HIP;
   }
   c.cur = c.stream;
+  c.eagerSolve = (flags & XB_CTX_EAGER) != 0;
   *out = x;
   return XB_OK;
 }

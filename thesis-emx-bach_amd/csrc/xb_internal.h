@@ -128,6 +128,16 @@ struct Ctx {
   hipStream_t stream = nullptr;  // primary stream: every host read-back syncs it
   hipStream_t side = nullptr;    // second stream for the other image's independent work
   hipStream_t cur = nullptr;     // stream the launchers use (stream or side)
+  // captured global solves, one per (system, output, skip flag); buffers are
+  // allocated once per context, so an entry stays valid until freeAll
+  struct SolveGraph {
+    const double* M;
+    const double* out;
+    const int* skip;
+    hipGraphExec_t exec;
+  };
+  std::vector<SolveGraph> solveGraphs;
+  bool eagerSolve = false;       // XB_CTX_EAGER: no capture
   hipEvent_t evFork = nullptr, evJoin = nullptr;
   std::string err;
   char name[256] = {0};

@@ -384,6 +384,9 @@ def fits_write_native(path, data: np.ndarray) -> None:
 
 
 # forms of the path (include/xbach.h XB_FORM_*)
+# xb_ctx_create flags (include/xbach.h)
+CTX_ONE_STREAM = 1
+CTX_EAGER = 2  # global solves launched kernel by kernel instead of replayed hipGraphs
 FORM_PLAIN, FORM_MIXED, FORM_SCREEN, FORM_RECORD = 0, 1, 2, 4
 # the screen's network (include/xbach.h XB_MLP_*)
 MLP_NFEAT, MLP_NPARAM, MLP_NGRAD = 32, 2242, 2177
@@ -413,10 +416,10 @@ def synth_pairs(cfgs, threads: int = 8) -> list[tuple[np.ndarray, np.ndarray]]:
 class Context:
     """One device context (xb_ctx): stream, resident buffers, stage API."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, flags: int = 0):
         self._L = lib()
         h = C.c_void_p()
-        rc = self._L.xb_ctx_create(device, 0, C.byref(h))
+        rc = self._L.xb_ctx_create(device, flags, C.byref(h))
         if rc != 0:
             raise XbachError(f"xb_ctx_create failed: {STATUS_NAMES.get(rc, rc)}")
         self._h = h
@@ -663,10 +666,10 @@ class Batch:
     """Many independent pairs through the pipeline at once (xb_batch): worker
     threads, each with its own context, pull pairs largest first."""
 
-    def __init__(self, device: int = 0, workers: int = 4):
+    def __init__(self, device: int = 0, workers: int = 4, flags: int = 0):
         self._L = lib()
         h = C.c_void_p()
-        rc = self._L.xb_batch_create(device, workers, 0, C.byref(h))
+        rc = self._L.xb_batch_create(device, workers, flags, C.byref(h))
         if rc != 0:
             raise XbachError(f"xb_batch_create failed: {STATUS_NAMES.get(rc, rc)}")
         self._h = h
