@@ -15,7 +15,11 @@
 // count as an FP32 one; its price is the unfused multiply-add.)  The output mask needs
 // sum |k| over unmasked pixels only when the window holds a flagged pixel; that
 // rare case re-walks the window in the reference order in FP64 so the
-// BAD_CONV decision is exact.  maskAfterConv (conv.cl:99-113) is fused.
+// BAD_CONV decision is exact.  The flags of the tile's 41 x 41 input window
+// are tested once per pixel (one ballot per window row at the start, a bit
+// mask per row kept in one lane), so neither the accumulation loop nor the
+// masked re-walk tests pixel values; the re-walk reads the tile kernel from
+// LDS.  maskAfterConv (conv.cl:99-113) is fused.
 //
 // sub (cl_kern/sub.cl:15-32): elementwise, HBM-bound.
 #include <algorithm>
@@ -154,41 +158,65 @@ __device__ __forceinline__ uint16_t after_flags(float t, double hi, double lo) {
 constexpr int CR = 7;  // output rows per lane
 
 __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
-  const int wave = threadIdx.x / WAVE;
-  const int lane = threadIdx.x % WAVE;
-  const int xs = blockIdx.x * 4 + wave;
-  const int ys = blockIdx.y;
+  __shared__ double kTile[4][KK + 7];  // the wave's tile kernel, for the masked sums
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  const int xs = blockIdx.x * 4 + wave, ys = blockIdx.y;
   if (xs >= A.xSteps) return;
   const int x0 = HKW + xs * KW, y0t = HKW + ys * KW;
   if (x0 >= A.w - HKW || y0t >= A.h - HKW) return;
   const int tile = __builtin_amdgcn_readfirstlane(xs + ys * A.xSteps);
   const int cx = lane % KW, rg = lane / KW;  // rg 0..2 active, 3 idle
-  const int x = x0 + cx;
-  const int y0 = y0t + rg * CR;
+  const int x = x0 + cx, y0 = y0t + rg * CR;
   const bool laneOn = rg < 3;
-  // All lanes share the tile, and the kernel index of every term depends
-  // only on the loop counters, so K is wave-uniform (scalar loads).
   const double* K = A.tileK64 + (size_t)tile * KK;
-  const int w = A.w;
+  const int w = A.w, h = A.h;
+  double* kl = kTile[wave];
+  for (int e = lane; e < KK; e += WAVE) kl[e] = K[e];
+  // Flag masks of the tile's 41 x 41 input window, one ballot per window row:
+  // lane l < 41 keeps bit c of row l = conv_flags of pixel
+  // (y0t - 10 + l, x0 - 10 + c), read with the same clamping as the values.
+  uint32_t mlo = 0, mhi = 0;
+  {
+    const int xx = min(max(x0 - HKW + lane, 0), w - 1);
+    for (int l = 0; l < 3 * CR + 2 * HKW; l++) {
+      const int yy = min(y0t - HKW + l, h - 1);
+      const float v = A.T[(size_t)yy * w + xx];
+      const unsigned long long m =
+          __ballot(lane < KW + 2 * HKW && conv_flags(v, A.hi, A.lo) != 0);
+      if (lane == l) {
+        mlo = (uint32_t)m;
+        mhi = (uint32_t)(m >> 32);
+      }
+    }
+  }
+  // flags of window row wr of this lane's row group, shifted to its column
+  const int rgc = min(rg, 2);
+  auto rowMask = [&](int wr) -> unsigned long long {
+    const uint32_t l0 = __builtin_amdgcn_readlane(mlo, wr);
+    const uint32_t h0 = __builtin_amdgcn_readlane(mhi, wr);
+    const uint32_t l1 = __builtin_amdgcn_readlane(mlo, wr + CR);
+    const uint32_t h1 = __builtin_amdgcn_readlane(mhi, wr + CR);
+    const uint32_t l2 = __builtin_amdgcn_readlane(mlo, wr + 2 * CR);
+    const uint32_t h2 = __builtin_amdgcn_readlane(mhi, wr + 2 * CR);
+    const uint32_t lo = rgc == 0 ? l0 : (rgc == 1 ? l1 : l2);
+    const uint32_t hi = rgc == 0 ? h0 : (rgc == 1 ? h1 : h2);
+    return (((unsigned long long)hi << 32) | lo) >> cx;
+  };
   double acc[CR];
 #pragma unroll
   for (int r = 0; r < CR; r++) acc[r] = 0.0;
   unsigned rowBad = 0;  // bit jr: a flagged pixel in window row jr (27 rows)
   const int xc = min(x, w - 1);
-  // Reference order (conv.cl:409-426): image rows ascending, then columns
-  // ascending; acc = acc + k*t without fusion.  Output row r (y = y0 + r)
-  // takes window rows jr = r .. r+20 with kernel row jk = r - jr + 20.
   for (int jr = 0; jr < CR + 2 * HKW; jr++) {
-    const int yy = min(y0 - HKW + jr, A.h - 1);
+    const int yy = min(y0 - HKW + jr, h - 1);
     const float* Trow = A.T + (size_t)yy * w;
     double row[KW];
 #pragma unroll
     for (int ii = 0; ii < KW; ii++) {
       const int xi = min(max(xc - HKW + ii, 0), w - 1);
-      const float v = Trow[xi];
-      row[ii] = (double)v;
-      if (conv_flags(v, A.hi, A.lo)) rowBad |= 1u << jr;
+      row[ii] = (double)Trow[xi];
     }
+    if (rowMask(jr) & ((1ull << KW) - 1ull)) rowBad |= 1u << jr;
 #pragma unroll
     for (int r = 0; r < CR; r++) {
       if (jr < r || jr > r + 2 * HKW) continue;  // uniform
@@ -203,29 +231,27 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   const double thr = (double)0.99f;
   for (int r = 0; r < CR; r++) {
     const int y = y0 + r;
-    if (y >= A.h - HKW) break;
+    if (y >= h - HKW) break;
     const size_t id = (size_t)y * w + x;
     double v = acc[r];
-    v = v + background_c(A.sol, x, y, A.w, A.h);
+    v = v + background_c(A.sol, x, y, w, h);
     v = v * inv;
     A.outD[id] = v;
     A.out[id] = (float)v;
     const uint16_t cm = conv_flags(A.T[id], A.hi, A.lo);
     uint16_t nm = cm;
     if (cm & XB_MASK_BAD_INPUT) nm |= XB_MASK_BAD_OUTPUT;
-    const bool anyBad = ((rowBad >> r) & ((1u << KW) - 1u)) != 0u;
-    if (anyBad) {
-      // exact sum |k| over unflagged pixels, reference loop order
-      const double* K64 = A.tileK64 + (size_t)tile * KK;
+    if ((rowBad >> r) & ((1u << KW) - 1u)) {
+      // sum |k| over unflagged pixels in the reference's loop order (rows
+      // j ascending, columns i ascending); a flagged pixel adds +0.0, which
+      // leaves the sum (never -0) unchanged
       double uks = 0.0;
-      for (int j = y - HKW; j <= y + HKW; j++) {
-        int jk = y - j + HKW;
-        for (int i = x - HKW; i <= x + HKW; i++) {
-          int ik = x - i + HKW;
-          double kk = K64[ik + jk * KW];
-          if ((conv_flags(A.T[(size_t)i + (size_t)w * j], A.hi, A.lo) & XB_MASK_BAD_INPUT) == 0)
-            uks += fabs(kk);
-        }
+      for (int jj = 0; jj <= 2 * HKW; jj++) {
+        const unsigned long long m = rowMask(r + jj);
+        const double* Kr = kl + (2 * HKW - jj) * KW;
+#pragma unroll
+        for (int ii = 0; ii <= 2 * HKW; ii++)
+          uks = uks + (((m >> ii) & 1ull) ? 0.0 : fabs(Kr[2 * HKW - ii]));
       }
       if ((uks / aks) < thr) nm |= XB_MASK_BAD_OUTPUT | XB_MASK_BAD_CONV;
       else nm |= XB_MASK_OK_CONV;
