@@ -54,7 +54,7 @@ PEAK_HBM_GBS = 8000.0
 # and WRITE_SIZE of k_conv, each scaled by what the same counter reads for a
 # device copy of known size (bench.py --copy-calibration); the summary with
 # both commands is profiles/k_conv_2048_traffic.md.
-CONV_TRAFFIC_BYTES = 206_582_626
+CONV_TRAFFIC_BYTES = 236_902_796
 
 WORKLOADS = {
     "pair2048": ("2048x2048 FP32 pair, 6000 stars (dN/dF ~ F^-1.5, 1e2-1e5), template FWHM 1.8 px, "
@@ -132,7 +132,7 @@ def upload_pairs(ctx, pairs):
 
 
 def run_batched(ctx, params, pairs, bufs, workers, steps, warmup, form=0, model=None,
-                keep=4, tune=True):
+                keep=4, tune=True, flags=0):
     """Time `steps` batches of all pairs (device-resident in and out); returns
     ms per batch, the per-pair results, the diffs of the first `keep` pairs and
     the autotuner's pairs in flight per size bucket (set up before the timed
@@ -141,7 +141,7 @@ def run_batched(ctx, params, pairs, bufs, workers, steps, warmup, form=0, model=
     hs = [t.shape[0] for t, _ in pairs]
     cols = [[q[k] for q in bufs] for k in range(4)]
     table = None
-    with xbach.Batch(0, workers=workers) as b:
+    with xbach.Batch(0, workers=workers, flags=flags) as b:
         if model is not None:
             b.screen_load(model)
         b.set_form(form)
@@ -217,6 +217,9 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--workers", type=int, default=16,
                     help="worker contexts (the most pairs in flight)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the global solves kernel by kernel instead of replaying their "
+                         "hipGraphs (for counter collection, which does not follow graph launches)")
     ap.add_argument("--no-tune", action="store_true",
                     help="run all workers' pairs at once instead of the autotuned counts")
     ap.add_argument("--form", default="both", choices=["both", "plain"])
@@ -247,7 +250,8 @@ def main() -> int:
     h, w = tmpl.shape
     params = xbach.Params.default()
 
-    ctx = xbach.Context(0)
+    ctx_flags = xbach.CTX_EAGER if args.eager else 0
+    ctx = xbach.Context(0, flags=ctx_flags)
     d_t = ctx.buf_alloc(tmpl.nbytes)
     d_s = ctx.buf_alloc(sci.nbytes)
     ctx.buf_upload(d_t, tmpl)
@@ -291,7 +295,7 @@ def main() -> int:
         keep = 8
         b_ms, res, dplain, btab = run_batched(ctx, params, pairs, bufs, args.workers,
                                               args.batch_steps, 1, keep=keep,
-                                              tune=not args.no_tune)
+                                              tune=not args.no_tune, flags=ctx_flags)
         ok = all(r.status == 0 for r in res)
         b_value = args.batch * 1e3 / b_ms
         forms["batched"] = {
@@ -310,7 +314,8 @@ def main() -> int:
         fform = xbach.FORM_MIXED | (xbach.FORM_SCREEN if model is not None else 0)
         f_ms, fres, dfull, ftab = run_batched(ctx, params, pairs, bufs, args.workers,
                                               args.batch_steps, 1, form=fform, model=model,
-                                              keep=keep, tune=not args.no_tune)
+                                              keep=keep, tune=not args.no_tune,
+                                              flags=ctx_flags)
         for q in bufs:
             for d in q:
                 ctx.buf_free(d)
