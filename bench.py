@@ -67,12 +67,13 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(params, tmpl, sci, seconds: float) -> dict:
-    """Time the oracle on this host on whole diffs of the same pair."""
+def cpu_baseline(params, tmpl, sci, seconds: float, threads: int = 0) -> dict:
+    """Time the oracle on this host on whole diffs of the same pair (threads 0:
+    all host cores)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import _oracle  # test infrastructure: the CPU restatement
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     _oracle.run(params, tmpl, sci, threads=threads)  # warm-up
     n = 0
     t0 = time.perf_counter()
@@ -281,9 +282,27 @@ def main() -> int:
                for k, v in prof.items()}
     kernels["sub"]["achieved_GBs"] = sub_bytes / sub_avg_s / 1e9
     kernels["conv"]["achieved_TFLOPs"] = conv_tflops
+    # conv bytes: algorithmic (FP32 template in, FP32 + FP64 conv and the u16
+    # mask out, mask in, FP64 tile kernels) and as measured by the counters
+    conv_bytes = 12.0 * w * h + 8.0 * w * h + 8.0 * 441 * -(-(w - 20) // 21) * -(-(h - 20) // 21)
+    kernels["conv"]["achieved_GBs_algorithmic"] = conv_bytes / conv_avg_s / 1e9
+    if (w, h) == (2048, 2048):
+        kernels["conv"]["achieved_GBs_measured_traffic"] = CONV_TRAFFIC_BYTES / conv_avg_s / 1e9
+
+    # the same pipeline with host buffers in and out (H2D of both images, D2H of
+    # the convolved and difference images, pageable host memory)
+    for _ in range(2):
+        ctx.run(params, tmpl, sci)
+    t0 = time.perf_counter()
+    nh = max(3, args.steps // 4)
+    for _ in range(nh):
+        ctx.run(params, tmpl, sci)
+    host_ms = (time.perf_counter() - t0) * 1e3 / nh
 
     forms = {"plain": {"value": plain_value, "ms_per_diff": plain_ms, "steps": args.steps,
-                       "warmup": args.warmup, "residual_rms": rms}}
+                       "warmup": args.warmup, "residual_rms": rms},
+             "plain_host_buffers": {"value": 1e3 / host_ms, "ms_per_diff": host_ms, "steps": nh,
+                                    "note": "PCIe-inclusive, not the headline value"}}
     value, ms_per_step, steps, warmup, form = plain_value, plain_ms, args.steps, args.warmup, "plain"
     ref_rms = None
     if not args.no_cpu:
@@ -372,6 +391,8 @@ def main() -> int:
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(params, tmpl, sci, args.cpu_seconds)
+        # the same restatement on one host thread (a bounded sample of whole diffs)
+        line["cpu_baseline_1thread"] = cpu_baseline(params, tmpl, sci, args.cpu_seconds / 2, 1)
         line["residual_rms_fp64_oracle"] = ref_rms
         line["residual_rms_rel_diff"] = abs(rms - ref_rms) / ref_rms
         line["gpu_vs_cpu"] = value / line["cpu_baseline"]["value"]
