@@ -7,6 +7,8 @@
 // that steer it (kept-stamp counts, merits, the ksc `check` flag).
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -243,9 +245,29 @@ int ensureStamps(Ctx& c, int N, int fsw) {
   return XB_OK;
 }
 
+// Waits for the work queued on the primary stream by polling an event: the
+// scalar read-backs between stages are on the critical path of every pair, and
+// a blocking wait added ~40 us of wake-up to each.
+hipError_t spinSync(Ctx& c) {
+  hipError_t e = hipEventRecord(c.evSync, c.stream);
+  if (e != hipSuccess) return e;
+  while ((e = hipEventQuery(c.evSync)) == hipErrorNotReady) {
+  }
+  return e;
+}
+
+// Copies `bytes` of the device scalars at byte offset `off` into the pinned
+// mirror and waits for them.
+hipError_t readBack(Ctx& c, size_t off, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(reinterpret_cast<char*>(c.hsc) + off,
+                                reinterpret_cast<const char*>(c.sc) + off, bytes,
+                                hipMemcpyDeviceToHost, c.stream);
+  return e != hipSuccess ? e : spinSync(c);
+}
+
 int readScalars(Ctx& c, DevScalars& out) {
-  XB_HIP(hipMemcpyAsync(&out, c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
-  XB_HIP(hipStreamSynchronize(c.stream));
+  XB_HIP(readBack(c, 0, sizeof(DevScalars)));
+  out = *c.hsc;
   return XB_OK;
 }
 
@@ -320,26 +342,41 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
   if (rc) return rc;
   if (!c.basis) XB_HIP(hipMalloc((void**)&c.basis, sizeof(Basis)));
   if (!c.sc) XB_HIP(hipMalloc((void**)&c.sc, sizeof(DevScalars)));
+  if (!c.hsc) XB_HIP(hipHostMalloc((void**)&c.hsc, sizeof(DevScalars), hipHostMallocDefault));
   {
     int rc2 = ensureSolver(c);
     if (rc2) return rc2;
   }
+  // The basis depends only on the parameters: it is built and uploaded when
+  // they change (and the mixed form's table when the basis widths do), so a
+  // run with the same parameters starts without a host wait.
   static_assert(sizeof(Basis) < (1 << 20), "basis size");
-  Basis hb;
-  std::memset(&hb, 0, sizeof(hb));
-  buildBasis(c.p, hb);
-  XB_HIP(hipMemcpyAsync(c.basis, &hb, sizeof(Basis), hipMemcpyHostToDevice, c.stream));
+  const bool newBasis =
+      !c.basisValid || std::memcmp(&c.basisParams, p, sizeof(xb_params)) != 0;
+  const bool newTab = (c.form & XB_FORM_MIXED) &&
+                      (!c.mixTab || std::memcmp(c.mixBg, c.p.bg, sizeof(c.mixBg)) != 0);
   std::vector<uint16_t> tab;
-  if ((c.form & XB_FORM_MIXED) &&
-      (!c.mixTab || std::memcmp(c.mixBg, c.p.bg, sizeof(c.mixBg)) != 0)) {
-    mixedTable(hb, tab);
-    if (!c.mixTab) XB_HIP(hipMalloc(&c.mixTab, tab.size() * sizeof(uint16_t)));
-    XB_HIP(hipMemcpyAsync(c.mixTab, tab.data(), tab.size() * sizeof(uint16_t),
-                          hipMemcpyHostToDevice, c.stream));
-    std::memcpy(c.mixBg, c.p.bg, sizeof(c.mixBg));
+  if (newBasis || newTab) {
+    Basis hb;
+    std::memset(&hb, 0, sizeof(hb));
+    buildBasis(c.p, hb);
+    if (newBasis)
+      XB_HIP(hipMemcpyAsync(c.basis, &hb, sizeof(Basis), hipMemcpyHostToDevice, c.stream));
+    if (newTab) {
+      mixedTable(hb, tab);
+      if (!c.mixTab) XB_HIP(hipMalloc(&c.mixTab, tab.size() * sizeof(uint16_t)));
+      XB_HIP(hipMemcpyAsync(c.mixTab, tab.data(), tab.size() * sizeof(uint16_t),
+                            hipMemcpyHostToDevice, c.stream));
+      std::memcpy(c.mixBg, c.p.bg, sizeof(c.mixBg));
+    }
+    // the host sources (hb, tab) end with this scope
+    XB_HIP(hipStreamSynchronize(c.stream));
+    if (newBasis) {
+      c.basisParams = *p;
+      c.basisValid = true;
+    }
   }
   XB_HIP(hipMemsetAsync(c.sc, 0, sizeof(DevScalars), c.stream));
-  XB_HIP(hipStreamSynchronize(c.stream));
   return XB_OK;
 }
 
@@ -510,10 +547,8 @@ int stageKsc(Ctx& c) {
     if (rec) XB_HIP(launch_screen(c, im, it, screen));
     XB_HIP(launch_fill(c, im, c.st[im].list, c.nKept[im]));
     it++;
-    int check = 0;
-    XB_HIP(hipMemcpyAsync(&check, &c.sc->check, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    XB_HIP(hipStreamSynchronize(c.stream));
-    if (!check) break;
+    XB_HIP(readBack(c, offsetof(DevScalars, check), sizeof(int)));
+    if (!c.hsc->check) break;
     if (it > 3 * XB_MAX_STAMPS + 2) {
       c.err = "ksc loop did not terminate";
       return XB_ERR_HIP;
@@ -523,10 +558,8 @@ int stageKsc(Ctx& c) {
   if (keep) {
     const int nRec = std::min(it, SCR_PASSES);
     XB_HIP(launch_screen_labels(c, im, nRec));
-    if (screen)
-      XB_HIP(hipMemcpyAsync(&c.nScreened, &c.sc->nScreened, sizeof(int), hipMemcpyDeviceToHost,
-                            c.stream));
-    XB_HIP(hipStreamSynchronize(c.stream));
+    XB_HIP(readBack(c, offsetof(DevScalars, nScreened), sizeof(int)));
+    if (screen) c.nScreened = c.hsc->nScreened;
     c.scrSamples = nRec * c.nKept[im];
   }
   return XB_OK;
@@ -535,9 +568,8 @@ int stageKsc(Ctx& c) {
 int stageConv(Ctx& c, double* kernSum, bool readSum) {
   XB_HIP(launch_conv(c));
   if (readSum) {
-    XB_HIP(hipMemcpyAsync(&c.kernSum, &c.sc->kernSum, sizeof(double), hipMemcpyDeviceToHost,
-                          c.stream));
-    XB_HIP(hipStreamSynchronize(c.stream));
+    XB_HIP(readBack(c, offsetof(DevScalars, kernSum), sizeof(double)));
+    c.kernSum = c.hsc->kernSum;
     if (kernSum) *kernSum = c.kernSum;
   }
   return XB_OK;
@@ -566,7 +598,12 @@ void freeAll(Ctx& c) {
     dfree(&s.ksum); dfree(&s.list);
   }
   dfree(&c.sc);
+  if (c.hsc) {
+    (void)hipHostFree(c.hsc);
+    c.hsc = nullptr;
+  }
   dfree(&c.basis);
+  c.basisValid = false;
   dfree(&c.ran1);
   dfree(&c.goodPix);
   for (int i = 0; i < 2; i++) {
@@ -705,9 +742,11 @@ int xb_ctx_create(int device, int flags, xb_ctx** out) {
       (one ? (c.side = c.stream, false)
            : hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess) ||
       hipEventCreateWithFlags(&c.evFork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c.evJoin, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c.evJoin, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.evSync, hipEventDisableTiming) != hipSuccess) {
     if (c.evFork) (void)hipEventDestroy(c.evFork);
     if (c.evJoin) (void)hipEventDestroy(c.evJoin);
+    if (c.evSync) (void)hipEventDestroy(c.evSync);
     if (c.side && c.side != c.stream) (void)hipStreamDestroy(c.side);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete x;
@@ -735,6 +774,7 @@ int xb_ctx_destroy(xb_ctx* x) {
   freeAll(c);
   if (c.evFork) (void)hipEventDestroy(c.evFork);
   if (c.evJoin) (void)hipEventDestroy(c.evJoin);
+  if (c.evSync) (void)hipEventDestroy(c.evSync);
   if (c.side && c.side != c.stream) (void)hipStreamDestroy(c.side);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete x;
@@ -759,6 +799,8 @@ int xb_init(xb_ctx* x, const xb_params* p, int w, int h, const float* tmpl, cons
   size_t n = (size_t)w * h;
   XB_HIP(hipMemcpyAsync(c.img[0], tmpl, n * sizeof(float), hipMemcpyHostToDevice, c.stream));
   XB_HIP(hipMemcpyAsync(c.img[1], sci, n * sizeof(float), hipMemcpyHostToDevice, c.stream));
+  // the caller's arrays are copied before this returns (they may change next)
+  XB_HIP(spinSync(c));
   return stageMask(c);
 }
 

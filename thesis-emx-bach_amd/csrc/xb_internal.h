@@ -156,6 +156,8 @@ struct Ctx {
   float* diff = nullptr;
   DevStamps st[2];
   DevScalars* sc = nullptr;
+  DevScalars* hsc = nullptr;     // pinned host mirror for the scalar read-backs
+  hipEvent_t evSync = nullptr;   // completion of the read-backs (polled, no sleep)
   Basis* basis = nullptr;
   double* ran1 = nullptr;       // ran1 draws (identical per stamp: seed -666)
   int ran1Count = 0;
@@ -169,6 +171,8 @@ struct Ctx {
   int scrSamples = 0;           // samples the last ksc recorded
   void* mixTab = nullptr;       // mixed form: split BF16 2-D basis, MFMA fragment order
   float mixBg[3] = {0, 0, 0};   // basis widths the table was built for
+  xb_params basisParams{};      // parameters the device basis was built from
+  bool basisValid = false;
   double* tileK64 = nullptr;    // [tiles][KK] FP64 tile kernels
   double* tileAks = nullptr;    // [tiles] sum |k| in the conv loop order
   int capTiles = 0;
