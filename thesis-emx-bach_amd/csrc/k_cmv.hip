@@ -10,12 +10,17 @@
 //
 // k_fill_gram: the 52x52 Gram matrix G = W^T W of the final rows, one wave
 // per 16x16 tile, staged through LDS in 64-pixel chunks, each entry a
-// sequential sum over the 961 pixels (cmv.cl:195-215 order).
+// sequential sum over the 961 pixels (cmv.cl:195-215 order).  The next chunk
+// is loaded raw (its even-row subtraction happens when it is stored to LDS,
+// so no wait sits before the products), and the products read LDS one 8-step
+// block ahead (a sched_barrier keeps the reads early): a refill of a few
+// stamps is bound by one tile's latency.
 // k_fill_target: one workgroup per stamp forms the target dot products:
 // column-major order for all 52 rows (createB, cmv.cl:217-248, and the ksc
 // right-hand side, src/cdkscUtil.cpp:459-470) and row-major order for the 3
-// background rows (cd right-hand side, cl_kern/cd.cl:318-344).  Q (51x51,
-// 1-based) is G shifted by one.
+// background rows (cd right-hand side, cl_kern/cd.cl:318-344), the latter on
+// wave 1 in slices beside the former; the next chunk's gather is in flight
+// while the current one is summed.  Q (51x51, 1-based) is G shifted by one.
 #include "xb_dev.h"
 
 namespace xb {
@@ -97,10 +102,8 @@ __global__ __launch_bounds__(256) void k_fill_target(FillArgs A) {
   const int tid = threadIdx.x;
   const double* W = A.W + (size_t)s * WROWS * NPIX;
   const int* evenSub = A.basis->evenSub;
-  // target dot products.  tc[c]: the target at column-major position c
-  // (x = c / 31, y = c % 31); tr[k]: at row-major position k.
   __shared__ double tc[NPIX], tr[NPIX];
-  __shared__ double mono[2][SSW];  // xf by column, yf by row
+  __shared__ double mono[2][SSW];
   __shared__ int rsub[WROWS];
   if (!valid) {
     if (tid < WROWS) A.Bc[(size_t)s * WROWS + tid] = 0.0;
@@ -115,59 +118,97 @@ __global__ __launch_bounds__(256) void k_fill_target(FillArgs A) {
     tc[(i % SSW) * SSW + i / SSW] = v;
   }
   if (tid < SSW) {
-    // the background monomials of k_fill_conv, same FP32 expressions
     const float hw = (float)A.w * 0.5f, hh = (float)A.h * 0.5f;
     mono[0][tid] = (double)(((float)(ssx - HSS + tid) - hw) / hw);
     mono[1][tid] = (double)(((float)(ssy - HSS + tid) - hh) / hh);
   }
   if (tid < WROWS) rsub[tid] = tid < NPSF && evenSub[tid];
-  __syncthreads();
-  // Bc: column-major pixel order (x outer, y inner): chunks of 64 such
-  // positions staged through LDS, one thread per row summing in order.
-  __shared__ double wc[WROWS][GK + 1];
-  double acc = 0.0;
-  for (int c0 = 0; c0 < NPIX; c0 += GK) {
-    const int kn = min(GK, NPIX - c0);
-    for (int i = tid; i < WROWS * GK; i += blockDim.x) {
-      const int r = i / GK, cc = i % GK;
-      double v = 0.0;
-      if (cc < kn) {
-        const int c = c0 + cc, k = (c / SSW) + SSW * (c % SSW);
-        v = W[(size_t)r * NPIX + k];
-        if (rsub[r]) v = v - W[k];
-      }
-      wc[r][cc] = v;
+  constexpr int IT = (WROWS * GK + 255) / 256;  // 13 staged values per thread
+  __shared__ double wc[2][WROWS][GK + 1];
+  double nv[IT], n0[IT];
+  auto fetch = [&](int c0) {
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const int i = tid + 256 * j;
+      const int r = min(i / GK, WROWS - 1), cc = i % GK;
+      const int c = min(c0 + cc, NPIX - 1), k = (c / SSW) + SSW * (c % SSW);
+      nv[j] = W[(size_t)r * NPIX + k];
+      n0[j] = W[k];
     }
-    __syncthreads();
+  };
+  auto store = [&](int buf, int c0) {
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const int i = tid + 256 * j;
+      const int r = i / GK, cc = i % GK;
+      if (r < WROWS) {
+        double v = 0.0;
+        if (c0 + cc < NPIX) {
+          v = nv[j];
+          if (rsub[r]) v = v - n0[j];
+        }
+        wc[buf][r][cc] = v;
+      }
+    }
+  };
+  __syncthreads();  // rsub, tc, tr, mono
+  fetch(0);
+  store(0, 0);
+  __syncthreads();
+  double acc = 0.0, a2 = 0.0;
+  const int b = tid - 64;  // background row of wave 1 lanes 0..2
+  int buf = 0, yb = 0;
+  constexpr int NCH = (NPIX + GK - 1) / GK;
+  for (int ch = 0; ch < NCH; ch++) {
+    const int c0 = ch * GK;
+    const int kn = min(GK, NPIX - c0);
+    const bool more = ch + 1 < NCH;
+    if (more) fetch(c0 + GK);
     if (tid < WROWS) {
       const double* t = tc + c0;
-      int cc = 0;
-      for (; cc + 4 <= kn; cc += 4) {
-        const double p0 = wc[tid][cc] * t[cc], p1 = wc[tid][cc + 1] * t[cc + 1];
-        const double p2 = wc[tid][cc + 2] * t[cc + 2], p3 = wc[tid][cc + 3] * t[cc + 3];
-        acc += p0;
-        acc += p1;
-        acc += p2;
-        acc += p3;
+      if (kn == GK) {
+        double wv[2][8], tv[2][8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          wv[0][j] = wc[buf][tid][j];
+          tv[0][j] = t[j];
+        }
+#pragma unroll
+        for (int blk = 0; blk < GK / 8; blk++) {
+          const int cu = blk & 1, nx = cu ^ 1;
+          if (blk + 1 < GK / 8) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+              wv[nx][j] = wc[buf][tid][8 * (blk + 1) + j];
+              tv[nx][j] = t[8 * (blk + 1) + j];
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int j = 0; j < 8; j++) acc += wv[cu][j] * tv[cu][j];
+        }
+      } else {
+        for (int cc = 0; cc < kn; cc++) acc += wc[buf][tid][cc] * t[cc];
       }
-      for (; cc < kn; cc++) acc += wc[tid][cc] * t[cc];
+    } else if (b >= 0 && b < NBG) {
+      // rows yb .. of the background chain (row-major), spread over the chunks
+      const int yEnd = (ch + 1) * SSW / NCH;
+      for (; yb < yEnd; yb++) {
+        const double yf = b == 1 ? mono[1][yb] : 1.0;
+        for (int x = 0; x < SSW; x++) {
+          const double bgv = (b == 2 ? mono[0][x] : 1.0) * yf;
+          a2 += bgv * tr[yb * SSW + x];
+        }
+      }
     }
-    __syncthreads();
+    if (more) {
+      store(buf ^ 1, c0 + GK);
+      __syncthreads();
+      buf ^= 1;
+    }
   }
   if (tid < WROWS) A.Bc[(size_t)s * WROWS + tid] = acc;
-  // Br: row-major order over the background rows 1, yf, xf
-  if (tid >= 64 && tid < 64 + NBG) {
-    const int b = tid - 64;
-    double a2 = 0.0;
-    for (int y = 0; y < SSW; y++) {
-      const double yf = b == 1 ? mono[1][y] : 1.0;
-      for (int x = 0; x < SSW; x++) {
-        const double bgv = (b == 2 ? mono[0][x] : 1.0) * yf;
-        a2 += bgv * tr[y * SSW + x];
-      }
-    }
-    A.Br[(size_t)s * NBG + b] = a2;
-  }
+  if (b >= 0 && b < NBG) A.Br[(size_t)s * NBG + b] = a2;
 }
 
 constexpr int NGT = 10;  // lower-triangular 16x16 tiles of the 52x52 Gram matrix
@@ -203,7 +244,6 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
   const int* evenSub = A.basis->evenSub;
   __shared__ __align__(16) double Wa[2][GK][GP];
   __shared__ __align__(16) double Wb[2][GK][GP];
-  // per-lane staging: pixel k0 + lane of rows bi*16 + rr and bj*16 + rr
   bool subA[GT], subB[GT];
 #pragma unroll
   for (int rr = 0; rr < GT; rr++) {
@@ -211,40 +251,43 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
     subA[rr] = ga < NPSF && evenSub[ga];
     subB[rr] = gb < NPSF && evenSub[gb];
   }
-  double va[GT], vb[GT];
+  double va[GT], vb[GT], w0;
+  // raw loads only (rows past W and pixels past the window read clamped
+  // addresses and are zeroed at store time)
   auto fetch = [&](int k0) {
-    const int k = k0 + lane;
-    const bool in = k < NPIX;
-    const double w0 = in ? W[k] : 0.0;
+    const int k = min(k0 + lane, NPIX - 1);
+    w0 = W[k];
+#pragma unroll
+    for (int rr = 0; rr < GT; rr++) {
+      const int ga = min(bi * GT + rr, WROWS - 1), gb = min(bj * GT + rr, WROWS - 1);
+      va[rr] = W[(size_t)ga * NPIX + k];
+      vb[rr] = W[(size_t)gb * NPIX + k];
+    }
+  };
+  auto store = [&](int buf, int k0) {
+    const bool in = k0 + lane < NPIX;
 #pragma unroll
     for (int rr = 0; rr < GT; rr++) {
       const int ga = bi * GT + rr, gb = bj * GT + rr;
-      double x = (in && ga < WROWS) ? W[(size_t)ga * NPIX + k] : 0.0;
-      double y = (in && gb < WROWS) ? W[(size_t)gb * NPIX + k] : 0.0;
-      if (subA[rr]) x = x - w0;
-      if (subB[rr]) y = y - w0;
-      va[rr] = x;
-      vb[rr] = y;
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int rr = 0; rr < GT; rr++) {
-      Wa[buf][lane][rr] = va[rr];
-      Wb[buf][lane][rr] = vb[rr];
+      double x = (in && ga < WROWS) ? va[rr] : 0.0;
+      double y = (in && gb < WROWS) ? vb[rr] : 0.0;
+      if (subA[rr]) x = x - (in ? w0 : 0.0);
+      if (subB[rr]) y = y - (in ? w0 : 0.0);
+      Wa[buf][lane][rr] = x;
+      Wb[buf][lane][rr] = y;
     }
   };
   const int lr = lane / 8, lc = lane % 8;
   double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
   fetch(0);
-  store(0);
+  store(0, 0);
   __syncthreads();
   int buf = 0;
   for (int k0 = 0; k0 < NPIX; k0 += GK) {
     const int kn = min(GK, NPIX - k0);
     const bool more = k0 + GK < NPIX;
     if (more) fetch(k0 + GK);
-    for (int kk = 0; kk < kn; kk++) {
+    auto step = [&](int kk) {
       const double2 x = *reinterpret_cast<const double2*>(&Wa[buf][kk][2 * lr]);
       const double2 y = *reinterpret_cast<const double2*>(&Wb[buf][kk][2 * lc]);
       const double p00 = x.x * y.x, p01 = x.x * y.y, p10 = x.y * y.x, p11 = x.y * y.y;
@@ -252,9 +295,45 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
       c01 += p01;
       c10 += p10;
       c11 += p11;
+    };
+    {
+      if (kn == GK) {
+        // 8 blocks of 8 steps; block j + 1's LDS reads are issued before
+        // block j is computed (two register sets, the index resolved by the
+        // unrolling), so the reads overlap the products
+        double2 xa[2][8], yb[2][8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          xa[0][j] = *reinterpret_cast<const double2*>(&Wa[buf][j][2 * lr]);
+          yb[0][j] = *reinterpret_cast<const double2*>(&Wb[buf][j][2 * lc]);
+        }
+#pragma unroll
+        for (int blk = 0; blk < GK / 8; blk++) {
+          const int cu = blk & 1, nx = cu ^ 1;
+          if (blk + 1 < GK / 8) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+              xa[nx][j] = *reinterpret_cast<const double2*>(&Wa[buf][8 * (blk + 1) + j][2 * lr]);
+              yb[nx][j] = *reinterpret_cast<const double2*>(&Wb[buf][8 * (blk + 1) + j][2 * lc]);
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);  // keep the next block's reads here
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const double2 x = xa[cu][j], y = yb[cu][j];
+            const double p00 = x.x * y.x, p01 = x.x * y.y, p10 = x.y * y.x, p11 = x.y * y.y;
+            c00 += p00;
+            c01 += p01;
+            c10 += p10;
+            c11 += p11;
+          }
+        }
+      } else {
+        for (int kk = 0; kk < kn; kk++) step(kk);
+      }
     }
     if (more) {
-      store(buf ^ 1);
+      store(buf ^ 1, k0 + GK);
       __syncthreads();
       buf ^= 1;
     }
