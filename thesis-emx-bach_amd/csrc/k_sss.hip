@@ -495,20 +495,81 @@ struct FindArgs {
   int* candAll;  // [N][fsw*fsw] candidate-list scratch
 };
 
-// Wave-wide reductions (every lane gets the result).
+// Wave-wide reductions by DPP, result wave-uniform: rotations by 1, 2, 4, 8
+// within each row of 16 lanes give every lane its row's result; the row
+// broadcasts of lanes 15 and 31 carry them into the last row, and lane 63
+// holds the wave's.  Lanes a broadcast does not write keep their own value
+// (min, max and argmax are idempotent) or take 0 (sums).  Every reduction
+// here is exact in any combining order.
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_keep(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_keep_f64(double v) {
+  const int lo = dpp_keep<CTRL, RM>(__double2loint(v)), hi = dpp_keep<CTRL, RM>(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_zero_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, RM, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, RM, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane63_f64(double v) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                          __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+
 // (value, index) maximum: larger value wins, ties -> smaller index; NaN never
 // wins (callers pass -inf for lanes without a candidate).
-__device__ __forceinline__ int wave_argmax_first(double v, int idx) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const double ov = __shfl_xor(v, o);
-    const int oi = __shfl_xor(idx, o);
-    if (ov > v || (ov == v && oi < idx)) {
-      v = ov;
-      idx = oi;
-    }
+template <int CTRL, int RM>
+__device__ __forceinline__ void argmax_first_step(double& v, int& idx) {
+  const double ov = dpp_keep_f64<CTRL, RM>(v);
+  const int oi = dpp_keep<CTRL, RM>(idx);
+  if (ov > v || (ov == v && oi < idx)) {
+    v = ov;
+    idx = oi;
   }
-  return idx;
+}
+__device__ __forceinline__ int wave_argmax_first(double v, int idx) {
+  argmax_first_step<0x121, 0xF>(v, idx);
+  argmax_first_step<0x122, 0xF>(v, idx);
+  argmax_first_step<0x124, 0xF>(v, idx);
+  argmax_first_step<0x128, 0xF>(v, idx);
+  argmax_first_step<0x142, 0xA>(v, idx);
+  argmax_first_step<0x143, 0xC>(v, idx);
+  return __builtin_amdgcn_readlane(idx, 63);
+}
+
+// The check's reductions in one pass: first bad position (with its type),
+// exponent range, partial sum.
+template <int CTRL, int RM>
+__device__ __forceinline__ void check_step(int& fb, int& bt, int& emn, int& emx, double& part) {
+  const int ofb = dpp_keep<CTRL, RM>(fb), obt = dpp_keep<CTRL, RM>(bt);
+  const int omn = dpp_keep<CTRL, RM>(emn), omx = dpp_keep<CTRL, RM>(emx);
+  const double op = dpp_zero_f64<CTRL, RM>(part);
+  if (ofb < fb) {
+    fb = ofb;
+    bt = obt;
+  }
+  emn = min(emn, omn);
+  emx = max(emx, omx);
+  part += op;
+}
+__device__ __forceinline__ void wave_check_reduce(int& fb, int& bt, int& emn, int& emx,
+                                                  double& part) {
+  check_step<0x121, 0xF>(fb, bt, emn, emx, part);
+  check_step<0x122, 0xF>(fb, bt, emn, emx, part);
+  check_step<0x124, 0xF>(fb, bt, emn, emx, part);
+  check_step<0x128, 0xF>(fb, bt, emn, emx, part);
+  check_step<0x142, 0xA>(fb, bt, emn, emx, part);
+  check_step<0x143, 0xC>(fb, bt, emn, emx, part);
+  fb = __builtin_amdgcn_readlane(fb, 63);
+  bt = __builtin_amdgcn_readlane(bt, 63);
+  emn = __builtin_amdgcn_readlane(emn, 63);
+  emx = __builtin_amdgcn_readlane(emx, 63);
+  part = readlane63_f64(part);
 }
 
 __device__ __forceinline__ int float_exp_field(float f) {
@@ -703,7 +764,8 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
           if (bal == 0ull) break;
           const int first = __ffsll((unsigned long long)bal) - 1;
           pending &= ~((2ull << first) - 1ull);  // drop this lane and those before it
-          const int absx = __shfl(qx, first), absy = __shfl(qy, first);
+          const int absx = __builtin_amdgcn_readlane(qx, first);
+          const int absy = __builtin_amdgcn_readlane(qy, first);
           // ---- window maximum (sss.cl:427-459)
           const int startX = max(absx - HSS, g.ox), startY = max(absy - HSS, g.oy);
           const int endX = min(absx + HSS, g.ox + fsw - 1), endY = min(absy + HSS, g.oy + fsw - 1);
@@ -795,20 +857,8 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
             }
           }
           // one pass of reductions: first bad pixel (with its type), exponent
-          // range and the partial sums, interleaved
-#pragma unroll
-          for (int o = 32; o > 0; o >>= 1) {
-            const int ofb = __shfl_xor(firstBad, o), obt = __shfl_xor(badType, o);
-            const int omn = __shfl_xor(eMin, o), omx = __shfl_xor(eMax, o);
-            const double op = __shfl_xor(part, o);
-            if (ofb < firstBad) {
-              firstBad = ofb;
-              badType = obt;
-            }
-            eMin = min(eMin, omn);
-            eMax = max(eMax, omx);
-            part += op;
-          }
+          // range and the partial sums
+          wave_check_reduce(firstBad, badType, eMin, eMax, part);
           // first bad pixel in raster order
           const int fb = firstBad;
           if (fb != INT_MAX) {
@@ -817,8 +867,8 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
             continue;  // checkSStamp returned 0
           }
           // sum of qualifying values in raster order: exact in any order when
-          // the exponent spread is small (all partial sums representable; the
-          // butterfly gives every lane the same value), otherwise summed
+          // the exponent spread is small (all partial sums representable, so
+          // the reduction's order does not matter), otherwise summed
           // sequentially by one lane.
           const int gMin = eMin;
           const int gMax = eMax;
@@ -832,7 +882,8 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
                 const double v = (double)P.value(cx0 + p % cw, cy0 + p / cw);
                 if ((v - sky) / fwhm > A.tkf) r += v;
               }
-            total = __shfl(r, 0);
+            total = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(r), 0),
+                                     __builtin_amdgcn_readlane(__double2loint(r), 0));
           }
           if (total == 0.0) continue;
           if (lane == 0) {
