@@ -6,7 +6,10 @@
 // from LDS and the raw W row is written once.  Background rows are the
 // monomials of cmv.cl:156-193 (FP32 xf, yf).  Even-even rows keep the raw
 // value; every consumer applies W[n] - W[0] on load (cmv.cl:137-154), a single
-// subtraction exactly as the reference does in place.
+// subtraction exactly as the reference does in place.  Each row is written
+// twice: in raster order (W, for the Gram matrix and the fits) and in
+// column-major pixel order (Wc, for the target products, which the reference
+// sums in that order), so both consumers read contiguous rows.
 //
 // k_fill_gram: the 52x52 Gram matrix G = W^T W of the final rows, one wave
 // per 16x16 tile, staged through LDS in 64-pixel chunks, each entry a
@@ -33,8 +36,13 @@ __global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
   const int n = blockIdx.y;
   const int tid = threadIdx.x;
   double* Wrow = A.W + ((size_t)s * WROWS + n) * NPIX;
+  double* Wcrow = A.Wc + ((size_t)s * WROWS + n) * NPIX;  // column-major copy
+  auto put = [&](int p, double v) {
+    Wrow[p] = v;
+    Wcrow[(p % SSW) * SSW + p / SSW] = v;
+  };
   if (!valid) {
-    for (int p = tid; p < NPIX; p += blockDim.x) Wrow[p] = 0.0;
+    for (int p = tid; p < NPIX; p += blockDim.x) put(p, 0.0);
     return;
   }
   if (n >= NPSF) {
@@ -48,7 +56,7 @@ __global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
       double yf = (double)(((float)y - hh) / hh);
       double ax = b == 2 ? xf : 1.0;
       double ay = b == 1 ? yf : 1.0;
-      Wrow[p] = ax * ay;
+      put(p, ax * ay);
     }
     return;
   }
@@ -87,7 +95,7 @@ __global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
       int idx = (i + xx) + TW / 2 + (j + HSS) * TW;
       w0 += (double)tmp[idx] * fx[HKW - xx];
     }
-    Wrow[p] = w0;
+    put(p, w0);
   }
 }
 
@@ -100,7 +108,6 @@ __global__ __launch_bounds__(256) void k_fill_target(FillArgs A) {
   const int g = A.kept[s];
   const bool valid = A.cur[s] < A.count[g];
   const int tid = threadIdx.x;
-  const double* W = A.W + (size_t)s * WROWS * NPIX;
   const int* evenSub = A.basis->evenSub;
   __shared__ double tc[NPIX], tr[NPIX];
   __shared__ double mono[2][SSW];
@@ -126,14 +133,17 @@ __global__ __launch_bounds__(256) void k_fill_target(FillArgs A) {
   constexpr int IT = (WROWS * GK + 255) / 256;  // 13 staged values per thread
   __shared__ double wc[2][WROWS][GK + 1];
   double nv[IT], n0[IT];
+  // the rows in column-major order (written beside W by k_fill_conv): a
+  // chunk of positions is contiguous in every row
+  const double* Wc = A.Wc + (size_t)s * WROWS * NPIX;
   auto fetch = [&](int c0) {
 #pragma unroll
     for (int j = 0; j < IT; j++) {
       const int i = tid + 256 * j;
       const int r = min(i / GK, WROWS - 1), cc = i % GK;
-      const int c = min(c0 + cc, NPIX - 1), k = (c / SSW) + SSW * (c % SSW);
-      nv[j] = W[(size_t)r * NPIX + k];
-      n0[j] = W[k];
+      const int c = min(c0 + cc, NPIX - 1);
+      nv[j] = Wc[(size_t)r * NPIX + c];
+      n0[j] = Wc[c];
     }
   };
   auto store = [&](int buf, int c0) {
@@ -366,6 +376,7 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax) {
   A.ssy = c.st[im].ssy;
   A.basis = c.basis;
   A.W = c.st[im].W;
+  A.Wc = c.st[im].Wc;
   A.G = c.st[im].G;
   A.Bc = c.st[im].Bc;
   A.Br = c.st[im].Br;

@@ -216,6 +216,7 @@ int ensureStamps(Ctx& c, int N, int fsw) {
       XB_HIP(dalloc(&s.kept, N));
       XB_HIP(dalloc(&s.cur, N));
       XB_HIP(dalloc(&s.W, (size_t)N * WROWS * NPIX));
+      XB_HIP(dalloc(&s.Wc, (size_t)N * WROWS * NPIX));
       XB_HIP(dalloc(&s.G, (size_t)N * WROWS * WROWS));
       XB_HIP(dalloc(&s.Bc, (size_t)N * WROWS));
       XB_HIP(dalloc(&s.Br, (size_t)N * NBG));
@@ -594,7 +595,7 @@ void freeAll(Ctx& c) {
     DevStamps& s = c.st[im];
     dfree(&s.sky); dfree(&s.fwhm); dfree(&s.found); dfree(&s.count);
     dfree(&s.ssx); dfree(&s.ssy); dfree(&s.ssv); dfree(&s.kept); dfree(&s.cur);
-    dfree(&s.W); dfree(&s.G); dfree(&s.Bc); dfree(&s.Br); dfree(&s.sig);
+    dfree(&s.W); dfree(&s.Wc); dfree(&s.G); dfree(&s.Bc); dfree(&s.Br); dfree(&s.sig);
     dfree(&s.ksum); dfree(&s.list);
   }
   dfree(&c.sc);

@@ -207,6 +207,7 @@ struct FillArgs {
   const int* ssy;
   const Basis* basis;
   double* W;
+  double* Wc;  // W in column-major pixel order (plain form's target products)
   double* G;
   double* Bc;
   double* Br;

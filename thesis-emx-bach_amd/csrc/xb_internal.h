@@ -72,6 +72,7 @@ struct DevStamps {
   int* kept = nullptr;     // [N] grid index of kept stamp k
   int* cur = nullptr;      // [N] current substamp of kept stamp k
   double* W = nullptr;     // [N][WROWS][NPIX] (raw rows; even rows minus row 0 on use)
+  double* Wc = nullptr;    // the same rows in column-major pixel order (target products)
   double* G = nullptr;     // [N][WROWS][WROWS] Gram of final W rows (lower part)
   double* Bc = nullptr;    // [N][WROWS] W_r . target, column-major pixel order
   double* Br = nullptr;    // [N][NBG]   bg rows . target, row-major pixel order
