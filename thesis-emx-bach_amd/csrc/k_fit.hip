@@ -18,6 +18,31 @@ namespace xb {
 // ---------------------------------------------------------------------------
 // per-stamp 50x50 test system (cdkscUtil.cpp:42-72, bach.cl:68-194)
 // ---------------------------------------------------------------------------
+// Wave-wide (value, row) maximum, ties to the larger row, wave-uniform
+// result: DPP rotations within rows of 16 lanes, then the row broadcasts of
+// lanes 15 and 31 into the last row (lane 63).  A maximum under a total
+// order: any combining order gives the same row.
+template <int CTRL, int RM>
+__device__ __forceinline__ void argmax_last_step(double& v, int& idx) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, RM, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, RM, 0xF, false);
+  const int oi = __builtin_amdgcn_update_dpp(idx, idx, CTRL, RM, 0xF, false);
+  const double ov = __hiloint2double(hi, lo);
+  if (ov > v || (ov == v && oi > idx)) {
+    v = ov;
+    idx = oi;
+  }
+}
+__device__ __forceinline__ int stamp_argmax_last(double v, int idx) {
+  argmax_last_step<0x121, 0xF>(v, idx);
+  argmax_last_step<0x122, 0xF>(v, idx);
+  argmax_last_step<0x124, 0xF>(v, idx);
+  argmax_last_step<0x128, 0xF>(v, idx);
+  argmax_last_step<0x142, 0xA>(v, idx);
+  argmax_last_step<0x143, 0xC>(v, idx);
+  return __builtin_amdgcn_readlane(idx, 63);
+}
+
 __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall,
                                                  const double* __restrict__ Bcall,
                                                  const int* nKept, double* ksum) {
@@ -27,18 +52,27 @@ __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall
   __shared__ double A[QN][QN + 1];
   __shared__ double b[QN], vv[QN];
   __shared__ int index[QN];
-  __shared__ double rv[WAVE];
-  __shared__ int ri[WAVE];
   const double* G = Gall + (size_t)s * WROWS * WROWS;
   // createTestMat / createTestVec: A[i][j] = Q[max][min], Q[a][b] = G[a-1][b-1]
-  for (int e = tid; e < QN * QN; e += blockDim.x) {
-    int i = e / QN, j = e % QN;
-    double v = 0.0;
-    if (i > 0 && j > 0) {
-      int a = max(i, j), bb = min(i, j);
-      v = G[(a - 1) * WROWS + (bb - 1)];
+  // (every load of a lane issued before its stores: one memory round trip)
+  {
+    constexpr int PER = (QN * QN + WAVE - 1) / WAVE;
+    double v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = min(tid + WAVE * u, QN * QN - 1);
+      const int i = e / QN, j = e % QN;
+      const int a = max(max(i, j), 1), bb = max(min(i, j), 1);
+      v[u] = G[(a - 1) * WROWS + (bb - 1)];
     }
-    A[i][j] = v;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = tid + WAVE * u;
+      if (e < QN * QN) {
+        const int i = e / QN, j = e % QN;
+        A[i][j] = (i > 0 && j > 0) ? v[u] : 0.0;
+      }
+    }
   }
   for (int i = tid; i < QN; i += blockDim.x) b[i] = i > 0 ? Bcall[(size_t)s * WROWS + i - 1] : 0.0;
   __syncthreads();
@@ -57,25 +91,30 @@ __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall
   }
   __syncthreads();
   const int n = QN - 1;
-  __shared__ int s_maxI;
-  if (tid == 0) s_maxI = 0;
-  __syncthreads();
+  // One wave: its LDS accesses complete in issue order, so between the steps
+  // only the compiler needs fencing.
+  auto fence = [] {
+    __builtin_amdgcn_wave_barrier();
+    __asm__ __volatile__("" ::: "memory");
+  };
+  int maxI = 0;
   for (int j = 1; j <= n; j++) {
-    // pivot: last i in [j, n] with vv[i]*|a_ij| equal to the running max
+    // pivot: vv[i]*|a_ij| over i in [j, n], one row per lane (n < 64), the
+    // last row attaining the maximum (ties to the larger row); NaN never wins
     double bv = -1.0;
     int bi = -1;
-    for (int i = j + tid; i <= n; i += blockDim.x) {
-      double dum = vv[i] * fabs(A[i][j]);
-      if (dum >= 0.0 && (dum > bv || dum == bv)) { bv = dum; bi = i; }
+    {
+      const int i = j + tid;
+      if (i <= n) {
+        const double dum = vv[i] * fabs(A[i][j]);
+        if (dum >= 0.0) {
+          bv = dum;
+          bi = i;
+        }
+      }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      double ov = __shfl_xor(bv, o);
-      int oi = __shfl_xor(bi, o);
-      if (ov > bv || (ov == bv && oi > bi)) { bv = ov; bi = oi; }
-    }
-    if (tid == 0 && bi >= 0) s_maxI = bi;
-    __syncthreads();
-    const int maxI = s_maxI;
+    const int best = stamp_argmax_last(bv, bi);
+    if (best >= 0) maxI = best;
     if (j != maxI) {
       for (int k = 1 + tid; k <= n; k += blockDim.x) {
         double t = A[maxI][k];
@@ -88,19 +127,35 @@ __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall
       index[j] = maxI;
       if (A[j][j] == 0.0) A[j][j] = 1.0e-20;
     }
-    __syncthreads();
+    fence();
     if (j != n) {
       const double dum = 1.0 / A[j][j];
-      for (int i = j + 1 + tid; i <= n; i += blockDim.x) A[i][j] *= dum;
-      __syncthreads();
-      // threads across the columns (n <= 64), rows in turn: no index division
+      {
+        const int i = j + 1 + tid;
+        if (i <= n) A[i][j] *= dum;
+      }
+      fence();
+      // threads across the columns, rows in batches of 8 with every read of
+      // a batch issued before its writes (each element still gets the
+      // column-j update after the earlier columns')
       const int l = j + 1 + tid;
       if (l <= n) {
         const double ujl = A[j][l];
-        for (int i = j + 1; i <= n; i++) A[i][l] = A[i][l] - A[i][j] * ujl;
+        int i = j + 1;
+        for (; i + 8 <= n + 1; i += 8) {
+          double aij[8], ail[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            aij[u] = A[i + u][j];
+            ail[u] = A[i + u][l];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) A[i + u][l] = ail[u] - aij[u] * ujl;
+        }
+        for (; i <= n; i++) A[i][l] = A[i][l] - A[i][j] * ujl;
       }
     }
-    __syncthreads();
+    fence();
   }
   // lubksb (bach.cl:156-194), sequential on one lane (50 unknowns)
   if (tid == 0) {
@@ -123,8 +178,6 @@ __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall
     }
     ksum[s] = b[1];
   }
-  (void)rv;
-  (void)ri;
 }
 
 // ---------------------------------------------------------------------------
