@@ -263,53 +263,81 @@ __device__ __forceinline__ void stamp_weights(int w, int h, int x, int y, double
 }
 
 constexpr int BUILD_STAMPS_LDS = 256;
+constexpr int WT_ONE = NC2;     // slot of the staged weight row that holds 1.0
+constexpr int WT_STRIDE = NC2 + 2;
 
-// One stamp's contribution to global element (r, c) (r >= c) or to rhs entry
-// rr (r < 0), in the reference's expression forms.
-__device__ __forceinline__ double build_term(const BuildArgs& A, int s, const double* wt, int r,
-                                             int c, int rr) {
-  const double* G = A.G + (size_t)s * WROWS * WROWS;
+// Where global element (r, c) (r >= c), or rhs entry rr (r < 0), takes its
+// per-stamp term from. Every term of the reference's expressions has the form
+// (wt[ia] * wt[ib]) * src[s * stride + off] with wt[WT_ONE] = 1.0:
+//   wt[i2] * wt[j2] * G[..]  is  (wt[i2] * wt[j2]) * G[..]
+//   wt[i2] * G[..], G[..] * wt[i2], Bc[i] * wt[j]  are  (wt[.] * 1.0) * g
+//   G[..], Bc[..], Br[..]  are  (1.0 * 1.0) * g
+// (x * 1.0 == x and products commute, so each term is bit-identical to the
+// expression written out). Elements whose terms are all zero set `zero`.
+struct TermPlan {
+  const double* src;
+  size_t stride;
+  int off, ia, ib;
+  bool zero;
+};
+
+__device__ __forceinline__ TermPlan plan_term(const BuildArgs& A, int r, int c, int rr) {
+  TermPlan P{A.G, (size_t)WROWS * WROWS, 0, WT_ONE, WT_ONE, false};
   if (r >= 0) {
-    if (r == 0 || c == 0) return 0.0;
-    if (r == 1 && c == 1) return G[0];
-    if (r <= NCOMP + 1) {
+    if (r == 0 || c == 0) {
+      P.zero = true;
+    } else if (r == 1) {  // c == 1
+      P.off = 0;
+    } else if (r <= NCOMP + 1) {
       // basis rows 2..289
       const int i = r - 2, i1 = i / NC2, i2 = i - i1 * NC2;
-      if (c == 1) return wt[i2] * G[(i1 + 1) * WROWS + 0];
-      const int j = c - 2, j1 = j / NC2, j2 = j - j1 * NC2;
-      return wt[i2] * wt[j2] * G[(i1 + 1) * WROWS + (j1 + 1)];
+      P.ia = i2;
+      if (c == 1) {
+        P.off = (i1 + 1) * WROWS;
+      } else {
+        const int j = c - 2, j1 = j / NC2, j2 = j - j1 * NC2;
+        P.off = (i1 + 1) * WROWS + (j1 + 1);
+        P.ib = j2;
+      }
+    } else {
+      // background rows 290..292: W row iv = 49 + iBG
+      const int iv = NC1 + (r - (NCOMP + 2)) + 1;
+      if (c == 1) {
+        P.off = iv * WROWS;
+      } else if (c <= NCOMP + 1) {
+        const int jj = c - 1;
+        P.off = iv * WROWS + (jj - 1) / NC2 + 1;
+        P.ia = (jj - 1) % NC2;
+      } else {
+        P.off = iv * WROWS + NC1 + (c - (NCOMP + 2)) + 1;
+      }
     }
-    // background rows 290..292: W row iv = 49 + iBG
-    const int iv = NC1 + (r - (NCOMP + 2)) + 1;
-    if (c == 1) return G[iv * WROWS + 0];
-    if (c <= NCOMP + 1) {
-      const int jj = c - 1;
-      const int i1 = (jj - 1) / NC2 + 1, i2 = (jj - 1) % NC2;
-      return G[iv * WROWS + i1] * wt[i2];
+  } else if (rr >= 1 && rr < NCOMP + 2 + NBG) {
+    P.src = A.Bc;
+    P.stride = WROWS;
+    if (rr == 1) {
+      P.off = 0;
+    } else if (rr <= NCOMP + 1) {
+      P.off = (rr - 2) / NC2 + 1;
+      P.ia = (rr - 2) % NC2;
+    } else if (A.cdForm) {
+      P.src = A.Br;
+      P.stride = NBG;
+      P.off = rr - (NCOMP + 2);
+    } else {
+      P.off = NPSF + rr - (NCOMP + 2);
     }
-    const int jv = NC1 + (c - (NCOMP + 2)) + 1;
-    return G[iv * WROWS + jv];
+  } else {
+    P.zero = true;  // rhs slot 0, slots past the background, threads past the end
   }
-  if (rr >= 0) {
-    const double* Bc = A.Bc + (size_t)s * WROWS;
-    if (rr == 1) return Bc[0];
-    if (rr >= 2 && rr <= NCOMP + 1) {
-      const int i = (rr - 2) / NC2 + 1, j = (rr - 2) % NC2;
-      return Bc[i] * wt[j];
-    }
-    if (rr >= NCOMP + 2 && rr < NCOMP + 2 + NBG) {
-      const int b = rr - (NCOMP + 2);
-      return A.cdForm ? A.Br[(size_t)s * NBG + b] : Bc[NPSF + b];
-    }
-  }
-  return 0.0;
+  return P;
 }
 
 __global__ __launch_bounds__(256) void k_global_build(BuildArgs A) {
   const int n = *A.nList;
   const int nLower = MATN * (MATN + 1) / 2;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  __shared__ double wts[BUILD_STAMPS_LDS][NC2];
+  __shared__ double wts[BUILD_STAMPS_LDS][WT_STRIDE];
   __shared__ int sidx[BUILD_STAMPS_LDS];
   __shared__ int svalid[BUILD_STAMPS_LDS];
   // element coordinates (r >= c) of the lower triangle, or rhs entries after
@@ -324,6 +352,7 @@ __global__ __launch_bounds__(256) void k_global_build(BuildArgs A) {
   } else if (e < nLower + NSOL) {
     rr = e - nLower;
   }
+  const TermPlan P = plan_term(A, r, c, rr);
   double acc = 0.0;
   for (int s0 = 0; s0 < n; s0 += BUILD_STAMPS_LDS) {
     const int cn = min(BUILD_STAMPS_LDS, n - s0);
@@ -340,21 +369,30 @@ __global__ __launch_bounds__(256) void k_global_build(BuildArgs A) {
       else
         for (int q = 0; q < NC2; q++) wt[q] = 0.0;
       for (int q = 0; q < NC2; q++) wts[u][q] = wt[q];
+      wts[u][WT_ONE] = 1.0;
+      wts[u][WT_ONE + 1] = 0.0;
     }
     __syncthreads();
-    // terms of 8 stamps loaded together, then added in stamp order
-    for (int u0 = 0; u0 < cn; u0 += 8) {
-      double t[8];
+    // Terms of 16 stamps loaded together, then added in stamp order; the
+    // loads are unconditional (every stamp index in the list addresses real
+    // rows), so nothing between them waits on memory. A stamp whose substamps
+    // are exhausted adds nothing.
+    for (int u0 = 0; u0 < cn; u0 += 16) {
+      double t[16];
 #pragma unroll
-      for (int v = 0; v < 8; v++) {
-        const int u = u0 + v;
-        t[v] = (u < cn && svalid[u]) ? build_term(A, sidx[u], wts[u], r, c, rr) : 0.0;
+      for (int v = 0; v < 16; v++) {
+        const int u = min(u0 + v, cn - 1);
+        const double g = P.src[(size_t)sidx[u] * P.stride + P.off];
+        t[v] = (wts[u][P.ia] * wts[u][P.ib]) * g;
       }
 #pragma unroll
-      for (int v = 0; v < 8; v++)
-        if (u0 + v < cn && svalid[u0 + v]) acc += t[v];  // exhausted stamps add nothing
+      for (int v = 0; v < 16; v++) {
+        const bool take = u0 + v < cn && svalid[min(u0 + v, cn - 1)];
+        acc = take ? acc + t[v] : acc;
+      }
     }
   }
+  if (P.zero) acc = 0.0;
   if (r >= 0) {
     A.M[(size_t)r * MATN + c] = acc;
     A.M[(size_t)c * MATN + r] = acc;
