@@ -91,6 +91,7 @@ __device__ __forceinline__ void wave_argmax(double& bv, int& bi) {
 }
 
 constexpr int WSLOT = (MATSIZE + WAVE - 1) / WAVE;  // panel rows per lane (one wave)
+static_assert(WSLOT == 5, "the step dispatches 1..5 row slots");
 
 // DPP move of a double (both halves), lanes outside RMASK's rows keep v.
 template <int CTRL, int RMASK>
@@ -135,7 +136,7 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 // 0 only; no workgroup barrier inside).  S[r - c1][q] holds matrix element
 // (r, c1 + q) (updated by all earlier columns); vin[r - c1] the row scale of
 // position r; lastMaxI the pivot row of column c1 - 1.  Lane l holds rows
-// c1 + l + 64 s (s < 5) in registers.  Per column: each lane's best
+// c1 + l + 64 s (s < NS) in registers.  Per column: each lane's best
 // candidate (selects, no branches), the wave maximum of vv*|a| (DPP), then
 // the largest position attaining it (ludcmp's `>=` keeps the last row with
 // the maximum; NaN never qualifies): when a single lane holds the maximum,
@@ -149,6 +150,9 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 // at positions c1.., the other rows' panel columns at their final positions,
 // vv by final position, piv, and dst[physical row] = final position, with
 // which the next step loads the trailing columns.
+// NS = row slots per lane: 64 * NS >= n - c1 + 1 (slots past the last row
+// would only hold empty rows).
+template <int NS>
 __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1, int cB,
                                                   const double* vin, int lastMaxI, double* M,
                                                   double* vout, int* piv, int* dst,
@@ -157,11 +161,11 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
   const int lane = lane_id();
   __shared__ double prow[LB][LB];
   __shared__ int ppiv[LB];
-  double a[WSLOT][LB];
-  double vr[WSLOT];
-  int lidx[WSLOT];
+  double a[NS][LB];
+  double vr[NS];
+  int lidx[NS];
 #pragma unroll
-  for (int s = 0; s < WSLOT; s++) {
+  for (int s = 0; s < NS; s++) {
     const int r = c1 + lane + WAVE * s;
     const bool ok = r <= n;
     lidx[s] = ok ? r : -1;
@@ -179,7 +183,7 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
       double bv = -1.0;
       int bi = -1;
 #pragma unroll
-      for (int s = 0; s < WSLOT; s++) {
+      for (int s = 0; s < NS; s++) {
         const double dum = vr[s] * fabs(a[s][q]);
         const bool take =
             lidx[s] >= k && dum >= 0.0 && (dum > bv || (dum == bv && lidx[s] > bi));
@@ -204,7 +208,7 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
         double u[LB];
         bool found = false;
 #pragma unroll
-        for (int s = 0; s < WSLOT; s++) {
+        for (int s = 0; s < NS; s++) {
           const unsigned long long own = __ballot(lidx[s] == maxI);
           if (!found && own != 0ull) {
             const int src = __builtin_ctzll(own);
@@ -221,17 +225,17 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
         }
         if (maxI != k) {
 #pragma unroll
-          for (int s = 0; s < WSLOT; s++)
+          for (int s = 0; s < NS; s++)
             lidx[s] = lidx[s] == k ? maxI : (lidx[s] == maxI ? k : lidx[s]);
         }
         if (k != n) {
           const double dinv = 1.0 / u[q];
 #pragma unroll
-          for (int s = 0; s < WSLOT; s++) a[s][q] *= dinv;
+          for (int s = 0; s < NS; s++) a[s][q] *= dinv;
 #pragma unroll
           for (int l = q + 1; l < LB; l++) {
 #pragma unroll
-            for (int s = 0; s < WSLOT; s++) a[s][l] = a[s][l] - a[s][q] * u[l];
+            for (int s = 0; s < NS; s++) a[s][l] = a[s][l] - a[s][q] * u[l];
           }
         }
       }
@@ -251,7 +255,7 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
   }
   if (lane < cB) piv[c1 + lane] = ppiv[lane];
 #pragma unroll
-  for (int s = 0; s < WSLOT; s++) {
+  for (int s = 0; s < NS; s++) {
     if (lidx[s] >= c1 + cB) {
       double* row = M + (size_t)lidx[s] * N + c1;
 #pragma unroll
@@ -356,7 +360,7 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, 
     st->ii = 0;
   }
   if (s_zero) return;  // ludcmp returns before factorising
-  if (tid < WAVE) wave_panel_factor(S, 1, LB, vv + 1, 0, M, vv, piv, lu_dst(st, 0), st);
+  if (tid < WAVE) wave_panel_factor<WSLOT>(S, 1, LB, vv + 1, 0, M, vv, piv, lu_dst(st, 0), st);
 }
 
 __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, int* piv,
@@ -475,8 +479,21 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
       const int r = e / LB, t = e % LB;
       if (t < nc) M[(size_t)(c0 + r) * N + l0 + t] = T[r][t];
     }
-    if (tid < WAVE)
-      wave_panel_factor(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, lu_dst(st, (p + 1) & 1), st);
+    if (tid < WAVE) {
+      // rows l0..n of the next panel: as many register slots as they need
+      int* dnext = lu_dst(st, (p + 1) & 1);
+      const int rows = n - l0 + 1;
+      if (rows > 4 * WAVE)
+        wave_panel_factor<5>(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, dnext, st);
+      else if (rows > 3 * WAVE)
+        wave_panel_factor<4>(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, dnext, st);
+      else if (rows > 2 * WAVE)
+        wave_panel_factor<3>(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, dnext, st);
+      else if (rows > WAVE)
+        wave_panel_factor<2>(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, dnext, st);
+      else
+        wave_panel_factor<1>(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, dnext, st);
+    }
     return;
   }
   for (int e = tid; e < R * LB; e += LU_THREADS) {
