@@ -107,6 +107,20 @@ def test_saturation_cosmics_and_nans(ctx):
     check_case(ctx, t, s)
 
 
+def test_nans_in_both_images(ctx):
+    # NaN pixels scattered through both images (some at the same pixel): each
+    # image's statistics and substamp search see the other image's NaN marks
+    # in the reference's order (calcStats T, calcStats S, then the searches)
+    t, s = xbach.synth_pair(xbach.synth_cfg("pair512", seed=41))
+    rng = np.random.default_rng(41)
+    both = rng.integers(20, 492, (150, 2))
+    for arr, seed in ((t, 1), (s, 2)):
+        pts = np.random.default_rng(seed).integers(20, 492, (1500, 2))
+        arr[pts[:, 0], pts[:, 1]] = np.nan
+        arr[both[:, 0], both[:, 1]] = np.nan
+    check_case(ctx, t, s)
+
+
 def test_stage_api_equals_pipeline(ctx):
     t, s = xbach.synth_pair(xbach.synth_cfg("pair512", seed=21))
     p = xbach.Params.default()
