@@ -212,15 +212,17 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
           const unsigned long long own = __ballot(lidx[s] == maxI);
           if (!found && own != 0ull) {
             const int src = __builtin_ctzll(own);
+            // columns q.. only: the row's L part (columns before q) is
+            // final in its owner lane's registers and stored from there
 #pragma unroll
-            for (int l = 0; l < LB; l++) u[l] = readlane_f64(a[s][l], src);
+            for (int l = q; l < LB; l++) u[l] = readlane_f64(a[s][l], src);
             found = true;
           }
         }
         if (u[q] == 0.0) u[q] = 1.0e-20;
         if (lane == 0) {
 #pragma unroll
-          for (int l = 0; l < LB; l++) prow[q][l] = u[l];
+          for (int l = q; l < LB; l++) prow[q][l] = u[l];
           ppiv[q] = maxI;
         }
         if (maxI != k) {
@@ -249,18 +251,23 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
     if (lane == 0) st->fallback = 1;
     return;
   }
+  // the pivot rows' U parts (columns from their own on) from LDS ...
   for (int e = lane; e < LB * LB; e += WAVE) {
     const int q = e / LB, l = e % LB;
-    if (q < cB && l < cB) M[(size_t)(c1 + q) * N + c1 + l] = prow[q][l];
+    if (q < cB && l < cB && l >= q) M[(size_t)(c1 + q) * N + c1 + l] = prow[q][l];
   }
   if (lane < cB) piv[c1 + lane] = ppiv[lane];
+  // ... and every other element from its row's registers: all panel
+  // columns of a row below the panel, and a pivot row's L part (the columns
+  // before its own, untouched since their scaling)
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    if (lidx[s] >= c1 + cB) {
+    if (lidx[s] >= c1) {
+      const int own = lidx[s] - c1;  // the row's pivot column, or >= cB
       double* row = M + (size_t)lidx[s] * N + c1;
 #pragma unroll
       for (int q = 0; q < LB; q++)
-        if (q < cB) row[q] = a[s][q];
+        if (q < cB && q < own) row[q] = a[s][q];
     }
     if (lidx[s] > 0) {
       vout[lidx[s]] = vr[s];
