@@ -448,21 +448,35 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
     return;
   }
   // trailing column block (panels with trailing columns are full: cB == LB)
-  if (tid < LB) {
-    // U12 = L11^-1 A12 (k ascending per element), one thread per column with
-    // the 16 rows in registers
-    double u[LB];
+  {
+    // U12 = L11^-1 A12, one thread per element: row r = tid % 16 and column
+    // t = tid / 16, so the 16 lanes of a DPP row hold one column.  At step q
+    // lane q's value is final (it took steps 0..q-1), every lane of the row
+    // receives it by DPP row broadcast, and the rows below q subtract
+    // L_rq u_q: each element takes k ascending, as the reference's Crout sums.
+    // The product is fma(u_q, L_rq, -0.0), which is the rounded product
+    // L_rq u_q exactly (signed zeros included); the subtraction is separate.
+    // The s_nop covers the DPP read of s after the previous step's VALU write.
+    static_assert(LB * LB == LU_THREADS, "one thread per U12 element");
+    const int r = tid & (LB - 1), t = tid / LB;
+    double s = T[r][t];
+    double Lr[LB];
 #pragma unroll
-    for (int r = 0; r < LB; r++) u[r] = T[r][tid];
-#pragma unroll
-    for (int r = 1; r < LB; r++) {
-      double s = u[r];
-#pragma unroll
-      for (int q = 0; q < r; q++) s = s - Lp[r][q] * u[q];
-      u[r] = s;
-    }
-#pragma unroll
-    for (int r = 0; r < LB; r++) T[r][tid] = u[r];
+    for (int q = 0; q < LB - 1; q++) Lr[q] = Lp[r][q];
+#define XB_U12_STEP(K)                                                              \
+  {                                                                                 \
+    double p = -0.0;                                                                \
+    __asm__ __volatile__("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:" #K      \
+                         " row_mask:0xf bank_mask:0xf"                              \
+                         : "+v"(p)                                                  \
+                         : "v"(s), "v"(Lr[K]));                                     \
+    if (r > K) s = s - p;                                                           \
+  }
+    XB_U12_STEP(0) XB_U12_STEP(1) XB_U12_STEP(2) XB_U12_STEP(3) XB_U12_STEP(4)
+    XB_U12_STEP(5) XB_U12_STEP(6) XB_U12_STEP(7) XB_U12_STEP(8) XB_U12_STEP(9)
+    XB_U12_STEP(10) XB_U12_STEP(11) XB_U12_STEP(12) XB_U12_STEP(13) XB_U12_STEP(14)
+#undef XB_U12_STEP
+    T[r][t] = s;
   }
   __syncthreads();
   {
