@@ -460,7 +460,11 @@ struct SigArgs {
   double* sigOut;    // indexed by list position
 };
 
-__global__ __launch_bounds__(256) void k_calc_sigs(SigArgs A) {
+// One workgroup per stamp, one thread per substamp pixel (992 >= 31 x 31 slots
+// of the reference's 32 x 31 reduction layout), the 48 basis terms of a pixel
+// loaded together before its accumulation chain.
+constexpr int SIG_THREADS = 1024;
+__global__ __launch_bounds__(SIG_THREADS) void k_calc_sigs(SigArgs A) {
   const int u = blockIdx.x;
   if (u >= *A.nList) return;
   const int tid = threadIdx.x;
@@ -499,6 +503,7 @@ __global__ __launch_bounds__(256) void k_calc_sigs(SigArgs A) {
     if (valid && gy < SSW) {
       // makeModel (cd.cl:489-531): FP32 accumulator, FP64 terms
       float m0 = (float)(sol[1] * W[gi]);
+#pragma unroll
       for (int i = 1; i < NPSF; i++) {
         double wv = W[(size_t)i * NPIX + gi];
         if (evenSub[i]) wv = wv - W[gi];
@@ -755,7 +760,7 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol) {
   A.sol = d_sol;
   A.sigOut = c.st[im].sig;
   if (c.nKept[im] <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(256), 0, c.cur, A);
+  hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur, A);
   return hipGetLastError();
 }
 
