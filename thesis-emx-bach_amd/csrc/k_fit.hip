@@ -43,6 +43,11 @@ __device__ __forceinline__ int stamp_argmax_last(double v, int idx) {
   return __builtin_amdgcn_readlane(idx, 63);
 }
 
+__device__ __forceinline__ double readlane_f64_at(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
 __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall,
                                                  const double* __restrict__ Bcall,
                                                  const int* nKept, double* ksum) {
@@ -157,27 +162,63 @@ __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall
     }
     fence();
   }
-  // lubksb (bach.cl:156-194), sequential on one lane (50 unknowns)
-  if (tid == 0) {
-    int ii = 0;
-    for (int i = 1; i <= n; i++) {
-      int ip = index[i];
-      double sum = b[ip];
-      b[ip] = b[i];
-      if (ii != 0) {
-        for (int jj = ii; jj <= i - 1; jj++) sum -= A[i][jj] * b[jj];
-      } else if (sum != 0.0) {
-        ii = i;
-      }
-      b[i] = sum;
+  // lubksb (bach.cl:156-194) by the wave: lane j holds b[j].  Each row's
+  // products A[i][j] * b[j] are formed by their lanes and staged in LDS, then
+  // subtracted in ascending j as one chain: fma(P, -1, s) is s - P rounded
+  // once, with P broadcast by DPP from lane K of each 16-lane row (every row
+  // of the wave holds the same block, so every lane runs the same chain).
+  // Slots outside a row's range hold +0.0, which leaves the chain unchanged
+  // (it never holds -0.0 there: s - (+0.0) is s).
+  __shared__ double Pl[64];
+  const double m1 = -1.0;
+  double bj = tid <= n ? b[tid] : 0.0;
+#define XB_SCHAIN(K)                                                                \
+  __asm__ __volatile__("v_fmac_f64_dpp %0, %1, %2 row_newbcast:" #K                 \
+                       " row_mask:0xf bank_mask:0xf"                                \
+                       : "+v"(sv)                                                   \
+                       : "v"(pk), "v"(m1));
+  // sv - (sum of the staged products of blocks m0..m1b), in ascending order
+  auto chain = [&](double sv, int m0, int m1b) -> double {
+    for (int m = m0; m <= m1b; m++) {
+      const double pk = Pl[16 * m + (tid & 15)];
+      // two wait states before the first DPP read of pk (a copy into another
+      // register would be a VALU write)
+      __asm__ __volatile__("s_nop 1");
+      XB_SCHAIN(0) XB_SCHAIN(1) XB_SCHAIN(2) XB_SCHAIN(3) XB_SCHAIN(4) XB_SCHAIN(5)
+      XB_SCHAIN(6) XB_SCHAIN(7) XB_SCHAIN(8) XB_SCHAIN(9) XB_SCHAIN(10) XB_SCHAIN(11)
+      XB_SCHAIN(12) XB_SCHAIN(13) XB_SCHAIN(14) XB_SCHAIN(15)
     }
-    for (int i = n; i >= 1; i--) {
-      double sum = b[i];
-      for (int jj = i + 1; jj <= n; jj++) sum -= A[i][jj] * b[jj];
-      b[i] = sum / A[i][i];
+    return sv;
+  };
+  const int jc = min(tid, n);  // a real column for every lane (masked below)
+  int ii = 0;
+  for (int i = 1; i <= n; i++) {
+    const int ip = __builtin_amdgcn_readfirstlane(index[i]);
+    double sum = readlane_f64_at(bj, ip);
+    const double bi = readlane_f64_at(bj, i);
+    if (tid == ip) bj = bi;  // b[ip] = b[i] (ip >= i: the b[j], j < i, stay)
+    if (ii != 0) {
+      Pl[tid] = (tid >= ii && tid <= i - 1) ? A[i][jc] * bj : 0.0;
+      fence();
+      sum = chain(sum, ii >> 4, (i - 1) >> 4);
+      fence();
+    } else if (sum != 0.0) {
+      ii = i;
     }
-    ksum[s] = b[1];
+    if (tid == i) bj = sum;
   }
+  for (int i = n; i >= 1; i--) {
+    double sum = readlane_f64_at(bj, i);
+    Pl[tid] = (tid > i && tid <= n) ? A[i][jc] * bj : 0.0;
+    fence();
+    sum = chain(sum, (i + 1) >> 4, n >> 4);
+    fence();
+    const double x = sum / A[i][i];
+    if (tid == i) bj = x;
+  }
+#undef XB_SCHAIN
+  const double k1 = readlane_f64_at(bj, 1);
+  if (tid == 0) ksum[s] = k1;
 }
 
 // ---------------------------------------------------------------------------
