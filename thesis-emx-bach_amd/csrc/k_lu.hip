@@ -585,15 +585,17 @@ __global__ __launch_bounds__(64) void k_lu_fallback(double* M, const double* M0,
 // ascending j, the reference's order.  Wave 0 runs the chains; waves 1-3
 // prepare the next row one phase ahead, so a phase (one row) costs the chain
 // plus one division and one barrier:
-//   * the products P_j = U_tj x_j (j >= t+2; +0.0 elsewhere) go to LDS, and
+//   * the products P_j = U_tj x_j (j >= t+3; +0.0 elsewhere) go to LDS, and
 //     wave 0 loads them into registers laid out by 16-column block (lane l of
 //     every row of 16 lanes holds column 16 m + l of block m), so each chain
 //     step is one v_fmac_f64 whose operand arrives by DPP row broadcast:
 //     fma(P_j, -1, s) is s - P_j rounded once, the reference's subtraction;
-//     the +0.0 entries before t+2 let every chain start at a block boundary
+//     the +0.0 entries before t+3 let every chain start at a block boundary
 //     (s - (+0.0) is s bit for bit);
-//   * the first term U_t,t+1 x_t+1 needs the x computed in the phase before,
-//     so wave 0 forms it itself;
+//   * the first two terms, U_t,t+1 x_t+1 and U_t,t+2 x_t+2, wave 0 forms
+//     itself: so a phase opens with the barrier, then wave 0 divides for
+//     x_t+1 while its loads of row t's products are in flight, and the
+//     producers of row t-1 need no x newer than x_t+2;
 //   * U rows stream from global memory into an 8-row LDS ring, loaded four
 //     phases ahead into four register sets by row mod 4.
 // The phase barrier waits only for LDS traffic (the ring's global loads stay
@@ -625,7 +627,7 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(const double* M, cons
   __shared__ double xs[NSOL];                // y, replaced by x as rows finish
   __shared__ double Pb[2][BS_COLS];          // products of a row, by row parity
   __shared__ double ring[BS_RING][BS_COLS];  // U rows, row r in slot r % 8
-  __shared__ double scb[2][3];               // y_t, U_tt, U_t,t+1 by row parity
+  __shared__ double scb[2][4];               // y_t, U_tt, U_t,t+1, U_t,t+2 by row parity
   if (st->skip) {
     for (int i = tid; i < NSOL; i += BS_THREADS)
       if (out) out[i] = 0.0;
@@ -654,30 +656,37 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(const double* M, cons
     lds_barrier();  // ring filled
     lds_barrier();  // row n prepared
     const double m1 = -1.0;
-    double xnext = 0.0;
+    double s = scb[n & 1][0];  // row n: no terms
+    double ud = scb[n & 1][1];
+    double xn1 = 0.0;  // x_t+1 (none for t = n)
 #pragma unroll
     for (int mb = BS_BLK - 1; mb >= 0; mb--) {
-      // rows t whose products start in block mb: (t + 2) / 16 == mb
-      const int tHi = min(n, 16 * mb + 13), tLo = max(1, 16 * mb - 2);
-      for (int t = tHi; t >= tLo; t--) {
-        const int par = t & 1;
+      // phases whose chain row r has its products from block mb on:
+      // (r + 3) / 16 == mb; phase r finishes row t = r + 1 and runs row r
+      const int rHi = min(n - 1, 16 * mb + 12), rLo = max(0, 16 * mb - 3);
+      for (int r = rHi; r >= rLo; r--) {
+        lds_barrier();
+        const int par = r & 1;
         double P[BS_BLK];
 #pragma unroll
         for (int m = mb; m < BS_BLK; m++) P[m] = Pb[par][16 * m + (lane & 15)];
-        const double ud = scb[par][1], u1 = scb[par][2];
-        double s = scb[par][0];
-        if (t < n) s = s - u1 * xnext;
+        const double y = scb[par][0], udr = scb[par][1], u1 = scb[par][2], u2 = scb[par][3];
+        const double xt = s / ud;  // x_{r+1}
+        if (lane == 0) xs[r + 1] = xt;
+        if (r >= 1) {
+          // row r: j = r+1, r+2 here, then the staged products in j order
+          s = y - u1 * xt;
+          s = s - u2 * xn1;
 #pragma unroll
-        for (int m = mb; m < BS_BLK; m++) {
-          const double p = P[m];
-          XB_CHAIN(0) XB_CHAIN(1) XB_CHAIN(2) XB_CHAIN(3) XB_CHAIN(4) XB_CHAIN(5)
-          XB_CHAIN(6) XB_CHAIN(7) XB_CHAIN(8) XB_CHAIN(9) XB_CHAIN(10) XB_CHAIN(11)
-          XB_CHAIN(12) XB_CHAIN(13) XB_CHAIN(14) XB_CHAIN(15)
+          for (int m = mb; m < BS_BLK; m++) {
+            const double p = P[m];
+            XB_CHAIN(0) XB_CHAIN(1) XB_CHAIN(2) XB_CHAIN(3) XB_CHAIN(4) XB_CHAIN(5)
+            XB_CHAIN(6) XB_CHAIN(7) XB_CHAIN(8) XB_CHAIN(9) XB_CHAIN(10) XB_CHAIN(11)
+            XB_CHAIN(12) XB_CHAIN(13) XB_CHAIN(14) XB_CHAIN(15)
+          }
+          ud = udr;
         }
-        const double xt = s / ud;
-        xnext = xt;
-        if (lane == 0) xs[t] = xt;
-        lds_barrier();
+        xn1 = xt;
       }
     }
   } else {
@@ -700,18 +709,20 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(const double* M, cons
         if (r >= 1 && j < BS_COLS) ring[r % BS_RING][j] = (j >= r && j <= n) ? v[h] : 0.0;
       }
     };
-    // products and scalars of row r (r >= 1), from its staged U row
+    // products (j >= r+3) and scalars of row r (r >= 1), from its staged U
+    // row; x_j for j >= r+3 is published by then
     auto prep = [&](int r) {
       const double* U = ring[r % BS_RING];
 #pragma unroll
       for (int h = 0; h < BS_PER; h++) {
         const int j = pl + BS_PL * h;
-        if (j < BS_COLS) Pb[r & 1][j] = (j >= r + 2 && j <= n) ? U[j] * xs[j] : 0.0;
+        if (j < BS_COLS) Pb[r & 1][j] = (j >= r + 3 && j <= n) ? U[j] * xs[j] : 0.0;
       }
       if (pl == 0) {
         scb[r & 1][0] = xs[r];
         scb[r & 1][1] = U[r];
-        scb[r & 1][2] = r < n ? U[r + 1] : 0.0;
+        scb[r & 1][2] = r + 1 <= n ? U[r + 1] : 0.0;
+        scb[r & 1][3] = r + 2 <= n ? U[r + 2] : 0.0;
       }
     };
     double ld[4][BS_PER];  // row r in flight in set r % 4
@@ -727,10 +738,13 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(const double* M, cons
     lds_barrier();
     prep(n);
     lds_barrier();
+    // Phase k (between barriers 1 + k and 2 + k) prepares row t - 1 = n-1-k,
+    // which wave 0 runs in the next phase; its products need x_j for
+    // j >= n-k+2, all written by wave 0 before barrier 1 + k.
     for (int c = 0; c < n / 4; c++) {
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int t = n - 4 * c - u;  // wave 0's row in this phase
+        const int t = n - 4 * c - u;
         const int S = (n - u) & 3;  // == t % 4, a constant after unrolling
         if (t - 1 >= 1) prep(t - 1);
         // vmcnt(6): the set loaded four phases ago (expcnt, lgkmcnt not waited)
@@ -742,6 +756,7 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(const double* M, cons
       }
     }
   }
+  lds_barrier();  // x_1, written after the last phase barrier
   for (int i = tid; i < NSOL; i += BS_THREADS) {
     b[i] = xs[i];
     if (out) out[i] = xs[i];
