@@ -54,9 +54,10 @@ def test_batch_host_equals_single_runs(pairs, singles):
     p = xbach.Params.default()
     with xbach.Batch(0, workers=4) as b:
         outs, res = b.run(p, pairs, check=False)
+        msg = b._L.xb_batch_last_error(b._h)
     for i, (conv, diff, ks, ct, it) in enumerate(singles):
         r = res[i]
-        assert r.status == 0, (i, r.status)
+        assert r.status == 0, (i, r.status, msg)
         assert _same(outs[i][0], conv) and _same(outs[i][1], diff), i
         assert (r.kernSum, r.convTemplate, r.kscIterations) == (ks, ct, it), i
     assert res[-1].status == NO_SUBSTAMPS

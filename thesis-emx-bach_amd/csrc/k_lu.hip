@@ -175,22 +175,32 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
   }
   int maxI = lastMaxI;
   bool fb = false;
+  // The branch-free part of column q's pivot vote (max vv*|a| over positions
+  // >= c1 + q, ties to the larger position): each lane's best candidate and
+  // the wave maximum.  Column q + 1's vote runs as soon as that column has
+  // its last update, before the rest of column q's updates, which then fill
+  // the DPP chain's latency; every element still gets its updates in column
+  // order.
+  double bv, big;
+  int bi;
+  auto vote = [&](const int q) __attribute__((always_inline)) {
+    const int k = c1 + q;
+    bv = -1.0;
+    bi = -1;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const double dum = vr[s] * fabs(a[s][q]);
+      const bool take = lidx[s] >= k && dum >= 0.0 && (dum > bv || (dum == bv && lidx[s] > bi));
+      bv = take ? dum : bv;
+      bi = take ? lidx[s] : bi;
+    }
+    big = wave_max_f64(bv);
+  };
+  vote(0);
 #pragma unroll
   for (int q = 0; q < LB; q++) {
     const int k = c1 + q;
     if (q < cB && !fb) {
-      // pivot search: max vv*|a| over positions >= k, ties to the larger position
-      double bv = -1.0;
-      int bi = -1;
-#pragma unroll
-      for (int s = 0; s < NS; s++) {
-        const double dum = vr[s] * fabs(a[s][q]);
-        const bool take =
-            lidx[s] >= k && dum >= 0.0 && (dum > bv || (dum == bv && lidx[s] > bi));
-        bv = take ? dum : bv;
-        bi = take ? lidx[s] : bi;
-      }
-      const double big = wave_max_f64(bv);
       const bool tie = bi >= 0 && bv == big;
       const unsigned long long cand = __ballot(tie);
       int best = -1;
@@ -234,10 +244,15 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
           const double dinv = 1.0 / u[q];
 #pragma unroll
           for (int s = 0; s < NS; s++) a[s][q] *= dinv;
+          if (q + 1 < LB) {
 #pragma unroll
-          for (int l = q + 1; l < LB; l++) {
+            for (int s = 0; s < NS; s++) a[s][q + 1] = a[s][q + 1] - a[s][q] * u[q + 1];
+            vote(q + 1);
 #pragma unroll
-            for (int s = 0; s < NS; s++) a[s][l] = a[s][l] - a[s][q] * u[l];
+            for (int l = q + 2; l < LB; l++) {
+#pragma unroll
+              for (int s = 0; s < NS; s++) a[s][l] = a[s][l] - a[s][q] * u[l];
+            }
           }
         }
       }
