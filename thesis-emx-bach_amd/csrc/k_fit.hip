@@ -102,63 +102,64 @@ __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall
     __builtin_amdgcn_wave_barrier();
     __asm__ __volatile__("" ::: "memory");
   };
-  int maxI = 0;
-  for (int j = 1; j <= n; j++) {
-    // pivot: vv[i]*|a_ij| over i in [j, n], one row per lane (n < 64), the
-    // last row attaining the maximum (ties to the larger row); NaN never wins
-    double bv = -1.0;
-    int bi = -1;
-    {
-      const int i = j + tid;
-      if (i <= n) {
-        const double dum = vv[i] * fabs(A[i][j]);
+  // ludcmp in registers: lane l holds row l (columns 1..n) and its vv.  A
+  // row interchange relabels the two rows (position numbers) instead of
+  // moving them, vv staying with its row (NR's vv[imax] = vv[j] only touches
+  // positions that are never searched again); the column update is masked to
+  // the rows still below the pivot, so a pivoted row keeps its values and
+  // even a swap with an already factorised row (no candidate, the previous
+  // pivot kept) is the reference's.  Per column: the DPP vote over the
+  // labels >= j (ties to the larger, NaN never), the pivot row's columns
+  // from its lane by v_readlane, then scaling and the rank-1 update with the
+  // reference's rounding (multiply, then subtract).
+  {
+    static_assert(QN <= WAVE, "one row per lane");
+    // lane 0 holds row 0 (zeros, never a candidate): the reference swaps
+    // with it when the first column has no qualifying row (imax still 0)
+    double x[QN];
+#pragma unroll
+    for (int c = 0; c < QN; c++) x[c] = tid <= n ? A[tid][c] : 0.0;
+    const double vr = (tid >= 1 && tid <= n) ? vv[tid] : 0.0;
+    int lab = tid <= n ? tid : -1;
+    int maxI = 0;
+#pragma unroll
+    for (int j = 1; j <= n; j++) {
+      double bv = -1.0;
+      int bi = -1;
+      if (lab >= j) {
+        const double dum = vr * fabs(x[j]);
         if (dum >= 0.0) {
           bv = dum;
-          bi = i;
+          bi = lab;
+        }
+      }
+      const int best = stamp_argmax_last(bv, bi);
+      if (best >= 0) maxI = best;
+      // the lane holding position maxI becomes position j
+      const int src = __builtin_ctzll(__ballot(lab == maxI));
+      if (j != maxI) lab = lab == j ? maxI : (lab == maxI ? j : lab);
+      if (tid == 0) index[j] = maxI;
+      double piv = readlane_f64_at(x[j], src);
+      if (piv == 0.0) {
+        piv = 1.0e-20;
+        if (tid == src) x[j] = 1.0e-20;
+      }
+      if (j != n) {
+        const double dinv = 1.0 / piv;
+        const bool below = lab > j;
+        if (below) x[j] *= dinv;
+#pragma unroll
+        for (int l = j + 1; l <= n; l++) {
+          const double u = readlane_f64_at(x[l], src);
+          if (below) x[l] = x[l] - x[j] * u;
         }
       }
     }
-    const int best = stamp_argmax_last(bv, bi);
-    if (best >= 0) maxI = best;
-    if (j != maxI) {
-      for (int k = 1 + tid; k <= n; k += blockDim.x) {
-        double t = A[maxI][k];
-        A[maxI][k] = A[j][k];
-        A[j][k] = t;
-      }
-      if (tid == 0) vv[maxI] = vv[j];
-    }
-    if (tid == 0) {
-      index[j] = maxI;
-      if (A[j][j] == 0.0) A[j][j] = 1.0e-20;
-    }
+    // rows to their final positions for the substitution
     fence();
-    if (j != n) {
-      const double dum = 1.0 / A[j][j];
-      {
-        const int i = j + 1 + tid;
-        if (i <= n) A[i][j] *= dum;
-      }
-      fence();
-      // threads across the columns, rows in batches of 8 with every read of
-      // a batch issued before its writes (each element still gets the
-      // column-j update after the earlier columns')
-      const int l = j + 1 + tid;
-      if (l <= n) {
-        const double ujl = A[j][l];
-        int i = j + 1;
-        for (; i + 8 <= n + 1; i += 8) {
-          double aij[8], ail[8];
+    if (lab >= 1) {
 #pragma unroll
-          for (int u = 0; u < 8; u++) {
-            aij[u] = A[i + u][j];
-            ail[u] = A[i + u][l];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; u++) A[i + u][l] = ail[u] - aij[u] * ujl;
-        }
-        for (; i <= n; i++) A[i][l] = A[i][l] - A[i][j] * ujl;
-      }
+      for (int c = 1; c <= n; c++) A[lab][c] = x[c];
     }
     fence();
   }
