@@ -212,7 +212,7 @@ __device__ __forceinline__ void wave_panel_factor(const double (*S)[LDP], int c1
       if (maxI < k) {
         // No row qualified and the kept pivot row lies above this column:
         // the reference swaps with an already-factorised row, which only a
-        // literal replay reproduces (k_lu_fallback).
+        // literal replay reproduces (at the start of k_lu_backsub).
         fb = true;
       } else {
         double u[LB];
@@ -395,7 +395,12 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
   const int cB = min(LB, n - c0 + 1);
   const int R = n - c0 + 1;  // rows c0..n
   const bool rhsBlock = (int)blockIdx.x == nColBlocks;
-  const int l0 = c0 + cB + (int)blockIdx.x * LB;  // first column of this block
+  // Workgroups go to the 8 XCDs round robin.  Column block 1 (the panel after
+  // next) is updated by workgroup 8, on workgroup 0's XCD, so the next step's
+  // workgroup 0 finds those columns in its own L2; workgroup 1 takes block 8.
+  int cbk = (int)blockIdx.x;
+  if (nColBlocks > 8 && (cbk == 1 || cbk == 8)) cbk = 9 - cbk;
+  const int l0 = c0 + cB + cbk * LB;  // first column of this block
   const int nc = rhsBlock ? 0 : min(LB, n - l0 + 1);
   __shared__ double Lp[MATSIZE][LDP];  // the factorised panel, rows c0..n
   __shared__ double T[MATSIZE][LDP];   // this block's columns, rows c0..n
@@ -539,15 +544,14 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
 }
 
 // Literal replay of ludcmp (src/bachUtil.cpp:319-384) on the saved matrix,
-// one thread, only after a column had no finite pivot candidate.
-__global__ __launch_bounds__(64) void k_lu_fallback(double* M, const double* M0, int* piv,
-                                                    LuState* st) {
-  if (!st->fallback || st->skip || st->zero) return;
+// only after a column had no finite pivot candidate: the back substitution's
+// workgroup copies the matrix back, then its thread 0 replays.
+__device__ void lu_fallback_replay(double* M, const double* M0, int* piv) {
   const int n = MATSIZE, N = MATN;
   for (int i = threadIdx.x; i < N * N; i += blockDim.x) M[i] = M0[i];
   __syncthreads();
+  __shared__ double vv[MATN];  // LDS, not per-thread scratch
   if (threadIdx.x != 0) return;
-  double vv[MATN];
   for (int i = 0; i < N; i++) piv[i] = 0;
   vv[0] = 0.0;
   for (int i = 1; i <= n; i++) {
@@ -633,10 +637,11 @@ __device__ __forceinline__ void lds_barrier() {
                        : "+v"(s)                                                      \
                        : "v"(p), "v"(m1));
 
-__global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(const double* M, const double* M0,
-                                                           const int* piv, double* b,
+__global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(double* Mw, const double* M0,
+                                                           int* piv, double* b,
                                                            const double* b0, const LuState* st,
                                                            double* out) {
+  const double* M = Mw;
   const int n = MATSIZE, N = MATN;
   const int tid = threadIdx.x, lane = lane_id();
   __shared__ double xs[NSOL];                // y, replaced by x as rows finish
@@ -647,6 +652,10 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(const double* M, cons
     for (int i = tid; i < NSOL; i += BS_THREADS)
       if (out) out[i] = 0.0;
     return;
+  }
+  if (st->fallback && !st->zero) {
+    lu_fallback_replay(Mw, M0, piv);
+    __syncthreads();
   }
   const bool full = st->zero || st->fallback;  // forward pass not fused
   if (st->zero) M = M0;                         // ludcmp returned before factorising
@@ -799,9 +808,6 @@ static hipError_t enqueue_solve(hipStream_t s, const LuSet& L, double* d_out,
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_lu_fallback, dim3(1), dim3(64), 0, s, L.M, L.M0, L.luIndex, st);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(BS_THREADS), 0, s, L.M, L.M0, L.luIndex, L.rhs,
                      L.rhs0, st, d_out);
   return hipGetLastError();
