@@ -839,8 +839,16 @@ hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int*
         if (graph) (void)hipGraphDestroy(graph);
         if (e == hipSuccess) c.solveGraphs.push_back({L.M, d_out, d_skipIfZero, g});
       }
+      if (e != hipSuccess) {
+        // The capture failed (for instance invalidated by another thread's
+        // work on the device): nothing of it ran, so this solve's kernels are
+        // launched directly; a later call captures again.
+        g = nullptr;
+        (void)hipGetLastError();
+        e = enqueue_solve(c.cur, L, d_out, d_skipIfZero);
+      }
     }
-    if (e == hipSuccess) e = hipGraphLaunch(g, c.cur);
+    if (e == hipSuccess && g) e = hipGraphLaunch(g, c.cur);
   }
   prof_end(c, XB_PROF_SOLVE);
   return e;
