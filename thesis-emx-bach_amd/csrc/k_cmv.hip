@@ -12,8 +12,9 @@
 // sums in that order), so both consumers read contiguous rows.
 //
 // k_fill_gram: the 52x52 Gram matrix G = W^T W of the final rows, one wave
-// per 16x16 tile, staged through LDS in 64-pixel chunks, each entry a
-// sequential sum over the 961 pixels (cmv.cl:195-215 order).  The next chunk
+// per 16x16 tile of rows/columns 0..47 (64-pixel chunks staged through LDS)
+// and one per strip of rows 48..51 (32-pixel chunks of all rows), each entry
+// a sequential sum over the 961 pixels (cmv.cl:195-215 order).  The next chunk
 // is loaded raw (its even-row subtraction happens when it is stored to LDS,
 // so no wait sits before the products), and the products read LDS one 8-step
 // block ahead (a sched_barrier keeps the reads early): a refill of a few
@@ -221,15 +222,84 @@ __global__ __launch_bounds__(256) void k_fill_target(FillArgs A) {
   if (b >= 0 && b < NBG) A.Br[(size_t)s * NBG + b] = a2;
 }
 
-constexpr int NGT = 10;  // lower-triangular 16x16 tiles of the 52x52 Gram matrix
-constexpr int GP = GT + 2;  // LDS pitch: [pixel][row], 16-byte aligned row pairs
+// Per stamp, the lower triangle of the 52x52 Gram matrix is split into NGT
+// work items of equal length (961 pixel steps each): the 6 lower-triangular
+// 16x16 tiles of rows/columns 0..47, and one strip holding rows 48..51 against
+// columns 0..51.
+constexpr int NGT = 7;
+constexpr int GP = GT + 2;     // tile LDS pitch: [pixel][row], 16-byte aligned row pairs
+constexpr int SR0 = 3 * GT;    // first strip row
+constexpr int SK = 32;         // strip pixel chunk
+constexpr int SH = WROWS / 2;  // W rows fetched by each half of the wave
+constexpr int SP = WROWS + 2;  // strip LDS pitch: [pixel][row] over all W rows
+static_assert(SR0 + 4 == WROWS, "the strip holds the last four rows");
+static_assert(SK * SP <= 2 * GK * GP, "a strip chunk fits the tile LDS");
+static_assert(SP % 2 == 0, "row pairs stay 16-byte aligned");
 
-// One wave per (stamp, tile).  Blocks b and b + 8 run on the same XCD (blocks
-// are dealt to the 8 XCDs in turn), so each XCD receives a contiguous range of
-// (stamp, tile) work and a stamp's W is read through a single L2.  Lane
-// (lr, lc) = (lane / 8, lane % 8) owns the 2x2 entries rows bi*16 + 2lr + i,
-// columns bj*16 + 2lc + j.  Pixel chunks are double-buffered: the next chunk
-// is loaded into registers while the current one is summed from LDS.
+// Adds 8*NB pixel steps to a lane's 2x2 block: the row pair at xb and the
+// column pair at yb, consecutive pixels P doubles apart in LDS.  Block j + 1's
+// LDS reads are issued before block j is computed (two register sets, the
+// index resolved by the unrolling), so the reads overlap the products.  Every
+// entry adds its products in ascending pixel order.
+template <int P, int NB>
+__device__ __forceinline__ void gram_block(const double* xb, const double* yb, double& c00,
+                                           double& c01, double& c10, double& c11) {
+  double2 xa[2][8], ya[2][8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    xa[0][j] = *reinterpret_cast<const double2*>(xb + j * P);
+    ya[0][j] = *reinterpret_cast<const double2*>(yb + j * P);
+  }
+#pragma unroll
+  for (int blk = 0; blk < NB; blk++) {
+    const int cu = blk & 1, nx = cu ^ 1;
+    if (blk + 1 < NB) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        xa[nx][j] = *reinterpret_cast<const double2*>(xb + (8 * (blk + 1) + j) * P);
+        ya[nx][j] = *reinterpret_cast<const double2*>(yb + (8 * (blk + 1) + j) * P);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the next block's reads here
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const double2 x = xa[cu][j], y = ya[cu][j];
+      const double p00 = x.x * y.x, p01 = x.x * y.y, p10 = x.y * y.x, p11 = x.y * y.y;
+      c00 += p00;
+      c01 += p01;
+      c10 += p10;
+      c11 += p11;
+    }
+  }
+}
+
+// The same for a partial chunk of kn steps.
+template <int P>
+__device__ __forceinline__ void gram_steps(const double* xb, const double* yb, int kn,
+                                           double& c00, double& c01, double& c10, double& c11) {
+  for (int kk = 0; kk < kn; kk++) {
+    const double2 x = *reinterpret_cast<const double2*>(xb + kk * P);
+    const double2 y = *reinterpret_cast<const double2*>(yb + kk * P);
+    const double p00 = x.x * y.x, p01 = x.x * y.y, p10 = x.y * y.x, p11 = x.y * y.y;
+    c00 += p00;
+    c01 += p01;
+    c10 += p10;
+    c11 += p11;
+  }
+}
+
+// One wave per (stamp, work item).  Blocks b and b + 8 run on the same XCD
+// (blocks are dealt to the 8 XCDs in turn), so each XCD receives a contiguous
+// range of work items and a stamp's W is read through a single L2.
+//   tile (bi, bj): lane (lr, lc) = (lane / 8, lane % 8) owns rows
+//     bi*16 + 2lr + i, columns bj*16 + 2lc + j; 64-pixel chunks of 16 + 16 rows.
+//   strip: lane (rp, cp) = (lane / 26, lane % 26), lanes 0..51, owns rows
+//     48 + 2rp + i, columns 2cp + j; 32-pixel chunks of all 52 rows, fetched
+//     by lane (pixel, half) = (lane % 32, lane / 32), 26 rows per half.
+// Rows of even basis terms have W row 0 subtracted as they are stored to LDS.
+// The next pixel chunk is loaded into registers while the current one is
+// summed from LDS.  One LDS buffer suffices (a single wave's LDS reads and
+// writes run in program order), which leaves room for two waves per SIMD.
 __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
   const int total = nMax * NGT;
   const int per = (total + 7) / 8;
@@ -240,112 +310,134 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
   const int g = A.kept[s];
   const bool valid = A.cur[s] < A.count[g];
   const int lane = threadIdx.x;
-  int t = wk % NGT, bi = 0;
-  while (t > bi) { t -= bi + 1; bi++; }
-  const int bj = t;
+  const int item = wk % NGT;
+  const bool strip = item == NGT - 1;
+  int bi = 0, bj = item;
+  while (bj > bi) { bj -= bi + 1; bi++; }
+  double* G = A.G + (size_t)s * WROWS * WROWS;
   if (!valid) {
-    for (int e = lane; e < GT * GT; e += WAVE) {
-      const int row = bi * GT + e / GT, col = bj * GT + e % GT;
-      if (row < WROWS && col <= row) A.G[((size_t)s * WROWS + row) * WROWS + col] = 0.0;
+    if (strip) {
+      for (int e = lane; e < 4 * WROWS; e += WAVE) {
+        const int row = SR0 + e / WROWS, col = e % WROWS;
+        if (col <= row) G[row * WROWS + col] = 0.0;
+      }
+    } else {
+      for (int e = lane; e < GT * GT; e += WAVE) {
+        const int row = bi * GT + e / GT, col = bj * GT + e % GT;
+        if (col <= row) G[row * WROWS + col] = 0.0;
+      }
     }
     return;
   }
   const double* W = A.W + (size_t)s * WROWS * NPIX;
   const int* evenSub = A.basis->evenSub;
-  __shared__ __align__(16) double Wa[2][GK][GP];
-  __shared__ __align__(16) double Wb[2][GK][GP];
+  __shared__ __align__(16) double Wl[2 * GK * GP];
+  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+  if (strip) {
+    double(*Ws)[SP] = reinterpret_cast<double(*)[SP]>(Wl);
+    const int px = lane % SK, h = lane / SK;
+    unsigned long long sub = 0;  // bit r: row h*SH + r is an even basis term
+    for (int r = 0; r < SH; r++) {
+      const int row = h * SH + r;
+      if (row < NPSF && evenSub[row]) sub |= 1ull << r;
+    }
+    double v[SH], w0;
+    // raw loads only (pixels past the window read a clamped address and are
+    // zeroed at store time)
+    auto fetch = [&](int k0) {
+      const int k = min(k0 + px, NPIX - 1);
+      w0 = W[k];
+#pragma unroll
+      for (int r = 0; r < SH; r++) v[r] = W[(size_t)(h * SH + r) * NPIX + k];
+    };
+    auto store = [&](int k0) {
+      const bool in = k0 + px < NPIX;
+#pragma unroll
+      for (int r = 0; r < SH; r++) {
+        double x = in ? v[r] : 0.0;
+        if ((sub >> r) & 1) x = x - (in ? w0 : 0.0);
+        Ws[px][h * SH + r] = x;
+      }
+    };
+    const int rp = (lane / SH) & 1, cp = lane % SH;
+    const double* xb = &Ws[0][SR0 + 2 * rp];
+    const double* yb = &Ws[0][2 * cp];
+    fetch(0);
+    store(0);
+    __syncthreads();
+    for (int k0 = 0; k0 < NPIX; k0 += SK) {
+      const int kn = min(SK, NPIX - k0);
+      const bool more = k0 + SK < NPIX;
+      if (more) fetch(k0 + SK);
+      if (kn == SK)
+        gram_block<SP, SK / 8>(xb, yb, c00, c01, c10, c11);
+      else
+        gram_steps<SP>(xb, yb, kn, c00, c01, c10, c11);
+      if (more) {
+        store(k0 + SK);
+        __syncthreads();
+      }
+    }
+    if (lane < 2 * SH) {
+      const int r0 = SR0 + 2 * rp, q0 = 2 * cp;
+      const double cv[2][2] = {{c00, c01}, {c10, c11}};
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          if (q0 + j <= r0 + i) G[(r0 + i) * WROWS + q0 + j] = cv[i][j];
+    }
+    return;
+  }
+  double(*Wa)[GP] = reinterpret_cast<double(*)[GP]>(Wl);
+  double(*Wb)[GP] = reinterpret_cast<double(*)[GP]>(Wl + GK * GP);
   bool subA[GT], subB[GT];
 #pragma unroll
   for (int rr = 0; rr < GT; rr++) {
-    const int ga = bi * GT + rr, gb = bj * GT + rr;
-    subA[rr] = ga < NPSF && evenSub[ga];
-    subB[rr] = gb < NPSF && evenSub[gb];
+    subA[rr] = evenSub[bi * GT + rr] != 0;
+    subB[rr] = evenSub[bj * GT + rr] != 0;
   }
   double va[GT], vb[GT], w0;
-  // raw loads only (rows past W and pixels past the window read clamped
-  // addresses and are zeroed at store time)
+  // raw loads only (pixels past the window read a clamped address and are
+  // zeroed at store time)
   auto fetch = [&](int k0) {
     const int k = min(k0 + lane, NPIX - 1);
     w0 = W[k];
 #pragma unroll
     for (int rr = 0; rr < GT; rr++) {
-      const int ga = min(bi * GT + rr, WROWS - 1), gb = min(bj * GT + rr, WROWS - 1);
-      va[rr] = W[(size_t)ga * NPIX + k];
-      vb[rr] = W[(size_t)gb * NPIX + k];
+      va[rr] = W[(size_t)(bi * GT + rr) * NPIX + k];
+      vb[rr] = W[(size_t)(bj * GT + rr) * NPIX + k];
     }
   };
-  auto store = [&](int buf, int k0) {
+  auto store = [&](int k0) {
     const bool in = k0 + lane < NPIX;
 #pragma unroll
     for (int rr = 0; rr < GT; rr++) {
-      const int ga = bi * GT + rr, gb = bj * GT + rr;
-      double x = (in && ga < WROWS) ? va[rr] : 0.0;
-      double y = (in && gb < WROWS) ? vb[rr] : 0.0;
+      double x = in ? va[rr] : 0.0;
+      double y = in ? vb[rr] : 0.0;
       if (subA[rr]) x = x - (in ? w0 : 0.0);
       if (subB[rr]) y = y - (in ? w0 : 0.0);
-      Wa[buf][lane][rr] = x;
-      Wb[buf][lane][rr] = y;
+      Wa[lane][rr] = x;
+      Wb[lane][rr] = y;
     }
   };
   const int lr = lane / 8, lc = lane % 8;
-  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+  const double* xb = &Wa[0][2 * lr];
+  const double* yb = &Wb[0][2 * lc];
   fetch(0);
-  store(0, 0);
+  store(0);
   __syncthreads();
-  int buf = 0;
   for (int k0 = 0; k0 < NPIX; k0 += GK) {
     const int kn = min(GK, NPIX - k0);
     const bool more = k0 + GK < NPIX;
     if (more) fetch(k0 + GK);
-    auto step = [&](int kk) {
-      const double2 x = *reinterpret_cast<const double2*>(&Wa[buf][kk][2 * lr]);
-      const double2 y = *reinterpret_cast<const double2*>(&Wb[buf][kk][2 * lc]);
-      const double p00 = x.x * y.x, p01 = x.x * y.y, p10 = x.y * y.x, p11 = x.y * y.y;
-      c00 += p00;
-      c01 += p01;
-      c10 += p10;
-      c11 += p11;
-    };
-    {
-      if (kn == GK) {
-        // 8 blocks of 8 steps; block j + 1's LDS reads are issued before
-        // block j is computed (two register sets, the index resolved by the
-        // unrolling), so the reads overlap the products
-        double2 xa[2][8], yb[2][8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          xa[0][j] = *reinterpret_cast<const double2*>(&Wa[buf][j][2 * lr]);
-          yb[0][j] = *reinterpret_cast<const double2*>(&Wb[buf][j][2 * lc]);
-        }
-#pragma unroll
-        for (int blk = 0; blk < GK / 8; blk++) {
-          const int cu = blk & 1, nx = cu ^ 1;
-          if (blk + 1 < GK / 8) {
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-              xa[nx][j] = *reinterpret_cast<const double2*>(&Wa[buf][8 * (blk + 1) + j][2 * lr]);
-              yb[nx][j] = *reinterpret_cast<const double2*>(&Wb[buf][8 * (blk + 1) + j][2 * lc]);
-            }
-          }
-          __builtin_amdgcn_sched_barrier(0);  // keep the next block's reads here
-#pragma unroll
-          for (int j = 0; j < 8; j++) {
-            const double2 x = xa[cu][j], y = yb[cu][j];
-            const double p00 = x.x * y.x, p01 = x.x * y.y, p10 = x.y * y.x, p11 = x.y * y.y;
-            c00 += p00;
-            c01 += p01;
-            c10 += p10;
-            c11 += p11;
-          }
-        }
-      } else {
-        for (int kk = 0; kk < kn; kk++) step(kk);
-      }
-    }
+    if (kn == GK)
+      gram_block<GP, GK / 8>(xb, yb, c00, c01, c10, c11);
+    else
+      gram_steps<GP>(xb, yb, kn, c00, c01, c10, c11);
     if (more) {
-      store(buf ^ 1, k0 + GK);
+      store(k0 + GK);
       __syncthreads();
-      buf ^= 1;
     }
   }
   const int r0 = bi * GT + 2 * lr, q0 = bj * GT + 2 * lc;
@@ -353,10 +445,8 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int row = r0 + i, col = q0 + j;
-      if (row < WROWS && col <= row) A.G[((size_t)s * WROWS + row) * WROWS + col] = cv[i][j];
-    }
+    for (int j = 0; j < 2; j++)
+      if (q0 + j <= r0 + i) G[(r0 + i) * WROWS + q0 + j] = cv[i][j];
 }
 
 hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax) {
