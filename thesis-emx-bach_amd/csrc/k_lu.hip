@@ -829,22 +829,37 @@ hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int*
         break;
       }
     if (!g) {
-      e = hipStreamBeginCapture(c.cur, hipStreamCaptureModeThreadLocal);
-      if (e == hipSuccess) {
-        const hipError_t e1 = enqueue_solve(c.cur, L, d_out, d_skipIfZero);
-        hipGraph_t graph = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(c.cur, &graph);
-        e = e1 != hipSuccess ? e1 : e2;
-        if (e == hipSuccess) e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (e == hipSuccess) c.solveGraphs.push_back({L.M, d_out, d_skipIfZero, g});
+      hipGraph_t graph = nullptr;
+      {
+        // no thread allocates or frees while the capture is open
+        std::shared_lock<std::shared_mutex> lk(alloc_lock());
+        e = hipStreamBeginCapture(c.cur, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+          const hipError_t e1 = enqueue_solve(c.cur, L, d_out, d_skipIfZero);
+          const hipError_t e2 = hipStreamEndCapture(c.cur, &graph);
+          e = e1 != hipSuccess ? e1 : e2;
+        }
       }
+      if (e == hipSuccess) {
+        std::unique_lock<std::shared_mutex> lk(alloc_lock());
+        e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+      }
+      if (graph) (void)hipGraphDestroy(graph);
+      if (e == hipSuccess) c.solveGraphs.push_back({L.M, d_out, d_skipIfZero, g});
       if (e != hipSuccess) {
         // The capture failed (for instance invalidated by another thread's
-        // work on the device): nothing of it ran, so this solve's kernels are
-        // launched directly; a later call captures again.
+        // work on the device): nothing of it ran.  A capture still open on
+        // the stream is ended, then this solve's kernels are launched
+        // directly; a later call captures again.
         g = nullptr;
         (void)hipGetLastError();
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(c.cur, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+          hipGraph_t left = nullptr;
+          (void)hipStreamEndCapture(c.cur, &left);
+          if (left) (void)hipGraphDestroy(left);
+          (void)hipGetLastError();
+        }
         e = enqueue_solve(c.cur, L, d_out, d_skipIfZero);
       }
     }

@@ -489,9 +489,9 @@ namespace {
 struct DevBuf {
   void* p = nullptr;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) (void)locked_free(p);
   }
-  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+  hipError_t alloc(size_t bytes) { return locked_malloc(&p, bytes); }
   template <class T>
   T* as() const {
     return static_cast<T*>(p);

@@ -20,6 +20,11 @@
 
 using namespace xb;
 
+std::shared_mutex& xb::alloc_lock() {
+  static std::shared_mutex m;
+  return m;
+}
+
 namespace {
 
 #define XB_HIP(expr)                                                              \
@@ -34,18 +39,18 @@ namespace {
 template <class T>
 hipError_t dalloc(T** p, size_t n) {
   if (*p) {
-    hipError_t e = hipFree(*p);
+    hipError_t e = locked_free(*p);
     if (e != hipSuccess) return e;
     *p = nullptr;
   }
   if (n == 0) n = 1;
-  return hipMalloc((void**)p, n * sizeof(T));
+  return locked_malloc(p, n * sizeof(T));
 }
 
 template <class T>
 void dfree(T** p) {
   if (*p) {
-    (void)hipFree(*p);
+    (void)locked_free(*p);
     *p = nullptr;
   }
 }
@@ -284,16 +289,16 @@ int ensureSolver(Ctx& c) {
   if (c.sol) return XB_OK;
   for (int i = 0; i < 2; i++) {
     LuSet& L = c.lu[i];
-    XB_HIP(hipMalloc((void**)&L.M, sizeof(double) * MATN * MATN));
-    XB_HIP(hipMalloc((void**)&L.rhs, sizeof(double) * NSOL));
-    XB_HIP(hipMalloc((void**)&L.M0, sizeof(double) * MATN * MATN));
-    XB_HIP(hipMalloc((void**)&L.rhs0, sizeof(double) * NSOL));
-    XB_HIP(hipMalloc((void**)&L.luIndex, sizeof(int) * MATN));
-    XB_HIP(hipMalloc((void**)&L.luVV, sizeof(double) * MATN));
-    XB_HIP(hipMalloc((void**)&L.luState, LU_STATE_BYTES));
+    XB_HIP(locked_malloc((void**)&L.M, sizeof(double) * MATN * MATN));
+    XB_HIP(locked_malloc((void**)&L.rhs, sizeof(double) * NSOL));
+    XB_HIP(locked_malloc((void**)&L.M0, sizeof(double) * MATN * MATN));
+    XB_HIP(locked_malloc((void**)&L.rhs0, sizeof(double) * NSOL));
+    XB_HIP(locked_malloc((void**)&L.luIndex, sizeof(int) * MATN));
+    XB_HIP(locked_malloc((void**)&L.luVV, sizeof(double) * MATN));
+    XB_HIP(locked_malloc((void**)&L.luState, LU_STATE_BYTES));
   }
-  XB_HIP(hipMalloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
-  XB_HIP(hipMalloc((void**)&c.sol, sizeof(double) * NSOL));
+  XB_HIP(locked_malloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
+  XB_HIP(locked_malloc((void**)&c.sol, sizeof(double) * NSOL));
   return XB_OK;
 }
 
@@ -341,9 +346,9 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
   c.haveResult = false;
   rc = ensureImage(c, w, h);
   if (rc) return rc;
-  if (!c.basis) XB_HIP(hipMalloc((void**)&c.basis, sizeof(Basis)));
-  if (!c.sc) XB_HIP(hipMalloc((void**)&c.sc, sizeof(DevScalars)));
-  if (!c.hsc) XB_HIP(hipHostMalloc((void**)&c.hsc, sizeof(DevScalars), hipHostMallocDefault));
+  if (!c.basis) XB_HIP(locked_malloc((void**)&c.basis, sizeof(Basis)));
+  if (!c.sc) XB_HIP(locked_malloc((void**)&c.sc, sizeof(DevScalars)));
+  if (!c.hsc) XB_HIP(locked_host_malloc((void**)&c.hsc, sizeof(DevScalars)));
   {
     int rc2 = ensureSolver(c);
     if (rc2) return rc2;
@@ -365,7 +370,7 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
       XB_HIP(hipMemcpyAsync(c.basis, &hb, sizeof(Basis), hipMemcpyHostToDevice, c.stream));
     if (newTab) {
       mixedTable(hb, tab);
-      if (!c.mixTab) XB_HIP(hipMalloc(&c.mixTab, tab.size() * sizeof(uint16_t)));
+      if (!c.mixTab) XB_HIP(locked_malloc(&c.mixTab, tab.size() * sizeof(uint16_t)));
       XB_HIP(hipMemcpyAsync(c.mixTab, tab.data(), tab.size() * sizeof(uint16_t),
                             hipMemcpyHostToDevice, c.stream));
       std::memcpy(c.mixBg, c.p.bg, sizeof(c.mixBg));
@@ -583,7 +588,10 @@ int stageSub(Ctx& c) {
 }
 
 void freeAll(Ctx& c) {
-  for (auto& sg : c.solveGraphs) (void)hipGraphExecDestroy(sg.exec);
+  {
+    std::unique_lock<std::shared_mutex> g(alloc_lock());
+    for (auto& sg : c.solveGraphs) (void)hipGraphExecDestroy(sg.exec);
+  }
   c.solveGraphs.clear();
   dfree(&c.img[0]);
   dfree(&c.img[1]);
@@ -600,7 +608,7 @@ void freeAll(Ctx& c) {
   }
   dfree(&c.sc);
   if (c.hsc) {
-    (void)hipHostFree(c.hsc);
+    (void)locked_host_free(c.hsc);
     c.hsc = nullptr;
   }
   dfree(&c.basis);
@@ -618,7 +626,7 @@ void freeAll(Ctx& c) {
   dfree(&c.scr.flag); dfree(&c.scr.zPrev); dfree(&c.scr.model);
   c.scr.cap = 0;
   c.scr.haveModel = false;
-  if (c.mixTab) (void)hipFree(c.mixTab);
+  if (c.mixTab) (void)locked_free(c.mixTab);
   c.mixTab = nullptr;
 }
 
@@ -834,7 +842,7 @@ int xb_screen_load(xb_ctx* x, const float* model) {
       return XB_ERR_ARG;
     }
   XB_HIP(hipSetDevice(c.device));
-  if (!c.scr.model) XB_HIP(hipMalloc((void**)&c.scr.model, sizeof(float) * XB_MLP_NPARAM));
+  if (!c.scr.model) XB_HIP(locked_malloc((void**)&c.scr.model, sizeof(float) * XB_MLP_NPARAM));
   XB_HIP(hipMemcpy(c.scr.model, model, sizeof(float) * XB_MLP_NPARAM, hipMemcpyHostToDevice));
   c.scr.haveModel = true;
   return XB_OK;
@@ -933,7 +941,7 @@ int xb_buf_alloc(xb_ctx* x, size_t bytes, void** d_out) {
   *d_out = nullptr;
   int rc = setDev(c);
   if (rc) return rc;
-  XB_HIP(hipMalloc(d_out, bytes ? bytes : 1));
+  XB_HIP(locked_malloc(d_out, bytes ? bytes : 1));
   return XB_OK;
 }
 
@@ -942,7 +950,7 @@ int xb_buf_free(xb_ctx* x, void* d) {
   Ctx& c = x->c;
   if (!d) return XB_OK;
   XB_HIP(hipStreamSynchronize(c.stream));
-  XB_HIP(hipFree(d));
+  XB_HIP(locked_free(d));
   return XB_OK;
 }
 
@@ -1031,7 +1039,7 @@ int xb_get_diagnostics(xb_ctx* x, xb_diag* out) {
   const int rc = setDev(c);
   if (rc) return rc;
   double* d = nullptr;
-  XB_HIP(hipMalloc((void**)&d, sizeof(double) * (DIAG_BLOCKS * 10 + 10)));
+  XB_HIP(locked_malloc((void**)&d, sizeof(double) * (DIAG_BLOCKS * 10 + 10)));
   hipError_t e = launch_diag(c, d, d + DIAG_BLOCKS * 10);
   double q[10] = {0};
   DevScalars sc;
@@ -1040,7 +1048,7 @@ int xb_get_diagnostics(xb_ctx* x, xb_diag* out) {
   if (e == hipSuccess)
     e = hipMemcpyAsync(&sc, c.sc, sizeof(sc), hipMemcpyDeviceToHost, c.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
-  (void)hipFree(d);
+  (void)locked_free(d);
   XB_HIP(e);
   auto fill = [](const double* s, int64_t& n, double& mean, double& sige, double& sig,
                  double* x2) {

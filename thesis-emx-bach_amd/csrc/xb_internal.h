@@ -11,12 +11,38 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/xbach.h"
 
 namespace xb {
+
+// A HIP allocation or release made while another thread has a stream capture
+// open can invalidate that capture and leave its stream failing every later
+// call (hipFree synchronizes the device).  Every allocation, release and
+// graph instantiation takes this lock exclusively; an open capture holds it
+// shared, so the captures of different contexts still overlap.
+std::shared_mutex& alloc_lock();
+template <class T>
+inline hipError_t locked_malloc(T** p, size_t bytes) {
+  std::unique_lock<std::shared_mutex> g(alloc_lock());
+  return hipMalloc((void**)p, bytes);
+}
+inline hipError_t locked_free(void* p) {
+  std::unique_lock<std::shared_mutex> g(alloc_lock());
+  return hipFree(p);
+}
+inline hipError_t locked_host_malloc(void** p, size_t bytes) {
+  std::unique_lock<std::shared_mutex> g(alloc_lock());
+  return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
+inline hipError_t locked_host_free(void* p) {
+  std::unique_lock<std::shared_mutex> g(alloc_lock());
+  return hipHostFree(p);
+}
 
 constexpr int KW = 21;             // fKernelWidth
 constexpr int HKW = 10;            // hKernelWidth
