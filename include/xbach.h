@@ -313,6 +313,23 @@ int xb_mlp_grad(xb_ctx* ctx, const float* model, const float* X, const float* y,
 int xb_mlp_train(xb_ctx* ctx, float* model, const float* X, const float* y, const float* wt,
                  int n, int epochs, int batch, float lr, uint64_t seed, float* loss);
 
+/* ---- allocation-free gradient ---------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight training loops, harnesses that
+ * forbid hipMalloc): xb_mlp_grad on DEVICE-resident arrays (d_model
+ * [XB_MLP_NPARAM], d_X [n][32], d_y [n], d_wt [n] in, d_grad [XB_MLP_NGRAD + 1]
+ * out, the loss last), with the per-block partial sums in a static per-device
+ * array of the library (about 2.2 MB, held by each GPU the library runs on
+ * until the process ends) instead of an allocation.  It makes no hipMalloc,
+ * hipFree or other allocating call; n above XB_MLP_NOALLOC_MAX is refused with
+ * XB_ERR_CAPACITY.  The result equals xb_mlp_grad's bit for bit.  The work is
+ * enqueued on the context's primary stream and the call returns without
+ * waiting (xb_sync waits).  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (two contexts, two threads) race. */
+#define XB_MLP_NOALLOC_MAX 16384
+int xb_mlp_grad_noalloc(xb_ctx* ctx, const float* d_model, const float* d_X, const float* d_y,
+                        const float* d_wt, int n, float* d_grad);
+
 /* xb_ctx_create flag: run both images' independent work on the one primary
  * stream instead of forking a second stream (used by the batch runtime, where
  * the concurrency comes from several pairs in flight). */

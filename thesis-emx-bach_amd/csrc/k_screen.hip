@@ -397,6 +397,15 @@ __global__ __launch_bounds__(WAVE) void k_mlp_grad(const float* __restrict__ P,
   }
 }
 
+// Block partials of the allocation-free gradient (xb_mlp_grad_noalloc): one
+// row of MLP_NGRAD + 1 floats per gradient block, so at most
+// XB_MLP_NOALLOC_MAX / MLP_CHUNK blocks (2.2 MB per device, held until the
+// process ends).  Host code never passes this name to a kernel: it uses the
+// per-device address from hipGetSymbolAddress (scratch_address below).
+constexpr int MLP_SCRATCH_BLOCKS = XB_MLP_NOALLOC_MAX / MLP_CHUNK;
+static_assert(XB_MLP_NOALLOC_MAX % MLP_CHUNK == 0, "whole blocks");
+static __device__ __align__(256) float scratch[(size_t)MLP_SCRATCH_BLOCKS * (MLP_NGRAD + 1)];
+
 // sum of the block partials in block order (deterministic); out[MLP_NGRAD]
 // receives the loss
 __global__ void k_mlp_reduce(const float* __restrict__ part, int nb, float* __restrict__ out) {
@@ -475,6 +484,34 @@ hipError_t launch_screen_labels(Ctx& c, int im, int nRec) {
   hipLaunchKernelGGL(k_screen_labels, dim3((N * nRec + 255) / 256), dim3(256), 0, c.cur, N,
                      nRec, c.scr.passCur, c.st[im].cur, c.scr.label);
   return hipGetLastError();
+}
+
+// Device address of `scratch` on `device`, looked up once per device with that
+// device current (each GPU holds its own copy of the array at its own
+// address).  Called by xb_ctx_create, so the allocation-free entry point only
+// reads the cached value.
+namespace {
+std::mutex g_scratchMu;
+float* g_scratch[XB_MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t scratch_address(int device, float** out) {
+  *out = nullptr;
+  if (device < 0 || device >= XB_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_scratchMu);
+  if (!g_scratch[device]) {
+    int prev = 0;
+    hipError_t e = hipGetDevice(&prev);
+    if (e != hipSuccess) return e;
+    if (prev != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev != device) (void)hipSetDevice(prev);
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<float*>(p);
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
 }
 
 }  // namespace xb
@@ -562,6 +599,35 @@ int xb_mlp_grad(xb_ctx* x, const float* params, const float* X, const float* y,
   MLP_HIP(hipMemcpyAsync(grad, dg.p, sizeof(float) * (MLP_NGRAD + 1), hipMemcpyDeviceToHost,
                          c.stream));
   MLP_HIP(hipStreamSynchronize(c.stream));
+  return XB_OK;
+}
+
+// xb_mlp_grad on device-resident arrays, with the block partials in the
+// library's static per-device `scratch` instead of an allocation: no hipMalloc,
+// hipFree or other allocating call.  Same kernels, same block order, so the
+// result equals xb_mlp_grad's bit for bit.  Enqueued on the context's primary
+// stream; the caller waits with xb_sync.
+int xb_mlp_grad_noalloc(xb_ctx* x, const float* d_params, const float* d_X, const float* d_y,
+                        const float* d_wt, int n, float* d_grad) {
+  if (!x || !d_params || !d_X || !d_y || !d_wt || !d_grad || n <= 0) return XB_ERR_ARG;
+  Ctx& c = x->c;
+  if (n > XB_MLP_NOALLOC_MAX) {
+    c.err = "xb_mlp_grad_noalloc: n = " + std::to_string(n) + " exceeds the maximum of " +
+            std::to_string(XB_MLP_NOALLOC_MAX) + " samples";
+    return XB_ERR_CAPACITY;
+  }
+  if (!c.mlpScratch) {
+    c.err = "xb_mlp_grad_noalloc: no scratch address for this device";
+    return XB_ERR_HIP;
+  }
+  MLP_HIP(hipSetDevice(c.device));
+  const int nb = (n + MLP_CHUNK - 1) / MLP_CHUNK;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(WAVE), 0, c.stream, d_params, d_X, d_y, d_wt,
+                     n, 1.0f / n, c.mlpScratch);
+  MLP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NGRAD + 256) / 256), dim3(256), 0, c.stream,
+                     c.mlpScratch, nb, d_grad);
+  MLP_HIP(hipGetLastError());
   return XB_OK;
 }
 

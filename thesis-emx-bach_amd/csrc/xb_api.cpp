@@ -763,6 +763,10 @@ int xb_ctx_create(int device, int flags, xb_ctx** out) {
   }
   c.cur = c.stream;
   c.eagerSolve = (flags & XB_CTX_EAGER) != 0;
+  // the allocation-free gradient's scratch: its address on this device is
+  // looked up here, once per device, so that entry point makes no HIP call
+  // beyond its launches
+  if (scratch_address(device, &c.mlpScratch) != hipSuccess) c.mlpScratch = nullptr;
   *out = x;
   return XB_OK;
 }

@@ -223,6 +223,9 @@ struct Ctx {
   hipEvent_t evOpen[XB_PROF_N] = {};
   double profMs[XB_PROF_N] = {0};
   int profCnt[XB_PROF_N] = {0};
+  // device address of the static gradient scratch of xb_mlp_grad_noalloc on
+  // this context's device (looked up by xb_ctx_create)
+  float* mlpScratch = nullptr;
 };
 
 // Bracket a launch group with HIP events on the context's stream (no-ops
@@ -257,6 +260,10 @@ constexpr int DIAG_BLOCKS = 512;
 // apply, the network's decision merged into the pass's refill list
 hipError_t launch_screen(Ctx& c, int im, int pass, bool apply);
 hipError_t launch_screen_labels(Ctx& c, int im, int nRec);
+// Device address of the static gradient scratch on `device` (k_screen.hip);
+// looked up with hipGetSymbolAddress once per device and cached.
+constexpr int XB_MAX_DEVICES = 64;
+hipError_t scratch_address(int device, float** out);
 
 }  // namespace xb
 

@@ -342,6 +342,7 @@ def lib() -> C.CDLL:
     L.xb_screen_samples.argtypes = [vp, vp, vp, vp, C.c_int, P(C.c_int), P(C.c_int)]
     L.xb_mlp_forward.argtypes = [vp, vp, vp, C.c_int, vp]
     L.xb_mlp_grad.argtypes = [vp, vp, vp, vp, vp, C.c_int, vp]
+    L.xb_mlp_grad_noalloc.argtypes = [vp, vp, vp, vp, vp, C.c_int, vp]
     L.xb_mlp_train.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float,
                                C.c_uint64, vp]
     L.xb_batch_set_form.argtypes = [vp, C.c_int]
@@ -390,6 +391,7 @@ CTX_EAGER = 2  # global solves launched kernel by kernel instead of replayed hip
 FORM_PLAIN, FORM_MIXED, FORM_SCREEN, FORM_RECORD = 0, 1, 2, 4
 # the screen's network (include/xbach.h XB_MLP_*)
 MLP_NFEAT, MLP_NPARAM, MLP_NGRAD = 32, 2242, 2177
+MLP_NOALLOC_MAX = 16384  # XB_MLP_NOALLOC_MAX
 
 # per-kernel timing slots (include/xbach.h XB_PROF_*)
 PROF_SLOTS = ["conv", "sub", "solve", "fill", "sss", "mask"]
@@ -614,6 +616,12 @@ class Context:
         self._check(self._L.xb_mlp_grad(self._h, m.ctypes.data, X.ctypes.data, y.ctypes.data,
                                         wt.ctypes.data, len(X), g.ctypes.data), "xb_mlp_grad")
         return g[:MLP_NGRAD], float(g[MLP_NGRAD])
+
+    def mlp_grad_noalloc(self, d_model: int, d_X: int, d_y: int, d_wt: int, n: int,
+                         d_grad: int) -> int:
+        """xb_mlp_grad_noalloc on device pointers: enqueues the gradient and
+        returns its status code (0 = queued; wait with sync())."""
+        return int(self._L.xb_mlp_grad_noalloc(self._h, d_model, d_X, d_y, d_wt, n, d_grad))
 
     def mlp_train(self, model, X, y, wt, epochs=40, batch=1024, lr=3e-3, seed=1):
         m = np.array(model, np.float32)
