@@ -449,7 +449,7 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
       if (q0 + j <= r0 + i) G[(r0 + i) * WROWS + q0 + j] = cv[i][j];
 }
 
-hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax) {
+hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, int nMax) {
   if (nMax <= 0) return hipSuccess;
   const int src = im, tgt = 1 - im;
   FillArgs A;
@@ -458,7 +458,7 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax) {
   A.w = c.w;
   A.h = c.h;
   A.list = d_list;
-  A.nList = d_list ? &c.sc->nList[im] : &c.sc->nKept[im];
+  A.nList = d_list ? d_nList : &c.sc->nKept[im];
   A.kept = c.st[im].kept;
   A.cur = c.st[im].cur;
   A.count = c.st[im].count;

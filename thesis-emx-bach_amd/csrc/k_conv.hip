@@ -260,12 +260,16 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   }
 }
 
-// border pixels: conv = 1e-30, mask keeps its value plus maskAfterConv flags
+// border pixels: conv = 1e-30, mask keeps its value plus maskAfterConv flags.
+// NaN marks held back by a clip-refit loop that ran before the direction was
+// known become XB_MASK_NAN_PIXEL for the chosen direction (pend) and vanish for
+// the other; k_conv rewrites every interior pixel's mask from scratch, so the
+// border is the only place they can remain.
 __global__ __launch_bounds__(256) void k_conv_border(const float* __restrict__ S,
                                                      uint16_t* __restrict__ mask,
                                                      float* __restrict__ out,
                                                      double* __restrict__ outD, int w, int h,
-                                                     double hi, double lo) {
+                                                     double hi, double lo, uint16_t pend) {
   const size_t n = (size_t)w * h;
   for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
        id += (size_t)gridDim.x * blockDim.x) {
@@ -274,7 +278,10 @@ __global__ __launch_bounds__(256) void k_conv_border(const float* __restrict__ S
     if (interior) continue;
     out[id] = 1e-30f;
     outD[id] = 1e-30;
-    mask[id] = mask[id] | after_flags(S[id], hi, lo);
+    uint16_t m = mask[id];
+    if (m & (mask_pend(0) | mask_pend(1)))
+      m = (uint16_t)((m & XB_MASK_ALL) | ((m & pend) ? XB_MASK_NAN_PIXEL : 0));
+    mask[id] = m | after_flags(S[id], hi, lo);
   }
 }
 
@@ -362,8 +369,9 @@ hipError_t launch_conv(Ctx& c) {
   const int nTiles = xSteps * ySteps;
   const int ti = c.tIdx;
   double* kernSum = &c.sc->kernSum;
+  const double* sol = c.ksSol + (size_t)ti * NSOL;  // the chosen direction's fit
   hipLaunchKernelGGL(k_tile_kernels, dim3((nTiles + 1 + TK_TILES - 1) / TK_TILES), dim3(512), 0,
-                     c.cur, c.basis, c.sol, c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks,
+                     c.cur, c.basis, sol, c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks,
                      kernSum);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -375,7 +383,7 @@ hipError_t launch_conv(Ctx& c) {
   A.outD = c.convD;
   A.tileK64 = c.tileK64;
   A.tileAks = c.tileAks;
-  A.sol = c.sol;
+  A.sol = sol;
   A.kernSum = kernSum;
   A.w = c.w;
   A.h = c.h;
@@ -390,7 +398,7 @@ hipError_t launch_conv(Ctx& c) {
   prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_conv_border, dim3(1024), dim3(256), 0, c.cur, c.img[1 - ti], c.mask,
-                     c.conv, c.convD, c.w, c.h, c.m.threshHigh, c.m.threshLow);
+                     c.conv, c.convD, c.w, c.h, c.m.threshHigh, c.m.threshLow, mask_pend(ti));
   return hipGetLastError();
 }
 

@@ -500,6 +500,7 @@ struct SigArgs {
   const Basis* basis;
   const double* sol;
   double* sigOut;    // indexed by list position
+  uint16_t nanMark;  // mask bit of a NaN model pixel
 };
 
 // One workgroup per stamp, one thread per substamp pixel (992 >= 31 x 31 slots
@@ -559,7 +560,7 @@ __global__ __launch_bounds__(SIG_THREADS) void k_calc_sigs(SigArgs A) {
       const double diff = tDat - sDat + s_bg;
       if ((A.mask[ai] & XB_MASK_BAD_INPUT) == 0 && fabs(sDat) > 1e-10) {
         if (isnan(tDat) || isnan(sDat)) {
-          mask_or_atomic(A.mask, ai, XB_MASK_NAN_PIXEL);
+          mask_or_atomic(A.mask, ai, A.nanMark);
         } else {
           cnt = 1;
           s0 = diff * diff / (fabs((double)A.tImg[ai]) + fabs(sDat));
@@ -672,7 +673,8 @@ __global__ __launch_bounds__(1024) void k_cd_finish(const Basis* basis, const do
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int* kept,
                                                     int* cur, const int* count, int* list,
-                                                    DevScalars* sc, int im, double alpha,
+                                                    const int* nKept, KscScalars* ks,
+                                                    int first, double alpha,
                                                     double sigKernFit) {
   __shared__ double data[XB_MAX_STAMPS];
   __shared__ uint8_t clip[XB_MAX_STAMPS];
@@ -682,7 +684,7 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
   __shared__ int isc[20];
   __shared__ int s_n;
   const int tid = threadIdx.x;
-  const int n = sc->nKept[im];
+  const int n = *nKept;
   if (tid == 0) s_n = 0;
   for (int s = tid; s < n; s += blockDim.x) refill[s] = 0;
   __syncthreads();
@@ -733,11 +735,16 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
     __syncthreads();
   }
   if (tid == 0) {
-    sc->nList[im] = s_n;
-    sc->check = s_n > 0 ? 1 : 0;
-    sc->nChi2 = nChi;
-    sc->clipMean = mean;
-    sc->clipStd = sd;
+    ks->nList = s_n;
+    ks->check = s_n > 0 ? 1 : 0;
+    ks->nChi2 = nChi;
+    ks->clipMean = mean;
+    ks->clipStd = sd;
+    if (first) {  // the first pass's clip statistics (the diagnostics report them)
+      ks->firstClip[0] = mean;
+      ks->firstClip[1] = sd;
+      ks->nScreened = 0;
+    }
   }
 }
 
@@ -773,16 +780,18 @@ hipError_t launch_global_build(Ctx& c, int im, bool cdForm) {
   A.w = c.w;
   A.h = c.h;
   A.cdForm = cdForm ? 1 : 0;
-  A.M = c.lu[im].M;
-  A.rhs = c.lu[im].rhs;
-  A.M0 = c.lu[im].M0;
-  A.rhs0 = c.lu[im].rhs0;
+  const LuSet& L = cdForm ? c.lu[im] : c.ksLu[im];
+  A.M = L.M;
+  A.rhs = L.rhs;
+  A.M0 = L.M0;
+  A.rhs0 = L.rhs0;
   const int total = MATN * (MATN + 1) / 2 + NSOL;
   hipLaunchKernelGGL(k_global_build, dim3((total + 255) / 256), dim3(256), 0, c.cur, A);
   return hipGetLastError();
 }
 
-hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol) {
+hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
+                            uint16_t nanMark) {
   SigArgs A;
   const int src = im;
   A.tImg = c.img[src];
@@ -800,7 +809,8 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol) {
   A.nList = useList ? &c.sc->nList[im] : &c.sc->nKept[im];
   A.basis = c.basis;
   A.sol = d_sol;
-  A.sigOut = c.st[im].sig;
+  A.sigOut = useList ? c.st[im].sig : c.st[im].sigK;
+  A.nanMark = nanMark;
   if (c.nKept[im] <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur, A);
   return hipGetLastError();
@@ -813,11 +823,46 @@ hipError_t launch_cd_finish(Ctx& c, int im) {
   return hipGetLastError();
 }
 
-hipError_t launch_ksc_check(Ctx& c, int im) {
-  hipLaunchKernelGGL(k_ksc_check, dim3(1), dim3(1024), 0, c.cur, c.st[im].sig,
-                     c.st[im].kept, c.st[im].cur, c.st[im].count, c.st[im].list, c.sc, im,
-                     c.m.sigClipAlpha, c.m.sigKernFit);
+hipError_t launch_ksc_check(Ctx& c, int im, bool first) {
+  hipLaunchKernelGGL(k_ksc_check, dim3(1), dim3(1024), 0, c.cur, c.st[im].sigK,
+                     c.st[im].kept, c.st[im].cur, c.st[im].count, c.st[im].listK,
+                     &c.sc->nKept[im], &c.sc->k[im], first ? 1 : 0, c.m.sigClipAlpha,
+                     c.m.sigKernFit);
   return hipGetLastError();
+}
+
+// every kept stamp moved on from its first substamp goes back to it and into
+// the list; one workgroup, the list in ascending stamp order
+__global__ __launch_bounds__(1024) void k_restore_list(const int* nKept, int* cur, int* list,
+                                                       KscScalars* ks) {
+  __shared__ int isc[20];
+  __shared__ int s_n;
+  const int tid = threadIdx.x;
+  const int n = *nKept;
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += blockDim.x) {
+    const int s = base + tid;
+    const bool r = s < n && cur[s] != 0;
+    if (r) cur[s] = 0;
+    int tot;
+    const int pos = block_prefix_count(r, isc, &tot);
+    const int c0 = s_n;
+    if (r) list[c0 + pos] = s;
+    __syncthreads();
+    if (tid == 0) s_n = c0 + tot;
+    __syncthreads();
+  }
+  if (tid == 0) ks->nList = s_n;
+}
+
+hipError_t launch_restore_fill(Ctx& c, int im) {
+  if (c.nKept[im] <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_restore_list, dim3(1), dim3(1024), 0, c.cur, &c.sc->nKept[im],
+                     c.st[im].cur, c.st[im].listK, &c.sc->k[im]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_fill(c, im, c.st[im].listK, &c.sc->k[im].nList, c.nKept[im]);
 }
 
 }  // namespace xb

@@ -102,7 +102,7 @@ struct ScreenArgs {
   const double* sdT;     // sky sigma of sss (the reference's "fwhm" arrays)
   const double* sdS;
   const double* sig;     // this pass's chi2
-  const DevScalars* sc;  // clip mean / sd of this pass
+  const KscScalars* ks;  // clip mean / sd of this pass
   double threshHigh;
   int pass;
   const int* prevCur;    // substamp of each stamp in the previous pass (or null)
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(SCR_THREADS) void k_screen_features(ScreenArgs A) {
   if (tid == 0) {
     const float D = red[0][23] + red[1][23] + red[2][23] + red[3][23];
     const float chi = (float)A.sig[s];
-    const float mean = (float)A.sc->clipMean, sd = (float)A.sc->clipStd;
+    const float mean = (float)A.ks->clipMean, sd = (float)A.ks->clipStd;
     float f[MLP_IN];
     const float z = sd > 0.0f ? (chi - mean) / sd : 0.0f;
     f[0] = fminf(fmaxf(z, -10.0f), 50.0f) / 10.0f;
@@ -284,11 +284,11 @@ __global__ __launch_bounds__(WAVE) void k_screen_decide(const float* __restrict_
 // (cur != passCur) and the screened ones
 __global__ __launch_bounds__(1024) void k_screen_merge(const int* passCur, int* cur,
                                                        const uint8_t* flag, int* list,
-                                                       DevScalars* sc, int im) {
+                                                       const int* nKept, KscScalars* ks) {
   __shared__ int isc[20];
   __shared__ int s_n, s_scr;
   const int tid = threadIdx.x;
-  const int n = sc->nKept[im];
+  const int n = *nKept;
   if (tid == 0) {
     s_n = 0;
     s_scr = 0;
@@ -313,9 +313,9 @@ __global__ __launch_bounds__(1024) void k_screen_merge(const int* passCur, int* 
     __syncthreads();
   }
   if (tid == 0) {
-    sc->nList[im] = s_n;
-    sc->check = s_n > 0 ? 1 : 0;
-    sc->nScreened += s_scr;  // over the screened passes (zero from init)
+    ks->nList = s_n;
+    ks->check = s_n > 0 ? 1 : 0;
+    ks->nScreened += s_scr;  // over the screened passes (zeroed by the first check)
   }
 }
 
@@ -453,8 +453,8 @@ hipError_t launch_screen(Ctx& c, int im, int pass, bool apply) {
   A.skyS = c.st[1 - im].sky;
   A.sdT = c.st[im].fwhm;
   A.sdS = c.st[1 - im].fwhm;
-  A.sig = c.st[im].sig;
-  A.sc = c.sc;
+  A.sig = c.st[im].sigK;
+  A.ks = &c.sc->k[im];
   A.threshHigh = c.m.threshHigh;
   A.pass = pass;
   A.prevCur = pass > 0 ? passCur - N : nullptr;
@@ -474,7 +474,7 @@ hipError_t launch_screen(Ctx& c, int im, int pass, bool apply) {
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_screen_merge, dim3(1), dim3(1024), 0, c.cur, passCur, c.st[im].cur,
-                     c.scr.flag, c.st[im].list, c.sc, im);
+                     c.scr.flag, c.st[im].listK, &c.sc->nKept[im], &c.sc->k[im]);
   return hipGetLastError();
 }
 

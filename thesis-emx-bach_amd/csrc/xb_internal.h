@@ -102,26 +102,35 @@ struct DevStamps {
   double* G = nullptr;     // [N][WROWS][WROWS] Gram of final W rows (lower part)
   double* Bc = nullptr;    // [N][WROWS] W_r . target, column-major pixel order
   double* Br = nullptr;    // [N][NBG]   bg rows . target, row-major pixel order
-  double* sig = nullptr;   // [N] chi2 per kept stamp
+  double* sig = nullptr;   // [N] cd: chi2 of the test stamps (list order)
   double* ksum = nullptr;  // [N] per-stamp test-fit kernel sums
-  int* list = nullptr;     // [N] stamp list scratch (test stamps / refill)
+  int* list = nullptr;     // [N] cd: test-stamp list
+  double* sigK = nullptr;  // [N] ksc: chi2 per kept stamp of the current pass
+  int* listK = nullptr;    // [N] ksc: refill list of the current pass
+};
+
+// Scalars of one direction's clip-refit loop (ksc), indexed by the image it
+// convolves: each direction has its own, so both loops can run at once.
+struct KscScalars {
+  int check;            // the pass replaced a substamp
+  int nList;            // entries of the refill list
+  int nChi2;            // stamps with a valid chi2 in the pass
+  int nScreened;        // substamps the learned screen replaced
+  double clipMean, clipStd;
+  double firstClip[2];  // clip mean and sd of the first pass
 };
 
 // Small device-side scalars of one pass (one struct, one allocation).
 struct DevScalars {
   int nKept[2];
-  int nList[2];         // entries of the current stamp list, per image
-  int check;            // ksc: a substamp was replaced
+  int nList[2];         // cd: entries of the test-stamp list, per image
   int nTest[2];
   int retry;            // sss: refill ratio below 10 %
   int status;           // device-detected error
   double merit[2];
   double kernelSumCd[2];
-  double clipMean, clipStd;
-  double firstClip[2];  // clip mean and sd of the first ksc pass
   double kernSum;       // kernel sum at the image centre (conv)
-  int nChi2;
-  int nScreened;        // substamps the learned screen replaced
+  KscScalars k[2];      // ksc, per convolved image
 };
 
 // The global FP64 system of one fit and its LU work buffers.
@@ -150,11 +159,26 @@ struct ScreenBufs {
   bool haveModel = false;
 };
 
+// NaN marks of a clip-refit loop that runs before the direction is known
+// (one spare mask bit per convolved image); the convolution keeps the chosen
+// direction's as XB_MASK_NAN_PIXEL and drops both.
+constexpr uint16_t MASK_PEND0 = 1u << 13;
+__host__ __device__ constexpr uint16_t mask_pend(int im) { return (uint16_t)(MASK_PEND0 << im); }
+static_assert((XB_MASK_ALL & (MASK_PEND0 | (MASK_PEND0 << 1))) == 0, "spare mask bits");
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;  // primary stream: every host read-back syncs it
   hipStream_t side = nullptr;    // second stream for the other image's independent work
   hipStream_t cur = nullptr;     // stream the launchers use (stream or side)
+  // the two directions' clip-refit loops, run beside cd when the pipeline
+  // runs whole (multi-stream contexts; see stageCdKsc)
+  hipStream_t kstream[2] = {nullptr, nullptr};
+  hipEvent_t evCdSig[2] = {nullptr, nullptr};  // cd's last read of a direction's fit data
+  hipEvent_t evK[2] = {nullptr, nullptr};      // a ksc pass's check flag has arrived
+  hipEvent_t evKEnd[2] = {nullptr, nullptr};   // a direction's loop has ended
+  hipEvent_t evMerit = nullptr;                // both cd merits have arrived
+  int specRestore = -1;         // image whose fit data a restore is refilling (-1: none)
   // captured global solves, one per (system, output, skip flag); buffers are
   // allocated once per context, so an entry stays valid until freeAll
   struct SolveGraph {
@@ -189,8 +213,10 @@ struct Ctx {
   double* ran1 = nullptr;       // ran1 draws (identical per stamp: seed -666)
   int ran1Count = 0;
   float* goodPix = nullptr;     // [2][N][fsw*fsw] good-pixel / candidate scratch of sss
-  LuSet lu[2];                  // one global system per direction (cd runs both at once)
-  double* sol = nullptr;        // [NSOL] current kernel solution
+  LuSet lu[2];                  // cd: one global system per direction (both run at once)
+  LuSet ksLu[2];                // ksc: the clip-refit system of each direction
+  double* ksSol = nullptr;      // [2][NSOL] ksc solution of each direction
+  double* sol = nullptr;        // [NSOL] xb_lu_solve's output
   double* cdSol = nullptr;      // [2][NSOL]
   int form = 0;                 // XB_FORM_* of the next run
   ScreenBufs scr;
@@ -239,7 +265,9 @@ hipError_t launch_sss_stats(Ctx& c, int im);
 hipError_t launch_sss_find(Ctx& c, int im);
 hipError_t launch_sss_compact(Ctx& c);
 hipError_t launch_reset_skip(Ctx& c);
-hipError_t launch_fill(Ctx& c, int im, const int* d_list, int nMax);
+// Fills the stamps of image im listed in d_list (d_nList entries on the
+// device), or every kept stamp when d_list is null; nMax bounds the count.
+hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, int nMax);
 hipError_t launch_stamp_lu(Ctx& c, int im);
 hipError_t launch_cd_select(Ctx& c, int im);
 hipError_t launch_global_build(Ctx& c, int im, bool cdForm);
@@ -248,9 +276,15 @@ hipError_t launch_global_build(Ctx& c, int im, bool cdForm);
 // empty fit).
 hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int* d_skipIfZero);
 hipError_t launch_cd_finish(Ctx& c, int im);
-hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol);
-hipError_t launch_ksc_check(Ctx& c, int im);
+// cd form (useList): the test stamps' chi2 into st.sig; ksc form: every kept
+// stamp's into st.sigK.  nanMark is the mask bit a NaN model pixel sets.
+hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
+                            uint16_t nanMark = XB_MASK_NAN_PIXEL);
+hipError_t launch_ksc_check(Ctx& c, int im, bool first);
 hipError_t launch_conv(Ctx& c);
+// Restores the fit data of image im to what cmv left (every kept stamp on its
+// first substamp): stamps a clip-refit loop moved on are listed and refilled.
+hipError_t launch_restore_fill(Ctx& c, int im);
 hipError_t launch_sub(Ctx& c);
 // difference-image diagnostics: out[10] = good pixels (n, sum d, sum d^2,
 // sum noise, sum (d/noise)^2), then the same over OK_CONV pixels
