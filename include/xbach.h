@@ -111,6 +111,10 @@ typedef struct xb_trace {
   int32_t finalCurrent[XB_MAX_STAMPS];         /* chosen direction, kept order */
   /* conv */
   double kernSum;
+  /* graph captures of the global solve that failed on this context since it
+   * was created (the solve then ran its kernels directly; after two failures
+   * the context stops capturing).  Zero in a healthy run. */
+  int32_t captureFailures;
 } xb_trace;
 
 void xb_params_default(xb_params* p);
@@ -358,10 +362,17 @@ typedef struct xb_pair_result {
   int32_t worker;          /* worker that ran the pair                        */
   double kernSum;          /* kernel sum at the image centre                  */
   double ms;               /* wall time of the pair on its worker             */
+  int32_t captureFailures; /* failed graph captures of its worker's context   */
+  int32_t pad;
 } xb_pair_result;
 
-/* workers >= 1 (contexts created up front, flags as xb_ctx_create plus
- * XB_CTX_ONE_STREAM implied). */
+/* 1 <= workers <= XB_BATCH_MAX_WORKERS (more is refused with
+ * XB_ERR_CAPACITY).  Contexts are created up front with the flags of
+ * xb_ctx_create plus XB_CTX_ONE_STREAM and XB_CTX_EAGER implied: worker
+ * threads launch the global solve's kernels directly and never capture a
+ * graph, since a stream capture on one thread can be invalidated by another
+ * thread's work on the device. */
+#define XB_BATCH_MAX_WORKERS 32
 int xb_batch_create(int device, int workers, int flags, xb_batch** out);
 int xb_batch_destroy(xb_batch* b);
 const char* xb_batch_last_error(xb_batch* b);
@@ -377,8 +388,9 @@ int xb_batch_run(xb_batch* b, const xb_params* p, int n, const int* w, const int
 int xb_batch_run_device(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
                         const void* const* d_tmpl, const void* const* d_sci,
                         void* const* d_conv, void* const* d_diff, xb_pair_result* results);
-/* Number of pairs in flight used by the next run (1..workers); 0 restores
- * "all workers".  Clears the autotuner's per-bucket counts. */
+/* Number of pairs in flight used by the next run (1..workers; more is
+ * refused with XB_ERR_CAPACITY); 0 restores "all workers".  Clears the
+ * autotuner's per-bucket counts. */
 int xb_batch_set_inflight(xb_batch* b, int inflight);
 /* Form (XB_FORM_*) of every pair of the next run. */
 int xb_batch_set_form(xb_batch* b, int form);

@@ -62,6 +62,8 @@ def test_batch_host_equals_single_runs(pairs, singles):
         assert (r.kernSum, r.convTemplate, r.kscIterations) == (ks, ct, it), i
     assert res[-1].status == NO_SUBSTAMPS
     assert all(0 <= r.worker < 4 for r in res)
+    # worker contexts never capture graphs, so none can fail
+    assert all(r.captureFailures == 0 for r in res)
 
 
 def test_batch_pair_against_oracle(pairs):
@@ -107,9 +109,11 @@ def test_batch_device_resident(pairs, singles, inflight):
 
 def test_batch_argument_errors():
     p = xbach.Params.default()
+    with pytest.raises(xbach.XbachError, match="ERR_CAPACITY"):
+        xbach.Batch(0, workers=33)  # above XB_BATCH_MAX_WORKERS
     with xbach.Batch(0, workers=2) as b:
-        with pytest.raises(xbach.XbachError, match="ERR_ARG"):
-            b.set_inflight(3)
+        with pytest.raises(xbach.XbachError, match="ERR_CAPACITY"):
+            b.set_inflight(3)  # more pairs in flight than workers
         t = np.ones((64, 64), np.float32)
         with pytest.raises(xbach.XbachError, match="ERR_ARG"):
             b.run(xbach.Params.default(fKernelWidth=19), [(t, t)])

@@ -31,6 +31,9 @@ def _runs(flags, pairs, reps):
             for t, s in pairs:
                 conv, diff, _ = c.run(p, t, s)
                 tr = c.trace()
+                # a failed capture would fall back to direct launches with the
+                # same results: counted, and required to be zero
+                assert tr.captureFailures == 0
                 out.append((conv, diff, tr.kernSum, tr.convTemplate, tr.kscIterations))
     return out
 

@@ -6,6 +6,7 @@ import pathlib
 import re
 
 import numpy as np
+import pytest
 
 import xbach
 
@@ -105,3 +106,58 @@ def test_fits_round_trip(tmp_path):
     assert hdr["BITPIX"] == "-32" and int(hdr["NAXIS1"]) == 35 and int(hdr["NAXIS2"]) == 20
     assert np.array_equal(b.astype(np.float32), a)
     assert p.stat().st_size % 2880 == 0
+
+
+def _device_code_objects(lib_path, workdir):
+    """The gfx950 code objects of the built library: one offload bundle per
+    translation unit in its .hip_fatbin section, unbundled with the ROCm tools."""
+    import shutil
+    import subprocess
+
+    bundler = pathlib.Path("/opt/rocm/llvm/bin/clang-offload-bundler")
+    if not bundler.exists() or shutil.which("objcopy") is None:
+        pytest.skip("objcopy or the ROCm offload bundler is missing")
+    fb = workdir / "fatbin"
+    # objcopy rewrites the file it is given: it works on a copy, never on the
+    # library this process has loaded
+    copy = workdir / "lib.so"
+    shutil.copyfile(lib_path, copy)
+    subprocess.run(["objcopy", "--dump-section", f".hip_fatbin={fb}", str(copy)], check=True)
+    data = fb.read_bytes()
+    magic = b"__CLANG_OFFLOAD_BUNDLE__"
+    starts = [m.start() for m in re.finditer(re.escape(magic), data)]
+    out = []
+    for i, a in enumerate(starts):
+        b = starts[i + 1] if i + 1 < len(starts) else len(data)
+        part = workdir / f"bundle{i}"
+        part.write_bytes(data[a:b])
+        co = workdir / f"code{i}.co"
+        subprocess.run([str(bundler), "--unbundle", "--type=o",
+                        "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--input={part}",
+                        f"--output={co}"], check=True)
+        out.append(co)
+    return out
+
+
+def test_no_kernel_uses_private_scratch(tmp_path):
+    """Every kernel of libxbach has a zero private segment: per-queue scratch
+    for a kernel with a large private segment, multiplied over many hardware
+    queues, once made the runtime abort with OUT_OF_RESOURCES."""
+    import subprocess
+
+    readelf = pathlib.Path("/opt/rocm/llvm/bin/llvm-readelf")
+    if not readelf.exists():
+        pytest.skip("llvm-readelf is missing")
+    kernels = {}
+    for co in _device_code_objects(xbach.LIB_PATH, tmp_path):
+        notes = subprocess.run([str(readelf), "--notes", str(co)], check=True,
+                               capture_output=True, text=True).stdout
+        # one YAML map per kernel under amdhsa.kernels ("  - " items); its own
+        # keys sit at four spaces (argument maps are deeper)
+        for block in re.split(r"\n  - ", notes)[1:]:
+            name = re.search(r"^    \.name:\s+(\S+)", block, re.M)
+            size = re.search(r"^    \.private_segment_fixed_size:\s+(\d+)", block, re.M)
+            if name and size:
+                kernels[name.group(1)] = int(size.group(1))
+    assert len(kernels) >= 30, sorted(kernels)
+    assert {k: v for k, v in kernels.items() if v} == {}

@@ -19,12 +19,11 @@
 // so no wait sits before the products), and the products read LDS one 8-step
 // block ahead (a sched_barrier keeps the reads early): a refill of a few
 // stamps is bound by one tile's latency.
-// k_fill_target: one workgroup per stamp forms the target dot products:
-// column-major order for all 52 rows (createB, cmv.cl:217-248, and the ksc
-// right-hand side, src/cdkscUtil.cpp:459-470) and row-major order for the 3
-// background rows (cd right-hand side, cl_kern/cd.cl:318-344), the latter on
-// wave 1 in slices beside the former; the next chunk's gather is in flight
-// while the current one is summed.  Q (51x51, 1-based) is G shifted by one.
+// The same launch's last work item per stamp forms the target dot products
+// with one wave: column-major order for all 52 rows (createB, cmv.cl:217-248,
+// and the ksc right-hand side, src/cdkscUtil.cpp:459-470) and row-major order
+// for the 3 background rows (cd right-hand side, cl_kern/cd.cl:318-344), beside
+// the Gram items instead of after them.  Q (51x51, 1-based) is G shifted by one.
 #include "xb_dev.h"
 
 namespace xb {
@@ -103,130 +102,12 @@ __global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
 constexpr int GT = 16;   // Gram tile
 constexpr int GK = 64;   // pixel chunk
 
-__global__ __launch_bounds__(256) void k_fill_target(FillArgs A) {
-  int s, ssx, ssy;
-  if (!fill_stamp(A, blockIdx.x, &s, &ssx, &ssy)) return;
-  const int g = A.kept[s];
-  const bool valid = A.cur[s] < A.count[g];
-  const int tid = threadIdx.x;
-  const int* evenSub = A.basis->evenSub;
-  __shared__ double tc[NPIX], tr[NPIX];
-  __shared__ double mono[2][SSW];
-  __shared__ int rsub[WROWS];
-  if (!valid) {
-    if (tid < WROWS) A.Bc[(size_t)s * WROWS + tid] = 0.0;
-    if (tid < NBG) A.Br[(size_t)s * NBG + tid] = 0.0;
-    return;
-  }
-  for (int i = tid; i < NPIX; i += blockDim.x) {
-    const int xx = i % SSW - HSS, yy = i / SSW - HSS;
-    const int xr = min(max(ssx + xx, 0), A.w - 1), yr = min(max(ssy + yy, 0), A.h - 1);
-    const double v = (double)A.img2[(size_t)xr + (size_t)yr * A.w];
-    tr[i] = v;
-    tc[(i % SSW) * SSW + i / SSW] = v;
-  }
-  if (tid < SSW) {
-    const float hw = (float)A.w * 0.5f, hh = (float)A.h * 0.5f;
-    mono[0][tid] = (double)(((float)(ssx - HSS + tid) - hw) / hw);
-    mono[1][tid] = (double)(((float)(ssy - HSS + tid) - hh) / hh);
-  }
-  if (tid < WROWS) rsub[tid] = tid < NPSF && evenSub[tid];
-  constexpr int IT = (WROWS * GK + 255) / 256;  // 13 staged values per thread
-  __shared__ double wc[2][WROWS][GK + 1];
-  double nv[IT], n0[IT];
-  // the rows in column-major order (written beside W by k_fill_conv): a
-  // chunk of positions is contiguous in every row
-  const double* Wc = A.Wc + (size_t)s * WROWS * NPIX;
-  auto fetch = [&](int c0) {
-#pragma unroll
-    for (int j = 0; j < IT; j++) {
-      const int i = tid + 256 * j;
-      const int r = min(i / GK, WROWS - 1), cc = i % GK;
-      const int c = min(c0 + cc, NPIX - 1);
-      nv[j] = Wc[(size_t)r * NPIX + c];
-      n0[j] = Wc[c];
-    }
-  };
-  auto store = [&](int buf, int c0) {
-#pragma unroll
-    for (int j = 0; j < IT; j++) {
-      const int i = tid + 256 * j;
-      const int r = i / GK, cc = i % GK;
-      if (r < WROWS) {
-        double v = 0.0;
-        if (c0 + cc < NPIX) {
-          v = nv[j];
-          if (rsub[r]) v = v - n0[j];
-        }
-        wc[buf][r][cc] = v;
-      }
-    }
-  };
-  __syncthreads();  // rsub, tc, tr, mono
-  fetch(0);
-  store(0, 0);
-  __syncthreads();
-  double acc = 0.0, a2 = 0.0;
-  const int b = tid - 64;  // background row of wave 1 lanes 0..2
-  int buf = 0, yb = 0;
-  constexpr int NCH = (NPIX + GK - 1) / GK;
-  for (int ch = 0; ch < NCH; ch++) {
-    const int c0 = ch * GK;
-    const int kn = min(GK, NPIX - c0);
-    const bool more = ch + 1 < NCH;
-    if (more) fetch(c0 + GK);
-    if (tid < WROWS) {
-      const double* t = tc + c0;
-      if (kn == GK) {
-        double wv[2][8], tv[2][8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          wv[0][j] = wc[buf][tid][j];
-          tv[0][j] = t[j];
-        }
-#pragma unroll
-        for (int blk = 0; blk < GK / 8; blk++) {
-          const int cu = blk & 1, nx = cu ^ 1;
-          if (blk + 1 < GK / 8) {
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-              wv[nx][j] = wc[buf][tid][8 * (blk + 1) + j];
-              tv[nx][j] = t[8 * (blk + 1) + j];
-            }
-          }
-          __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-          for (int j = 0; j < 8; j++) acc += wv[cu][j] * tv[cu][j];
-        }
-      } else {
-        for (int cc = 0; cc < kn; cc++) acc += wc[buf][tid][cc] * t[cc];
-      }
-    } else if (b >= 0 && b < NBG) {
-      // rows yb .. of the background chain (row-major), spread over the chunks
-      const int yEnd = (ch + 1) * SSW / NCH;
-      for (; yb < yEnd; yb++) {
-        const double yf = b == 1 ? mono[1][yb] : 1.0;
-        for (int x = 0; x < SSW; x++) {
-          const double bgv = (b == 2 ? mono[0][x] : 1.0) * yf;
-          a2 += bgv * tr[yb * SSW + x];
-        }
-      }
-    }
-    if (more) {
-      store(buf ^ 1, c0 + GK);
-      __syncthreads();
-      buf ^= 1;
-    }
-  }
-  if (tid < WROWS) A.Bc[(size_t)s * WROWS + tid] = acc;
-  if (b >= 0 && b < NBG) A.Br[(size_t)s * NBG + b] = a2;
-}
-
 // Per stamp, the lower triangle of the 52x52 Gram matrix is split into NGT
 // work items of equal length (961 pixel steps each): the 6 lower-triangular
 // 16x16 tiles of rows/columns 0..47, and one strip holding rows 48..51 against
 // columns 0..51.
 constexpr int NGT = 7;
+constexpr int NGI = NGT + 1;   // work items per stamp: the Gram items and the target products
 constexpr int GP = GT + 2;     // tile LDS pitch: [pixel][row], 16-byte aligned row pairs
 constexpr int SR0 = 3 * GT;    // first strip row
 constexpr int SK = 32;         // strip pixel chunk
@@ -288,6 +169,113 @@ __device__ __forceinline__ void gram_steps(const double* xb, const double* yb, i
   }
 }
 
+// Bit r set when W row r (r < 49) is an even basis term: one load per lane of
+// the wave and a ballot (a loop of guarded loads would wait on each in turn).
+__device__ __forceinline__ unsigned long long even_rows(const int* evenSub, int lane) {
+  return __ballot(lane < NPSF && evenSub[min(lane, NPSF - 1)] != 0);
+}
+
+// The target products of one stamp by one wave (the last work item of
+// k_fill_gram, beside the Gram items): lane r < 52 sums row r of W (minus row
+// 0 for even basis terms) times the target in column-major pixel order (createB,
+// cmv.cl:217-248, and the ksc right-hand side, src/cdkscUtil.cpp:459-470);
+// lanes 52..54 sum the three background monomials times the target in
+// row-major order (the cd right-hand side, cl_kern/cd.cl:318-344).  Every sum
+// runs in its reference order.  32-position chunks are staged through LDS,
+// [position][column]: the 52 column-major W rows, the target in both orders,
+// and the background weights of each row-major position (1, y monomial, x
+// monomial: (1.0 * 1.0), (1.0 * yf) and (xf * 1.0) of the reference are these
+// values exactly), so all 55 chains run one uniform loop.  Two chunks' loads
+// are in flight while one is summed.
+constexpr int TQ_C = WROWS;       // target, column-major order
+constexpr int TQ_R = WROWS + 1;   // target, row-major order
+constexpr int TQ_B = WROWS + 2;   // background weights 1, yf, xf (3 columns)
+constexpr int TP = WROWS + 6;     // LDS pitch (16-byte aligned rows)
+constexpr int TQ_N = SH + 2;      // values a lane loads per chunk
+static_assert(TQ_B + NBG <= TP && SK * TP <= 2 * GK * GP, "target chunks fit the LDS");
+
+__device__ __forceinline__ void fill_target_wave(const FillArgs& A, int s, int ssx, int ssy,
+                                                 bool valid, double* Wl) {
+  const int lane = threadIdx.x;
+  if (!valid) {
+    if (lane < WROWS) A.Bc[(size_t)s * WROWS + lane] = 0.0;
+    if (lane < NBG) A.Br[(size_t)s * NBG + lane] = 0.0;
+    return;
+  }
+  double(*Ts)[TP] = reinterpret_cast<double(*)[TP]>(Wl);
+  const int px = lane % SK, h = lane / SK;
+  // bit r: row h*SH + r is an even basis term
+  const unsigned long long sub = even_rows(A.basis->evenSub, lane) >> (h * SH);
+  const double* Wc = A.Wc + (size_t)s * WROWS * NPIX;
+  const float hw = (float)A.w * 0.5f, hh = (float)A.h * 0.5f;
+  // raw loads of chunk k0 into v: 26 rows, row 0, and the target value at
+  // position k0 + px (half 0 column-major, half 1 row-major), clamped
+  auto fetch = [&](int k0, double (&v)[TQ_N]) {
+    const int k = min(k0 + px, NPIX - 1);
+#pragma unroll
+    for (int r = 0; r < SH; r++) v[r] = Wc[(size_t)(h * SH + r) * NPIX + k];
+    v[SH] = Wc[k];
+    const int xx = h == 0 ? k / SSW : k % SSW, yy = h == 0 ? k % SSW : k / SSW;
+    const int xr = min(max(ssx - HSS + xx, 0), A.w - 1);
+    const int yr = min(max(ssy - HSS + yy, 0), A.h - 1);
+    v[SH + 1] = (double)A.img2[(size_t)xr + (size_t)yr * A.w];
+  };
+  auto store = [&](int k0, const double (&v)[TQ_N]) {
+#pragma unroll
+    for (int r = 0; r < SH; r++) {
+      double x = v[r];
+      if ((sub >> r) & 1) x = x - v[SH];
+      Ts[px][h * SH + r] = x;
+    }
+    Ts[px][TQ_C + h] = v[SH + 1];
+    // background weights of row-major position k: monomials as the reference
+    // forms them (FP32 xf, yf)
+    const int k = k0 + px;
+    if (h == 0) {
+      Ts[px][TQ_B] = 1.0;
+      Ts[px][TQ_B + 1] = (double)(((float)(ssy - HSS + k / SSW) - hh) / hh);
+    } else {
+      Ts[px][TQ_B + 2] = (double)(((float)(ssx - HSS + k % SSW) - hw) / hw);
+    }
+  };
+  // lane r: W row r times the column-major target; lane 52 + b: background
+  // weight b times the row-major target
+  const int ca = lane < WROWS ? lane : TQ_B + min(lane - WROWS, NBG - 1);
+  const int cb = lane < WROWS ? TQ_C : TQ_R;
+  double acc = 0.0;
+  auto sum = [&](int k0) {
+    const int kn = min(SK, NPIX - k0);
+    if (kn == SK) {
+#pragma unroll
+      for (int kk = 0; kk < SK; kk++) acc += Ts[kk][ca] * Ts[kk][cb];
+    } else {
+      for (int kk = 0; kk < kn; kk++) acc += Ts[kk][ca] * Ts[kk][cb];
+    }
+  };
+  constexpr int NCH = (NPIX + SK - 1) / SK;
+  double va[TQ_N], vb[TQ_N];
+  fetch(0, va);
+  fetch(SK, vb);
+  store(0, va);
+  __syncthreads();  // chunk 0 (a single-wave block)
+  for (int ch = 0; ch < NCH; ch += 2) {
+    // chunk ch is in LDS, ch + 1 in vb; va is free
+    if (ch + 2 < NCH) fetch((ch + 2) * SK, va);
+    sum(ch * SK);
+    if (ch + 1 >= NCH) break;
+    store((ch + 1) * SK, vb);
+    __syncthreads();
+    // chunk ch + 1 is in LDS, ch + 2 in va; vb is free
+    if (ch + 3 < NCH) fetch((ch + 3) * SK, vb);
+    sum((ch + 1) * SK);
+    if (ch + 2 >= NCH) break;
+    store((ch + 2) * SK, va);
+    __syncthreads();
+  }
+  if (lane < WROWS) A.Bc[(size_t)s * WROWS + lane] = acc;
+  else if (lane < WROWS + NBG) A.Br[(size_t)s * NBG + lane - WROWS] = acc;
+}
+
 // One wave per (stamp, work item).  Blocks b and b + 8 run on the same XCD
 // (blocks are dealt to the 8 XCDs in turn), so each XCD receives a contiguous
 // range of work items and a stamp's W is read through a single L2.
@@ -301,16 +289,21 @@ __device__ __forceinline__ void gram_steps(const double* xb, const double* yb, i
 // summed from LDS.  One LDS buffer suffices (a single wave's LDS reads and
 // writes run in program order), which leaves room for two waves per SIMD.
 __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
-  const int total = nMax * NGT;
+  const int total = nMax * NGI;
   const int per = (total + 7) / 8;
   const int wk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (wk >= total) return;
   int s, ssx, ssy;
-  if (!fill_stamp(A, wk / NGT, &s, &ssx, &ssy)) return;
+  if (!fill_stamp(A, wk / NGI, &s, &ssx, &ssy)) return;
   const int g = A.kept[s];
   const bool valid = A.cur[s] < A.count[g];
   const int lane = threadIdx.x;
-  const int item = wk % NGT;
+  const int item = wk % NGI;
+  __shared__ __align__(16) double Wl[2 * GK * GP];
+  if (item == NGT) {
+    fill_target_wave(A, s, ssx, ssy, valid, Wl);
+    return;
+  }
   const bool strip = item == NGT - 1;
   int bi = 0, bj = item;
   while (bj > bi) { bj -= bi + 1; bi++; }
@@ -331,16 +324,12 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
   }
   const double* W = A.W + (size_t)s * WROWS * NPIX;
   const int* evenSub = A.basis->evenSub;
-  __shared__ __align__(16) double Wl[2 * GK * GP];
   double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
   if (strip) {
     double(*Ws)[SP] = reinterpret_cast<double(*)[SP]>(Wl);
     const int px = lane % SK, h = lane / SK;
-    unsigned long long sub = 0;  // bit r: row h*SH + r is an even basis term
-    for (int r = 0; r < SH; r++) {
-      const int row = h * SH + r;
-      if (row < NPSF && evenSub[row]) sub |= 1ull << r;
-    }
+    // bit r: row h*SH + r is an even basis term
+    const unsigned long long sub = even_rows(evenSub, lane) >> (h * SH);
     double v[SH], w0;
     // raw loads only (pixels past the window read a clamped address and are
     // zeroed at store time)
@@ -392,10 +381,11 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
   double(*Wa)[GP] = reinterpret_cast<double(*)[GP]>(Wl);
   double(*Wb)[GP] = reinterpret_cast<double(*)[GP]>(Wl + GK * GP);
   bool subA[GT], subB[GT];
+  const unsigned long long em = even_rows(evenSub, lane);
 #pragma unroll
   for (int rr = 0; rr < GT; rr++) {
-    subA[rr] = evenSub[bi * GT + rr] != 0;
-    subB[rr] = evenSub[bj * GT + rr] != 0;
+    subA[rr] = (em >> (bi * GT + rr)) & 1ull;
+    subB[rr] = (em >> (bj * GT + rr)) & 1ull;
   }
   double va[GT], vb[GT], w0;
   // raw loads only (pixels past the window read a clamped address and are
@@ -479,11 +469,8 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, in
   hipLaunchKernelGGL(k_fill_conv, dim3(nMax, WROWS), dim3(256), 0, c.cur, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill_gram, dim3(8 * ((nMax * NGT + 7) / 8)), dim3(WAVE), 0, c.cur, A,
+  hipLaunchKernelGGL(k_fill_gram, dim3(8 * ((nMax * NGI + 7) / 8)), dim3(WAVE), 0, c.cur, A,
                      nMax);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill_target, dim3(nMax), dim3(256), 0, c.cur, A);
   prof_end(c, XB_PROF_FILL);
   return hipGetLastError();
 }

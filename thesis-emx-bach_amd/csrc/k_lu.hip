@@ -850,8 +850,10 @@ hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int*
         // The capture failed (for instance invalidated by another thread's
         // work on the device): nothing of it ran.  A capture still open on
         // the stream is ended, then this solve's kernels are launched
-        // directly; a later call captures again.
+        // directly.  The failure is counted (xb_trace.captureFailures); after
+        // two the context launches every solve directly.
         g = nullptr;
+        if (++c.captureFailures >= 2) c.eagerSolve = true;
         (void)hipGetLastError();
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(c.cur, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {

@@ -1347,6 +1347,7 @@ int xb_get_trace(xb_ctx* x, xb_trace* tr) {
     for (int k = 0; k < c.nKept[c.tIdx]; k++) tr->finalCurrent[k] = cur[k];
   }
   tr->kernSum = sc.kernSum;
+  tr->captureFailures = c.captureFailures;
   return XB_OK;
 }
 

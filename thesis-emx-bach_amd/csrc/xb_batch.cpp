@@ -59,6 +59,7 @@ struct Job {
   xb_pair_result* res;
   std::vector<int> order;   // pair indices, largest first
   std::vector<int> status;  // per pair
+  std::vector<char> ran;    // per pair: a worker took it
   std::vector<int> cap;     // per order position: pairs that may run at once
   int next = 0;             // next order position to hand out (under gate)
   int running = 0;          // pairs running now (under gate)
@@ -104,13 +105,16 @@ int runPair(xb_ctx* x, const Job& J, int i, xb_pair_result* r) {
     r->convTemplate = c.convTemplate;
     r->kscIterations = c.kscIterations;
     r->kernSum = c.kernSum;
+    r->captureFailures = c.captureFailures;
   }
   return XB_OK;
 }
 
 void worker(xb_batch* b, int wi, Job* J) {
   xb_ctx* x = b->ctx[wi];
-  if (hipSetDevice(b->device) != hipSuccess) return;  // pairs stay XB_ERR_HIP
+  // a worker that cannot select the device takes no pair; runBatch reports
+  // the pairs no worker ran
+  if (hipSetDevice(b->device) != hipSuccess) return;
   for (;;) {
     // the next pair starts once fewer pairs run than its size bucket allows
     int k;
@@ -125,6 +129,7 @@ void worker(xb_batch* b, int wi, Job* J) {
     xb_pair_result local{};
     xb_pair_result* r = J->res ? &J->res[i] : &local;
     const double t0 = nowMs();
+    J->ran[i] = 1;
     const int rc = runPair(x, *J, i, r);
     r->status = rc;
     r->worker = wi;
@@ -167,6 +172,7 @@ int runBatch(xb_batch* b, Job& J) {
     return (long long)J.w[a] * J.h[a] > (long long)J.w[c] * J.h[c];
   });
   J.status.assign(J.n, XB_ERR_HIP);
+  J.ran.assign(J.n, 0);
   const int all = b->inflight > 0 ? std::min<int>(b->inflight, (int)b->ctx.size())
                                   : (int)b->ctx.size();
   // pairs in flight: the autotuner's count for the pair's size bucket, else
@@ -184,6 +190,20 @@ int runBatch(xb_batch* b, Job& J) {
   th.reserve(nw);
   for (int wi = 0; wi < nw; wi++) th.emplace_back(worker, b, wi, &J);
   for (auto& t : th) t.join();
+  // pairs no worker took (every worker failed to select the device)
+  for (int i = 0; i < J.n; i++) {
+    if (J.ran[i]) continue;
+    if (J.res) {
+      J.res[i] = xb_pair_result{};
+      J.res[i].status = XB_ERR_HIP;
+      J.res[i].worker = -1;
+    }
+    if (J.firstBad < 0 || i < J.firstBad) {
+      J.firstBad = i;
+      J.firstMsg = "pair " + std::to_string(i) + ": no worker could select device " +
+                   std::to_string(b->device);
+    }
+  }
   if (J.firstBad >= 0) {
     b->err = J.firstMsg;
     return J.status[J.firstBad];
@@ -197,13 +217,17 @@ int runBatch(xb_batch* b, Job& J) {
 extern "C" {
 
 int xb_batch_create(int device, int workers, int flags, xb_batch** out) {
-  if (!out || workers < 1 || workers > 64) return XB_ERR_ARG;
+  if (!out || workers < 1) return XB_ERR_ARG;
   *out = nullptr;
+  if (workers > XB_BATCH_MAX_WORKERS) return XB_ERR_CAPACITY;
   xb_batch* b = new xb_batch();
   b->device = device;
   for (int i = 0; i < workers; i++) {
     xb_ctx* x = nullptr;
-    const int rc = xb_ctx_create(device, flags | XB_CTX_ONE_STREAM, &x);
+    // worker threads never capture graphs: a capture open on one thread's
+    // stream has been invalidated by other threads' work on the device, and
+    // the graphs measured no faster for a batch than direct launches
+    const int rc = xb_ctx_create(device, flags | XB_CTX_ONE_STREAM | XB_CTX_EAGER, &x);
     if (rc) {
       for (auto* y : b->ctx) xb_ctx_destroy(y);
       delete b;
@@ -225,7 +249,11 @@ int xb_batch_destroy(xb_batch* b) {
 const char* xb_batch_last_error(xb_batch* b) { return b ? b->err.c_str() : "no batch"; }
 
 int xb_batch_set_inflight(xb_batch* b, int inflight) {
-  if (!b || inflight < 0 || inflight > (int)b->ctx.size()) return XB_ERR_ARG;
+  if (!b || inflight < 0) return XB_ERR_ARG;
+  if (inflight > (int)b->ctx.size()) {
+    b->err = "pairs in flight exceed the batch's " + std::to_string(b->ctx.size()) + " workers";
+    return XB_ERR_CAPACITY;
+  }
   b->inflight = inflight;
   std::fill(b->tuned, b->tuned + XB_NBUCKET, 0);
   return XB_OK;

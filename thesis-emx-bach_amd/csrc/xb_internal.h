@@ -189,6 +189,7 @@ struct Ctx {
   };
   std::vector<SolveGraph> solveGraphs;
   bool eagerSolve = false;       // XB_CTX_EAGER: no capture
+  int captureFailures = 0;       // failed captures of a solve (two stop capturing)
   hipEvent_t evFork = nullptr, evJoin = nullptr;
   std::string err;
   char name[256] = {0};

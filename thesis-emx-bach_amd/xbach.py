@@ -115,6 +115,7 @@ class Trace(C.Structure):
         ("solution", C.c_double * MAX_SOL),
         ("finalCurrent", C.c_int32 * MAX_STAMPS),
         ("kernSum", C.c_double),
+        ("captureFailures", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -663,7 +664,8 @@ class PairResult(C.Structure):
     """xb_pair_result: per-pair outcome of a batch run."""
     _fields_ = [("status", C.c_int32), ("convTemplate", C.c_int32),
                 ("kscIterations", C.c_int32), ("worker", C.c_int32),
-                ("kernSum", C.c_double), ("ms", C.c_double)]
+                ("kernSum", C.c_double), ("ms", C.c_double),
+                ("captureFailures", C.c_int32), ("pad", C.c_int32)]
 
 
 def _ptr_array(ptrs) -> C.Array:
@@ -677,6 +679,7 @@ class Batch:
     def __init__(self, device: int = 0, workers: int = 4, flags: int = 0):
         self._L = lib()
         h = C.c_void_p()
+        self._h = None
         rc = self._L.xb_batch_create(device, workers, flags, C.byref(h))
         if rc != 0:
             raise XbachError(f"xb_batch_create failed: {STATUS_NAMES.get(rc, rc)}")
