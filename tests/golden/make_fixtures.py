@@ -2,6 +2,8 @@
 
 Reads (as data/text, nothing imported or executed):
   res/test0.fits, res/test1.fits           test 1 inputs (science, template)
+  res/test0_bg.fits, res/test2.fits        the reference's other held images
+                                           (330x330, no goldens: oracle parity)
   tests/test01_conv.fits, test01_sub.fits  HOTPANTS 5.1.11 goldens of test 1
                                            (tools/run_test.py:13 tolerances)
   tools/cond_num.py:3-297                  the dumped 293x293 global normal
@@ -22,7 +24,11 @@ OUT = pathlib.Path(__file__).resolve().parent
 
 
 def copy_gz(src, dst):
-    with open(src, "rb") as f, gzip.open(dst, "wb", compresslevel=9) as g:
+    # a fixture already written is kept byte for byte (gzip headers carry a time)
+    if dst.exists():
+        return
+    with open(src, "rb") as f, open(dst, "wb") as raw, \
+            gzip.GzipFile(filename="", mode="wb", compresslevel=9, fileobj=raw, mtime=0) as g:
         shutil.copyfileobj(f, g)
 
 
@@ -40,6 +46,8 @@ def main():
     copy_gz(REF / "res/test1.fits", OUT / "test01_template.fits.gz")
     copy_gz(REF / "tests/test01_conv.fits", OUT / "test01_conv_hotpants.fits.gz")
     copy_gz(REF / "tests/test01_sub.fits", OUT / "test01_sub_hotpants.fits.gz")
+    copy_gz(REF / "res/test0_bg.fits", OUT / "test0_bg.fits.gz")
+    copy_gz(REF / "res/test2.fits", OUT / "test2.fits.gz")
     text = (REF / "tools/cond_num.py").read_text()
     m = parse_matrix(text, "test01_mtx")
     assert m.shape == (293, 293), m.shape

@@ -156,3 +156,15 @@ def test_cli_test01_end_to_end(tmp_path, test01_pair):
     print("quality cards:", {k: hdr[k] for k in ("KSUM00", "SSSIG00", "FSIG00", "X2NRM00",
                                                  "NX2NRM00", "DMEAN00", "DSIGE00", "DSIG00")},
           "HOTPANTS:", {k: ghdr[k] for k in ("SSSIG00", "FSIG00", "X2NRM00", "NX2NRM00")})
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not CLI.exists(), reason="xbach CLI not built (make -C thesis-emx-bach_amd)")
+def test_cli_size_mismatch(tmp_path):
+    # src/bach.cpp:20-24: the reference's message, exit status 1, no output
+    (tmp_path / "t.fits").write_bytes(_fits_bytes(np.full((64, 64), 100.0), -32))
+    (tmp_path / "s.fits").write_bytes(_fits_bytes(np.full((80, 64), 100.0), -32))
+    r = _cli("-t", "t.fits", "-s", "s.fits", "-ip", str(tmp_path), "-op", str(tmp_path) + "/")
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert "Template image and science image must be the same size!" in r.stdout
+    assert not (tmp_path / "diff.fits").exists() and not (tmp_path / "sub.fits").exists()

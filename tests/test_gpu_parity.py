@@ -7,6 +7,8 @@ and the FP32 convolved and difference images.  (tests/_compare.py's looser
 image tolerance, 1e-5 |ref| + 1e-3 sigma_pix for conv and 1e-3 sigma_pix for
 diff, is checked first so a failure reports its size.)
 """
+import pathlib
+
 import numpy as np
 import pytest
 
@@ -172,3 +174,46 @@ def test_unsupported_method_constants_rejected(ctx):
     t = np.ones((128, 128), np.float32)
     with pytest.raises(xbach.XbachError, match="ERR_ARG"):
         ctx.run(xbach.Params.default(fKernelWidth=19, hKernelWidth=9), t, t)
+
+
+def test_sparse_field_runs_the_refill_retry(ctx):
+    # 20 stars on 512x512: fewer than 10 % of the stamps hold a substamp, so
+    # sss clears the SKIP marks and searches again (src/bach.cpp:142-161)
+    t, s = xbach.synth_pair(xbach.synth_cfg("pair512", nstars=20, seed=1))
+    _, _, tr = check_case(ctx, t, s)
+    assert tr["retried"] == 1
+
+
+def test_template_without_substamps_has_merit_666(ctx):
+    # a star-free template keeps no stamp: its direction's merit is the
+    # reference's 666 (src/cdkscUtil.cpp:101-103, bach.cpp:282-300), the science
+    # direction wins, and the clip-refit loop runs on a template with no stamps
+    tn, _ = xbach.synth_pair(xbach.synth_cfg("pair512", nstars=0, seed=8))
+    _, s = xbach.synth_pair(xbach.synth_cfg("pair512", seed=7))
+    _, _, tr = check_case(ctx, tn, s)
+    assert tr["stampCount"][0] == 0 and tr["merit"][0] == 666.0
+    assert tr["convTemplate"] == 1 and tr["retried"] == 1
+
+
+@pytest.mark.parametrize("tmpl_name,sci_name", [
+    ("test01_template", "test0_bg"),
+    ("test01_template", "test2"),
+    ("test0_bg", "test2"),
+    ("test0_bg", "test01_science"),
+])
+def test_reference_held_images(ctx, tmpl_name, sci_name):
+    # the reference's other 330x330 images (res/test0_bg.fits, res/test2.fits;
+    # no goldens): negative merits, 1 to 7 clip-refit passes, both directions
+    g = pathlib.Path(__file__).resolve().parent / "golden"
+    _, t = xbach.read_fits(g / f"{tmpl_name}.fits.gz")
+    _, s = xbach.read_fits(g / f"{sci_name}.fits.gz")
+    check_case(ctx, t.astype(np.float32), s.astype(np.float32))
+
+
+def test_image_too_small_for_a_stamp(ctx):
+    # 40x40 holds no 51-pixel stamp (the reference's fallback grid is 0x0)
+    z = np.full((40, 40), 100.0, np.float32)
+    with pytest.raises(xbach.XbachError, match="CAPACITY"):
+        ctx.run(xbach.Params.default(), z, z)
+    with pytest.raises(RuntimeError, match="CAPACITY"):
+        _oracle.run(xbach.Params.default(), z, z)

@@ -287,3 +287,41 @@ def test_histogram_recovers_gaussian_sky():
     sky, fw = _oracle.histogram(pix, padded, 100, mu, 1.0 / sig)
     assert abs(sky - mu) < 0.1 * sig
     assert abs(fw / sig - 1.0) < 0.1  # IQR / 1.35 of a normal is sigma
+
+
+def _reference_stamp_areas(w, h, sx, sy, fkw=21, fssw=31):
+    """Stamp sizes by the reference's arithmetic: the stamp width and the
+    "too many stamps" fallback (src/bach.cpp:51-66) and createStampBounds
+    (cl_kern/sss.cl:19-38)."""
+    fsw = min(w // sx, h // sy) - fkw
+    fsw -= 1 if fsw % 2 == 0 else 0
+    if fsw < fssw:
+        fsw = fssw + fkw
+        fsw -= 1 if fsw % 2 == 0 else 0
+        sx, sy = w // fsw, h // fsw
+    areas = []
+    for j in range(sy):
+        for i in range(sx):
+            x0, y0 = i * w // sx, j * h // sy
+            areas.append((min(x0 + fsw, w) - x0) * (min(y0 + fsw, h) - y0))
+    return areas
+
+
+def test_stamp_pixel_check_cannot_fire():
+    """The reference exits with "Not enough pixels in a stamp" when a stamp has
+    fewer than 100 pixels (src/bachUtil.cpp:149-155); the library keeps the
+    check (XB_ERR_STAMP_PIXELS).  Over every image side from 1 to 700 and every
+    grid up to 40x40 the reference's own geometry never makes such a stamp:
+    every stamp of a non-empty grid is at least 31 x 31, so no input can reach
+    that error (an empty grid is refused before it)."""
+    worst = None
+    for w in range(1, 701, 7):
+        for h in (w, max(1, w // 3), w + 57):
+            for sx in (1, 2, 3, 5, 8, 10, 13, 20, 27, 40):
+                for sy in (1, 2, 4, 10, 17, 40):
+                    a = _reference_stamp_areas(w, h, sx, sy)
+                    if a:
+                        m = min(a)
+                        worst = m if worst is None else min(worst, m)
+                        assert m >= 31 * 31, (w, h, sx, sy, m)
+    assert worst == 31 * 31

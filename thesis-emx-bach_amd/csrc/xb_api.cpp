@@ -1334,8 +1334,10 @@ int xb_get_trace(xb_ctx* x, xb_trace* tr) {
     tr->nTest[im] = c.nKept[im] ? sc.nTest[im] : 0;
     tr->merit[im] = c.nKept[im] ? sc.merit[im] : 666;
     tr->kernelSumCd[im] = c.nKept[im] ? sc.kernelSumCd[im] : 0.0;
-    XB_HIP(hipMemcpy(tr->cdSolution[im], c.cdSol + (size_t)im * NSOL, NSOL * sizeof(double),
-                     hipMemcpyDeviceToHost));
+    // a direction without stamps has no trial fit (cd returns 666 before it)
+    if (c.nKept[im])
+      XB_HIP(hipMemcpy(tr->cdSolution[im], c.cdSol + (size_t)im * NSOL, NSOL * sizeof(double),
+                       hipMemcpyDeviceToHost));
   }
   tr->convTemplate = c.convTemplate;
   tr->kscIterations = c.kscIterations;
