@@ -12,17 +12,21 @@ pairs that are already resident in HBM (outputs stay in HBM):
   runtime (up to --workers pairs in flight, the count per size bucket chosen
   by the batch autotuner before the timed batches, each pair through the same
   bit-identical pipeline); pair 0 of the batch is the plain pair, and its
-  outputs are checked to equal the plain run's bit for bit.
+  outputs are checked to equal the plain run's bit for bit;
+* mixed+batched+screened: the same batches with cmv on the matrix cores and
+  the learned substamp screen.
 
-`value` is the batched form's diffs/s (the plain form's figures are under
-"forms"); both forms produce identical difference images, so the residual
-RMS is equal by construction.  Prints one JSON line.
+Every form does --warmup untimed steps, then times --steps steps.  `value` is
+the mixed+batched+screened form's diffs/s when its residual RMS is within 1 %
+of the plain form's and of the FP64 oracle's (the metric's comparison); the
+other forms' figures are under "forms".  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--workers N] [--form both]
 
 cpu_baseline: the CPU restatement of the same path (oracle/, FP64 with the
-reference's FP32 steps) timed on this host's cores on a bounded number of
-whole diffs of the same pair.
+reference's FP32 steps) on this host's cores: one warm-up, then the median of
+--cpu-runs whole diffs of the same pair, with the 95 % rank interval of the
+reference's timing rule (tools/exetime.py:6-49).
 """
 from __future__ import annotations
 
@@ -55,6 +59,11 @@ PEAK_HBM_GBS = 8000.0
 # device copy of known size (bench.py --copy-calibration); the summary with
 # both commands is profiles/k_conv_2048_traffic.md.
 CONV_TRAFFIC_BYTES = 236_902_796
+# HBM bytes per global solve (the 23 kernels k_lu_scale, k_lu_begin, 19 x
+# k_lu_step, k_lu_backsub), measured the same way; profiles/lu_solve_traffic.md.
+LU_TRAFFIC_BYTES = 23_401_549
+LU_N = 292                                   # unknowns of the global system
+LU_FLOPS = 2.0 / 3.0 * LU_N ** 3             # ludcmp's multiply-adds, as flops
 
 WORKLOADS = {
     "pair2048": ("2048x2048 FP32 pair, 6000 stars (dN/dF ~ F^-1.5, 1e2-1e5), template FWHM 1.8 px, "
@@ -67,25 +76,48 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(params, tmpl, sci, seconds: float, threads: int = 0) -> dict:
-    """Time the oracle on this host on whole diffs of the same pair (threads 0:
-    all host cores)."""
+def rank_ci(times):
+    """95 % rank interval of the median by the reference's rule
+    (tools/exetime.py:6-21: z = (0.50798 + 0.51197) / 2, ranks
+    floor((n - z sqrt n) / 2) and ceil(1 + (n + z sqrt n) / 2) of the sorted
+    times)."""
+    import math
+
+    z = (0.50798 + 0.51197) / 2
+    t = sorted(times)
+    n = len(t)
+    lo = math.floor((n - z * math.sqrt(n)) / 2)
+    hi = min(n - 1, math.ceil(1 + (n + z * math.sqrt(n)) / 2))
+    return t[lo], t[hi]
+
+
+STAGES = ["Ini", "SSS", "CMV", "CD", "KSC", "Conv", "Sub", "Fin", "BACH"]
+
+
+def cpu_baseline(params, tmpl, sci, runs: int, threads: int = 0) -> dict:
+    """The oracle on this host: one warm-up, then `runs` whole diffs of the same
+    pair; diffs/s of the median run (threads 0: all host cores)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import _oracle  # test infrastructure: the CPU restatement
 
     threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     _oracle.run(params, tmpl, sci, threads=threads)  # warm-up
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        _oracle.run(params, tmpl, sci, threads=threads)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 200:
-            break
-    return {"value": n / el, "unit": "diffs/s", "cores": threads, "kind": "port",
-            "sample": f"{n} whole {tmpl.shape[1]}x{tmpl.shape[0]} diffs of the same pair "
-                      f"({el:.1f} s, OpenMP over stamps and rows, global LU serial)"}
+    times, stages = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        _, _, _, _, ms = _oracle.run(params, tmpl, sci, threads=threads)
+        times.append(time.perf_counter() - t0)
+        stages.append(ms)
+    med = float(np.median(times))
+    lo, hi = rank_ci(times)
+    st = np.median(np.array(stages), axis=0)
+    return {"value": 1.0 / med, "unit": "diffs/s", "cores": threads, "kind": "port",
+            "sample": f"median of {runs} whole {tmpl.shape[1]}x{tmpl.shape[0]} diffs of the same "
+                      f"pair after one warm-up ({sum(times):.1f} s; OpenMP over stamps and rows, "
+                      f"global LU serial)",
+            "median_s": med, "ci95_s": [lo, hi],
+            "ci_within_10pct": bool(lo >= 0.9 * med and hi <= 1.1 * med),
+            "stage_ms": {k: float(v) for k, v in zip(STAGES, st)}}
 
 
 def fp64_residual(params, tmpl, sci) -> float:
@@ -212,10 +244,10 @@ def same_bits(a, b) -> bool:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40, help="plain pipelines timed")
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps of every form (plain: pipelines; batched: batches)")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=48, help="pairs per batched step")
-    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--workers", type=int, default=16,
                     help="worker contexts (the most pairs in flight)")
     ap.add_argument("--eager", action="store_true",
@@ -225,7 +257,9 @@ def main() -> int:
                     help="run all workers' pairs at once instead of the autotuned counts")
     ap.add_argument("--form", default="both", choices=["both", "plain"])
     ap.add_argument("--workload", default="pair2048", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-runs", type=int, default=15,
+                    help="timed whole diffs of the CPU baseline (all cores; half as many, at "
+                         "least 10, on one thread)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 256 MiB (counter scale for "
@@ -280,6 +314,17 @@ def main() -> int:
     sub_bytes = 14.0 * w * h
     kernels = {k: {"ms_per_step": v[0] / args.steps, "launch_groups_per_step": v[1] / args.steps}
                for k, v in prof.items()}
+    # the global FP64 solve (ludcmp + lubksb of the 292-unknown system): the
+    # largest share of a plain diff.  Each launch group is one whole solve
+    # (23 kernels, HIP events on the stream it runs on); its arithmetic is
+    # (2/3) 292^3 flops, bound by the latency of 292 dependent pivot columns
+    # and of the 42,486-step ordered back substitution, not by FP64 throughput
+    solve_ms, solve_n = prof["solve"]
+    solve_avg_s = solve_ms / max(solve_n, 1) * 1e-3
+    solve_tflops = LU_FLOPS / solve_avg_s / 1e12
+    kernels["solve"].update({"ms_per_solve": solve_avg_s * 1e3,
+                             "us_per_column": solve_avg_s * 1e6 / LU_N,
+                             "achieved_TFLOPs": solve_tflops})
     kernels["sub"]["achieved_GBs"] = sub_bytes / sub_avg_s / 1e9
     kernels["conv"]["achieved_TFLOPs"] = conv_tflops
     # conv bytes: algorithmic (FP32 template in, FP32 + FP64 conv and the u16
@@ -290,19 +335,27 @@ def main() -> int:
         kernels["conv"]["achieved_GBs_measured_traffic"] = CONV_TRAFFIC_BYTES / conv_avg_s / 1e9
 
     # the same pipeline with host buffers in and out (H2D of both images, D2H of
-    # the convolved and difference images, pageable host memory)
+    # the convolved and difference images, pageable host memory), with the
+    # reference's nine-stage -vt breakdown (each stage synchronised)
     for _ in range(2):
         ctx.run(params, tmpl, sci)
     t0 = time.perf_counter()
     nh = max(3, args.steps // 4)
+    stage = []
     for _ in range(nh):
-        ctx.run(params, tmpl, sci)
+        stage.append(ctx.run(params, tmpl, sci)[2])
     host_ms = (time.perf_counter() - t0) * 1e3 / nh
+    capture_failures = int(ctx.trace().captureFailures)
 
     forms = {"plain": {"value": plain_value, "ms_per_diff": plain_ms, "steps": args.steps,
                        "warmup": args.warmup, "residual_rms": rms},
-             "plain_host_buffers": {"value": 1e3 / host_ms, "ms_per_diff": host_ms, "steps": nh,
-                                    "note": "PCIe-inclusive, not the headline value"}}
+             "plain_host_buffers": {
+                 "value": 1e3 / host_ms, "ms_per_diff": host_ms, "steps": nh,
+                 "note": "PCIe-inclusive, not the headline value",
+                 "stage_ms": {k: float(v) for k, v in zip(STAGES, np.median(stage, axis=0))},
+                 "stage_note": "-vt breakdown, a sync after each stage; cd and both "
+                               "directions' clip-refit loops run at once, so CD reads 0 and "
+                               "KSC holds both"}}
     value, ms_per_step, steps, warmup, form = plain_value, plain_ms, args.steps, args.warmup, "plain"
     ref_rms = None
     if not args.no_cpu:
@@ -313,7 +366,7 @@ def main() -> int:
         bufs = upload_pairs(ctx, pairs)
         keep = 8
         b_ms, res, dplain, btab = run_batched(ctx, params, pairs, bufs, args.workers,
-                                              args.batch_steps, 1, keep=keep,
+                                              args.steps, args.warmup, keep=keep,
                                               tune=not args.no_tune, flags=ctx_flags)
         ok = all(r.status == 0 for r in res)
         b_value = args.batch * 1e3 / b_ms
@@ -321,18 +374,16 @@ def main() -> int:
             "value": b_value, "ms_per_batch": b_ms, "pairs_per_batch": args.batch,
             "workers": args.workers, "pairs_in_flight_per_bucket": btab,
             "hw_queues": HW_QUEUES,
-            "steps": args.batch_steps, "warmup": 1, "all_pairs_ok": ok,
+            "steps": args.steps, "warmup": args.warmup, "all_pairs_ok": ok,
             "pair0_bit_identical_to_plain": same_bits(dplain[0], diff),
             "pair_ms_mean": float(np.mean([r.ms for r in res])),
+            "capture_failures": int(sum(r.captureFailures for r in res)),
             "speedup_vs_plain": b_value / plain_value}
-        if ok and forms["batched"]["pair0_bit_identical_to_plain"]:
-            value, ms_per_step, steps, warmup, form = (b_value, b_ms, args.batch_steps, 1,
-                                                       "batched")
         # mixed precision + batched + learned screen
         model = np.load(screen.MODEL_PATH) if screen.MODEL_PATH.exists() else None
         fform = xbach.FORM_MIXED | (xbach.FORM_SCREEN if model is not None else 0)
         f_ms, fres, dfull, ftab = run_batched(ctx, params, pairs, bufs, args.workers,
-                                              args.batch_steps, 1, form=fform, model=model,
+                                              args.steps, args.warmup, form=fform, model=model,
                                               keep=keep, tune=not args.no_tune,
                                               flags=ctx_flags)
         for q in bufs:
@@ -346,7 +397,7 @@ def main() -> int:
             "value": f_value, "ms_per_batch": f_ms, "pairs_per_batch": args.batch,
             "workers": args.workers, "pairs_in_flight_per_bucket": ftab, "form": "mixed" + ("+screened" if model is not None
                                                               else ""),
-            "steps": args.batch_steps, "warmup": 1, "all_pairs_ok": fok,
+            "steps": args.steps, "warmup": args.warmup, "all_pairs_ok": fok,
             "residual_rms_rel_vs_plain_max": float(max(rel)), "pairs_checked": len(rel),
             "clip_refit_passes_mean": float(np.mean([r.kscIterations for r in fres])),
             "clip_refit_passes_mean_plain": float(np.mean([r.kscIterations for r in res])),
@@ -357,9 +408,13 @@ def main() -> int:
             ref_rms is None or full["residual_rms_rel_vs_fp64_oracle_pair0"] <= 0.01)
         full["equal_residual_rms"] = bool(equal_rms)
         forms["mixed_batched_screened"] = full
-        if equal_rms and f_value > value:
-            value, ms_per_step, steps, warmup, form = (f_value, f_ms, args.batch_steps, 1,
+        full["speedup_vs_batched_plain"] = f_value / b_value
+        if equal_rms:
+            value, ms_per_step, steps, warmup, form = (f_value, f_ms, args.steps, args.warmup,
                                                        full["form"] + "+batched")
+        elif ok and forms["batched"]["pair0_bit_identical_to_plain"]:
+            value, ms_per_step, steps, warmup, form = (b_value, b_ms, args.steps, args.warmup,
+                                                       "batched")
         forms["cmv_mixed_mfma"] = cmv_mfma_rate(ctx, params, w, h, d_t, d_s)
 
     line = {
@@ -378,21 +433,36 @@ def main() -> int:
         "config": {"workload": WORKLOADS[args.workload], "name": args.workload,
                    "form": form, "w": w, "h": h, "seed": int(cfg.seed),
                    "pairs_per_step": 1 if form == "plain" else args.batch},
-        # k_conv issues FP64 VALU multiply-adds (unfused, in the reference's
-        # term order); MI355X's dense FP64 peak is 78.6 TFLOP/s
-        "roofline": {"bound": "mfma", "kernel": "k_conv (fp64 VALU, unfused mul+add)",
-                     "achieved": conv_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": conv_tflops / PEAK_FP64_TFLOPS,
-                     "traffic": CONV_TRAFFIC_BYTES if (w, h) == (2048, 2048) else None,
-                     "algorithmic": f"2*441*(W-20)*(H-20) = {conv_flops:.4g} flop per launch"},
+        # the dominant kernel group: the global FP64 solve, on the FP64 peak
+        # (78.6 TFLOP/s, the same for the vector units it runs on and the
+        # matrix cores); it is latency-bound, so its fraction is tiny
+        "roofline": {"bound": "mfma", "kernel": "global FP64 LU solve (k_lu_scale, k_lu_begin, "
+                     "19 x k_lu_step, k_lu_backsub; FP64 VALU)",
+                     "achieved": solve_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": solve_tflops / PEAK_FP64_TFLOPS,
+                     "traffic": LU_TRAFFIC_BYTES,
+                     "algorithmic": f"(2/3)*292^3 = {LU_FLOPS:.4g} flop per solve",
+                     "us_per_solve": solve_avg_s * 1e6, "us_per_column": solve_avg_s * 1e6 / LU_N,
+                     "limiter": "latency: 292 dependent pivot columns and a 42,486-step "
+                                "ordered back substitution (bit-identical to ludcmp/lubksb)"},
+        # the data-parallel stencil: FP64 VALU multiply-adds (unfused, in the
+        # reference's term order)
+        "roofline_conv": {"bound": "fp64 valu", "kernel": "k_conv",
+                          "achieved": conv_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                          "frac": conv_tflops / PEAK_FP64_TFLOPS,
+                          "traffic": CONV_TRAFFIC_BYTES if (w, h) == (2048, 2048) else None,
+                          "algorithmic": f"2*441*(W-20)*(H-20) = {conv_flops:.4g} flop per "
+                                         "launch"},
+        "capture_failures": capture_failures,
         "forms": forms,
         "kernels": kernels,
         "residual_rms": rms,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(params, tmpl, sci, args.cpu_seconds)
-        # the same restatement on one host thread (a bounded sample of whole diffs)
-        line["cpu_baseline_1thread"] = cpu_baseline(params, tmpl, sci, args.cpu_seconds / 2, 1)
+        line["cpu_baseline"] = cpu_baseline(params, tmpl, sci, args.cpu_runs)
+        # the same restatement on one host thread
+        line["cpu_baseline_1thread"] = cpu_baseline(params, tmpl, sci,
+                                                    max(10, args.cpu_runs // 2), 1)
         line["residual_rms_fp64_oracle"] = ref_rms
         line["residual_rms_rel_diff"] = abs(rms - ref_rms) / ref_rms
         line["gpu_vs_cpu"] = value / line["cpu_baseline"]["value"]
