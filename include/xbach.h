@@ -115,6 +115,10 @@ typedef struct xb_trace {
    * was created (the solve then ran its kernels directly; after two failures
    * the context stops capturing).  Zero in a healthy run. */
   int32_t captureFailures;
+  /* XB_FORM_SCREEN: substamps the screen replaced, and of those, the ones the
+   * chi2 re-check after convergence kept (restored, the loop then ran again
+   * without the screen) and the ones it rejected too.  Zero otherwise. */
+  int32_t screened, screenRestored, screenConfirmed;
 } xb_trace;
 
 void xb_params_default(xb_params* p);
@@ -297,9 +301,11 @@ int xb_set_form(xb_ctx* ctx, int form);
 int xb_screen_load(xb_ctx* ctx, const float* model);
 /* Samples of the last run with XB_FORM_RECORD or XB_FORM_SCREEN: for each
  * recorded pass (first pass first) the kept stamps of the convolved image in
- * kept order, feat[n][32], label[n] (2 replaced by that pass itself, 1 kept by
- * that pass but replaced later — by the loop or by the screen, 0 used by the
- * final fit, -1 no substamp), prob[n] (screen only, may be NULL).
+ * kept order, feat[n][32], label[n] (2 replaced by that pass itself; kept by
+ * that pass but not used by the final fit: 1 when the chi2 test under the
+ * converged fit rejects it, 3 when it passes that test (XB_FORM_RECORD probes
+ * each such substamp; under XB_FORM_SCREEN all are 1); 0 used by the final
+ * fit, -1 no substamp), prob[n] (screen only, may be NULL).
  * cap = capacity in samples; *n receives the count, *screened the number of
  * substamps the screen replaced.  Any output may be NULL. */
 int xb_screen_samples(xb_ctx* ctx, float* feat, int* label, float* prob, int cap, int* n,
@@ -363,6 +369,8 @@ typedef struct xb_pair_result {
   double kernSum;          /* kernel sum at the image centre                  */
   double ms;               /* wall time of the pair on its worker             */
   int32_t captureFailures; /* failed graph captures of its worker's context   */
+  int32_t screened;        /* XB_FORM_SCREEN: substamps the screen replaced    */
+  int32_t screenRestored;  /* ... that the chi2 re-check restored              */
   int32_t pad;
 } xb_pair_result;
 

@@ -3,14 +3,17 @@
 Bars: recording the screen's samples changes nothing — images and every
 decision bit-identical to the plain run; the recorded samples cover every
 kept stamp of the convolved image in every recorded pass, with labels in
-{-1, 0, 1, 2}.  With the committed model (screen_model.npy, threshold 0.95)
+{-1, 0, 1, 2, 3}.  With the committed model (screen_model.npy)
 on held-out contaminated crowded pairs (seeds never used in training), every
 pair's difference-image residual RMS stays within 1 % of the plain form's (two
 pairs also within 1 % of the FP64 oracle's), the screen replaces substamps,
-and the clip-refit passes in total are no more than the plain method's
-(measured on ten such pairs: 97 passes plain, 80 screened, 28.4 -> 26.2 ms per
-diff; on ten clean pairs of the 2048x2048 law 32 and 32, of the batching law
-33 and 31, residual RMS within 0.05 %).
+every substamp it replaced is judged by the chi2 re-check after the loop
+converged (restored or confirmed).  The re-check costs passes: about half of
+the screened substamps pass the chi2 test under the converged fit, are
+restored, and the loop runs again (six held-out pairs: 59 passes plain, 72
+screened, 88 screened substamps of which 39 restored); the test bounds that
+cost at 1.5x.  A model whose threshold screens nothing gives the plain form's
+results bit for bit.
 """
 import pathlib
 import sys
@@ -62,16 +65,16 @@ def test_record_changes_nothing(ctx, held_out):
     conv_im = 0 if tr0["convTemplate"] else 1
     npass = min(tr0["kscIterations"], 16)
     assert len(lab) == npass * tr0["stampCount"][conv_im]
-    assert set(np.unique(lab)) <= {-1, 0, 1, 2}
+    assert set(np.unique(lab)) <= {-1, 0, 1, 2, 3}
     assert f.shape == (len(lab), 32) and np.all(np.isfinite(f)) and scr == 0
 
 
 @pytest.mark.skipif(not screen.MODEL_PATH.exists(), reason="no trained screen model")
-def test_screen_cuts_clip_refit_passes(ctx, held_out):
+def test_screen_rechecked_by_chi2(ctx, held_out):
     p, pairs = held_out
     model = np.load(screen.MODEL_PATH)
     ctx.screen_load(model)
-    it_plain, it_scr, n_scr = [], [], 0
+    it_plain, it_scr, n_scr, n_res, n_conf = [], [], 0, 0, 0
     for i, (t, s) in enumerate(pairs):
         ctx.set_form(xbach.FORM_PLAIN)
         _, dp, _ = ctx.run(p, t, s)
@@ -79,8 +82,12 @@ def test_screen_cuts_clip_refit_passes(ctx, held_out):
         ctx.set_form(xbach.FORM_SCREEN)
         try:
             _, ds, _ = ctx.run(p, t, s)
-            it_scr.append(ctx.trace().kscIterations)
+            tr = ctx.trace()
+            it_scr.append(tr.kscIterations)
             n_scr += ctx.screen_samples()[3]
+            assert tr.screened == ctx.screen_samples()[3]
+            n_res += tr.screenRestored
+            n_conf += tr.screenConfirmed
         finally:
             ctx.set_form(xbach.FORM_PLAIN)
         rp, rs = _compare.residual_rms(dp), _compare.residual_rms(ds)
@@ -89,6 +96,38 @@ def test_screen_cuts_clip_refit_passes(ctx, held_out):
             _, d64, _, _, _ = _oracle.run(p, t, s, mode=_oracle.MODE_FP64)
             r64 = _compare.residual_rms(d64)
             assert abs(rs - r64) <= 0.01 * r64, (i, rs, r64)
-    print(f"clip-refit passes plain {it_plain} screened {it_scr}; substamps screened {n_scr}")
+    print(f"clip-refit passes plain {it_plain} screened {it_scr}; substamps screened {n_scr}, "
+          f"restored by the chi2 re-check {n_res}, confirmed {n_conf}")
     assert n_scr > 0
-    assert sum(it_scr) <= sum(it_plain)
+    # every screened substamp was judged by the chi2 re-check
+    assert n_res + n_conf == n_scr
+    # what the screen costs once every screened substamp is re-judged: about
+    # half of them pass the chi2 test under the converged fit and re-open the
+    # loop (measured: 72 passes against 59 on these six pairs); bounded here
+    assert sum(it_scr) <= 1.5 * sum(it_plain)
+
+
+@pytest.mark.skipif(not screen.MODEL_PATH.exists(), reason="no trained screen model")
+def test_screen_that_screens_nothing_equals_plain(ctx, held_out):
+    # threshold above 1: no substamp is screened, so the screened form's
+    # decisions and images equal the plain form's bit for bit
+    p, pairs = held_out
+    t, s = pairs[1]
+    model = np.load(screen.MODEL_PATH).copy()
+    model[screen.TAU] = 2.0
+    ctx.set_form(xbach.FORM_PLAIN)
+    c0, d0, _ = ctx.run(p, t, s)
+    tr0 = ctx.trace().as_dict()
+    ctx.screen_load(model)
+    ctx.set_form(xbach.FORM_SCREEN)
+    try:
+        c1, d1, _ = ctx.run(p, t, s)
+        tr = ctx.trace()
+        tr1 = tr.as_dict()
+    finally:
+        ctx.set_form(xbach.FORM_PLAIN)
+    assert tr.screened == 0 and tr.screenRestored == 0 and tr.screenConfirmed == 0
+    assert not _compare.trace_mismatches(tr1, tr0)
+    for a, b in ((c1, c0), (d1, d0)):
+        nan = np.isnan(b)
+        assert np.array_equal(a[~nan].view(np.uint32), b[~nan].view(np.uint32))

@@ -744,6 +744,8 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
       ks->firstClip[0] = mean;
       ks->firstClip[1] = sd;
       ks->nScreened = 0;
+      ks->nRestored = 0;
+      ks->nConfirmed = 0;
     }
   }
 }
@@ -811,6 +813,31 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
   A.sol = d_sol;
   A.sigOut = useList ? c.st[im].sig : c.st[im].sigK;
   A.nanMark = nanMark;
+  if (c.nKept[im] <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d_nList,
+                                 const double* d_sol, double* out) {
+  SigArgs A;
+  A.tImg = c.img[im];
+  A.sImg = c.img[1 - im];
+  A.mask = c.mask;
+  A.w = c.w;
+  A.h = c.h;
+  A.W = c.st[im].W;
+  A.kept = c.st[im].kept;
+  A.cur = c.st[im].cur;
+  A.count = c.st[im].count;
+  A.ssx = c.st[im].ssx;
+  A.ssy = c.st[im].ssy;
+  A.list = d_list;
+  A.nList = d_nList;
+  A.basis = c.basis;
+  A.sol = d_sol;
+  A.sigOut = out;
+  A.nanMark = XB_MASK_NAN_PIXEL;
   if (c.nKept[im] <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur, A);
   return hipGetLastError();

@@ -283,8 +283,9 @@ __global__ __launch_bounds__(WAVE) void k_screen_decide(const float* __restrict_
 // list is rebuilt in ascending stamp order from the pass's own replacements
 // (cur != passCur) and the screened ones
 __global__ __launch_bounds__(1024) void k_screen_merge(const int* passCur, int* cur,
-                                                       const uint8_t* flag, int* list,
-                                                       const int* nKept, KscScalars* ks) {
+                                                       const uint8_t* flag, uint8_t* bits,
+                                                       int* list, const int* nKept,
+                                                       KscScalars* ks) {
   __shared__ int isc[20];
   __shared__ int s_n, s_scr;
   const int tid = threadIdx.x;
@@ -298,7 +299,8 @@ __global__ __launch_bounds__(1024) void k_screen_merge(const int* passCur, int* 
     const int s = base + tid;
     bool r = false;
     if (s < n) {
-      if (flag[s]) {
+      if (flag[s]) {  // screened out: remembered for the chi2 re-check
+        bits[s] = (uint8_t)(bits[s] | (1u << cur[s]));
         cur[s] = cur[s] + 1;
         atomicAdd(&s_scr, 1);
       }
@@ -319,6 +321,95 @@ __global__ __launch_bounds__(1024) void k_screen_merge(const int* passCur, int* 
   }
 }
 
+// ---------------------------------------------------------------------------
+// chi2 re-check of the screened substamps, after the loop converged
+// ---------------------------------------------------------------------------
+// Each stamp with screened substamps left probes its lowest one: its current
+// substamp is kept in vOrig and the screened one made current (the fill and
+// the chi2 pass that follow see it); the probed stamps are listed in stamp
+// order.
+__global__ __launch_bounds__(1024) void k_verify_select(const int* nKept, int* cur,
+                                                        const uint8_t* bits, int* vOrig,
+                                                        int* vList, KscScalars* ks) {
+  __shared__ int isc[20];
+  __shared__ int s_n;
+  const int tid = threadIdx.x;
+  const int n = *nKept;
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += blockDim.x) {
+    const int s = base + tid;
+    const unsigned b = s < n ? bits[s] : 0u;
+    if (b) {
+      vOrig[s] = cur[s];
+      cur[s] = __builtin_ctz(b);
+    }
+    int tot;
+    const int pos = block_prefix_count(b != 0u, isc, &tot);
+    const int c0 = s_n;
+    if (b) vList[c0 + pos] = s;
+    __syncthreads();
+    if (tid == 0) s_n = c0 + tot;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    ks->nVerify = s_n;
+    ks->nRestoredRound = 0;
+  }
+}
+
+// The reference's rejection tests of a pass (ksc.cl:1-72,
+// src/cdkscUtil.cpp:686-770) applied to each probe with the converged fit and
+// the last pass's clip: chi2 -1 (no valid pixel) or chi2 - mean > sigKernFit
+// sd rejects.  A kept probe stays the stamp's substamp (restored: the loop
+// then runs again without the screen); a rejected one is confirmed and the
+// stamp returns to its substamp, listed for a refill.  Either way the probed
+// bit is cleared.
+__global__ __launch_bounds__(1024) void k_verify_decide(const int* vList, int* cur,
+                                                        const int* vOrig, uint8_t* bits,
+                                                        const double* vSig, int* list,
+                                                        KscScalars* ks, double sigKernFit) {
+  __shared__ int isc[20];
+  __shared__ int s_n, s_kept;
+  const int tid = threadIdx.x;
+  const int n = ks->nVerify;
+  const double mean = ks->clipMean, sd = ks->clipStd;
+  if (tid == 0) {
+    s_n = 0;
+    s_kept = 0;
+  }
+  __syncthreads();
+  for (int base = 0; base < n; base += blockDim.x) {
+    const int u = base + tid;
+    bool back = false;
+    if (u < n) {
+      const int s = vList[u];
+      const double v = vSig[u];
+      const bool reject = v == -1.0 || (v - mean) > sigKernFit * sd;
+      bits[s] = (uint8_t)(bits[s] & ~(1u << cur[s]));
+      if (reject) {
+        cur[s] = vOrig[s];
+        back = true;
+      } else {
+        atomicAdd(&s_kept, 1);
+      }
+    }
+    int tot;
+    const int pos = block_prefix_count(back, isc, &tot);
+    const int c0 = s_n;
+    if (back) list[c0 + pos] = vList[u];
+    __syncthreads();
+    if (tid == 0) s_n = c0 + tot;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    ks->nList = s_n;
+    ks->nRestoredRound = s_kept;
+    ks->nRestored += s_kept;
+    ks->nConfirmed += n - s_kept;
+  }
+}
+
 // labels of a pass's substamps, right after its check: 2 = the pass
 // replaced it, 0 = kept so far, -1 = no substamp
 __global__ void k_screen_first(const int* nKept, const int* kept, const int* passCur,
@@ -329,13 +420,54 @@ __global__ void k_screen_first(const int* nKept, const int* kept, const int* pas
   label[s] = passCur[s] < count[g] ? (cur[s] != passCur[s] ? 2 : 0) : -1;
 }
 
-// after the loop, over nRec passes of n stamps: 1 = kept by its pass but
-// replaced later (by the loop or by the screen), 0 = the final fit uses it
+// after the loop, over nRec passes of n stamps: a substamp kept by its pass
+// that the final fit does not use is 1 when the chi2 test under the converged
+// fit rejects it (finalRej, from the recording's probes; all of them when
+// finalRej is null) and 3 when it passes that test; 0 = the final fit uses it
 __global__ void k_screen_labels(int n, int nRec, const int* passCur, const int* cur,
-                                int* label) {
+                                const uint8_t* finalRej, int* label) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n * nRec) return;
-  if (label[e] == 0 && cur[e % n] != passCur[e]) label[e] = 1;
+  const int s = e % n;
+  if (label[e] == 0 && cur[s] != passCur[e])
+    label[e] = (!finalRej || finalRej[s * MAXSS + passCur[e]]) ? 1 : 3;
+}
+
+// The substamps a recorded pass used that the converged fit does not: bit c of
+// bits[s] for each such substamp c of stamp s (the recording probes them)
+__global__ void k_label_bits(int n, int nRec, const int* passCur, const int* cur,
+                             const int* kept, const int* count, uint8_t* bits) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  unsigned b = 0;
+  const int cnt = count[kept[s]];
+  for (int p = 0; p < nRec; p++) {
+    const int c = passCur[(size_t)p * n + s];
+    if (c != cur[s] && c < cnt) b |= 1u << c;
+  }
+  bits[s] = (uint8_t)b;
+}
+
+// A recording's probe round: the chi2 test of each probe under the converged
+// fit and the last pass's clip (the rule of k_verify_decide) is stored in
+// finalRej[s][c]; every stamp returns to its own substamp and is listed for a
+// refill.
+__global__ __launch_bounds__(1024) void k_label_decide(const int* vList, int* cur,
+                                                       const int* vOrig, uint8_t* bits,
+                                                       const double* vSig, int* list,
+                                                       KscScalars* ks, double sigKernFit,
+                                                       uint8_t* finalRej) {
+  const int n = ks->nVerify;
+  const double mean = ks->clipMean, sd = ks->clipStd;
+  for (int u = threadIdx.x; u < n; u += blockDim.x) {
+    const int s = vList[u], c = cur[s];
+    const double v = vSig[u];
+    finalRej[s * MAXSS + c] = (v == -1.0 || (v - mean) > sigKernFit * sd) ? 1 : 0;
+    bits[s] = (uint8_t)(bits[s] & ~(1u << c));
+    cur[s] = vOrig[s];
+    list[u] = s;
+  }
+  if (threadIdx.x == 0) ks->nList = n;
 }
 
 // ---------------------------------------------------------------------------
@@ -474,15 +606,44 @@ hipError_t launch_screen(Ctx& c, int im, int pass, bool apply) {
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_screen_merge, dim3(1), dim3(1024), 0, c.cur, passCur, c.st[im].cur,
-                     c.scr.flag, c.st[im].listK, &c.sc->nKept[im], &c.sc->k[im]);
+                     c.scr.flag, c.scr.bits, c.st[im].listK, &c.sc->nKept[im], &c.sc->k[im]);
   return hipGetLastError();
 }
 
-hipError_t launch_screen_labels(Ctx& c, int im, int nRec) {
+hipError_t launch_screen_verify_select(Ctx& c, int im) {
+  hipLaunchKernelGGL(k_verify_select, dim3(1), dim3(1024), 0, c.cur, &c.sc->nKept[im],
+                     c.st[im].cur, c.scr.bits, c.scr.vOrig, c.scr.vList, &c.sc->k[im]);
+  return hipGetLastError();
+}
+
+hipError_t launch_screen_verify_decide(Ctx& c, int im) {
+  hipLaunchKernelGGL(k_verify_decide, dim3(1), dim3(1024), 0, c.cur, c.scr.vList,
+                     c.st[im].cur, c.scr.vOrig, c.scr.bits, c.scr.vSig, c.st[im].listK,
+                     &c.sc->k[im], c.m.sigKernFit);
+  return hipGetLastError();
+}
+
+hipError_t launch_screen_labels(Ctx& c, int im, int nRec, bool finalFit) {
   const int N = c.nKept[im];
   if (N <= 0 || nRec <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_screen_labels, dim3((N * nRec + 255) / 256), dim3(256), 0, c.cur, N,
-                     nRec, c.scr.passCur, c.st[im].cur, c.scr.label);
+                     nRec, c.scr.passCur, c.st[im].cur, finalFit ? c.scr.finalRej : nullptr,
+                     c.scr.label);
+  return hipGetLastError();
+}
+
+hipError_t launch_label_bits(Ctx& c, int im, int nRec) {
+  const int N = c.nKept[im];
+  if (N <= 0 || nRec <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_label_bits, dim3((N + 255) / 256), dim3(256), 0, c.cur, N, nRec,
+                     c.scr.passCur, c.st[im].cur, c.st[im].kept, c.st[im].count, c.scr.bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_label_decide(Ctx& c, int im) {
+  hipLaunchKernelGGL(k_label_decide, dim3(1), dim3(1024), 0, c.cur, c.scr.vList, c.st[im].cur,
+                     c.scr.vOrig, c.scr.bits, c.scr.vSig, c.st[im].listK, &c.sc->k[im],
+                     c.m.sigKernFit, c.scr.finalRej);
   return hipGetLastError();
 }
 

@@ -520,6 +520,11 @@ int ensureScreen(Ctx& c) {
   dfree(&S.prob);
   dfree(&S.flag);
   dfree(&S.zPrev);
+  dfree(&S.bits);
+  dfree(&S.vOrig);
+  dfree(&S.vList);
+  dfree(&S.vSig);
+  dfree(&S.finalRej);
   const size_t np = (size_t)SCR_PASSES * c.capStamps;
   XB_HIP(dalloc(&S.feat, np * XB_MLP_NFEAT));
   XB_HIP(dalloc(&S.passCur, np));
@@ -527,6 +532,11 @@ int ensureScreen(Ctx& c) {
   XB_HIP(dalloc(&S.prob, np));
   XB_HIP(dalloc(&S.flag, (size_t)c.capStamps));
   XB_HIP(dalloc(&S.zPrev, (size_t)c.capStamps));
+  XB_HIP(dalloc(&S.bits, (size_t)c.capStamps));
+  XB_HIP(dalloc(&S.vOrig, (size_t)c.capStamps));
+  XB_HIP(dalloc(&S.vList, (size_t)c.capStamps));
+  XB_HIP(dalloc(&S.vSig, (size_t)c.capStamps));
+  XB_HIP(dalloc(&S.finalRej, (size_t)c.capStamps * MAXSS));
   S.cap = c.capStamps;
   return XB_OK;
 }
@@ -573,13 +583,40 @@ int kscPass(Ctx& c, int im, int it, bool keep, bool screen, uint16_t nanMark,
 
 constexpr int KSC_MAX_PASSES = 3 * XB_MAX_STAMPS + 2;
 
-// ksc (src/cdkscUtil.cpp:585-684): the loop of the chosen direction, after cd
+// One chi2 re-check round of the screened substamps (stageKsc): each stamp
+// with screened substamps left probes its lowest one (filled, then its chi2
+// under the converged fit), the reference's reject rule decides, and stamps
+// whose probe is rejected get their own substamp's fit data back.  *probed:
+// stamps probed; *kept: probes the chi2 test kept (restored).
+int screenRecheck(Ctx& c, int im, int* probed, int* kept) {
+  const double* sol = c.ksSol + (size_t)im * NSOL;
+  KscScalars* ks = &c.sc->k[im];
+  XB_HIP(launch_screen_verify_select(c, im));
+  XB_HIP(launch_fill(c, im, c.scr.vList, &ks->nVerify, c.nKept[im]));
+  XB_HIP(launch_calc_sigs_list(c, im, c.scr.vList, &ks->nVerify, sol, c.scr.vSig));
+  XB_HIP(launch_screen_verify_decide(c, im));
+  XB_HIP(launch_fill(c, im, c.st[im].listK, &ks->nList, c.nKept[im]));
+  XB_HIP(readBack(c, kscOffset(im) + offsetof(KscScalars, nVerify), 2 * sizeof(int)));
+  *probed = c.hsc->k[im].nVerify;
+  *kept = c.hsc->k[im].nRestoredRound;
+  return XB_OK;
+}
+
+// ksc (src/cdkscUtil.cpp:585-684): the loop of the chosen direction, after cd.
+// With the learned screen, the loop that converged is followed by the chi2
+// re-check of every substamp the screen replaced: re-check rounds run until
+// none is left or one is kept; a kept one is restored and the loop runs again
+// without the screen, then the re-check resumes on what is left.  Every
+// screened substamp is thus judged by the true chi2 test under a converged
+// fit, and the fit finally uses only substamps that passed it.
 int stageKsc(Ctx& c) {
   const int im = c.tIdx;
   const bool screen = (c.form & XB_FORM_SCREEN) != 0;
   const bool keep = screen || (c.form & XB_FORM_RECORD) != 0;
   c.scrSamples = 0;
   c.nScreened = 0;
+  c.nRestored = 0;
+  c.nConfirmed = 0;
   if (screen && !c.scr.haveModel) {
     c.err = "XB_FORM_SCREEN needs a model (xb_screen_load)";
     return XB_ERR_ARG;
@@ -589,24 +626,62 @@ int stageKsc(Ctx& c) {
     if (rc) return rc;
   }
   c.cur = c.stream;
+  if (screen) XB_HIP(hipMemsetAsync(c.scr.bits, 0, (size_t)c.nKept[im], c.stream));
+  bool screenOn = screen;
   int it = 0;
   for (;;) {
-    int rc = kscPass(c, im, it, keep, screen, XB_MASK_NAN_PIXEL, nullptr);
+    int rc = kscPass(c, im, it, keep, screenOn, XB_MASK_NAN_PIXEL, nullptr);
     if (rc) return rc;
     it++;
     XB_HIP(spinEvent(c.evK[im]));
-    if (!c.hsc->k[im].check) break;
     if (it > KSC_MAX_PASSES) {
       c.err = "ksc loop did not terminate";
       return XB_ERR_HIP;
     }
+    if (c.hsc->k[im].check) continue;
+    if (!screen) break;
+    bool restored = false;
+    for (;;) {
+      int probed = 0, kept = 0;
+      rc = screenRecheck(c, im, &probed, &kept);
+      if (rc) return rc;
+      if (probed == 0) break;
+      if (kept > 0) {
+        restored = true;
+        break;
+      }
+    }
+    if (!restored) break;
+    screenOn = false;  // the restored substamps are judged by plain passes
   }
   c.kscIterations = it;
   if (keep) {
     const int nRec = std::min(it, SCR_PASSES);
-    XB_HIP(launch_screen_labels(c, im, nRec));
-    XB_HIP(readBack(c, kscOffset(im) + offsetof(KscScalars, nScreened), sizeof(int)));
-    if (screen) c.nScreened = c.hsc->k[im].nScreened;
+    if (!screen) {
+      // recording: every substamp a recorded pass used that the converged fit
+      // does not is probed (filled, chi2 under that fit, the reference's
+      // reject rule with the last clip), then its stamp gets its own
+      // substamp's fit data back; the labels follow those verdicts
+      const double* sol = c.ksSol + (size_t)im * NSOL;
+      KscScalars* ks = &c.sc->k[im];
+      XB_HIP(launch_label_bits(c, im, nRec));
+      for (;;) {
+        XB_HIP(launch_screen_verify_select(c, im));
+        XB_HIP(launch_fill(c, im, c.scr.vList, &ks->nVerify, c.nKept[im]));
+        XB_HIP(launch_calc_sigs_list(c, im, c.scr.vList, &ks->nVerify, sol, c.scr.vSig));
+        XB_HIP(launch_label_decide(c, im));
+        XB_HIP(launch_fill(c, im, c.st[im].listK, &ks->nList, c.nKept[im]));
+        XB_HIP(readBack(c, kscOffset(im) + offsetof(KscScalars, nVerify), sizeof(int)));
+        if (c.hsc->k[im].nVerify == 0) break;
+      }
+    }
+    XB_HIP(launch_screen_labels(c, im, nRec, !screen));
+    XB_HIP(readBack(c, kscOffset(im), sizeof(KscScalars)));
+    if (screen) {
+      c.nScreened = c.hsc->k[im].nScreened;
+      c.nRestored = c.hsc->k[im].nRestored;
+      c.nConfirmed = c.hsc->k[im].nConfirmed;
+    }
     c.scrSamples = nRec * c.nKept[im];
   }
   return XB_OK;
@@ -809,6 +884,8 @@ void freeAll(Ctx& c) {
   dfree(&c.tileK64); dfree(&c.tileAks);
   dfree(&c.scr.feat); dfree(&c.scr.passCur); dfree(&c.scr.label); dfree(&c.scr.prob);
   dfree(&c.scr.flag); dfree(&c.scr.zPrev); dfree(&c.scr.model);
+  dfree(&c.scr.bits); dfree(&c.scr.vOrig); dfree(&c.scr.vList); dfree(&c.scr.vSig);
+  dfree(&c.scr.finalRej);
   c.scr.cap = 0;
   c.scr.haveModel = false;
   if (c.mixTab) (void)locked_free(c.mixTab);
@@ -1350,6 +1427,9 @@ int xb_get_trace(xb_ctx* x, xb_trace* tr) {
   }
   tr->kernSum = sc.kernSum;
   tr->captureFailures = c.captureFailures;
+  tr->screened = c.nScreened;
+  tr->screenRestored = c.nRestored;
+  tr->screenConfirmed = c.nConfirmed;
   return XB_OK;
 }
 

@@ -106,6 +106,8 @@ int runPair(xb_ctx* x, const Job& J, int i, xb_pair_result* r) {
     r->kscIterations = c.kscIterations;
     r->kernSum = c.kernSum;
     r->captureFailures = c.captureFailures;
+    r->screened = c.nScreened;
+    r->screenRestored = c.nRestored;
   }
   return XB_OK;
 }

@@ -116,6 +116,10 @@ struct KscScalars {
   int nList;            // entries of the refill list
   int nChi2;            // stamps with a valid chi2 in the pass
   int nScreened;        // substamps the learned screen replaced
+  int nVerify;          // screened substamps probed by this verification round
+  int nRestoredRound;   // ... of which the chi2 test kept (restored)
+  int nRestored;        // restored screened substamps, whole loop
+  int nConfirmed;       // screened substamps the chi2 test also rejects
   double clipMean, clipStd;
   double firstClip[2];  // clip mean and sd of the first pass
 };
@@ -153,6 +157,11 @@ struct ScreenBufs {
   int* label = nullptr;     // [pass][N] 2 replaced by the pass, 1 later, 0 kept, -1 none
   float* prob = nullptr;    // [pass][N] p(reject)
   uint8_t* flag = nullptr;  // [N] screened out in the current pass
+  uint8_t* bits = nullptr;  // [N] bit c: substamp c was screened out, not yet re-checked
+  int* vOrig = nullptr;     // [N] substamp before a re-check probe
+  int* vList = nullptr;     // [N] stamps probed by a re-check round
+  double* vSig = nullptr;   // [N] their chi2 under the final fit (list order)
+  uint8_t* finalRej = nullptr;  // [N][MAXSS] recording: the converged fit rejects substamp c
   float* zPrev = nullptr;   // [N] chi2 z-score of the previous pass
   float* model = nullptr;   // [XB_MLP_NPARAM]
   int cap = 0;
@@ -221,7 +230,9 @@ struct Ctx {
   double* cdSol = nullptr;      // [2][NSOL]
   int form = 0;                 // XB_FORM_* of the next run
   ScreenBufs scr;
-  int nScreened = 0;
+  int nScreened = 0;            // substamps the screen replaced (last run)
+  int nRestored = 0;            // ... that the chi2 re-check kept (restored)
+  int nConfirmed = 0;           // ... that the chi2 re-check also rejected
   int scrSamples = 0;           // samples the last ksc recorded
   void* mixTab = nullptr;       // mixed form: split BF16 2-D basis, MFMA fragment order
   float mixBg[3] = {0, 0, 0};   // basis widths the table was built for
@@ -281,6 +292,9 @@ hipError_t launch_cd_finish(Ctx& c, int im);
 // stamp's into st.sigK.  nanMark is the mask bit a NaN model pixel sets.
 hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
                             uint16_t nanMark = XB_MASK_NAN_PIXEL);
+// chi2 of the listed stamps (d_list, d_nList entries) into out (list order)
+hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d_nList,
+                                 const double* d_sol, double* out);
 hipError_t launch_ksc_check(Ctx& c, int im, bool first);
 hipError_t launch_conv(Ctx& c);
 // Restores the fit data of image im to what cmv left (every kept stamp on its
@@ -294,7 +308,18 @@ constexpr int DIAG_BLOCKS = 512;
 // learned screen (k_screen.hip): features of a pass's substamps, and with
 // apply, the network's decision merged into the pass's refill list
 hipError_t launch_screen(Ctx& c, int im, int pass, bool apply);
-hipError_t launch_screen_labels(Ctx& c, int im, int nRec);
+// Labels of the recorded samples; finalFit: by the recording's probes under
+// the converged fit (launch_label_bits / launch_label_decide rounds)
+hipError_t launch_screen_labels(Ctx& c, int im, int nRec, bool finalFit);
+hipError_t launch_label_bits(Ctx& c, int im, int nRec);
+hipError_t launch_label_decide(Ctx& c, int im);
+// The chi2 re-check of screened substamps after the loop converged: select
+// (each stamp's lowest screened substamp becomes current: probe), then, after
+// the probes are filled and their chi2 formed, decide (the reference's reject
+// rule with the last pass's clip: a kept probe stays, a rejected one returns
+// its stamp to the substamp it had and lists it for a refill).
+hipError_t launch_screen_verify_select(Ctx& c, int im);
+hipError_t launch_screen_verify_decide(Ctx& c, int im);
 // Device address of the static gradient scratch on `device` (k_screen.hip);
 // looked up with hipGetSymbolAddress once per device and cached.
 constexpr int XB_MAX_DEVICES = 64;

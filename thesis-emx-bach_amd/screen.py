@@ -6,15 +6,16 @@ substamps one pass at a time.  The screen (XB_FORM_SCREEN, k_screen.hip) is a
 32-64-1 network that, after the loop's first pass, predicts which substamps the
 loop will replace later and replaces them at once; the loop still runs and
 judges.  Its training data come from the method itself: XB_FORM_RECORD keeps
-the 32 features of every first-pass substamp and whether the plain loop
-finally replaced it.
+the 32 features of every substamp each recorded pass kept, and whether the chi2
+test under the plain loop's converged fit rejects it (the test the screened
+form's re-check applies to every substamp the screen replaces).
 
 Crowded-field pairs follow the learned-screen workload: crowded fields with 20 %
 of the selected substamps contaminated by cosmic-ray hits (1-3 px, 5e3-5e4
 counts), stars saturated and clipped at 6e4, and variable sources (science
 flux x U[0.5, 2]).
 
-    python screen.py --pairs 120 --clean 120 --out screen_model.npy   (needs the GPU)
+    python screen.py --pairs 340 --clean 160 --out screen_model.npy   (needs the GPU)
 """
 from __future__ import annotations
 
@@ -146,10 +147,36 @@ def training_sets(ctx, n_crowded: int, n_clean: int, seed0: int = 900000):
             (p10, contaminated_pairs(ctx, p10, pair2048, frac=0.0))]
 
 
-def collect(ctx, params, pairs) -> tuple[np.ndarray, np.ndarray, list[int]]:
+def label_samples(f, lab, passes, target="final"):
+    """Samples and targets of one recorded run.  The recording holds, pass by
+    pass, every kept stamp's label (2 replaced by that pass; of the ones the
+    pass kept: 1 rejected by the chi2 test under the converged fit, 3 replaced
+    later but passing that test, 0 used by the final fit; -1 no substamp).
+    target "next": a substamp a pass kept is positive when the next pass
+    replaces it (label 2 there), so screening it does that pass's work one
+    solve early; the last recorded pass has no next pass unless it is the
+    loop's last (which replaces nothing).  target "final": positive when the
+    converged fit's chi2 test rejects it."""
+    nrec = min(passes, 16)
+    n = len(lab) // max(nrec, 1)
+    L = lab.reshape(nrec, n)
+    F = f.reshape(nrec, n, -1)
+    kept = (L == 0) | (L == 1) | (L == 3)
+    if target == "final":
+        return F[kept], (L[kept] == 1).astype(np.float32)
+    y = np.zeros_like(L, dtype=np.float32)
+    y[:-1] = (L[1:] == 2)
+    if nrec < passes:  # the pass after the last recorded one is unknown
+        kept[-1] = False
+    return F[kept], y[kept]
+
+
+def collect(ctx, params, pairs, target="final") -> tuple[np.ndarray, np.ndarray, list[int]]:
     """Method-labelled samples: run the plain method with XB_FORM_RECORD and
     keep, for every recorded pass, the substamps that pass kept, labelled 1
-    when a later pass replaced them."""
+    when the converged fit does not use them and its chi2 test rejects them
+    (label 3, replaced but passing that test, and 0, used by the final fit, are
+    the negatives)."""
     Xs, ys, its = [], [], []
     try:
         for t, s in pairs:
@@ -159,10 +186,13 @@ def collect(ctx, params, pairs) -> tuple[np.ndarray, np.ndarray, list[int]]:
             except xbach.XbachError:
                 continue
             f, lab, _, _ = ctx.screen_samples()
-            ok = (lab == 0) | (lab == 1)  # the substamps each pass kept
-            Xs.append(f[ok])
-            ys.append(lab[ok].astype(np.float32))
-            its.append(ctx.trace().kscIterations)
+            if len(its) % 20 == 19:
+                print(f"  {len(its) + 1} pairs recorded", flush=True)
+            it = ctx.trace().kscIterations
+            X1, y1 = label_samples(f, lab, it, target)
+            Xs.append(X1)
+            ys.append(y1)
+            its.append(it)
     finally:
         ctx.set_form(xbach.FORM_PLAIN)
     return np.concatenate(Xs), np.concatenate(ys), its
@@ -213,18 +243,19 @@ def train(ctx, X, y, seed=1, epochs=60, batch=1024, lr=3e-3):
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--pairs", type=int, default=120, help="contaminated crowded pairs")
-    ap.add_argument("--clean", type=int, default=120, help="clean pairs")
+    ap.add_argument("--pairs", type=int, default=340, help="contaminated crowded pairs")
+    ap.add_argument("--clean", type=int, default=160, help="clean pairs")
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--out", default=str(MODEL_PATH))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--min-precision", type=float, default=0.9)
+    ap.add_argument("--target", default="final", choices=["next", "final"])
     args = ap.parse_args()
     ctx = xbach.Context(0)
     t0 = time.time()
     Xs, ys, its = [], [], []
     for params, src in training_sets(ctx, args.pairs, args.clean):
-        X1, y1, it1 = collect(ctx, params, src)
+        X1, y1, it1 = collect(ctx, params, src, args.target)
         Xs.append(X1)
         ys.append(y1)
         its += it1

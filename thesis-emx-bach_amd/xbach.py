@@ -116,6 +116,8 @@ class Trace(C.Structure):
         ("finalCurrent", C.c_int32 * MAX_STAMPS),
         ("kernSum", C.c_double),
         ("captureFailures", C.c_int32),
+        ("screened", C.c_int32), ("screenRestored", C.c_int32),
+        ("screenConfirmed", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -665,7 +667,8 @@ class PairResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("convTemplate", C.c_int32),
                 ("kscIterations", C.c_int32), ("worker", C.c_int32),
                 ("kernSum", C.c_double), ("ms", C.c_double),
-                ("captureFailures", C.c_int32), ("pad", C.c_int32)]
+                ("captureFailures", C.c_int32), ("screened", C.c_int32),
+                ("screenRestored", C.c_int32), ("pad", C.c_int32)]
 
 
 def _ptr_array(ptrs) -> C.Array:
