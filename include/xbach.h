@@ -288,6 +288,14 @@ int xb_sync(xb_ctx* ctx);
  * method itself.  Forms combine (bitwise or). */
 #define XB_FORM_SCREEN 2
 #define XB_FORM_RECORD 4
+/* With XB_FORM_MIXED: XB_FORM_FP16 forms W in FP16 split precision (hi + lo,
+ * 22 significand bits, each stamp's window and the basis table scaled by
+ * powers of two into FP16's range) instead of the BF16 three-part split;
+ * XB_FORM_REFINE adds one step of FP64 iterative refinement to every global
+ * solve (residual b - A x of the system as built, with compensated FP64
+ * products and sums, solved with the same LU factors and added to x). */
+#define XB_FORM_FP16 8
+#define XB_FORM_REFINE 16
 int xb_set_form(xb_ctx* ctx, int form);
 
 /* ---- the learned substamp screen ------------------------------------------

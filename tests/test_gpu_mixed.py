@@ -1,7 +1,7 @@
 """The mixed-precision form (XB_FORM_MIXED): cmv's basis convolution on the
 matrix cores in BF16 split precision (three BF16 parts, six part products,
-FP32 accumulation) and its normal-matrix products on the FP64 matrix cores;
-everything else FP64.
+FP32 accumulation) or FP16 split precision (XB_FORM_FP16) and its
+normal-matrix products on the FP64 matrix cores; everything else FP64.
 
 Bars:
   * the stamp fit data read back per stamp (xb_get_stamp_fit) of the plain
@@ -13,13 +13,17 @@ Bars:
     the dropped part products are below 2^-24, and FP32 accumulation over
     504 taps leaves ~1e-6 in W, which Q and B inherit; measured worst 1.7e-6
     on pair512);
-  * end to end on the 2048x2048 pair: the substamps selected (sss) are the
-    plain form's, bit for bit, and the difference image's residual RMS over
-    good interior pixels is within 1 % of the FP64 oracle's (measured
-    +0.000 %), the pixels within 0.01 sigma_pix = sqrt(|T| + |S|) of the
-    FP64 oracle's (measured: median 2e-6, max 2.2e-4), and the kernel
-    coefficients within 1e-2 of the largest (measured 1e-3: the
-    ill-conditioned system leaves some directions loosely determined).
+  * the same with W in FP16 split precision (XB_FORM_FP16: two FP16 parts,
+    22 bits, range-scaled): within 2e-5;
+  * end to end on the 2048x2048 pair, for the BF16 leg, the FP16 leg and the
+    BF16 leg with FP64 iterative refinement of every global solve: the
+    substamps selected (sss) are the plain form's, bit for bit; the difference
+    image's residual RMS over good interior pixels is within 1 % of the FP64
+    oracle's, its pixels within 1e-3 sigma_pix = sqrt(|T| + |S|) of the FP64
+    oracle's, and the kernel coefficients normwise within 1e-2 (reported per
+    component and normwise: the ill-conditioned system leaves some directions
+    loosely determined, so per-component agreement at 1e-6 is out of reach for
+    any W that differs from the FP64 one, refined or not).
 """
 import numpy as np
 import pytest
@@ -92,10 +96,15 @@ def test_plain_stamp_fit_readback_is_the_reference_fill(ctx, pair512):
             assert np.array_equal(B, Br), (im, k)
 
 
-def test_mixed_stamp_fit_against_fp64(ctx, pair512):
+LEGS = {"bf16": xbach.FORM_MIXED, "fp16": xbach.FORM_MIXED | xbach.FORM_FP16}
+FILL_BAR = {"bf16": 5e-6, "fp16": 2e-5}
+
+
+@pytest.mark.parametrize("leg", ["bf16", "fp16"])
+def test_mixed_stamp_fit_against_fp64(ctx, pair512, leg):
     p = xbach.Params.default()
     t, s = pair512
-    tr = stamps_after_cmv(ctx, p, t, s, xbach.FORM_MIXED)
+    tr = stamps_after_cmv(ctx, p, t, s, LEGS[leg])
     worst = {"W": 0.0, "Q": 0.0, "B": 0.0}
     for im in (0, 1):
         for k in range(0, tr.stampCount[im], 5):
@@ -112,24 +121,47 @@ def test_mixed_stamp_fit_against_fp64(ctx, pair512):
             worst["W"] = max(worst["W"], float(dw.max()))
             worst["Q"] = max(worst["Q"], float(dq.max()))
             worst["B"] = max(worst["B"], float(db.max()))
-    print("mixed fill errors vs FP64:", worst)
-    assert worst["W"] <= 5e-6 and worst["Q"] <= 5e-6 and worst["B"] <= 5e-6, worst
+    ctx.set_form(xbach.FORM_PLAIN)
+    print(f"mixed fill ({leg}) errors vs FP64:", worst)
+    bar = FILL_BAR[leg]
+    assert worst["W"] <= bar and worst["Q"] <= bar and worst["B"] <= bar, worst
 
 
-def test_mixed_pipeline_2048_against_fp64_oracle(ctx):
+def coefficient_errors(x, x64):
+    """Kernel-solution errors against the FP64 oracle's: largest per-component
+    error relative to that component (over components above 1e-6 of the
+    largest), the largest error relative to the largest component, and the
+    normwise error ||x - x64|| / ||x64||."""
+    x, x64 = np.asarray(x, np.float64), np.asarray(x64, np.float64)
+    d = np.abs(x - x64)
+    big = np.abs(x64) > 1e-6 * np.abs(x64).max()
+    return {"per_component": float(np.max(d[big] / np.abs(x64[big]))),
+            "of_largest": float(d.max() / np.abs(x64).max()),
+            "normwise": float(np.linalg.norm(x - x64) / np.linalg.norm(x64))}
+
+
+@pytest.fixture(scope="module")
+def pair2048_fp64():
     p = xbach.Params.default()
     t, s = xbach.synth_pair(xbach.synth_cfg("pair2048"))
+    _, d64, _, tr64, _ = _oracle.run(p, t, s, mode=_oracle.MODE_FP64)
+    return t, s, d64, tr64.as_dict()
+
+
+@pytest.mark.parametrize("leg", ["bf16", "fp16", "bf16+refine"])
+def test_mixed_pipeline_2048_against_fp64_oracle(ctx, pair2048_fp64, leg):
+    p = xbach.Params.default()
+    t, s, d64, tr64 = pair2048_fp64
+    form = LEGS[leg.split("+")[0]] | (xbach.FORM_REFINE if "refine" in leg else 0)
     ctx.set_form(xbach.FORM_PLAIN)
     _, dplain, _ = ctx.run(p, t, s)
     trp = ctx.trace().as_dict()
-    ctx.set_form(xbach.FORM_MIXED)
+    ctx.set_form(form)
     try:
         _, dmix, _ = ctx.run(p, t, s)
         trm = ctx.trace().as_dict()
     finally:
         ctx.set_form(xbach.FORM_PLAIN)
-    _, d64, _, tr64, _ = _oracle.run(p, t, s, mode=_oracle.MODE_FP64)
-    tr64 = tr64.as_dict()
     for k in ("sky0", "sky1", "fwhm0", "fwhm1", "ssX0", "ssX1", "ssY0", "ssY1", "ssVal0",
               "ssVal1", "stampIndex0", "stampIndex1"):
         assert _compare._eq(trm[k], trp[k]), k
@@ -139,13 +171,13 @@ def test_mixed_pipeline_2048_against_fp64_oracle(ctx):
     sp = _compare.sigma_pix(t, s)
     good = (d64 != np.float32(1e-30)) & (dmix != np.float32(1e-30))
     rel = np.abs(dmix[good].astype(np.float64) - d64[good]) / sp[good]
-    x64 = np.asarray(tr64["solution"])
-    dsol = np.max(np.abs(np.asarray(trm["solution"]) - x64)) / np.max(np.abs(x64))
-    print(f"residual RMS: fp64 {r64:.6f} plain {rp:.6f} mixed {rm:.6f} "
+    ce = coefficient_errors(trm["solution"], tr64["solution"])
+    cp = coefficient_errors(trp["solution"], tr64["solution"])
+    print(f"{leg}: residual RMS fp64 {r64:.6f} plain {rp:.6f} mixed {rm:.6f} "
           f"({(rm - r64) / r64:+.3%}); |d diff|/sigma_pix median {np.median(rel):.2e} "
-          f"p99 {np.percentile(rel, 99):.2e} max {rel.max():.2e}; coefficient error "
-          f"{dsol:.3e} of the largest; ksc iterations fp64 {tr64['kscIterations']} "
+          f"p99 {np.percentile(rel, 99):.2e} max {rel.max():.2e}; coefficient errors "
+          f"{ce} (plain form {cp}); ksc iterations fp64 {tr64['kscIterations']} "
           f"plain {trp['kscIterations']} mixed {trm['kscIterations']}")
     assert abs(rm - r64) <= 0.01 * r64
-    assert rel.max() <= 0.01
-    assert dsol <= 1e-2
+    assert rel.max() <= 1e-3
+    assert ce["normwise"] <= 1e-2

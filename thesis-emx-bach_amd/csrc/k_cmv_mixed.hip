@@ -29,6 +29,13 @@
 //       ill-conditioned fit lost a quarter of its stamps on a crowded field.
 //       The W values are FP32, so their products are exact in FP64.
 //
+//   With XB_FORM_FP16 the W product uses FP16 instead: x = h + l, two FP16
+//       parts (22 significand bits), after scaling by a power of two so the
+//       values fit FP16's range: the stamp's window by 2^(14 - e) with 2^e above
+//       its largest magnitude, the basis table likewise on the host; a*b ~
+//       h*h + h*l + l*h (v_mfma_f32_32x32x16_f16, FP32 accumulation), then W
+//       is scaled back (exact: powers of two).
+//
 // k_fill_mixed: one workgroup (8 waves, two per SIMD) per stamp.  Each wave
 // owns 4 of the 31 pixel tiles (two passes of 2) and keeps 2x2 32x32
 // accumulators; after each pass it stages its 64-pixel slab in LDS and writes
@@ -43,6 +50,7 @@
 namespace xb {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int MX_WAVES = 8;            // two waves per SIMD: one unpacks while the other multiplies
@@ -81,6 +89,42 @@ struct Split8 {  // 8 values as the three MFMA operands
   bf16x8 h, m, l;
 };
 
+// x = h + l, two FP16 parts (x already scaled into FP16's range), packed as
+// h | l << 16 in .x
+__device__ __forceinline__ uint2 split2h(float x) {
+  const _Float16 h = (_Float16)x;
+  const _Float16 l = (_Float16)(x - (float)h);
+  return make_uint2((uint32_t)__builtin_bit_cast(uint16_t, h) |
+                        ((uint32_t)__builtin_bit_cast(uint16_t, l) << 16),
+                    0u);
+}
+
+struct Split8h {  // 8 values as the two FP16 MFMA operands
+  f16x8 h, l;
+};
+
+__device__ __forceinline__ Split8h unpack8h(const uint2 w[8]) {
+  uint4 h, l;
+  h.x = __builtin_amdgcn_perm(w[1].x, w[0].x, 0x05040100u);
+  h.y = __builtin_amdgcn_perm(w[3].x, w[2].x, 0x05040100u);
+  h.z = __builtin_amdgcn_perm(w[5].x, w[4].x, 0x05040100u);
+  h.w = __builtin_amdgcn_perm(w[7].x, w[6].x, 0x05040100u);
+  l.x = __builtin_amdgcn_perm(w[1].x, w[0].x, 0x07060302u);
+  l.y = __builtin_amdgcn_perm(w[3].x, w[2].x, 0x07060302u);
+  l.z = __builtin_amdgcn_perm(w[5].x, w[4].x, 0x07060302u);
+  l.w = __builtin_amdgcn_perm(w[7].x, w[6].x, 0x07060302u);
+  return {__builtin_bit_cast(f16x8, h), __builtin_bit_cast(f16x8, l)};
+}
+
+// c += a * b over the three leading FP16 part products, smallest first
+__device__ __forceinline__ f32x16 mfma3h(const Split8h& a, const Split8h& b, f32x16 c) {
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.l, b.h, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.h, b.l, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.h, b.h, c, 0, 0, 0);
+  return c;
+}
+
+
 __device__ __forceinline__ Split8 unpack8(const uint2 w[8]) {
   uint4 h, m, l;
   h.x = __builtin_amdgcn_perm(w[1].x, w[0].x, 0x05040100u);
@@ -110,9 +154,40 @@ __device__ __forceinline__ f32x16 mfma6(const Split8& a, const Split8& b, f32x16
   return c;
 }
 
-// vtab: [MX_KSTEPS][2 n-tiles][h, m, l][64 lanes] x 8 BF16 (uint4)
+// the split type's operands and products
+template <bool F16>
+struct MxOps {
+  using A = Split8;
+  static constexpr int PARTS = 3;
+  static __device__ __forceinline__ A unpack(const uint2 w[8]) { return unpack8(w); }
+  static __device__ __forceinline__ A load(const uint4* vn) {
+    return {__builtin_bit_cast(bf16x8, vn[0]), __builtin_bit_cast(bf16x8, vn[WAVE]),
+            __builtin_bit_cast(bf16x8, vn[2 * WAVE])};
+  }
+  static __device__ __forceinline__ f32x16 mma(const A& a, const A& b, f32x16 c) {
+    return mfma6(a, b, c);
+  }
+};
+template <>
+struct MxOps<true> {
+  using A = Split8h;
+  static constexpr int PARTS = 2;
+  static __device__ __forceinline__ A unpack(const uint2 w[8]) { return unpack8h(w); }
+  static __device__ __forceinline__ A load(const uint4* vn) {
+    return {__builtin_bit_cast(f16x8, vn[0]), __builtin_bit_cast(f16x8, vn[WAVE])};
+  }
+  static __device__ __forceinline__ f32x16 mma(const A& a, const A& b, f32x16 c) {
+    return mfma3h(a, b, c);
+  }
+};
+
+// vtab: [MX_KSTEPS][2 n-tiles][parts][64 lanes] x 8 values (uint4): BF16
+// h, m, l, or FP16 h, l scaled by 2^vscale
+template <bool F16>
 __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
-                                                           const uint4* __restrict__ vtab) {
+                                                           const uint4* __restrict__ vtab,
+                                                           int vscale) {
+  using Ops = MxOps<F16>;
   int s, ssx, ssy;
   if (!fill_stamp(A, blockIdx.x, &s, &ssx, &ssy)) return;
   const int g = A.kept[s];
@@ -129,6 +204,8 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
   float* mono = reinterpret_cast<float*>(mx_lds + MX_WIN_B + MX_STAGE_B);
 
   const int x0 = ssx - (HSS + HKW), y0 = ssy - (HSS + HKW);
+  __shared__ float wmax[MX_WAVES];
+  float vmax = 0.0f;
   for (int i = tid; i < TW * MX_WP; i += MX_THREADS) {
     const int r = i / MX_WP, c = i % MX_WP;
     float v = 0.0f;
@@ -136,7 +213,25 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
       const int xr = min(max(x0 + c, 0), A.w - 1), yr = min(max(y0 + r, 0), A.h - 1);
       v = A.img[(size_t)xr + (size_t)yr * A.w];
     }
-    win[i] = split3(v);
+    if (F16) {
+      if (fabsf(v) <= 3.0e38f) vmax = fmaxf(vmax, fabsf(v));  // finite values
+      win[i] = make_uint2(__float_as_uint(v), 0u);
+    } else {
+      win[i] = split3(v);
+    }
+  }
+  int wexp = 0;  // FP16: the window is scaled by 2^(14 - wexp)
+  if (F16) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
+    if (lane == 0) wmax[wv] = vmax;
+    __syncthreads();
+    float m = 0.0f;
+#pragma unroll
+    for (int w = 0; w < MX_WAVES; w++) m = fmaxf(m, wmax[w]);
+    if (m > 0.0f) (void)frexpf(m, &wexp);  // m < 2^wexp
+    for (int i = tid; i < TW * MX_WP; i += MX_THREADS)
+      win[i] = split2h(ldexpf(__uint_as_float(win[i].x), 14 - wexp));
   }
   if (tid < 2 * SSW) {  // background monomials (FP32, as cmv.cl:156-193)
     const int b = tid / SSW, i = tid % SSW;
@@ -176,16 +271,13 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
     // (the other wave on the SIMD covers the B loads' latency)
     const uint4* vt = vtab + lane;
     for (int ks = 0; ks < MX_KSTEPS; ks++) {
-      const uint4* vn = vt + (size_t)ks * 6 * WAVE;
-      const Split8 b0 = {__builtin_bit_cast(bf16x8, vn[0]), __builtin_bit_cast(bf16x8, vn[WAVE]),
-                         __builtin_bit_cast(bf16x8, vn[2 * WAVE])};
-      const Split8 b1 = {__builtin_bit_cast(bf16x8, vn[3 * WAVE]),
-                         __builtin_bit_cast(bf16x8, vn[4 * WAVE]),
-                         __builtin_bit_cast(bf16x8, vn[5 * WAVE])};
+      const uint4* vn = vt + (size_t)ks * 2 * Ops::PARTS * WAVE;
+      const typename Ops::A b0 = Ops::load(vn);
+      const typename Ops::A b1 = Ops::load(vn + Ops::PARTS * WAVE);
       int rowOff;
       uint32_t m45, m67, m03;
       tapRow(ks, rowOff, m45, m67, m03);
-      Split8 a[MX_TPP];
+      typename Ops::A a[MX_TPP];
 #pragma unroll
       for (int gi = 0; gi < MX_TPP; gi++) {
         uint2 w[8];
@@ -197,12 +289,12 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
           w[t].x &= mk[t];
           w[t].y &= mk[t];
         }
-        a[gi] = unpack8(w);
+        a[gi] = Ops::unpack(w);
       }
 #pragma unroll
       for (int gi = 0; gi < MX_TPP; gi++) {
-        acc[gi][0] = mfma6(a[gi], b0, acc[gi][0]);
-        acc[gi][1] = mfma6(a[gi], b1, acc[gi][1]);
+        acc[gi][0] = Ops::mma(a[gi], b0, acc[gi][0]);
+        acc[gi][1] = Ops::mma(a[gi], b1, acc[gi][1]);
       }
     }
     // ---- stage the slab (rows 0..48)
@@ -218,12 +310,14 @@ __global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
       }
     const int pBase = mt0 * 32;
     __syncthreads();
-    // ---- W rows 0..48 of the slab in FP64 (one pixel per lane and row)
+    // ---- W rows 0..48 of the slab in FP64 (one pixel per lane and row);
+    // FP16: the two scales undone (powers of two, exact in FP64)
     const bool inW = pBase + lane < NPIX;
+    const int unscale = F16 ? wexp - 14 - vscale : 0;
 #pragma unroll 7
     for (int n = 0; n < NPSF; n++) {
       const float v = st[n * MX_SP + lane];
-      if (inW) Ws[(size_t)n * NPIX + pBase + lane] = (double)v;
+      if (inW) Ws[(size_t)n * NPIX + pBase + lane] = ldexp((double)v, unscale);
     }
     __syncthreads();  // the stage is rewritten by the next pass
   }
@@ -359,10 +453,17 @@ __global__ __launch_bounds__(G64_THREADS) void k_fill_gram64(FillArgs A) {
 
 hipError_t launch_fill_mixed(Ctx& c, const FillArgs& A, int nMax) {
   static const hipError_t attrRc = hipFuncSetAttribute(
-      (const void*)k_fill_mixed, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MX_LDS);
+      (const void*)k_fill_mixed<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MX_LDS);
+  static const hipError_t attrRc16 = hipFuncSetAttribute(
+      (const void*)k_fill_mixed<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MX_LDS);
   if (attrRc != hipSuccess) return attrRc;
-  hipLaunchKernelGGL(k_fill_mixed, dim3(nMax), dim3(MX_THREADS), MX_LDS, c.cur, A,
-                     reinterpret_cast<const uint4*>(c.mixTab));
+  if (attrRc16 != hipSuccess) return attrRc16;
+  if (c.form & XB_FORM_FP16)
+    hipLaunchKernelGGL(k_fill_mixed<true>, dim3(nMax), dim3(MX_THREADS), MX_LDS, c.cur, A,
+                       reinterpret_cast<const uint4*>(c.mixTab16), c.mixScale16);
+  else
+    hipLaunchKernelGGL(k_fill_mixed<false>, dim3(nMax), dim3(MX_THREADS), MX_LDS, c.cur, A,
+                       reinterpret_cast<const uint4*>(c.mixTab), 0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fill_gram64, dim3(nMax), dim3(G64_THREADS), 0, c.cur, A);

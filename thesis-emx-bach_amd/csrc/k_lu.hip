@@ -296,11 +296,15 @@ __device__ __forceinline__ int* lu_zero_flags(LuState* st) {
   return reinterpret_cast<int*>(reinterpret_cast<char*>(st) + 64);
 }
 
-// final position of each physical row after a panel's interchanges, two
-// buffers by panel parity (a step reads its panel's while its workgroup 0
-// writes the next panel's)
-__device__ __forceinline__ int* lu_dst(LuState* st, int parity) {
-  return lu_zero_flags(st) + MATN * (1 + parity);
+// final position of each physical row after panel p's interchanges (a step
+// reads its panel's while its workgroup 0 writes the next panel's); all panels'
+// maps are kept for the iterative refinement's forward substitution
+static_assert(NPANEL == LU_PANELS, "one row map per panel");
+__device__ __forceinline__ int* lu_dst(LuState* st, int p) {
+  return lu_zero_flags(st) + MATN * (1 + p);
+}
+__device__ __forceinline__ const int* lu_dst(const LuState* st, int p) {
+  return reinterpret_cast<const int*>(reinterpret_cast<const char*>(st) + 64) + MATN * (1 + p);
 }
 
 // vv (ludcmp's implicit row scaling) and the zero-row flags: one wave per row,
@@ -414,7 +418,7 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
     // panel's own columns were stored at their final positions then.
     constexpr int IT = (MATSIZE * LB + LU_THREADS - 1) / LU_THREADS;
     constexpr int IV = (MATSIZE + LU_THREADS - 1) / LU_THREADS;
-    const int* dmap = lu_dst(st, p & 1);
+    const int* dmap = lu_dst(st, p);
     double va[IT], vt[IT], vr[IV];
     int dt[IT], dv[IV];
 #pragma unroll
@@ -522,7 +526,7 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, i
     }
     if (tid < WAVE) {
       // rows l0..n of the next panel: as many register slots as they need
-      int* dnext = lu_dst(st, (p + 1) & 1);
+      int* dnext = lu_dst(st, p + 1);
       const int rows = n - l0 + 1;
       if (rows > 4 * WAVE)
         wave_panel_factor<5>(&T[LB], l0, nc, vS + LB, s_int, M, vv, piv, dnext, st);
@@ -637,10 +641,13 @@ __device__ __forceinline__ void lds_barrier() {
                        : "+v"(s)                                                      \
                        : "v"(p), "v"(m1));
 
+// refine: the iterative-refinement solve (k_lu_refine_*): lubksb of b0 with
+// the factors already in Mw (forward and back substitution, no replay), the
+// correction left in b; nothing to correct after a skipped or zero-row solve.
 __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(double* Mw, const double* M0,
                                                            int* piv, double* b,
                                                            const double* b0, const LuState* st,
-                                                           double* out) {
+                                                           double* out, int refine) {
   const double* M = Mw;
   const int n = MATSIZE, N = MATN;
   const int tid = threadIdx.x, lane = lane_id();
@@ -648,17 +655,21 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(double* Mw, const dou
   __shared__ double Pb[2][BS_COLS];          // products of a row, by row parity
   __shared__ double ring[BS_RING][BS_COLS];  // U rows, row r in slot r % 8
   __shared__ double scb[2][4];               // y_t, U_tt, U_t,t+1, U_t,t+2 by row parity
-  if (st->skip) {
-    for (int i = tid; i < NSOL; i += BS_THREADS)
+  if (st->skip || (refine && st->zero)) {
+    for (int i = tid; i < NSOL; i += BS_THREADS) {
       if (out) out[i] = 0.0;
+      if (refine) b[i] = 0.0;
+    }
     return;
   }
-  if (st->fallback && !st->zero) {
+  if (!refine && st->fallback && !st->zero) {
     lu_fallback_replay(Mw, M0, piv);
     __syncthreads();
   }
-  const bool full = st->zero || st->fallback;  // forward pass not fused
-  if (st->zero) M = M0;                         // ludcmp returned before factorising
+  // forward pass not fused: after a zero row or a replay (whose factors are in
+  // the reference's layout, so the refinement uses its lubksb too)
+  const bool full = refine ? st->fallback != 0 : (st->zero || st->fallback);
+  if (!refine && st->zero) M = M0;  // ludcmp returned before factorising
   for (int i = tid; i < NSOL; i += BS_THREADS) xs[i] = full ? b0[i] : b[i];
   __syncthreads();
   if (full && tid == 0) {
@@ -788,6 +799,96 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(double* Mw, const dou
 }
 #undef XB_CHAIN
 
+// ---------------------------------------------------------------------------
+// FP64 iterative refinement (XB_FORM_REFINE): r = b - A x with the system as
+// built (M0, rhs0), each product's rounding error kept (fma) and the sums in
+// double-double (TwoSum), so r is the residual rounded once; then A d = r with
+// the same LU factors (k_lu_backsub in refine mode) and x += d.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double h2, double l2) {
+  double s, e;
+  two_sum(hi, h2, s, e);
+  e = e + (lo + l2);
+  hi = s + e;
+  lo = e - (hi - s);
+}
+
+// one wave per row i = 1..292
+__global__ __launch_bounds__(LU_THREADS) void k_lu_refine_residual(const double* M0,
+                                                                   const double* rhs0,
+                                                                   const double* x,
+                                                                   double* r,
+                                                                   const LuState* st) {
+  const int n = MATSIZE, N = MATN;
+  const int lane = lane_id();
+  const int i = 1 + (int)blockIdx.x * (LU_THREADS / WAVE) + (int)threadIdx.x / WAVE;
+  if (i > n) return;
+  if (st->skip || st->zero) {
+    if (lane == 0) r[i] = 0.0;
+    return;
+  }
+  double hi = lane == 0 ? rhs0[i] : 0.0, lo = 0.0;
+#pragma unroll
+  for (int sl = 0; sl < NSLOT; sl++) {
+    const int j = 1 + lane + WAVE * sl;
+    if (j <= n) {
+      const double a = M0[(size_t)i * N + j], xj = x[j];
+      const double p = a * xj;
+      const double e = fma(a, xj, -p);  // a*xj = p + e exactly
+      dd_add(hi, lo, -p, -e);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double h2 = __shfl_xor(hi, o), l2 = __shfl_xor(lo, o);
+    dd_add(hi, lo, h2, l2);
+  }
+  if (lane == 0) r[i] = hi + lo;
+}
+
+// The refinement's forward substitution with the blocked factors: panel by
+// panel, the residual's rows go to their positions after that panel's
+// interchanges (its row map), then the panel's L columns (stored in that row
+// order) update the rows below, as the factorisation's right-hand-side block
+// does.  One workgroup; nothing to do after a replay (lubksb then) or when the
+// solve was skipped or hit a zero row.
+__global__ __launch_bounds__(LU_THREADS) void k_lu_refine_forward(const double* M, double* y,
+                                                                  const LuState* st) {
+  if (st->skip || st->zero || st->fallback) return;
+  const int n = MATSIZE, N = MATN;
+  const int tid = threadIdx.x;
+  __shared__ double v[MATN], t[MATN];
+  for (int i = 1 + tid; i <= n; i += LU_THREADS) v[i] = y[i];
+  __syncthreads();
+  for (int p = 0; p < NPANEL; p++) {
+    const int c0 = 1 + p * LB, cB = min(LB, n - c0 + 1), R = n - c0 + 1;
+    const int* dmap = lu_dst(st, p);
+    for (int r = tid; r < R; r += LU_THREADS) t[min(max(dmap[c0 + r] - c0, 0), R - 1)] = v[c0 + r];
+    __syncthreads();
+    for (int r = tid; r < R; r += LU_THREADS) v[c0 + r] = t[r];
+    __syncthreads();
+    for (int q = 0; q < cB; q++) {
+      const double yk = v[c0 + q];
+      for (int r = q + 1 + tid; r < R; r += LU_THREADS)
+        v[c0 + r] = v[c0 + r] - M[(size_t)(c0 + r) * N + c0 + q] * yk;
+      __syncthreads();
+    }
+  }
+  for (int i = 1 + tid; i <= n; i += LU_THREADS) y[i] = v[i];
+}
+
+__global__ void k_lu_refine_add(double* x, const double* d, const LuState* st) {
+  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i > MATSIZE || st->skip || st->zero) return;
+  x[i] = x[i] + d[i];
+}
+
 // The solve's kernels in order on stream s.
 static hipError_t enqueue_solve(hipStream_t s, const LuSet& L, double* d_out,
                                 const int* d_skipIfZero) {
@@ -809,7 +910,22 @@ static hipError_t enqueue_solve(hipStream_t s, const LuSet& L, double* d_out,
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(BS_THREADS), 0, s, L.M, L.M0, L.luIndex, L.rhs,
-                     L.rhs0, st, d_out);
+                     L.rhs0, st, d_out, 0);
+  return hipGetLastError();
+}
+
+// One step of iterative refinement of the solution x (d_out, else the rhs
+// buffer the substitution left it in) on stream s.
+static hipError_t enqueue_refine(hipStream_t s, const LuSet& L, double* d_out) {
+  LuState* st = reinterpret_cast<LuState*>(L.luState);
+  double* x = d_out ? d_out : L.rhs;
+  hipLaunchKernelGGL(k_lu_refine_residual,
+                     dim3((MATSIZE + LU_THREADS / WAVE - 1) / (LU_THREADS / WAVE)),
+                     dim3(LU_THREADS), 0, s, L.M0, L.rhs0, x, L.rr, st);
+  hipLaunchKernelGGL(k_lu_refine_forward, dim3(1), dim3(LU_THREADS), 0, s, L.M, L.rr, st);
+  hipLaunchKernelGGL(k_lu_backsub, dim3(1), dim3(BS_THREADS), 0, s, L.M, L.M0, L.luIndex, L.rr,
+                     L.rr, st, nullptr, 1);
+  hipLaunchKernelGGL(k_lu_refine_add, dim3((MATSIZE + 255) / 256), dim3(256), 0, s, x, L.rr, st);
   return hipGetLastError();
 }
 
@@ -867,6 +983,7 @@ hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int*
     }
     if (e == hipSuccess && g) e = hipGraphLaunch(g, c.cur);
   }
+  if (e == hipSuccess && (c.form & XB_FORM_REFINE)) e = enqueue_refine(c.cur, L, d_out);
   prof_end(c, XB_PROF_SOLVE);
   return e;
 }

@@ -142,6 +142,40 @@ void mixedTable(const Basis& B, std::vector<uint16_t>& out) {
         }
 }
 
+// The same table for the FP16 split (XB_FORM_FP16): [step][basis tile][h, l]
+// [lane][8], value * 2^scale rounded to FP32, then x = h + l in FP16 (round to
+// nearest even); the kernel scales W back.  scale puts the largest magnitude
+// just below 2^14 (the odd terms, x^a y^b times a Gaussian, are not normalised
+// and exceed 1).  Returns scale.
+int mixedTable16(const Basis& B, std::vector<uint16_t>& out) {
+  constexpr int KSTEPS = 32, TAPW = 24, PARTS = 2;
+  out.assign((size_t)KSTEPS * 2 * PARTS * 64 * 8, 0);
+  double big = 0.0;
+  for (int n = 0; n < NPSF; n++)
+    for (int u = 0; u < KW; u++)
+      for (int v = 0; v < KW; v++)
+        big = std::max(big, std::fabs((double)(float)(B.fy[n * KW + u] * B.fx[n * KW + v])));
+  int e = 0;
+  if (big > 0.0) (void)std::frexp(big, &e);  // big < 2^e
+  const int scale = 14 - e;
+  for (int ks = 0; ks < KSTEPS; ks++)
+    for (int nt = 0; nt < 2; nt++)
+      for (int l = 0; l < 64; l++)
+        for (int t = 0; t < 8; t++) {
+          const int k = 16 * ks + 8 * (l / 32) + t, u = k / TAPW, v = k % TAPW;
+          const int n = 32 * nt + l % 32;
+          double x = 0.0;
+          if (u < KW && v < KW && n < NPSF) x = B.fy[n * KW + 20 - u] * B.fx[n * KW + 20 - v];
+          const float xf = std::ldexp((float)x, scale);
+          const _Float16 h = (_Float16)xf;
+          const _Float16 lo = (_Float16)(xf - (float)h);
+          const size_t at = ((((size_t)ks * 2 + nt) * PARTS) * 64 + l) * 8 + t;
+          std::memcpy(&out[at], &h, 2);
+          std::memcpy(&out[at + 64 * 8], &lo, 2);
+        }
+  return scale;
+}
+
 void ran1Table(int n, std::vector<double>& out) {
   const long M1 = 259200, IA1 = 7141, IC1 = 54773;
   const long M2 = 134456, IA2 = 8121, IC2 = 28411;
@@ -298,6 +332,7 @@ int ensureSolver(Ctx& c) {
     XB_HIP(locked_malloc((void**)&L.luIndex, sizeof(int) * MATN));
     XB_HIP(locked_malloc((void**)&L.luVV, sizeof(double) * MATN));
     XB_HIP(locked_malloc((void**)&L.luState, LU_STATE_BYTES));
+    XB_HIP(locked_malloc((void**)&L.rr, sizeof(double) * NSOL));
   }
   XB_HIP(locked_malloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
   XB_HIP(locked_malloc((void**)&c.ksSol, sizeof(double) * 2 * NSOL));
@@ -369,8 +404,9 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
   const bool newBasis =
       !c.basisValid || std::memcmp(&c.basisParams, p, sizeof(xb_params)) != 0;
   const bool newTab = (c.form & XB_FORM_MIXED) &&
-                      (!c.mixTab || std::memcmp(c.mixBg, c.p.bg, sizeof(c.mixBg)) != 0);
-  std::vector<uint16_t> tab;
+                      (!c.mixTab || !c.mixTab16 ||
+                       std::memcmp(c.mixBg, c.p.bg, sizeof(c.mixBg)) != 0);
+  std::vector<uint16_t> tab, tab16;
   if (newBasis || newTab) {
     Basis hb;
     std::memset(&hb, 0, sizeof(hb));
@@ -379,8 +415,12 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
       XB_HIP(hipMemcpyAsync(c.basis, &hb, sizeof(Basis), hipMemcpyHostToDevice, c.stream));
     if (newTab) {
       mixedTable(hb, tab);
+      c.mixScale16 = mixedTable16(hb, tab16);
       if (!c.mixTab) XB_HIP(locked_malloc(&c.mixTab, tab.size() * sizeof(uint16_t)));
+      if (!c.mixTab16) XB_HIP(locked_malloc(&c.mixTab16, tab16.size() * sizeof(uint16_t)));
       XB_HIP(hipMemcpyAsync(c.mixTab, tab.data(), tab.size() * sizeof(uint16_t),
+                            hipMemcpyHostToDevice, c.stream));
+      XB_HIP(hipMemcpyAsync(c.mixTab16, tab16.data(), tab16.size() * sizeof(uint16_t),
                             hipMemcpyHostToDevice, c.stream));
       std::memcpy(c.mixBg, c.p.bg, sizeof(c.mixBg));
     }
@@ -878,7 +918,7 @@ void freeAll(Ctx& c) {
   for (int i = 0; i < 4; i++) {
     LuSet& L = i < 2 ? c.lu[i] : c.ksLu[i - 2];
     dfree(&L.M); dfree(&L.rhs); dfree(&L.M0); dfree(&L.rhs0);
-    dfree(&L.luIndex); dfree(&L.luVV); dfree(&L.luState);
+    dfree(&L.luIndex); dfree(&L.luVV); dfree(&L.luState); dfree(&L.rr);
   }
   dfree(&c.sol); dfree(&c.cdSol); dfree(&c.ksSol);
   dfree(&c.tileK64); dfree(&c.tileAks);
@@ -890,6 +930,8 @@ void freeAll(Ctx& c) {
   c.scr.haveModel = false;
   if (c.mixTab) (void)locked_free(c.mixTab);
   c.mixTab = nullptr;
+  if (c.mixTab16) (void)locked_free(c.mixTab16);
+  c.mixTab16 = nullptr;
 }
 
 int setDev(Ctx& c) {
@@ -1118,7 +1160,9 @@ int xb_init_device(xb_ctx* x, const xb_params* p, int w, int h, const void* d_tm
 }
 
 int xb_set_form(xb_ctx* x, int form) {
-  if (!x || (form & ~(XB_FORM_MIXED | XB_FORM_SCREEN | XB_FORM_RECORD))) return XB_ERR_ARG;
+  if (!x || (form & ~(XB_FORM_MIXED | XB_FORM_SCREEN | XB_FORM_RECORD | XB_FORM_FP16 |
+                      XB_FORM_REFINE)))
+    return XB_ERR_ARG;
   x->c.form = form;
   return XB_OK;
 }

@@ -59,7 +59,10 @@ constexpr int MATSIZE = NCOMP + NBG + 1;  // 292 unknowns (1-based)
 constexpr int MATN = MATSIZE + 1;         // 293 rows/cols stored
 constexpr int NSOL = NPSF * NC2 + NBG + 1;  // 298 solution slots
 // LU bookkeeping block: a 64-byte state record, then one zero-row flag per row
-constexpr size_t LU_STATE_BYTES = 64 + sizeof(int) * MATN * 3;  // flags, zero rows, 2 row maps
+// a 64-byte state record, the zero-row flags, and one row map per 16-column
+// panel (19 for 292 unknowns)
+constexpr int LU_PANELS = (MATSIZE + 15) / 16;
+constexpr size_t LU_STATE_BYTES = 64 + sizeof(int) * MATN * (1 + LU_PANELS);
 constexpr int QN = NPSF + 2;              // 51: per-stamp test system (1-based)
 constexpr int MAXSS = 6;                  // 2 * maxKSStamps
 constexpr int KEEPSS = 3;                 // maxKSStamps
@@ -146,6 +149,7 @@ struct LuSet {
   int* luIndex = nullptr;       // [MATN] pivot rows
   void* luState = nullptr;      // LU bookkeeping (zero row, last pivot, ii, zero flags)
   double* luVV = nullptr;       // [MATN]
+  double* rr = nullptr;         // [NSOL] iterative refinement: residual, then correction
 };
 
 // The learned substamp screen's per-pass buffers and model (the first
@@ -235,6 +239,8 @@ struct Ctx {
   int nConfirmed = 0;           // ... that the chi2 re-check also rejected
   int scrSamples = 0;           // samples the last ksc recorded
   void* mixTab = nullptr;       // mixed form: split BF16 2-D basis, MFMA fragment order
+  void* mixTab16 = nullptr;     // ... split FP16, scaled by 2^mixScale16 (XB_FORM_FP16)
+  int mixScale16 = 0;
   float mixBg[3] = {0, 0, 0};   // basis widths the table was built for
   xb_params basisParams{};      // parameters the device basis was built from
   bool basisValid = false;

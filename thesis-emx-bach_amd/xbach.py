@@ -392,6 +392,7 @@ def fits_write_native(path, data: np.ndarray) -> None:
 CTX_ONE_STREAM = 1
 CTX_EAGER = 2  # global solves launched kernel by kernel instead of replayed hipGraphs
 FORM_PLAIN, FORM_MIXED, FORM_SCREEN, FORM_RECORD = 0, 1, 2, 4
+FORM_FP16, FORM_REFINE = 8, 16  # with FORM_MIXED: FP16 split W; FP64 iterative refinement
 # the screen's network (include/xbach.h XB_MLP_*)
 MLP_NFEAT, MLP_NPARAM, MLP_NGRAD = 32, 2242, 2177
 MLP_NOALLOC_MAX = 16384  # XB_MLP_NOALLOC_MAX
