@@ -379,6 +379,20 @@ def main() -> int:
             "pair_ms_mean": float(np.mean([r.ms for r in res])),
             "capture_failures": int(sum(r.captureFailures for r in res)),
             "speedup_vs_plain": b_value / plain_value}
+        # the same batches in lockstep groups: every global solve of a group
+        # (all its pairs' direction trials, then each clip-refit pass) is one
+        # batched solve
+        l_ms, lres, dls, ltab = run_batched(ctx, params, pairs, bufs, args.workers,
+                                            args.steps, args.warmup, keep=keep,
+                                            tune=not args.no_tune,
+                                            flags=ctx_flags | xbach.BATCH_LOCKSTEP)
+        l_value = args.batch * 1e3 / l_ms
+        forms["batched_lockstep"] = {
+            "value": l_value, "ms_per_batch": l_ms, "pairs_per_batch": args.batch,
+            "pairs_per_group_per_bucket": ltab, "steps": args.steps, "warmup": args.warmup,
+            "all_pairs_ok": all(r.status == 0 for r in lres),
+            "bit_identical_to_batched": all(same_bits(a, b) for a, b in zip(dls, dplain)),
+            "speedup_vs_batched": l_value / b_value}
         # mixed precision + batched + learned screen
         model = np.load(screen.MODEL_PATH) if screen.MODEL_PATH.exists() else None
         fform = xbach.FORM_MIXED | (xbach.FORM_SCREEN if model is not None else 0)

@@ -389,6 +389,16 @@ typedef struct xb_pair_result {
  * graph, since a stream capture on one thread can be invalidated by another
  * thread's work on the device. */
 #define XB_BATCH_MAX_WORKERS 32
+/* xb_batch_create flag: lockstep mode.  One host thread runs groups of pairs
+ * together (each on its own worker context and stream), stage by stage, and
+ * every global solve of a group — the direction trials of all its pairs, then
+ * each clip-refit pass of the pairs still in their loop — is one batched
+ * solve: the 23 launches of a single solve, each covering all the group's
+ * systems (up to 32, so up to 16 pairs per group).  The group size is the
+ * pairs-in-flight count of the pair's size bucket (set_inflight / autotune).
+ * Outputs are bit-identical to single runs.  Runs with XB_FORM_SCREEN or
+ * XB_FORM_RECORD use the worker threads instead. */
+#define XB_BATCH_LOCKSTEP 4
 int xb_batch_create(int device, int workers, int flags, xb_batch** out);
 int xb_batch_destroy(xb_batch* b);
 const char* xb_batch_last_error(xb_batch* b);

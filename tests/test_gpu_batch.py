@@ -156,3 +156,74 @@ def test_batch_autotune(pairs, singles):
             for q in bufs:
                 for d in q:
                     c.buf_free(d)
+
+
+@pytest.mark.parametrize("inflight", [3, 16])
+def test_lockstep_batch_equals_single_runs(pairs, singles, inflight):
+    """XB_BATCH_LOCKSTEP: groups of pairs run stage by stage on one host thread,
+    every global solve of a group as one batched solve; each pair bit-identical
+    to its lone run, the failing pair reports its own status."""
+    p = xbach.Params.default()
+    with xbach.Batch(0, workers=16, flags=xbach.BATCH_LOCKSTEP) as b:
+        b.set_inflight(inflight)
+        outs, res = b.run(p, pairs, check=False)
+        msg = b._L.xb_batch_last_error(b._h)
+    for i, (conv, diff, ks, ct, it) in enumerate(singles):
+        r = res[i]
+        assert r.status == 0, (i, r.status, msg)
+        assert _same(outs[i][0], conv) and _same(outs[i][1], diff), i
+        assert (r.kernSum, r.convTemplate, r.kscIterations) == (ks, ct, it), i
+    assert res[-1].status == NO_SUBSTAMPS
+    assert all(0 <= r.worker < inflight for r in res)
+
+
+def test_lockstep_refined_equals_single_refined(pairs):
+    # the batched solves followed by each system's iterative refinement give
+    # the single-context refined form's results bit for bit
+    p = xbach.Params.default()
+    use = pairs[:4]
+    form = xbach.FORM_MIXED | xbach.FORM_REFINE
+    ref = []
+    with xbach.Context(0) as c:
+        c.set_form(form)
+        for t, s in use:
+            conv, diff, _ = c.run(p, t, s)
+            ref.append((conv, diff, c.trace().kernSum))
+    with xbach.Batch(0, workers=4, flags=xbach.BATCH_LOCKSTEP) as b:
+        b.set_form(form)
+        outs, res = b.run(p, use)
+    for i, (conv, diff, ks) in enumerate(ref):
+        assert res[i].status == 0
+        assert _same(outs[i][0], conv) and _same(outs[i][1], diff), i
+        assert res[i].kernSum == ks, i
+
+
+def test_lockstep_autotune(pairs, singles):
+    p = xbach.Params.default()
+    use = pairs[:-1]
+    w = [t.shape[1] for t, _ in use]
+    h = [t.shape[0] for t, _ in use]
+    with xbach.Context(0) as c, xbach.Batch(0, workers=8, flags=xbach.BATCH_LOCKSTEP) as b:
+        bufs = []
+        try:
+            for t, s in use:
+                dt, ds, dc, dd = (c.buf_alloc(t.nbytes) for _ in range(4))
+                c.buf_upload(dt, t)
+                c.buf_upload(ds, s)
+                bufs.append((dt, ds, dc, dd))
+            args = ([q[0] for q in bufs], [q[1] for q in bufs], [q[2] for q in bufs],
+                    [q[3] for q in bufs])
+            table = b.autotune(p, w, h, *args, reps=1)
+            assert all(0 <= t <= 8 for t in table) and max(table) > 0, table
+            res = b.run_device(p, w, h, *args)
+            for i, (t, _) in enumerate(use):
+                conv = np.empty_like(t)
+                diff = np.empty_like(t)
+                c.buf_download(conv, bufs[i][2])
+                c.buf_download(diff, bufs[i][3])
+                assert res[i].status == 0
+                assert _same(conv, singles[i][0]) and _same(diff, singles[i][1]), i
+        finally:
+            for q in bufs:
+                for d in q:
+                    c.buf_free(d)

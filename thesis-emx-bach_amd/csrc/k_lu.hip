@@ -310,8 +310,8 @@ __device__ __forceinline__ const int* lu_dst(const LuState* st, int p) {
 // vv (ludcmp's implicit row scaling) and the zero-row flags: one wave per row,
 // lanes across the columns.  The maximum of |a| with NaN never taken is the
 // same in any order.
-__global__ __launch_bounds__(LU_THREADS) void k_lu_scale(const double* M, double* vv,
-                                                         LuState* st, const int* skipIfZero) {
+__device__ __forceinline__ void lu_scale_dev(const double* M, double* vv, LuState* st,
+                                             const int* skipIfZero) {
   if (skipIfZero && *skipIfZero == 0) return;
   const int n = MATSIZE, N = MATN;
   const int lane = lane_id();
@@ -359,8 +359,8 @@ __device__ __forceinline__ void stage_slab(double (*D)[LDP], const double* M, in
   }
 }
 
-__global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, int* piv,
-                                                         LuState* st, const int* skipIfZero) {
+__device__ __forceinline__ void lu_begin_dev(double* M, double* vv, int* piv, LuState* st,
+                                             const int* skipIfZero) {
   const int n = MATSIZE, N = MATN;
   const int tid = threadIdx.x;
   __shared__ int s_zero;
@@ -389,9 +389,8 @@ __global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, 
   if (tid < WAVE) wave_panel_factor<WSLOT>(S, 1, LB, vv + 1, 0, M, vv, piv, lu_dst(st, 0), st);
 }
 
-__global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, int* piv,
-                                                        double* b, LuState* st, int p,
-                                                        int nColBlocks) {
+__device__ __forceinline__ void lu_step_dev(double* M, double* vv, int* piv, double* b,
+                                            LuState* st, int p, int nColBlocks) {
   if (st->zero || st->skip || st->fallback) return;
   const int n = MATSIZE, N = MATN;
   const int tid = threadIdx.x;
@@ -644,10 +643,9 @@ __device__ __forceinline__ void lds_barrier() {
 // refine: the iterative-refinement solve (k_lu_refine_*): lubksb of b0 with
 // the factors already in Mw (forward and back substitution, no replay), the
 // correction left in b; nothing to correct after a skipped or zero-row solve.
-__global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(double* Mw, const double* M0,
-                                                           int* piv, double* b,
-                                                           const double* b0, const LuState* st,
-                                                           double* out, int refine) {
+__device__ __forceinline__ void lu_backsub_dev(double* Mw, const double* M0, int* piv, double* b,
+                                               const double* b0, const LuState* st, double* out,
+                                               int refine) {
   const double* M = Mw;
   const int n = MATSIZE, N = MATN;
   const int tid = threadIdx.x, lane = lane_id();
@@ -800,6 +798,50 @@ __global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(double* Mw, const dou
 #undef XB_CHAIN
 
 // ---------------------------------------------------------------------------
+// Entry points: one system (the arguments), or a batch of systems of the same
+// size, one per blockIdx.y, their buffers passed by value in the kernel
+// arguments (read with scalar loads; each workgroup works on one system, with
+// the same per-element arithmetic as alone, so every system's result is
+// bit-identical to its single solve).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(LU_THREADS) void k_lu_scale(const double* M, double* vv,
+                                                         LuState* st, const int* skipIfZero) {
+  lu_scale_dev(M, vv, st, skipIfZero);
+}
+__global__ __launch_bounds__(LU_THREADS) void k_lu_begin(double* M, double* vv, int* piv,
+                                                         LuState* st, const int* skipIfZero) {
+  lu_begin_dev(M, vv, piv, st, skipIfZero);
+}
+__global__ __launch_bounds__(LU_THREADS) void k_lu_step(double* M, double* vv, int* piv,
+                                                        double* b, LuState* st, int p,
+                                                        int nColBlocks) {
+  lu_step_dev(M, vv, piv, b, st, p, nColBlocks);
+}
+__global__ __launch_bounds__(BS_THREADS) void k_lu_backsub(double* Mw, const double* M0,
+                                                           int* piv, double* b,
+                                                           const double* b0, const LuState* st,
+                                                           double* out, int refine) {
+  lu_backsub_dev(Mw, M0, piv, b, b0, st, out, refine);
+}
+
+__global__ __launch_bounds__(LU_THREADS) void k_lu_scale_b(LuBatch B) {
+  const LuSys& S = B.s[blockIdx.y];
+  lu_scale_dev(S.M, S.vv, S.st, S.skip);
+}
+__global__ __launch_bounds__(LU_THREADS) void k_lu_begin_b(LuBatch B) {
+  const LuSys& S = B.s[blockIdx.y];
+  lu_begin_dev(S.M, S.vv, S.piv, S.st, S.skip);
+}
+__global__ __launch_bounds__(LU_THREADS) void k_lu_step_b(LuBatch B, int p, int nColBlocks) {
+  const LuSys& S = B.s[blockIdx.y];
+  lu_step_dev(S.M, S.vv, S.piv, S.rhs, S.st, p, nColBlocks);
+}
+__global__ __launch_bounds__(BS_THREADS) void k_lu_backsub_b(LuBatch B) {
+  const LuSys& S = B.s[blockIdx.y];
+  lu_backsub_dev(S.M, S.M0, S.piv, S.rhs, S.rhs0, S.st, S.out, 0);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 iterative refinement (XB_FORM_REFINE): r = b - A x with the system as
 // built (M0, rhs0), each product's rounding error kept (fma) and the sums in
 // double-double (TwoSum), so r is the residual rounded once; then A d = r with
@@ -927,6 +969,50 @@ static hipError_t enqueue_refine(hipStream_t s, const LuSet& L, double* d_out) {
                      L.rr, st, nullptr, 1);
   hipLaunchKernelGGL(k_lu_refine_add, dim3((MATSIZE + 255) / 256), dim3(256), 0, s, x, L.rr, st);
   return hipGetLastError();
+}
+
+// The global solves of several systems (of different pairs) as one batch on
+// stream s: the same 23 launches as one solve, each covering every system.
+hipError_t launch_global_solve_batch(hipStream_t s, const LuBatch& B) {
+  if (B.n <= 0) return hipSuccess;
+  if (B.n > LU_BATCH_MAX) return hipErrorInvalidValue;
+  const int nb = B.n;
+  hipLaunchKernelGGL(k_lu_scale_b,
+                     dim3((MATSIZE + LU_THREADS / WAVE - 1) / (LU_THREADS / WAVE), nb),
+                     dim3(LU_THREADS), 0, s, B);
+  hipLaunchKernelGGL(k_lu_begin_b, dim3(1, nb), dim3(LU_THREADS), 0, s, B);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  for (int p = 0; p < NPANEL; p++) {
+    const int c0 = 1 + p * LB;
+    const int cB = std::min(LB, MATSIZE - c0 + 1);
+    const int trailing = MATSIZE - (c0 + cB) + 1;
+    const int nColBlocks = (trailing + LB - 1) / LB;
+    hipLaunchKernelGGL(k_lu_step_b, dim3(nColBlocks + 1, nb), dim3(LU_THREADS), 0, s, B, p,
+                       nColBlocks);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_lu_backsub_b, dim3(1, nb), dim3(BS_THREADS), 0, s, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_refine(hipStream_t s, const LuSet& L, double* d_out) {
+  return enqueue_refine(s, L, d_out);
+}
+
+LuSys lu_sys(const LuSet& L, double* d_out, const int* d_skipIfZero) {
+  LuSys S;
+  S.M = L.M;
+  S.vv = L.luVV;
+  S.piv = L.luIndex;
+  S.rhs = L.rhs;
+  S.st = reinterpret_cast<LuState*>(L.luState);
+  S.M0 = L.M0;
+  S.rhs0 = L.rhs0;
+  S.out = d_out;
+  S.skip = d_skipIfZero;
+  return S;
 }
 
 // The global solve on the context's current stream: replayed from a hipGraph

@@ -441,7 +441,8 @@ int stageMask(Ctx& c) {
 }
 
 // sss (src/bach.cpp:44-170)
-int stageSss(Ctx& c) {
+// sss geometry and buffers (src/bach.cpp:44-66, bachUtil.cpp:144-156)
+int sssSetup(Ctx& c) {
   const xb_params& a = c.p;
   int fsw = std::min(c.w / a.stampsx, c.h / a.stampsy);
   int nx = a.stampsx, ny = a.stampsy;
@@ -479,35 +480,25 @@ int stageSss(Ctx& c) {
       return XB_ERR_STAMP_PIXELS;
     }
   }
-  int rc = ensureStamps(c, c.nGrid, fsw);
-  if (rc) return rc;
-  auto identify = [&]() -> int {
-    // calcStats T, calcStats S, then both searches (src/sssUtil.cpp:11-14);
-    // the searches read only their own image's mask bits, so they overlap
-    XB_HIP(launch_sss_stats(c, 0));
-    XB_HIP(launch_sss_stats(c, 1));
-    int rc2 = forkBoth(c, [&](int im) -> int {
-      XB_HIP(launch_sss_find(c, im));
-      return XB_OK;
-    });
-    if (rc2) return rc2;
-    XB_HIP(launch_sss_compact(c));
-    return XB_OK;
-  };
-  rc = identify();
-  if (rc) return rc;
-  DevScalars sc;
-  rc = readScalars(c, sc);
-  if (rc) return rc;
   c.retried = 0;
-  if (sc.retry) {
-    c.retried = 1;
-    XB_HIP(launch_reset_skip(c));
-    rc = identify();
-    if (rc) return rc;
-    rc = readScalars(c, sc);
-    if (rc) return rc;
-  }
+  return ensureStamps(c, c.nGrid, fsw);
+}
+
+// calcStats T, calcStats S, then both searches (src/sssUtil.cpp:11-14); the
+// searches read only their own image's mask bits, so they overlap
+int sssIdentify(Ctx& c) {
+  XB_HIP(launch_sss_stats(c, 0));
+  XB_HIP(launch_sss_stats(c, 1));
+  int rc = forkBoth(c, [&](int im) -> int {
+    XB_HIP(launch_sss_find(c, im));
+    return XB_OK;
+  });
+  if (rc) return rc;
+  XB_HIP(launch_sss_compact(c));
+  return XB_OK;
+}
+
+int sssDone(Ctx& c, const DevScalars& sc) {
   c.nKept[0] = sc.nKept[0];
   c.nKept[1] = sc.nKept[1];
   if (c.nKept[0] == 0 && c.nKept[1] == 0) {
@@ -515,6 +506,25 @@ int stageSss(Ctx& c) {
     return XB_ERR_NO_SUBSTAMPS;
   }
   return XB_OK;
+}
+
+int stageSss(Ctx& c) {
+  int rc = sssSetup(c);
+  if (rc) return rc;
+  rc = sssIdentify(c);
+  if (rc) return rc;
+  DevScalars sc;
+  rc = readScalars(c, sc);
+  if (rc) return rc;
+  if (sc.retry) {  // fewer than 10 % of the stamps filled (src/bach.cpp:142-161)
+    c.retried = 1;
+    XB_HIP(launch_reset_skip(c));
+    rc = sssIdentify(c);
+    if (rc) return rc;
+    rc = readScalars(c, sc);
+    if (rc) return rc;
+  }
+  return sssDone(c, sc);
 }
 
 // cmv (src/bach.cpp:172-268)
@@ -969,6 +979,113 @@ int drainProfile(Ctx& c) {
 }  // namespace
 
 namespace xb {
+// ---- the pipeline in pieces, for a driver that interleaves several contexts
+// on one host thread (the batch runtime's lockstep mode).  Each piece
+// enqueues on the context's primary stream; a read-back is copied to the
+// pinned scalar mirror and signalled by c.evSync (c.evK[image] for a
+// clip-refit pass), which the driver waits for before the next piece.
+static hipError_t copyScalarsAsync(Ctx& c, size_t off, size_t bytes, hipEvent_t ev) {
+  hipError_t e = hipMemcpyAsync(reinterpret_cast<char*>(c.hsc) + off,
+                                reinterpret_cast<const char*>(c.sc) + off, bytes,
+                                hipMemcpyDeviceToHost, c.stream);
+  return e != hipSuccess ? e : hipEventRecord(ev, c.stream);
+}
+
+int ls_sss_begin(Ctx& c) {
+  c.cur = c.stream;
+  int rc = sssSetup(c);
+  if (rc) return rc;
+  rc = sssIdentify(c);
+  if (rc) return rc;
+  XB_HIP(copyScalarsAsync(c, 0, sizeof(DevScalars), c.evSync));
+  return XB_OK;
+}
+
+int ls_sss_next(Ctx& c, bool* more) {
+  *more = false;
+  XB_HIP(spinEvent(c.evSync));
+  if (!c.retried && c.hsc->retry) {
+    c.retried = 1;
+    XB_HIP(launch_reset_skip(c));
+    int rc = sssIdentify(c);
+    if (rc) return rc;
+    XB_HIP(copyScalarsAsync(c, 0, sizeof(DevScalars), c.evSync));
+    *more = true;
+    return XB_OK;
+  }
+  const DevScalars sc = *c.hsc;
+  return sssDone(c, sc);
+}
+
+int ls_cmv(Ctx& c) { return stageCmv(c); }
+
+int ls_cd_build(Ctx& c, LuBatch& B) {
+  for (int im = 0; im < 2; im++) {
+    if (c.nKept[im] == 0) continue;
+    XB_HIP(launch_stamp_lu(c, im));
+    XB_HIP(launch_cd_select(c, im));
+    XB_HIP(launch_global_build(c, im, true));
+    B.s[B.n++] = lu_sys(c.lu[im], c.cdSol + (size_t)im * NSOL, &c.sc->nTest[im]);
+  }
+  return XB_OK;
+}
+
+int ls_cd_post(Ctx& c) {
+  for (int im = 0; im < 2; im++) {
+    if (c.nKept[im] == 0) continue;
+    XB_HIP(launch_calc_sigs(c, im, true, c.cdSol + (size_t)im * NSOL));
+    XB_HIP(launch_cd_finish(c, im));
+  }
+  XB_HIP(copyScalarsAsync(c, offsetof(DevScalars, merit), 2 * sizeof(double), c.evSync));
+  return XB_OK;
+}
+
+int ls_cd_decide(Ctx& c) {
+  XB_HIP(spinEvent(c.evSync));
+  double merit[2];
+  for (int im = 0; im < 2; im++) merit[im] = c.nKept[im] == 0 ? 666 : c.hsc->merit[im];
+  c.convTemplate = merit[1] > merit[0] ? 1 : 0;
+  c.tIdx = c.convTemplate ? 0 : 1;
+  c.scrSamples = c.nScreened = c.nRestored = c.nConfirmed = 0;
+  return XB_OK;
+}
+
+int ls_ksc_build(Ctx& c, LuBatch& B) {
+  const int im = c.tIdx;
+  XB_HIP(launch_global_build(c, im, false));
+  B.s[B.n++] = lu_sys(c.ksLu[im], c.ksSol + (size_t)im * NSOL, nullptr);
+  return XB_OK;
+}
+
+int ls_ksc_post(Ctx& c, int it) {
+  const int im = c.tIdx;
+  const double* sol = c.ksSol + (size_t)im * NSOL;
+  XB_HIP(launch_calc_sigs(c, im, false, sol));
+  XB_HIP(launch_ksc_check(c, im, it == 0));
+  XB_HIP(launch_fill(c, im, c.st[im].listK, &c.sc->k[im].nList, c.nKept[im]));
+  XB_HIP(copyScalarsAsync(c, kscOffset(im) + offsetof(KscScalars, check), sizeof(int),
+                          c.evK[im]));
+  return XB_OK;
+}
+
+int ls_ksc_check(Ctx& c, int it, bool* more) {
+  const int im = c.tIdx;
+  XB_HIP(spinEvent(c.evK[im]));
+  *more = c.hsc->k[im].check != 0;
+  c.kscIterations = it;
+  if (*more && it > KSC_MAX_PASSES) {
+    c.err = "ksc loop did not terminate";
+    return XB_ERR_HIP;
+  }
+  return XB_OK;
+}
+
+int ls_finish(Ctx& c) {
+  int rc = stageConv(c, nullptr, false);
+  if (rc) return rc;
+  return stageSub(c);
+}
+
 void prof_begin(Ctx& c, int slot) {
   if (!c.profOn || slot < 0 || slot >= XB_PROF_N) return;
   hipEvent_t e = takeEvent(c);

@@ -31,6 +31,12 @@ struct xb_batch {
   int tuned[XB_NBUCKET] = {0, 0, 0, 0};  // pairs in flight per size bucket (0: untuned)
   std::vector<xb_ctx*> ctx;
   std::string err;
+  // lockstep mode (XB_BATCH_LOCKSTEP): the batched solves' stream and events
+  bool lockstep = false;
+  hipStream_t sstream = nullptr;
+  hipEvent_t evSolve = nullptr;
+  std::vector<hipEvent_t> evBuild;     // per context
+  int form = 0;
 };
 
 namespace {
@@ -154,6 +160,210 @@ void worker(xb_batch* b, int wi, Job* J) {
   }
 }
 
+// ---- lockstep mode -----------------------------------------------------------
+// One host thread runs a group of pairs together, each on its own context and
+// stream, stage by stage: every global solve of the group (the direction
+// trials of all pairs, then each clip-refit pass of the pairs still in their
+// loop) is one batched solve (launch_global_solve_batch: 23 launches for all
+// the group's systems) on the batch's solve stream, which waits for each
+// pair's build and is waited for by each pair's next stage.  A pair's
+// arithmetic is the single-pair pipeline's, so its outputs are bit-identical.
+// Groups are consecutive pairs of the largest-first order, up to the size
+// bucket's count (the autotuner's) and LU_BATCH_MAX / 2 pairs.
+int lsSolve(xb_batch* b, const std::vector<int>& slots, const xb::LuBatch& B,
+            std::string& err) {
+  hipError_t e;
+  for (int j : slots) {
+    xb::Ctx& c = b->ctx[j]->c;
+    if ((e = hipEventRecord(b->evBuild[j], c.stream)) != hipSuccess ||
+        (e = hipStreamWaitEvent(b->sstream, b->evBuild[j], 0)) != hipSuccess) {
+      err = std::string("lockstep solve: ") + hipGetErrorString(e);
+      return XB_ERR_HIP;
+    }
+  }
+  if ((e = xb::launch_global_solve_batch(b->sstream, B)) != hipSuccess) {
+    err = std::string("batched solve: ") + hipGetErrorString(e);
+    return XB_ERR_HIP;
+  }
+  if (b->form & XB_FORM_REFINE) {
+    // each system's refinement on the solve stream: the entry's system set is
+    // found among its pair's four (two trials, two clip-refit directions)
+    for (int k = 0; k < B.n; k++) {
+      const xb::LuSet* L = nullptr;
+      for (int j : slots) {
+        xb::Ctx& c = b->ctx[j]->c;
+        for (const xb::LuSet* q : {&c.lu[0], &c.lu[1], &c.ksLu[0], &c.ksLu[1]})
+          if (q->M == B.s[k].M) L = q;
+      }
+      if (L && (e = xb::launch_refine(b->sstream, *L, B.s[k].out)) != hipSuccess) {
+        err = std::string("refinement: ") + hipGetErrorString(e);
+        return XB_ERR_HIP;
+      }
+    }
+  }
+  if ((e = hipEventRecord(b->evSolve, b->sstream)) != hipSuccess) {
+    err = std::string("lockstep solve: ") + hipGetErrorString(e);
+    return XB_ERR_HIP;
+  }
+  for (int j : slots)
+    if ((e = hipStreamWaitEvent(b->ctx[j]->c.stream, b->evSolve, 0)) != hipSuccess) {
+      err = std::string("lockstep solve: ") + hipGetErrorString(e);
+      return XB_ERR_HIP;
+    }
+  return XB_OK;
+}
+
+void lsGroup(xb_batch* b, Job& J, const std::vector<int>& pairs) {
+  const int g = (int)pairs.size();
+  std::vector<int> rc(g, XB_OK);
+  std::vector<char> live(g, 1);
+  const double t0 = nowMs();
+  auto fail = [&](int j, int code) {
+    rc[j] = code;
+    live[j] = 0;
+    (void)hipStreamSynchronize(b->ctx[j]->c.stream);
+  };
+  auto liveSlots = [&]() {
+    std::vector<int> v;
+    for (int j = 0; j < g; j++)
+      if (live[j]) v.push_back(j);
+    return v;
+  };
+  // init + sss
+  for (int j = 0; j < g; j++) {
+    const int i = pairs[j];
+    xb_ctx* x = b->ctx[j];
+    int r = J.device ? xb_init_device(x, J.p, J.w[i], J.h[i], J.tin[i], J.sin[i])
+                     : xb_init(x, J.p, J.w[i], J.h[i], static_cast<const float*>(J.tin[i]),
+                               static_cast<const float*>(J.sin[i]));
+    if (!r) r = xb::ls_sss_begin(x->c);
+    if (r) fail(j, r);
+  }
+  for (bool again = true; again;) {
+    again = false;
+    for (int j : liveSlots()) {
+      bool more = false;
+      const int r = xb::ls_sss_next(b->ctx[j]->c, &more);
+      if (r) fail(j, r);
+      again = again || more;
+    }
+  }
+  // cmv, then the direction trials as one batched solve
+  xb::LuBatch B;
+  B.n = 0;
+  std::vector<int> slots;
+  for (int j : liveSlots()) {
+    int r = xb::ls_cmv(b->ctx[j]->c);
+    if (!r) r = xb::ls_cd_build(b->ctx[j]->c, B);
+    if (r) fail(j, r);
+    else slots.push_back(j);
+  }
+  std::string err;
+  if (!slots.empty() && lsSolve(b, slots, B, err) != XB_OK) {
+    for (int j : slots) {
+      b->ctx[j]->c.err = err;
+      fail(j, XB_ERR_HIP);
+    }
+  }
+  for (int j : liveSlots()) {
+    int r = xb::ls_cd_post(b->ctx[j]->c);
+    if (r) fail(j, r);
+  }
+  for (int j : liveSlots()) {
+    int r = xb::ls_cd_decide(b->ctx[j]->c);
+    if (r) fail(j, r);
+  }
+  // the clip-refit passes of the pairs still in their loop, one batched solve each
+  std::vector<char> inLoop(live.begin(), live.end());
+  for (int it = 0;; it++) {
+    B.n = 0;
+    slots.clear();
+    for (int j = 0; j < g; j++) {
+      if (!live[j] || !inLoop[j]) continue;
+      const int r = xb::ls_ksc_build(b->ctx[j]->c, B);
+      if (r) fail(j, r);
+      else slots.push_back(j);
+    }
+    if (slots.empty()) break;
+    if (lsSolve(b, slots, B, err) != XB_OK) {
+      for (int j : slots) {
+        b->ctx[j]->c.err = err;
+        fail(j, XB_ERR_HIP);
+      }
+      break;
+    }
+    for (int j : slots) {
+      const int r = xb::ls_ksc_post(b->ctx[j]->c, it);
+      if (r) fail(j, r);
+    }
+    for (int j : slots) {
+      if (!live[j]) continue;
+      bool more = false;
+      const int r = xb::ls_ksc_check(b->ctx[j]->c, it + 1, &more);
+      if (r) fail(j, r);
+      else if (!more) inLoop[j] = 0;
+    }
+  }
+  // conv, sub and the outputs
+  for (int j : liveSlots()) {
+    const int r = xb::ls_finish(b->ctx[j]->c);
+    if (r) fail(j, r);
+  }
+  for (int j : liveSlots()) {
+    const int i = pairs[j];
+    xb::Ctx& c = b->ctx[j]->c;
+    const size_t bytes = (size_t)J.w[i] * J.h[i] * sizeof(float);
+    const hipMemcpyKind kind = J.device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    hipError_t e = hipSuccess;
+    if (J.cout && J.cout[i]) e = hipMemcpyAsync(J.cout[i], c.conv, bytes, kind, c.stream);
+    if (e == hipSuccess && J.dout && J.dout[i])
+      e = hipMemcpyAsync(J.dout[i], c.diff, bytes, kind, c.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&c.kernSum, &c.sc->kernSum, sizeof(double), hipMemcpyDeviceToHost,
+                         c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) {
+      c.err = std::string("lockstep outputs: ") + hipGetErrorString(e);
+      fail(j, XB_ERR_HIP);
+    }
+  }
+  const double ms = nowMs() - t0;
+  for (int j = 0; j < g; j++) {
+    const int i = pairs[j];
+    xb::Ctx& c = b->ctx[j]->c;
+    J.ran[i] = 1;
+    J.status[i] = rc[j];
+    if (J.res) {
+      xb_pair_result& r = J.res[i];
+      r = xb_pair_result{};
+      r.status = rc[j];
+      r.worker = j;
+      r.ms = ms;
+      if (!rc[j]) {
+        r.convTemplate = c.convTemplate;
+        r.kscIterations = c.kscIterations;
+        r.kernSum = c.kernSum;
+        r.captureFailures = c.captureFailures;
+      }
+    }
+    if (rc[j] && (J.firstBad < 0 || i < J.firstBad)) {
+      J.firstBad = i;
+      J.firstMsg = "pair " + std::to_string(i) + ": " + c.err;
+    }
+  }
+}
+
+void runLockstep(xb_batch* b, Job& J) {
+  if (hipSetDevice(b->device) != hipSuccess) return;  // pairs reported as not run
+  const int maxGroup = std::min<int>((int)b->ctx.size(), xb::LU_BATCH_MAX / 2);
+  for (int k = 0; k < J.n;) {
+    const int cap = std::max(1, std::min(J.cap[k], maxGroup));
+    std::vector<int> pairs;
+    for (; k < J.n && (int)pairs.size() < cap; k++) pairs.push_back(J.order[k]);
+    lsGroup(b, J, pairs);
+  }
+}
+
 int runBatch(xb_batch* b, Job& J) {
   if (!b) return XB_ERR_ARG;
   if (!J.p || J.n < 0 || (J.n > 0 && (!J.w || !J.h || !J.tin || !J.sin))) {
@@ -188,10 +398,14 @@ int runBatch(xb_batch* b, Job& J) {
     nw = std::max(nw, J.cap[k]);
   }
   nw = std::min(nw, J.n);
-  std::vector<std::thread> th;
-  th.reserve(nw);
-  for (int wi = 0; wi < nw; wi++) th.emplace_back(worker, b, wi, &J);
-  for (auto& t : th) t.join();
+  if (b->lockstep && (b->form & (XB_FORM_SCREEN | XB_FORM_RECORD)) == 0) {
+    runLockstep(b, J);
+  } else {
+    std::vector<std::thread> th;
+    th.reserve(nw);
+    for (int wi = 0; wi < nw; wi++) th.emplace_back(worker, b, wi, &J);
+    for (auto& t : th) t.join();
+  }
   // pairs no worker took (every worker failed to select the device)
   for (int i = 0; i < J.n; i++) {
     if (J.ran[i]) continue;
@@ -237,13 +451,31 @@ int xb_batch_create(int device, int workers, int flags, xb_batch** out) {
     }
     b->ctx.push_back(x);
   }
+  if (flags & XB_BATCH_LOCKSTEP) {
+    b->lockstep = true;
+    b->evBuild.assign(workers, nullptr);
+    bool ok = hipSetDevice(device) == hipSuccess &&
+              hipStreamCreateWithFlags(&b->sstream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->evSolve, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < workers; i++)
+      ok = hipEventCreateWithFlags(&b->evBuild[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      xb_batch_destroy(b);
+      return XB_ERR_HIP;
+    }
+  }
   *out = b;
   return XB_OK;
 }
 
 int xb_batch_destroy(xb_batch* b) {
   if (!b) return XB_OK;
+  if (b->sstream) (void)hipStreamSynchronize(b->sstream);
   for (auto* x : b->ctx) xb_ctx_destroy(x);
+  for (auto e : b->evBuild)
+    if (e) (void)hipEventDestroy(e);
+  if (b->evSolve) (void)hipEventDestroy(b->evSolve);
+  if (b->sstream) (void)hipStreamDestroy(b->sstream);
   delete b;
   return XB_OK;
 }
@@ -317,6 +549,7 @@ int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, cons
 
 int xb_batch_set_form(xb_batch* b, int form) {
   if (!b) return XB_ERR_ARG;
+  b->form = form;
   for (auto* x : b->ctx) {
     const int rc = xb_set_form(x, form);
     if (rc) return rc;

@@ -289,6 +289,27 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, in
 hipError_t launch_stamp_lu(Ctx& c, int im);
 hipError_t launch_cd_select(Ctx& c, int im);
 hipError_t launch_global_build(Ctx& c, int im, bool cdForm);
+// A batch of global systems solved together (launch_global_solve_batch): each
+// system's buffers, passed by value in the kernel arguments.
+struct LuState;
+struct LuSys {
+  double* M;
+  double* vv;
+  int* piv;
+  double* rhs;
+  LuState* st;
+  const double* M0;
+  const double* rhs0;
+  double* out;
+  const int* skip;
+};
+constexpr int LU_BATCH_MAX = 32;  // systems per batched launch (kernel argument space)
+struct LuBatch {
+  LuSys s[LU_BATCH_MAX];
+  int n;
+};
+LuSys lu_sys(const LuSet& L, double* d_out, const int* d_skipIfZero);
+hipError_t launch_global_solve_batch(hipStream_t s, const LuBatch& B);
 // Solves system set L.  d_skipIfZero: when it points at a zero count the solve
 // is skipped and d_out is zeroed (the reference returns before solving an
 // empty fit).
@@ -330,6 +351,24 @@ hipError_t launch_screen_verify_decide(Ctx& c, int im);
 // looked up with hipGetSymbolAddress once per device and cached.
 constexpr int XB_MAX_DEVICES = 64;
 hipError_t scratch_address(int device, float** out);
+
+// The pipeline in pieces for the batch runtime's lockstep mode (xb_api.cpp):
+// sss (begin, then next until *more is false), cmv, cd (build: the trial
+// systems appended to B; post after the batched solve; decide after its
+// read-back), ksc passes (build: the system appended to B; post after the
+// batched solve; check after its read-back), then conv and sub.
+int ls_sss_begin(Ctx& c);
+int ls_sss_next(Ctx& c, bool* more);
+int ls_cmv(Ctx& c);
+int ls_cd_build(Ctx& c, LuBatch& B);
+int ls_cd_post(Ctx& c);
+int ls_cd_decide(Ctx& c);
+int ls_ksc_build(Ctx& c, LuBatch& B);
+int ls_ksc_post(Ctx& c, int it);
+int ls_ksc_check(Ctx& c, int it, bool* more);
+int ls_finish(Ctx& c);
+// FP64 iterative refinement of system L's solution (XB_FORM_REFINE) on stream s
+hipError_t launch_refine(hipStream_t s, const LuSet& L, double* d_out);
 
 }  // namespace xb
 

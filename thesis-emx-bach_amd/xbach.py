@@ -391,6 +391,7 @@ def fits_write_native(path, data: np.ndarray) -> None:
 # xb_ctx_create flags (include/xbach.h)
 CTX_ONE_STREAM = 1
 CTX_EAGER = 2  # global solves launched kernel by kernel instead of replayed hipGraphs
+BATCH_LOCKSTEP = 4  # xb_batch_create: groups of pairs in lockstep, one batched solve per solve point
 FORM_PLAIN, FORM_MIXED, FORM_SCREEN, FORM_RECORD = 0, 1, 2, 4
 FORM_FP16, FORM_REFINE = 8, 16  # with FORM_MIXED: FP16 split W; FP64 iterative refinement
 # the screen's network (include/xbach.h XB_MLP_*)
