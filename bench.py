@@ -58,7 +58,7 @@ PEAK_HBM_GBS = 8000.0
 # and WRITE_SIZE of k_conv, each scaled by what the same counter reads for a
 # device copy of known size (bench.py --copy-calibration); the summary with
 # both commands is profiles/k_conv_2048_traffic.md.
-CONV_TRAFFIC_BYTES = 236_902_796
+CONV_TRAFFIC_BYTES = 215_493_418
 # HBM bytes per global solve (the 23 kernels k_lu_scale, k_lu_begin, 19 x
 # k_lu_step, k_lu_backsub), measured the same way; profiles/lu_solve_traffic.md.
 LU_TRAFFIC_BYTES = 23_401_549
@@ -309,9 +309,6 @@ def main() -> int:
     conv_avg_s = conv_ms / max(conv_n, 1) * 1e-3
     conv_flops = 2.0 * 441 * (w - 20) * (h - 20)
     conv_tflops = conv_flops / conv_avg_s / 1e12
-    sub_ms, sub_n = prof["sub"]
-    sub_avg_s = sub_ms / max(sub_n, 1) * 1e-3
-    sub_bytes = 14.0 * w * h
     kernels = {k: {"ms_per_step": v[0] / args.steps, "launch_groups_per_step": v[1] / args.steps}
                for k, v in prof.items()}
     # the global FP64 solve (ludcmp + lubksb of the 292-unknown system): the
@@ -325,11 +322,12 @@ def main() -> int:
     kernels["solve"].update({"ms_per_solve": solve_avg_s * 1e3,
                              "us_per_column": solve_avg_s * 1e6 / LU_N,
                              "achieved_TFLOPs": solve_tflops})
-    kernels["sub"]["achieved_GBs"] = sub_bytes / sub_avg_s / 1e9
+    kernels.pop("sub", None)  # sub is formed in k_conv's epilogue
     kernels["conv"]["achieved_TFLOPs"] = conv_tflops
-    # conv bytes: algorithmic (FP32 template in, FP32 + FP64 conv and the u16
-    # mask out, mask in, FP64 tile kernels) and as measured by the counters
-    conv_bytes = 12.0 * w * h + 8.0 * w * h + 8.0 * 441 * -(-(w - 20) // 21) * -(-(h - 20) // 21)
+    # conv + sub bytes (one kernel): algorithmic (FP32 template and science and
+    # the u16 mask in, FP32 convolved and difference images and the u16 mask
+    # out, FP64 tile kernels in) and as measured by the counters
+    conv_bytes = 20.0 * w * h + 8.0 * 441 * -(-(w - 20) // 21) * -(-(h - 20) // 21)
     kernels["conv"]["achieved_GBs_algorithmic"] = conv_bytes / conv_avg_s / 1e9
     if (w, h) == (2048, 2048):
         kernels["conv"]["achieved_GBs_measured_traffic"] = CONV_TRAFFIC_BYTES / conv_avg_s / 1e9

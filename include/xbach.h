@@ -151,9 +151,12 @@ int xb_cmv(xb_ctx* ctx);
 int xb_cd(xb_ctx* ctx, int* convTemplate);
 /* ksc (src/bach.cpp:315-323): global fit with the chi2 clip-refit loop. */
 int xb_ksc(xb_ctx* ctx);
-/* conv (src/bach.cpp:325-413): spatially varying convolution; *kernSum out. */
+/* conv (src/bach.cpp:325-413): spatially varying convolution; *kernSum out.
+ * The same kernel forms the difference image (src/bach.cpp:415-440) in its
+ * epilogue, from the unrounded FP64 convolution of each pixel. */
 int xb_conv(xb_ctx* ctx, double* kernSum);
-/* sub (src/bach.cpp:415-440): difference image. */
+/* sub (src/bach.cpp:415-440): marks the difference image formed by xb_conv
+ * as the pipeline's result (no kernel of its own). */
 int xb_sub(xb_ctx* ctx);
 
 /* Results of the last pipeline (host copies). */
@@ -223,6 +226,7 @@ int xb_buf_upload(xb_ctx* ctx, void* d, const void* host, size_t bytes);
 int xb_buf_download(xb_ctx* ctx, void* host, const void* d, size_t bytes);
 
 /* ---- per-kernel timing with HIP events on the context's stream ---------- */
+/* XB_PROF_SUB stays at zero: the difference is formed inside XB_PROF_CONV. */
 enum { XB_PROF_CONV = 0, XB_PROF_SUB = 1, XB_PROF_SOLVE = 2, XB_PROF_FILL = 3,
        XB_PROF_SSS = 4, XB_PROF_MASK = 5, XB_PROF_N = 6 };
 /* enable != 0 starts recording (and clears the totals). */

@@ -127,7 +127,7 @@ struct ConvArgs {
   const float* S;      // target image
   uint16_t* mask;
   float* out;
-  double* outD;        // FP64 convolved image (what sub reads, as the reference)
+  float* diff;         // difference image (sub, fused: formed from the FP64 value)
   const double* tileK64;
   const double* tileAks;
   const double* sol;
@@ -226,7 +226,12 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
     }
   }
   if (!laneOn || x >= w - HKW) return;
-  const double inv = A.scaleConv ? 1.0 / *A.kernSum : 1.0;
+  const double ks = *A.kernSum;
+  const double inv = A.scaleConv ? 1.0 / ks : 1.0;
+  // sub (cl_kern/sub.cl:15-32, src/bach.cpp:415-440) on the unrounded FP64
+  // convolution, as the reference's sub reads its FP64 conv buffer
+  const double convFactor = A.scaleConv ? ks : 1.0;
+  const double finalFactor = A.scaleConv ? -(1.0 / ks) : 1.0;
   const double aks = A.tileAks[tile];
   const double thr = (double)0.99f;
   for (int r = 0; r < CR; r++) {
@@ -236,7 +241,6 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
     double v = acc[r];
     v = v + background_c(A.sol, x, y, w, h);
     v = v * inv;
-    A.outD[id] = v;
     A.out[id] = (float)v;
     const uint16_t cm = conv_flags(A.T[id], A.hi, A.lo);
     uint16_t nm = cm;
@@ -256,11 +260,17 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
       if ((uks / aks) < thr) nm |= XB_MASK_BAD_OUTPUT | XB_MASK_BAD_CONV;
       else nm |= XB_MASK_OK_CONV;
     }
-    A.mask[id] = nm | after_flags(A.S[id], A.hi, A.lo);
+    const float sv = A.S[id];
+    const uint16_t fm = nm | after_flags(sv, A.hi, A.lo);
+    A.mask[id] = fm;
+    double d = 1e-30;
+    if ((fm & XB_MASK_BAD_OUTPUT) == 0) d = (v * convFactor - (double)sv) * finalFactor;
+    A.diff[id] = (float)d;
   }
 }
 
-// border pixels: conv = 1e-30, mask keeps its value plus maskAfterConv flags.
+// border pixels: conv and difference = 1e-30, mask keeps its value plus
+// maskAfterConv flags.
 // NaN marks held back by a clip-refit loop that ran before the direction was
 // known become XB_MASK_NAN_PIXEL for the chosen direction (pend) and vanish for
 // the other; k_conv rewrites every interior pixel's mask from scratch, so the
@@ -268,7 +278,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
 __global__ __launch_bounds__(256) void k_conv_border(const float* __restrict__ S,
                                                      uint16_t* __restrict__ mask,
                                                      float* __restrict__ out,
-                                                     double* __restrict__ outD, int w, int h,
+                                                     float* __restrict__ diff, int w, int h,
                                                      double hi, double lo, uint16_t pend) {
   const size_t n = (size_t)w * h;
   for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
@@ -277,31 +287,11 @@ __global__ __launch_bounds__(256) void k_conv_border(const float* __restrict__ S
     bool interior = x >= HKW && x < w - HKW && y >= HKW && y < h - HKW;
     if (interior) continue;
     out[id] = 1e-30f;
-    outD[id] = 1e-30;
+    diff[id] = 1e-30f;
     uint16_t m = mask[id];
     if (m & (mask_pend(0) | mask_pend(1)))
       m = (uint16_t)((m & XB_MASK_ALL) | ((m & pend) ? XB_MASK_NAN_PIXEL : 0));
     mask[id] = m | after_flags(S[id], hi, lo);
-  }
-}
-
-__global__ __launch_bounds__(256) void k_sub(const float* __restrict__ S,
-                                             const double* __restrict__ I,
-                                             const uint16_t* __restrict__ mask,
-                                             float* __restrict__ D, int w, int h,
-                                             const double* kernSum, int scaleConv) {
-  const double ks = *kernSum;
-  const double convFactor = scaleConv ? ks : 1.0;
-  const double finalFactor = scaleConv ? -(1.0 / ks) : 1.0;
-  const size_t n = (size_t)w * h;
-  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
-       id += (size_t)gridDim.x * blockDim.x) {
-    int x = (int)(id % w), y = (int)(id / w);
-    double d = 1e-30;
-    if (x >= HKW && x < w - HKW && y >= HKW && y < h - HKW)
-      if ((mask[id] & XB_MASK_BAD_OUTPUT) == 0)
-        d = (I[id] * convFactor - (double)S[id]) * finalFactor;
-    D[id] = (float)d;
   }
 }
 
@@ -312,7 +302,7 @@ __global__ __launch_bounds__(256) void k_sub(const float* __restrict__ S,
 // of the reference's chi2 (cl_kern/cd.cl:578) — and the sum of (d / noise)^2.
 // Block partials in FP64, added in block order by k_diag_final.
 __global__ __launch_bounds__(256) void k_diag(const float* __restrict__ D,
-                                              const double* __restrict__ conv,
+                                              const float* __restrict__ conv,
                                               const float* __restrict__ S,
                                               const uint16_t* __restrict__ mask, int w, int h,
                                               double* __restrict__ part) {
@@ -327,7 +317,7 @@ __global__ __launch_bounds__(256) void k_diag(const float* __restrict__ D,
     const uint16_t m = mask[id];
     if (m & XB_MASK_BAD_OUTPUT) continue;
     const double d = (double)D[id];
-    const double nz = sqrt(fabs(conv[id]) + fabs((double)S[id]));
+    const double nz = sqrt(fabs((double)conv[id]) + fabs((double)S[id]));
     const double r = nz > 0.0 ? d / nz : 0.0;
     q[0] += 1.0; q[1] += d; q[2] += d * d; q[3] += nz; q[4] += r * r;
     if (m & XB_MASK_OK_CONV) {
@@ -356,7 +346,7 @@ __global__ void k_diag_final(const double* __restrict__ part, int nb, double* __
 }
 
 hipError_t launch_diag(Ctx& c, double* d_part, double* d_out) {
-  hipLaunchKernelGGL(k_diag, dim3(DIAG_BLOCKS), dim3(256), 0, c.stream, c.diff, c.convD,
+  hipLaunchKernelGGL(k_diag, dim3(DIAG_BLOCKS), dim3(256), 0, c.stream, c.diff, c.conv,
                      c.img[1 - c.tIdx], c.mask, c.w, c.h, d_part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -380,7 +370,7 @@ hipError_t launch_conv(Ctx& c) {
   A.S = c.img[1 - ti];
   A.mask = c.mask;
   A.out = c.conv;
-  A.outD = c.convD;
+  A.diff = c.diff;
   A.tileK64 = c.tileK64;
   A.tileAks = c.tileAks;
   A.sol = sol;
@@ -398,19 +388,7 @@ hipError_t launch_conv(Ctx& c) {
   prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_conv_border, dim3(1024), dim3(256), 0, c.cur, c.img[1 - ti], c.mask,
-                     c.conv, c.convD, c.w, c.h, c.m.threshHigh, c.m.threshLow, mask_pend(ti));
-  return hipGetLastError();
-}
-
-hipError_t launch_sub(Ctx& c) {
-  const int ti = c.tIdx;
-  const int scaleConv = (c.m.normalizeTemplate && c.convTemplate) || (!c.m.normalizeTemplate && !c.convTemplate);
-  size_t n = (size_t)c.w * c.h;
-  unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-  prof_begin(c, XB_PROF_SUB);
-  hipLaunchKernelGGL(k_sub, dim3(blocks), dim3(256), 0, c.cur, c.img[1 - ti], c.convD, c.mask,
-                     c.diff, c.w, c.h, &c.sc->kernSum, scaleConv);
-  prof_end(c, XB_PROF_SUB);
+                     c.conv, c.diff, c.w, c.h, c.m.threshHigh, c.m.threshLow, mask_pend(ti));
   return hipGetLastError();
 }
 

@@ -217,7 +217,6 @@ struct Ctx {
   float* img[2] = {nullptr, nullptr};  // 0 template, 1 science (FP32)
   uint16_t* mask = nullptr;
   float* conv = nullptr;
-  double* convD = nullptr;      // FP64 convolved image (sub reads it)
   float* diff = nullptr;
   DevStamps st[2];
   DevScalars* sc = nullptr;
@@ -327,7 +326,6 @@ hipError_t launch_conv(Ctx& c);
 // Restores the fit data of image im to what cmv left (every kept stamp on its
 // first substamp): stamps a clip-refit loop moved on are listed and refilled.
 hipError_t launch_restore_fill(Ctx& c, int im);
-hipError_t launch_sub(Ctx& c);
 // difference-image diagnostics: out[10] = good pixels (n, sum d, sum d^2,
 // sum noise, sum (d/noise)^2), then the same over OK_CONV pixels
 hipError_t launch_diag(Ctx& c, double* d_part, double* d_out);
