@@ -286,7 +286,9 @@ int xb_sync(xb_ctx* ctx);
  * (ksc), a small network (xb_screen_load) predicts from 32 features per stamp
  * which of the substamps the pass kept a later pass would replace, and
  * replaces them at once; the loop runs as usual and judges every substamp by
- * its chi2.  XB_FORM_RECORD: keep the 32 features of every substamp each of
+ * its chi2, and when it converges every screened substamp is re-judged by the
+ * chi2 test under the converged fit (kept ones are restored and the loop runs
+ * again without the screen).  XB_FORM_RECORD: keep the 32 features of every substamp each of
  * those passes kept and whether the loop finally replaced it
  * (xb_screen_samples): the training data of the network, labelled by the
  * method itself.  Forms combine (bitwise or). */
