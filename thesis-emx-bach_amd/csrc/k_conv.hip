@@ -6,22 +6,20 @@
 // convolution and the mask test), and sum |k| in the convolution loop's order
 // (cl_kern/conv.cl:409-426).
 //
-// Convolution (cl_kern/conv.cl:31-97): one wave per tile, lane = (column,
-// 7-row strip); the tile kernel is wave-uniform (scalar loads), image rows
-// stream through registers, FP64 accumulation in the reference's term order
-// (multiply and add unfused), so the convolved and difference images equal
-// the CPU restatement bit for bit.  (With packed FP32 disabled an FP64 VALU
-// op issues at the FP32 rate, so the FP64 form costs the same instruction
-// count as an FP32 one; its price is the unfused multiply-add.)  The output mask needs
-// sum |k| over unmasked pixels only when the window holds a flagged pixel; that
-// rare case re-walks the window in the reference order in FP64 so the
-// BAD_CONV decision is exact.  The flags of the tile's 41 x 41 input window
-// are tested once per pixel (one ballot per window row at the start, a bit
-// mask per row kept in one lane), so neither the accumulation loop nor the
-// masked re-walk tests pixel values; the re-walk reads the tile kernel from
-// LDS.  maskAfterConv (conv.cl:99-113) is fused.
+// Convolution (cl_kern/conv.cl:31-97): two waves per 21 x 21 tile over the
+// tile's input window staged in LDS, lane = (column, row group), FP64
+// accumulation in the reference's term order (multiply and add unfused), so
+// the convolved and difference images equal the CPU restatement bit for bit.
+// The output mask needs sum |k| over unmasked pixels only when the window
+// holds a flagged pixel; that rare case re-walks the window in the reference
+// order in FP64 so the BAD_CONV decision is exact.  The flags of the 41 x 41
+// window are tested once per pixel (one ballot per window row at the start, a
+// bit mask per row kept in one lane), so neither the accumulation loop nor
+// the masked re-walk tests pixel values; the re-walk reads the tile kernel
+// from LDS.  maskAfterConv (conv.cl:99-113) is fused.
 //
-// sub (cl_kern/sub.cl:15-32): elementwise, HBM-bound.
+// sub (cl_kern/sub.cl:15-32) is formed in the convolution's epilogue, from the
+// unrounded FP64 value of each pixel.
 #include <algorithm>
 
 #include "xb_dev.h"
@@ -155,77 +153,151 @@ __device__ __forceinline__ uint16_t after_flags(float t, double hi, double lo) {
   return m;
 }
 
-constexpr int CR = 7;  // output rows per lane
+// The convolution of one 21 x 21 output tile by two waves.  The tile's
+// 41 x 41 input window is staged in LDS once (the same clamped reads as the
+// reference's edge handling); lane = (column, row group) with three groups per
+// wave: the first wave's groups are output rows 0-3, 7-10 and 14-17 (CR = 4),
+// the second's 4-6, 11-13 and 18-20 (CR = 3), so every lane of both waves
+// computes a row of its own.  Output (x, y) sums K[20-d][20-i] * T[y-10+d][x-10+i]
+// over d, then i, ascending (the reference's term order: window rows, then
+// columns, ascending; multiply and add unfused).  Step d applies kernel row
+// 20 - d to all CR rows of a lane at once, so each kernel row is read once per
+// wave, as broadcast LDS reads of the tile kernel's LDS copy (the kernel is
+// read once per output row group instead of once per output row); the CR
+// window rows a step needs sit in a register ring of FP32 values (widened to
+// FP64 at each use, exact), one new row per step read from LDS with the
+// kernel row before the step's sums.
+constexpr int CTW = KW + 2 * HKW;  // 41: the tile's input window
+constexpr int CTP = CTW + 1;       // its LDS pitch
+
+// FP32 -> FP64 (exact) at the point of use: a volatile conversion the
+// compiler cannot merge across steps, which would keep the widened window in
+// twice the registers
+__device__ __forceinline__ double widen(float f) {
+  double d;
+  __asm__ __volatile__("v_cvt_f64_f32 %0, %1" : "=v"(d) : "v"(f));
+  return d;
+}
+
+template <int CR>
+__device__ __forceinline__ void conv_rows(const double* __restrict__ K, const float (*win)[CTP],
+                                          int g0, int cx, double (&acc)[CR]) {
+  float ring[CR][KW];
+#pragma unroll
+  for (int r = 0; r < CR; r++) acc[r] = 0.0;
+#pragma unroll
+  for (int r = 0; r + 1 < CR; r++) {
+    const float* src = win[g0 + r] + cx;
+#pragma unroll
+    for (int i = 0; i < KW; i++) ring[r][i] = src[i];
+  }
+  for (int d0 = 0; d0 <= 2 * HKW; d0 += CR) {
+#pragma unroll
+    for (int u = 0; u < CR; u++) {
+      const int d = d0 + u;  // (d + r) % CR == (u + r) % CR: ring slots are constants
+      if (d <= 2 * HKW) {
+        // the step's reads first, all from LDS: the kernel row (the same
+        // address in every lane) and the window row that replaces row d - 1
+        const double* Kr = K + (2 * HKW - d) * KW;
+        const float* src = win[g0 + d + CR - 1] + cx;
+        double kr[KW];
+#pragma unroll
+        for (int i = 0; i < KW; i++) kr[i] = Kr[2 * HKW - i];
+#pragma unroll
+        for (int i = 0; i < KW; i++) ring[(u + CR - 1) % CR][i] = src[i];
+        // columns outer, rows inner: the CR accumulators' chains interleave
+        // (each still takes its terms in column order)
+#pragma unroll
+        for (int i = 0; i < KW; i++) {
+          double pr[CR];
+#pragma unroll
+          for (int r = 0; r < CR; r++) pr[r] = kr[i] * widen(ring[(u + r) % CR][i]);
+#pragma unroll
+          for (int r = 0; r < CR; r++) acc[r] = acc[r] + pr[r];
+        }
+      }
+    }
+  }
+}
 
 __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
-  __shared__ double kTile[4][KK + 7];  // the wave's tile kernel, for the masked sums
-  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-  const int xs = blockIdx.x * 4 + wave, ys = blockIdx.y;
-  if (xs >= A.xSteps) return;
+  __shared__ float win[2][CTW][CTP];    // each tile's input window
+  __shared__ double kTile[2][KK + 7];   // each tile's kernel, for the masked sums
+  // wave-uniform (scalar) tile, half and so the tile kernel's address
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE);
+  const int lane = threadIdx.x % WAVE;
+  const int tl = wave >> 1, half = wave & 1;
+  const int xs = blockIdx.x * 2 + tl, ys = blockIdx.y;
   const int x0 = HKW + xs * KW, y0t = HKW + ys * KW;
-  if (x0 >= A.w - HKW || y0t >= A.h - HKW) return;
-  const int tile = __builtin_amdgcn_readfirstlane(xs + ys * A.xSteps);
-  const int cx = lane % KW, rg = lane / KW;  // rg 0..2 active, 3 idle
-  const int x = x0 + cx, y0 = y0t + rg * CR;
-  const bool laneOn = rg < 3;
-  const double* K = A.tileK64 + (size_t)tile * KK;
   const int w = A.w, h = A.h;
-  double* kl = kTile[wave];
-  for (int e = lane; e < KK; e += WAVE) kl[e] = K[e];
-  // Flag masks of the tile's 41 x 41 input window, one ballot per window row:
-  // lane l < 41 keeps bit c of row l = conv_flags of pixel
-  // (y0t - 10 + l, x0 - 10 + c), read with the same clamping as the values.
+  const bool tileOn = xs < A.xSteps && x0 < w - HKW && y0t < h - HKW;
+  const int tile = xs + ys * A.xSteps;
+  const double* K = A.tileK64 + (size_t)tile * KK;
+  if (tileOn) {
+    // the window, clamped as the reference's reads: row min(y, h - 1),
+    // column min(x, w - 1) (both start inside the image)
+    const int t2 = threadIdx.x % (2 * WAVE);
+    for (int e = t2; e < CTW * CTW; e += 2 * WAVE) {
+      const int l = e / CTW, c = e % CTW;
+      const int yy = min(y0t - HKW + l, h - 1), xx = min(x0 - HKW + c, w - 1);
+      win[tl][l][c] = A.T[(size_t)yy * w + xx];
+    }
+    for (int e = t2; e < KK; e += 2 * WAVE) kTile[tl][e] = K[e];
+  }
+  __syncthreads();
+  if (!tileOn) return;
+  const float (*W)[CTP] = win[tl];
+  // Flag masks of the window, one ballot per window row: lane l < 41 keeps
+  // bit c of row l = conv_flags of window pixel (l, c).
   uint32_t mlo = 0, mhi = 0;
   {
-    const int xx = min(max(x0 - HKW + lane, 0), w - 1);
-    for (int l = 0; l < 3 * CR + 2 * HKW; l++) {
-      const int yy = min(y0t - HKW + l, h - 1);
-      const float v = A.T[(size_t)yy * w + xx];
+    float col[CTW];
+#pragma unroll
+    for (int l = 0; l < CTW; l++) col[l] = W[l][min(lane, CTW - 1)];
+#pragma unroll
+    for (int l = 0; l < CTW; l++) {
       const unsigned long long m =
-          __ballot(lane < KW + 2 * HKW && conv_flags(v, A.hi, A.lo) != 0);
+          __ballot(lane < CTW && conv_flags(col[l], A.hi, A.lo) != 0);
       if (lane == l) {
         mlo = (uint32_t)m;
         mhi = (uint32_t)(m >> 32);
       }
     }
   }
-  // flags of window row wr of this lane's row group, shifted to its column
-  const int rgc = min(rg, 2);
+  const int cx = lane % KW, gi = lane / KW;  // gi 0..2 active, 3 idle
+  const int gic = min(gi, 2);
+  const int off = half ? 4 : 0;              // first row of the wave's groups
+  const int g0 = gic * 7 + off;              // the lane's first output row in the tile
+  const int x = x0 + cx;
+  // flags of window row g0 + wr (relative to the lane's group), shifted to its column
   auto rowMask = [&](int wr) -> unsigned long long {
-    const uint32_t l0 = __builtin_amdgcn_readlane(mlo, wr);
-    const uint32_t h0 = __builtin_amdgcn_readlane(mhi, wr);
-    const uint32_t l1 = __builtin_amdgcn_readlane(mlo, wr + CR);
-    const uint32_t h1 = __builtin_amdgcn_readlane(mhi, wr + CR);
-    const uint32_t l2 = __builtin_amdgcn_readlane(mlo, wr + 2 * CR);
-    const uint32_t h2 = __builtin_amdgcn_readlane(mhi, wr + 2 * CR);
-    const uint32_t lo = rgc == 0 ? l0 : (rgc == 1 ? l1 : l2);
-    const uint32_t hi = rgc == 0 ? h0 : (rgc == 1 ? h1 : h2);
+    const uint32_t l0 = __builtin_amdgcn_readlane(mlo, wr + off);
+    const uint32_t h0 = __builtin_amdgcn_readlane(mhi, wr + off);
+    const uint32_t l1 = __builtin_amdgcn_readlane(mlo, wr + off + 7);
+    const uint32_t h1 = __builtin_amdgcn_readlane(mhi, wr + off + 7);
+    const uint32_t l2 = __builtin_amdgcn_readlane(mlo, wr + off + 14);
+    const uint32_t h2 = __builtin_amdgcn_readlane(mhi, wr + off + 14);
+    const uint32_t lo = gic == 0 ? l0 : (gic == 1 ? l1 : l2);
+    const uint32_t hi = gic == 0 ? h0 : (gic == 1 ? h1 : h2);
     return (((unsigned long long)hi << 32) | lo) >> cx;
   };
-  double acc[CR];
-#pragma unroll
-  for (int r = 0; r < CR; r++) acc[r] = 0.0;
-  unsigned rowBad = 0;  // bit jr: a flagged pixel in window row jr (27 rows)
-  const int xc = min(x, w - 1);
-  for (int jr = 0; jr < CR + 2 * HKW; jr++) {
-    const int yy = min(y0 - HKW + jr, h - 1);
-    const float* Trow = A.T + (size_t)yy * w;
-    double row[KW];
-#pragma unroll
-    for (int ii = 0; ii < KW; ii++) {
-      const int xi = min(max(xc - HKW + ii, 0), w - 1);
-      row[ii] = (double)Trow[xi];
-    }
-    if (rowMask(jr) & ((1ull << KW) - 1ull)) rowBad |= 1u << jr;
-#pragma unroll
-    for (int r = 0; r < CR; r++) {
-      if (jr < r || jr > r + 2 * HKW) continue;  // uniform
-      const double* Kr = K + (r - jr + 2 * HKW) * KW;
-#pragma unroll
-      for (int ii = 0; ii < KW; ii++) acc[r] = acc[r] + Kr[2 * HKW - ii] * row[ii];
-    }
+  const int nr = half ? 3 : 4;
+  unsigned rowBad = 0;  // bit wr: a flagged pixel in the lane's window row g0 + wr
+  for (int wr = 0; wr < nr + 2 * HKW; wr++)
+    if (rowMask(wr) & ((1ull << KW) - 1ull)) rowBad |= 1u << wr;
+  const double* kl = kTile[tl];
+  double acc[4];
+  if (half) {
+    double a3[3];
+    conv_rows<3>(kl, W, g0, cx, a3);
+    acc[0] = a3[0];
+    acc[1] = a3[1];
+    acc[2] = a3[2];
+    acc[3] = 0.0;
+  } else {
+    conv_rows<4>(kl, W, g0, cx, acc);
   }
-  if (!laneOn || x >= w - HKW) return;
+  if (gi > 2 || x >= w - HKW) return;
   const double ks = *A.kernSum;
   const double inv = A.scaleConv ? 1.0 / ks : 1.0;
   // sub (cl_kern/sub.cl:15-32, src/bach.cpp:415-440) on the unrounded FP64
@@ -234,15 +306,15 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   const double finalFactor = A.scaleConv ? -(1.0 / ks) : 1.0;
   const double aks = A.tileAks[tile];
   const double thr = (double)0.99f;
-  for (int r = 0; r < CR; r++) {
-    const int y = y0 + r;
+  for (int r = 0; r < nr; r++) {
+    const int y = y0t + g0 + r;
     if (y >= h - HKW) break;
     const size_t id = (size_t)y * w + x;
     double v = acc[r];
     v = v + background_c(A.sol, x, y, w, h);
     v = v * inv;
     A.out[id] = (float)v;
-    const uint16_t cm = conv_flags(A.T[id], A.hi, A.lo);
+    const uint16_t cm = conv_flags(W[g0 + r + HKW][cx + HKW], A.hi, A.lo);
     uint16_t nm = cm;
     if (cm & XB_MASK_BAD_INPUT) nm |= XB_MASK_BAD_OUTPUT;
     if ((rowBad >> r) & ((1u << KW) - 1u)) {
@@ -383,7 +455,7 @@ hipError_t launch_conv(Ctx& c) {
   A.hi = c.m.threshHigh;
   A.lo = c.m.threshLow;
   prof_begin(c, XB_PROF_CONV);
-  hipLaunchKernelGGL(k_conv, dim3((xSteps + 3) / 4, ySteps), dim3(256), 0, c.cur, A);
+  hipLaunchKernelGGL(k_conv, dim3((xSteps + 1) / 2, ySteps), dim3(256), 0, c.cur, A);
   e = hipGetLastError();
   prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;
