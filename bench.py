@@ -58,7 +58,7 @@ PEAK_HBM_GBS = 8000.0
 # and WRITE_SIZE of k_conv, each scaled by what the same counter reads for a
 # device copy of known size (bench.py --copy-calibration); the summary with
 # both commands is profiles/k_conv_2048_traffic.md.
-CONV_TRAFFIC_BYTES = 215_493_418
+CONV_TRAFFIC_BYTES = 163_815_488
 # HBM bytes per global solve (the 23 kernels k_lu_scale, k_lu_begin, 19 x
 # k_lu_step, k_lu_backsub), measured the same way; profiles/lu_solve_traffic.md.
 LU_TRAFFIC_BYTES = 23_401_549
@@ -374,10 +374,10 @@ def main() -> int:
                              "achieved_TFLOPs": solve_tflops})
     kernels.pop("sub", None)  # sub is formed in k_conv's epilogue
     kernels["conv"]["achieved_TFLOPs"] = conv_tflops
-    # conv + sub bytes (one kernel): algorithmic (FP32 template and science and
-    # the u16 mask in, FP32 convolved and difference images and the u16 mask
-    # out, FP64 tile kernels in) and as measured by the counters
-    conv_bytes = 20.0 * w * h + 8.0 * 441 * -(-(w - 20) // 21) * -(-(h - 20) // 21)
+    # conv + sub bytes (one kernel): algorithmic (FP32 template and science
+    # in, FP32 convolved and difference images and the u16 mask out, FP64 tile
+    # kernels in) and as measured by the counters
+    conv_bytes = 18.0 * w * h + 8.0 * 441 * -(-(w - 20) // 21) * -(-(h - 20) // 21)
     kernels["conv"]["achieved_GBs_algorithmic"] = conv_bytes / conv_avg_s / 1e9
     if (w, h) == (2048, 2048):
         kernels["conv"]["achieved_GBs_measured_traffic"] = CONV_TRAFFIC_BYTES / conv_avg_s / 1e9
