@@ -542,6 +542,29 @@ __device__ __forceinline__ int wave_argmax_first(double v, int idx) {
   return __builtin_amdgcn_readlane(idx, 63);
 }
 
+// The walk's workgroup barrier.  Staged, every read of the walk is of LDS
+// (the stamp's values and bad bits), and the global mask's atomic ORs are
+// output only, so the barrier waits for LDS traffic alone; unstaged, the
+// walk reads the global mask, and the full barrier with its fences is needed.
+template <bool STAGED>
+__device__ __forceinline__ void walk_barrier() {
+  if (STAGED) {
+    __asm__ __volatile__("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  } else {
+    __syncthreads();
+  }
+}
+
+// The winning value of wave_argmax_first (the lane holding index wi, or
+// -inf when no lane has a candidate).
+__device__ __forceinline__ double readlane63_max_f64(double v, int idx, int wi) {
+  const unsigned long long own = __ballot(idx == wi && wi != INT_MAX);
+  if (own == 0ull) return -INFINITY;
+  const int src = __builtin_ctzll(own);
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), src),
+                          __builtin_amdgcn_readlane(__double2loint(v), src));
+}
+
 // The check's reductions in one pass: first bad position (with its type),
 // exponent range, partial sum.
 template <int CTRL, int RM>
@@ -586,6 +609,9 @@ constexpr int FIND_WAVES = FIND_THREADS / WAVE;
 // pp & 31, row pp >> 5, so no lane divides by the window width; the raster
 // index row * width + column keeps the reference's scan order for ties.
 constexpr int WIN_PER_LANE = 32 * 32 / WAVE;
+// The candidate walk's window work: four waves (one per SIMD), four slots each
+constexpr int WALK_WAVES = 4;
+constexpr int SLOTS_W = WIN_PER_LANE / WALK_WAVES;
 
 // Pixel access of one stamp's search.  With STAGED, the stamp's fsw x fsw
 // region lives in LDS: the pixel values (float) and one bit per pixel telling
@@ -625,15 +651,6 @@ struct StampPix {
       atomicOr(&sbad[k >> 5], 1u << (k & 31));
     }
   }
-  // make this wave's marks visible to its later reads
-  __device__ __forceinline__ void settle() const {
-    if (STAGED) {
-      __builtin_amdgcn_wave_barrier();  // LDS accesses of a wave complete in order
-      __asm__ __volatile__("" ::: "memory");
-    } else {
-      __threadfence_block();
-    }
-  }
 };
 
 // One 1024-thread workgroup per stamp.  Per threshold level, all 16 waves
@@ -641,9 +658,11 @@ struct StampPix {
 // raster segment [w*seg, (w+1)*seg) and appends its candidates, in raster
 // order, to its own slice of the scratch list (ballot + popcount).  Mask bits
 // are only ever added, so a pixel rejected here stays rejected; flagging
-// pixels above threshHigh commutes with everything else.  Wave 0 then
+// pixels above threshHigh commutes with everything else.  The workgroup then
 // processes the candidates one by one in raster order, re-checking each
-// against the current mask, with wave-level reductions only.  Mask bits are
+// against the current mask: every wave makes the same decisions, four of
+// them evaluate the 32 x 32 windows, and each step meets at a barrier (with
+// the stamp staged in LDS, one that waits for LDS traffic only).  Mask bits are
 // set with 32-bit atomic ORs: the template and science searches run at the
 // same time, each reading only its own bits (badMask) but sharing mask words.
 template <bool STAGED>
@@ -671,6 +690,10 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
   __shared__ int lx[MAXSS], ly[MAXSS];
   __shared__ double lv[MAXSS];
   __shared__ int s_counter;
+  __shared__ double wWinV[2][WALK_WAVES], wPart[2][WALK_WAVES], s_total;
+  __shared__ int wWinI[2][WALK_WAVES], wFb[2][WALK_WAVES], wBt[2][WALK_WAVES];
+  __shared__ int wEmn[2][WALK_WAVES], wEmx[2][WALK_WAVES];
+  static_assert(WALK_WAVES * SLOTS_W == WIN_PER_LANE, "the walkers cover the window");
   if (tid == 0) s_counter = 0;
   if (STAGED) {
     // stage the stamp: values, and the bad bits 64 pixels per ballot
@@ -738,18 +761,30 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
       if (tid == 0) pre[0] = 0;
     }
     __syncthreads();
-    if (wid == 0) {
-      if (lane == 0)
-        for (int w = 0; w < FIND_WAVES; w++) pre[w + 1] += pre[w];
-      __threadfence_block();
+    if (tid == 0)
+      for (int w = 0; w < FIND_WAVES; w++) pre[w + 1] += pre[w];
+    __syncthreads();
+    {
+      // The walk, by all 16 waves in step: each runs the same candidate
+      // selection (same data, same decisions); the first WALK_WAVES (one per
+      // SIMD) evaluate each 32 x 32 window, eight rows each (slots u = 4w ..
+      // 4w + 3 of the one-wave layout), and the others only meet them at the
+      // barriers.  Partial results meet in LDS (double-buffered by candidate
+      // parity) and are combined in the order-independent way the one-wave
+      // reductions use (maximum with the first index among ties, minimum
+      // first-bad position, exponent range, sums that are exact when the
+      // exponents are close).
       const int nc = pre[FIND_WAVES];
       int counter = s_counter;
+      int par = 0;
       // logical list position -> raster index
       auto candAt = [&](int q) {
         int w = 0;
         while (w + 1 < FIND_WAVES && pre[w + 1] <= q) w++;
         return cand[w * seg + (q - pre[w])];
       };
+      const bool walker = wid < WALK_WAVES;
+      const int px = lane & 31, py0 = (lane >> 5) + 2 * SLOTS_W * wid;  // this wave's first slot
       // candidates in batches of 64 (one per lane, raster order); within a
       // batch the next candidate is the first pending one whose pixel is
       // unmasked now -- the same one a sequential walk reaches next
@@ -766,20 +801,18 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
           pending &= ~((2ull << first) - 1ull);  // drop this lane and those before it
           const int absx = __builtin_amdgcn_readlane(qx, first);
           const int absy = __builtin_amdgcn_readlane(qy, first);
-          // ---- window maximum (sss.cl:427-459)
+          // ---- window maximum (sss.cl:427-459), this wave's slot
           const int startX = max(absx - HSS, g.ox), startY = max(absy - HSS, g.oy);
           const int endX = min(absx + HSS, g.ox + fsw - 1), endY = min(absy + HSS, g.oy + fsw - 1);
           const int ww = endX - startX + 1, wh = endY - startY + 1;
-          double bestV = 0.0;
-          int bestI = INT_MAX;
-          {
-            bool kb[WIN_PER_LANE];
-            float kvf[WIN_PER_LANE];
-            // loads at coordinates clamped into the window (always in
-            // bounds, no branch per slot); slots outside count as bad
+          if (walker) {
+            double bestV = 0.0;
+            int bestI = INT_MAX;
+            bool kb[SLOTS_W];
+            float kvf[SLOTS_W];
 #pragma unroll
-            for (int u = 0; u < WIN_PER_LANE; u++) {
-              const int pp = lane + u * WAVE, px = pp & 31, py = pp >> 5;
+            for (int u = 0; u < SLOTS_W; u++) {
+              const int py = py0 + 2 * u;
               const bool in = px < ww && py < wh;
               const int x = startX + min(px, ww - 1), y = startY + min(py, wh - 1);
               const bool b = P.bad(x, y);
@@ -787,10 +820,9 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
               kb[u] = !in || b;
             }
 #pragma unroll
-            for (int u = 0; u < WIN_PER_LANE; u++) {
-              const int pp = lane + u * WAVE, px = pp & 31, py = pp >> 5;
+            for (int u = 0; u < SLOTS_W; u++) {
+              const int py = py0 + 2 * u;
               if (kb[u]) continue;  // also every slot outside the window
-              const int p = py * ww + px;
               const double kv = (double)kvf[u];
               if (kv >= A.hi) {
                 P.mark(startX + px, startY + py, A.badPixelMask);
@@ -799,51 +831,68 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
               if ((kv - sky) * (1.0 / fwhm) < A.tkf) continue;
               if (kv > bestV) {
                 bestV = kv;
-                bestI = p;
+                bestI = py * ww + px;
+              }
+            }
+            if (bestI == INT_MAX) bestV = -INFINITY;
+            const int wi = wave_argmax_first(bestV, bestI);
+            const double wv = readlane63_max_f64(bestV, bestI, wi);
+            if (lane == 0) {
+              wWinV[par][wid] = wv;
+              wWinI[par][wid] = wi;
+            }
+          }
+          walk_barrier<STAGED>();  // partials and the window's marks
+          int win = INT_MAX;
+          {
+            double bv = -INFINITY;
+#pragma unroll
+            for (int w = 0; w < WALK_WAVES; w++) {
+              const double v = wWinV[par][w];
+              const int i = wWinI[par][w];
+              if (i != INT_MAX && (v > bv || (v == bv && i < win))) {
+                bv = v;
+                win = i;
               }
             }
           }
-          if (bestI == INT_MAX) bestV = -INFINITY;
-          const int win = wave_argmax_first(bestV, bestI);
-          P.settle();  // the window's marks before the check reads
           int mx = absx, my = absy;
           if (win != INT_MAX) {
             mx = startX + win % ww;
             my = startY + win / ww;
           }
-          // ---- checkSStamp (sss.cl:326-361)
+          // ---- checkSStamp (sss.cl:326-361), this wave's slot
           const int cx0 = max(mx - HSS, g.ox), cy0 = max(my - HSS, g.oy);
           const int cx1 = min(mx + HSS, g.ox + g.sw - 1), cy1 = min(my + HSS, g.oy + g.sh - 1);
           const int cw = cx1 - cx0 + 1, ch = cy1 - cy0 + 1;
-          int firstBad = INT_MAX;
-          int badType = 0;
-          double part = 0.0;
-          int eMin = 1000, eMax = -1000;
-          {
-            bool cb[WIN_PER_LANE];
-            float cvf[WIN_PER_LANE];
-            bool cin[WIN_PER_LANE];
+          if (walker) {
+            int firstBad = INT_MAX;
+            int badType = 0;
+            double part = 0.0;
+            int eMin = 1000, eMax = -1000;
+            bool cb[SLOTS_W], cin[SLOTS_W];
+            float cvf[SLOTS_W];
 #pragma unroll
-            for (int u = 0; u < WIN_PER_LANE; u++) {
-              const int pp = lane + u * WAVE, px = pp & 31, py = pp >> 5;
+            for (int u = 0; u < SLOTS_W; u++) {
+              const int py = py0 + 2 * u;
               cin[u] = px < cw && py < ch;
               const int x = cx0 + min(px, cw - 1), y = cy0 + min(py, ch - 1);
               cb[u] = P.bad(x, y);
               cvf[u] = P.value(x, y);
             }
 #pragma unroll
-            for (int u = 0; u < WIN_PER_LANE; u++) {
-              const int pp = lane + u * WAVE, px = pp & 31, py = pp >> 5;
+            for (int u = 0; u < SLOTS_W; u++) {
+              const int py = py0 + 2 * u;
               if (!cin[u]) continue;
-              const int p = py * cw + px;
+              const int pc = py * cw + px;
               if (cb[u]) {
-                if (p < firstBad) { firstBad = p; badType = 1; }
+                if (pc < firstBad) { firstBad = pc; badType = 1; }
                 continue;
               }
               const float fv = cvf[u];
               const double v = (double)fv;
               if (v > A.hi) {
-                if (p < firstBad) { firstBad = p; badType = 2; }
+                if (pc < firstBad) { firstBad = pc; badType = 2; }
                 continue;
               }
               if ((v - sky) / fwhm > A.tkf) {
@@ -855,52 +904,73 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
                 }
               }
             }
+            wave_check_reduce(firstBad, badType, eMin, eMax, part);
+            if (lane == 0) {
+              wFb[par][wid] = firstBad;
+              wBt[par][wid] = badType;
+              wEmn[par][wid] = eMin;
+              wEmx[par][wid] = eMax;
+              wPart[par][wid] = part;
+            }
           }
-          // one pass of reductions: first bad pixel (with its type), exponent
-          // range and the partial sums
-          wave_check_reduce(firstBad, badType, eMin, eMax, part);
-          // first bad pixel in raster order
-          const int fb = firstBad;
+          walk_barrier<STAGED>();  // the check's partials
+          int fb = INT_MAX, bt = 0, gMin = 1000, gMax = -1000;
+          double total = 0.0;
+#pragma unroll
+          for (int w = 0; w < WALK_WAVES; w++) {
+            const int f = wFb[par][w];
+            if (f < fb) {
+              fb = f;
+              bt = wBt[par][w];
+            }
+            gMin = min(gMin, wEmn[par][w]);
+            gMax = max(gMax, wEmx[par][w]);
+            total += wPart[par][w];
+          }
+          par ^= 1;
           if (fb != INT_MAX) {
-            if (lane == 0 && badType == 2) P.mark(cx0 + fb % cw, cy0 + fb / cw, A.badPixelMask);
-            P.settle();
-            continue;  // checkSStamp returned 0
+            // first bad pixel in raster order: checkSStamp returned 0
+            if (bt == 2) {
+              if (tid == 0) P.mark(cx0 + fb % cw, cy0 + fb / cw, A.badPixelMask);
+              walk_barrier<STAGED>();  // the mark before the next candidate's reads
+            }
+            continue;
           }
           // sum of qualifying values in raster order: exact in any order when
-          // the exponent spread is small (all partial sums representable, so
-          // the reduction's order does not matter), otherwise summed
-          // sequentially by one lane.
-          const int gMin = eMin;
-          const int gMax = eMax;
-          double total;
-          if (gMax < gMin || gMax - gMin <= 19) {
-            total = part;
-          } else {
-            double r = 0.0;
-            if (lane == 0)
+          // the exponent spread is small (all partial sums representable),
+          // otherwise summed sequentially by one lane
+          if (!(gMax < gMin || gMax - gMin <= 19)) {
+            if (tid == 0) {
+              double r = 0.0;
               for (int p = 0; p < cw * ch; p++) {
                 const double v = (double)P.value(cx0 + p % cw, cy0 + p / cw);
                 if ((v - sky) / fwhm > A.tkf) r += v;
               }
-            total = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(r), 0),
-                                     __builtin_amdgcn_readlane(__double2loint(r), 0));
+              s_total = r;
+            }
+            walk_barrier<STAGED>();
+            total = s_total;
           }
           if (total == 0.0) continue;
-          if (lane == 0) {
+          if (tid == 0) {
             lx[counter] = mx;
             ly[counter] = my;
             lv[counter] = total;
           }
           counter++;
-          // skip marks (sss.cl:473-482), clamped with the stamp size
-          for (int pp = lane; pp < 32 * 32; pp += WAVE) {
-            const int px = pp & 31, py = pp >> 5;
-            if (px < cw && py < ch) P.mark(cx0 + px, cy0 + py, A.skipMask);
+          // skip marks (sss.cl:473-482), clamped with the stamp size: this
+          // wave's eight rows of the box
+          if (walker) {
+#pragma unroll
+            for (int u = 0; u < SLOTS_W; u++) {
+              const int py = py0 + 2 * u;
+              if (px < cw && py < ch) P.mark(cx0 + px, cy0 + py, A.skipMask);
+            }
           }
-          P.settle();
+          walk_barrier<STAGED>();  // the marks before the next candidate's reads
         }
       }
-      if (lane == 0) s_counter = counter;
+      if (tid == 0) s_counter = counter;
     }
     __syncthreads();
     if (lim == floorV) break;
