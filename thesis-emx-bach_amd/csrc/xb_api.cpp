@@ -605,7 +605,8 @@ constexpr size_t kscOffset(int im) { return offsetof(DevScalars, k) + im * sizeo
 // checkFitSolution + the refill, src/cdkscUtil.cpp:585-684) on c.cur: the
 // global fit, every stamp's chi2, the clip check (which moves rejected stamps
 // to their next substamp), the screen, the refill; then the pass's check flag
-// is copied to the pinned mirror and evK[im] recorded.  waitCd (may be null):
+// (with the refill, chi2 and screened counts beside it) is copied to the
+// pinned mirror and evK[im] recorded.  waitCd (may be null):
 // cd's last read of image im's fit data, which the check and the refill must
 // follow.
 int kscPass(Ctx& c, int im, int it, bool keep, bool screen, uint16_t nanMark,
@@ -622,9 +623,11 @@ int kscPass(Ctx& c, int im, int it, bool keep, bool screen, uint16_t nanMark,
   XB_HIP(launch_ksc_check(c, im, it == 0));
   if (rec) XB_HIP(launch_screen(c, im, it, screen));
   XB_HIP(launch_fill(c, im, c.st[im].listK, &c.sc->k[im].nList, c.nKept[im]));
+  static_assert(offsetof(KscScalars, nScreened) == offsetof(KscScalars, check) + 3 * sizeof(int),
+                "check .. nScreened are one copy");
   const size_t off = kscOffset(im) + offsetof(KscScalars, check);
   XB_HIP(hipMemcpyAsync(reinterpret_cast<char*>(c.hsc) + off,
-                        reinterpret_cast<const char*>(c.sc) + off, sizeof(int),
+                        reinterpret_cast<const char*>(c.sc) + off, 4 * sizeof(int),
                         hipMemcpyDeviceToHost, c.cur));
   XB_HIP(hipEventRecord(c.evK[im], c.cur));
   return XB_OK;
@@ -688,7 +691,8 @@ int stageKsc(Ctx& c) {
       return XB_ERR_HIP;
     }
     if (c.hsc->k[im].check) continue;
-    if (!screen) break;
+    // nothing screened (the count is cumulative): no substamp to re-check
+    if (!screen || c.hsc->k[im].nScreened == 0) break;
     bool restored = false;
     for (;;) {
       int probed = 0, kept = 0;
@@ -725,11 +729,16 @@ int stageKsc(Ctx& c) {
       }
     }
     XB_HIP(launch_screen_labels(c, im, nRec, !screen));
-    XB_HIP(readBack(c, kscOffset(im), sizeof(KscScalars)));
-    if (screen) {
-      c.nScreened = c.hsc->k[im].nScreened;
-      c.nRestored = c.hsc->k[im].nRestored;
-      c.nConfirmed = c.hsc->k[im].nConfirmed;
+    // the counts: read back, unless the screen replaced nothing (all zero)
+    if (!screen || c.hsc->k[im].nScreened > 0) {
+      XB_HIP(readBack(c, kscOffset(im), sizeof(KscScalars)));
+      if (screen) {
+        c.nScreened = c.hsc->k[im].nScreened;
+        c.nRestored = c.hsc->k[im].nRestored;
+        c.nConfirmed = c.hsc->k[im].nConfirmed;
+      }
+    } else {
+      c.nScreened = c.nRestored = c.nConfirmed = 0;
     }
     c.scrSamples = nRec * c.nKept[im];
   }
