@@ -76,6 +76,11 @@ WORKLOADS = {
 }
 
 
+def progress(msg: str) -> None:
+    """One line on stderr per finished part of the run (the JSON line stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def rank_ci(times):
     """95 % rank interval of the median by the reference's rule
     (tools/exetime.py:6-21: z = (0.50798 + 0.51197) / 2, ranks
@@ -344,6 +349,7 @@ def main() -> int:
 
     plain_ms, prof = run_plain(ctx, params, w, h, d_t, d_s, args.steps, args.warmup)
     plain_value = 1e3 / plain_ms
+    progress(f"plain {plain_value:.1f} diffs/s")
     d_conv, d_diff = ctx.device_outputs()
     conv = np.empty_like(tmpl)
     diff = np.empty_like(tmpl)
@@ -408,6 +414,7 @@ def main() -> int:
     ref_rms = None
     if not args.no_cpu:
         ref_rms = fp64_residual(params, tmpl, sci)
+        progress("FP64 oracle residual done")
     if args.form == "both":
         pairs = synth_batch(cfg, args.batch)
         assert np.array_equal(pairs[0][0], tmpl) and np.array_equal(pairs[0][1], sci)
@@ -418,6 +425,7 @@ def main() -> int:
                                               tune=not args.no_tune, flags=ctx_flags)
         ok = all(r.status == 0 for r in res)
         b_value = args.batch * 1e3 / b_ms
+        progress(f"batched {b_value:.1f} diffs/s")
         forms["batched"] = {
             "value": b_value, "ms_per_batch": b_ms, "pairs_per_batch": args.batch,
             "workers": args.workers, "pairs_in_flight_per_bucket": btab,
@@ -435,6 +443,7 @@ def main() -> int:
                                             tune=not args.no_tune,
                                             flags=ctx_flags | xbach.BATCH_LOCKSTEP)
         l_value = args.batch * 1e3 / l_ms
+        progress(f"lockstep {l_value:.1f} diffs/s")
         forms["batched_lockstep"] = {
             "value": l_value, "ms_per_batch": l_ms, "pairs_per_batch": args.batch,
             "pairs_per_group_per_bucket": ltab, "steps": args.steps, "warmup": args.warmup,
@@ -454,6 +463,7 @@ def main() -> int:
         del pairs
         fok = all(r.status == 0 for r in fres)
         f_value = args.batch * 1e3 / f_ms
+        progress(f"mixed+batched+screened {f_value:.1f} diffs/s")
         rel = [abs(residual(a) - residual(b)) / residual(b) for a, b in zip(dfull, dplain)]
         full = {
             "value": f_value, "ms_per_batch": f_ms, "pairs_per_batch": args.batch,
@@ -527,6 +537,7 @@ def main() -> int:
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(params, tmpl, sci, args.cpu_runs)
+        progress(f"cpu baseline {line['cpu_baseline']['value']:.3f} diffs/s")
         # the same restatement on one host thread
         line["cpu_baseline_1thread"] = cpu_baseline(params, tmpl, sci,
                                                     max(10, args.cpu_runs // 2), 1)

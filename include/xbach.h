@@ -302,6 +302,17 @@ int xb_sync(xb_ctx* ctx);
  * products and sums, solved with the same LU factors and added to x). */
 #define XB_FORM_FP16 8
 #define XB_FORM_REFINE 16
+/* XB_FORM_LDLT: every global solve factors the symmetric positive definite
+ * normal matrix as L D L^T without pivoting (FP64; the trailing updates on the
+ * FP64 matrix cores) and substitutes, in one launch per system, instead of
+ * Numerical Recipes' ludcmp + lubksb in the reference's order.  The solution
+ * is not bit-identical to the reference's; on the fit's systems its error
+ * against the exact solution is that of ludcmp's (~1e-8 normwise), and with
+ * XB_FORM_REFINE (one step of FP64 iterative refinement in the same launch:
+ * compensated residual of the system as built, the same factors) ~1e-11.  A
+ * system with a zero row takes the reference's path (lubksb on the
+ * unfactorised matrix), bit for bit. */
+#define XB_FORM_LDLT 32
 int xb_set_form(xb_ctx* ctx, int form);
 
 /* ---- the learned substamp screen ------------------------------------------

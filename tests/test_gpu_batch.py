@@ -177,12 +177,15 @@ def test_lockstep_batch_equals_single_runs(pairs, singles, inflight):
     assert all(0 <= r.worker < inflight for r in res)
 
 
-def test_lockstep_refined_equals_single_refined(pairs):
-    # the batched solves followed by each system's iterative refinement give
-    # the single-context refined form's results bit for bit
+@pytest.mark.parametrize("form", [xbach.FORM_MIXED | xbach.FORM_REFINE,
+                                  xbach.FORM_MIXED | xbach.FORM_LDLT,
+                                  xbach.FORM_MIXED | xbach.FORM_LDLT | xbach.FORM_REFINE])
+def test_lockstep_refined_equals_single_refined(pairs, form):
+    # the batched solves (with each system's iterative refinement, or one
+    # L D L^T launch for all systems) give the single-context form's results
+    # bit for bit
     p = xbach.Params.default()
     use = pairs[:4]
-    form = xbach.FORM_MIXED | xbach.FORM_REFINE
     ref = []
     with xbach.Context(0) as c:
         c.set_form(form)

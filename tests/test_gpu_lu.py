@@ -149,3 +149,87 @@ def test_iterative_refinement(ctx, cond):
     assert errs["refined"] <= max(errs["plain"], 1e-15)
     if cond >= 1e10:
         assert errs["refined"] < 0.1 * errs["plain"]
+
+
+# ---------------------------------------------------------------------------
+# XB_FORM_LDLT: L D L^T of the symmetric positive definite normal matrix, no
+# pivoting (not the reference's order, so not bit-identical to it); bars
+# against the extended-precision solution.
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def fit_systems():
+    """The global normal systems the fit builds on the seeded 512x512 pair (the
+    oracle's cd direction trials and clip-refit passes): exactly symmetric,
+    positive definite, condition ~1e10 after diagonal scaling."""
+    O = _oracle.lib()
+    t, s = xbach.synth_pair(xbach.synth_cfg("pair512"))
+    O.xo_capture(1)
+    try:
+        _oracle.run(xbach.Params.default(), t, s)
+        out = []
+        for k in range(O.xo_captured_count()):
+            m = np.zeros(293 * 293)
+            r = np.zeros(298)
+            O.xo_captured_get(k, m.ctypes.data, r.ctypes.data)
+            out.append((m.reshape(293, 293), r))
+    finally:
+        O.xo_capture(0)
+    assert len(out) >= 4
+    return out
+
+
+def _ldlt(ctx, a, b, refine):
+    ctx.set_form(xbach.FORM_LDLT | (xbach.FORM_REFINE if refine else 0))
+    try:
+        return ctx.lu_solve(a, b)
+    finally:
+        ctx.set_form(xbach.FORM_PLAIN)
+
+
+def _err(x, xt):
+    d = x[1:N + 1].astype(np.longdouble) - xt
+    return float(np.sqrt(np.sum(d * d)) / np.sqrt(np.sum(xt * xt)))
+
+
+def test_ldlt_on_the_fits_systems(ctx, fit_systems):
+    """Unrefined L D L^T is as accurate as ludcmp on the fit's own systems (its
+    normwise error within 5x of the reference's, measured 0.3-1x); one
+    refinement step takes it to ~1e-11 (bar 1e-9, and 100x below ludcmp's).
+    The solution slots outside 1..292 are the right-hand side's, as lubksb
+    leaves them."""
+    for k, (a, b) in enumerate(fit_systems):
+        xt = _true_solution(a, b)
+        e_ref = _err(oracle_solve(a, b), xt)
+        x0 = _ldlt(ctx, a, b, False)
+        x1 = _ldlt(ctx, a, b, True)
+        e0, e1 = _err(x0, xt), _err(x1, xt)
+        print(f"system {k}: normwise error ludcmp {e_ref:.2e} ldlt {e0:.2e} ldlt+refine {e1:.2e}")
+        assert e0 <= max(5 * e_ref, 1e-12)
+        assert e1 <= 1e-9 and e1 <= 0.01 * e_ref
+        for x in (x0, x1):
+            assert x[0] == b[0] and np.array_equal(x[N + 1:], b[N + 1:len(x)])
+
+
+def test_ldlt_normal_matrix_like(ctx):
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(400, N)) * np.logspace(0, 5, N)
+    a = padded(x.T @ x)
+    b = rhs(rng)
+    xt = _true_solution(a, b)
+    e_ref = _err(oracle_solve(a, b), xt)
+    e0 = _err(_ldlt(ctx, a, b, False), xt)
+    e1 = _err(_ldlt(ctx, a, b, True), xt)
+    print(f"normwise error ludcmp {e_ref:.2e} ldlt {e0:.2e} ldlt+refine {e1:.2e}")
+    assert e0 <= max(5 * e_ref, 1e-12)
+    assert e1 <= max(e0, 1e-15)
+
+
+def test_ldlt_zero_row_takes_the_reference_path(ctx, fit_systems):
+    """A zero row: ludcmp returns before factorising and lubksb runs on the
+    unfactorised matrix; XB_FORM_LDLT does the same, bit for bit."""
+    a, b = fit_systems[0]
+    a = a.copy()
+    a[58, :] = 0.0
+    a[:, 58] = 0.0
+    for refine in (False, True):
+        assert same(_ldlt(ctx, a, b, refine), oracle_solve(a, b))

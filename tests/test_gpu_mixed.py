@@ -15,8 +15,9 @@ Bars:
     on pair512);
   * the same with W in FP16 split precision (XB_FORM_FP16: two FP16 parts,
     22 bits, range-scaled): within 2e-5;
-  * end to end on the 2048x2048 pair, for the BF16 leg, the FP16 leg and the
-    BF16 leg with FP64 iterative refinement of every global solve: the
+  * end to end on the 2048x2048 pair, for the BF16 leg, the FP16 leg, the
+    BF16 leg with FP64 iterative refinement of every global solve, and the BF16
+    leg with the global solves by L D L^T (XB_FORM_LDLT), unrefined and refined: the
     substamps selected (sss) are the plain form's, bit for bit; the difference
     image's residual RMS over good interior pixels is within 1 % of the FP64
     oracle's, its pixels within 1e-3 sigma_pix = sqrt(|T| + |S|) of the FP64
@@ -148,11 +149,12 @@ def pair2048_fp64():
     return t, s, d64, tr64.as_dict()
 
 
-@pytest.mark.parametrize("leg", ["bf16", "fp16", "bf16+refine"])
+@pytest.mark.parametrize("leg", ["bf16", "fp16", "bf16+refine", "bf16+ldlt", "bf16+ldlt+refine"])
 def test_mixed_pipeline_2048_against_fp64_oracle(ctx, pair2048_fp64, leg):
     p = xbach.Params.default()
     t, s, d64, tr64 = pair2048_fp64
-    form = LEGS[leg.split("+")[0]] | (xbach.FORM_REFINE if "refine" in leg else 0)
+    form = (LEGS[leg.split("+")[0]] | (xbach.FORM_REFINE if "refine" in leg else 0)
+            | (xbach.FORM_LDLT if "ldlt" in leg else 0))
     ctx.set_form(xbach.FORM_PLAIN)
     _, dplain, _ = ctx.run(p, t, s)
     trp = ctx.trace().as_dict()

@@ -1,0 +1,591 @@
+// k_ldlt.hip — the global FP64 solve of the fast forms (XB_FORM_LDLT).
+//
+// The fit's normal matrix (built by k_global_build, cl_kern/cd.cl:152-352) is
+// symmetric and positive definite: a sum over stamps of Gram matrices of the
+// model's derivatives with positive weights.  The plain form solves it with
+// Numerical Recipes' ludcmp + lubksb in their exact order (k_lu.hip,
+// src/bachUtil.cpp:319-411): partial pivoting makes 292 dependent pivot votes
+// (on the fit's systems it swaps rows at ~240 of the 292 columns) and lubksb's
+// back substitution is one ordered chain of ~42,500 subtractions.  A form that
+// does not have to round as the reference does can factor the same matrix as
+// L D L^T without pivoting, which is stable for a positive definite matrix and
+// on the fit's systems as accurate as ludcmp (~1e-8 normwise), and refine the
+// solution once in FP64 with a compensated residual (~1e-11).
+//
+// One workgroup (8 waves) solves one system in one launch:
+//   copy-in  the lower triangle of M0 (1-based, 293 x 293) into F (0-based,
+//            padded to 304 = 19 blocks of 16 with an identity tail); a row
+//            without a nonzero entry sends the system down the reference's
+//            path (below);
+//   factor   19 block steps: the panel (block column k) in LDS, its diagonal
+//            block's L D L^T by one wave, the panel rows below by one thread
+//            each, then the trailing lower tiles on the FP64 matrix cores
+//            (v_mfma_f64_16x16x4f64): A_IJ -= L_Ik W_Jk^T with W = L D;
+//   solve    L y = b, z = D^-1 y, L^T x = z with one thread per row: per block
+//            the diagonal block by 16 lanes of one wave (values passed by
+//            shuffles), then one barrier, then every later row (earlier column)
+//            takes the block's contribution; each thread loads the next block's
+//            factors while the current block runs;
+//   refine   (with XB_FORM_REFINE) r = b - M0 x with each product's rounding
+//            error kept (fma) and the sums compensated, then L D L^T d = r and
+//            x += d.
+//
+// A row of M0 with no nonzero finite entry is where ludcmp stops ("singular
+// matrix": it returns before factorising, src/bachUtil.cpp:326-334) and
+// lubksb then runs on the unfactorised matrix with an index array of zeros.
+// That case runs the same literal lubksb here, so such systems give what the
+// plain form gives, bit for bit.  A skipped solve (empty stamp list) zeroes the
+// output as the plain form does.
+#include <type_traits>
+
+#include "xb_dev.h"
+
+namespace xb {
+
+constexpr int LT_B = 16;                            // block order (one MFMA tile)
+constexpr int LT_NB = (MATSIZE + LT_B - 1) / LT_B;  // 19 blocks
+constexpr int LT_NP = LT_NB * LT_B;                 // 304: padded order
+constexpr int LT_THREADS = 512;
+constexpr int LT_WAVES = LT_THREADS / WAVE;
+constexpr int LT_LDW = LT_B + 1;  // LDS pitch of a panel row (doubles)
+constexpr int LT_GRP = 4;         // trailing tiles per load group
+constexpr int LT_TILES = LT_NB * (LT_NB + 1) / 2;  // 190 lower tiles
+constexpr int LT_CB = 16;         // copy-in elements per thread in flight
+constexpr int LT_RK = (MATSIZE + 7) / 8;  // residual terms per lane of a row
+constexpr double LT_TINY = 1.0e-20;  // an exact zero pivot, as ludcmp's TINY
+static_assert(LT_NP <= LT_THREADS, "one thread per row in the substitutions");
+static_assert(LT_TILES <= LT_THREADS, "one thread per tile entry");
+static_assert(LT_NP * LT_NP == LDLT_WORK_DOUBLES, "workspace size");
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+struct LtShared {
+  double P[2][LT_NP][LT_LDW];  // panels: w = l d (unscaled), this step's and the next
+  double G[LT_NP][LT_LDW];     // this panel's -l (the trailing products' A operand)
+  double d[LT_NP];             // D
+  double rinv[LT_NP];          // 1 / D
+  double y[LT_NP];             // block values published by the substitutions
+  double x[LT_NP];             // the solution (0-based)
+  double r[LT_NP];             // the refinement's residual
+  int nz[LT_NP];               // row has a nonzero finite entry
+  unsigned char tI[LT_TILES], tJ[LT_TILES];  // lower tiles, column by column
+  int zero;
+};
+
+__device__ __forceinline__ double lt_readlane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// A 32-bit DPP move of each half of a double (CTRL: a DPP control code).
+template <int CTRL>
+__device__ __forceinline__ double lt_dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ void lt_two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void lt_dd_add(double& hi, double& lo, double h2, double l2) {
+  double s, e;
+  lt_two_sum(hi, h2, s, e);
+  e = e + (lo + l2);
+  hi = s + e;
+  lo = e - (hi - s);
+}
+
+// One step of a 16-lane diagonal-block substitution: v += v[lane K of the
+// lane's row of 16] * m (64-bit DPP broadcast).  The s_nop covers the DPP read
+// of v after the previous step's write.
+#define LT_BCAST_FMA(K, M)                                                           \
+  __asm__ __volatile__("s_nop 1\n\tv_fmac_f64_dpp %0, %0, %1 row_newbcast:" #K         \
+                       " row_mask:0xf bank_mask:0xf"                                 \
+                       : "+v"(v)                                                     \
+                       : "v"(M));
+
+// A barrier for LDS traffic only: outstanding global loads stay in flight.
+__device__ __forceinline__ void lt_lds_barrier() {
+  __asm__ __volatile__("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// x = (L D L^T)^-1 b for the factors in F and sh.rinv.  Thread r (< LT_NP)
+// passes b_r and receives x_r; the other threads pass and receive 0.  Every
+// thread of the workgroup calls it (it holds barriers).  A block's 16 rows are
+// 16 lanes of one wave, one DPP row, so the block's own triangle is solved by a
+// chain of broadcast multiply-adds without memory traffic.  Each thread keeps
+// the factors of the next two blocks in flight (a ring of three), and the
+// barriers wait for LDS traffic only.
+__device__ __forceinline__ double lt_substitute(const double* __restrict__ F, LtShared& sh,
+                                                double v) {
+  const int r = threadIdx.x;
+  const bool act = r < LT_NP;
+  const int kr = r / LT_B, ir = r % LT_B;
+  double R[3][LT_B];
+  // forward: L y = b.  Row r needs L[r][16k .. 16k + 15] for blocks k <= kr,
+  // read from the transposed copy above the diagonal (F[16k + j][r]), so that
+  // neighbouring rows read neighbouring addresses.
+  // (Every load is unconditional from an address inside F; rows that do not
+  // need a block never read its values.  A guarded load would become a branch
+  // whose join waits for the load.)
+  // A wave loads a block only when one of its rows needs it (a wave-uniform
+  // test: its last row is at or below the block).
+  const int wLast = min((r | (WAVE - 1)), LT_NP - 1);  // the wave's last row
+  auto loadRow = [&](int k, double (&o)[LT_B]) __attribute__((always_inline)) {
+    if (k < LT_NB && wLast >= LT_B * k) {
+      const double* p = F + (size_t)(LT_B * k) * LT_NP + min(r, LT_NP - 1);
+#pragma unroll
+      for (int j = 0; j < LT_B; j++) o[j] = p[(size_t)j * LT_NP];
+    }
+  };
+  auto fwd = [&](auto slot, int k) __attribute__((always_inline)) {
+    constexpr int SL = decltype(slot)::value;
+    loadRow(k + 2, R[(SL + 2) % 3]);
+    const double(&Lk)[LT_B] = R[SL];
+    if (act && kr == k) {
+      double m[LT_B];
+#pragma unroll
+      for (int j = 0; j < LT_B; j++) m[j] = ir > j ? -Lk[j] : 0.0;
+      LT_BCAST_FMA(0, m[0]) LT_BCAST_FMA(1, m[1]) LT_BCAST_FMA(2, m[2])
+      LT_BCAST_FMA(3, m[3]) LT_BCAST_FMA(4, m[4]) LT_BCAST_FMA(5, m[5])
+      LT_BCAST_FMA(6, m[6]) LT_BCAST_FMA(7, m[7]) LT_BCAST_FMA(8, m[8])
+      LT_BCAST_FMA(9, m[9]) LT_BCAST_FMA(10, m[10]) LT_BCAST_FMA(11, m[11])
+      LT_BCAST_FMA(12, m[12]) LT_BCAST_FMA(13, m[13]) LT_BCAST_FMA(14, m[14])
+      sh.y[r] = v;
+    }
+    lt_lds_barrier();
+    if (act && kr > k) {
+      const double* yb = sh.y + LT_B * k;
+      double v2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < LT_B; j += 2) {
+        v = fma(-Lk[j], yb[j], v);
+        v2 = fma(-Lk[j + 1], yb[j + 1], v2);
+      }
+      v = v + v2;
+    }
+  };
+  loadRow(0, R[0]);
+  loadRow(1, R[1]);
+  for (int k = 0; k < LT_NB; k += 3) {
+    fwd(std::integral_constant<int, 0>(), k);
+    if (k + 1 < LT_NB) fwd(std::integral_constant<int, 1>(), k + 1);
+    if (k + 2 < LT_NB) fwd(std::integral_constant<int, 2>(), k + 2);
+  }
+  if (act) v = v * sh.rinv[r];
+  lt_lds_barrier();  // the forward pass's reads of sh.y are done
+  // backward: L^T x = z.  Row r needs L[16k .. 16k + 15][r] for blocks k >= kr
+  // (below the diagonal, row-major).
+  const int wFirst = r & ~(WAVE - 1);  // the wave's first row
+  auto loadCol = [&](int k, double (&o)[LT_B]) __attribute__((always_inline)) {
+    if (k >= 0 && wFirst <= LT_B * k + LT_B - 1 && wFirst < LT_NP) {
+      const double* p = F + (size_t)(LT_B * k) * LT_NP + min(r, LT_NP - 1);
+#pragma unroll
+      for (int j = 0; j < LT_B; j++) o[j] = p[(size_t)j * LT_NP];
+    }
+  };
+  auto bwd = [&](auto slot, int k) __attribute__((always_inline)) {
+    constexpr int SL = decltype(slot)::value;
+    loadCol(k - 2, R[(SL + 2) % 3]);
+    const double(&Lk)[LT_B] = R[SL];
+    if (act && kr == k) {
+      double m[LT_B];
+#pragma unroll
+      for (int j = 0; j < LT_B; j++) m[j] = ir < j ? -Lk[j] : 0.0;
+      LT_BCAST_FMA(15, m[15]) LT_BCAST_FMA(14, m[14]) LT_BCAST_FMA(13, m[13])
+      LT_BCAST_FMA(12, m[12]) LT_BCAST_FMA(11, m[11]) LT_BCAST_FMA(10, m[10])
+      LT_BCAST_FMA(9, m[9]) LT_BCAST_FMA(8, m[8]) LT_BCAST_FMA(7, m[7])
+      LT_BCAST_FMA(6, m[6]) LT_BCAST_FMA(5, m[5]) LT_BCAST_FMA(4, m[4])
+      LT_BCAST_FMA(3, m[3]) LT_BCAST_FMA(2, m[2]) LT_BCAST_FMA(1, m[1])
+      sh.y[r] = v;
+    }
+    lt_lds_barrier();
+    if (act && kr < k) {
+      const double* xb = sh.y + LT_B * k;
+      double v2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < LT_B; j += 2) {
+        v = fma(-Lk[j], xb[j], v);
+        v2 = fma(-Lk[j + 1], xb[j + 1], v2);
+      }
+      v = v + v2;
+    }
+  };
+  loadCol(LT_NB - 1, R[0]);
+  loadCol(LT_NB - 2, R[1]);
+  for (int k = LT_NB - 1; k >= 0; k -= 3) {
+    bwd(std::integral_constant<int, 0>(), k);
+    if (k - 1 >= 0) bwd(std::integral_constant<int, 1>(), k - 1);
+    if (k - 2 >= 0) bwd(std::integral_constant<int, 2>(), k - 2);
+  }
+  lt_lds_barrier();  // sh.y free again
+  return act ? v : 0.0;
+}
+#undef LT_BCAST_FMA
+
+// The reference's path for a system with a zero row: lubksb on the
+// unfactorised matrix with a zero index array (ludcmp returned first).  One
+// thread, in lubksb's order (src/bachUtil.cpp:384-411).
+__device__ void lt_zero_row_path(const double* __restrict__ M0, double* xs) {
+  const int n = MATSIZE, N = MATN;
+  int ii = 0;
+  for (int i = 1; i <= n; i++) {
+    const int ip = 0;
+    double sum = xs[ip];
+    xs[ip] = xs[i];
+    if (ii) {
+      for (int j = ii; j <= i - 1; j++) sum -= M0[(size_t)i * N + j] * xs[j];
+    } else if (sum != 0.0) {
+      ii = i;
+    }
+    xs[i] = sum;
+  }
+  for (int i = n; i >= 1; i--) {
+    double sum = xs[i];
+    for (int j = i + 1; j <= n; j++) sum -= M0[(size_t)i * N + j] * xs[j];
+    xs[i] = sum / M0[(size_t)i * N + i];
+  }
+}
+
+// lane K's value of x within each row of 16 lanes (64-bit DPP broadcast; the
+// s_nop covers a DPP read of a VGPR the previous instruction wrote)
+#define LT_BCAST(OUT, X, K)                                                           \
+  {                                                                                  \
+    double z_ = 0.0;                                                                 \
+    __asm__ __volatile__("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3"          \
+                         " row_mask:0xf bank_mask:0xf"                               \
+                         : "+v"(z_)                                                  \
+                         : "v"(X), "v"(1.0), "i"(K));                                \
+    OUT = z_;                                                                        \
+  }
+
+// The diagonal block of panel k (rows 0..15 of P) as L D L^T, in registers of
+// wave 0: lane i holds row i & 15 (its entries right of the diagonal taken
+// from the lower triangle by symmetry, then kept up to date as the matrix
+// stays symmetric).  Column j: d_j and row j's entries right of j reach every
+// lane of the DPP row from lane j; each row takes l_ij = a_ij / d_j and
+// updates its entries right of j.  Stores W (unscaled, below the diagonal),
+// d and 1/d, and L and D into F (lower part, and L transposed into the upper).
+template <int J>
+__device__ __forceinline__ void lt_diag_col(double (&a)[LT_B], double& dj, double& ri) {
+  double dd;
+  LT_BCAST(dd, a[J], J)
+  if (dd == 0.0) dd = LT_TINY;
+  dj = dd;
+  ri = 1.0 / dd;
+  const double l = a[J] * ri;
+#pragma unroll
+  for (int q = J + 1; q < LT_B; q++) {
+    double cq;
+    LT_BCAST(cq, a[q], J)
+    a[q] = fma(-l, cq, a[q]);
+  }
+}
+
+__device__ __forceinline__ void lt_diag_block(LtShared& sh, int k, double* __restrict__ F) {
+  const int lane = lane_id(), i = lane & (LT_B - 1), c0 = LT_B * k;
+  double (*P)[LT_LDW] = sh.P[k & 1];
+  double a[LT_B], dv[LT_B], rv[LT_B];
+#pragma unroll
+  for (int q = 0; q < LT_B; q++) a[q] = q <= i ? P[i][q] : P[q][i];
+  lt_diag_col<0>(a, dv[0], rv[0]);   lt_diag_col<1>(a, dv[1], rv[1]);
+  lt_diag_col<2>(a, dv[2], rv[2]);   lt_diag_col<3>(a, dv[3], rv[3]);
+  lt_diag_col<4>(a, dv[4], rv[4]);   lt_diag_col<5>(a, dv[5], rv[5]);
+  lt_diag_col<6>(a, dv[6], rv[6]);   lt_diag_col<7>(a, dv[7], rv[7]);
+  lt_diag_col<8>(a, dv[8], rv[8]);   lt_diag_col<9>(a, dv[9], rv[9]);
+  lt_diag_col<10>(a, dv[10], rv[10]); lt_diag_col<11>(a, dv[11], rv[11]);
+  lt_diag_col<12>(a, dv[12], rv[12]); lt_diag_col<13>(a, dv[13], rv[13]);
+  lt_diag_col<14>(a, dv[14], rv[14]); lt_diag_col<15>(a, dv[15], rv[15]);
+  if (lane < LT_B) {
+    sh.d[c0 + i] = dv[i];
+    sh.rinv[c0 + i] = rv[i];
+    double* Frow = F + (size_t)(c0 + i) * LT_NP + c0;
+#pragma unroll
+    for (int q = 0; q < LT_B; q++) {
+      if (q < i) {
+        P[i][q] = a[q];
+        const double l = a[q] * rv[q];
+        Frow[q] = l;                                  // L, row-major (backward pass)
+        F[(size_t)(c0 + q) * LT_NP + c0 + i] = l;     // L^T (forward pass)
+      } else if (q == i) {
+        Frow[q] = dv[q];
+      }
+    }
+  }
+}
+#undef LT_BCAST
+
+__device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
+  extern __shared__ double lt_dyn[];
+  LtShared& sh = *reinterpret_cast<LtShared*>(lt_dyn);
+  const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
+  const double* __restrict__ M0 = S.M0;
+  const double* __restrict__ b0 = S.rhs0;
+  double* __restrict__ F = S.F;
+  if (S.skip && *S.skip == 0) {
+    // testFit returns 666 before solving (src/cdkscUtil.cpp:101-103)
+    for (int i = tid; i < NSOL; i += LT_THREADS)
+      if (S.out) S.out[i] = 0.0;
+    return;
+  }
+  // ---- copy-in, the zero-row check, the tile list -------------------------
+  for (int u = tid; u < LT_NP; u += LT_THREADS) sh.nz[u] = 0;
+  if (tid == 0) sh.zero = 0;
+  if (tid < LT_TILES) {
+    int J = 0, rem = tid;
+    while (rem >= LT_NB - J) {
+      rem -= LT_NB - J;
+      J++;
+    }
+    sh.tI[tid] = (unsigned char)(J + rem);
+    sh.tJ[tid] = (unsigned char)J;
+  }
+  __syncthreads();
+  {
+    // the padded lower triangle, row by row, LT_CB elements per thread in
+    // flight at a time
+    constexpr int total = LT_NP * (LT_NP + 1) / 2;
+    for (int e0 = tid; e0 < total; e0 += LT_THREADS * LT_CB) {
+      double val[LT_CB];
+      int uu[LT_CB], vv[LT_CB];
+#pragma unroll
+      for (int i = 0; i < LT_CB; i++) {
+        const int e = min(e0 + LT_THREADS * i, total - 1);
+        int u = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
+        if ((u + 1) * (u + 2) / 2 <= e) u++;
+        if (u * (u + 1) / 2 > e) u--;
+        uu[i] = u;
+        vv[i] = e - u * (u + 1) / 2;
+        const double m = M0[(size_t)(min(u, MATSIZE - 1) + 1) * MATN + min(vv[i], MATSIZE - 1) + 1];
+        val[i] = u < MATSIZE ? m : (u == vv[i] ? 1.0 : 0.0);
+      }
+#pragma unroll
+      for (int i = 0; i < LT_CB; i++) {
+        if (e0 + LT_THREADS * i < total) {
+          if (uu[i] < MATSIZE && fabs(val[i]) > 0.0) {  // NaN is not taken, as ludcmp's row maximum
+            sh.nz[uu[i]] = 1;
+            sh.nz[vv[i]] = 1;
+          }
+          F[(size_t)uu[i] * LT_NP + vv[i]] = val[i];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < MATSIZE && !sh.nz[tid]) sh.zero = 1;
+  __syncthreads();
+  if (sh.zero) {
+    double* xs = sh.x;  // NSOL entries
+    for (int i = tid; i < NSOL; i += LT_THREADS) xs[i] = b0[i];
+    __syncthreads();
+    if (tid == 0) lt_zero_row_path(M0, xs);
+    __syncthreads();
+    for (int i = tid; i < NSOL; i += LT_THREADS) {
+      S.rhs[i] = xs[i];
+      if (S.out) S.out[i] = xs[i];
+    }
+    return;
+  }
+  // panel 0 (block column 0, lower part) into LDS
+  for (int e = tid; e < LT_NP * LT_B; e += LT_THREADS) {
+    const int i = e / LT_B, q = e % LT_B;
+    if (i >= q) sh.P[0][i][q] = F[(size_t)i * LT_NP + q];
+  }
+  lt_lds_barrier();
+  // ---- factorisation --------------------------------------------------------
+  // Tile t of the column-by-column list of lower tiles belongs to wave t % 8 in
+  // every step, so a wave reads back only tiles it wrote itself and the steps
+  // need no global-memory fence; panels pass through LDS.
+  for (int k = 0; k < LT_NB; k++) {
+    const int c0 = LT_B * k;
+    const int R = LT_NP - c0;  // panel rows c0 .. LT_NP - 1
+    double (*Pc)[LT_LDW] = sh.P[k & 1];
+    double (*Pn)[LT_LDW] = sh.P[(k + 1) & 1];
+    if (wid == 0) lt_diag_block(sh, k, F);
+    lt_lds_barrier();
+    if (tid >= LT_B && tid < R) {
+      // panel row i below the diagonal block, in column order
+      const int i = tid;
+      double a[LT_B];
+#pragma unroll
+      for (int q = 0; q < LT_B; q++) a[q] = Pc[i][q];
+      double* Frow = F + (size_t)(c0 + i) * LT_NP + c0;
+#pragma unroll
+      for (int j = 0; j < LT_B; j++) {
+        const double l = a[j] * sh.rinv[c0 + j];
+        sh.G[i][j] = -l;
+        Frow[j] = l;                                 // L, row-major (backward pass)
+        F[(size_t)(c0 + j) * LT_NP + c0 + i] = l;    // L^T (forward pass)
+#pragma unroll
+        for (int m = j + 1; m < LT_B; m++) a[m] = fma(-l, Pc[m][j], a[m]);
+      }
+#pragma unroll
+      for (int q = 0; q < LT_B; q++) Pc[i][q] = a[q];
+    }
+    lt_lds_barrier();
+    // trailing lower tiles (I, J), k < J <= I: A_IJ -= L_I W_J^T on the matrix
+    // cores.  MFMA operands: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]
+    // of a 16 x 4 by 4 x 16 product; the accumulator holds element
+    // (l / 16 + 4 v, l % 16) in value v.  LT_GRP tiles at a time, the next
+    // group's loads in flight while the current group's products run; the
+    // tiles of block column k + 1 go to the next panel in LDS, the others back
+    // to F.
+    const int tStart = LT_NB * (k + 1) - k * (k + 1) / 2;  // first tile with J > k
+    const int t0 = tStart + (((wid - tStart) % LT_WAVES) + LT_WAVES) % LT_WAVES;
+    const int ns = t0 < LT_TILES ? (LT_TILES - t0 + LT_WAVES - 1) / LT_WAVES : 0;
+    const int lr = lane >> 4, lc = lane & 15;
+    auto tileAddr = [&](int s) __attribute__((always_inline)) -> size_t {
+      const int t = min(t0 + LT_WAVES * s, LT_TILES - 1);
+      return (size_t)(LT_B * sh.tI[t] + lr) * LT_NP + LT_B * sh.tJ[t] + lc;
+    };
+    v4d cur[LT_GRP], nxt[LT_GRP];
+#pragma unroll
+    for (int g = 0; g < LT_GRP; g++) {
+      const size_t o = tileAddr(g);
+#pragma unroll
+      for (int v = 0; v < 4; v++) cur[g][v] = F[o + (size_t)4 * v * LT_NP];
+    }
+    for (int s0 = 0; s0 < ns; s0 += LT_GRP) {
+#pragma unroll
+      for (int g = 0; g < LT_GRP; g++) {
+        const size_t o = tileAddr(s0 + LT_GRP + g);
+#pragma unroll
+        for (int v = 0; v < 4; v++) nxt[g][v] = F[o + (size_t)4 * v * LT_NP];
+      }
+      int pi[LT_GRP], pj[LT_GRP];
+#pragma unroll
+      for (int g = 0; g < LT_GRP; g++) {
+        const int t = min(t0 + LT_WAVES * (s0 + g), LT_TILES - 1);
+        pi[g] = LT_B * sh.tI[t] - c0 + lc;
+        pj[g] = LT_B * sh.tJ[t] - c0 + lc;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int kk = 4 * c + lr;
+#pragma unroll
+        for (int g = 0; g < LT_GRP; g++)
+          cur[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(sh.G[pi[g]][kk], Pc[pj[g]][kk], cur[g],
+                                                       0, 0, 0);
+      }
+#pragma unroll
+      for (int g = 0; g < LT_GRP; g++) {
+        if (s0 + g < ns) {
+          const int t = t0 + LT_WAVES * (s0 + g);
+          if (sh.tJ[t] == k + 1) {
+            const int pr = LT_B * (sh.tI[t] - k - 1) + lr;
+#pragma unroll
+            for (int v = 0; v < 4; v++) Pn[pr + 4 * v][lc] = cur[g][v];
+          } else {
+            const size_t o = tileAddr(s0 + g);
+#pragma unroll
+            for (int v = 0; v < 4; v++) F[o + (size_t)4 * v * LT_NP] = cur[g][v];
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < LT_GRP; g++) cur[g] = nxt[g];
+    }
+    lt_lds_barrier();
+  }
+  __syncthreads();  // L and D in F are read by other waves from here on
+  // ---- solve and one refinement step --------------------------------------
+  const int r = tid;
+  const double b = r < MATSIZE ? b0[r + 1] : 0.0;
+  double x = lt_substitute(F, sh, b);
+  if (r < LT_NP) sh.x[r] = x;
+  __syncthreads();
+  if (refine) {
+  // r = b - M0 x: 8 lanes per row (8 rows per wave at a time), compensated
+  // (Dot2); the 8 partial sums of a row meet by DPP exchanges within the lane
+  // octet in double-double
+  {
+    const int sub = lane & 7;
+    for (int u0 = wid * 8; u0 < MATSIZE; u0 += LT_WAVES * 8) {
+      const int u = u0 + (lane >> 3);
+      double hi = 0.0, lo = 0.0;
+      if (u < MATSIZE) {
+        const double* row = M0 + (size_t)(u + 1) * MATN + 1;
+        double av[LT_RK];
+#pragma unroll
+        for (int i = 0; i < LT_RK; i++) av[i] = row[min(sub + 8 * i, MATSIZE - 1)];
+        // Dot2 (Ogita, Rump and Oishi, 2005): each product split exactly by
+        // an fma, each addition's error taken exactly by TwoSum, the errors
+        // summed apart
+        double sum = sub == 0 ? b0[u + 1] : 0.0, err = 0.0;
+#pragma unroll
+        for (int i = 0; i < LT_RK; i++) {
+          const int v = sub + 8 * i;
+          if (v < MATSIZE) {
+            const double a = av[i], xv = sh.x[v];
+            const double pr = a * xv;
+            const double pe = fma(a, xv, -pr);  // a xv = pr + pe exactly
+            double t, te;
+            lt_two_sum(sum, -pr, t, te);
+            sum = t;
+            err = err + (te - pe);
+          }
+        }
+        hi = sum;
+        lo = err;
+      }
+      lt_dd_add(hi, lo, lt_dpp<0xB1>(hi), lt_dpp<0xB1>(lo));    // quad_perm 1,0,3,2
+      lt_dd_add(hi, lo, lt_dpp<0x4E>(hi), lt_dpp<0x4E>(lo));    // quad_perm 2,3,0,1
+      lt_dd_add(hi, lo, lt_dpp<0x141>(hi), lt_dpp<0x141>(lo));  // row_half_mirror
+      if (sub == 0 && u < MATSIZE) sh.r[u] = hi + lo;
+    }
+  }
+  __syncthreads();
+  const double dx = lt_substitute(F, sh, r < MATSIZE ? sh.r[r] : 0.0);
+  x = x + dx;
+  if (r < MATSIZE) sh.x[r] = x;
+  __syncthreads();
+  }
+  // 1-based output, the other slots as the system's rhs holds them (as lubksb
+  // leaves them)
+  for (int i = tid; i < NSOL; i += LT_THREADS) {
+    const double val = (i >= 1 && i <= MATSIZE) ? sh.x[i - 1] : b0[i];
+    S.rhs[i] = val;
+    if (S.out) S.out[i] = val;
+  }
+}
+
+__global__ __launch_bounds__(LT_THREADS) void k_ldlt_solve(LuSys S, int refine) {
+  ldlt_solve_dev(S, refine);
+}
+
+__global__ __launch_bounds__(LT_THREADS) void k_ldlt_solve_b(LuBatch B, int refine) {
+  ldlt_solve_dev(B.s[blockIdx.x], refine);
+}
+
+static hipError_t lt_attr() {
+  static const hipError_t rc1 = hipFuncSetAttribute(
+      (const void*)k_ldlt_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LtShared));
+  static const hipError_t rc2 = hipFuncSetAttribute(
+      (const void*)k_ldlt_solve_b, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)sizeof(LtShared));
+  return rc1 != hipSuccess ? rc1 : rc2;
+}
+
+hipError_t launch_ldlt_solve(hipStream_t s, const LuSys& S, bool refine) {
+  const hipError_t e = lt_attr();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ldlt_solve, dim3(1), dim3(LT_THREADS), sizeof(LtShared), s, S,
+                     refine ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_ldlt_solve_batch(hipStream_t s, const LuBatch& B, bool refine) {
+  if (B.n <= 0) return hipSuccess;
+  if (B.n > LU_BATCH_MAX) return hipErrorInvalidValue;
+  const hipError_t e = lt_attr();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ldlt_solve_b, dim3(B.n), dim3(LT_THREADS), sizeof(LtShared), s, B,
+                     refine ? 1 : 0);
+  return hipGetLastError();
+}
+
+}  // namespace xb

@@ -973,7 +973,8 @@ static hipError_t enqueue_refine(hipStream_t s, const LuSet& L, double* d_out) {
 
 // The global solves of several systems (of different pairs) as one batch on
 // stream s: the same 23 launches as one solve, each covering every system.
-hipError_t launch_global_solve_batch(hipStream_t s, const LuBatch& B) {
+hipError_t launch_global_solve_batch(hipStream_t s, const LuBatch& B, bool ldlt, bool refine) {
+  if (ldlt) return launch_ldlt_solve_batch(s, B, refine);
   if (B.n <= 0) return hipSuccess;
   if (B.n > LU_BATCH_MAX) return hipErrorInvalidValue;
   const int nb = B.n;
@@ -1012,6 +1013,7 @@ LuSys lu_sys(const LuSet& L, double* d_out, const int* d_skipIfZero) {
   S.rhs0 = L.rhs0;
   S.out = d_out;
   S.skip = d_skipIfZero;
+  S.F = L.F;
   return S;
 }
 
@@ -1021,6 +1023,12 @@ LuSys lu_sys(const LuSet& L, double* d_out, const int* d_skipIfZero) {
 hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int* d_skipIfZero) {
   prof_begin(c, XB_PROF_SOLVE);
   hipError_t e = hipSuccess;
+  if (c.form & XB_FORM_LDLT) {
+    // one launch that factors, solves (and refines): nothing to capture
+    e = launch_ldlt_solve(c.cur, lu_sys(L, d_out, d_skipIfZero), (c.form & XB_FORM_REFINE) != 0);
+    prof_end(c, XB_PROF_SOLVE);
+    return e;
+  }
   if (c.eagerSolve) {
     e = enqueue_solve(c.cur, L, d_out, d_skipIfZero);
   } else {

@@ -332,6 +332,7 @@ int ensureSolver(Ctx& c) {
     XB_HIP(locked_malloc((void**)&L.luVV, sizeof(double) * MATN));
     XB_HIP(locked_malloc((void**)&L.luState, LU_STATE_BYTES));
     XB_HIP(locked_malloc((void**)&L.rr, sizeof(double) * NSOL));
+    XB_HIP(locked_malloc((void**)&L.F, sizeof(double) * LDLT_WORK_DOUBLES));
   }
   XB_HIP(locked_malloc((void**)&c.cdSol, sizeof(double) * 2 * NSOL));
   XB_HIP(locked_malloc((void**)&c.ksSol, sizeof(double) * 2 * NSOL));
@@ -938,6 +939,7 @@ void freeAll(Ctx& c) {
     LuSet& L = i < 2 ? c.lu[i] : c.ksLu[i - 2];
     dfree(&L.M); dfree(&L.rhs); dfree(&L.M0); dfree(&L.rhs0);
     dfree(&L.luIndex); dfree(&L.luVV); dfree(&L.luState); dfree(&L.rr);
+    dfree(&L.F);
   }
   dfree(&c.sol); dfree(&c.cdSol); dfree(&c.ksSol);
   dfree(&c.tileK64); dfree(&c.tileAks);
@@ -1287,7 +1289,7 @@ int xb_init_device(xb_ctx* x, const xb_params* p, int w, int h, const void* d_tm
 
 int xb_set_form(xb_ctx* x, int form) {
   if (!x || (form & ~(XB_FORM_MIXED | XB_FORM_SCREEN | XB_FORM_RECORD | XB_FORM_FP16 |
-                      XB_FORM_REFINE)))
+                      XB_FORM_REFINE | XB_FORM_LDLT)))
     return XB_ERR_ARG;
   x->c.form = form;
   return XB_OK;

@@ -181,11 +181,13 @@ int lsSolve(xb_batch* b, const std::vector<int>& slots, const xb::LuBatch& B,
       return XB_ERR_HIP;
     }
   }
-  if ((e = xb::launch_global_solve_batch(b->sstream, B)) != hipSuccess) {
+  const bool ldlt = (b->form & XB_FORM_LDLT) != 0;
+  if ((e = xb::launch_global_solve_batch(b->sstream, B, ldlt,
+                                         (b->form & XB_FORM_REFINE) != 0)) != hipSuccess) {
     err = std::string("batched solve: ") + hipGetErrorString(e);
     return XB_ERR_HIP;
   }
-  if (b->form & XB_FORM_REFINE) {
+  if ((b->form & XB_FORM_REFINE) && !ldlt) {
     // each system's refinement on the solve stream: the entry's system set is
     // found among its pair's four (two trials, two clip-refit directions)
     for (int k = 0; k < B.n; k++) {

@@ -63,6 +63,8 @@ constexpr int NSOL = NPSF * NC2 + NBG + 1;  // 298 solution slots
 // panel (19 for 292 unknowns)
 constexpr int LU_PANELS = (MATSIZE + 15) / 16;
 constexpr size_t LU_STATE_BYTES = 64 + sizeof(int) * MATN * (1 + LU_PANELS);
+// XB_FORM_LDLT's factor workspace: the system padded to 19 blocks of 16, 0-based
+constexpr int LDLT_WORK_DOUBLES = (LU_PANELS * 16) * (LU_PANELS * 16);
 constexpr int QN = NPSF + 2;              // 51: per-stamp test system (1-based)
 constexpr int MAXSS = 6;                  // 2 * maxKSStamps
 constexpr int KEEPSS = 3;                 // maxKSStamps
@@ -150,6 +152,7 @@ struct LuSet {
   void* luState = nullptr;      // LU bookkeeping (zero row, last pivot, ii, zero flags)
   double* luVV = nullptr;       // [MATN]
   double* rr = nullptr;         // [NSOL] iterative refinement: residual, then correction
+  double* F = nullptr;          // [LDLT_WORK_DOUBLES] L D L^T factors (XB_FORM_LDLT)
 };
 
 // The learned substamp screen's per-pass buffers and model (the first
@@ -301,6 +304,7 @@ struct LuSys {
   const double* rhs0;
   double* out;
   const int* skip;
+  double* F;
 };
 constexpr int LU_BATCH_MAX = 32;  // systems per batched launch (kernel argument space)
 struct LuBatch {
@@ -308,7 +312,14 @@ struct LuBatch {
   int n;
 };
 LuSys lu_sys(const LuSet& L, double* d_out, const int* d_skipIfZero);
-hipError_t launch_global_solve_batch(hipStream_t s, const LuBatch& B);
+// ldlt: the systems are solved by launch_ldlt_solve_batch (XB_FORM_LDLT),
+// refined once when refine is set
+hipError_t launch_global_solve_batch(hipStream_t s, const LuBatch& B, bool ldlt = false,
+                                     bool refine = false);
+// XB_FORM_LDLT (k_ldlt.hip): L D L^T without pivoting, with refine one step of
+// FP64 iterative refinement; one workgroup and one launch per system
+hipError_t launch_ldlt_solve(hipStream_t s, const LuSys& S, bool refine);
+hipError_t launch_ldlt_solve_batch(hipStream_t s, const LuBatch& B, bool refine);
 // Solves system set L.  d_skipIfZero: when it points at a zero count the solve
 // is skipped and d_out is zeroed (the reference returns before solving an
 // empty fit).

@@ -394,6 +394,7 @@ CTX_EAGER = 2  # global solves launched kernel by kernel instead of replayed hip
 BATCH_LOCKSTEP = 4  # xb_batch_create: groups of pairs in lockstep, one batched solve per solve point
 FORM_PLAIN, FORM_MIXED, FORM_SCREEN, FORM_RECORD = 0, 1, 2, 4
 FORM_FP16, FORM_REFINE = 8, 16  # with FORM_MIXED: FP16 split W; FP64 iterative refinement
+FORM_LDLT = 32  # global solves by L D L^T + one FP64 refinement step (not the reference's order)
 # the screen's network (include/xbach.h XB_MLP_*)
 MLP_NFEAT, MLP_NPARAM, MLP_NGRAD = 32, 2242, 2177
 MLP_NOALLOC_MAX = 16384  # XB_MLP_NOALLOC_MAX
