@@ -64,6 +64,7 @@ CONV_TRAFFIC_BYTES = 163_815_488
 LU_TRAFFIC_BYTES = 23_401_549
 LU_N = 292                                   # unknowns of the global system
 LU_FLOPS = 2.0 / 3.0 * LU_N ** 3             # ludcmp's multiply-adds, as flops
+LDLT_FLOPS = LU_N ** 3 / 3.0 + 8.0 * LU_N ** 2  # L D L^T, substitutions, one refinement
 
 WORKLOADS = {
     "pair2048": ("2048x2048 FP32 pair, 6000 stars (dN/dF ~ F^-1.5, 1e2-1e5), template FWHM 1.8 px, "
@@ -357,6 +358,16 @@ def main() -> int:
     ctx.buf_download(diff, d_diff)
     rms = residual(diff)
 
+    # the fast form's global solve (L D L^T, one refinement step) timed the
+    # same way, one pair at a time with the mixed cmv
+    ctx.set_form(xbach.FORM_MIXED | xbach.FORM_LDLT | xbach.FORM_REFINE)
+    mixed1_ms, mprof = run_plain(ctx, params, w, h, d_t, d_s, max(5, args.steps // 2),
+                                 max(1, args.warmup // 2))
+    ctx.set_form(xbach.FORM_PLAIN)
+    ldlt_ms, ldlt_n = mprof["solve"]
+    ldlt_avg_s = ldlt_ms / max(ldlt_n, 1) * 1e-3
+    progress(f"mixed+ldlt single pair {1e3 / mixed1_ms:.1f} diffs/s, solve {ldlt_avg_s * 1e6:.1f} us")
+
     # roofline of the convolution kernel (the data-parallel hot kernel), from
     # the plain run's HIP events on the stream k_conv runs on: the FP64 441-tap
     # stencil at 882 flop per output pixel is compute-bound (about 70 flop per
@@ -403,6 +414,10 @@ def main() -> int:
 
     forms = {"plain": {"value": plain_value, "ms_per_diff": plain_ms, "steps": args.steps,
                        "warmup": args.warmup, "residual_rms": rms},
+             "mixed_ldlt_single_pair": {"value": 1e3 / mixed1_ms, "ms_per_diff": mixed1_ms,
+                                        "solve_us": ldlt_avg_s * 1e6,
+                                        "note": "one pair at a time, mixed cmv, L D L^T "
+                                                "solves with one refinement step"},
              "plain_host_buffers": {
                  "value": 1e3 / host_ms, "ms_per_diff": host_ms, "steps": nh,
                  "note": "PCIe-inclusive, not the headline value",
@@ -450,13 +465,28 @@ def main() -> int:
             "all_pairs_ok": all(r.status == 0 for r in lres),
             "bit_identical_to_batched": all(same_bits(a, b) for a, b in zip(dls, dplain)),
             "speedup_vs_batched": l_value / b_value}
-        # mixed precision + batched + learned screen
+        # mixed precision + batched + learned screen: cmv on the matrix cores,
+        # the global solves by L D L^T with one FP64 refinement step, the screen
         model = np.load(screen.MODEL_PATH) if screen.MODEL_PATH.exists() else None
-        fform = xbach.FORM_MIXED | (xbach.FORM_SCREEN if model is not None else 0)
+        scr = xbach.FORM_SCREEN if model is not None else 0
+        fform = xbach.FORM_MIXED | xbach.FORM_LDLT | xbach.FORM_REFINE | scr
         f_ms, fres, dfull, ftab = run_batched(ctx, params, pairs, bufs, args.workers,
                                               args.steps, args.warmup, form=fform, model=model,
                                               keep=keep, tune=not args.no_tune,
                                               flags=ctx_flags)
+        # the same with the global solves in Numerical Recipes' order (ludcmp +
+        # lubksb, the plain form's solve)
+        n_ms, nres, dnr, ntab = run_batched(ctx, params, pairs, bufs, args.workers,
+                                            args.steps, args.warmup,
+                                            form=xbach.FORM_MIXED | scr, model=model, keep=keep,
+                                            tune=not args.no_tune, flags=ctx_flags)
+        n_value = args.batch * 1e3 / n_ms
+        progress(f"mixed+batched+screened, ludcmp solves {n_value:.1f} diffs/s")
+        forms["mixed_batched_screened_ludcmp"] = {
+            "value": n_value, "ms_per_batch": n_ms, "pairs_per_batch": args.batch,
+            "pairs_in_flight_per_bucket": ntab, "steps": args.steps, "warmup": args.warmup,
+            "all_pairs_ok": all(r.status == 0 for r in nres),
+            "speedup_vs_batched_plain": n_value / b_value}
         for q in bufs:
             for d in q:
                 ctx.buf_free(d)
@@ -467,8 +497,8 @@ def main() -> int:
         rel = [abs(residual(a) - residual(b)) / residual(b) for a, b in zip(dfull, dplain)]
         full = {
             "value": f_value, "ms_per_batch": f_ms, "pairs_per_batch": args.batch,
-            "workers": args.workers, "pairs_in_flight_per_bucket": ftab, "form": "mixed" + ("+screened" if model is not None
-                                                              else ""),
+            "workers": args.workers, "pairs_in_flight_per_bucket": ftab,
+            "form": "mixed+ldlt+refine" + ("+screened" if model is not None else ""),
             "steps": args.steps, "warmup": args.warmup, "all_pairs_ok": fok,
             "residual_rms_rel_vs_plain_max": float(max(rel)), "pairs_checked": len(rel),
             "clip_refit_passes_mean": float(np.mean([r.kscIterations for r in fres])),
@@ -522,6 +552,19 @@ def main() -> int:
                      "us_per_solve": solve_avg_s * 1e6, "us_per_column": solve_avg_s * 1e6 / LU_N,
                      "limiter": "latency: 292 dependent pivot columns and a 42,486-step "
                                 "ordered back substitution (bit-identical to ludcmp/lubksb)"},
+        # the fast forms' global solve: L D L^T (n^3 / 3 flop, the trailing
+        # updates on the FP64 matrix cores) plus the substitutions and one
+        # refinement step (8 n^2 flop), one launch per system
+        "roofline_ldlt": {"bound": "mfma", "compute_unit": "fp64 mfma",
+                          "kernel": "k_ldlt_solve (L D L^T + substitution + one refinement "
+                                    "step, one workgroup)",
+                          "achieved": LDLT_FLOPS / ldlt_avg_s / 1e12, "peak": PEAK_FP64_TFLOPS,
+                          "unit": "TFLOP/s",
+                          "frac": LDLT_FLOPS / ldlt_avg_s / 1e12 / PEAK_FP64_TFLOPS,
+                          "traffic": None,
+                          "algorithmic": f"292^3/3 + 8*292^2 = {LDLT_FLOPS:.4g} flop per solve",
+                          "us_per_solve": ldlt_avg_s * 1e6,
+                          "us_per_solve_ludcmp": solve_avg_s * 1e6},
         # the data-parallel stencil: FP64 VALU multiply-adds (unfused, in the
         # reference's term order)
         "roofline_conv": {"bound": "fp64 valu", "kernel": "k_conv",

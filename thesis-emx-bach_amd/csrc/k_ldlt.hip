@@ -65,6 +65,7 @@ struct LtShared {
   double d[LT_NP];             // D
   double rinv[LT_NP];          // 1 / D
   double y[LT_NP];             // block values published by the substitutions
+  double zb[LT_NP];            // the bordered row: b, then z = D^-1 L^-1 b
   double x[LT_NP];             // the solution (0-based)
   double r[LT_NP];             // the refinement's residual
   int nz[LT_NP];               // row has a nonzero finite entry
@@ -120,12 +121,14 @@ __device__ __forceinline__ void lt_lds_barrier() {
 // chain of broadcast multiply-adds without memory traffic.  Each thread keeps
 // the factors of the next two blocks in flight (a ring of three), and the
 // barriers wait for LDS traffic only.
+template <bool FORWARD>
 __device__ __forceinline__ double lt_substitute(const double* __restrict__ F, LtShared& sh,
                                                 double v) {
   const int r = threadIdx.x;
   const bool act = r < LT_NP;
   const int kr = r / LT_B, ir = r % LT_B;
   double R[3][LT_B];
+  if constexpr (FORWARD) {
   // forward: L y = b.  Row r needs L[r][16k .. 16k + 15] for blocks k <= kr,
   // read from the transposed copy above the diagonal (F[16k + j][r]), so that
   // neighbouring rows read neighbouring addresses.
@@ -178,6 +181,7 @@ __device__ __forceinline__ double lt_substitute(const double* __restrict__ F, Lt
   }
   if (act) v = v * sh.rinv[r];
   lt_lds_barrier();  // the forward pass's reads of sh.y are done
+  }
   // backward: L^T x = z.  Row r needs L[16k .. 16k + 15][r] for blocks k >= kr
   // (below the diagonal, row-major).
   const int wFirst = r & ~(WAVE - 1);  // the wave's first row
@@ -251,73 +255,85 @@ __device__ void lt_zero_row_path(const double* __restrict__ M0, double* xs) {
   }
 }
 
-// lane K's value of x within each row of 16 lanes (64-bit DPP broadcast; the
-// s_nop covers a DPP read of a VGPR the previous instruction wrote)
-#define LT_BCAST(OUT, X, K)                                                           \
-  {                                                                                  \
-    double z_ = 0.0;                                                                 \
-    __asm__ __volatile__("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3"          \
-                         " row_mask:0xf bank_mask:0xf"                               \
-                         : "+v"(z_)                                                  \
-                         : "v"(X), "v"(1.0), "i"(K));                                \
-    OUT = z_;                                                                        \
-  }
+// Lane K's x within each row of 16 lanes (64-bit DPP broadcast).  The s_nop
+// covers a DPP read of a register the previous instruction wrote.
+template <int K>
+__device__ __forceinline__ double lt_bcast(double x) {
+  double y;
+  __asm__ __volatile__(
+      "s_nop 1\n\tv_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf"
+      : "=v"(y)
+      : "v"(x), "i"(K));
+  return y;
+}
 
-// The diagonal block of panel k (rows 0..15 of P) as L D L^T, in registers of
-// wave 0: lane i holds row i & 15 (its entries right of the diagonal taken
-// from the lower triangle by symmetry, then kept up to date as the matrix
-// stays symmetric).  Column j: d_j and row j's entries right of j reach every
-// lane of the DPP row from lane j; each row takes l_ij = a_ij / d_j and
-// updates its entries right of j.  Stores W (unscaled, below the diagonal),
-// d and 1/d, and L and D into F (lower part, and L transposed into the upper).
+// 1 / d: the hardware reciprocal and two Newton steps
+__device__ __forceinline__ double lt_recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return r;
+}
+
+// Column J of the diagonal block's L D L^T, in registers of wave 0 (lane i
+// holds row i & 15, its entries right of the diagonal taken from the lower
+// triangle by symmetry and kept up to date as the matrix stays symmetric).
+// d_J and 1/d_J are in dj, ri.  Row J's entries right of J (by symmetry column
+// J) reach every lane from lane J; the next column's entry is updated first
+// and its pivot and reciprocal started, then the rest of the row.
 template <int J>
-__device__ __forceinline__ void lt_diag_col(double (&a)[LT_B], double& dj, double& ri) {
-  double dd;
-  LT_BCAST(dd, a[J], J)
-  if (dd == 0.0) dd = LT_TINY;
-  dj = dd;
-  ri = 1.0 / dd;
+__device__ __forceinline__ void lt_diag_col(double (&a)[LT_B], double& dj, double& ri,
+                                            double* sd, double* sr, int lane) {
+  if (lane == 0) {
+    sd[J] = dj;
+    sr[J] = ri;
+  }
   const double l = a[J] * ri;
+  if constexpr (J + 1 < LT_B) {
+    a[J + 1] = fma(-l, lt_bcast<J>(a[J + 1]), a[J + 1]);
+    double dn = lt_bcast<J + 1>(a[J + 1]);
+    if (dn == 0.0) dn = LT_TINY;
+    const double rn = lt_recip(dn);
 #pragma unroll
-  for (int q = J + 1; q < LT_B; q++) {
-    double cq;
-    LT_BCAST(cq, a[q], J)
-    a[q] = fma(-l, cq, a[q]);
+    for (int q = J + 2; q < LT_B; q++) a[q] = fma(-l, lt_bcast<J>(a[q]), a[q]);
+    dj = dn;
+    ri = rn;
   }
 }
 
-__device__ __forceinline__ void lt_diag_block(LtShared& sh, int k, double* __restrict__ F) {
-  const int lane = lane_id(), i = lane & (LT_B - 1), c0 = LT_B * k;
-  double (*P)[LT_LDW] = sh.P[k & 1];
-  double a[LT_B], dv[LT_B], rv[LT_B];
+// The diagonal block of the panel (its rows 0..15) as L D L^T by wave 0.
+// Leaves W (unscaled) below the diagonal in the panel and D, 1/D in sd, sr.
+__device__ __forceinline__ void lt_diag_block(double (*P)[LT_LDW], double* sd, double* sr,
+                                              int lane) {
+  const int i = lane & (LT_B - 1);
+  double a[LT_B];
 #pragma unroll
   for (int q = 0; q < LT_B; q++) a[q] = q <= i ? P[i][q] : P[q][i];
-  lt_diag_col<0>(a, dv[0], rv[0]);   lt_diag_col<1>(a, dv[1], rv[1]);
-  lt_diag_col<2>(a, dv[2], rv[2]);   lt_diag_col<3>(a, dv[3], rv[3]);
-  lt_diag_col<4>(a, dv[4], rv[4]);   lt_diag_col<5>(a, dv[5], rv[5]);
-  lt_diag_col<6>(a, dv[6], rv[6]);   lt_diag_col<7>(a, dv[7], rv[7]);
-  lt_diag_col<8>(a, dv[8], rv[8]);   lt_diag_col<9>(a, dv[9], rv[9]);
-  lt_diag_col<10>(a, dv[10], rv[10]); lt_diag_col<11>(a, dv[11], rv[11]);
-  lt_diag_col<12>(a, dv[12], rv[12]); lt_diag_col<13>(a, dv[13], rv[13]);
-  lt_diag_col<14>(a, dv[14], rv[14]); lt_diag_col<15>(a, dv[15], rv[15]);
+  double dj = lt_bcast<0>(a[0]);
+  if (dj == 0.0) dj = LT_TINY;
+  double ri = lt_recip(dj);
+  lt_diag_col<0>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<1>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<2>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<3>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<4>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<5>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<6>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<7>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<8>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<9>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<10>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<11>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<12>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<13>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<14>(a, dj, ri, sd, sr, lane);
+  lt_diag_col<15>(a, dj, ri, sd, sr, lane);
   if (lane < LT_B) {
-    sh.d[c0 + i] = dv[i];
-    sh.rinv[c0 + i] = rv[i];
-    double* Frow = F + (size_t)(c0 + i) * LT_NP + c0;
 #pragma unroll
-    for (int q = 0; q < LT_B; q++) {
-      if (q < i) {
-        P[i][q] = a[q];
-        const double l = a[q] * rv[q];
-        Frow[q] = l;                                  // L, row-major (backward pass)
-        F[(size_t)(c0 + q) * LT_NP + c0 + i] = l;     // L^T (forward pass)
-      } else if (q == i) {
-        Frow[q] = dv[q];
-      }
-    }
+    for (int q = 0; q < LT_B; q++)
+      if (q < i) P[i][q] = a[q];
   }
 }
-#undef LT_BCAST
 
 __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
   extern __shared__ double lt_dyn[];
@@ -333,7 +349,10 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
     return;
   }
   // ---- copy-in, the zero-row check, the tile list -------------------------
-  for (int u = tid; u < LT_NP; u += LT_THREADS) sh.nz[u] = 0;
+  for (int u = tid; u < LT_NP; u += LT_THREADS) {
+    sh.nz[u] = 0;
+    sh.zb[u] = u < MATSIZE ? b0[u + 1] : 0.0;
+  }
   if (tid == 0) sh.zero = 0;
   if (tid < LT_TILES) {
     int J = 0, rem = tid;
@@ -405,28 +424,60 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
     const int R = LT_NP - c0;  // panel rows c0 .. LT_NP - 1
     double (*Pc)[LT_LDW] = sh.P[k & 1];
     double (*Pn)[LT_LDW] = sh.P[(k + 1) & 1];
-    if (wid == 0) lt_diag_block(sh, k, F);
+    if (wid == 0) lt_diag_block(Pc, sh.d + c0, sh.rinv + c0, lane);
     lt_lds_barrier();
-    if (tid >= LT_B && tid < R) {
-      // panel row i below the diagonal block, in column order
+    if (tid >= LT_B && tid <= R) {
+      // panel row i below the diagonal block, or (i = R) the bordered row, the
+      // right-hand side as an extra row of the matrix: its L part is
+      // z = D^-1 L^-1 b, so the forward substitution is done here.  In column
+      // order: w, and -l into G (z for the bordered row).
       const int i = tid;
+      const bool brow = i == R;
       double a[LT_B];
 #pragma unroll
-      for (int q = 0; q < LT_B; q++) a[q] = Pc[i][q];
-      double* Frow = F + (size_t)(c0 + i) * LT_NP + c0;
+      for (int q = 0; q < LT_B; q++) a[q] = brow ? sh.zb[c0 + q] : Pc[i][q];
+      double lv[LT_B];
 #pragma unroll
       for (int j = 0; j < LT_B; j++) {
         const double l = a[j] * sh.rinv[c0 + j];
-        sh.G[i][j] = -l;
-        Frow[j] = l;                                 // L, row-major (backward pass)
-        F[(size_t)(c0 + j) * LT_NP + c0 + i] = l;    // L^T (forward pass)
+        lv[j] = l;
 #pragma unroll
         for (int m = j + 1; m < LT_B; m++) a[m] = fma(-l, Pc[m][j], a[m]);
       }
+      if (brow) {
 #pragma unroll
-      for (int q = 0; q < LT_B; q++) Pc[i][q] = a[q];
+        for (int q = 0; q < LT_B; q++) sh.zb[c0 + q] = lv[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < LT_B; q++) {
+          Pc[i][q] = a[q];
+          sh.G[i][q] = -lv[q];
+        }
+      }
     }
     lt_lds_barrier();
+    // the panel's L (and D on the diagonal) into F by all threads, neighbouring
+    // threads on neighbouring addresses: row-major below the diagonal (the
+    // backward pass), and transposed above it (the forward pass)
+    for (int e = tid; e < R * LT_B; e += LT_THREADS) {
+      const int i = e / LT_B, q = e % LT_B;
+      if (i >= q)
+        F[(size_t)(c0 + i) * LT_NP + c0 + q] =
+            i == q ? sh.d[c0 + q] : (i < LT_B ? Pc[i][q] * sh.rinv[c0 + q] : -sh.G[i][q]);
+    }
+    for (int e = tid; e < R * LT_B; e += LT_THREADS) {
+      const int q = e / R, i = e % R;
+      if (i > q)
+        F[(size_t)(c0 + q) * LT_NP + c0 + i] =
+            i < LT_B ? Pc[i][q] * sh.rinv[c0 + q] : -sh.G[i][q];
+    }
+    // the bordered row right of the panel: zb_c -= sum_j z_j W_cj
+    for (int c = c0 + LT_B + tid; c < LT_NP; c += LT_THREADS) {
+      double acc = sh.zb[c];
+#pragma unroll
+      for (int j = 0; j < LT_B; j++) acc = fma(-sh.zb[c0 + j], Pc[c - c0][j], acc);
+      sh.zb[c] = acc;
+    }
     // trailing lower tiles (I, J), k < J <= I: A_IJ -= L_I W_J^T on the matrix
     // cores.  MFMA operands: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]
     // of a 16 x 4 by 4 x 16 product; the accumulator holds element
@@ -494,8 +545,8 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
   __syncthreads();  // L and D in F are read by other waves from here on
   // ---- solve and one refinement step --------------------------------------
   const int r = tid;
-  const double b = r < MATSIZE ? b0[r + 1] : 0.0;
-  double x = lt_substitute(F, sh, b);
+  // the forward substitution and D^-1 were done with the bordered row
+  double x = lt_substitute<false>(F, sh, r < LT_NP ? sh.zb[r] : 0.0);
   if (r < LT_NP) sh.x[r] = x;
   __syncthreads();
   if (refine) {
@@ -539,7 +590,7 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
     }
   }
   __syncthreads();
-  const double dx = lt_substitute(F, sh, r < MATSIZE ? sh.r[r] : 0.0);
+  const double dx = lt_substitute<true>(F, sh, r < MATSIZE ? sh.r[r] : 0.0);
   x = x + dx;
   if (r < MATSIZE) sh.x[r] = x;
   __syncthreads();
