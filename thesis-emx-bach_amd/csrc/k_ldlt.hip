@@ -45,17 +45,18 @@ namespace xb {
 constexpr int LT_B = 16;                            // block order (one MFMA tile)
 constexpr int LT_NB = (MATSIZE + LT_B - 1) / LT_B;  // 19 blocks
 constexpr int LT_NP = LT_NB * LT_B;                 // 304: padded order
-constexpr int LT_THREADS = 512;
+constexpr int LT_THREADS = 512;                    // eight tile waves
 constexpr int LT_WAVES = LT_THREADS / WAVE;
+constexpr int LT_BLOCK = LT_THREADS + WAVE;         // and the panel wave
+constexpr int LT_SPIN_MAX = 1 << 22;               // bound of the panel wave's wait
 constexpr int LT_LDW = LT_B + 1;  // LDS pitch of a panel row (doubles)
 constexpr int LT_GRP = 4;         // trailing tiles per load group
 constexpr int LT_TILES = LT_NB * (LT_NB + 1) / 2;  // 190 lower tiles
-constexpr int LT_CB = 16;         // copy-in elements per thread in flight
 constexpr int LT_RK = (MATSIZE + 7) / 8;  // residual terms per lane of a row
 constexpr double LT_TINY = 1.0e-20;  // an exact zero pivot, as ludcmp's TINY
 static_assert(LT_NP <= LT_THREADS, "one thread per row in the substitutions");
 static_assert(LT_TILES <= LT_THREADS, "one thread per tile entry");
-static_assert(LT_NP * LT_NP == LDLT_WORK_DOUBLES, "workspace size");
+static_assert(LT_NP * LT_NP + LT_TILES * LT_B * LT_B == LDLT_WORK_DOUBLES, "workspace size");
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -71,6 +72,8 @@ struct LtShared {
   int nz[LT_NP];               // row has a nonzero finite entry
   unsigned char tI[LT_TILES], tJ[LT_TILES];  // lower tiles, column by column
   int zero;
+  int dflag;    // the next panel's diagonal block is in place (its step number)
+  int spinFail;  // the panel wave's wait ran out (never expected)
 };
 
 __device__ __forceinline__ double lt_readlane(double v, int l) {
@@ -342,6 +345,7 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
   const double* __restrict__ M0 = S.M0;
   const double* __restrict__ b0 = S.rhs0;
   double* __restrict__ F = S.F;
+  double* __restrict__ Ft = S.F + (size_t)LT_NP * LT_NP;  // the tile store
   if (S.skip && *S.skip == 0) {
     // testFit returns 666 before solving (src/cdkscUtil.cpp:101-103)
     for (int i = tid; i < NSOL; i += LT_THREADS)
@@ -353,7 +357,11 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
     sh.nz[u] = 0;
     sh.zb[u] = u < MATSIZE ? b0[u + 1] : 0.0;
   }
-  if (tid == 0) sh.zero = 0;
+  if (tid == 0) {
+    sh.zero = 0;
+    sh.dflag = -1;
+    sh.spinFail = 0;
+  }
   if (tid < LT_TILES) {
     int J = 0, rem = tid;
     while (rem >= LT_NB - J) {
@@ -365,33 +373,33 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
   }
   __syncthreads();
   {
-    // the padded lower triangle, row by row, LT_CB elements per thread in
-    // flight at a time
-    constexpr int total = LT_NP * (LT_NP + 1) / 2;
-    for (int e0 = tid; e0 < total; e0 += LT_THREADS * LT_CB) {
-      double val[LT_CB];
-      int uu[LT_CB], vv[LT_CB];
+    // Every lower tile into the tile store Ft (each lane's accumulator values
+    // v = 2h, 2h + 1 at Ft[((2 t + h) 64 + lane) 2]: a tile is two contiguous
+    // 1 KB halves), block column 0 also into the first panel.  A wave takes one
+    // tile per round: lane l gets rows l / 16 + 4 v, column l % 16 of it.
+    const int lr = lane >> 4, lc = lane & 15;
+    for (int t = wid; t < LT_TILES && wid < LT_WAVES; t += LT_WAVES) {
+      const int I = sh.tI[t], J = sh.tJ[t];
+      double val[4];
 #pragma unroll
-      for (int i = 0; i < LT_CB; i++) {
-        const int e = min(e0 + LT_THREADS * i, total - 1);
-        int u = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
-        if ((u + 1) * (u + 2) / 2 <= e) u++;
-        if (u * (u + 1) / 2 > e) u--;
-        uu[i] = u;
-        vv[i] = e - u * (u + 1) / 2;
-        const double m = M0[(size_t)(min(u, MATSIZE - 1) + 1) * MATN + min(vv[i], MATSIZE - 1) + 1];
-        val[i] = u < MATSIZE ? m : (u == vv[i] ? 1.0 : 0.0);
+      for (int v = 0; v < 4; v++) {
+        const int row = LT_B * I + lr + 4 * v, col = LT_B * J + lc;
+        const double m =
+            M0[(size_t)(min(row, MATSIZE - 1) + 1) * MATN + min(col, MATSIZE - 1) + 1];
+        val[v] = (row < MATSIZE && col < MATSIZE) ? m : (row == col ? 1.0 : 0.0);
       }
 #pragma unroll
-      for (int i = 0; i < LT_CB; i++) {
-        if (e0 + LT_THREADS * i < total) {
-          if (uu[i] < MATSIZE && fabs(val[i]) > 0.0) {  // NaN is not taken, as ludcmp's row maximum
-            sh.nz[uu[i]] = 1;
-            sh.nz[vv[i]] = 1;
-          }
-          F[(size_t)uu[i] * LT_NP + vv[i]] = val[i];
+      for (int v = 0; v < 4; v++) {
+        const int row = LT_B * I + lr + 4 * v, col = LT_B * J + lc;
+        if (row < MATSIZE && col < MATSIZE && fabs(val[v]) > 0.0) {
+          sh.nz[row] = 1;  // NaN is not taken, as ludcmp's row maximum
+          sh.nz[col] = 1;
         }
+        if (J == 0) sh.P[0][row][lc] = val[v];
       }
+      double2* d = reinterpret_cast<double2*>(Ft) + (size_t)2 * t * WAVE + lane;
+      d[0] = make_double2(val[0], val[1]);
+      d[WAVE] = make_double2(val[2], val[3]);
     }
   }
   __syncthreads();
@@ -409,13 +417,12 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
     }
     return;
   }
-  // panel 0 (block column 0, lower part) into LDS
-  for (int e = tid; e < LT_NP * LT_B; e += LT_THREADS) {
-    const int i = e / LT_B, q = e % LT_B;
-    if (i >= q) sh.P[0][i][q] = F[(size_t)i * LT_NP + q];
-  }
-  lt_lds_barrier();
   // ---- factorisation --------------------------------------------------------
+  // The panel wave factors the diagonal block of panel 0 here and that of
+  // panel k + 1 during step k, as soon as the owner of tile (k + 1, k + 1) has
+  // left it in the next panel (an LDS flag), beside the trailing products.
+  if (wid == LT_WAVES) lt_diag_block(sh.P[0], sh.d, sh.rinv, lane);
+  lt_lds_barrier();
   // Tile t of the column-by-column list of lower tiles belongs to wave t % 8 in
   // every step, so a wave reads back only tiles it wrote itself and the steps
   // need no global-memory fence; panels pass through LDS.
@@ -424,8 +431,6 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
     const int R = LT_NP - c0;  // panel rows c0 .. LT_NP - 1
     double (*Pc)[LT_LDW] = sh.P[k & 1];
     double (*Pn)[LT_LDW] = sh.P[(k + 1) & 1];
-    if (wid == 0) lt_diag_block(Pc, sh.d + c0, sh.rinv + c0, lane);
-    lt_lds_barrier();
     if (tid >= LT_B && tid <= R) {
       // panel row i below the diagonal block, or (i = R) the bordered row, the
       // right-hand side as an extra row of the matrix: its L part is
@@ -458,18 +463,28 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
     lt_lds_barrier();
     // the panel's L (and D on the diagonal) into F by all threads, neighbouring
     // threads on neighbouring addresses: row-major below the diagonal (the
-    // backward pass), and transposed above it (the forward pass)
-    for (int e = tid; e < R * LT_B; e += LT_THREADS) {
+    // backward pass), and transposed above it (the forward pass); the rows
+    // below the diagonal block in 16-byte pairs
+    for (int e = tid; e < LT_B * LT_B && wid < LT_WAVES; e += LT_THREADS) {
       const int i = e / LT_B, q = e % LT_B;
-      if (i >= q)
-        F[(size_t)(c0 + i) * LT_NP + c0 + q] =
-            i == q ? sh.d[c0 + q] : (i < LT_B ? Pc[i][q] * sh.rinv[c0 + q] : -sh.G[i][q]);
+      if (i > q) {
+        const double l = Pc[i][q] * sh.rinv[c0 + q];
+        F[(size_t)(c0 + i) * LT_NP + c0 + q] = l;
+        F[(size_t)(c0 + q) * LT_NP + c0 + i] = l;
+      } else if (i == q) {
+        F[(size_t)(c0 + i) * LT_NP + c0 + q] = sh.d[c0 + q];
+      }
     }
-    for (int e = tid; e < R * LT_B; e += LT_THREADS) {
-      const int q = e / R, i = e % R;
-      if (i > q)
-        F[(size_t)(c0 + q) * LT_NP + c0 + i] =
-            i < LT_B ? Pc[i][q] * sh.rinv[c0 + q] : -sh.G[i][q];
+    const int RB = R - LT_B;  // rows below the diagonal block (even)
+    for (int e = tid; e < RB * (LT_B / 2) && wid < LT_WAVES; e += LT_THREADS) {
+      const int i = LT_B + e / (LT_B / 2), q = 2 * (e % (LT_B / 2));
+      *reinterpret_cast<double2*>(F + (size_t)(c0 + i) * LT_NP + c0 + q) =
+          make_double2(-sh.G[i][q], -sh.G[i][q + 1]);
+    }
+    for (int e = tid; e < LT_B * (RB / 2) && wid < LT_WAVES; e += LT_THREADS) {
+      const int q = e / (RB / 2), i = LT_B + 2 * (e % (RB / 2));
+      *reinterpret_cast<double2*>(F + (size_t)(c0 + q) * LT_NP + c0 + i) =
+          make_double2(-sh.G[i][q], -sh.G[i + 1][q]);
     }
     // the bordered row right of the panel: zb_c -= sum_j z_j W_cj
     for (int c = c0 + LT_B + tid; c < LT_NP; c += LT_THREADS) {
@@ -487,26 +502,21 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
     // to F.
     const int tStart = LT_NB * (k + 1) - k * (k + 1) / 2;  // first tile with J > k
     const int t0 = tStart + (((wid - tStart) % LT_WAVES) + LT_WAVES) % LT_WAVES;
-    const int ns = t0 < LT_TILES ? (LT_TILES - t0 + LT_WAVES - 1) / LT_WAVES : 0;
+    const int ns = (t0 < LT_TILES && wid < LT_WAVES) ? (LT_TILES - t0 + LT_WAVES - 1) / LT_WAVES : 0;
     const int lr = lane >> 4, lc = lane & 15;
-    auto tileAddr = [&](int s) __attribute__((always_inline)) -> size_t {
+    // the lane's two halves of tile s of this wave's list in the tile store
+    auto tileLoad = [&](int s) __attribute__((always_inline)) -> v4d {
       const int t = min(t0 + LT_WAVES * s, LT_TILES - 1);
-      return (size_t)(LT_B * sh.tI[t] + lr) * LT_NP + LT_B * sh.tJ[t] + lc;
+      const double2* d = reinterpret_cast<const double2*>(Ft) + (size_t)2 * t * WAVE + lane;
+      const double2 a = d[0], b = d[WAVE];
+      return v4d{a.x, a.y, b.x, b.y};
     };
     v4d cur[LT_GRP], nxt[LT_GRP];
 #pragma unroll
-    for (int g = 0; g < LT_GRP; g++) {
-      const size_t o = tileAddr(g);
-#pragma unroll
-      for (int v = 0; v < 4; v++) cur[g][v] = F[o + (size_t)4 * v * LT_NP];
-    }
+    for (int g = 0; g < LT_GRP; g++) cur[g] = tileLoad(g);
     for (int s0 = 0; s0 < ns; s0 += LT_GRP) {
 #pragma unroll
-      for (int g = 0; g < LT_GRP; g++) {
-        const size_t o = tileAddr(s0 + LT_GRP + g);
-#pragma unroll
-        for (int v = 0; v < 4; v++) nxt[g][v] = F[o + (size_t)4 * v * LT_NP];
-      }
+      for (int g = 0; g < LT_GRP; g++) nxt[g] = tileLoad(s0 + LT_GRP + g);
       int pi[LT_GRP], pj[LT_GRP];
 #pragma unroll
       for (int g = 0; g < LT_GRP; g++) {
@@ -530,15 +540,25 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
             const int pr = LT_B * (sh.tI[t] - k - 1) + lr;
 #pragma unroll
             for (int v = 0; v < 4; v++) Pn[pr + 4 * v][lc] = cur[g][v];
+            if (sh.tI[t] == k + 1)
+              __hip_atomic_store(&sh.dflag, k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
           } else {
-            const size_t o = tileAddr(s0 + g);
-#pragma unroll
-            for (int v = 0; v < 4; v++) F[o + (size_t)4 * v * LT_NP] = cur[g][v];
+            double2* d = reinterpret_cast<double2*>(Ft) + (size_t)2 * t * WAVE + lane;
+            d[0] = make_double2(cur[g][0], cur[g][1]);
+            d[WAVE] = make_double2(cur[g][2], cur[g][3]);
           }
         }
       }
 #pragma unroll
       for (int g = 0; g < LT_GRP; g++) cur[g] = nxt[g];
+    }
+    if (wid == LT_WAVES && k + 1 < LT_NB) {
+      int it = 0;
+      while (__hip_atomic_load(&sh.dflag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != k + 1 &&
+             ++it < LT_SPIN_MAX)
+        __builtin_amdgcn_s_sleep(1);
+      if (it >= LT_SPIN_MAX && lane == 0) sh.spinFail = 1;
+      lt_diag_block(Pn, sh.d + c0 + LT_B, sh.rinv + c0 + LT_B, lane);
     }
     lt_lds_barrier();
   }
@@ -555,7 +575,7 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
   // octet in double-double
   {
     const int sub = lane & 7;
-    for (int u0 = wid * 8; u0 < MATSIZE; u0 += LT_WAVES * 8) {
+    for (int u0 = wid * 8; u0 < MATSIZE && wid < LT_WAVES; u0 += LT_WAVES * 8) {
       const int u = u0 + (lane >> 3);
       double hi = 0.0, lo = 0.0;
       if (u < MATSIZE) {
@@ -598,17 +618,18 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
   // 1-based output, the other slots as the system's rhs holds them (as lubksb
   // leaves them)
   for (int i = tid; i < NSOL; i += LT_THREADS) {
-    const double val = (i >= 1 && i <= MATSIZE) ? sh.x[i - 1] : b0[i];
+    double val = (i >= 1 && i <= MATSIZE) ? sh.x[i - 1] : b0[i];
+    if (sh.spinFail && i >= 1 && i <= MATSIZE) val = __builtin_nan("");
     S.rhs[i] = val;
     if (S.out) S.out[i] = val;
   }
 }
 
-__global__ __launch_bounds__(LT_THREADS) void k_ldlt_solve(LuSys S, int refine) {
+__global__ __launch_bounds__(LT_BLOCK) void k_ldlt_solve(LuSys S, int refine) {
   ldlt_solve_dev(S, refine);
 }
 
-__global__ __launch_bounds__(LT_THREADS) void k_ldlt_solve_b(LuBatch B, int refine) {
+__global__ __launch_bounds__(LT_BLOCK) void k_ldlt_solve_b(LuBatch B, int refine) {
   ldlt_solve_dev(B.s[blockIdx.x], refine);
 }
 
@@ -624,7 +645,7 @@ static hipError_t lt_attr() {
 hipError_t launch_ldlt_solve(hipStream_t s, const LuSys& S, bool refine) {
   const hipError_t e = lt_attr();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ldlt_solve, dim3(1), dim3(LT_THREADS), sizeof(LtShared), s, S,
+  hipLaunchKernelGGL(k_ldlt_solve, dim3(1), dim3(LT_BLOCK), sizeof(LtShared), s, S,
                      refine ? 1 : 0);
   return hipGetLastError();
 }
@@ -634,7 +655,7 @@ hipError_t launch_ldlt_solve_batch(hipStream_t s, const LuBatch& B, bool refine)
   if (B.n > LU_BATCH_MAX) return hipErrorInvalidValue;
   const hipError_t e = lt_attr();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ldlt_solve_b, dim3(B.n), dim3(LT_THREADS), sizeof(LtShared), s, B,
+  hipLaunchKernelGGL(k_ldlt_solve_b, dim3(B.n), dim3(LT_BLOCK), sizeof(LtShared), s, B,
                      refine ? 1 : 0);
   return hipGetLastError();
 }

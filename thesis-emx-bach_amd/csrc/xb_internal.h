@@ -63,8 +63,11 @@ constexpr int NSOL = NPSF * NC2 + NBG + 1;  // 298 solution slots
 // panel (19 for 292 unknowns)
 constexpr int LU_PANELS = (MATSIZE + 15) / 16;
 constexpr size_t LU_STATE_BYTES = 64 + sizeof(int) * MATN * (1 + LU_PANELS);
-// XB_FORM_LDLT's factor workspace: the system padded to 19 blocks of 16, 0-based
-constexpr int LDLT_WORK_DOUBLES = (LU_PANELS * 16) * (LU_PANELS * 16);
+// XB_FORM_LDLT's workspace: L and D of the system padded to 19 blocks of 16
+// (0-based, L also transposed above the diagonal), then the 190 lower 16 x 16
+// tiles of the trailing matrix in the factorisation's lane order
+constexpr int LDLT_WORK_DOUBLES =
+    (LU_PANELS * 16) * (LU_PANELS * 16) + LU_PANELS * (LU_PANELS + 1) / 2 * 256;
 constexpr int QN = NPSF + 2;              // 51: per-stamp test system (1-based)
 constexpr int MAXSS = 6;                  // 2 * maxKSStamps
 constexpr int KEEPSS = 3;                 // maxKSStamps
