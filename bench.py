@@ -62,6 +62,9 @@ CONV_TRAFFIC_BYTES = 163_815_488
 # HBM bytes per global solve (the 23 kernels k_lu_scale, k_lu_begin, 19 x
 # k_lu_step, k_lu_backsub), measured the same way; profiles/lu_solve_traffic.md.
 LU_TRAFFIC_BYTES = 23_401_549
+# HBM bytes per L D L^T solve (one k_ldlt_solve launch, with one refinement
+# step), measured the same way; profiles/ldlt_solve_traffic.md.
+LDLT_TRAFFIC_BYTES = 2_992_845
 LU_N = 292                                   # unknowns of the global system
 LU_FLOPS = 2.0 / 3.0 * LU_N ** 3             # ludcmp's multiply-adds, as flops
 LDLT_FLOPS = LU_N ** 3 / 3.0 + 8.0 * LU_N ** 2  # L D L^T, substitutions, one refinement
@@ -561,7 +564,7 @@ def main() -> int:
                           "achieved": LDLT_FLOPS / ldlt_avg_s / 1e12, "peak": PEAK_FP64_TFLOPS,
                           "unit": "TFLOP/s",
                           "frac": LDLT_FLOPS / ldlt_avg_s / 1e12 / PEAK_FP64_TFLOPS,
-                          "traffic": None,
+                          "traffic": LDLT_TRAFFIC_BYTES,
                           "algorithmic": f"292^3/3 + 8*292^2 = {LDLT_FLOPS:.4g} flop per solve",
                           "us_per_solve": ldlt_avg_s * 1e6,
                           "us_per_solve_ludcmp": solve_avg_s * 1e6},
