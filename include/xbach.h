@@ -276,8 +276,10 @@ int xb_sync(xb_ctx* ctx);
 /* ---- forms of the path ----------------------------------------------------
  * XB_FORM_PLAIN (0): the reference's arithmetic, bit-identical to it.
  * XB_FORM_MIXED: cmv's basis convolution and normal-matrix products run on
- * the matrix cores in BF16 split precision (hi + lo) with FP32 accumulation;
- * the stamp and global LU solves, chi2, convolution and difference stay FP64.
+ * the matrix cores in BF16 split precision (hi + lo) with FP32 accumulation
+ * for every kept stamp (the clip-refit loop's refills of a few stamps take the
+ * FP64 fill, faster for a short list); the stamp and global LU solves, chi2,
+ * convolution and difference stay FP64.
  * Results differ from the plain form at the 1e-5 level of W and Q; the
  * selected substamps (sss) are the same.  Applies from the next xb_init. */
 #define XB_FORM_PLAIN 0

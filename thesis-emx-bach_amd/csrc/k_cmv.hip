@@ -461,7 +461,12 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, in
   A.Bc = c.st[im].Bc;
   A.Br = c.st[im].Br;
   prof_begin(c, XB_PROF_FILL);
-  if (c.form & XB_FORM_MIXED) {
+  // The mixed form fills every kept stamp on the matrix cores (one workgroup
+  // per stamp for ~80 us: throughput over 100+ stamps).  A listed refill (the
+  // few stamps a clip-refit pass moved on, a restore, the screen's probes)
+  // takes the FP64 fill below instead, a few microseconds for a short list,
+  // and no less accurate.
+  if ((c.form & XB_FORM_MIXED) && !d_list) {
     const hipError_t em = launch_fill_mixed(c, A, nMax);
     prof_end(c, XB_PROF_FILL);
     return em;
