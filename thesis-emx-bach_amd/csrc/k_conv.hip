@@ -120,6 +120,99 @@ __global__ __launch_bounds__(512) void k_tile_kernels(const Basis* __restrict__ 
   }
 }
 
+// The mixed form's tile kernels: the same product, tile kernel = sum_n
+// coefficient_n(tile) * vec_n, as one FP64 matrix-core product per 16 tiles,
+// K (16 tiles x 441 taps) = C (16 x 49) V (49 x 441): wave w of four takes the
+// 16-tap column blocks w, w + 4, ... (28 blocks, 13 products of depth 4
+// each; the depth padded 49 -> 52 with zeros).  Sum |k| per tile and the
+// kernel sum at the image centre (tile index nTiles) in any order (the lane
+// sums of a row of 16 lanes by shuffles, the four waves' partials added in
+// wave order).
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_tile_kernels_mfma(const Basis* __restrict__ basis,
+                                                           const double* __restrict__ sol, int w,
+                                                           int h, int xSteps, int nTiles,
+                                                           double* tileK64, double* tileAks,
+                                                           double* kernSumOut) {
+  constexpr int KD = 52;                // depth padded to a multiple of 4
+  constexpr int NBLK = (KK + 15) / 16;  // 28 column blocks of 16 taps
+  const int t0 = blockIdx.x * 16;
+  const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+  const int lr = lane >> 4, lc = lane & 15;
+  __shared__ double co[16][KD + 1];
+  __shared__ double part[2][4][16];  // per wave and tile: sum |k|, sum k
+  const double hw = 0.5 * w, hh = 0.5 * h;
+  for (int e = tid; e < 16 * KD; e += blockDim.x) {
+    const int q = e / KD, n = e % KD, t = t0 + q;
+    double v = 0.0;
+    if (n < NPSF && t <= nTiles) {
+      int x, y;
+      if (t < nTiles) {
+        x = (t % xSteps) * KW + HKW + HKW;
+        y = (t / xSteps) * KW + HKW + HKW;
+      } else {
+        x = w / 2;
+        y = h / 2;
+      }
+      v = kernel_coeff_c(sol, n, (x - hw) / hw, (y - hh) / hh);
+    }
+    co[q][n] = v;
+  }
+  __syncthreads();
+  double aks[4] = {0.0, 0.0, 0.0, 0.0}, ksum[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int nb = wave; nb < NBLK; nb += 4) {
+    const int tap = 16 * nb + lc;
+    const int tapc = min(tap, KK - 1);
+    double bv[KD / 4];
+#pragma unroll
+    for (int c = 0; c < KD / 4; c++) {
+      const int n = 4 * c + lr;
+      const double b = basis->vec[min(n, NPSF - 1) * KK + tapc];
+      bv[c] = (n < NPSF && tap < KK) ? b : 0.0;
+    }
+    v4d_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < KD / 4; c++)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(co[lc][4 * c + lr], bv[c], acc, 0, 0, 0);
+    // element (tile lr + 4 v, tap 16 nb + lc) in acc[v]
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int t = t0 + lr + 4 * v;
+      if (tap < KK) {
+        if (t < nTiles) tileK64[(size_t)t * KK + tap] = acc[v];
+        aks[v] += fabs(acc[v]);
+        ksum[v] += acc[v];
+      }
+    }
+  }
+  // sums over the 16 lanes of each row of lanes (the block's taps)
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    double a = aks[v], k = ksum[v];
+    a += __shfl_xor(a, 8);
+    k += __shfl_xor(k, 8);
+    a += __shfl_xor(a, 4);
+    k += __shfl_xor(k, 4);
+    a += __shfl_xor(a, 2);
+    k += __shfl_xor(k, 2);
+    a += __shfl_xor(a, 1);
+    k += __shfl_xor(k, 1);
+    if (lc == 0) {
+      part[0][wave][lr + 4 * v] = a;
+      part[1][wave][lr + 4 * v] = k;
+    }
+  }
+  __syncthreads();
+  if (tid < 16) {
+    const int t = t0 + tid;
+    const double a = ((part[0][0][tid] + part[0][1][tid]) + part[0][2][tid]) + part[0][3][tid];
+    const double k = ((part[1][0][tid] + part[1][1][tid]) + part[1][2][tid]) + part[1][3][tid];
+    if (t < nTiles) tileAks[t] = a;
+    else if (t == nTiles) *kernSumOut = k;
+  }
+}
+
 struct ConvArgs {
   const float* T;      // image being convolved
   const float* S;      // target image
@@ -179,7 +272,7 @@ __device__ __forceinline__ double widen(float f) {
   return d;
 }
 
-template <int CR>
+template <int CR, bool FUSED>
 __device__ __forceinline__ void conv_rows(const double* __restrict__ K, const float (*win)[CTP],
                                           int g0, int cx, double (&acc)[CR]) {
   float ring[CR][KW];
@@ -209,17 +302,25 @@ __device__ __forceinline__ void conv_rows(const double* __restrict__ K, const fl
         // (each still takes its terms in column order)
 #pragma unroll
         for (int i = 0; i < KW; i++) {
-          double pr[CR];
+          if constexpr (FUSED) {
 #pragma unroll
-          for (int r = 0; r < CR; r++) pr[r] = kr[i] * widen(ring[(u + r) % CR][i]);
+            for (int r = 0; r < CR; r++) acc[r] = fma(kr[i], widen(ring[(u + r) % CR][i]), acc[r]);
+          } else {
+            double pr[CR];
 #pragma unroll
-          for (int r = 0; r < CR; r++) acc[r] = acc[r] + pr[r];
+            for (int r = 0; r < CR; r++) pr[r] = kr[i] * widen(ring[(u + r) % CR][i]);
+#pragma unroll
+            for (int r = 0; r < CR; r++) acc[r] = acc[r] + pr[r];
+          }
         }
       }
     }
   }
 }
 
+// FUSED (the mixed form): each term a fused multiply-add, in the same order;
+// otherwise the reference's unfused multiply, then add.
+template <bool FUSED>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   __shared__ float win[2][CTW][CTP];    // each tile's input window
   __shared__ double kTile[2][KK + 7];   // each tile's kernel, for the masked sums
@@ -289,13 +390,13 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   double acc[4];
   if (half) {
     double a3[3];
-    conv_rows<3>(kl, W, g0, cx, a3);
+    conv_rows<3, FUSED>(kl, W, g0, cx, a3);
     acc[0] = a3[0];
     acc[1] = a3[1];
     acc[2] = a3[2];
     acc[3] = 0.0;
   } else {
-    conv_rows<4>(kl, W, g0, cx, acc);
+    conv_rows<4, FUSED>(kl, W, g0, cx, acc);
   }
   if (gi > 2 || x >= w - HKW) return;
   const double ks = *A.kernSum;
@@ -432,9 +533,13 @@ hipError_t launch_conv(Ctx& c) {
   const int ti = c.tIdx;
   double* kernSum = &c.sc->kernSum;
   const double* sol = c.ksSol + (size_t)ti * NSOL;  // the chosen direction's fit
-  hipLaunchKernelGGL(k_tile_kernels, dim3((nTiles + 1 + TK_TILES - 1) / TK_TILES), dim3(512), 0,
-                     c.cur, c.basis, sol, c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks,
-                     kernSum);
+  if (c.form & XB_FORM_MIXED)
+    hipLaunchKernelGGL(k_tile_kernels_mfma, dim3((nTiles + 1 + 15) / 16), dim3(256), 0, c.cur,
+                       c.basis, sol, c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks, kernSum);
+  else
+    hipLaunchKernelGGL(k_tile_kernels, dim3((nTiles + 1 + TK_TILES - 1) / TK_TILES), dim3(512),
+                       0, c.cur, c.basis, sol, c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks,
+                       kernSum);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   ConvArgs A;
@@ -455,7 +560,10 @@ hipError_t launch_conv(Ctx& c) {
   A.hi = c.m.threshHigh;
   A.lo = c.m.threshLow;
   prof_begin(c, XB_PROF_CONV);
-  hipLaunchKernelGGL(k_conv, dim3((xSteps + 1) / 2, ySteps), dim3(256), 0, c.cur, A);
+  if (c.form & XB_FORM_MIXED)
+    hipLaunchKernelGGL(k_conv<true>, dim3((xSteps + 1) / 2, ySteps), dim3(256), 0, c.cur, A);
+  else
+    hipLaunchKernelGGL(k_conv<false>, dim3((xSteps + 1) / 2, ySteps), dim3(256), 0, c.cur, A);
   e = hipGetLastError();
   prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;
