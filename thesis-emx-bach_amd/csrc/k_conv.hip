@@ -246,73 +246,52 @@ __device__ __forceinline__ uint16_t after_flags(float t, double hi, double lo) {
   return m;
 }
 
-// The convolution of one 21 x 21 output tile by two waves.  The tile's
-// 41 x 41 input window is staged in LDS once (the same clamped reads as the
-// reference's edge handling); lane = (column, row group) with three groups per
-// wave: the first wave's groups are output rows 0-3, 7-10 and 14-17 (CR = 4),
-// the second's 4-6, 11-13 and 18-20 (CR = 3), so every lane of both waves
-// computes a row of its own.  Output (x, y) sums K[20-d][20-i] * T[y-10+d][x-10+i]
-// over d, then i, ascending (the reference's term order: window rows, then
-// columns, ascending; multiply and add unfused).  Step d applies kernel row
-// 20 - d to all CR rows of a lane at once, so each kernel row is read once per
-// wave, as broadcast LDS reads of the tile kernel's LDS copy (the kernel is
-// read once per output row group instead of once per output row); the CR
-// window rows a step needs sit in a register ring of FP32 values (widened to
-// FP64 at each use, exact), one new row per step read from LDS with the
-// kernel row before the step's sums.
+// The convolution: one wave per 21 x 21 output tile, CV_TILES tiles per
+// workgroup.  The tile's 41 x 41 input window is staged in LDS (FP32, the same
+// clamped reads as the reference's edge handling).  Lane = (output row y,
+// column group g): 21 rows x 3 groups of CG = 7 adjacent columns, so each lane
+// computes 7 outputs of one row (lane 63 idles).  Output (x, y) sums
+// K[20-d][20-i] * T[y-10+d][x-10+i] over d, then i, ascending (the
+// reference's term order: window rows, then columns, ascending; multiply and
+// add unfused).  The tile kernel is the same for every lane of the wave: its
+// rows are read by scalar loads into SGPRs (the constant address space) and
+// each product takes its coefficient from there.  Along a window row the 7
+// outputs share their inputs: tap i of output c reads window column
+// 7g + c + i, so each step i reads one new window value (widened to FP64
+// once, exact) into a 7-slot register ring and applies it to all 7
+// accumulators: 27 LDS reads and 147 products per lane and kernel row.
 constexpr int CTW = KW + 2 * HKW;  // 41: the tile's input window
-constexpr int CTP = CTW + 1;       // its LDS pitch
+constexpr int CTP = CTW;           // its LDS pitch (odd: rows fall on distinct banks)
+constexpr int CG = 7;              // output columns per lane
+constexpr int CV_TILES = 4;        // tiles (waves) per workgroup
+static_assert(3 * CG == KW, "three column groups cover a tile row");
 
-// FP32 -> FP64 (exact) at the point of use: a volatile conversion the
-// compiler cannot merge across steps, which would keep the widened window in
-// twice the registers
-__device__ __forceinline__ double widen(float f) {
-  double d;
-  __asm__ __volatile__("v_cvt_f64_f32 %0, %1" : "=v"(d) : "v"(f));
-  return d;
-}
+typedef const __attribute__((address_space(4))) double* KConst;
 
-template <int CR, bool FUSED>
-__device__ __forceinline__ void conv_rows(const double* __restrict__ K, const float (*win)[CTP],
-                                          int g0, int cx, double (&acc)[CR]) {
-  float ring[CR][KW];
+template <bool FUSED>
+__device__ __forceinline__ void conv_row_group(KConst K, const float (*win)[CTP], int y, int xc0,
+                                               double (&acc)[CG]) {
 #pragma unroll
-  for (int r = 0; r < CR; r++) acc[r] = 0.0;
+  for (int c = 0; c < CG; c++) acc[c] = 0.0;
+  for (int d = 0; d <= 2 * HKW; d++) {
+    const float* src = win[y + d] + xc0;
+    const KConst Kr = K + (2 * HKW - d) * KW;
+    double t[CG];  // window column xc0 + j sits in slot j % CG
 #pragma unroll
-  for (int r = 0; r + 1 < CR; r++) {
-    const float* src = win[g0 + r] + cx;
+    for (int j = 0; j + 1 < CG; j++) t[j] = (double)src[j];
 #pragma unroll
-    for (int i = 0; i < KW; i++) ring[r][i] = src[i];
-  }
-  for (int d0 = 0; d0 <= 2 * HKW; d0 += CR) {
+    for (int i = 0; i < KW; i++) {
+      t[(i + CG - 1) % CG] = (double)src[i + CG - 1];
+      const double k = Kr[2 * HKW - i];
+      if constexpr (FUSED) {
 #pragma unroll
-    for (int u = 0; u < CR; u++) {
-      const int d = d0 + u;  // (d + r) % CR == (u + r) % CR: ring slots are constants
-      if (d <= 2 * HKW) {
-        // the step's reads first, all from LDS: the kernel row (the same
-        // address in every lane) and the window row that replaces row d - 1
-        const double* Kr = K + (2 * HKW - d) * KW;
-        const float* src = win[g0 + d + CR - 1] + cx;
-        double kr[KW];
+        for (int c = 0; c < CG; c++) acc[c] = fma(k, t[(i + c) % CG], acc[c]);
+      } else {
+        double pr[CG];
 #pragma unroll
-        for (int i = 0; i < KW; i++) kr[i] = Kr[2 * HKW - i];
+        for (int c = 0; c < CG; c++) pr[c] = k * t[(i + c) % CG];
 #pragma unroll
-        for (int i = 0; i < KW; i++) ring[(u + CR - 1) % CR][i] = src[i];
-        // columns outer, rows inner: the CR accumulators' chains interleave
-        // (each still takes its terms in column order)
-#pragma unroll
-        for (int i = 0; i < KW; i++) {
-          if constexpr (FUSED) {
-#pragma unroll
-            for (int r = 0; r < CR; r++) acc[r] = fma(kr[i], widen(ring[(u + r) % CR][i]), acc[r]);
-          } else {
-            double pr[CR];
-#pragma unroll
-            for (int r = 0; r < CR; r++) pr[r] = kr[i] * widen(ring[(u + r) % CR][i]);
-#pragma unroll
-            for (int r = 0; r < CR; r++) acc[r] = acc[r] + pr[r];
-          }
-        }
+        for (int c = 0; c < CG; c++) acc[c] = acc[c] + pr[c];
       }
     }
   }
@@ -321,84 +300,48 @@ __device__ __forceinline__ void conv_rows(const double* __restrict__ K, const fl
 // FUSED (the mixed form): each term a fused multiply-add, in the same order;
 // otherwise the reference's unfused multiply, then add.
 template <bool FUSED>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
-  __shared__ float win[2][CTW][CTP];    // each tile's input window
-  __shared__ double kTile[2][KK + 7];   // each tile's kernel, for the masked sums
-  // wave-uniform (scalar) tile, half and so the tile kernel's address
+__global__ __launch_bounds__(WAVE * CV_TILES) void k_conv(ConvArgs A) {
+  __shared__ float win[CV_TILES][CTW][CTP];                 // each tile's input window
+  __shared__ unsigned long long rowFlag[CV_TILES][CTW];     // bit c: window pixel (l, c) flagged
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE);
   const int lane = threadIdx.x % WAVE;
-  const int tl = wave >> 1, half = wave & 1;
-  const int xs = blockIdx.x * 2 + tl, ys = blockIdx.y;
+  const int xs = blockIdx.x * CV_TILES + wave, ys = blockIdx.y;
   const int x0 = HKW + xs * KW, y0t = HKW + ys * KW;
   const int w = A.w, h = A.h;
   const bool tileOn = xs < A.xSteps && x0 < w - HKW && y0t < h - HKW;
   const int tile = xs + ys * A.xSteps;
-  const double* K = A.tileK64 + (size_t)tile * KK;
+  float(*W)[CTP] = win[wave];
   if (tileOn) {
     // the window, clamped as the reference's reads: row min(y, h - 1),
     // column min(x, w - 1) (both start inside the image)
-    const int t2 = threadIdx.x % (2 * WAVE);
-    for (int e = t2; e < CTW * CTW; e += 2 * WAVE) {
+    for (int e = lane; e < CTW * CTW; e += WAVE) {
       const int l = e / CTW, c = e % CTW;
       const int yy = min(y0t - HKW + l, h - 1), xx = min(x0 - HKW + c, w - 1);
-      win[tl][l][c] = A.T[(size_t)yy * w + xx];
+      W[l][c] = A.T[(size_t)yy * w + xx];
     }
-    for (int e = t2; e < KK; e += 2 * WAVE) kTile[tl][e] = K[e];
   }
-  __syncthreads();
   if (!tileOn) return;
-  const float (*W)[CTP] = win[tl];
-  // Flag masks of the window, one ballot per window row: lane l < 41 keeps
-  // bit c of row l = conv_flags of window pixel (l, c).
-  uint32_t mlo = 0, mhi = 0;
-  {
-    float col[CTW];
-#pragma unroll
-    for (int l = 0; l < CTW; l++) col[l] = W[l][min(lane, CTW - 1)];
-#pragma unroll
-    for (int l = 0; l < CTW; l++) {
-      const unsigned long long m =
-          __ballot(lane < CTW && conv_flags(col[l], A.hi, A.lo) != 0);
-      if (lane == l) {
-        mlo = (uint32_t)m;
-        mhi = (uint32_t)(m >> 32);
-      }
-    }
+  // each wave reads only its own tile's LDS, and a wave's LDS accesses run
+  // in program order: a wave barrier (no workgroup barrier) orders them
+  __builtin_amdgcn_wave_barrier();
+  // flag masks of the window rows, one ballot per row
+  for (int l = 0; l < CTW; l++) {
+    const float v = W[l][min(lane, CTW - 1)];
+    const unsigned long long m = __ballot(lane < CTW && conv_flags(v, A.hi, A.lo) != 0);
+    if (lane == 0) rowFlag[wave][l] = m;
   }
-  const int cx = lane % KW, gi = lane / KW;  // gi 0..2 active, 3 idle
-  const int gic = min(gi, 2);
-  const int off = half ? 4 : 0;              // first row of the wave's groups
-  const int g0 = gic * 7 + off;              // the lane's first output row in the tile
-  const int x = x0 + cx;
-  // flags of window row g0 + wr (relative to the lane's group), shifted to its column
-  auto rowMask = [&](int wr) -> unsigned long long {
-    const uint32_t l0 = __builtin_amdgcn_readlane(mlo, wr + off);
-    const uint32_t h0 = __builtin_amdgcn_readlane(mhi, wr + off);
-    const uint32_t l1 = __builtin_amdgcn_readlane(mlo, wr + off + 7);
-    const uint32_t h1 = __builtin_amdgcn_readlane(mhi, wr + off + 7);
-    const uint32_t l2 = __builtin_amdgcn_readlane(mlo, wr + off + 14);
-    const uint32_t h2 = __builtin_amdgcn_readlane(mhi, wr + off + 14);
-    const uint32_t lo = gic == 0 ? l0 : (gic == 1 ? l1 : l2);
-    const uint32_t hi = gic == 0 ? h0 : (gic == 1 ? h1 : h2);
-    return (((unsigned long long)hi << 32) | lo) >> cx;
-  };
-  const int nr = half ? 3 : 4;
-  unsigned rowBad = 0;  // bit wr: a flagged pixel in the lane's window row g0 + wr
-  for (int wr = 0; wr < nr + 2 * HKW; wr++)
-    if (rowMask(wr) & ((1ull << KW) - 1ull)) rowBad |= 1u << wr;
-  const double* kl = kTile[tl];
-  double acc[4];
-  if (half) {
-    double a3[3];
-    conv_rows<3, FUSED>(kl, W, g0, cx, a3);
-    acc[0] = a3[0];
-    acc[1] = a3[1];
-    acc[2] = a3[2];
-    acc[3] = 0.0;
-  } else {
-    conv_rows<4, FUSED>(kl, W, g0, cx, acc);
-  }
-  if (gi > 2 || x >= w - HKW) return;
+  const int y = min(lane / 3, KW - 1), g = lane % 3;  // lane 63: row 21, idle
+  const int xc0 = g * CG;
+  const KConst K = (KConst)(A.tileK64 + (size_t)tile * KK);  // uniform: scalar loads
+  double acc[CG];
+  conv_row_group<FUSED>(K, W, y, xc0, acc);
+  __builtin_amdgcn_wave_barrier();  // rowFlag, written by lane 0 above
+  if (lane >= 3 * KW) return;
+  const int yo = y0t + y;
+  if (yo >= h - HKW) return;
+  // flagged pixels anywhere in the lane's 21 window rows
+  unsigned long long vOr = 0;
+  for (int jj = 0; jj <= 2 * HKW; jj++) vOr |= rowFlag[wave][y + jj];
   const double ks = *A.kernSum;
   const double inv = A.scaleConv ? 1.0 / ks : 1.0;
   // sub (cl_kern/sub.cl:15-32, src/bach.cpp:415-440) on the unrounded FP64
@@ -407,26 +350,27 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
   const double finalFactor = A.scaleConv ? -(1.0 / ks) : 1.0;
   const double aks = A.tileAks[tile];
   const double thr = (double)0.99f;
-  for (int r = 0; r < nr; r++) {
-    const int y = y0t + g0 + r;
-    if (y >= h - HKW) break;
-    const size_t id = (size_t)y * w + x;
-    double v = acc[r];
-    v = v + background_c(A.sol, x, y, w, h);
+#pragma unroll
+  for (int c = 0; c < CG; c++) {
+    const int xo = xc0 + c;  // output column in the tile
+    const int x = x0 + xo;
+    if (x >= w - HKW) break;
+    const size_t id = (size_t)yo * w + x;
+    double v = acc[c];
+    v = v + background_c(A.sol, x, yo, w, h);
     v = v * inv;
     A.out[id] = (float)v;
-    const uint16_t cm = conv_flags(W[g0 + r + HKW][cx + HKW], A.hi, A.lo);
+    const uint16_t cm = conv_flags(W[y + HKW][xo + HKW], A.hi, A.lo);
     uint16_t nm = cm;
     if (cm & XB_MASK_BAD_INPUT) nm |= XB_MASK_BAD_OUTPUT;
-    if ((rowBad >> r) & ((1u << KW) - 1u)) {
+    if ((vOr >> xo) & ((1ull << KW) - 1ull)) {
       // sum |k| over unflagged pixels in the reference's loop order (rows
       // j ascending, columns i ascending); a flagged pixel adds +0.0, which
       // leaves the sum (never -0) unchanged
       double uks = 0.0;
       for (int jj = 0; jj <= 2 * HKW; jj++) {
-        const unsigned long long m = rowMask(r + jj);
-        const double* Kr = kl + (2 * HKW - jj) * KW;
-#pragma unroll
+        const unsigned long long m = rowFlag[wave][y + jj] >> xo;
+        const KConst Kr = K + (2 * HKW - jj) * KW;
         for (int ii = 0; ii <= 2 * HKW; ii++)
           uks = uks + (((m >> ii) & 1ull) ? 0.0 : fabs(Kr[2 * HKW - ii]));
       }
@@ -436,9 +380,9 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
     const float sv = A.S[id];
     const uint16_t fm = nm | after_flags(sv, A.hi, A.lo);
     A.mask[id] = fm;
-    double d = 1e-30;
-    if ((fm & XB_MASK_BAD_OUTPUT) == 0) d = (v * convFactor - (double)sv) * finalFactor;
-    A.diff[id] = (float)d;
+    double dd = 1e-30;
+    if ((fm & XB_MASK_BAD_OUTPUT) == 0) dd = (v * convFactor - (double)sv) * finalFactor;
+    A.diff[id] = (float)dd;
   }
 }
 
@@ -561,9 +505,11 @@ hipError_t launch_conv(Ctx& c) {
   A.lo = c.m.threshLow;
   prof_begin(c, XB_PROF_CONV);
   if (c.form & XB_FORM_MIXED)
-    hipLaunchKernelGGL(k_conv<true>, dim3((xSteps + 1) / 2, ySteps), dim3(256), 0, c.cur, A);
+    hipLaunchKernelGGL(k_conv<true>, dim3((xSteps + CV_TILES - 1) / CV_TILES, ySteps),
+                       dim3(WAVE * CV_TILES), 0, c.cur, A);
   else
-    hipLaunchKernelGGL(k_conv<false>, dim3((xSteps + 1) / 2, ySteps), dim3(256), 0, c.cur, A);
+    hipLaunchKernelGGL(k_conv<false>, dim3((xSteps + CV_TILES - 1) / CV_TILES, ySteps),
+                       dim3(WAVE * CV_TILES), 0, c.cur, A);
   e = hipGetLastError();
   prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;
