@@ -24,16 +24,26 @@
 // and the ksc right-hand side, src/cdkscUtil.cpp:459-470) and row-major order
 // for the 3 background rows (cd right-hand side, cl_kern/cd.cl:318-344), beside
 // the Gram items instead of after them.  Q (51x51, 1-based) is G shifted by one.
+#include <algorithm>
+
 #include "xb_dev.h"
 
 namespace xb {
 
-__global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
-  int s, ssx, ssy;
-  if (!fill_stamp(A, blockIdx.x, &s, &ssx, &ssy)) return;
+// entries of a listed refill covered by one pass of its grid
+constexpr int FILL_LIST_STAMPS = 16;
+
+struct FillConvLds {
+  float win[TW][TW + 1];
+  float tmp[TH * TW];
+  double fx[KW], fy[KW];
+};
+
+// W row n of stamp s (substamp centre ssx, ssy)
+__device__ __forceinline__ void fill_conv_row(const FillArgs& A, FillConvLds& L, int s, int ssx,
+                                              int ssy, int n) {
   const int g = A.kept[s];
   const bool valid = A.cur[s] < A.count[g];
-  const int n = blockIdx.y;
   const int tid = threadIdx.x;
   double* Wrow = A.W + ((size_t)s * WROWS + n) * NPIX;
   double* Wcrow = A.Wc + ((size_t)s * WROWS + n) * NPIX;  // column-major copy
@@ -60,9 +70,10 @@ __global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
     }
     return;
   }
-  __shared__ float win[TW][TW + 1];
-  __shared__ float tmp[TH * TW];
-  __shared__ double fx[KW], fy[KW];
+  auto& win = L.win;
+  auto& tmp = L.tmp;
+  auto& fx = L.fx;
+  auto& fy = L.fy;
   if (tid < KW) {
     fx[tid] = A.basis->fx[n * KW + tid];
     fy[tid] = A.basis->fy[n * KW + tid];
@@ -96,6 +107,22 @@ __global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
       w0 += (double)tmp[idx] * fx[HKW - xx];
     }
     put(p, w0);
+  }
+}
+
+// One workgroup per (list entry, W row).  LOOP (listed refills): the grid
+// covers FILL_LIST_STAMPS entries and loops over the rest (the list's length
+// is on the device), so a short list does not dispatch a workgroup per kept
+// stamp.
+template <bool LOOP>
+__global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
+  __shared__ FillConvLds L;
+  for (int k = blockIdx.x;; k += gridDim.x) {
+    int s, ssx, ssy;
+    if (!fill_stamp(A, k, &s, &ssx, &ssy)) return;
+    fill_conv_row(A, L, s, ssx, ssy, blockIdx.y);
+    if (!LOOP) return;
+    __syncthreads();  // the next entry reuses the LDS
   }
 }
 
@@ -288,18 +315,11 @@ __device__ __forceinline__ void fill_target_wave(const FillArgs& A, int s, int s
 // The next pixel chunk is loaded into registers while the current one is
 // summed from LDS.  One LDS buffer suffices (a single wave's LDS reads and
 // writes run in program order), which leaves room for two waves per SIMD.
-__global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
-  const int total = nMax * NGI;
-  const int per = (total + 7) / 8;
-  const int wk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-  if (wk >= total) return;
-  int s, ssx, ssy;
-  if (!fill_stamp(A, wk / NGI, &s, &ssx, &ssy)) return;
+__device__ __forceinline__ void gram_item(const FillArgs& A, int s, int ssx, int ssy, int item,
+                                          double* Wl) {
   const int g = A.kept[s];
   const bool valid = A.cur[s] < A.count[g];
   const int lane = threadIdx.x;
-  const int item = wk % NGI;
-  __shared__ __align__(16) double Wl[2 * GK * GP];
   if (item == NGT) {
     fill_target_wave(A, s, ssx, ssy, valid, Wl);
     return;
@@ -439,6 +459,25 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nMax) {
       if (q0 + j <= r0 + i) G[(r0 + i) * WROWS + q0 + j] = cv[i][j];
 }
 
+// nCover: list entries covered by one pass of the grid (the kept stamps for
+// a full fill, FILL_LIST_STAMPS at most for a listed refill, LOOP); later
+// entries are taken by further passes.
+template <bool LOOP>
+__global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nCover) {
+  const int total = nCover * NGI;
+  const int per = (total + 7) / 8;
+  const int wk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (wk >= total) return;
+  __shared__ __align__(16) double Wl[2 * GK * GP];
+  for (int base = 0;; base += nCover) {
+    int s, ssx, ssy;
+    if (!fill_stamp(A, base + wk / NGI, &s, &ssx, &ssy)) return;
+    gram_item(A, s, ssx, ssy, wk % NGI, Wl);
+    if (!LOOP) return;
+    __syncthreads();  // the next entry reuses the LDS
+  }
+}
+
 hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, int nMax) {
   if (nMax <= 0) return hipSuccess;
   const int src = im, tgt = 1 - im;
@@ -471,11 +510,18 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, in
     prof_end(c, XB_PROF_FILL);
     return em;
   }
-  hipLaunchKernelGGL(k_fill_conv, dim3(nMax, WROWS), dim3(256), 0, c.cur, A);
+  const int nCover = d_list ? std::min(nMax, FILL_LIST_STAMPS) : nMax;
+  const dim3 gConv(nCover, WROWS), gGram(8 * ((nCover * NGI + 7) / 8));
+  if (d_list)
+    hipLaunchKernelGGL(k_fill_conv<true>, gConv, dim3(256), 0, c.cur, A);
+  else
+    hipLaunchKernelGGL(k_fill_conv<false>, gConv, dim3(256), 0, c.cur, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill_gram, dim3(8 * ((nMax * NGI + 7) / 8)), dim3(WAVE), 0, c.cur, A,
-                     nMax);
+  if (d_list)
+    hipLaunchKernelGGL(k_fill_gram<true>, gGram, dim3(WAVE), 0, c.cur, A, nCover);
+  else
+    hipLaunchKernelGGL(k_fill_gram<false>, gGram, dim3(WAVE), 0, c.cur, A, nCover);
   prof_end(c, XB_PROF_FILL);
   return hipGetLastError();
 }

@@ -314,30 +314,50 @@ __global__ __launch_bounds__(WAVE * CV_TILES) void k_conv(ConvArgs A) {
   if (tileOn) {
     // the window, clamped as the reference's reads: row min(y, h - 1),
     // column min(x, w - 1) (both start inside the image)
-    for (int e = lane; e < CTW * CTW; e += WAVE) {
+    // all of a lane's loads in flight together
+    constexpr int NS = (CTW * CTW + WAVE - 1) / WAVE;
+    float v[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const int e = min(lane + k * WAVE, CTW * CTW - 1);
       const int l = e / CTW, c = e % CTW;
       const int yy = min(y0t - HKW + l, h - 1), xx = min(x0 - HKW + c, w - 1);
-      W[l][c] = A.T[(size_t)yy * w + xx];
+      v[k] = A.T[(size_t)yy * w + xx];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const int e = lane + k * WAVE;
+      if (e < CTW * CTW) W[e / CTW][e % CTW] = v[k];
     }
   }
   if (!tileOn) return;
   // each wave reads only its own tile's LDS, and a wave's LDS accesses run
   // in program order: a wave barrier (no workgroup barrier) orders them
   __builtin_amdgcn_wave_barrier();
-  // flag masks of the window rows, one ballot per row
-  for (int l = 0; l < CTW; l++) {
-    const float v = W[l][min(lane, CTW - 1)];
-    const unsigned long long m = __ballot(lane < CTW && conv_flags(v, A.hi, A.lo) != 0);
-    if (lane == 0) rowFlag[wave][l] = m;
+  // flag masks of the window rows: lane l < 41 tests row l
+  {
+    const float* row = W[min(lane, CTW - 1)];
+    unsigned long long m = 0;
+#pragma unroll
+    for (int c = 0; c < CTW; c++)
+      if (conv_flags(row[c], A.hi, A.lo) != 0) m |= 1ull << c;
+    if (lane < CTW) rowFlag[wave][lane] = m;
   }
   const int y = min(lane / 3, KW - 1), g = lane % 3;  // lane 63: row 21, idle
   const int xc0 = g * CG;
   const KConst K = (KConst)(A.tileK64 + (size_t)tile * KK);  // uniform: scalar loads
+  const int yo = y0t + y;
+  // the target pixels of the lane's outputs, read before the products
+  float sv[CG];
+#pragma unroll
+  for (int c = 0; c < CG; c++) {
+    const int x = min(x0 + xc0 + c, w - 1);
+    sv[c] = A.S[(size_t)min(yo, h - 1) * w + x];
+  }
   double acc[CG];
   conv_row_group<FUSED>(K, W, y, xc0, acc);
-  __builtin_amdgcn_wave_barrier();  // rowFlag, written by lane 0 above
+  __builtin_amdgcn_wave_barrier();  // rowFlag, written by lanes 0..40 above
   if (lane >= 3 * KW) return;
-  const int yo = y0t + y;
   if (yo >= h - HKW) return;
   // flagged pixels anywhere in the lane's 21 window rows
   unsigned long long vOr = 0;
@@ -377,11 +397,10 @@ __global__ __launch_bounds__(WAVE * CV_TILES) void k_conv(ConvArgs A) {
       if ((uks / aks) < thr) nm |= XB_MASK_BAD_OUTPUT | XB_MASK_BAD_CONV;
       else nm |= XB_MASK_OK_CONV;
     }
-    const float sv = A.S[id];
-    const uint16_t fm = nm | after_flags(sv, A.hi, A.lo);
+    const uint16_t fm = nm | after_flags(sv[c], A.hi, A.lo);
     A.mask[id] = fm;
     double dd = 1e-30;
-    if ((fm & XB_MASK_BAD_OUTPUT) == 0) dd = (v * convFactor - (double)sv) * finalFactor;
+    if ((fm & XB_MASK_BAD_OUTPUT) == 0) dd = (v * convFactor - (double)sv[c]) * finalFactor;
     A.diff[id] = (float)dd;
   }
 }
@@ -392,23 +411,60 @@ __global__ __launch_bounds__(WAVE * CV_TILES) void k_conv(ConvArgs A) {
 // known become XB_MASK_NAN_PIXEL for the chosen direction (pend) and vanish for
 // the other; k_conv rewrites every interior pixel's mask from scratch, so the
 // border is the only place they can remain.
+__device__ __forceinline__ void border_pixel(const float* __restrict__ S, uint16_t* __restrict__ mask,
+                                             float* __restrict__ out, float* __restrict__ diff,
+                                             size_t id, double hi, double lo, uint16_t pend) {
+  out[id] = 1e-30f;
+  diff[id] = 1e-30f;
+  uint16_t m = mask[id];
+  if (m & (mask_pend(0) | mask_pend(1)))
+    m = (uint16_t)((m & XB_MASK_ALL) | ((m & pend) ? XB_MASK_NAN_PIXEL : 0));
+  mask[id] = m | after_flags(S[id], hi, lo);
+}
+
+// Index e over the border pixels only (w, h > 2 HKW): the HKW top rows, the
+// HKW bottom rows, then the HKW left and HKW right pixels of each row between.
 __global__ __launch_bounds__(256) void k_conv_border(const float* __restrict__ S,
                                                      uint16_t* __restrict__ mask,
                                                      float* __restrict__ out,
                                                      float* __restrict__ diff, int w, int h,
                                                      double hi, double lo, uint16_t pend) {
+  const size_t band = (size_t)HKW * w;
+  const size_t n = 2 * band + (size_t)2 * HKW * (h - 2 * HKW);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int x, y;
+    if (e < band) {
+      y = (int)(e / w);
+      x = (int)(e % w);
+    } else if (e < 2 * band) {
+      const size_t j = e - band;
+      y = h - HKW + (int)(j / w);
+      x = (int)(j % w);
+    } else {
+      const size_t j = e - 2 * band;
+      y = HKW + (int)(j / (2 * HKW));
+      const int k = (int)(j % (2 * HKW));
+      x = k < HKW ? k : w - 2 * HKW + k;
+    }
+    border_pixel(S, mask, out, diff, (size_t)y * w + x, hi, lo, pend);
+  }
+}
+
+// The same over every pixel, skipping the interior (images of at most
+// 2 HKW rows or columns, where the border covers all of it).
+__global__ __launch_bounds__(256) void k_conv_border_all(const float* __restrict__ S,
+                                                         uint16_t* __restrict__ mask,
+                                                         float* __restrict__ out,
+                                                         float* __restrict__ diff, int w, int h,
+                                                         double hi, double lo, uint16_t pend) {
   const size_t n = (size_t)w * h;
   for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
        id += (size_t)gridDim.x * blockDim.x) {
     int x = (int)(id % w), y = (int)(id / w);
     bool interior = x >= HKW && x < w - HKW && y >= HKW && y < h - HKW;
     if (interior) continue;
-    out[id] = 1e-30f;
-    diff[id] = 1e-30f;
-    uint16_t m = mask[id];
-    if (m & (mask_pend(0) | mask_pend(1)))
-      m = (uint16_t)((m & XB_MASK_ALL) | ((m & pend) ? XB_MASK_NAN_PIXEL : 0));
-    mask[id] = m | after_flags(S[id], hi, lo);
+    border_pixel(S, mask, out, diff, id, hi, lo, pend);
   }
 }
 
@@ -513,8 +569,16 @@ hipError_t launch_conv(Ctx& c) {
   e = hipGetLastError();
   prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_conv_border, dim3(1024), dim3(256), 0, c.cur, c.img[1 - ti], c.mask,
-                     c.conv, c.diff, c.w, c.h, c.m.threshHigh, c.m.threshLow, mask_pend(ti));
+  if (c.w > 2 * HKW && c.h > 2 * HKW) {
+    const size_t nb = (size_t)2 * HKW * c.w + (size_t)2 * HKW * (c.h - 2 * HKW);
+    const int blocks = (int)std::min<size_t>((nb + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_conv_border, dim3(blocks), dim3(256), 0, c.cur, c.img[1 - ti], c.mask,
+                       c.conv, c.diff, c.w, c.h, c.m.threshHigh, c.m.threshLow, mask_pend(ti));
+  } else {
+    hipLaunchKernelGGL(k_conv_border_all, dim3(1024), dim3(256), 0, c.cur, c.img[1 - ti],
+                       c.mask, c.conv, c.diff, c.w, c.h, c.m.threshHigh, c.m.threshLow,
+                       mask_pend(ti));
+  }
   return hipGetLastError();
 }
 

@@ -670,12 +670,16 @@ __global__ __launch_bounds__(1024) void k_cd_finish(const Basis* basis, const do
 // (ksc.cl:1-72, cdkscUtil.cpp:686-802).  Both rejection passes use the same
 // chi2 values; stamps rejected by either get their next substamp (refilled
 // once with the final index, which equals the reference's two refills).
+// rec (may be null): receives every stamp's substamp index as the pass found
+// it (the screen's record of the pass); zeroBits (may be null): cleared, one
+// byte per stamp (the screen's flags, at a loop's first pass).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int* kept,
                                                     int* cur, const int* count, int* list,
                                                     const int* nKept, KscScalars* ks,
                                                     int first, double alpha,
-                                                    double sigKernFit) {
+                                                    double sigKernFit, int* rec,
+                                                    uint8_t* zeroBits) {
   __shared__ double data[XB_MAX_STAMPS];
   __shared__ uint8_t clip[XB_MAX_STAMPS];
   __shared__ uint8_t refill[XB_MAX_STAMPS];
@@ -686,7 +690,12 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
   const int tid = threadIdx.x;
   const int n = *nKept;
   if (tid == 0) s_n = 0;
-  for (int s = tid; s < n; s += blockDim.x) refill[s] = 0;
+  // each thread records the entries it changes below (s = tid + k blockDim)
+  for (int s = tid; s < n; s += blockDim.x) {
+    refill[s] = 0;
+    if (rec) rec[s] = cur[s];
+    if (zeroBits) zeroBits[s] = 0;
+  }
   __syncthreads();
   for (int base = 0; base < n; base += blockDim.x) {
     int s = base + tid;
@@ -850,11 +859,11 @@ hipError_t launch_cd_finish(Ctx& c, int im) {
   return hipGetLastError();
 }
 
-hipError_t launch_ksc_check(Ctx& c, int im, bool first) {
+hipError_t launch_ksc_check(Ctx& c, int im, bool first, int* rec, uint8_t* zeroBits) {
   hipLaunchKernelGGL(k_ksc_check, dim3(1), dim3(1024), 0, c.cur, c.st[im].sigK,
                      c.st[im].kept, c.st[im].cur, c.st[im].count, c.st[im].listK,
                      &c.sc->nKept[im], &c.sc->k[im], first ? 1 : 0, c.m.sigClipAlpha,
-                     c.m.sigKernFit);
+                     c.m.sigKernFit, rec, zeroBits);
   return hipGetLastError();
 }
 

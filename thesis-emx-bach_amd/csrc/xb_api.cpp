@@ -431,11 +431,12 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
       c.basisValid = true;
     }
   }
-  XB_HIP(hipMemsetAsync(c.sc, 0, sizeof(DevScalars), c.stream));
   return XB_OK;
 }
 
+// the mask kernel also clears the run's device scalars (c.sc)
 int stageMask(Ctx& c) {
+  c.cur = c.stream;
   XB_HIP(launch_mask_input(c));
   return XB_OK;
 }
@@ -618,10 +619,11 @@ int kscPass(Ctx& c, int im, int it, bool keep, bool screen, uint16_t nanMark,
   XB_HIP(launch_calc_sigs(c, im, false, sol, nanMark));
   if (waitCd) XB_HIP(hipStreamWaitEvent(c.cur, waitCd, 0));
   const bool rec = keep && it < SCR_PASSES;
-  if (rec)  // the substamps of this pass
-    XB_HIP(hipMemcpyAsync(c.scr.passCur + (size_t)it * c.nKept[im], c.st[im].cur,
-                          sizeof(int) * c.nKept[im], hipMemcpyDeviceToDevice, c.cur));
-  XB_HIP(launch_ksc_check(c, im, it == 0));
+  // the check records the substamps of this pass; the screen's flags start
+  // cleared at a screened loop's first pass
+  XB_HIP(launch_ksc_check(c, im, it == 0,
+                          rec ? c.scr.passCur + (size_t)it * c.nKept[im] : nullptr,
+                          (screen && it == 0) ? c.scr.bits : nullptr));
   if (rec) XB_HIP(launch_screen(c, im, it, screen));
   XB_HIP(launch_fill(c, im, c.st[im].listK, &c.sc->k[im].nList, c.nKept[im]));
   static_assert(offsetof(KscScalars, nScreened) == offsetof(KscScalars, check) + 3 * sizeof(int),
@@ -679,7 +681,6 @@ int stageKsc(Ctx& c) {
     if (rc) return rc;
   }
   c.cur = c.stream;
-  if (screen) XB_HIP(hipMemsetAsync(c.scr.bits, 0, (size_t)c.nKept[im], c.stream));
   bool screenOn = screen;
   int it = 0;
   for (;;) {
@@ -1652,6 +1653,51 @@ static int runStages(Ctx& c, double* ms, bool timed) {
   if ((rc = mark(6))) return rc;
   return XB_OK;
 }
+
+}  // extern "C"
+
+namespace xb {
+
+// The whole pipeline on the caller's device buffers: the images are read where
+// they lie and the convolved and difference images written where the caller
+// wants them (null: the context's own buffers), so no image is copied.  The
+// context's pointers are restored before this returns, after its streams
+// drained; c.kernSum holds the kernel sum.
+int pipelineRunInto(Ctx& c, const xb_params* p, int w, int h, const void* tin, const void* sin,
+                    void* cout, void* dout) {
+  if (!tin || !sin) return XB_ERR_ARG;
+  int rc = setDev(c);
+  if (rc) return rc;
+  rc = stageInitCommon(c, p, w, h);
+  if (rc) return rc;
+  float* const own[4] = {c.img[0], c.img[1], c.conv, c.diff};
+  c.img[0] = const_cast<float*>(static_cast<const float*>(tin));
+  c.img[1] = const_cast<float*>(static_cast<const float*>(sin));
+  if (cout) c.conv = static_cast<float*>(cout);
+  if (dout) c.diff = static_cast<float*>(dout);
+  rc = stageMask(c);
+  double ms[9];
+  if (!rc) rc = runStages(c, ms, false);
+  if (!rc) {
+    const hipError_t e = hipMemcpyAsync(&c.kernSum, &c.sc->kernSum, sizeof(double),
+                                        hipMemcpyDeviceToHost, c.stream);
+    if (e != hipSuccess) {
+      c.err = std::string("read kernel sum: ") + hipGetErrorString(e);
+      rc = XB_ERR_HIP;
+    }
+  }
+  const int rs = syncAll(c);  // nothing may still use the caller's buffers
+  c.img[0] = own[0];
+  c.img[1] = own[1];
+  c.conv = own[2];
+  c.diff = own[3];
+  c.haveResult = false;  // the outputs are not in the context's buffers
+  return rc ? rc : rs;
+}
+
+}  // namespace xb
+
+extern "C" {
 
 int xb_pipeline_run(xb_ctx* x, const xb_params* p, int w, int h, const float* tmpl,
                     const float* sci, float* conv, float* diff, double* stage_ms) {

@@ -76,30 +76,14 @@ struct Job {
   std::string firstMsg;
 };
 
-int hipFail(xb_ctx* x, hipError_t e, const char* what) {
-  x->c.err = std::string(what) + ": " + hipGetErrorString(e);
-  return XB_ERR_HIP;
-}
-
-// One pair on one worker's context: the single-pair pipeline, then the outputs
-// copied to the caller's buffers on the context's stream.
+// One pair on one worker's context: the single-pair pipeline.  Device pairs
+// are read and written in the caller's buffers directly.
 int runPair(xb_ctx* x, const Job& J, int i, xb_pair_result* r) {
   xb::Ctx& c = x->c;
-  const size_t bytes = (size_t)J.w[i] * J.h[i] * sizeof(float);
   if (J.device) {
-    int rc = xb_pipeline_run_device(x, J.p, J.w[i], J.h[i], J.tin[i], J.sin[i]);
+    int rc = xb::pipelineRunInto(c, J.p, J.w[i], J.h[i], J.tin[i], J.sin[i],
+                                 J.cout ? J.cout[i] : nullptr, J.dout ? J.dout[i] : nullptr);
     if (rc) return rc;
-    hipError_t e;
-    if (J.cout && J.cout[i] &&
-        (e = hipMemcpyAsync(J.cout[i], c.conv, bytes, hipMemcpyDeviceToDevice, c.stream)))
-      return hipFail(x, e, "copy conv");
-    if (J.dout && J.dout[i] &&
-        (e = hipMemcpyAsync(J.dout[i], c.diff, bytes, hipMemcpyDeviceToDevice, c.stream)))
-      return hipFail(x, e, "copy diff");
-    if ((e = hipMemcpyAsync(&c.kernSum, &c.sc->kernSum, sizeof(double), hipMemcpyDeviceToHost,
-                            c.stream)))
-      return hipFail(x, e, "read kernel sum");
-    if ((e = hipStreamSynchronize(c.stream))) return hipFail(x, e, "sync");
   } else {
     int rc = xb_pipeline_run(x, J.p, J.w[i], J.h[i], static_cast<const float*>(J.tin[i]),
                              static_cast<const float*>(J.sin[i]),

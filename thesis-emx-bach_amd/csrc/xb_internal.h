@@ -335,7 +335,8 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
 // chi2 of the listed stamps (d_list, d_nList entries) into out (list order)
 hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d_nList,
                                  const double* d_sol, double* out);
-hipError_t launch_ksc_check(Ctx& c, int im, bool first);
+hipError_t launch_ksc_check(Ctx& c, int im, bool first, int* rec = nullptr,
+                            uint8_t* zeroBits = nullptr);
 hipError_t launch_conv(Ctx& c);
 // Restores the fit data of image im to what cmv left (every kept stamp on its
 // first substamp): stamps a clip-refit loop moved on are listed and refilled.
@@ -379,6 +380,9 @@ int ls_ksc_build(Ctx& c, LuBatch& B);
 int ls_ksc_post(Ctx& c, int it);
 int ls_ksc_check(Ctx& c, int it, bool* more);
 int ls_finish(Ctx& c);
+// the whole pipeline reading and writing the caller's device buffers
+int pipelineRunInto(Ctx& c, const xb_params* p, int w, int h, const void* tin, const void* sin,
+                    void* cout, void* dout);
 // FP64 iterative refinement of system L's solution (XB_FORM_REFINE) on stream s
 hipError_t launch_refine(hipStream_t s, const LuSet& L, double* d_out);
 
