@@ -176,19 +176,22 @@ def upload_pairs(ctx, pairs):
 def run_batched(ctx, params, pairs, bufs, workers, steps, warmup, form=0, model=None,
                 keep=4, tune=True, flags=0):
     """Time `steps` batches of all pairs (device-resident in and out); returns
-    ms per batch, the per-pair results, the diffs of the first `keep` pairs and
-    the autotuner's pairs in flight per size bucket (set up before the timed
+    ms per batch, the per-pair results, the diffs of the first `keep` pairs,
+    the autotuner's pairs in flight and launch shapes per size bucket (set up before the timed
     batches)."""
     ws = [t.shape[1] for t, _ in pairs]
     hs = [t.shape[0] for t, _ in pairs]
     cols = [[q[k] for q in bufs] for k in range(4)]
     table = None
+    shapes = None
     with xbach.Batch(0, workers=workers, flags=flags) as b:
         if model is not None:
             b.screen_load(model)
         b.set_form(form)
         if tune:
             table = b.autotune(params, ws, hs, *cols, reps=1)
+            # the tuned launch shapes of each size bucket present
+            shapes = {k: b.shapes(k).as_dict() for k in range(4) if table[k]}
         for _ in range(warmup):
             b.run_device(params, ws, hs, *cols)
         t0 = time.perf_counter()
@@ -200,7 +203,7 @@ def run_batched(ctx, params, pairs, bufs, workers, steps, warmup, form=0, model=
         d = np.empty_like(pairs[i][0])
         ctx.buf_download(d, bufs[i][3])
         diffs.append(d)
-    return (t1 - t0) * 1e3 / steps, res, diffs, table
+    return (t1 - t0) * 1e3 / steps, res, diffs, table, shapes
 
 
 def cmv_mfma_rate(ctx, params, w, h, d_t, d_s, reps=10) -> dict:
@@ -438,7 +441,7 @@ def main() -> int:
         assert np.array_equal(pairs[0][0], tmpl) and np.array_equal(pairs[0][1], sci)
         bufs = upload_pairs(ctx, pairs)
         keep = 8
-        b_ms, res, dplain, btab = run_batched(ctx, params, pairs, bufs, args.workers,
+        b_ms, res, dplain, btab, bshp = run_batched(ctx, params, pairs, bufs, args.workers,
                                               args.steps, args.warmup, keep=keep,
                                               tune=not args.no_tune, flags=ctx_flags)
         ok = all(r.status == 0 for r in res)
@@ -447,6 +450,7 @@ def main() -> int:
         forms["batched"] = {
             "value": b_value, "ms_per_batch": b_ms, "pairs_per_batch": args.batch,
             "workers": args.workers, "pairs_in_flight_per_bucket": btab,
+            "launch_shapes_per_bucket": bshp,
             "hw_queues": HW_QUEUES,
             "steps": args.steps, "warmup": args.warmup, "all_pairs_ok": ok,
             "pair0_bit_identical_to_plain": same_bits(dplain[0], diff),
@@ -456,7 +460,7 @@ def main() -> int:
         # the same batches in lockstep groups: every global solve of a group
         # (all its pairs' direction trials, then each clip-refit pass) is one
         # batched solve
-        l_ms, lres, dls, ltab = run_batched(ctx, params, pairs, bufs, args.workers,
+        l_ms, lres, dls, ltab, lshp = run_batched(ctx, params, pairs, bufs, args.workers,
                                             args.steps, args.warmup, keep=keep,
                                             tune=not args.no_tune,
                                             flags=ctx_flags | xbach.BATCH_LOCKSTEP)
@@ -473,13 +477,13 @@ def main() -> int:
         model = np.load(screen.MODEL_PATH) if screen.MODEL_PATH.exists() else None
         scr = xbach.FORM_SCREEN if model is not None else 0
         fform = xbach.FORM_MIXED | xbach.FORM_LDLT | xbach.FORM_REFINE | scr
-        f_ms, fres, dfull, ftab = run_batched(ctx, params, pairs, bufs, args.workers,
+        f_ms, fres, dfull, ftab, fshp = run_batched(ctx, params, pairs, bufs, args.workers,
                                               args.steps, args.warmup, form=fform, model=model,
                                               keep=keep, tune=not args.no_tune,
                                               flags=ctx_flags)
         # the same with the global solves in Numerical Recipes' order (ludcmp +
         # lubksb, the plain form's solve)
-        n_ms, nres, dnr, ntab = run_batched(ctx, params, pairs, bufs, args.workers,
+        n_ms, nres, dnr, ntab, nshp = run_batched(ctx, params, pairs, bufs, args.workers,
                                             args.steps, args.warmup,
                                             form=xbach.FORM_MIXED | scr, model=model, keep=keep,
                                             tune=not args.no_tune, flags=ctx_flags)
@@ -501,6 +505,7 @@ def main() -> int:
         full = {
             "value": f_value, "ms_per_batch": f_ms, "pairs_per_batch": args.batch,
             "workers": args.workers, "pairs_in_flight_per_bucket": ftab,
+            "launch_shapes_per_bucket": fshp,
             "form": "mixed+ldlt+refine" + ("+screened" if model is not None else ""),
             "steps": args.steps, "warmup": args.warmup, "all_pairs_ok": fok,
             "residual_rms_rel_vs_plain_max": float(max(rel)), "pairs_checked": len(rel),

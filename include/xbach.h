@@ -317,6 +317,24 @@ int xb_sync(xb_ctx* ctx);
 #define XB_FORM_LDLT 32
 int xb_set_form(xb_ctx* ctx, int form);
 
+/* ---- launch shapes ----------------------------------------------------------
+ * The tiling of the kernels where it is a free choice (no counterpart in the
+ * reference, whose kernels have fixed work-group sizes).  Every shape gives
+ * the same results bit for bit; the batch autotuner picks them per size
+ * bucket (xb_batch_autotune, xb_batch_get_shapes). */
+typedef struct xb_shapes {
+  int32_t conv_tiles;  /* 21x21 output tiles (one wave each) per convolution
+                          workgroup: 1, 2, 4 or 8 (default 4)                 */
+  int32_t mask_rows;   /* rows per 64-column tile of the input-mask pass: 16,
+                          32 or 64 (default 64)                               */
+  int32_t ldlt_group;  /* trailing tiles per load group of the L D L^T solve
+                          (XB_FORM_LDLT): 2 or 4 (default 4)                  */
+  int32_t pad;
+} xb_shapes;
+/* s == NULL restores the defaults; other values are refused (XB_ERR_ARG). */
+int xb_set_shapes(xb_ctx* ctx, const xb_shapes* s);
+int xb_get_shapes(xb_ctx* ctx, xb_shapes* s);
+
 /* ---- the learned substamp screen ------------------------------------------
  * Model: XB_MLP_NPARAM floats = feature mean[32], feature sd[32],
  * W1[64][32] (row = hidden unit), b1[64], w2[64], b2, threshold.  The network
@@ -452,6 +470,14 @@ int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, cons
                       void* const* d_diff, int reps, int* table);
 /* Screen model of every worker (as xb_screen_load). */
 int xb_batch_screen_load(xb_batch* b, const float* model);
+/* Launch shapes used for pairs of size bucket `bucket` (0..3, as the
+ * autotuner's table): the defaults until xb_batch_autotune has tuned that
+ * bucket (after its pairs-in-flight count, it tries each shape value in turn
+ * at that count, keeping one only when it is faster by 3 %), or
+ * xb_batch_set_shapes set them. */
+int xb_batch_get_shapes(xb_batch* b, int bucket, xb_shapes* s);
+/* The same shapes for every bucket (s == NULL: defaults). */
+int xb_batch_set_shapes(xb_batch* b, const xb_shapes* s);
 
 #ifdef __cplusplus
 }

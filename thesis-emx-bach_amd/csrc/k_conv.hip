@@ -246,8 +246,9 @@ __device__ __forceinline__ uint16_t after_flags(float t, double hi, double lo) {
   return m;
 }
 
-// The convolution: one wave per 21 x 21 output tile, CV_TILES tiles per
-// workgroup.  The tile's 41 x 41 input window is staged in LDS (FP32, the same
+// The convolution: one wave per 21 x 21 output tile, NT tiles per workgroup
+// (1, 2, 4 or 8: a launch shape the batch autotuner picks; every shape gives
+// the same bits).  The tile's 41 x 41 input window is staged in LDS (FP32, the same
 // clamped reads as the reference's edge handling).  Lane = (output row y,
 // column group g): 21 rows x 3 groups of CG = 7 adjacent columns, so each lane
 // computes 7 outputs of one row (lane 63 idles).  Output (x, y) sums
@@ -263,7 +264,6 @@ __device__ __forceinline__ uint16_t after_flags(float t, double hi, double lo) {
 constexpr int CTW = KW + 2 * HKW;  // 41: the tile's input window
 constexpr int CTP = CTW;           // its LDS pitch (odd: rows fall on distinct banks)
 constexpr int CG = 7;              // output columns per lane
-constexpr int CV_TILES = 4;        // tiles (waves) per workgroup
 static_assert(3 * CG == KW, "three column groups cover a tile row");
 
 typedef const __attribute__((address_space(4))) double* KConst;
@@ -299,13 +299,13 @@ __device__ __forceinline__ void conv_row_group(KConst K, const float (*win)[CTP]
 
 // FUSED (the mixed form): each term a fused multiply-add, in the same order;
 // otherwise the reference's unfused multiply, then add.
-template <bool FUSED>
-__global__ __launch_bounds__(WAVE * CV_TILES) void k_conv(ConvArgs A) {
-  __shared__ float win[CV_TILES][CTW][CTP];                 // each tile's input window
-  __shared__ unsigned long long rowFlag[CV_TILES][CTW];     // bit c: window pixel (l, c) flagged
+template <bool FUSED, int NT>
+__global__ __launch_bounds__(WAVE * NT) void k_conv(ConvArgs A) {
+  __shared__ float win[NT][CTW][CTP];                 // each tile's input window
+  __shared__ unsigned long long rowFlag[NT][CTW];     // bit c: window pixel (l, c) flagged
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / WAVE);
   const int lane = threadIdx.x % WAVE;
-  const int xs = blockIdx.x * CV_TILES + wave, ys = blockIdx.y;
+  const int xs = blockIdx.x * NT + wave, ys = blockIdx.y;
   const int x0 = HKW + xs * KW, y0t = HKW + ys * KW;
   const int w = A.w, h = A.h;
   const bool tileOn = xs < A.xSteps && x0 < w - HKW && y0t < h - HKW;
@@ -527,6 +527,23 @@ hipError_t launch_diag(Ctx& c, double* d_part, double* d_out) {
   return hipGetLastError();
 }
 
+template <bool FUSED, int NT>
+void launch_conv_shape(Ctx& c, const ConvArgs& A, int xSteps, int ySteps) {
+  hipLaunchKernelGGL((k_conv<FUSED, NT>), dim3((xSteps + NT - 1) / NT, ySteps), dim3(WAVE * NT),
+                     0, c.cur, A);
+}
+
+// tiles per workgroup from the context's launch shapes (checked when set)
+template <bool FUSED>
+void launch_conv_nt(Ctx& c, const ConvArgs& A, int xSteps, int ySteps) {
+  switch (c.shapes.conv_tiles) {
+    case 1: launch_conv_shape<FUSED, 1>(c, A, xSteps, ySteps); break;
+    case 2: launch_conv_shape<FUSED, 2>(c, A, xSteps, ySteps); break;
+    case 8: launch_conv_shape<FUSED, 8>(c, A, xSteps, ySteps); break;
+    default: launch_conv_shape<FUSED, 4>(c, A, xSteps, ySteps); break;
+  }
+}
+
 hipError_t launch_conv(Ctx& c) {
   const int xSteps = (c.w + KW - 1) / KW, ySteps = (c.h + KW - 1) / KW;
   const int nTiles = xSteps * ySteps;
@@ -560,12 +577,8 @@ hipError_t launch_conv(Ctx& c) {
   A.hi = c.m.threshHigh;
   A.lo = c.m.threshLow;
   prof_begin(c, XB_PROF_CONV);
-  if (c.form & XB_FORM_MIXED)
-    hipLaunchKernelGGL(k_conv<true>, dim3((xSteps + CV_TILES - 1) / CV_TILES, ySteps),
-                       dim3(WAVE * CV_TILES), 0, c.cur, A);
-  else
-    hipLaunchKernelGGL(k_conv<false>, dim3((xSteps + CV_TILES - 1) / CV_TILES, ySteps),
-                       dim3(WAVE * CV_TILES), 0, c.cur, A);
+  if (c.form & XB_FORM_MIXED) launch_conv_nt<true>(c, A, xSteps, ySteps);
+  else launch_conv_nt<false>(c, A, xSteps, ySteps);
   e = hipGetLastError();
   prof_end(c, XB_PROF_CONV);
   if (e != hipSuccess) return e;

@@ -1,7 +1,9 @@
 // k_init.hip — input masking (the reference's init stage).
 //
 // One pass replaces cl_kern/ini.cl:15-35 (maskInput) and 37-66 (spreadMask).
-// A workgroup of four waves makes a 64 x 64 output tile.  Each row of the
+// A workgroup of four waves makes a 64-column output tile of MT_Y rows (16,
+// 32 or 64: a launch shape the batch autotuner picks; every shape gives the
+// same bits).  Each row of the
 // tile plus its halo (w2 rows above and below, w2 columns either side) is
 // read once, every pixel's flags formed, and the row's BAD_INPUT pixels
 // collected as a bit mask by ballots; the horizontal spread over
@@ -13,7 +15,7 @@
 
 namespace xb {
 
-constexpr int MT_X = 64, MT_Y = 64, MAX_W2 = 8;
+constexpr int MT_X = 64, MAX_W2 = 8;
 constexpr int MT_WAVES = 4;
 
 // flags of pixel (x, y) from its two values (0 outside the image)
@@ -31,14 +33,14 @@ __device__ __forceinline__ uint16_t input_flags(float tf, float sf, int x, int y
   return m;
 }
 
-constexpr int MT_RPW = (MT_Y + 2 * MAX_W2 + MT_WAVES - 1) / MT_WAVES;  // rows per wave
-
 // scZero (block (0, 0)): the device scalars of the run, cleared here so the
 // pipeline's first kernel needs no separate fill
+template <int MT_Y>
 __global__ __launch_bounds__(WAVE * MT_WAVES) void k_mask_input(
     const float* __restrict__ T, const float* __restrict__ S, uint16_t* __restrict__ mask, int w,
     int h, int border, double hi, double lo, int w2, uint32_t* __restrict__ scZero,
     int scWords) {
+  constexpr int MT_RPW = (MT_Y + 2 * MAX_W2 + MT_WAVES - 1) / MT_WAVES;  // rows per wave
   __shared__ unsigned long long hor[MT_Y + 2 * MAX_W2];  // spread row masks
   __shared__ uint16_t own[MT_Y][MT_X];
   const int lane = threadIdx.x % WAVE;
@@ -104,10 +106,12 @@ hipError_t launch_mask_input(Ctx& c) {
   const int w2 = spreadWidth / 2;
   if (w2 < 0 || w2 > MAX_W2) return hipErrorInvalidValue;
   static_assert(sizeof(DevScalars) % 4 == 0, "device scalars are whole words");
-  dim3 grid((c.w + MT_X - 1) / MT_X, (c.h + MT_Y - 1) / MT_Y);
+  const int rows = c.shapes.mask_rows;  // 16, 32 or 64 (checked when set)
+  dim3 grid((c.w + MT_X - 1) / MT_X, (c.h + rows - 1) / rows);
+  auto* kern = rows == 16 ? k_mask_input<16> : (rows == 32 ? k_mask_input<32> : k_mask_input<64>);
   prof_begin(c, XB_PROF_MASK);
-  hipLaunchKernelGGL(k_mask_input, grid, dim3(WAVE * MT_WAVES), 0, c.cur, c.img[0], c.img[1],
-                     c.mask, c.w, c.h, border, c.m.threshHigh, c.m.threshLow, w2,
+  hipLaunchKernelGGL(kern, grid, dim3(WAVE * MT_WAVES), 0, c.cur, c.img[0], c.img[1], c.mask,
+                     c.w, c.h, border, c.m.threshHigh, c.m.threshLow, w2,
                      reinterpret_cast<uint32_t*>(c.sc), (int)(sizeof(DevScalars) / 4));
   prof_end(c, XB_PROF_MASK);
   return hipGetLastError();

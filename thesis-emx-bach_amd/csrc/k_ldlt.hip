@@ -50,7 +50,8 @@ constexpr int LT_WAVES = LT_THREADS / WAVE;
 constexpr int LT_BLOCK = LT_THREADS + WAVE;         // and the panel wave
 constexpr int LT_SPIN_MAX = 1 << 22;               // bound of the panel wave's wait
 constexpr int LT_LDW = LT_B + 1;  // LDS pitch of a panel row (doubles)
-constexpr int LT_GRP = 4;         // trailing tiles per load group
+// trailing tiles per load group: LT_GRP (2 or 4), a launch shape the batch
+// autotuner picks (the same products in the same order either way)
 constexpr int LT_TILES = LT_NB * (LT_NB + 1) / 2;  // 190 lower tiles
 constexpr int LT_RK = (MATSIZE + 7) / 8;  // residual terms per lane of a row
 constexpr double LT_TINY = 1.0e-20;  // an exact zero pivot, as ludcmp's TINY
@@ -338,6 +339,7 @@ __device__ __forceinline__ void lt_diag_block(double (*P)[LT_LDW], double* sd, d
   }
 }
 
+template <int LT_GRP>
 __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
   extern __shared__ double lt_dyn[];
   LtShared& sh = *reinterpret_cast<LtShared*>(lt_dyn);
@@ -625,39 +627,54 @@ __device__ __forceinline__ void ldlt_solve_dev(const LuSys& S, int refine) {
   }
 }
 
+template <int GRP>
 __global__ __launch_bounds__(LT_BLOCK) void k_ldlt_solve(LuSys S, int refine) {
-  ldlt_solve_dev(S, refine);
+  ldlt_solve_dev<GRP>(S, refine);
 }
 
+template <int GRP>
 __global__ __launch_bounds__(LT_BLOCK) void k_ldlt_solve_b(LuBatch B, int refine) {
-  ldlt_solve_dev(B.s[blockIdx.x], refine);
+  ldlt_solve_dev<GRP>(B.s[blockIdx.x], refine);
 }
 
+template <int GRP>
 static hipError_t lt_attr() {
-  static const hipError_t rc1 = hipFuncSetAttribute(
-      (const void*)k_ldlt_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LtShared));
-  static const hipError_t rc2 = hipFuncSetAttribute(
-      (const void*)k_ldlt_solve_b, hipFuncAttributeMaxDynamicSharedMemorySize,
-      (int)sizeof(LtShared));
+  static const hipError_t rc1 =
+      hipFuncSetAttribute((const void*)k_ldlt_solve<GRP>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LtShared));
+  static const hipError_t rc2 =
+      hipFuncSetAttribute((const void*)k_ldlt_solve_b<GRP>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LtShared));
   return rc1 != hipSuccess ? rc1 : rc2;
 }
 
-hipError_t launch_ldlt_solve(hipStream_t s, const LuSys& S, bool refine) {
-  const hipError_t e = lt_attr();
+template <int GRP>
+static hipError_t launch_ldlt_grp(hipStream_t s, const LuSys& S, bool refine) {
+  const hipError_t e = lt_attr<GRP>();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ldlt_solve, dim3(1), dim3(LT_BLOCK), sizeof(LtShared), s, S,
+  hipLaunchKernelGGL(k_ldlt_solve<GRP>, dim3(1), dim3(LT_BLOCK), sizeof(LtShared), s, S,
                      refine ? 1 : 0);
   return hipGetLastError();
 }
 
-hipError_t launch_ldlt_solve_batch(hipStream_t s, const LuBatch& B, bool refine) {
-  if (B.n <= 0) return hipSuccess;
-  if (B.n > LU_BATCH_MAX) return hipErrorInvalidValue;
-  const hipError_t e = lt_attr();
+template <int GRP>
+static hipError_t launch_ldlt_batch_grp(hipStream_t s, const LuBatch& B, bool refine) {
+  const hipError_t e = lt_attr<GRP>();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ldlt_solve_b, dim3(B.n), dim3(LT_BLOCK), sizeof(LtShared), s, B,
+  hipLaunchKernelGGL(k_ldlt_solve_b<GRP>, dim3(B.n), dim3(LT_BLOCK), sizeof(LtShared), s, B,
                      refine ? 1 : 0);
   return hipGetLastError();
+}
+
+hipError_t launch_ldlt_solve(hipStream_t s, const LuSys& S, bool refine, int grp) {
+  return grp == 2 ? launch_ldlt_grp<2>(s, S, refine) : launch_ldlt_grp<4>(s, S, refine);
+}
+
+hipError_t launch_ldlt_solve_batch(hipStream_t s, const LuBatch& B, bool refine, int grp) {
+  if (B.n <= 0) return hipSuccess;
+  if (B.n > LU_BATCH_MAX) return hipErrorInvalidValue;
+  return grp == 2 ? launch_ldlt_batch_grp<2>(s, B, refine)
+                  : launch_ldlt_batch_grp<4>(s, B, refine);
 }
 
 }  // namespace xb

@@ -973,8 +973,9 @@ static hipError_t enqueue_refine(hipStream_t s, const LuSet& L, double* d_out) {
 
 // The global solves of several systems (of different pairs) as one batch on
 // stream s: the same 23 launches as one solve, each covering every system.
-hipError_t launch_global_solve_batch(hipStream_t s, const LuBatch& B, bool ldlt, bool refine) {
-  if (ldlt) return launch_ldlt_solve_batch(s, B, refine);
+hipError_t launch_global_solve_batch(hipStream_t s, const LuBatch& B, bool ldlt, bool refine,
+                                     int ldltGroup) {
+  if (ldlt) return launch_ldlt_solve_batch(s, B, refine, ldltGroup);
   if (B.n <= 0) return hipSuccess;
   if (B.n > LU_BATCH_MAX) return hipErrorInvalidValue;
   const int nb = B.n;
@@ -1025,7 +1026,8 @@ hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int*
   hipError_t e = hipSuccess;
   if (c.form & XB_FORM_LDLT) {
     // one launch that factors, solves (and refines): nothing to capture
-    e = launch_ldlt_solve(c.cur, lu_sys(L, d_out, d_skipIfZero), (c.form & XB_FORM_REFINE) != 0);
+    e = launch_ldlt_solve(c.cur, lu_sys(L, d_out, d_skipIfZero), (c.form & XB_FORM_REFINE) != 0,
+                          c.shapes.ldlt_group);
     prof_end(c, XB_PROF_SOLVE);
     return e;
   }

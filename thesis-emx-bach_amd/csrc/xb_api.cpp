@@ -1296,6 +1296,21 @@ int xb_set_form(xb_ctx* x, int form) {
   return XB_OK;
 }
 
+int xb_set_shapes(xb_ctx* x, const xb_shapes* s) {
+  if (!x) return XB_ERR_ARG;
+  const xb_shapes v = s ? *s : default_shapes();
+  if (!shapes_valid(v)) return XB_ERR_ARG;
+  x->c.shapes = v;
+  x->c.shapes.pad = 0;
+  return XB_OK;
+}
+
+int xb_get_shapes(xb_ctx* x, xb_shapes* s) {
+  if (!x || !s) return XB_ERR_ARG;
+  *s = x->c.shapes;
+  return XB_OK;
+}
+
 int xb_screen_load(xb_ctx* x, const float* model) {
   if (!x || !model) return XB_ERR_ARG;
   Ctx& c = x->c;

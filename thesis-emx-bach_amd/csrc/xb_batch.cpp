@@ -29,6 +29,8 @@ struct xb_batch {
   int device = 0;
   int inflight = 0;                    // 0: all workers
   int tuned[XB_NBUCKET] = {0, 0, 0, 0};  // pairs in flight per size bucket (0: untuned)
+  xb_shapes shapes[XB_NBUCKET] = {xb::default_shapes(), xb::default_shapes(),
+                                  xb::default_shapes(), xb::default_shapes()};
   std::vector<xb_ctx*> ctx;
   std::string err;
   // lockstep mode (XB_BATCH_LOCKSTEP): the batched solves' stream and events
@@ -63,6 +65,7 @@ struct Job {
   void* const* cout;
   void* const* dout;
   xb_pair_result* res;
+  const xb_shapes* shapes = nullptr;  // launch shapes per size bucket
   std::vector<int> order;   // pair indices, largest first
   std::vector<int> status;  // per pair
   std::vector<char> ran;    // per pair: a worker took it
@@ -80,6 +83,7 @@ struct Job {
 // are read and written in the caller's buffers directly.
 int runPair(xb_ctx* x, const Job& J, int i, xb_pair_result* r) {
   xb::Ctx& c = x->c;
+  c.shapes = J.shapes[sizeBucket(J.w[i], J.h[i])];
   if (J.device) {
     int rc = xb::pipelineRunInto(c, J.p, J.w[i], J.h[i], J.tin[i], J.sin[i],
                                  J.cout ? J.cout[i] : nullptr, J.dout ? J.dout[i] : nullptr);
@@ -166,8 +170,10 @@ int lsSolve(xb_batch* b, const std::vector<int>& slots, const xb::LuBatch& B,
     }
   }
   const bool ldlt = (b->form & XB_FORM_LDLT) != 0;
-  if ((e = xb::launch_global_solve_batch(b->sstream, B, ldlt,
-                                         (b->form & XB_FORM_REFINE) != 0)) != hipSuccess) {
+  // the group's pairs share a size bucket's shapes (the first pair's)
+  const int grp = slots.empty() ? 4 : b->ctx[slots[0]]->c.shapes.ldlt_group;
+  if ((e = xb::launch_global_solve_batch(b->sstream, B, ldlt, (b->form & XB_FORM_REFINE) != 0,
+                                         grp)) != hipSuccess) {
     err = std::string("batched solve: ") + hipGetErrorString(e);
     return XB_ERR_HIP;
   }
@@ -219,6 +225,7 @@ void lsGroup(xb_batch* b, Job& J, const std::vector<int>& pairs) {
   for (int j = 0; j < g; j++) {
     const int i = pairs[j];
     xb_ctx* x = b->ctx[j];
+    x->c.shapes = J.shapes[sizeBucket(J.w[i], J.h[i])];
     int r = J.device ? xb_init_device(x, J.p, J.w[i], J.h[i], J.tin[i], J.sin[i])
                      : xb_init(x, J.p, J.w[i], J.h[i], static_cast<const float*>(J.tin[i]),
                                static_cast<const float*>(J.sin[i]));
@@ -363,6 +370,7 @@ int runBatch(xb_batch* b, Job& J) {
     }
   }
   if (J.n == 0) return XB_OK;
+  J.shapes = b->shapes;
   J.order.resize(J.n);
   std::iota(J.order.begin(), J.order.end(), 0);
   // largest pairs first (longest-processing-time order balances the workers)
@@ -505,19 +513,27 @@ int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, cons
     double best = 1e300;
     int bestC = 1;
     const int saved = b->inflight;
-    for (int c : cands) {
-      if (c > workers) break;
-      b->inflight = c;
-      double t = 1e300;
-      for (int r = 0; r < reps + 1; r++) {  // first run warms the contexts up
+    // best of reps timed runs after a warm-up run, at b's current settings
+    auto timeRuns = [&](double& t) -> int {
+      t = 1e300;
+      for (int r = 0; r < reps + 1; r++) {
         const double t0 = nowMs();
         const int rc = xb_batch_run_device(b, p, (int)idx.size(), bw.data(), bh.data(), bt.data(),
                                            bs.data(), bc.data(), bd.data(), nullptr);
-        if (rc) {
-          b->inflight = saved;
-          return rc;
-        }
+        if (rc) return rc;
         if (r > 0) t = std::min(t, nowMs() - t0);
+      }
+      return XB_OK;
+    };
+    b->shapes[bk] = xb::default_shapes();
+    for (int c : cands) {
+      if (c > workers) break;
+      b->inflight = c;
+      double t = 0.0;
+      const int rc = timeRuns(t);
+      if (rc) {
+        b->inflight = saved;
+        return rc;
       }
       if (t < best * 0.97) {  // a larger count must win by 3 %
         best = t;
@@ -525,11 +541,54 @@ int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, cons
       }
       if (c >= (int)idx.size()) break;
     }
+    // launch shapes at that count, one knob at a time from the defaults: a
+    // value other than the current one must win by 3 %
+    b->inflight = bestC;
+    b->shapes[bk] = xb::default_shapes();
+    struct Knob {
+      int32_t xb_shapes::*field;
+      std::vector<int> values;
+    };
+    std::vector<Knob> knobs = {{&xb_shapes::conv_tiles, {1, 2, 4, 8}},
+                               {&xb_shapes::mask_rows, {16, 32, 64}}};
+    if (b->form & XB_FORM_LDLT) knobs.push_back({&xb_shapes::ldlt_group, {2, 4}});
+    for (const Knob& k : knobs) {
+      const int cur = b->shapes[bk].*k.field;
+      for (int v : k.values) {
+        if (v == cur) continue;
+        const xb_shapes keep = b->shapes[bk];
+        b->shapes[bk].*k.field = v;
+        double t = 0.0;
+        const int rc = timeRuns(t);
+        if (rc) {
+          b->inflight = saved;
+          b->shapes[bk] = keep;
+          return rc;
+        }
+        if (t < best * 0.97) best = t;
+        else b->shapes[bk] = keep;
+      }
+    }
     b->inflight = saved;
     b->tuned[bk] = bestC;
   }
   if (table)
     for (int k = 0; k < XB_NBUCKET; k++) table[k] = b->tuned[k];
+  return XB_OK;
+}
+
+int xb_batch_get_shapes(xb_batch* b, int bucket, xb_shapes* s) {
+  if (!b || !s || bucket < 0 || bucket >= XB_NBUCKET) return XB_ERR_ARG;
+  *s = b->shapes[bucket];
+  return XB_OK;
+}
+
+int xb_batch_set_shapes(xb_batch* b, const xb_shapes* s) {
+  if (!b) return XB_ERR_ARG;
+  xb_shapes v = s ? *s : xb::default_shapes();
+  if (!xb::shapes_valid(v)) return XB_ERR_ARG;
+  v.pad = 0;
+  for (auto& q : b->shapes) q = v;
   return XB_OK;
 }
 

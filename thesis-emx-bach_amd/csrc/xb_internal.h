@@ -20,6 +20,25 @@
 
 namespace xb {
 
+// The launch shapes of kernels whose tiling is free (xb_set_shapes): every
+// shape gives the same results bit for bit.
+inline xb_shapes default_shapes() {
+  xb_shapes s;
+  s.conv_tiles = 4;
+  s.mask_rows = 64;
+  s.ldlt_group = 4;
+  s.pad = 0;
+  return s;
+}
+inline bool shapes_valid(const xb_shapes& s) {
+  const bool ct = s.conv_tiles == 1 || s.conv_tiles == 2 || s.conv_tiles == 4 ||
+                  s.conv_tiles == 8;
+  const bool mr = s.mask_rows == 16 || s.mask_rows == 32 || s.mask_rows == 64;
+  const bool lg = s.ldlt_group == 2 || s.ldlt_group == 4;
+  return ct && mr && lg;
+}
+
+
 // A HIP allocation or release made while another thread has a stream capture
 // open can invalidate that capture and leave its stream failing every later
 // call (hipFree synchronizes the device).  Every allocation, release and
@@ -238,6 +257,7 @@ struct Ctx {
   double* sol = nullptr;        // [NSOL] xb_lu_solve's output
   double* cdSol = nullptr;      // [2][NSOL]
   int form = 0;                 // XB_FORM_* of the next run
+  xb_shapes shapes = default_shapes();  // launch shapes (xb_set_shapes)
   ScreenBufs scr;
   int nScreened = 0;            // substamps the screen replaced (last run)
   int nRestored = 0;            // ... that the chi2 re-check kept (restored)
@@ -317,12 +337,14 @@ struct LuBatch {
 LuSys lu_sys(const LuSet& L, double* d_out, const int* d_skipIfZero);
 // ldlt: the systems are solved by launch_ldlt_solve_batch (XB_FORM_LDLT),
 // refined once when refine is set
+// (ldltGroup: the L D L^T solve's load group, 2 or 4)
 hipError_t launch_global_solve_batch(hipStream_t s, const LuBatch& B, bool ldlt = false,
-                                     bool refine = false);
+                                     bool refine = false, int ldltGroup = 4);
 // XB_FORM_LDLT (k_ldlt.hip): L D L^T without pivoting, with refine one step of
-// FP64 iterative refinement; one workgroup and one launch per system
-hipError_t launch_ldlt_solve(hipStream_t s, const LuSys& S, bool refine);
-hipError_t launch_ldlt_solve_batch(hipStream_t s, const LuBatch& B, bool refine);
+// FP64 iterative refinement; one workgroup and one launch per system; grp:
+// trailing tiles per load group (2 or 4)
+hipError_t launch_ldlt_solve(hipStream_t s, const LuSys& S, bool refine, int grp);
+hipError_t launch_ldlt_solve_batch(hipStream_t s, const LuBatch& B, bool refine, int grp);
 // Solves system set L.  d_skipIfZero: when it points at a zero count the solve
 // is skipped and d_out is zeroed (the reference returns before solving an
 // empty fit).

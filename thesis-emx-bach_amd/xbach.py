@@ -160,6 +160,21 @@ class Diag(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Shapes(C.Structure):
+    """xb_shapes: launch shapes of the kernels whose tiling is free (every
+    shape gives the same bits)."""
+    _fields_ = [("conv_tiles", C.c_int32), ("mask_rows", C.c_int32),
+                ("ldlt_group", C.c_int32), ("pad", C.c_int32)]
+
+    @classmethod
+    def make(cls, conv_tiles: int = 4, mask_rows: int = 64, ldlt_group: int = 4) -> "Shapes":
+        return cls(conv_tiles, mask_rows, ldlt_group, 0)
+
+    def as_dict(self) -> dict:
+        return {"conv_tiles": self.conv_tiles, "mask_rows": self.mask_rows,
+                "ldlt_group": self.ldlt_group}
+
+
 class SynthCfg(C.Structure):
     _fields_ = [
         ("w", C.c_int32), ("h", C.c_int32), ("nstars", C.c_int32),
@@ -339,6 +354,10 @@ def lib() -> C.CDLL:
     L.xb_fits_write.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, C.c_char_p, C.c_size_t]
     L.xb_sync.argtypes = [vp]
     L.xb_set_form.argtypes = [vp, C.c_int]
+    L.xb_set_shapes.argtypes = [vp, P(Shapes)]
+    L.xb_get_shapes.argtypes = [vp, P(Shapes)]
+    L.xb_batch_get_shapes.argtypes = [vp, C.c_int, P(Shapes)]
+    L.xb_batch_set_shapes.argtypes = [vp, P(Shapes)]
     L.xb_get_diagnostics.argtypes = [vp, P(Diag)]
     L.xb_screen_load.argtypes = [vp, vp]
     L.xb_batch_screen_load.argtypes = [vp, vp]
@@ -585,6 +604,16 @@ class Context:
         BF16 split precision); applies from the next init."""
         self._check(self._L.xb_set_form(self._h, form), "xb_set_form")
 
+    def set_shapes(self, shapes: "Shapes | None"):
+        """Launch shapes (None: defaults)."""
+        self._check(self._L.xb_set_shapes(self._h, C.byref(shapes) if shapes else None),
+                    "xb_set_shapes")
+
+    def shapes(self) -> "Shapes":
+        s = Shapes()
+        self._check(self._L.xb_get_shapes(self._h, C.byref(s)), "xb_get_shapes")
+        return s
+
     def screen_load(self, model: np.ndarray):
         m = np.ascontiguousarray(model, np.float32)
         if m.size != MLP_NPARAM:
@@ -739,6 +768,17 @@ class Batch:
 
     def set_inflight(self, n: int):
         self._check(self._L.xb_batch_set_inflight(self._h, n), "xb_batch_set_inflight")
+
+    def shapes(self, bucket: int) -> "Shapes":
+        """Launch shapes of a size bucket (the autotuner's, or the defaults)."""
+        s = Shapes()
+        self._check(self._L.xb_batch_get_shapes(self._h, bucket, C.byref(s)),
+                    "xb_batch_get_shapes")
+        return s
+
+    def set_shapes(self, shapes: "Shapes | None"):
+        self._check(self._L.xb_batch_set_shapes(self._h, C.byref(shapes) if shapes else None),
+                    "xb_batch_set_shapes")
 
     def run(self, params: Params, pairs, check: bool = True):
         """pairs: list of (tmpl, sci) host arrays.  Returns ([(conv, diff)], results)."""
