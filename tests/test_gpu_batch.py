@@ -201,6 +201,33 @@ def test_lockstep_refined_equals_single_refined(pairs, form):
         assert res[i].kernSum == ks, i
 
 
+@pytest.mark.parametrize("form", [xbach.FORM_PLAIN, xbach.FORM_MIXED | xbach.FORM_REFINE,
+                                  xbach.FORM_MIXED | xbach.FORM_LDLT | xbach.FORM_REFINE])
+def test_lockstep_graph_replays_equal_single_runs(pairs, form):
+    # each batched solve is captured into a hipGraph at its second occurrence
+    # and replayed after that: four runs of the same batch (the later ones
+    # replay every solve), with a lone pipeline run on another context between
+    # them, each bit-identical to the single runs, and no capture failed
+    p = xbach.Params.default()
+    use = pairs[:4]
+    ref = []
+    with xbach.Context(0) as c:
+        c.set_form(form)
+        for t, s in use:
+            conv, diff, _ = c.run(p, t, s)
+            ref.append((conv, diff, c.trace().kernSum))
+        with xbach.Batch(0, workers=4, flags=xbach.BATCH_LOCKSTEP) as b:
+            b.set_form(form)
+            for rep in range(4):
+                outs, res = b.run(p, use)
+                for i, (conv, diff, ks) in enumerate(ref):
+                    assert res[i].status == 0, (rep, i)
+                    assert res[i].captureFailures == 0, (rep, i)
+                    assert _same(outs[i][0], conv) and _same(outs[i][1], diff), (rep, i)
+                    assert res[i].kernSum == ks, (rep, i)
+                c.run(p, *use[rep % len(use)])  # eager work between the replays
+
+
 def test_lockstep_autotune(pairs, singles):
     p = xbach.Params.default()
     use = pairs[:-1]

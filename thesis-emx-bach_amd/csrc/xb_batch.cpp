@@ -39,6 +39,16 @@ struct xb_batch {
   hipEvent_t evSolve = nullptr;
   std::vector<hipEvent_t> evBuild;     // per context
   int form = 0;
+  // lockstep: each batched solve (its systems, form and shape) is captured
+  // into a hipGraph at its second occurrence and replayed from then on
+  struct SolveGraph {
+    xb::LuBatch key;
+    bool ldlt, refine;
+    int grp;
+    hipGraphExec_t exec;  // null while seen once
+  };
+  std::vector<SolveGraph> graphs;
+  int captureFailures = 0;  // failed captures (after two, solves launch directly)
 };
 
 namespace {
@@ -170,28 +180,84 @@ int lsSolve(xb_batch* b, const std::vector<int>& slots, const xb::LuBatch& B,
     }
   }
   const bool ldlt = (b->form & XB_FORM_LDLT) != 0;
+  const bool refine = (b->form & XB_FORM_REFINE) != 0;
   // the group's pairs share a size bucket's shapes (the first pair's)
   const int grp = slots.empty() ? 4 : b->ctx[slots[0]]->c.shapes.ldlt_group;
-  if ((e = xb::launch_global_solve_batch(b->sstream, B, ldlt, (b->form & XB_FORM_REFINE) != 0,
-                                         grp)) != hipSuccess) {
-    err = std::string("batched solve: ") + hipGetErrorString(e);
-    return XB_ERR_HIP;
-  }
-  if ((b->form & XB_FORM_REFINE) && !ldlt) {
-    // each system's refinement on the solve stream: the entry's system set is
-    // found among its pair's four (two trials, two clip-refit directions)
+  // the batched solve's kernels (and each system's refinement on the solve
+  // stream, its system set found among its pair's four: two trials, two
+  // clip-refit directions)
+  auto enqueue = [&]() -> hipError_t {
+    hipError_t q = xb::launch_global_solve_batch(b->sstream, B, ldlt, refine, grp);
+    if (q != hipSuccess || !refine || ldlt) return q;
     for (int k = 0; k < B.n; k++) {
       const xb::LuSet* L = nullptr;
       for (int j : slots) {
         xb::Ctx& c = b->ctx[j]->c;
-        for (const xb::LuSet* q : {&c.lu[0], &c.lu[1], &c.ksLu[0], &c.ksLu[1]})
-          if (q->M == B.s[k].M) L = q;
+        for (const xb::LuSet* r : {&c.lu[0], &c.lu[1], &c.ksLu[0], &c.ksLu[1]})
+          if (r->M == B.s[k].M) L = r;
       }
-      if (L && (e = xb::launch_refine(b->sstream, *L, B.s[k].out)) != hipSuccess) {
-        err = std::string("refinement: ") + hipGetErrorString(e);
-        return XB_ERR_HIP;
+      if (L && (q = xb::launch_refine(b->sstream, *L, B.s[k].out)) != hipSuccess) return q;
+    }
+    return hipSuccess;
+  };
+  // a solve seen before is replayed from its graph (captured at its second
+  // occurrence: the first is launched directly, which also sets the kernels'
+  // attributes outside any capture)
+  xb_batch::SolveGraph* sg = nullptr;
+  for (auto& g : b->graphs)
+    if (g.key.n == B.n && g.ldlt == ldlt && g.refine == refine && g.grp == grp &&
+        std::memcmp(g.key.s, B.s, sizeof(xb::LuSys) * B.n) == 0) {
+      sg = &g;
+      break;
+    }
+  constexpr size_t kMaxGraphs = 512;
+  if (!sg && b->graphs.size() < kMaxGraphs) {
+    b->graphs.push_back({B, ldlt, refine, grp, nullptr});
+    e = enqueue();
+  } else if (sg && !sg->exec && b->captureFailures < 2) {
+    hipGraph_t graph = nullptr;
+    {
+      // no thread allocates or frees while the capture is open
+      std::shared_lock<std::shared_mutex> lk(xb::alloc_lock());
+      e = hipStreamBeginCapture(b->sstream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        const hipError_t e1 = enqueue();
+        const hipError_t e2 = hipStreamEndCapture(b->sstream, &graph);
+        e = e1 != hipSuccess ? e1 : e2;
       }
     }
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess) {
+      std::unique_lock<std::shared_mutex> lk(xb::alloc_lock());
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e == hipSuccess) {
+      sg->exec = exec;
+      e = hipGraphLaunch(exec, b->sstream);
+    } else {
+      // nothing of the capture ran: end a capture left open, count the
+      // failure, and launch the kernels directly
+      b->captureFailures++;
+      (void)hipGetLastError();
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(b->sstream, &cs) == hipSuccess &&
+          cs != hipStreamCaptureStatusNone) {
+        hipGraph_t left = nullptr;
+        (void)hipStreamEndCapture(b->sstream, &left);
+        if (left) (void)hipGraphDestroy(left);
+        (void)hipGetLastError();
+      }
+      e = enqueue();
+    }
+  } else if (sg && sg->exec) {
+    e = hipGraphLaunch(sg->exec, b->sstream);
+  } else {
+    e = enqueue();
+  }
+  if (e != hipSuccess) {
+    err = std::string("batched solve: ") + hipGetErrorString(e);
+    return XB_ERR_HIP;
   }
   if ((e = hipEventRecord(b->evSolve, b->sstream)) != hipSuccess) {
     err = std::string("lockstep solve: ") + hipGetErrorString(e);
@@ -336,7 +402,7 @@ void lsGroup(xb_batch* b, Job& J, const std::vector<int>& pairs) {
         r.convTemplate = c.convTemplate;
         r.kscIterations = c.kscIterations;
         r.kernSum = c.kernSum;
-        r.captureFailures = c.captureFailures;
+        r.captureFailures = c.captureFailures + b->captureFailures;
       }
     }
     if (rc[j] && (J.firstBad < 0 || i < J.firstBad)) {
@@ -465,6 +531,8 @@ int xb_batch_create(int device, int workers, int flags, xb_batch** out) {
 int xb_batch_destroy(xb_batch* b) {
   if (!b) return XB_OK;
   if (b->sstream) (void)hipStreamSynchronize(b->sstream);
+  for (auto& g : b->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   for (auto* x : b->ctx) xb_ctx_destroy(x);
   for (auto e : b->evBuild)
     if (e) (void)hipEventDestroy(e);
