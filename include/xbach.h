@@ -433,8 +433,12 @@ typedef struct xb_pair_result {
  * solve: the 23 launches of a single solve, each covering all the group's
  * systems (up to 32, so up to 16 pairs per group).  The group size is the
  * pairs-in-flight count of the pair's size bucket (set_inflight / autotune).
- * Outputs are bit-identical to single runs.  Runs with XB_FORM_SCREEN or
- * XB_FORM_RECORD use the worker threads instead. */
+ * Each distinct batched solve (the same systems, form and launch shape) is
+ * captured into a hipGraph at its second occurrence and replayed from then on,
+ * unless the flags hold XB_CTX_EAGER (then its kernels launch one by one); a
+ * failed capture is counted in every pair's captureFailures.  Outputs are
+ * bit-identical to single runs.  Runs with XB_FORM_SCREEN or XB_FORM_RECORD
+ * use the worker threads instead. */
 #define XB_BATCH_LOCKSTEP 4
 int xb_batch_create(int device, int workers, int flags, xb_batch** out);
 int xb_batch_destroy(xb_batch* b);

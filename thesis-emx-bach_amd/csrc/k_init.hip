@@ -52,7 +52,9 @@ __global__ __launch_bounds__(WAVE * MT_WAVES) void k_mask_input(
   // halo columns: lane k < w2 holds x0 - w2 + k, lane w2 <= k < 2 w2 holds
   // x0 + 64 + k - w2
   const int hx = lane < w2 ? x0 - w2 + lane : x0 + MT_X + lane - w2;
-  const int xc = min(max(x0 + lane, 0), w - 1), hxc = min(max(hx, 0), w - 1);
+  // (lanes past the halo read their own column again: no extra traffic)
+  const int xc = min(max(x0 + lane, 0), w - 1);
+  const int hxc = lane < 2 * w2 ? min(max(hx, 0), w - 1) : xc;
   // every row's values of the wave loaded together (clamped addresses; the
   // flags test the true coordinates)
   float tv[MT_RPW], sv[MT_RPW], th[MT_RPW], sh[MT_RPW];

@@ -49,6 +49,7 @@ struct xb_batch {
   };
   std::vector<SolveGraph> graphs;
   int captureFailures = 0;  // failed captures (after two, solves launch directly)
+  bool eagerSolves = false; // XB_CTX_EAGER among the flags: no graphs
 };
 
 namespace {
@@ -211,7 +212,9 @@ int lsSolve(xb_batch* b, const std::vector<int>& slots, const xb::LuBatch& B,
       break;
     }
   constexpr size_t kMaxGraphs = 512;
-  if (!sg && b->graphs.size() < kMaxGraphs) {
+  if (b->eagerSolves) {
+    e = enqueue();
+  } else if (!sg && b->graphs.size() < kMaxGraphs) {
     b->graphs.push_back({B, ldlt, refine, grp, nullptr});
     e = enqueue();
   } else if (sg && !sg->exec && b->captureFailures < 2) {
@@ -511,6 +514,7 @@ int xb_batch_create(int device, int workers, int flags, xb_batch** out) {
     }
     b->ctx.push_back(x);
   }
+  b->eagerSolves = (flags & XB_CTX_EAGER) != 0;
   if (flags & XB_BATCH_LOCKSTEP) {
     b->lockstep = true;
     b->evBuild.assign(workers, nullptr);
