@@ -58,7 +58,7 @@ PEAK_HBM_GBS = 8000.0
 # and WRITE_SIZE of k_conv, each scaled by what the same counter reads for a
 # device copy of known size (bench.py --copy-calibration); the summary with
 # both commands is profiles/k_conv_2048_traffic.md.
-CONV_TRAFFIC_BYTES = 163_815_488
+CONV_TRAFFIC_BYTES = 179_734_812
 # HBM bytes per global solve (the 23 kernels k_lu_scale, k_lu_begin, 19 x
 # k_lu_step, k_lu_backsub), measured the same way; profiles/lu_solve_traffic.md.
 LU_TRAFFIC_BYTES = 23_401_549

@@ -279,7 +279,9 @@ int xb_sync(xb_ctx* ctx);
  * the matrix cores in BF16 split precision (hi + lo) with FP32 accumulation
  * for every kept stamp (the clip-refit loop's refills of a few stamps take the
  * FP64 fill, faster for a short list); the stamp and global LU solves, chi2,
- * convolution and difference stay FP64.
+ * convolution and difference stay FP64, the convolution's terms as fused
+ * multiply-adds (in the reference's order) and the tile kernels as one FP64
+ * matrix-core product per 16 tiles.
  * Results differ from the plain form at the 1e-5 level of W and Q; the
  * selected substamps (sss) are the same.  Applies from the next xb_init. */
 #define XB_FORM_PLAIN 0
