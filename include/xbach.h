@@ -331,7 +331,9 @@ typedef struct xb_shapes {
                           32 or 64 (default 64)                               */
   int32_t ldlt_group;  /* trailing tiles per load group of the L D L^T solve
                           (XB_FORM_LDLT): 2 or 4 (default 4)                  */
-  int32_t pad;
+  int32_t find_staged; /* 1: the substamp search stages each stamp in LDS when
+                          it fits (up to ~158 KB, one workgroup per CU); 0:
+                          it reads the image and mask in place (default 1)    */
 } xb_shapes;
 /* s == NULL restores the defaults; other values are refused (XB_ERR_ARG). */
 int xb_set_shapes(xb_ctx* ctx, const xb_shapes* s);

@@ -1074,7 +1074,7 @@ hipError_t launch_sss_find(Ctx& c, int im) {
   const size_t nAll = (size_t)c.fsw * c.fsw;
   const size_t lds = ((nAll + 63) / 64 * 64) * sizeof(float) + (nAll + 63) / 64 * 8;
   prof_begin(c, XB_PROF_SSS);
-  if (lds <= FIND_LDS_MAX) {
+  if (c.shapes.find_staged && lds <= FIND_LDS_MAX) {
     // once per process (the attribute belongs to the kernel); the static's
     // initialisation is thread-safe, so batch workers may race here
     static const hipError_t attrRc = hipFuncSetAttribute(

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "xb_internal.h"
@@ -286,15 +287,25 @@ int ensureStamps(Ctx& c, int N, int fsw) {
   return XB_OK;
 }
 
+// Waits for an event by polling: the scalar read-backs between passes are on
+// the critical path of every pair, and a blocking wait adds ~40 us of wake-up.
+// The polling thread yields its core between polls after the first few, so
+// many contexts waiting at once (batch workers) leave the host's cores to the
+// threads that have work.
+hipError_t spinEvent(hipEvent_t ev) {
+  hipError_t e;
+  for (int n = 0; (e = hipEventQuery(ev)) == hipErrorNotReady; n++)
+    if (n >= 32) std::this_thread::yield();
+  return e;
+}
+
 // Waits for the work queued on the primary stream by polling an event: the
 // scalar read-backs between stages are on the critical path of every pair, and
 // a blocking wait added ~40 us of wake-up to each.
 hipError_t spinSync(Ctx& c) {
   hipError_t e = hipEventRecord(c.evSync, c.stream);
   if (e != hipSuccess) return e;
-  while ((e = hipEventQuery(c.evSync)) == hipErrorNotReady) {
-  }
-  return e;
+  return spinEvent(c.evSync);
 }
 
 // Copies `bytes` of the device scalars at byte offset `off` into the pinned
@@ -592,14 +603,6 @@ int ensureScreen(Ctx& c) {
   return XB_OK;
 }
 
-// Waits for an event by polling: the scalar read-backs between passes are on
-// the critical path of every pair, and a blocking wait adds ~40 us of wake-up.
-hipError_t spinEvent(hipEvent_t ev) {
-  hipError_t e;
-  while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
-  }
-  return e;
-}
 
 constexpr size_t kscOffset(int im) { return offsetof(DevScalars, k) + im * sizeof(KscScalars); }
 
@@ -1301,7 +1304,6 @@ int xb_set_shapes(xb_ctx* x, const xb_shapes* s) {
   const xb_shapes v = s ? *s : default_shapes();
   if (!shapes_valid(v)) return XB_ERR_ARG;
   x->c.shapes = v;
-  x->c.shapes.pad = 0;
   return XB_OK;
 }
 

@@ -27,7 +27,7 @@ inline xb_shapes default_shapes() {
   s.conv_tiles = 4;
   s.mask_rows = 64;
   s.ldlt_group = 4;
-  s.pad = 0;
+  s.find_staged = 1;
   return s;
 }
 inline bool shapes_valid(const xb_shapes& s) {
@@ -35,7 +35,8 @@ inline bool shapes_valid(const xb_shapes& s) {
                   s.conv_tiles == 8;
   const bool mr = s.mask_rows == 16 || s.mask_rows == 32 || s.mask_rows == 64;
   const bool lg = s.ldlt_group == 2 || s.ldlt_group == 4;
-  return ct && mr && lg;
+  const bool fs = s.find_staged == 0 || s.find_staged == 1;
+  return ct && mr && lg && fs;
 }
 
 
