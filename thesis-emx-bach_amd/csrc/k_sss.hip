@@ -45,6 +45,7 @@ __host__ __device__ __forceinline__ unsigned fdiv_magic(int d) { return 0xFFFFFF
 
 constexpr int MAX_CLIP_BLOCKS = 2560;  // ceil(fsw^2 / 32) for fsw <= 286
 constexpr int MAX_LEVELS = 32;         // findSubStamps threshold levels
+constexpr int STATS_THREADS = 1024;    // k_sss_stats workgroup (at most 16 waves)
 
 
 // Block-wide helpers of k_sss_stats (256 threads).
@@ -125,7 +126,8 @@ __device__ __forceinline__ double ordered_sum(const double* a, int n) {
   return s;
 }
 
-__global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img,
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sss_stats(const float* __restrict__ img,
                                                    uint16_t* __restrict__ mask, int w, int h,
                                                    int nx, int ny, int fsw,
                                                    const double* __restrict__ ran1,
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
   const int nAll = fsw * fsw;
   const unsigned mf = fdiv_magic(fsw);
   const int lane = lane_id(), wid = tid / WAVE;
-  constexpr int NWS = 256 / WAVE;
+  constexpr int NWS = NT / WAVE;
   const int seg = ((nAll + NWS - 1) / NWS + WAVE - 1) / WAVE * WAVE;
   const int k0 = wid * seg, k1 = min(nAll, k0 + seg);
   __shared__ int wcnt[NWS];
@@ -383,8 +385,8 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
     }
     {
       int tot;
-      const int pre = block_prefix_sum(bins[tid], isc, &tot);
-      hC[tid] = pre;
+      const int pre = block_prefix_sum(tid < 256 ? bins[tid] : 0, isc, &tot);
+      if (tid < 256) hC[tid] = pre;
       if (tid == 0) hC[256] = tot;
       __syncthreads();
     }
@@ -402,7 +404,7 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
       V = lo;
     }
     const int Uincl = max(1, block_prefix_max_incl(V, isc));
-    hU[tid] = Uincl;
+    if (tid < 256) hU[tid] = Uincl;
     __syncthreads();
     double dens = -INFINITY;
     if (tid >= 1 && tid <= 254) {
@@ -417,7 +419,7 @@ __global__ __launch_bounds__(256) void k_sss_stats(const float* __restrict__ img
       // every thread took part, so hdv holds the per-wave maxima: no start
       // had a positive density when all of them are -inf
       bool any = false;
-      for (int i = 0; i < 4; i++) any = any || hdv[i] > -INFINITY;
+      for (int i = 0; i < NT / WAVE; i++) any = any || hdv[i] > -INFINITY;
       if (!any) maxIndex = -1;
     }
     if (maxIndex < 0 || maxIndex > 255) maxIndex = 0;
@@ -1040,7 +1042,8 @@ __global__ void k_reset_skip(uint16_t* mask, size_t n) {
 hipError_t launch_sss_stats(Ctx& c, int im) {
   if ((c.fsw * c.fsw + 31) / 32 > MAX_CLIP_BLOCKS) return hipErrorInvalidValue;
   prof_begin(c, XB_PROF_SSS);
-  hipLaunchKernelGGL(k_sss_stats, dim3(c.nGrid), dim3(256), 0, c.cur, c.img[im], c.mask, c.w,
+  hipLaunchKernelGGL(k_sss_stats<STATS_THREADS>, dim3(c.nGrid), dim3(STATS_THREADS), 0, c.cur,
+                     c.img[im], c.mask, c.w,
                      c.h, c.stampsx, c.stampsy, c.fsw, c.ran1, c.goodPix, c.m.sigClipAlpha,
                      c.m.iqRange, c.st[im].sky, c.st[im].fwhm);
   prof_end(c, XB_PROF_SSS);
