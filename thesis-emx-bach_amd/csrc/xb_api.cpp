@@ -607,13 +607,15 @@ int ensureScreen(Ctx& c) {
 constexpr size_t kscOffset(int im) { return offsetof(DevScalars, k) + im * sizeof(KscScalars); }
 
 // One pass of the clip-refit loop that convolves image im (fitKernel +
-// checkFitSolution + the refill, src/cdkscUtil.cpp:585-684) on c.cur: the
-// global fit, every stamp's chi2, the clip check (which moves rejected stamps
-// to their next substamp), the screen, the refill; then the pass's check flag
-// (with the refill, chi2 and screened counts beside it) is copied to the
-// pinned mirror and evK[im] recorded.  waitCd (may be null):
-// cd's last read of image im's fit data, which the check and the refill must
-// follow.
+// checkFitSolution, src/cdkscUtil.cpp:585-684) on c.cur: the global fit, every
+// stamp's chi2, the clip check (which moves rejected stamps to their next
+// substamp and lists them), the screen; then the pass's check flag (with the
+// refill list's length, chi2 and screened counts beside it) is copied to the
+// pinned mirror and evK[im] recorded.  The refill of the listed stamps
+// (kscRefill) is launched by the caller once it has read that length, before
+// the next pass, so its grid covers exactly the listed stamps and a converged
+// pass (empty list) launches none.  waitCd (may be null): cd's last read of
+// image im's fit data, which the check and the refill must follow.
 int kscPass(Ctx& c, int im, int it, bool keep, bool screen, uint16_t nanMark,
             hipEvent_t waitCd) {
   double* sol = c.ksSol + (size_t)im * NSOL;
@@ -628,14 +630,23 @@ int kscPass(Ctx& c, int im, int it, bool keep, bool screen, uint16_t nanMark,
                           rec ? c.scr.passCur + (size_t)it * c.nKept[im] : nullptr,
                           (screen && it == 0) ? c.scr.bits : nullptr));
   if (rec) XB_HIP(launch_screen(c, im, it, screen));
-  XB_HIP(launch_fill(c, im, c.st[im].listK, &c.sc->k[im].nList, c.nKept[im]));
   static_assert(offsetof(KscScalars, nScreened) == offsetof(KscScalars, check) + 3 * sizeof(int),
                 "check .. nScreened are one copy");
+  static_assert(offsetof(KscScalars, nList) == offsetof(KscScalars, check) + sizeof(int),
+                "the list's length is in that copy");
   const size_t off = kscOffset(im) + offsetof(KscScalars, check);
   XB_HIP(hipMemcpyAsync(reinterpret_cast<char*>(c.hsc) + off,
                         reinterpret_cast<const char*>(c.sc) + off, 4 * sizeof(int),
                         hipMemcpyDeviceToHost, c.cur));
   XB_HIP(hipEventRecord(c.evK[im], c.cur));
+  return XB_OK;
+}
+
+// The refill of the last pass of image im's loop (its list's length read
+// back with its check flag), on c.cur before the next pass.
+int kscRefill(Ctx& c, int im) {
+  const int n = c.hsc->k[im].nList;
+  if (n > 0) XB_HIP(launch_fill(c, im, c.st[im].listK, &c.sc->k[im].nList, n));
   return XB_OK;
 }
 
@@ -695,7 +706,11 @@ int stageKsc(Ctx& c) {
       c.err = "ksc loop did not terminate";
       return XB_ERR_HIP;
     }
-    if (c.hsc->k[im].check) continue;
+    if (c.hsc->k[im].check) {
+      rc = kscRefill(c, im);
+      if (rc) return rc;
+      continue;
+    }
     // nothing screened (the count is cumulative): no substamp to re-check
     if (!screen || c.hsc->k[im].nScreened == 0) break;
     bool restored = false;
@@ -847,7 +862,8 @@ int stageCdKsc(Ctx& c) {
         return XB_ERR_HIP;
       }
       c.cur = c.kstream[d];
-      int rc = kscPass(c, d, it[d], false, false, mask_pend(d), nullptr);
+      int rc = kscRefill(c, d);
+      if (!rc) rc = kscPass(c, d, it[d], false, false, mask_pend(d), nullptr);
       if (rc) {
         c.cur = c.stream;
         return rc;
