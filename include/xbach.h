@@ -334,6 +334,9 @@ typedef struct xb_shapes {
   int32_t find_staged; /* 1: the substamp search stages each stamp in LDS when
                           it fits (up to ~158 KB, one workgroup per CU); 0:
                           it reads the image and mask in place (default 1)    */
+  int32_t stats_threads; /* threads per stamp of the stamp statistics: 256 or
+                            1024 (default 1024)                               */
+  int32_t pad;
 } xb_shapes;
 /* s == NULL restores the defaults; other values are refused (XB_ERR_ARG). */
 int xb_set_shapes(xb_ctx* ctx, const xb_shapes* s);

@@ -1,7 +1,7 @@
 """Launch shapes (xb_set_shapes, xb_batch_autotune): the tiling of the
 convolution (tiles per workgroup), the input-mask pass (rows per tile), the
-L D L^T solve (trailing tiles per load group) and whether the substamp search
-stages each stamp in LDS are free.
+L D L^T solve (trailing tiles per load group), whether the substamp search
+stages each stamp in LDS, and the stamp statistics' threads per stamp are free.
 
 Bar: every shape gives the default shape's convolved image, difference image
 and mask bit for bit, in the plain form and in the fast form (mixed cmv,
@@ -58,6 +58,7 @@ def test_every_shape_gives_the_same_bits(ctx, pair, form):
     variants = [xbach.Shapes.make(conv_tiles=ct) for ct in (1, 2, 8)]
     variants += [xbach.Shapes.make(mask_rows=mr) for mr in (16, 32)]
     variants.append(xbach.Shapes.make(find_staged=0))
+    variants.append(xbach.Shapes.make(stats_threads=256))
     if form == FAST:
         variants.append(xbach.Shapes.make(ldlt_group=2))
     variants.append(xbach.Shapes.make(conv_tiles=1, mask_rows=16, ldlt_group=2))
@@ -71,7 +72,7 @@ def test_every_shape_gives_the_same_bits(ctx, pair, form):
 def test_invalid_shapes_are_refused(ctx):
     for bad in (xbach.Shapes.make(conv_tiles=3), xbach.Shapes.make(mask_rows=48),
                 xbach.Shapes.make(ldlt_group=8), xbach.Shapes.make(conv_tiles=0),
-                xbach.Shapes.make(find_staged=2)):
+                xbach.Shapes.make(find_staged=2), xbach.Shapes.make(stats_threads=512)):
         with pytest.raises(xbach.XbachError):
             ctx.set_shapes(bad)
     assert ctx.shapes().as_dict() == xbach.Shapes.make().as_dict()
@@ -105,6 +106,7 @@ def test_autotuned_shapes_are_valid_and_runs_equal_single_runs():
         assert sh["mask_rows"] in (16, 32, 64)
         assert sh["ldlt_group"] in (2, 4)
         assert sh["find_staged"] in (0, 1)
+        assert sh["stats_threads"] in (256, 1024)
         for k in (1, 2, 3):
             assert b.shapes(k).as_dict() == xbach.Shapes.make().as_dict()
         res = b.run_device(p, ws, hs, *cols)

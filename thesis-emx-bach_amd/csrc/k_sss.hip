@@ -45,7 +45,6 @@ __host__ __device__ __forceinline__ unsigned fdiv_magic(int d) { return 0xFFFFFF
 
 constexpr int MAX_CLIP_BLOCKS = 2560;  // ceil(fsw^2 / 32) for fsw <= 286
 constexpr int MAX_LEVELS = 32;         // findSubStamps threshold levels
-constexpr int STATS_THREADS = 1024;    // k_sss_stats workgroup (at most 16 waves)
 
 
 // Block-wide helpers of k_sss_stats (256 threads).
@@ -1042,10 +1041,15 @@ __global__ void k_reset_skip(uint16_t* mask, size_t n) {
 hipError_t launch_sss_stats(Ctx& c, int im) {
   if ((c.fsw * c.fsw + 31) / 32 > MAX_CLIP_BLOCKS) return hipErrorInvalidValue;
   prof_begin(c, XB_PROF_SSS);
-  hipLaunchKernelGGL(k_sss_stats<STATS_THREADS>, dim3(c.nGrid), dim3(STATS_THREADS), 0, c.cur,
-                     c.img[im], c.mask, c.w,
-                     c.h, c.stampsx, c.stampsy, c.fsw, c.ran1, c.goodPix, c.m.sigClipAlpha,
-                     c.m.iqRange, c.st[im].sky, c.st[im].fwhm);
+  // threads per stamp from the context's launch shapes (256 or 1024)
+  if (c.shapes.stats_threads == 256)
+    hipLaunchKernelGGL(k_sss_stats<256>, dim3(c.nGrid), dim3(256), 0, c.cur, c.img[im], c.mask,
+                       c.w, c.h, c.stampsx, c.stampsy, c.fsw, c.ran1, c.goodPix,
+                       c.m.sigClipAlpha, c.m.iqRange, c.st[im].sky, c.st[im].fwhm);
+  else
+    hipLaunchKernelGGL(k_sss_stats<1024>, dim3(c.nGrid), dim3(1024), 0, c.cur, c.img[im], c.mask,
+                       c.w, c.h, c.stampsx, c.stampsy, c.fsw, c.ran1, c.goodPix,
+                       c.m.sigClipAlpha, c.m.iqRange, c.st[im].sky, c.st[im].fwhm);
   prof_end(c, XB_PROF_SSS);
   return hipGetLastError();
 }

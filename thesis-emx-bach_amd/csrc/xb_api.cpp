@@ -1320,6 +1320,7 @@ int xb_set_shapes(xb_ctx* x, const xb_shapes* s) {
   const xb_shapes v = s ? *s : default_shapes();
   if (!shapes_valid(v)) return XB_ERR_ARG;
   x->c.shapes = v;
+  x->c.shapes.pad = 0;
   return XB_OK;
 }
 

@@ -623,7 +623,8 @@ int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, cons
     };
     std::vector<Knob> knobs = {{&xb_shapes::conv_tiles, {1, 2, 4, 8}},
                                {&xb_shapes::mask_rows, {16, 32, 64}},
-                               {&xb_shapes::find_staged, {0, 1}}};
+                               {&xb_shapes::find_staged, {0, 1}},
+                               {&xb_shapes::stats_threads, {256, 1024}}};
     if (b->form & XB_FORM_LDLT) knobs.push_back({&xb_shapes::ldlt_group, {2, 4}});
     for (const Knob& k : knobs) {
       const int cur = b->shapes[bk].*k.field;
@@ -658,8 +659,9 @@ int xb_batch_get_shapes(xb_batch* b, int bucket, xb_shapes* s) {
 
 int xb_batch_set_shapes(xb_batch* b, const xb_shapes* s) {
   if (!b) return XB_ERR_ARG;
-  const xb_shapes v = s ? *s : xb::default_shapes();
+  xb_shapes v = s ? *s : xb::default_shapes();
   if (!xb::shapes_valid(v)) return XB_ERR_ARG;
+  v.pad = 0;
   for (auto& q : b->shapes) q = v;
   return XB_OK;
 }

@@ -28,6 +28,8 @@ inline xb_shapes default_shapes() {
   s.mask_rows = 64;
   s.ldlt_group = 4;
   s.find_staged = 1;
+  s.stats_threads = 1024;
+  s.pad = 0;
   return s;
 }
 inline bool shapes_valid(const xb_shapes& s) {
@@ -36,7 +38,8 @@ inline bool shapes_valid(const xb_shapes& s) {
   const bool mr = s.mask_rows == 16 || s.mask_rows == 32 || s.mask_rows == 64;
   const bool lg = s.ldlt_group == 2 || s.ldlt_group == 4;
   const bool fs = s.find_staged == 0 || s.find_staged == 1;
-  return ct && mr && lg && fs;
+  const bool st = s.stats_threads == 256 || s.stats_threads == 1024;
+  return ct && mr && lg && fs && st;
 }
 
 

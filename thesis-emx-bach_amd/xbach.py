@@ -164,16 +164,18 @@ class Shapes(C.Structure):
     """xb_shapes: launch shapes of the kernels whose tiling is free (every
     shape gives the same bits)."""
     _fields_ = [("conv_tiles", C.c_int32), ("mask_rows", C.c_int32),
-                ("ldlt_group", C.c_int32), ("find_staged", C.c_int32)]
+                ("ldlt_group", C.c_int32), ("find_staged", C.c_int32),
+                ("stats_threads", C.c_int32), ("pad", C.c_int32)]
 
     @classmethod
     def make(cls, conv_tiles: int = 4, mask_rows: int = 64, ldlt_group: int = 4,
-             find_staged: int = 1) -> "Shapes":
-        return cls(conv_tiles, mask_rows, ldlt_group, find_staged)
+             find_staged: int = 1, stats_threads: int = 1024) -> "Shapes":
+        return cls(conv_tiles, mask_rows, ldlt_group, find_staged, stats_threads, 0)
 
     def as_dict(self) -> dict:
         return {"conv_tiles": self.conv_tiles, "mask_rows": self.mask_rows,
-                "ldlt_group": self.ldlt_group, "find_staged": self.find_staged}
+                "ldlt_group": self.ldlt_group, "find_staged": self.find_staged,
+                "stats_threads": self.stats_threads}
 
 
 class SynthCfg(C.Structure):
