@@ -458,7 +458,10 @@ int xb_batch_run(xb_batch* b, const xb_params* p, int n, const int* w, const int
                  const float* const* tmpl, const float* const* sci, float* const* conv,
                  float* const* diff, xb_pair_result* results);
 /* Same with device-resident inputs and outputs (device pointers from
- * xb_buf_alloc or any HIP allocation on the batch's device). */
+ * xb_buf_alloc or any HIP allocation on the batch's device).  The pipeline
+ * reads the inputs and writes the outputs in place (no copies): they must stay
+ * allocated and unchanged until the call returns, and a pair's outputs must not
+ * overlap its inputs. */
 int xb_batch_run_device(xb_batch* b, const xb_params* p, int n, const int* w, const int* h,
                         const void* const* d_tmpl, const void* const* d_sci,
                         void* const* d_conv, void* const* d_diff, xb_pair_result* results);

@@ -119,6 +119,23 @@ def test_batch_argument_errors():
             b.run(xbach.Params.default(fKernelWidth=19), [(t, t)])
         outs, res = b.run(p, [])
         assert outs == [] and res == []
+    # device pairs are read and written in place: an output that overlaps an
+    # input is refused for that pair (XB_ERR_ARG), the other pair still runs
+    t, s = xbach.synth_pair(xbach.synth_cfg("pair512"))
+    with xbach.Context(0) as c, xbach.Batch(0, workers=2) as b:
+        q = [c.buf_alloc(t.nbytes) for _ in range(6)]
+        try:
+            for k in (0, 3):
+                c.buf_upload(q[k], t)
+                c.buf_upload(q[k + 1], s)
+            w, h = [t.shape[1]] * 2, [t.shape[0]] * 2
+            res = b.run_device(p, w, h, [q[0], q[3]], [q[1], q[4]], [q[2], q[5]],
+                               [q[1], None], check=False)  # pair 0's difference over its science
+            assert res[0].status == 1, res[0].status
+            assert res[1].status == 0, res[1].status
+        finally:
+            for d in q:
+                c.buf_free(d)
 
 
 def test_batch_autotune(pairs, singles):

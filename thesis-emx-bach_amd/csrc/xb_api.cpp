@@ -1699,7 +1699,19 @@ namespace xb {
 // drained; c.kernSum holds the kernel sum.
 int pipelineRunInto(Ctx& c, const xb_params* p, int w, int h, const void* tin, const void* sin,
                     void* cout, void* dout) {
-  if (!tin || !sin) return XB_ERR_ARG;
+  if (!tin || !sin || !p || w <= 0 || h <= 0) return XB_ERR_ARG;
+  // an output written in place must not overlap an input
+  const size_t bytes = (size_t)w * h * sizeof(float);
+  auto overlap = [bytes](const void* a, const void* b) {
+    const char* x = static_cast<const char*>(a);
+    const char* y = static_cast<const char*>(b);
+    return a && b && x < y + bytes && y < x + bytes;
+  };
+  if (overlap(cout, tin) || overlap(cout, sin) || overlap(dout, tin) || overlap(dout, sin) ||
+      overlap(cout, dout)) {
+    c.err = "a pair's output buffer overlaps one of its inputs or the other output";
+    return XB_ERR_ARG;
+  }
   int rc = setDev(c);
   if (rc) return rc;
   rc = stageInitCommon(c, p, w, h);
