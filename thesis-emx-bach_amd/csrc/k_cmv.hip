@@ -114,8 +114,9 @@ __device__ __forceinline__ void fill_conv_row(const FillArgs& A, FillConvLds& L,
 // covers FILL_LIST_STAMPS entries and loops over the rest (the list's length
 // is on the device), so a short list does not dispatch a workgroup per kept
 // stamp.
-template <bool LOOP>
-__global__ __launch_bounds__(256) void k_fill_conv(FillArgs A) {
+template <bool LOOP, int NI>
+__global__ __launch_bounds__(256) void k_fill_conv(ArgsN<FillArgs, NI> P) {
+  const FillArgs& A = args_of(P);
   __shared__ FillConvLds L;
   for (int k = blockIdx.x;; k += gridDim.x) {
     int s, ssx, ssy;
@@ -462,8 +463,9 @@ __device__ __forceinline__ void gram_item(const FillArgs& A, int s, int ssx, int
 // nCover: list entries covered by one pass of the grid (the kept stamps for
 // a full fill, FILL_LIST_STAMPS at most for a listed refill, LOOP); later
 // entries are taken by further passes.
-template <bool LOOP>
-__global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nCover) {
+template <bool LOOP, int NI>
+__global__ __launch_bounds__(64) void k_fill_gram(ArgsN<FillArgs, NI> P, int nCover) {
+  const FillArgs& A = args_of(P);
   const int total = nCover * NGI;
   const int per = (total + 7) / 8;
   const int wk = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
@@ -478,12 +480,10 @@ __global__ __launch_bounds__(64) void k_fill_gram(FillArgs A, int nCover) {
   }
 }
 
-hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, int nMax) {
-  if (nMax <= 0) return hipSuccess;
-  const int src = im, tgt = 1 - im;
+static FillArgs fill_args(Ctx& c, int im, const int* d_list, const int* d_nList) {
   FillArgs A;
-  A.img = c.img[src];
-  A.img2 = c.img[tgt];
+  A.img = c.img[im];
+  A.img2 = c.img[1 - im];
   A.w = c.w;
   A.h = c.h;
   A.list = d_list;
@@ -499,31 +499,52 @@ hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, in
   A.G = c.st[im].G;
   A.Bc = c.st[im].Bc;
   A.Br = c.st[im].Br;
+  return A;
+}
+
+// The FP64 fill: W rows, then the Gram matrix and target products.  nCover:
+// the entries one pass of the grids covers; LOOP kernels take longer lists in
+// further passes.
+template <bool LOOP, int NI>
+static hipError_t launch_fill_fp64(Ctx& c, const ArgsN<FillArgs, NI>& P, int nCover) {
+  hipLaunchKernelGGL((k_fill_conv<LOOP, NI>), dim3(nCover, WROWS, NI), dim3(256), 0, c.cur, P);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_fill_gram<LOOP, NI>), dim3(8 * ((nCover * NGI + 7) / 8), 1, NI),
+                     dim3(WAVE), 0, c.cur, P, nCover);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, int nMax) {
+  if (nMax <= 0) return hipSuccess;
+  const FillArgs A = fill_args(c, im, d_list, d_nList);
   prof_begin(c, XB_PROF_FILL);
+  hipError_t e;
   // The mixed form fills every kept stamp on the matrix cores (one workgroup
   // per stamp for ~80 us: throughput over 100+ stamps).  A listed refill (the
   // few stamps a clip-refit pass moved on, a restore, the screen's probes)
-  // takes the FP64 fill below instead, a few microseconds for a short list,
-  // and no less accurate.
-  if ((c.form & XB_FORM_MIXED) && !d_list) {
-    const hipError_t em = launch_fill_mixed(c, A, nMax);
-    prof_end(c, XB_PROF_FILL);
-    return em;
-  }
-  const int nCover = d_list ? std::min(nMax, FILL_LIST_STAMPS) : nMax;
-  const dim3 gConv(nCover, WROWS), gGram(8 * ((nCover * NGI + 7) / 8));
-  if (d_list)
-    hipLaunchKernelGGL(k_fill_conv<true>, gConv, dim3(256), 0, c.cur, A);
+  // takes the FP64 fill instead, a few microseconds for a short list, and no
+  // less accurate.
+  if ((c.form & XB_FORM_MIXED) && !d_list)
+    e = launch_fill_mixed(c, A, nMax);
+  else if (d_list)
+    e = launch_fill_fp64<true, 1>(c, ArgsN<FillArgs, 1>{{A}}, std::min(nMax, FILL_LIST_STAMPS));
   else
-    hipLaunchKernelGGL(k_fill_conv<false>, gConv, dim3(256), 0, c.cur, A);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  if (d_list)
-    hipLaunchKernelGGL(k_fill_gram<true>, gGram, dim3(WAVE), 0, c.cur, A, nCover);
-  else
-    hipLaunchKernelGGL(k_fill_gram<false>, gGram, dim3(WAVE), 0, c.cur, A, nCover);
+    e = launch_fill_fp64<false, 1>(c, ArgsN<FillArgs, 1>{{A}}, nMax);
   prof_end(c, XB_PROF_FILL);
-  return hipGetLastError();
+  return e;
+}
+
+hipError_t launch_fill_pair(Ctx& c) {
+  const int nMax = std::max(c.nKept[0], c.nKept[1]);
+  if (nMax <= 0) return hipSuccess;
+  const FillArgs A0 = fill_args(c, 0, nullptr, nullptr), A1 = fill_args(c, 1, nullptr, nullptr);
+  prof_begin(c, XB_PROF_FILL);
+  const hipError_t e = (c.form & XB_FORM_MIXED)
+                           ? launch_fill_mixed_pair(c, A0, A1, nMax)
+                           : launch_fill_fp64<false, 2>(c, ArgsN<FillArgs, 2>{{A0, A1}}, nMax);
+  prof_end(c, XB_PROF_FILL);
+  return e;
 }
 
 }  // namespace xb

@@ -183,10 +183,11 @@ struct MxOps<true> {
 
 // vtab: [MX_KSTEPS][2 n-tiles][parts][64 lanes] x 8 values (uint4): BF16
 // h, m, l, or FP16 h, l scaled by 2^vscale
-template <bool F16>
-__global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(FillArgs A,
+template <bool F16, int NI>
+__global__ __launch_bounds__(MX_THREADS) void k_fill_mixed(ArgsN<FillArgs, NI> P,
                                                            const uint4* __restrict__ vtab,
                                                            int vscale) {
+  const FillArgs& A = args_of(P);
   using Ops = MxOps<F16>;
   int s, ssx, ssy;
   if (!fill_stamp(A, blockIdx.x, &s, &ssx, &ssy)) return;
@@ -342,7 +343,9 @@ constexpr int G64_TILES = 10;                     // lower 16x16 tiles of the 64
 
 __device__ __forceinline__ int g64_tile(int I, int J) { return I * (I + 1) / 2 + J; }
 
-__global__ __launch_bounds__(G64_THREADS) void k_fill_gram64(FillArgs A) {
+template <int NI>
+__global__ __launch_bounds__(G64_THREADS) void k_fill_gram64(ArgsN<FillArgs, NI> P) {
+  const FillArgs& A = args_of(P);
   int s, ssx, ssy;
   if (!fill_stamp(A, blockIdx.x, &s, &ssx, &ssy)) return;
   const int g = A.kept[s];
@@ -451,23 +454,35 @@ __global__ __launch_bounds__(G64_THREADS) void k_fill_gram64(FillArgs A) {
   }
 }
 
-hipError_t launch_fill_mixed(Ctx& c, const FillArgs& A, int nMax) {
+template <int NI>
+static hipError_t launch_fill_mixed_n(Ctx& c, const ArgsN<FillArgs, NI>& P, int nMax) {
   static const hipError_t attrRc = hipFuncSetAttribute(
-      (const void*)k_fill_mixed<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MX_LDS);
+      (const void*)k_fill_mixed<false, NI>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)MX_LDS);
   static const hipError_t attrRc16 = hipFuncSetAttribute(
-      (const void*)k_fill_mixed<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MX_LDS);
+      (const void*)k_fill_mixed<true, NI>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)MX_LDS);
   if (attrRc != hipSuccess) return attrRc;
   if (attrRc16 != hipSuccess) return attrRc16;
+  const dim3 grid(nMax, 1, NI);
   if (c.form & XB_FORM_FP16)
-    hipLaunchKernelGGL(k_fill_mixed<true>, dim3(nMax), dim3(MX_THREADS), MX_LDS, c.cur, A,
+    hipLaunchKernelGGL((k_fill_mixed<true, NI>), grid, dim3(MX_THREADS), MX_LDS, c.cur, P,
                        reinterpret_cast<const uint4*>(c.mixTab16), c.mixScale16);
   else
-    hipLaunchKernelGGL(k_fill_mixed<false>, dim3(nMax), dim3(MX_THREADS), MX_LDS, c.cur, A,
+    hipLaunchKernelGGL((k_fill_mixed<false, NI>), grid, dim3(MX_THREADS), MX_LDS, c.cur, P,
                        reinterpret_cast<const uint4*>(c.mixTab), 0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill_gram64, dim3(nMax), dim3(G64_THREADS), 0, c.cur, A);
+  hipLaunchKernelGGL(k_fill_gram64<NI>, grid, dim3(G64_THREADS), 0, c.cur, P);
   return hipGetLastError();
+}
+
+hipError_t launch_fill_mixed(Ctx& c, const FillArgs& A, int nMax) {
+  return launch_fill_mixed_n<1>(c, ArgsN<FillArgs, 1>{{A}}, nMax);
+}
+
+hipError_t launch_fill_mixed_pair(Ctx& c, const FillArgs& A0, const FillArgs& A1, int nMax) {
+  return launch_fill_mixed_n<2>(c, ArgsN<FillArgs, 2>{{A0, A1}}, nMax);
 }
 
 }  // namespace xb

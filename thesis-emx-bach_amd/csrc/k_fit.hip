@@ -11,6 +11,8 @@
 // the same pivot row swaps, so the factors are bit-identical while every step
 // is a parallel rank-1 update.  Pivot choice keeps the reference's `>=` rule
 // (the last row attaining the maximum wins; NaN never wins).
+#include <algorithm>
+
 #include "xb_dev.h"
 
 namespace xb {
@@ -48,9 +50,20 @@ __device__ __forceinline__ double readlane_f64_at(double v, int l) {
                           __builtin_amdgcn_readlane(__double2loint(v), l));
 }
 
-__global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall,
-                                                 const double* __restrict__ Bcall,
-                                                 const int* nKept, double* ksum) {
+struct StampLuArgs {
+  const double* Gall;
+  const double* Bcall;
+  const int* nKept;
+  double* ksum;
+};
+
+template <int NI>
+__global__ __launch_bounds__(64) void k_stamp_lu(ArgsN<StampLuArgs, NI> P) {
+  const StampLuArgs& SA = args_of(P);
+  const double* __restrict__ Gall = SA.Gall;
+  const double* __restrict__ Bcall = SA.Bcall;
+  const int* nKept = SA.nKept;
+  double* ksum = SA.ksum;
   const int s = blockIdx.x;
   if (s >= *nKept) return;
   const int tid = threadIdx.x;
@@ -225,9 +238,24 @@ __global__ __launch_bounds__(64) void k_stamp_lu(const double* __restrict__ Gall
 // ---------------------------------------------------------------------------
 // cd: kernel-sum clip and test-stamp selection (cdkscUtil.cpp:74-103)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_cd_select(const double* ksum, const int* nKeptP,
-                                                    int* list, DevScalars* sc, int im,
-                                                    double alpha, double sigKernFit) {
+struct CdSelectArgs {
+  const double* ksum;
+  const int* nKeptP;
+  int* list;
+  DevScalars* sc;
+  int im;
+  double alpha, sigKernFit;
+};
+
+template <int NI>
+__global__ __launch_bounds__(1024) void k_cd_select(ArgsN<CdSelectArgs, NI> P) {
+  const CdSelectArgs& CA = args_of(P);
+  const double* ksum = CA.ksum;
+  const int* nKeptP = CA.nKeptP;
+  int* list = CA.list;
+  DevScalars* sc = CA.sc;
+  const int im = CA.im;
+  const double alpha = CA.alpha, sigKernFit = CA.sigKernFit;
   __shared__ double data[XB_MAX_STAMPS];
   __shared__ uint8_t clip[XB_MAX_STAMPS];
   __shared__ double part[64];
@@ -375,7 +403,9 @@ __device__ __forceinline__ TermPlan plan_term(const BuildArgs& A, int r, int c, 
   return P;
 }
 
-__global__ __launch_bounds__(256) void k_global_build(BuildArgs A) {
+template <int NI>
+__global__ __launch_bounds__(256) void k_global_build(ArgsN<BuildArgs, NI> PA) {
+  const BuildArgs& A = args_of(PA);
   const int n = *A.nList;
   const int nLower = MATN * (MATN + 1) / 2;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -507,7 +537,9 @@ struct SigArgs {
 // of the reference's 32 x 31 reduction layout), the 48 basis terms of a pixel
 // loaded together before its accumulation chain.
 constexpr int SIG_THREADS = 1024;
-__global__ __launch_bounds__(SIG_THREADS) void k_calc_sigs(SigArgs A) {
+template <int NI>
+__global__ __launch_bounds__(SIG_THREADS) void k_calc_sigs(ArgsN<SigArgs, NI> P) {
+  const SigArgs& A = args_of(P);
   const int u = blockIdx.x;
   if (u >= *A.nList) return;
   const int tid = threadIdx.x;
@@ -600,9 +632,24 @@ __global__ __launch_bounds__(SIG_THREADS) void k_calc_sigs(SigArgs A) {
 // ---------------------------------------------------------------------------
 // cd: kernel sum at (0, 0) (cd.cl:385-442) and the merit (cdkscUtil.cpp:221-266)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_cd_finish(const Basis* basis, const double* sol,
-                                                    const double* merits, DevScalars* sc, int im,
-                                                    int w, int h, double alpha) {
+struct CdFinishArgs {
+  const Basis* basis;
+  const double* sol;
+  const double* merits;
+  DevScalars* sc;
+  int im, w, h;
+  double alpha;
+};
+
+template <int NI>
+__global__ __launch_bounds__(1024) void k_cd_finish(ArgsN<CdFinishArgs, NI> P) {
+  const CdFinishArgs& FA = args_of(P);
+  const Basis* basis = FA.basis;
+  const double* sol = FA.sol;
+  const double* merits = FA.merits;
+  DevScalars* sc = FA.sc;
+  const int im = FA.im, w = FA.w, h = FA.h;
+  const double alpha = FA.alpha;
   __shared__ double co[NPSF];
   __shared__ double kern[KK];
   __shared__ double part[64];
@@ -762,21 +809,29 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static StampLuArgs stamp_lu_args(Ctx& c, int im) {
+  return StampLuArgs{c.st[im].G, c.st[im].Bc, &c.sc->nKept[im], c.st[im].ksum};
+}
+
 hipError_t launch_stamp_lu(Ctx& c, int im) {
   if (c.nKept[im] <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_stamp_lu, dim3(c.nKept[im]), dim3(64), 0, c.cur, c.st[im].G,
-                     c.st[im].Bc, &c.sc->nKept[im], c.st[im].ksum);
+  hipLaunchKernelGGL(k_stamp_lu<1>, dim3(c.nKept[im]), dim3(64), 0, c.cur,
+                     ArgsN<StampLuArgs, 1>{{stamp_lu_args(c, im)}});
   return hipGetLastError();
+}
+
+static CdSelectArgs cd_select_args(Ctx& c, int im) {
+  return CdSelectArgs{c.st[im].ksum, &c.sc->nKept[im], c.st[im].list, c.sc, im,
+                      c.m.sigClipAlpha, c.m.sigKernFit};
 }
 
 hipError_t launch_cd_select(Ctx& c, int im) {
-  hipLaunchKernelGGL(k_cd_select, dim3(1), dim3(1024), 0, c.cur, c.st[im].ksum,
-                     &c.sc->nKept[im], c.st[im].list, c.sc, im, c.m.sigClipAlpha,
-                     c.m.sigKernFit);
+  hipLaunchKernelGGL(k_cd_select<1>, dim3(1), dim3(1024), 0, c.cur,
+                     ArgsN<CdSelectArgs, 1>{{cd_select_args(c, im)}});
   return hipGetLastError();
 }
 
-hipError_t launch_global_build(Ctx& c, int im, bool cdForm) {
+static BuildArgs build_args(Ctx& c, int im, bool cdForm) {
   BuildArgs A;
   A.G = c.st[im].G;
   A.Bc = c.st[im].Bc;
@@ -796,39 +851,19 @@ hipError_t launch_global_build(Ctx& c, int im, bool cdForm) {
   A.rhs = L.rhs;
   A.M0 = L.M0;
   A.rhs0 = L.rhs0;
-  const int total = MATN * (MATN + 1) / 2 + NSOL;
-  hipLaunchKernelGGL(k_global_build, dim3((total + 255) / 256), dim3(256), 0, c.cur, A);
+  return A;
+}
+
+constexpr int BUILD_TOTAL = MATN * (MATN + 1) / 2 + NSOL;
+
+hipError_t launch_global_build(Ctx& c, int im, bool cdForm) {
+  hipLaunchKernelGGL(k_global_build<1>, dim3((BUILD_TOTAL + 255) / 256), dim3(256), 0, c.cur,
+                     ArgsN<BuildArgs, 1>{{build_args(c, im, cdForm)}});
   return hipGetLastError();
 }
 
-hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
-                            uint16_t nanMark) {
-  SigArgs A;
-  const int src = im;
-  A.tImg = c.img[src];
-  A.sImg = c.img[1 - src];
-  A.mask = c.mask;
-  A.w = c.w;
-  A.h = c.h;
-  A.W = c.st[im].W;
-  A.kept = c.st[im].kept;
-  A.cur = c.st[im].cur;
-  A.count = c.st[im].count;
-  A.ssx = c.st[im].ssx;
-  A.ssy = c.st[im].ssy;
-  A.list = useList ? c.st[im].list : nullptr;
-  A.nList = useList ? &c.sc->nList[im] : &c.sc->nKept[im];
-  A.basis = c.basis;
-  A.sol = d_sol;
-  A.sigOut = useList ? c.st[im].sig : c.st[im].sigK;
-  A.nanMark = nanMark;
-  if (c.nKept[im] <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur, A);
-  return hipGetLastError();
-}
-
-hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d_nList,
-                                 const double* d_sol, double* out) {
+static SigArgs sig_args(Ctx& c, int im, const int* d_list, const int* d_nList,
+                        const double* d_sol, double* out, uint16_t nanMark) {
   SigArgs A;
   A.tImg = c.img[im];
   A.sImg = c.img[1 - im];
@@ -846,16 +881,72 @@ hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d
   A.basis = c.basis;
   A.sol = d_sol;
   A.sigOut = out;
-  A.nanMark = XB_MASK_NAN_PIXEL;
+  A.nanMark = nanMark;
+  return A;
+}
+
+hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
+                            uint16_t nanMark) {
   if (c.nKept[im] <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_calc_sigs, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur, A);
+  const SigArgs A =
+      sig_args(c, im, useList ? c.st[im].list : nullptr,
+               useList ? &c.sc->nList[im] : &c.sc->nKept[im], d_sol,
+               useList ? c.st[im].sig : c.st[im].sigK, nanMark);
+  hipLaunchKernelGGL(k_calc_sigs<1>, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur,
+                     ArgsN<SigArgs, 1>{{A}});
   return hipGetLastError();
 }
 
+hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d_nList,
+                                 const double* d_sol, double* out) {
+  if (c.nKept[im] <= 0) return hipSuccess;
+  const SigArgs A = sig_args(c, im, d_list, d_nList, d_sol, out, XB_MASK_NAN_PIXEL);
+  hipLaunchKernelGGL(k_calc_sigs<1>, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur,
+                     ArgsN<SigArgs, 1>{{A}});
+  return hipGetLastError();
+}
+
+static CdFinishArgs cd_finish_args(Ctx& c, int im) {
+  return CdFinishArgs{c.basis, c.cdSol + (size_t)im * NSOL, c.st[im].sig, c.sc, im, c.w, c.h,
+                      c.m.sigClipAlpha};
+}
+
 hipError_t launch_cd_finish(Ctx& c, int im) {
-  hipLaunchKernelGGL(k_cd_finish, dim3(1), dim3(1024), 0, c.cur, c.basis,
-                     c.cdSol + (size_t)im * NSOL, c.st[im].sig, c.sc, im, c.w, c.h,
-                     c.m.sigClipAlpha);
+  hipLaunchKernelGGL(k_cd_finish<1>, dim3(1), dim3(1024), 0, c.cur,
+                     ArgsN<CdFinishArgs, 1>{{cd_finish_args(c, im)}});
+  return hipGetLastError();
+}
+
+// cd's per-stamp fits, test-stamp selection and global build of both images
+// in one dispatch each (image z = blockIdx.z); both images must have stamps
+hipError_t launch_cd_begin_pair(Ctx& c) {
+  const int n = std::max(c.nKept[0], c.nKept[1]);
+  hipLaunchKernelGGL(k_stamp_lu<2>, dim3(n, 1, 2), dim3(64), 0, c.cur,
+                     ArgsN<StampLuArgs, 2>{{stamp_lu_args(c, 0), stamp_lu_args(c, 1)}});
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cd_select<2>, dim3(1, 1, 2), dim3(1024), 0, c.cur,
+                     ArgsN<CdSelectArgs, 2>{{cd_select_args(c, 0), cd_select_args(c, 1)}});
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_global_build<2>, dim3((BUILD_TOTAL + 255) / 256, 1, 2), dim3(256), 0,
+                     c.cur, ArgsN<BuildArgs, 2>{{build_args(c, 0, true), build_args(c, 1, true)}});
+  return hipGetLastError();
+}
+
+// cd's chi2 of the test stamps and the merits of both images in one dispatch
+// each (after their solves)
+hipError_t launch_cd_end_pair(Ctx& c) {
+  const int n = std::max(c.nKept[0], c.nKept[1]);
+  SigArgs A[2];
+  for (int im = 0; im < 2; im++)
+    A[im] = sig_args(c, im, c.st[im].list, &c.sc->nList[im], c.cdSol + (size_t)im * NSOL,
+                     c.st[im].sig, XB_MASK_NAN_PIXEL);
+  hipLaunchKernelGGL(k_calc_sigs<2>, dim3(n, 1, 2), dim3(SIG_THREADS), 0, c.cur,
+                     ArgsN<SigArgs, 2>{{A[0], A[1]}});
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cd_finish<2>, dim3(1, 1, 2), dim3(1024), 0, c.cur,
+                     ArgsN<CdFinishArgs, 2>{{cd_finish_args(c, 0), cd_finish_args(c, 1)}});
   return hipGetLastError();
 }
 

@@ -1018,6 +1018,30 @@ LuSys lu_sys(const LuSet& L, double* d_out, const int* d_skipIfZero) {
   return S;
 }
 
+// Two systems' solves as one batched launch sequence on the current stream
+// (a one-stream context's two cd trial fits side by side): the batched L D L^T
+// launch, or the batched ludcmp + lubksb kernels (bit-identical to single
+// solves) and each system's refinement; launched directly, not from a graph.
+hipError_t launch_global_solve_pair(Ctx& c, const LuSet& L0, double* out0, const int* skip0,
+                                    const LuSet& L1, double* out1, const int* skip1) {
+  prof_begin(c, XB_PROF_SOLVE);
+  LuBatch B;
+  B.n = 2;
+  B.s[0] = lu_sys(L0, out0, skip0);
+  B.s[1] = lu_sys(L1, out1, skip1);
+  const bool refine = (c.form & XB_FORM_REFINE) != 0;
+  hipError_t e;
+  if (c.form & XB_FORM_LDLT) {
+    e = launch_ldlt_solve_batch(c.cur, B, refine, c.shapes.ldlt_group);
+  } else {
+    e = launch_global_solve_batch(c.cur, B, false, false);
+    if (e == hipSuccess && refine) e = enqueue_refine(c.cur, L0, out0);
+    if (e == hipSuccess && refine) e = enqueue_refine(c.cur, L1, out1);
+  }
+  prof_end(c, XB_PROF_SOLVE);
+  return e;
+}
+
 // The global solve on the context's current stream: replayed from a hipGraph
 // captured at the first call with these buffers (kernels only, no memset
 // nodes), or launched kernel by kernel under XB_CTX_EAGER.

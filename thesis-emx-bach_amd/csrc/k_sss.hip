@@ -666,8 +666,9 @@ struct StampPix {
 // the stamp staged in LDS, one that waits for LDS traffic only).  Mask bits are
 // set with 32-bit atomic ORs: the template and science searches run at the
 // same time, each reading only its own bits (badMask) but sharing mask words.
-template <bool STAGED>
-__global__ __launch_bounds__(FIND_THREADS) void k_sss_find(FindArgs A) {
+template <bool STAGED, int NI>
+__global__ __launch_bounds__(FIND_THREADS) void k_sss_find(ArgsN<FindArgs, NI> PA) {
+  const FindArgs& A = args_of(PA);
   extern __shared__ uint32_t find_lds[];
   const int s = blockIdx.x;
   const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
@@ -1054,7 +1055,7 @@ hipError_t launch_sss_stats(Ctx& c, int im) {
   return hipGetLastError();
 }
 
-hipError_t launch_sss_find(Ctx& c, int im) {
+static FindArgs find_args(Ctx& c, int im) {
   FindArgs A;
   A.img = c.img[im];
   A.mask = c.mask;
@@ -1077,6 +1078,13 @@ hipError_t launch_sss_find(Ctx& c, int im) {
   // the good-pixel scratch of k_sss_stats is free again; each image's search
   // uses its own half (the two searches run concurrently)
   A.candAll = reinterpret_cast<int*>(c.goodPix) + (size_t)im * c.capStamps * c.capFsw * c.capFsw;
+  return A;
+}
+
+// The searches of NI images (image z = blockIdx.z): each reads only its own
+// image's mask bits and sets them with atomic ORs, so they may run at once.
+template <int NI>
+static hipError_t launch_find_n(Ctx& c, const ArgsN<FindArgs, NI>& P) {
   // the stamp region in LDS when it fits (float values + one bad bit per pixel)
   const size_t nAll = (size_t)c.fsw * c.fsw;
   const size_t lds = ((nAll + 63) / 64 * 64) * sizeof(float) + (nAll + 63) / 64 * 8;
@@ -1085,15 +1093,25 @@ hipError_t launch_sss_find(Ctx& c, int im) {
     // once per process (the attribute belongs to the kernel); the static's
     // initialisation is thread-safe, so batch workers may race here
     static const hipError_t attrRc = hipFuncSetAttribute(
-        (const void*)k_sss_find<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (const void*)k_sss_find<true, NI>, hipFuncAttributeMaxDynamicSharedMemorySize,
         (int)FIND_LDS_MAX);
     if (attrRc != hipSuccess) return attrRc;
-    hipLaunchKernelGGL(k_sss_find<true>, dim3(c.nGrid), dim3(FIND_THREADS), lds, c.cur, A);
+    hipLaunchKernelGGL((k_sss_find<true, NI>), dim3(c.nGrid, 1, NI), dim3(FIND_THREADS), lds,
+                       c.cur, P);
   } else {
-    hipLaunchKernelGGL(k_sss_find<false>, dim3(c.nGrid), dim3(FIND_THREADS), 0, c.cur, A);
+    hipLaunchKernelGGL((k_sss_find<false, NI>), dim3(c.nGrid, 1, NI), dim3(FIND_THREADS), 0,
+                       c.cur, P);
   }
   prof_end(c, XB_PROF_SSS);
   return hipGetLastError();
+}
+
+hipError_t launch_sss_find(Ctx& c, int im) {
+  return launch_find_n<1>(c, ArgsN<FindArgs, 1>{{find_args(c, im)}});
+}
+
+hipError_t launch_sss_find_pair(Ctx& c) {
+  return launch_find_n<2>(c, ArgsN<FindArgs, 2>{{find_args(c, 0), find_args(c, 1)}});
 }
 
 hipError_t launch_sss_compact(Ctx& c) {

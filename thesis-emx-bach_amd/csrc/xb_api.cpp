@@ -498,14 +498,19 @@ int sssSetup(Ctx& c) {
 
 // calcStats T, calcStats S, then both searches (src/sssUtil.cpp:11-14); the
 // searches read only their own image's mask bits, so they overlap
+// (a one-stream context launches both searches as one dispatch)
 int sssIdentify(Ctx& c) {
   XB_HIP(launch_sss_stats(c, 0));
   XB_HIP(launch_sss_stats(c, 1));
-  int rc = forkBoth(c, [&](int im) -> int {
-    XB_HIP(launch_sss_find(c, im));
-    return XB_OK;
-  });
-  if (rc) return rc;
+  if (c.side == c.stream) {
+    XB_HIP(launch_sss_find_pair(c));
+  } else {
+    int rc = forkBoth(c, [&](int im) -> int {
+      XB_HIP(launch_sss_find(c, im));
+      return XB_OK;
+    });
+    if (rc) return rc;
+  }
   XB_HIP(launch_sss_compact(c));
   return XB_OK;
 }
@@ -541,6 +546,10 @@ int stageSss(Ctx& c) {
 
 // cmv (src/bach.cpp:172-268)
 int stageCmv(Ctx& c) {
+  if (c.side == c.stream) {  // one stream: both images' fills as one dispatch per kernel
+    XB_HIP(launch_fill_pair(c));
+    return XB_OK;
+  }
   return forkBoth(c, [&](int im) -> int {
     XB_HIP(launch_fill(c, im, nullptr, nullptr, c.nKept[im]));
     return XB_OK;
@@ -549,8 +558,17 @@ int stageCmv(Ctx& c) {
 
 // cd (src/bach.cpp:270-313, testFit src/cdkscUtil.cpp:4-266)
 int stageCd(Ctx& c, int* convTemplate) {
-  // the two trial fits are independent: each direction has its own system
-  int rc = forkBoth(c, [&](int im) -> int {
+  // the two trial fits are independent: each direction has its own system.  A
+  // one-stream context with stamps in both images runs them side by side, as
+  // one dispatch per kernel and one batched solve of the two systems.
+  const bool pair = c.side == c.stream && c.nKept[0] > 0 && c.nKept[1] > 0;
+  int rc = XB_OK;
+  if (pair) {
+    XB_HIP(launch_cd_begin_pair(c));
+    XB_HIP(launch_global_solve_pair(c, c.lu[0], c.cdSol, &c.sc->nTest[0], c.lu[1],
+                                    c.cdSol + NSOL, &c.sc->nTest[1]));
+    XB_HIP(launch_cd_end_pair(c));
+  } else rc = forkBoth(c, [&](int im) -> int {
     if (c.nKept[im] == 0) return XB_OK;
     double* cdSol = c.cdSol + (size_t)im * NSOL;
     XB_HIP(launch_stamp_lu(c, im));

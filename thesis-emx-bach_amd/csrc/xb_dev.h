@@ -12,6 +12,18 @@
 
 namespace xb {
 
+// The arguments of a launch that covers N images at once: image z's work is
+// blockIdx.z's (both images of a pair in one dispatch, so a one-stream
+// context runs them side by side and a pair makes half the dispatches).
+template <typename T, int N>
+struct ArgsN {
+  T a[N];
+};
+template <typename T, int N>
+__device__ __forceinline__ const T& args_of(const ArgsN<T, N>& P) {
+  return P.a[N == 1 ? 0 : blockIdx.z];
+}
+
 constexpr int WAVE = 64;
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -232,5 +244,7 @@ __device__ __forceinline__ bool fill_stamp(const FillArgs& A, int k, int* s, int
 
 // the mixed-precision form of the fill (k_cmv_mixed.hip)
 hipError_t launch_fill_mixed(Ctx& c, const FillArgs& A, int nMax);
+// both images' full fills in one dispatch per kernel (image z = blockIdx.z)
+hipError_t launch_fill_mixed_pair(Ctx& c, const FillArgs& A0, const FillArgs& A1, int nMax);
 
 }  // namespace xb

@@ -310,11 +310,15 @@ void prof_end(Ctx& c, int slot);
 hipError_t launch_mask_input(Ctx& c);
 hipError_t launch_sss_stats(Ctx& c, int im);
 hipError_t launch_sss_find(Ctx& c, int im);
+// both images' searches in one dispatch (image z = blockIdx.z)
+hipError_t launch_sss_find_pair(Ctx& c);
 hipError_t launch_sss_compact(Ctx& c);
 hipError_t launch_reset_skip(Ctx& c);
 // Fills the stamps of image im listed in d_list (d_nList entries on the
 // device), or every kept stamp when d_list is null; nMax bounds the count.
 hipError_t launch_fill(Ctx& c, int im, const int* d_list, const int* d_nList, int nMax);
+// both images' full fills in one dispatch per kernel (image z = blockIdx.z)
+hipError_t launch_fill_pair(Ctx& c);
 hipError_t launch_stamp_lu(Ctx& c, int im);
 hipError_t launch_cd_select(Ctx& c, int im);
 hipError_t launch_global_build(Ctx& c, int im, bool cdForm);
@@ -354,6 +358,14 @@ hipError_t launch_ldlt_solve_batch(hipStream_t s, const LuBatch& B, bool refine,
 // empty fit).
 hipError_t launch_global_solve(Ctx& c, const LuSet& L, double* d_out, const int* d_skipIfZero);
 hipError_t launch_cd_finish(Ctx& c, int im);
+// Both images in one dispatch per kernel (one-stream contexts; both images
+// have stamps): cd's stamp fits, test-stamp selection and global build; then,
+// after the two solves, their chi2 and merits.
+hipError_t launch_cd_begin_pair(Ctx& c);
+hipError_t launch_cd_end_pair(Ctx& c);
+// The two systems' solves as one batched launch sequence.
+hipError_t launch_global_solve_pair(Ctx& c, const LuSet& L0, double* out0, const int* skip0,
+                                    const LuSet& L1, double* out1, const int* skip1);
 // cd form (useList): the test stamps' chi2 into st.sig; ksc form: every kept
 // stamp's into st.sigK.  nanMark is the mask bit a NaN model pixel sets.
 hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
