@@ -189,7 +189,7 @@ def run_batched(ctx, params, pairs, bufs, workers, steps, warmup, form=0, model=
             b.screen_load(model)
         b.set_form(form)
         if tune:
-            table = b.autotune(params, ws, hs, *cols, reps=1)
+            table = b.autotune(params, ws, hs, *cols, reps=2)
             # the tuned launch shapes of each size bucket present
             shapes = {k: b.shapes(k).as_dict() for k in range(4) if table[k]}
         for _ in range(warmup):

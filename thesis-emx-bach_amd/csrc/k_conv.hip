@@ -68,7 +68,7 @@ __global__ __launch_bounds__(512) void k_tile_kernels(const Basis* __restrict__ 
                                                       const double* __restrict__ sol, int w,
                                                       int h, int xSteps, int nTiles,
                                                       double* tileK64, double* tileAks,
-                                                      double* kernSumOut) {
+                                                      double* kernSumOut, double* kernSumHost) {
   const int t0 = blockIdx.x * TK_TILES;
   const int nt = min(TK_TILES, nTiles + 1 - t0);
   const int tid = threadIdx.x;
@@ -116,6 +116,10 @@ __global__ __launch_bounds__(512) void k_tile_kernels(const Basis* __restrict__ 
       double sum = 0.0;
       for (int i = 0; i < KK; i++) sum += kern[tid][i];
       *kernSumOut = sum;
+      if (kernSumHost) {  // the host's pinned mirror too
+        *kernSumHost = sum;
+        __threadfence_system();
+      }
     }
   }
 }
@@ -134,7 +138,8 @@ __global__ __launch_bounds__(256) void k_tile_kernels_mfma(const Basis* __restri
                                                            const double* __restrict__ sol, int w,
                                                            int h, int xSteps, int nTiles,
                                                            double* tileK64, double* tileAks,
-                                                           double* kernSumOut) {
+                                                           double* kernSumOut,
+                                                           double* kernSumHost) {
   constexpr int KD = 52;                // depth padded to a multiple of 4
   constexpr int NBLK = (KK + 15) / 16;  // 28 column blocks of 16 taps
   const int t0 = blockIdx.x * 16;
@@ -208,8 +213,15 @@ __global__ __launch_bounds__(256) void k_tile_kernels_mfma(const Basis* __restri
     const int t = t0 + tid;
     const double a = ((part[0][0][tid] + part[0][1][tid]) + part[0][2][tid]) + part[0][3][tid];
     const double k = ((part[1][0][tid] + part[1][1][tid]) + part[1][2][tid]) + part[1][3][tid];
-    if (t < nTiles) tileAks[t] = a;
-    else if (t == nTiles) *kernSumOut = k;
+    if (t < nTiles) {
+      tileAks[t] = a;
+    } else if (t == nTiles) {
+      *kernSumOut = k;
+      if (kernSumHost) {  // the host's pinned mirror too
+        *kernSumHost = k;
+        __threadfence_system();
+      }
+    }
   }
 }
 
@@ -552,11 +564,12 @@ hipError_t launch_conv(Ctx& c) {
   const double* sol = c.ksSol + (size_t)ti * NSOL;  // the chosen direction's fit
   if (c.form & XB_FORM_MIXED)
     hipLaunchKernelGGL(k_tile_kernels_mfma, dim3((nTiles + 1 + 15) / 16), dim3(256), 0, c.cur,
-                       c.basis, sol, c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks, kernSum);
+                       c.basis, sol, c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks, kernSum,
+                       &c.hscDev->kernSum);
   else
     hipLaunchKernelGGL(k_tile_kernels, dim3((nTiles + 1 + TK_TILES - 1) / TK_TILES), dim3(512),
                        0, c.cur, c.basis, sol, c.w, c.h, xSteps, nTiles, c.tileK64, c.tileAks,
-                       kernSum);
+                       kernSum, &c.hscDev->kernSum);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   ConvArgs A;

@@ -726,7 +726,7 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
                                                     const int* nKept, KscScalars* ks,
                                                     int first, double alpha,
                                                     double sigKernFit, int* rec,
-                                                    uint8_t* zeroBits) {
+                                                    uint8_t* zeroBits, KscScalars* hks) {
   __shared__ double data[XB_MAX_STAMPS];
   __shared__ uint8_t clip[XB_MAX_STAMPS];
   __shared__ uint8_t refill[XB_MAX_STAMPS];
@@ -803,6 +803,7 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
       ks->nRestored = 0;
       ks->nConfirmed = 0;
     }
+    publish_pass(*ks, hks);
   }
 }
 
@@ -950,11 +951,12 @@ hipError_t launch_cd_end_pair(Ctx& c) {
   return hipGetLastError();
 }
 
-hipError_t launch_ksc_check(Ctx& c, int im, bool first, int* rec, uint8_t* zeroBits) {
+hipError_t launch_ksc_check(Ctx& c, int im, bool first, int* rec, uint8_t* zeroBits,
+                            bool publish) {
   hipLaunchKernelGGL(k_ksc_check, dim3(1), dim3(1024), 0, c.cur, c.st[im].sigK,
                      c.st[im].kept, c.st[im].cur, c.st[im].count, c.st[im].listK,
                      &c.sc->nKept[im], &c.sc->k[im], first ? 1 : 0, c.m.sigClipAlpha,
-                     c.m.sigKernFit, rec, zeroBits);
+                     c.m.sigKernFit, rec, zeroBits, publish ? &c.hscDev->k[im] : nullptr);
   return hipGetLastError();
 }
 

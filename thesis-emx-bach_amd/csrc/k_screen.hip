@@ -285,7 +285,7 @@ __global__ __launch_bounds__(WAVE) void k_screen_decide(const float* __restrict_
 __global__ __launch_bounds__(1024) void k_screen_merge(const int* passCur, int* cur,
                                                        const uint8_t* flag, uint8_t* bits,
                                                        int* list, const int* nKept,
-                                                       KscScalars* ks) {
+                                                       KscScalars* ks, KscScalars* hks) {
   __shared__ int isc[20];
   __shared__ int s_n, s_scr;
   const int tid = threadIdx.x;
@@ -318,6 +318,7 @@ __global__ __launch_bounds__(1024) void k_screen_merge(const int* passCur, int* 
     ks->nList = s_n;
     ks->check = s_n > 0 ? 1 : 0;
     ks->nScreened += s_scr;  // over the screened passes (zeroed by the first check)
+    publish_pass(*ks, hks);  // the pass's final scalars (after its check's)
   }
 }
 
@@ -606,7 +607,8 @@ hipError_t launch_screen(Ctx& c, int im, int pass, bool apply) {
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_screen_merge, dim3(1), dim3(1024), 0, c.cur, passCur, c.st[im].cur,
-                     c.scr.flag, c.scr.bits, c.st[im].listK, &c.sc->nKept[im], &c.sc->k[im]);
+                     c.scr.flag, c.scr.bits, c.st[im].listK, &c.sc->nKept[im], &c.sc->k[im],
+                     &c.hscDev->k[im]);
   return hipGetLastError();
 }
 

@@ -403,7 +403,10 @@ int stageInitCommon(Ctx& c, const xb_params* p, int w, int h) {
   if (rc) return rc;
   if (!c.basis) XB_HIP(locked_malloc((void**)&c.basis, sizeof(Basis)));
   if (!c.sc) XB_HIP(locked_malloc((void**)&c.sc, sizeof(DevScalars)));
-  if (!c.hsc) XB_HIP(locked_host_malloc((void**)&c.hsc, sizeof(DevScalars)));
+  if (!c.hsc) {
+    XB_HIP(locked_host_malloc((void**)&c.hsc, sizeof(DevScalars)));
+    XB_HIP(hipHostGetDevicePointer((void**)&c.hscDev, c.hsc, 0));
+  }
   {
     int rc2 = ensureSolver(c);
     if (rc2) return rc2;
@@ -644,18 +647,12 @@ int kscPass(Ctx& c, int im, int it, bool keep, bool screen, uint16_t nanMark,
   const bool rec = keep && it < SCR_PASSES;
   // the check records the substamps of this pass; the screen's flags start
   // cleared at a screened loop's first pass
+  // the check (and the screen's merge after it, which has the last word)
+  // publishes the pass's scalars to the host mirror
   XB_HIP(launch_ksc_check(c, im, it == 0,
                           rec ? c.scr.passCur + (size_t)it * c.nKept[im] : nullptr,
-                          (screen && it == 0) ? c.scr.bits : nullptr));
+                          (screen && it == 0) ? c.scr.bits : nullptr, true));
   if (rec) XB_HIP(launch_screen(c, im, it, screen));
-  static_assert(offsetof(KscScalars, nScreened) == offsetof(KscScalars, check) + 3 * sizeof(int),
-                "check .. nScreened are one copy");
-  static_assert(offsetof(KscScalars, nList) == offsetof(KscScalars, check) + sizeof(int),
-                "the list's length is in that copy");
-  const size_t off = kscOffset(im) + offsetof(KscScalars, check);
-  XB_HIP(hipMemcpyAsync(reinterpret_cast<char*>(c.hsc) + off,
-                        reinterpret_cast<const char*>(c.sc) + off, 4 * sizeof(int),
-                        hipMemcpyDeviceToHost, c.cur));
   XB_HIP(hipEventRecord(c.evK[im], c.cur));
   return XB_OK;
 }
@@ -1742,15 +1739,9 @@ int pipelineRunInto(Ctx& c, const xb_params* p, int w, int h, const void* tin, c
   rc = stageMask(c);
   double ms[9];
   if (!rc) rc = runStages(c, ms, false);
-  if (!rc) {
-    const hipError_t e = hipMemcpyAsync(&c.kernSum, &c.sc->kernSum, sizeof(double),
-                                        hipMemcpyDeviceToHost, c.stream);
-    if (e != hipSuccess) {
-      c.err = std::string("read kernel sum: ") + hipGetErrorString(e);
-      rc = XB_ERR_HIP;
-    }
-  }
   const int rs = syncAll(c);  // nothing may still use the caller's buffers
+  // the tile kernels published the kernel sum into the pinned mirror
+  if (!rc && !rs) c.kernSum = c.hsc->kernSum;
   c.img[0] = own[0];
   c.img[1] = own[1];
   c.conv = own[2];

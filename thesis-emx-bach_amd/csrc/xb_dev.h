@@ -24,6 +24,20 @@ __device__ __forceinline__ const T& args_of(const ArgsN<T, N>& P) {
   return P.a[N == 1 ? 0 : blockIdx.z];
 }
 
+// A pass's scalars (check, the refill list's length, chi2 count, screened
+// count) written into the host's pinned mirror (host memory the device
+// addresses) by the kernel that set them last, followed by a system-scope
+// fence: the host reads them once the stream's event has completed, without a
+// copy dispatch.  One thread calls this.
+__device__ __forceinline__ void publish_pass(const KscScalars& ks, KscScalars* host) {
+  if (!host) return;
+  host->check = ks.check;
+  host->nList = ks.nList;
+  host->nChi2 = ks.nChi2;
+  host->nScreened = ks.nScreened;
+  __threadfence_system();
+}
+
 constexpr int WAVE = 64;
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

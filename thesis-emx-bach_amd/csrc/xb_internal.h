@@ -250,6 +250,8 @@ struct Ctx {
   DevStamps st[2];
   DevScalars* sc = nullptr;
   DevScalars* hsc = nullptr;     // pinned host mirror for the scalar read-backs
+  DevScalars* hscDev = nullptr;  // the same memory as the device addresses it
+                                 // (kernels publish a pass's scalars there)
   hipEvent_t evSync = nullptr;   // completion of the read-backs (polled, no sleep)
   Basis* basis = nullptr;
   double* ran1 = nullptr;       // ran1 draws (identical per stamp: seed -666)
@@ -373,8 +375,9 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
 // chi2 of the listed stamps (d_list, d_nList entries) into out (list order)
 hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d_nList,
                                  const double* d_sol, double* out);
+// publish: also write the pass's check .. nScreened into the host mirror
 hipError_t launch_ksc_check(Ctx& c, int im, bool first, int* rec = nullptr,
-                            uint8_t* zeroBits = nullptr);
+                            uint8_t* zeroBits = nullptr, bool publish = false);
 hipError_t launch_conv(Ctx& c);
 // Restores the fit data of image im to what cmv left (every kept stamp on its
 // first substamp): stamps a clip-refit loop moved on are listed and refilled.
