@@ -726,7 +726,8 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
                                                     const int* nKept, KscScalars* ks,
                                                     int first, double alpha,
                                                     double sigKernFit, int* rec,
-                                                    uint8_t* zeroBits, KscScalars* hks) {
+                                                    uint8_t* zeroBits, KscScalars* hks,
+                                                    int* label) {
   __shared__ double data[XB_MAX_STAMPS];
   __shared__ uint8_t clip[XB_MAX_STAMPS];
   __shared__ uint8_t refill[XB_MAX_STAMPS];
@@ -790,6 +791,14 @@ __global__ __launch_bounds__(1024) void k_ksc_check(const double* sig, const int
     if (tid == 0) s_n = c0 + tot;
     __syncthreads();
   }
+  // with rec: each stamp's label for the screen's record (2 = this pass
+  // replaced its substamp, 0 = kept so far, -1 = no substamp), by the thread
+  // that recorded and moved it
+  if (rec && label)
+    for (int s = tid; s < n; s += blockDim.x) {
+      const int pc = rec[s];
+      label[s] = pc < count[kept[s]] ? (cur[s] != pc ? 2 : 0) : -1;
+    }
   if (tid == 0) {
     ks->nList = s_n;
     ks->check = s_n > 0 ? 1 : 0;
@@ -952,11 +961,12 @@ hipError_t launch_cd_end_pair(Ctx& c) {
 }
 
 hipError_t launch_ksc_check(Ctx& c, int im, bool first, int* rec, uint8_t* zeroBits,
-                            bool publish) {
+                            bool publish, int* label) {
   hipLaunchKernelGGL(k_ksc_check, dim3(1), dim3(1024), 0, c.cur, c.st[im].sigK,
                      c.st[im].kept, c.st[im].cur, c.st[im].count, c.st[im].listK,
                      &c.sc->nKept[im], &c.sc->k[im], first ? 1 : 0, c.m.sigClipAlpha,
-                     c.m.sigKernFit, rec, zeroBits, publish ? &c.hscDev->k[im] : nullptr);
+                     c.m.sigKernFit, rec, zeroBits, publish ? &c.hscDev->k[im] : nullptr,
+                     label);
   return hipGetLastError();
 }
 

@@ -108,20 +108,20 @@ struct ScreenArgs {
   const int* prevCur;    // substamp of each stamp in the previous pass (or null)
   float* zPrev;          // [N] feature 0 of the previous pass, updated here
   float* feat;           // [N][32] of this pass
+  // the decision (apply): p(reject) of each stamp's features by the model, and
+  // the flag of an eligible stamp (its substamp kept by the pass's check)
+  int apply;
+  const float* model;
+  const int* curNow;     // substamp of each stamp after the check
+  float* prob;
+  uint8_t* flag;
 };
 
 constexpr int SCR_THREADS = 256;
 
-__global__ __launch_bounds__(SCR_THREADS) void k_screen_features(ScreenArgs A) {
-  const int s = blockIdx.x;
-  if (s >= *A.nKept) return;
-  const int tid = threadIdx.x, lane = tid % WAVE, wid = tid / WAVE;
-  float* out = A.feat + (size_t)s * MLP_IN;
-  const int g = A.kept[s], c = A.cur[s];
-  if (c >= A.count[g]) {  // no substamp: nothing to screen
-    if (tid < MLP_IN) out[tid] = 0.0f;
-    return;
-  }
+__device__ __forceinline__ void screen_features_body(const ScreenArgs& A, int s, int g, int c,
+                                                     int tid, int lane, int wid, float* out,
+                                                     float* fs) {
   const int cx = A.ssx[g * MAXSS + c], cy = A.ssy[g * MAXSS + c];
   const float skT = (float)A.skyT[g], skS = (float)A.skyS[g];
   const float sgT = fmaxf((float)A.sdT[g], 1e-3f), sgS = fmaxf((float)A.sdS[g], 1e-3f);
@@ -234,9 +234,51 @@ __global__ __launch_bounds__(SCR_THREADS) void k_screen_features(ScreenArgs A) {
     f[31] = (A.prevCur && A.prevCur[s] == c) ? A.zPrev[s] : 0.0f;
     A.zPrev[s] = f[0];
 #pragma unroll
-    for (int k = 0; k < MLP_IN; k++) out[k] = isfinite(f[k]) ? f[k] : 0.0f;
+    for (int k = 0; k < MLP_IN; k++) {
+      const float v = isfinite(f[k]) ? f[k] : 0.0f;
+      out[k] = v;
+      fs[k] = v;
+    }
   }
 }
+
+// The features of kept stamp s (one 256-thread block per stamp) and, with
+// apply, the screen's decision for it, by the block's first wave from the
+// features in LDS: flag = p(reject) > threshold for a stamp whose substamp the
+// pass's check kept.
+__global__ __launch_bounds__(SCR_THREADS) void k_screen_features(ScreenArgs A) {
+  const int s = blockIdx.x;
+  if (s >= *A.nKept) return;
+  const int tid = threadIdx.x, lane = tid % WAVE, wid = tid / WAVE;
+  float* out = A.feat + (size_t)s * MLP_IN;
+  const int g = A.kept[s], c = A.cur[s];
+  __shared__ float fs[MLP_IN];  // this stamp's features
+  if (c >= A.count[g]) {  // no substamp: nothing to screen
+    if (tid < MLP_IN) {
+      out[tid] = 0.0f;
+      fs[tid] = 0.0f;
+    }
+  } else {
+    screen_features_body(A, s, g, c, tid, lane, wid, out, fs);
+  }
+  if (!A.apply) return;
+  __syncthreads();
+  if (wid == 0) {
+    __shared__ float xs[MLP_IN];
+    const float* P = A.model;
+    if (lane < MLP_IN) xs[lane] = (fs[lane] - P[MP_MU + lane]) / P[MP_SD + lane];
+    __builtin_amdgcn_wave_barrier();  // one wave: its LDS accesses run in order
+    const float hj = mlp_hidden(P, xs, lane);
+    const float z = P[MP_B2] + wave_sum_f(h16(P[MP_W2 + lane]) * hj);
+    const float p = 1.0f / (1.0f + expf(-z));
+    if (lane == 0) {
+      const bool eligible = c < A.count[g] && A.curNow[s] == c;
+      A.prob[s] = p;
+      A.flag[s] = eligible && p > P[MP_TAU] ? 1 : 0;
+    }
+  }
+}
+
 
 // ---------------------------------------------------------------------------
 // inference: p(reject) per sample (one wave per sample)
@@ -257,26 +299,6 @@ __global__ __launch_bounds__(WAVE) void k_mlp_forward(const float* __restrict__ 
   __shared__ float xs[MLP_IN];
   const float p = mlp_prob(P, X + (size_t)k * MLP_IN, xs, threadIdx.x);
   if (threadIdx.x == 0) prob[k] = p;
-}
-
-// screen decision per kept stamp: flag = p(reject) > threshold for stamps
-// this pass kept
-__global__ __launch_bounds__(WAVE) void k_screen_decide(const float* __restrict__ P,
-                                                        const float* __restrict__ feat,
-                                                        const int* nKept, const int* kept,
-                                                        const int* passCur, const int* cur,
-                                                        const int* count, float* prob,
-                                                        uint8_t* flag) {
-  const int s = blockIdx.x;
-  if (s >= *nKept) return;
-  __shared__ float xs[MLP_IN];
-  const float p = mlp_prob(P, feat + (size_t)s * MLP_IN, xs, threadIdx.x);
-  if (threadIdx.x == 0) {
-    const int g = kept[s];
-    const bool eligible = passCur[s] < count[g] && cur[s] == passCur[s];
-    prob[s] = p;
-    flag[s] = eligible && p > P[MP_TAU] ? 1 : 0;
-  }
 }
 
 // apply the screen: flagged stamps move to their next substamp; the refill
@@ -409,16 +431,6 @@ __global__ __launch_bounds__(1024) void k_verify_decide(const int* vList, int* c
     ks->nRestored += s_kept;
     ks->nConfirmed += n - s_kept;
   }
-}
-
-// labels of a pass's substamps, right after its check: 2 = the pass
-// replaced it, 0 = kept so far, -1 = no substamp
-__global__ void k_screen_first(const int* nKept, const int* kept, const int* passCur,
-                               const int* cur, const int* count, int* label) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *nKept) return;
-  const int g = kept[s];
-  label[s] = passCur[s] < count[g] ? (cur[s] != passCur[s] ? 2 : 0) : -1;
 }
 
 // after the loop, over nRec passes of n stamps: a substamp kept by its pass
@@ -570,7 +582,6 @@ hipError_t launch_screen(Ctx& c, int im, int pass, bool apply) {
   const int N = c.nKept[im];
   if (N <= 0 || pass >= SCR_PASSES) return hipSuccess;
   int* passCur = c.scr.passCur + (size_t)pass * N;
-  int* label = c.scr.label + (size_t)pass * N;
   ScreenArgs A;
   A.T = c.img[im];
   A.S = c.img[1 - im];
@@ -593,19 +604,15 @@ hipError_t launch_screen(Ctx& c, int im, int pass, bool apply) {
   A.prevCur = pass > 0 ? passCur - N : nullptr;
   A.zPrev = c.scr.zPrev;
   A.feat = c.scr.feat + (size_t)pass * N * MLP_IN;
-  hipLaunchKernelGGL(k_screen_first, dim3((N + 255) / 256), dim3(256), 0, c.cur,
-                     &c.sc->nKept[im], c.st[im].kept, passCur, c.st[im].cur, c.st[im].count,
-                     label);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  // (the pass's labels come from its check, k_ksc_check)
+  A.apply = apply ? 1 : 0;
+  A.model = c.scr.model;
+  A.curNow = c.st[im].cur;
+  A.prob = c.scr.prob + (size_t)pass * N;
+  A.flag = c.scr.flag;
   hipLaunchKernelGGL(k_screen_features, dim3(N), dim3(SCR_THREADS), 0, c.cur, A);
-  e = hipGetLastError();
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess || !apply) return e;
-  hipLaunchKernelGGL(k_screen_decide, dim3(N), dim3(WAVE), 0, c.cur, c.scr.model, A.feat,
-                     &c.sc->nKept[im], c.st[im].kept, passCur, c.st[im].cur, c.st[im].count,
-                     c.scr.prob + (size_t)pass * N, c.scr.flag);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_screen_merge, dim3(1), dim3(1024), 0, c.cur, passCur, c.st[im].cur,
                      c.scr.flag, c.scr.bits, c.st[im].listK, &c.sc->nKept[im], &c.sc->k[im],
                      &c.hscDev->k[im]);

@@ -651,7 +651,8 @@ int kscPass(Ctx& c, int im, int it, bool keep, bool screen, uint16_t nanMark,
   // publishes the pass's scalars to the host mirror
   XB_HIP(launch_ksc_check(c, im, it == 0,
                           rec ? c.scr.passCur + (size_t)it * c.nKept[im] : nullptr,
-                          (screen && it == 0) ? c.scr.bits : nullptr, true));
+                          (screen && it == 0) ? c.scr.bits : nullptr, true,
+                          rec ? c.scr.label + (size_t)it * c.nKept[im] : nullptr));
   if (rec) XB_HIP(launch_screen(c, im, it, screen));
   XB_HIP(hipEventRecord(c.evK[im], c.cur));
   return XB_OK;

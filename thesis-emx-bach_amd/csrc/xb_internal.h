@@ -375,9 +375,11 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
 // chi2 of the listed stamps (d_list, d_nList entries) into out (list order)
 hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d_nList,
                                  const double* d_sol, double* out);
-// publish: also write the pass's check .. nScreened into the host mirror
+// publish: also write the pass's check .. nScreened into the host mirror;
+// label (with rec): the screen's per-stamp labels of the pass
 hipError_t launch_ksc_check(Ctx& c, int im, bool first, int* rec = nullptr,
-                            uint8_t* zeroBits = nullptr, bool publish = false);
+                            uint8_t* zeroBits = nullptr, bool publish = false,
+                            int* label = nullptr);
 hipError_t launch_conv(Ctx& c);
 // Restores the fit data of image im to what cmv left (every kept stamp on its
 // first substamp): stamps a clip-refit loop moved on are listed and refilled.
