@@ -639,6 +639,7 @@ struct CdFinishArgs {
   DevScalars* sc;
   int im, w, h;
   double alpha;
+  DevScalars* hsc;  // the host's pinned mirror (its merit) or null
 };
 
 template <int NI>
@@ -650,6 +651,7 @@ __global__ __launch_bounds__(1024) void k_cd_finish(ArgsN<CdFinishArgs, NI> P) {
   DevScalars* sc = FA.sc;
   const int im = FA.im, w = FA.w, h = FA.h;
   const double alpha = FA.alpha;
+  DevScalars* hsc = FA.hsc;
   __shared__ double co[NPSF];
   __shared__ double kern[KK];
   __shared__ double part[64];
@@ -659,9 +661,20 @@ __global__ __launch_bounds__(1024) void k_cd_finish(ArgsN<CdFinishArgs, NI> P) {
   __shared__ int isc[20];
   __shared__ int s_n;
   const int tid = threadIdx.x;
+  // the merit, also into the host's pinned mirror (one thread)
+  auto setMerit = [&](double m) {
+    sc->merit[im] = m;
+    if (hsc) {
+      hsc->merit[im] = m;
+      __threadfence_system();
+    }
+  };
   const int nTest = sc->nTest[im];
   if (nTest == 0) {
-    if (tid == 0) { sc->merit[im] = 666; sc->kernelSumCd[im] = 0.0; }
+    if (tid == 0) {
+      sc->kernelSumCd[im] = 0.0;
+      setMerit(666);
+    }
     return;
   }
   const double hw = 0.5 * w, hh = 0.5 * h;
@@ -705,11 +718,11 @@ __global__ __launch_bounds__(1024) void k_cd_finish(ArgsN<CdFinishArgs, NI> P) {
   }
   const int n = s_n;
   if (n == 0) {
-    if (tid == 0) sc->merit[im] = 666;
+    if (tid == 0) setMerit(666);
     return;
   }
   sigma_clip_small(data, n, 10, alpha, clip, part, res, isc);
-  if (tid == 0) sc->merit[im] = res[0] / s_ksum;
+  if (tid == 0) setMerit(res[0] / s_ksum);
 }
 
 // ---------------------------------------------------------------------------
@@ -918,7 +931,7 @@ hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d
 
 static CdFinishArgs cd_finish_args(Ctx& c, int im) {
   return CdFinishArgs{c.basis, c.cdSol + (size_t)im * NSOL, c.st[im].sig, c.sc, im, c.w, c.h,
-                      c.m.sigClipAlpha};
+                      c.m.sigClipAlpha, c.hscDev};
 }
 
 hipError_t launch_cd_finish(Ctx& c, int im) {

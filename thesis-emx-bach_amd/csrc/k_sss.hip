@@ -1002,7 +1002,8 @@ __global__ __launch_bounds__(FIND_THREADS) void k_sss_find(ArgsN<FindArgs, NI> P
 __global__ __launch_bounds__(1024) void k_sss_compact(const int* __restrict__ count0,
                                                       const int* __restrict__ count1,
                                                       int* kept0, int* kept1, int* cur0,
-                                                      int* cur1, DevScalars* sc, int nGrid) {
+                                                      int* cur1, DevScalars* sc, int nGrid,
+                                                      DevScalars* hsc) {
   __shared__ int isc[20];
   __shared__ int s_n;
   for (int im = 0; im < 2; im++) {
@@ -1031,6 +1032,12 @@ __global__ __launch_bounds__(1024) void k_sss_compact(const int* __restrict__ co
   if (threadIdx.x == 0) {
     double f0 = (double)sc->nKept[0] / nGrid, f1 = (double)sc->nKept[1] / nGrid;
     sc->retry = (f0 < 0.1 || f1 < 0.1) ? 1 : 0;
+    if (hsc) {  // the host's pinned mirror: the counts and the retry flag
+      hsc->nKept[0] = sc->nKept[0];
+      hsc->nKept[1] = sc->nKept[1];
+      hsc->retry = sc->retry;
+      __threadfence_system();
+    }
   }
 }
 
@@ -1117,7 +1124,7 @@ hipError_t launch_sss_find_pair(Ctx& c) {
 hipError_t launch_sss_compact(Ctx& c) {
   hipLaunchKernelGGL(k_sss_compact, dim3(1), dim3(1024), 0, c.cur, c.st[0].count,
                      c.st[1].count, c.st[0].kept, c.st[1].kept, c.st[0].cur, c.st[1].cur, c.sc,
-                     c.nGrid);
+                     c.nGrid, c.hscDev);
   return hipGetLastError();
 }
 

@@ -533,18 +533,16 @@ int stageSss(Ctx& c) {
   if (rc) return rc;
   rc = sssIdentify(c);
   if (rc) return rc;
-  DevScalars sc;
-  rc = readScalars(c, sc);
-  if (rc) return rc;
-  if (sc.retry) {  // fewer than 10 % of the stamps filled (src/bach.cpp:142-161)
+  // k_sss_compact published the counts and the retry flag to the pinned mirror
+  XB_HIP(spinSync(c));
+  if (c.hsc->retry) {  // fewer than 10 % of the stamps filled (src/bach.cpp:142-161)
     c.retried = 1;
     XB_HIP(launch_reset_skip(c));
     rc = sssIdentify(c);
     if (rc) return rc;
-    rc = readScalars(c, sc);
-    if (rc) return rc;
+    XB_HIP(spinSync(c));
   }
-  return sssDone(c, sc);
+  return sssDone(c, *c.hsc);
 }
 
 // cmv (src/bach.cpp:172-268)
@@ -583,11 +581,10 @@ int stageCd(Ctx& c, int* convTemplate) {
     return XB_OK;
   });
   if (rc) return rc;
-  DevScalars sc;
-  rc = readScalars(c, sc);
-  if (rc) return rc;
+  // k_cd_finish published the merits to the pinned mirror
+  XB_HIP(spinSync(c));
   double merit[2];
-  for (int im = 0; im < 2; im++) merit[im] = c.nKept[im] == 0 ? 666 : sc.merit[im];
+  for (int im = 0; im < 2; im++) merit[im] = c.nKept[im] == 0 ? 666 : c.hsc->merit[im];
   c.convTemplate = merit[1] > merit[0] ? 1 : 0;
   c.tIdx = c.convTemplate ? 0 : 1;
   if (convTemplate) *convTemplate = c.convTemplate;
