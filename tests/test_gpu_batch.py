@@ -194,6 +194,47 @@ def test_lockstep_batch_equals_single_runs(pairs, singles, inflight):
     assert all(0 <= r.worker < inflight for r in res)
 
 
+def _nan_pairs():
+    # a few NaN pixels in each image and some at the same pixel of both, on a
+    # square and a ragged pair (kernels stay finite, but the NaN pixels change
+    # the stamp statistics and the substamps chosen): batch workers run both
+    # images' statistics passes as one dispatch and write the NaN marks
+    # afterwards, which must leave every pass reading the mask as the lone
+    # run's T-then-S order does
+    out = []
+    for seed, w, h, n in ((41, 512, 512, 16), (43, 700, 530, 12)):
+        t, s = xbach.synth_pair(xbach.synth_cfg("pair512", w=w, h=h, seed=seed))
+        rng = np.random.default_rng(100 * n)
+        for arr in (t, s):
+            pts = rng.integers(20, min(w, h) - 20, (n, 2))
+            arr[pts[:, 0], pts[:, 1]] = np.nan
+        pts = rng.integers(20, min(w, h) - 20, (n // 2, 2))
+        t[pts[:, 0], pts[:, 1]] = np.nan
+        s[pts[:, 0], pts[:, 1]] = np.nan
+        out.append((t, s))
+    return out
+
+
+@pytest.mark.parametrize("flags", [0, xbach.BATCH_LOCKSTEP], ids=["workers", "lockstep"])
+def test_batch_nan_pixels_equal_single_runs(flags):
+    p = xbach.Params.default()
+    ps = _nan_pairs()
+    ref = []
+    with xbach.Context(0) as c:
+        for t, s in ps:
+            conv, diff, _ = c.run(p, t, s)
+            tr = c.trace()
+            ref.append((conv, diff, tr.kernSum, tr.convTemplate, tr.kscIterations))
+    with xbach.Batch(0, workers=2, flags=flags) as b:
+        outs, res = b.run(p, ps, check=False)
+        msg = b._L.xb_batch_last_error(b._h)
+    for i, (conv, diff, ks, ct, it) in enumerate(ref):
+        r = res[i]
+        assert r.status == 0, (i, r.status, msg)
+        assert _same(outs[i][0], conv) and _same(outs[i][1], diff), i
+        assert (r.kernSum, r.convTemplate, r.kscIterations) == (ks, ct, it), i
+
+
 @pytest.mark.parametrize("form", [xbach.FORM_MIXED | xbach.FORM_REFINE,
                                   xbach.FORM_MIXED | xbach.FORM_LDLT,
                                   xbach.FORM_MIXED | xbach.FORM_LDLT | xbach.FORM_REFINE])
