@@ -533,12 +533,16 @@ struct SigArgs {
   uint16_t nanMark;  // mask bit of a NaN model pixel
 };
 
-// One workgroup per stamp, one thread per substamp pixel (992 >= 31 x 31 slots
-// of the reference's 32 x 31 reduction layout), the 48 basis terms of a pixel
-// loaded together before its accumulation chain.
-constexpr int SIG_THREADS = 1024;
-template <int NI>
-__global__ __launch_bounds__(SIG_THREADS) void k_calc_sigs(ArgsN<SigArgs, NI> P) {
+// One workgroup per stamp, NT threads striding over the 992 >= 31 x 31 pixel
+// slots of the reference's 32 x 31 reduction layout (one slot per thread at
+// 1024), the 48 basis terms of a pixel loaded together before its
+// accumulation chain.  The slots' partial sums are added in a fixed order, so
+// every NT gives the same bits.  NT = 1024 is faster for a lone pair; 256 is
+// faster when many pairs share the GPU (a smaller workgroup finds room on a
+// busy CU sooner), so one-stream contexts (batch workers) launch 256.
+constexpr int SIG_THREADS = 1024, SIG_THREADS_SHARED = 256;
+template <int NI, int NT>
+__global__ __launch_bounds__(NT) void k_calc_sigs(ArgsN<SigArgs, NI> P) {
   const SigArgs& A = args_of(P);
   const int u = blockIdx.x;
   if (u >= *A.nList) return;
@@ -908,6 +912,18 @@ static SigArgs sig_args(Ctx& c, int im, const int* d_list, const int* d_nList,
   return A;
 }
 
+// threads per stamp of k_calc_sigs: 256 in one-stream contexts (batch
+// workers), 1024 otherwise
+template <int NI>
+static void launch_sigs_n(Ctx& c, int n, const ArgsN<SigArgs, NI>& P) {
+  if (c.side == c.stream)
+    hipLaunchKernelGGL((k_calc_sigs<NI, SIG_THREADS_SHARED>), dim3(n, 1, NI),
+                       dim3(SIG_THREADS_SHARED), 0, c.cur, P);
+  else
+    hipLaunchKernelGGL((k_calc_sigs<NI, SIG_THREADS>), dim3(n, 1, NI), dim3(SIG_THREADS), 0,
+                       c.cur, P);
+}
+
 hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
                             uint16_t nanMark) {
   if (c.nKept[im] <= 0) return hipSuccess;
@@ -915,8 +931,7 @@ hipError_t launch_calc_sigs(Ctx& c, int im, bool useList, const double* d_sol,
       sig_args(c, im, useList ? c.st[im].list : nullptr,
                useList ? &c.sc->nList[im] : &c.sc->nKept[im], d_sol,
                useList ? c.st[im].sig : c.st[im].sigK, nanMark);
-  hipLaunchKernelGGL(k_calc_sigs<1>, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur,
-                     ArgsN<SigArgs, 1>{{A}});
+  launch_sigs_n<1>(c, c.nKept[im], ArgsN<SigArgs, 1>{{A}});
   return hipGetLastError();
 }
 
@@ -924,8 +939,7 @@ hipError_t launch_calc_sigs_list(Ctx& c, int im, const int* d_list, const int* d
                                  const double* d_sol, double* out) {
   if (c.nKept[im] <= 0) return hipSuccess;
   const SigArgs A = sig_args(c, im, d_list, d_nList, d_sol, out, XB_MASK_NAN_PIXEL);
-  hipLaunchKernelGGL(k_calc_sigs<1>, dim3(c.nKept[im]), dim3(SIG_THREADS), 0, c.cur,
-                     ArgsN<SigArgs, 1>{{A}});
+  launch_sigs_n<1>(c, c.nKept[im], ArgsN<SigArgs, 1>{{A}});
   return hipGetLastError();
 }
 
@@ -964,8 +978,7 @@ hipError_t launch_cd_end_pair(Ctx& c) {
   for (int im = 0; im < 2; im++)
     A[im] = sig_args(c, im, c.st[im].list, &c.sc->nList[im], c.cdSol + (size_t)im * NSOL,
                      c.st[im].sig, XB_MASK_NAN_PIXEL);
-  hipLaunchKernelGGL(k_calc_sigs<2>, dim3(n, 1, 2), dim3(SIG_THREADS), 0, c.cur,
-                     ArgsN<SigArgs, 2>{{A[0], A[1]}});
+  launch_sigs_n<2>(c, n, ArgsN<SigArgs, 2>{{A[0], A[1]}});
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_cd_finish<2>, dim3(1, 1, 2), dim3(1024), 0, c.cur,
