@@ -334,8 +334,9 @@ typedef struct xb_shapes {
   int32_t find_staged; /* 1: the substamp search stages each stamp in LDS when
                           it fits (up to ~158 KB, one workgroup per CU); 0:
                           it reads the image and mask in place (default 1)    */
-  int32_t stats_threads; /* threads per stamp of the stamp statistics: 256 or
-                            1024 (default 1024)                               */
+  int32_t stats_threads; /* threads per stamp of the stamp statistics: 256,
+                            512 or 1024 (default 1024 for a context; 512 for
+                            a batch's pairs, which share the GPU)             */
   int32_t pad;
 } xb_shapes;
 /* s == NULL restores the defaults; other values are refused (XB_ERR_ARG). */
@@ -490,7 +491,8 @@ int xb_batch_screen_load(xb_batch* b, const float* model);
  * at that count, keeping one only when it is faster by 3 %), or
  * xb_batch_set_shapes set them. */
 int xb_batch_get_shapes(xb_batch* b, int bucket, xb_shapes* s);
-/* The same shapes for every bucket (s == NULL: defaults). */
+/* The same shapes for every bucket (s == NULL: a batch's defaults, which are a
+ * context's but for stats_threads 512). */
 int xb_batch_set_shapes(xb_batch* b, const xb_shapes* s);
 
 #ifdef __cplusplus

@@ -1049,9 +1049,13 @@ __global__ void k_reset_skip(uint16_t* mask, size_t n) {
 hipError_t launch_sss_stats(Ctx& c, int im) {
   if ((c.fsw * c.fsw + 31) / 32 > MAX_CLIP_BLOCKS) return hipErrorInvalidValue;
   prof_begin(c, XB_PROF_SSS);
-  // threads per stamp from the context's launch shapes (256 or 1024)
+  // threads per stamp from the context's launch shapes (256, 512 or 1024)
   if (c.shapes.stats_threads == 256)
     hipLaunchKernelGGL(k_sss_stats<256>, dim3(c.nGrid), dim3(256), 0, c.cur, c.img[im], c.mask,
+                       c.w, c.h, c.stampsx, c.stampsy, c.fsw, c.ran1, c.goodPix,
+                       c.m.sigClipAlpha, c.m.iqRange, c.st[im].sky, c.st[im].fwhm);
+  else if (c.shapes.stats_threads == 512)
+    hipLaunchKernelGGL(k_sss_stats<512>, dim3(c.nGrid), dim3(512), 0, c.cur, c.img[im], c.mask,
                        c.w, c.h, c.stampsx, c.stampsy, c.fsw, c.ran1, c.goodPix,
                        c.m.sigClipAlpha, c.m.iqRange, c.st[im].sky, c.st[im].fwhm);
   else

@@ -29,8 +29,8 @@ struct xb_batch {
   int device = 0;
   int inflight = 0;                    // 0: all workers
   int tuned[XB_NBUCKET] = {0, 0, 0, 0};  // pairs in flight per size bucket (0: untuned)
-  xb_shapes shapes[XB_NBUCKET] = {xb::default_shapes(), xb::default_shapes(),
-                                  xb::default_shapes(), xb::default_shapes()};
+  xb_shapes shapes[XB_NBUCKET] = {xb::default_batch_shapes(), xb::default_batch_shapes(),
+                                  xb::default_batch_shapes(), xb::default_batch_shapes()};
   std::vector<xb_ctx*> ctx;
   std::string err;
   // lockstep mode (XB_BATCH_LOCKSTEP): the batched solves' stream and events
@@ -597,7 +597,7 @@ int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, cons
       }
       return XB_OK;
     };
-    b->shapes[bk] = xb::default_shapes();
+    b->shapes[bk] = xb::default_batch_shapes();
     for (int c : cands) {
       if (c > workers) break;
       b->inflight = c;
@@ -616,7 +616,7 @@ int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, cons
     // launch shapes at that count, one knob at a time from the defaults: a
     // value other than the current one must win by 3 %
     b->inflight = bestC;
-    b->shapes[bk] = xb::default_shapes();
+    b->shapes[bk] = xb::default_batch_shapes();
     struct Knob {
       int32_t xb_shapes::*field;
       std::vector<int> values;
@@ -624,7 +624,7 @@ int xb_batch_autotune(xb_batch* b, const xb_params* p, int n, const int* w, cons
     std::vector<Knob> knobs = {{&xb_shapes::conv_tiles, {1, 2, 4, 8}},
                                {&xb_shapes::mask_rows, {16, 32, 64}},
                                {&xb_shapes::find_staged, {0, 1}},
-                               {&xb_shapes::stats_threads, {256, 1024}}};
+                               {&xb_shapes::stats_threads, {256, 512, 1024}}};
     if (b->form & XB_FORM_LDLT) knobs.push_back({&xb_shapes::ldlt_group, {2, 4}});
     for (const Knob& k : knobs) {
       const int cur = b->shapes[bk].*k.field;
@@ -659,7 +659,7 @@ int xb_batch_get_shapes(xb_batch* b, int bucket, xb_shapes* s) {
 
 int xb_batch_set_shapes(xb_batch* b, const xb_shapes* s) {
   if (!b) return XB_ERR_ARG;
-  xb_shapes v = s ? *s : xb::default_shapes();
+  xb_shapes v = s ? *s : xb::default_batch_shapes();
   if (!xb::shapes_valid(v)) return XB_ERR_ARG;
   v.pad = 0;
   for (auto& q : b->shapes) q = v;

@@ -32,13 +32,21 @@ inline xb_shapes default_shapes() {
   s.pad = 0;
   return s;
 }
+// A batch's defaults: the same but for the stamp statistics, which run with
+// 512 threads per stamp (faster when many pairs share the GPU; 1024 is faster
+// for a lone pair).
+inline xb_shapes default_batch_shapes() {
+  xb_shapes s = default_shapes();
+  s.stats_threads = 512;
+  return s;
+}
 inline bool shapes_valid(const xb_shapes& s) {
   const bool ct = s.conv_tiles == 1 || s.conv_tiles == 2 || s.conv_tiles == 4 ||
                   s.conv_tiles == 8;
   const bool mr = s.mask_rows == 16 || s.mask_rows == 32 || s.mask_rows == 64;
   const bool lg = s.ldlt_group == 2 || s.ldlt_group == 4;
   const bool fs = s.find_staged == 0 || s.find_staged == 1;
-  const bool st = s.stats_threads == 256 || s.stats_threads == 1024;
+  const bool st = s.stats_threads == 256 || s.stats_threads == 512 || s.stats_threads == 1024;
   return ct && mr && lg && fs && st;
 }
 
